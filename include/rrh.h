@@ -1,0 +1,280 @@
+/*
+ * rrh.h -- C ABI of librrh.so, the MI355X (gfx950) implementation of the
+ * reference renderer's hot path: reference path tracer + accumulation,
+ * ReSTIR DI, edge-avoiding A-trous, vMF path guiding (reference single-lobe
+ * hash grid and an 8-lobe EM-fitted dense grid) and a learned MLP guide.
+ *
+ * Conventions
+ *  - Every entry point returns int: RRH_OK (0) or a negative RRH_E_* code.
+ *    rrh_last_error() returns a thread-local description of the last failure.
+ *    No C++ exception crosses this boundary.
+ *  - Host pointers are owned by the caller; device memory is owned by the
+ *    library and released by the matching rrh_*_destroy.  Each context owns
+ *    one HIP stream; all work of a context is ordered on it.  One thread per
+ *    context.
+ *  - Scene records mirror the reference's GpuScene.h byte for byte.
+ *
+ * Each declaration cites the reference interface it replaces
+ * (paths relative to the reference repository root).
+ */
+#ifndef RRH_H
+#define RRH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RRH_OK 0
+#define RRH_E_ARG (-1)       /* invalid argument (null pointer, size mismatch, bad enum) */
+#define RRH_E_HIP (-2)       /* a HIP runtime call failed */
+#define RRH_E_STATE (-3)     /* object not in a usable state (e.g. scene with no triangles) */
+#define RRH_E_NOMEM (-4)     /* device or host allocation failed */
+#define RRH_E_RANGE (-5)     /* size above a documented limit */
+
+/* ---- scene records: src/scene/GpuScene.h:17-114 ------------------------- */
+typedef struct { float pos[3]; float normal[3]; float uv[2]; float tangent[4]; } rrh_vertex;      /* 48 B, GpuVertex */
+typedef struct { uint32_t vertex_base, vertex_count, index_byte_offset, index_count,
+                 material_index, _pad[3]; } rrh_mesh;                                              /* 32 B, GpuMesh */
+typedef struct { float base_color[4]; float metallic, roughness; float emissive[3]; float ior;
+                 uint32_t albedo_tex_idx, normal_tex_idx, metallic_roughness_tex_idx,
+                          emissive_tex_idx, sampler_idx;
+                 uint32_t flags; /* GpuMaterial._pad[0]: RRH_MAT_* */
+                 uint32_t _pad1, _pad2; } rrh_material;                                            /* 72 B, GpuMaterial */
+typedef struct { float transform[16]; /* row-major */ uint32_t mesh_index; uint32_t _pad[3]; } rrh_instance; /* 80 B */
+typedef struct { float position[3]; uint32_t type; /* 0 point, 1 directional */
+                 float direction[3]; float radius; float emission[3]; float intensity; } rrh_light; /* 48 B */
+typedef struct { float view[16], inv_view[16], inv_proj[16], prev_view_proj[16];
+                 float position[3]; float near_plane; float direction[3]; float far_plane;
+                 uint32_t frame_index, screen_width, screen_height, _pad; } rrh_camera;            /* 304 B, GpuCameraData */
+
+#define RRH_MAT_PURE_LAMBERT 1u   /* f = albedo/pi, cosine sampling (no GGX lobe) */
+#define RRH_SCENE_TRI_LIGHTS 1u   /* emissive triangles become NEE / ReSTIR lights, MIS on BSDF hits */
+
+/* ---- opaque handles -------------------------------------------------------- */
+typedef struct rrh_ctx rrh_ctx;
+typedef struct rrh_scene rrh_scene;
+typedef struct rrh_image rrh_image;
+typedef struct rrh_grid rrh_grid;     /* guiding grid (reference hash grid or EM grid) */
+typedef struct rrh_mlp rrh_mlp;       /* learned guide */
+
+/* ---- context: replaces rr::rhi::Device + the frame's command recorder
+ *      (src/rhi/Device.cpp:300-369 one_time_submit, :270-298 submit_frame) -- */
+int rrh_ctx_create(int device, rrh_ctx** out);
+int rrh_ctx_destroy(rrh_ctx* ctx);
+int rrh_ctx_synchronize(rrh_ctx* ctx);
+const char* rrh_last_error(void);
+/* device properties used for roofline arithmetic: CU count, clock, name */
+int rrh_device_info(rrh_ctx* ctx, int* cu_count, int* clock_khz, char* name, size_t name_len);
+/* elapsed ms between two markers recorded on the context stream */
+int rrh_marker_record(rrh_ctx* ctx, int slot);             /* slot in [0, 16) */
+int rrh_marker_elapsed(rrh_ctx* ctx, int a, int b, float* ms);
+
+/* ---- scene: replaces Scene::upload (src/scene/Scene.cpp:255-515) and the
+ *      BLAS/TLAS build (src/rhi/AccelStructure.cpp:98-330) with a flattened
+ *      world-space triangle list and a binned-SAH binary BVH ------------- */
+typedef struct {
+    uint32_t tri_count, node_count, bvh_depth, leaf_count;
+    uint32_t light_count;          /* reference lights + triangle lights */
+    uint32_t tri_light_count;
+    uint64_t device_bytes;         /* nodes + triangles + shading records */
+    float    build_ms;
+    float    bounds_min[3], bounds_max[3];
+} rrh_scene_info;
+int rrh_scene_upload(rrh_ctx* ctx, const rrh_vertex* verts, uint32_t nv, const uint32_t* idx,
+                     uint32_t ni, const rrh_mesh* meshes, uint32_t nm, const rrh_material* mats,
+                     uint32_t nmat, const rrh_instance* insts, uint32_t ninst,
+                     const rrh_light* lights, uint32_t nl, uint32_t flags, rrh_scene** out);
+int rrh_scene_destroy(rrh_scene* s);
+int rrh_scene_info_get(const rrh_scene* s, rrh_scene_info* out);
+/* world-space triangles (9 floats each) in flattening order, for checking */
+int rrh_scene_triangles(const rrh_scene* s, float* out, uint32_t max_tris);
+/* replaces Scene::update_camera (src/scene/Scene.cpp:554-585) */
+int rrh_camera_set(rrh_scene* s, const rrh_camera* cam);
+/* host restatement of Camera::look_at + rebuild_matrices (src/scene/Camera.cpp:27-36,130-146) */
+int rrh_camera_look_at(const float eye[3], const float center[3], float fov_y_deg, uint32_t width,
+                       uint32_t height, float near_plane, float far_plane, uint32_t frame_index,
+                       rrh_camera* out);
+
+/* ---- procedural scenes (the reference ships only build_cornell_box,
+ *      src/scene/Scene.cpp:144-251; the others are seeded generators) ----- */
+#define RRH_GEN_CORNELL_REF 0      /* the reference Cornell box incl. its point light */
+#define RRH_GEN_CORNELL_RANDOM 1   /* 36 tris, random axis-aligned blocks, albedo U[.1,.9], quad light L=15 */
+#define RRH_GEN_HEIGHTFIELD 2      /* 512^2-cell fBm heightfield + 2000 icospheres + 4096 emissive tris */
+#define RRH_GEN_INDOOR 3           /* 10x10x4 room, ~1M tris, 0.5x0.5 window, directional sun */
+typedef struct {
+    rrh_vertex* verts; uint32_t nv;
+    uint32_t* idx; uint32_t ni;
+    rrh_mesh* meshes; uint32_t nm;
+    rrh_material* mats; uint32_t nmat;
+    rrh_instance* insts; uint32_t ninst;
+    rrh_light* lights; uint32_t nl;
+    uint32_t flags;                /* suggested rrh_scene_upload flags */
+    float cam_eye[3], cam_center[3], fov_y_deg;
+} rrh_scene_desc;
+/* scale: 1.0 = full size; smaller values shrink tessellation for tests */
+int rrh_scene_generate(int kind, uint32_t seed, float scale, rrh_scene_desc* out);
+int rrh_scene_desc_free(rrh_scene_desc* d);
+
+/* ---- images: replace rr::rhi::Image + Readback (src/rhi/Readback.cpp:83-152) ---- */
+#define RRH_FMT_RGBA32F 0
+#define RRH_FMT_RGBA16F 1
+#define RRH_FMT_RGBA64F 2          /* FP64 accumulation */
+#define RRH_FMT_R32U 3
+int rrh_image_create(rrh_ctx* ctx, uint32_t width, uint32_t height, int format, rrh_image** out);
+int rrh_image_destroy(rrh_image* img);
+int rrh_image_upload(rrh_ctx* ctx, rrh_image* img, const void* host, size_t bytes);
+int rrh_image_download(rrh_ctx* ctx, const rrh_image* img, void* host, size_t bytes);
+int rrh_image_clear(rrh_ctx* ctx, rrh_image* img);
+
+/* ---- closest hit / occlusion: the RayQuery TraceRayInline + Proceed loop
+ *      (assets/shaders/passes/pathtracer_reference/pathtracer.slang:45-75) -- */
+typedef struct { float origin[3]; float tmin; float dir[3]; float tmax; } rrh_ray;   /* 32 B */
+typedef struct { float t; uint32_t prim; float b1, b2; } rrh_hit;  /* prim = flattened tri id, ~0u = miss */
+int rrh_trace_closest(rrh_ctx* ctx, const rrh_scene* s, const rrh_ray* rays, uint32_t n, rrh_hit* hits);
+int rrh_trace_occluded(rrh_ctx* ctx, const rrh_scene* s, const rrh_ray* rays, uint32_t n, uint32_t* occluded);
+/* same on device-resident buffers (no copies), for timing */
+int rrh_trace_closest_device(rrh_ctx* ctx, const rrh_scene* s, const void* d_rays, uint32_t n, void* d_hits);
+/* device scratch for the _device variant: allocate / free n rays + hits */
+int rrh_trace_buffers(rrh_ctx* ctx, uint32_t n, void** d_rays, void** d_hits);
+int rrh_trace_buffers_free(rrh_ctx* ctx, void* d_rays, void* d_hits);
+int rrh_copy_to_device(rrh_ctx* ctx, void* dst, const void* src, size_t bytes);
+int rrh_copy_to_host(rrh_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/* ---- path tracer: PathTracerPass::execute (src/passes/pathtracer_reference/
+ *      PathTracerPass.cpp:170-224) + pathtracer.slang cs_main :181-265 ---- */
+#define RRH_GUIDE_NONE 0
+#define RRH_GUIDE_VMM 1            /* 8-lobe EM grid (rrh_grid of kind RRH_GRID_VMM) */
+#define RRH_GUIDE_MLP 2            /* learned guide (rrh_mlp) */
+typedef struct {
+    uint32_t max_bounces;          /* PathTracerPass.h:46 default 5 */
+    uint32_t frame_seed;           /* PathTracerPass.cpp:215 frame_count_++ */
+    uint32_t rr_start;             /* Russian roulette from this bounce (pathtracer.slang:252 uses 3) */
+    uint32_t guide_mode;           /* RRH_GUIDE_* */
+    float    guide_alpha;          /* one-sample MIS probability of the guide */
+    uint32_t record_vertices;      /* > 0: record path vertices, keep about this many as training samples */
+    uint32_t flags;                /* RRH_PT_* */
+    uint32_t _pad;
+} rrh_pt_params;
+#define RRH_PT_COUNT_FETCHES 1u    /* instrumented traversal: count node / triangle fetches (slower) */
+typedef struct { uint64_t primary, extension, shadow; } rrh_ray_counts;
+int rrh_pt_render(rrh_ctx* ctx, rrh_scene* s, const rrh_pt_params* p, rrh_image* radiance);
+int rrh_pt_render_guided(rrh_ctx* ctx, rrh_scene* s, const rrh_pt_params* p, rrh_grid* grid,
+                         rrh_mlp* mlp, rrh_image* radiance);
+/* ray counts of the last render on this context (reads back device counters) */
+int rrh_pt_ray_counts(rrh_ctx* ctx, rrh_ray_counts* out);
+/* rays traced since the last reset by every render / ReSTIR DI call on this context,
+ * summed on the device (no host round trip per frame); reset != 0 clears them */
+int rrh_ray_totals(rrh_ctx* ctx, rrh_ray_counts* out, int reset);
+/* node (64 B) and triangle (48 B) fetches of closest-hit traversal counted by renders
+ * with RRH_PT_COUNT_FETCHES since the last reset, and the rays they cover */
+int rrh_fetch_counts(rrh_ctx* ctx, uint64_t* node_fetches, uint64_t* tri_fetches, uint64_t* rays, int reset);
+/* per-launch GPU time of the closest-hit traversal kernel inside rrh_pt_render, taken
+ * with HIP events on the context stream while profiling is on */
+int rrh_profile_enable(rrh_ctx* ctx, int on);
+int rrh_profile_traversal(rrh_ctx* ctx, float* total_ms, uint32_t* launches, int reset);
+
+/* ---- accumulate: AccumulatePass::execute (src/passes/accumulate/
+ *      AccumulatePass.cpp:167-218) + accumulate.slang:21-46 -------------- */
+int rrh_accumulate(rrh_ctx* ctx, const rrh_image* radiance, rrh_image* accumulated,
+                   uint32_t camera_moved, uint32_t accumulated_spp);
+/* FP64 running sum (RGBA64F); mean = sum / spp */
+int rrh_accumulate_f64(rrh_ctx* ctx, const rrh_image* radiance, rrh_image* accum_sum,
+                       uint32_t camera_moved, uint32_t accumulated_spp);
+
+/* ---- ReSTIR DI: ReSTIRDIPass::execute (src/passes/restir_di/
+ *      ReSTIRDIPass.cpp:306-447) + restir_di.slang cs_initial :186-262,
+ *      cs_temporal :266-335, cs_spatial_shade :339-472 ------------------- */
+typedef struct {
+    uint32_t frame_seed;               /* frame_count_++ (ReSTIRDIPass.cpp:343) */
+    uint32_t num_initial_candidates;   /* ReSTIRDIPass.h:91 default 16 */
+    uint32_t temporal_enable;
+    uint32_t spatial_enable;
+    uint32_t spatial_num_neighbors;    /* default 4 */
+    uint32_t spatial_radius;           /* pixels, default 15 */
+    uint32_t _pad0, _pad1;
+} rrh_di_params;
+/* gbuf_pos/gbuf_norm: RGBA32F; reservoir: RGBA32F (light, w_sum, W, M) + its uv plane (R32U) */
+int rrh_di_initial(rrh_ctx* ctx, rrh_scene* s, const rrh_di_params* p, rrh_image* gbuf_pos,
+                   rrh_image* gbuf_norm, rrh_image* res, rrh_image* res_uv);
+int rrh_di_temporal(rrh_ctx* ctx, rrh_scene* s, const rrh_di_params* p, const rrh_image* gbuf_pos,
+                    const rrh_image* gbuf_norm, rrh_image* res_curr, rrh_image* res_curr_uv,
+                    const rrh_image* res_prev, const rrh_image* res_prev_uv);
+int rrh_di_spatial_shade(rrh_ctx* ctx, rrh_scene* s, const rrh_di_params* p,
+                         const rrh_image* gbuf_pos, const rrh_image* gbuf_norm,
+                         const rrh_image* res, const rrh_image* res_uv, rrh_image* out);
+/* shadow rays traced by the last rrh_di_spatial_shade (one per shaded pixel) */
+int rrh_di_ray_counts(rrh_ctx* ctx, rrh_ray_counts* out);
+
+/* ---- A-trous: AtrousPass::execute (src/passes/denoise/AtrousPass.cpp:243-306)
+ *      + atrous.slang cs_main :41-106 ------------------------------------- */
+#define RRH_ATROUS_REFERENCE 0  /* RGBA32F radiance, RGBA32F position/normal G-buffer, Gaussian weights */
+#define RRH_ATROUS_CONFIG 1     /* RGBA16F radiance+variance, RGBA16F normal+linear depth, SVGF weights */
+typedef struct {
+    uint32_t iterations;        /* clamped to [1, 7] (AtrousPass.cpp:269) */
+    uint32_t mode;              /* RRH_ATROUS_* */
+    float sigma_position;       /* reference: .25 ; config: sigma_depth 1.0 */
+    float sigma_normal;         /* reference: .2  ; config: normal exponent 128 */
+    float sigma_luminance;      /* reference: 1.0 ; config: sigma_color 4.0 (variance-scaled) */
+    uint32_t _pad[3];
+} rrh_atrous_params;
+/* runs `iterations` passes with stride 1<<i ping-ponging between tmp0/tmp1;
+ * *out_slot receives the slot holding the result ((iterations-1)&1, AtrousPass.cpp:197-201).
+ * Reference mode: gbuf_a = positions (w<0 miss), gbuf_b = normals.  Config mode: gbuf_a =
+ * RGBA16F (n.xyz, linear depth), gbuf_b unused (may be NULL). */
+int rrh_atrous(rrh_ctx* ctx, const rrh_atrous_params* p, const rrh_image* input,
+               const rrh_image* gbuf_a, const rrh_image* gbuf_b, rrh_image* tmp0, rrh_image* tmp1,
+               int* out_slot);
+/* config-mode inputs from a DI frame: fp16 radiance + 3x3 luminance variance, fp16 normal + linear depth */
+int rrh_atrous_prepare(rrh_ctx* ctx, const rrh_scene* s, const rrh_image* radiance,
+                       const rrh_image* gbuf_pos, const rrh_image* gbuf_norm,
+                       rrh_image* rad16, rrh_image* gbuf16);
+
+/* ---- vMF guiding: the ReSTIR-PT guiding grid (assets/shaders/include/
+ *      GuidingGrid.slang:54-189; restir_pt.slang:520-563,982-1015; host
+ *      src/passes/restir_pt/ReSTIRPTPass.cpp:311-341,602-612,784-810) and the
+ *      8-lobe EM dense grid ------------------------------------------------ */
+#define RRH_GRID_HASH 0   /* reference: 2^20-cell hash grid, 1 lobe, int fixed-point splats */
+#define RRH_GRID_VMM 1    /* res^3 dense grid, 8 lobes, FP64 sufficient statistics, online EM */
+typedef struct {
+    int kind;
+    uint32_t cell_count;       /* HASH: 1<<20 (ReSTIRPTPass.h) */
+    float base_cell_size;      /* HASH: .10 */
+    float distance_slope;      /* HASH: .02 */
+    uint32_t min_samples;      /* HASH: 8 */
+    float kappa_max;           /* 512 */
+    uint32_t res;              /* VMM: cells per axis (32) */
+    uint32_t lobes;            /* VMM: 8 (max 8) */
+    float bounds_min[3], bounds_max[3];   /* VMM grid box */
+    float decay;               /* VMM: weight of previous statistics in online EM */
+    float min_weight;          /* VMM: cells with less total weight keep their lobes */
+} rrh_grid_params;
+int rrh_grid_create(rrh_ctx* ctx, const rrh_grid_params* p, rrh_grid** out);
+int rrh_grid_destroy(rrh_grid* g);
+int rrh_grid_reset(rrh_ctx* ctx, rrh_grid* g);
+/* HASH: guiding_cell_index + guiding_splat for n (position, bsdf id, direction) records */
+int rrh_grid_splat(rrh_ctx* ctx, rrh_grid* g, const float* pos, const uint32_t* bsdf_ids,
+                   const float* dirs, uint32_t n, const float cam_pos[3]);
+/* HASH: cs_fit_guiding over all cells (fits and resets the accumulators) */
+int rrh_grid_fit(rrh_ctx* ctx, rrh_grid* g);
+/* HASH: sample_primary_dir for n vertices; seeds in/out */
+int rrh_grid_sample_primary(rrh_ctx* ctx, rrh_grid* g, const float* pos, const float* nrm,
+                            const uint32_t* bsdf_ids, uint32_t* seeds, uint32_t n,
+                            const float cam_pos[3], float alpha, float* dirs, float* pdfs);
+/* VMM: E-step over n weighted (pos, dir) samples + M-step; host arrays */
+int rrh_grid_em_step(rrh_ctx* ctx, rrh_grid* g, const float* pos, const float* dirs,
+                     const float* weights, uint32_t n);
+/* VMM: E-step + M-step over the path vertices recorded by the last guided render */
+int rrh_grid_em_from_vertices(rrh_ctx* ctx, rrh_grid* g);
+/* read back state: HASH: fitted (4 floats/cell) and accum (4 u32/cell);
+ * VMM: lobes (5 floats/lobe: mu3, kappa, pi) and stats (4 doubles/lobe) */
+int rrh_grid_download(rrh_ctx* ctx, const rrh_grid* g, void* fitted_or_lobes, size_t bytes0,
+                      void* accum_or_stats, size_t bytes1);
+int rrh_grid_upload(rrh_ctx* ctx, rrh_grid* g, const void* fitted_or_lobes, size_t bytes0);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,582 @@
+// rrh_device.h -- device math shared by the gfx950 kernels.
+//
+// The shading arithmetic follows the reference Slang shaders operation by
+// operation (fp32, evaluated in source order; the library is compiled with
+// -ffp-contract=off so no FMA contraction changes a rounding).  sin/cos/exp/
+// log are the project's own routines (Cody-Waite reduction + Taylor/atanh
+// series) so results do not depend on a vendor math library's ULP behaviour.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/rrh.h"
+
+#define RRH_D __device__ __forceinline__
+
+namespace rrh {
+
+struct f3 { float x, y, z; };
+__host__ __device__ __forceinline__ f3 mk(float x, float y, float z) { f3 r; r.x = x; r.y = y; r.z = z; return r; }
+RRH_D f3 ld3(const float* p) { return mk(p[0], p[1], p[2]); }
+RRH_D f3 xyz(float4 a) { return mk(a.x, a.y, a.z); }
+RRH_D f3 add(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+RRH_D f3 sub(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+RRH_D f3 mul(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+RRH_D f3 scale(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+RRH_D f3 divs(f3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+RRH_D f3 neg(f3 a) { return mk(-a.x, -a.y, -a.z); }
+RRH_D float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+RRH_D f3 cross(f3 a, f3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+RRH_D float length(f3 a) { return __fsqrt_rn(dot(a, a)); }
+RRH_D f3 normalize(f3 a) { float inv = 1.0f / __fsqrt_rn(dot(a, a)); return scale(a, inv); }
+RRH_D float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+RRH_D float lerpf(float x, float y, float s) { return x * (1.0f - s) + y * s; }
+RRH_D float mrow(const float* M, int r, float x, float y, float z, float w) {
+    return M[r * 4 + 0] * x + M[r * 4 + 1] * y + M[r * 4 + 2] * z + M[r * 4 + 3] * w;
+}
+RRH_D float comp(f3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+// ---------------------------------------------------------------------------
+// deterministic transcendentals
+// ---------------------------------------------------------------------------
+RRH_D void sincos_rr(float x, float* s, float* c) {
+    float k = floorf(x * 0.636619747f + 0.5f);
+    float r = x - k * 1.5703125f;
+    r = r - k * 4.837512969970703e-4f;
+    r = r - k * 7.549790126404332e-8f;
+    int q = (int)k;
+    float r2 = r * r;
+    float sp = r + r * r2 * (-0.166666672f + r2 * (8.33333377e-3f + r2 * (-1.98412701e-4f + r2 * 2.75573188e-6f)));
+    float cp = 1.0f + r2 * (-0.5f + r2 * (4.16666679e-2f + r2 * (-1.38888892e-3f + r2 * (2.48015876e-5f + r2 * -2.75573188e-7f))));
+    int qq = q & 3;
+    float ss = (qq == 0) ? sp : ((qq == 1) ? cp : ((qq == 2) ? -sp : -cp));
+    float cc = (qq == 0) ? cp : ((qq == 1) ? -sp : ((qq == 2) ? -cp : sp));
+    *s = ss; *c = cc;
+}
+RRH_D float exp_rr(float x) {
+    if (x < -87.0f) return 0.0f;
+    if (x > 88.0f) x = 88.0f;
+    float k = floorf(x * 1.44269502f + 0.5f);
+    float r = x - k * 0.693145751953125f;
+    r = r - k * 1.42860677e-6f;
+    float p = 1.0f + r * (1.0f + r * (0.5f + r * (0.166666672f + r * (4.16666679e-2f + r * (8.33333377e-3f + r * (1.38888892e-3f + r * 1.98412701e-4f))))));
+    int ki = (int)k;
+    return p * __uint_as_float((uint32_t)(ki + 127) << 23);
+}
+RRH_D float log_rr(float x) {
+    if (!(x > 0.0f)) return (x == 0.0f) ? -__builtin_inff() : __builtin_nanf("");
+    if (__builtin_isinf(x)) return x;
+    int e = 0;
+    uint32_t u = __float_as_uint(x);
+    if (u < 0x00800000u) { u = __float_as_uint(x * 8388608.0f); e = -23; }
+    e += (int)((u >> 23) & 255u) - 127;
+    float m = __uint_as_float((u & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.41421354f) { m = m * 0.5f; e = e + 1; }
+    float s = (m - 1.0f) / (m + 1.0f);
+    float s2 = s * s;
+    float p = s * (2.0f + s2 * (0.666666687f + s2 * (0.400000006f + s2 * (0.285714298f + s2 * 0.222222224f))));
+    float ef = (float)e;
+    return ef * 0.693145751953125f + (p + ef * 1.42860677e-6f);
+}
+
+// ---------------------------------------------------------------------------
+// RNG: Common.slang:18-28, Sampling.slang:11-30
+// ---------------------------------------------------------------------------
+RRH_D uint32_t pcg_hash(uint32_t seed) {
+    uint32_t state = seed * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+RRH_D float u01(uint32_t v) { return (float)(v >> 8) * (1.0f / 16777216.0f); }
+RRH_D float rnd(uint32_t& seed) { seed = pcg_hash(seed); return u01(seed); }
+RRH_D uint32_t make_seed(uint32_t px, uint32_t py, uint32_t frame) {
+    return pcg_hash(px ^ (py * 1000003u) ^ (frame * 2654435761u));
+}
+
+// ---------------------------------------------------------------------------
+// camera rays: pathtracer.slang:195-205; pixel-centre sky ray restir_di.slang:355-358
+// ---------------------------------------------------------------------------
+RRH_D f3 cam_dir(const rrh_camera& cam, float ndcx, float ndcy) {
+    float vx = mrow(cam.inv_proj, 0, ndcx, ndcy, 1.0f, 1.0f);
+    float vy = mrow(cam.inv_proj, 1, ndcx, ndcy, 1.0f, 1.0f);
+    f3 w = mk(mrow(cam.inv_view, 0, vx, vy, -1.0f, 0.0f), mrow(cam.inv_view, 1, vx, vy, -1.0f, 0.0f),
+              mrow(cam.inv_view, 2, vx, vy, -1.0f, 0.0f));
+    return normalize(w);
+}
+RRH_D f3 primary_dir(const rrh_camera& cam, uint32_t px, uint32_t py, uint32_t& seed) {
+    float u0 = rnd(seed);
+    float u1 = rnd(seed);
+    float ndcx = ((float)px + u0) / (float)cam.screen_width * 2.0f - 1.0f;
+    float ndcy = ((float)py + u1) / (float)cam.screen_height * 2.0f - 1.0f;
+    return cam_dir(cam, ndcx, ndcy);
+}
+RRH_D f3 center_dir(const rrh_camera& cam, uint32_t px, uint32_t py) {
+    float ndcx = ((float)px + 0.5f) / (float)cam.screen_width * 2.0f - 1.0f;
+    float ndcy = ((float)py + 0.5f) / (float)cam.screen_height * 2.0f - 1.0f;
+    return cam_dir(cam, ndcx, ndcy);
+}
+RRH_D f3 sky(f3 d) {
+    float t = 0.5f * (d.y + 1.0f);
+    return mk(lerpf(0.1f, 0.2f, t), lerpf(0.1f, 0.3f, t), lerpf(0.1f, 0.5f, t));
+}
+
+// ---------------------------------------------------------------------------
+// device scene
+// ---------------------------------------------------------------------------
+struct DevScene {
+    const float4* nodes;       // 4 float4 per binary node (child boxes + child links)
+    const float4* tris;        // 3 float4 per triangle, leaf order; tris[3i].w = flattened id bits
+    const uint2* tri_ref;      // per flattened id: (instance, primitive within mesh)
+    const uint32_t* tri_light; // per flattened id: light index or ~0u
+    const float4* tl;          // per triangle light: (v0, Le.x) (v1, Le.y) (v2, Le.z) (id bits, 0, 0, 0)
+    const rrh_vertex* verts;
+    const uint32_t* idx;
+    const rrh_mesh* meshes;
+    const rrh_material* mats;
+    const rrh_instance* insts;
+    const rrh_light* lights;
+    const rrh_camera* cam;
+    uint32_t nl, ntl, ntri, flags;
+};
+
+// ---------------------------------------------------------------------------
+// watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013), two-sided
+// ---------------------------------------------------------------------------
+struct WRay { int kx, ky, kz; float Sx, Sy, Sz; f3 o, inv; };
+
+RRH_D WRay wray_setup(f3 o, f3 d) {
+    WRay r;
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = (ax > ay) ? ((ax > az) ? 0 : 2) : ((ay > az) ? 1 : 2);
+    int kx = kz + 1; if (kx == 3) kx = 0;
+    int ky = kx + 1; if (ky == 3) ky = 0;
+    float dkz = comp(d, kz);
+    if (dkz < 0.0f) { int t = kx; kx = ky; ky = t; }
+    r.kx = kx; r.ky = ky; r.kz = kz;
+    r.Sx = comp(d, kx) / dkz;
+    r.Sy = comp(d, ky) / dkz;
+    r.Sz = 1.0f / dkz;
+    r.o = o;
+    r.inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    return r;
+}
+
+RRH_D bool wt_hit(const WRay& r, f3 v0, f3 v1, f3 v2, float tmin, float tmax, float& t, float& b1, float& b2) {
+    f3 A = sub(v0, r.o), B = sub(v1, r.o), C = sub(v2, r.o);
+    float Az = comp(A, r.kz), Bz = comp(B, r.kz), Cz = comp(C, r.kz);
+    float Ax = comp(A, r.kx) - r.Sx * Az, Ay = comp(A, r.ky) - r.Sy * Az;
+    float Bx = comp(B, r.kx) - r.Sx * Bz, By = comp(B, r.ky) - r.Sy * Bz;
+    float Cx = comp(C, r.kx) - r.Sx * Cz, Cy = comp(C, r.ky) - r.Sy * Cz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    Az = r.Sz * Az; Bz = r.Sz * Bz; Cz = r.Sz * Cz;
+    float T = U * Az + V * Bz + W * Cz;
+    float rcp = 1.0f / det;
+    float tt = T * rcp;
+    if (!(tt >= tmin && tt <= tmax)) return false;
+    t = tt; b1 = V * rcp; b2 = W * rcp;
+    return true;
+}
+
+// conservative slab test (far distance scaled by 1 + 2*gamma(3), PBRT-v3 3.9.2)
+RRH_D bool box_hit(float lox, float hix, float loy, float hiy, float loz, float hiz, const WRay& r,
+                   float tmin, float tmax, float& tnear) {
+    float t0x = (lox - r.o.x) * r.inv.x, t1x = (hix - r.o.x) * r.inv.x;
+    float t0y = (loy - r.o.y) * r.inv.y, t1y = (hiy - r.o.y) * r.inv.y;
+    float t0z = (loz - r.o.z) * r.inv.z, t1z = (hiz - r.o.z) * r.inv.z;
+    float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+    // tmax is the current closest t: widened as well so an exact tie (equal t, smaller id) is never culled
+    float tf = fminf(fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * 1.0000008f, tmax * 1.000004f);
+    tnear = tn;
+    return tn <= tf;
+}
+
+RRH_D bool better(float t, uint32_t prim, float bt, uint32_t bp) { return t < bt || (t == bt && prim < bp); }
+
+// child link encoding: >= 0 inner node index; < 0 leaf = ~((first << 3) | (count - 1))
+#define RRH_LEAF_FIRST(c) ((uint32_t)(~(c)) >> 3)
+#define RRH_LEAF_COUNT(c) ((((uint32_t)(~(c))) & 7u) + 1u)
+#define RRH_STACK 64
+
+struct Hit { float t; uint32_t prim; float b1, b2; };
+
+// closest hit over the scene BVH; the result is independent of traversal
+// order (ties on t resolve to the smaller flattened id)
+// COUNT = true is the instrumented variant: it counts node and triangle fetches
+template <bool COUNT = false>
+RRH_D Hit trace_closest(const DevScene& S, f3 o, f3 d, float tmin, float tmax, uint32_t* n_nodes = nullptr,
+                        uint32_t* n_tris = nullptr) {
+    Hit h; h.t = tmax; h.prim = ~0u; h.b1 = 0.0f; h.b2 = 0.0f;
+    WRay r = wray_setup(o, d);
+    int stack[RRH_STACK];
+    int sp = 0;
+    int node = 0;
+    while (true) {
+        while (node >= 0) {
+            const float4* n = S.nodes + (size_t)node * 4;
+            if constexpr (COUNT) (*n_nodes)++;
+            float4 a = n[0], b = n[1], c = n[2];
+            int4 lk = *reinterpret_cast<const int4*>(n + 3);
+            float tl, tr;
+            bool hl = box_hit(a.x, a.y, a.z, a.w, c.x, c.y, r, tmin, h.t, tl);
+            bool hr = box_hit(b.x, b.y, b.z, b.w, c.z, c.w, r, tmin, h.t, tr);
+            if (hl && hr) {
+                int nearc = lk.x, farc = lk.y;
+                if (tr < tl) { nearc = lk.y; farc = lk.x; }
+                stack[sp++] = farc;
+                node = nearc;
+            } else if (hl) {
+                node = lk.x;
+            } else if (hr) {
+                node = lk.y;
+            } else {
+                if (sp == 0) return h;
+                node = stack[--sp];
+            }
+        }
+        // leaf
+        uint32_t first = RRH_LEAF_FIRST(node), cnt = RRH_LEAF_COUNT(node);
+        if constexpr (COUNT) (*n_tris) += cnt;
+        for (uint32_t i = 0; i < cnt; ++i) {
+            const float4* tv = S.tris + (size_t)(first + i) * 3;
+            float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
+            float tt, b1, b2;
+            uint32_t id = __float_as_uint(p0.w);
+            if (wt_hit(r, xyz(p0), xyz(p1), xyz(p2), tmin, h.t, tt, b1, b2) && better(tt, id, h.t, h.prim)) {
+                h.t = tt; h.prim = id; h.b1 = b1; h.b2 = b2;
+            }
+        }
+        if (sp == 0) return h;
+        node = stack[--sp];
+    }
+}
+
+RRH_D bool trace_occluded(const DevScene& S, f3 o, f3 d, float tmin, float tmax) {
+    if (!(tmax >= tmin)) return false;
+    WRay r = wray_setup(o, d);
+    int stack[RRH_STACK];
+    int sp = 0;
+    int node = 0;
+    while (true) {
+        while (node >= 0) {
+            const float4* n = S.nodes + (size_t)node * 4;
+            float4 a = n[0], b = n[1], c = n[2];
+            int4 lk = *reinterpret_cast<const int4*>(n + 3);
+            float tl, tr;
+            bool hl = box_hit(a.x, a.y, a.z, a.w, c.x, c.y, r, tmin, tmax, tl);
+            bool hr = box_hit(b.x, b.y, b.z, b.w, c.z, c.w, r, tmin, tmax, tr);
+            if (hl && hr) {
+                stack[sp++] = lk.y;
+                node = lk.x;
+            } else if (hl) {
+                node = lk.x;
+            } else if (hr) {
+                node = lk.y;
+            } else {
+                if (sp == 0) return false;
+                node = stack[--sp];
+            }
+        }
+        uint32_t first = RRH_LEAF_FIRST(node), cnt = RRH_LEAF_COUNT(node);
+        for (uint32_t i = 0; i < cnt; ++i) {
+            const float4* tv = S.tris + (size_t)(first + i) * 3;
+            float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
+            float tt, b1, b2;
+            if (wt_hit(r, xyz(p0), xyz(p1), xyz(p2), tmin, tmax, tt, b1, b2)) return true;
+        }
+        if (sp == 0) return false;
+        node = stack[--sp];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// hit attributes, pathtracer.slang:69-117
+// ---------------------------------------------------------------------------
+struct HitInfo { f3 pos, normal; uint32_t mat; };
+
+RRH_D f3 xform_point(const float* T, f3 p) {
+    return mk(mrow(T, 0, p.x, p.y, p.z, 1.0f), mrow(T, 1, p.x, p.y, p.z, 1.0f), mrow(T, 2, p.x, p.y, p.z, 1.0f));
+}
+RRH_D f3 xform_dir(const float* T, f3 p) {
+    return mk(T[0] * p.x + T[1] * p.y + T[2] * p.z, T[4] * p.x + T[5] * p.y + T[6] * p.z,
+              T[8] * p.x + T[9] * p.y + T[10] * p.z);
+}
+RRH_D HitInfo fetch_hit(const DevScene& S, uint32_t prim, float b1, float b2) {
+    HitInfo hi;
+    uint2 ref = S.tri_ref[prim];
+    const rrh_instance* in = S.insts + ref.x;
+    const rrh_mesh* me = S.meshes + in->mesh_index;
+    uint32_t base = me->index_byte_offset / 4u + ref.y * 3u;
+    uint32_t vb = me->vertex_base;
+    const rrh_vertex* v0 = S.verts + vb + S.idx[base];
+    const rrh_vertex* v1 = S.verts + vb + S.idx[base + 1];
+    const rrh_vertex* v2 = S.verts + vb + S.idx[base + 2];
+    float b0 = 1.0f - b1 - b2;
+    f3 lp = add(add(scale(ld3(v0->pos), b0), scale(ld3(v1->pos), b1)), scale(ld3(v2->pos), b2));
+    f3 ln = normalize(add(add(scale(ld3(v0->normal), b0), scale(ld3(v1->normal), b1)), scale(ld3(v2->normal), b2)));
+    hi.pos = xform_point(in->transform, lp);
+    hi.normal = normalize(xform_dir(in->transform, ln));
+    hi.mat = me->material_index;
+    return hi;
+}
+
+// ---------------------------------------------------------------------------
+// sampling + BRDF: Sampling.slang:37-66, BRDF.slang:16-187
+// ---------------------------------------------------------------------------
+#define K_PI 3.14159265358979323846f
+#define K_TWO_PI 6.28318530717958647692f
+#define K_INV_PI 0.31830988618379067154f
+#define K_EPS 1e-4f
+
+RRH_D f3 cosine_hemisphere(float u0, float u1, float& pdf) {
+    float r = __fsqrt_rn(u0);
+    float phi = K_TWO_PI * u1;
+    float s, c; sincos_rr(phi, &s, &c);
+    float x = r * c, y = r * s;
+    float z = __fsqrt_rn(fmaxf(0.0f, 1.0f - u0));
+    pdf = z * K_INV_PI;
+    return mk(x, y, z);
+}
+RRH_D void build_onb(f3 n, f3& T, f3& B) {
+    f3 up = (fabsf(n.x) < 0.9f) ? mk(1.0f, 0.0f, 0.0f) : mk(0.0f, 1.0f, 0.0f);
+    T = normalize(cross(up, n));
+    B = cross(n, T);
+}
+RRH_D f3 to_world(f3 l, f3 T, f3 B, f3 N) { return add(add(scale(T, l.x), scale(B, l.y)), scale(N, l.z)); }
+
+RRH_D f3 fresnel_schlick(float cos_theta, f3 F0) {
+    float x = clampf(1.0f - cos_theta, 0.0f, 1.0f);
+    float x2 = x * x;
+    float p = x2 * x2 * x;
+    return mk(F0.x + (1.0f - F0.x) * p, F0.y + (1.0f - F0.y) * p, F0.z + (1.0f - F0.z) * p);
+}
+RRH_D float distribution_ggx(float NdotH, float alpha) {
+    float a2 = alpha * alpha;
+    float d = NdotH * NdotH * (a2 - 1.0f) + 1.0f;
+    return a2 / fmaxf(K_PI * d * d, K_EPS);
+}
+RRH_D float geometry_smith_ggx(float NdotV, float NdotL, float alpha) {
+    float k = alpha * 0.5f;
+    float gv = NdotV / fmaxf(NdotV * (1.0f - k) + k, K_EPS);
+    float gl = NdotL / fmaxf(NdotL * (1.0f - k) + k, K_EPS);
+    return gv * gl;
+}
+RRH_D f3 brdf_specular_ggx(f3 n, f3 wo, f3 wi, f3 F0, float roughness) {
+    float alpha = roughness * roughness;
+    f3 h = normalize(add(wo, wi));
+    float NdotH = fmaxf(dot(n, h), 0.0f);
+    float NdotV = fmaxf(dot(n, wo), 0.0f);
+    float NdotL = fmaxf(dot(n, wi), 0.0f);
+    float VdotH = fmaxf(dot(wo, h), 0.0f);
+    f3 F = fresnel_schlick(VdotH, F0);
+    float D = distribution_ggx(NdotH, alpha);
+    float G = geometry_smith_ggx(NdotV, NdotL, alpha);
+    float den = fmaxf(4.0f * NdotV * NdotL, K_EPS);
+    float DG = D * G;
+    return mk(DG * F.x / den, DG * F.y / den, DG * F.z / den);
+}
+RRH_D f3 F0_of(f3 base, float metallic) {
+    return mk(lerpf(0.04f, base.x, metallic), lerpf(0.04f, base.y, metallic), lerpf(0.04f, base.z, metallic));
+}
+RRH_D f3 brdf_eval(f3 n, f3 wo, f3 wi, f3 base, float metallic, float roughness) {
+    float r = clampf(roughness, 0.04f, 1.0f);
+    f3 F0 = F0_of(base, metallic);
+    f3 kS = fresnel_schlick(fmaxf(dot(n, wo), 0.0f), F0);
+    f3 kD = mk((1.0f - kS.x) * (1.0f - metallic), (1.0f - kS.y) * (1.0f - metallic), (1.0f - kS.z) * (1.0f - metallic));
+    f3 diffuse = mul(kD, scale(base, K_INV_PI));
+    f3 spec = brdf_specular_ggx(n, wo, wi, F0, r);
+    return add(diffuse, spec);
+}
+RRH_D f3 sample_ggx_vndf(f3 wo, float roughness, float u0, float u1) {
+    float alpha = roughness * roughness;
+    f3 Vh = normalize(mk(alpha * wo.x, alpha * wo.y, wo.z));
+    float lensq = Vh.x * Vh.x + Vh.y * Vh.y;
+    f3 T1;
+    if (lensq > 0.0f) { float il = __fsqrt_rn(lensq); T1 = mk(-Vh.y / il, Vh.x / il, 0.0f / il); }
+    else T1 = mk(1.0f, 0.0f, 0.0f);
+    f3 T2 = cross(Vh, T1);
+    float r2 = __fsqrt_rn(u0);
+    float phi = K_TWO_PI * u1;
+    float s, c; sincos_rr(phi, &s, &c);
+    float t1 = r2 * c, t2 = r2 * s;
+    float sv = 0.5f * (1.0f + Vh.z);
+    t2 = (1.0f - sv) * __fsqrt_rn(1.0f - t1 * t1) + sv * t2;
+    float tz = __fsqrt_rn(fmaxf(0.0f, 1.0f - t1 * t1 - t2 * t2));
+    f3 Nh = add(add(scale(T1, t1), scale(T2, t2)), scale(Vh, tz));
+    return normalize(mk(alpha * Nh.x, alpha * Nh.y, fmaxf(0.0f, Nh.z)));
+}
+RRH_D float p_spec_of(f3 n, f3 wo, f3 base, float metallic) {
+    f3 kS = fresnel_schlick(fmaxf(dot(n, wo), 0.0f), F0_of(base, metallic));
+    float p = (kS.x + kS.y + kS.z) / 3.0f;
+    return clampf(p, 0.1f, 0.9f);
+}
+RRH_D f3 brdf_sample(f3 wo, f3 n, f3 base, float metallic, float roughness, float ux, float uy, f3& wi,
+                     float& pdf) {
+    float r = clampf(roughness, 0.04f, 1.0f);
+    float p_spec = p_spec_of(n, wo, base, metallic);
+    f3 T, B; build_onb(n, T, B);
+    f3 wol = mk(dot(wo, T), dot(wo, B), dot(wo, n));
+    f3 wil; float pdf_local;
+    if (ux < p_spec) {
+        f3 h = sample_ggx_vndf(wol, r, ux / p_spec, uy);
+        float d2 = 2.0f * dot(h, neg(wol));
+        wil = sub(neg(wol), scale(h, d2));
+        if (wil.z <= 0.0f) { wi = mk(0.0f, 1.0f, 0.0f); pdf = 1.0f; return mk(0.0f, 0.0f, 0.0f); }
+    } else {
+        wil = cosine_hemisphere((ux - p_spec) / (1.0f - p_spec), uy, pdf_local);
+    }
+    wi = to_world(wil, T, B, n);
+    float NdotWI = fmaxf(wil.z, 0.0f);
+    f3 h = normalize(add(wol, wil));
+    float NdotH = fmaxf(h.z, 1e-6f);
+    float VdotH = fmaxf(dot(wol, h), 1e-6f);
+    float D = distribution_ggx(NdotH, r * r);
+    float pdf_spec = D * NdotH / fmaxf(4.0f * VdotH, 1e-6f);
+    float pdf_diff = NdotWI * K_INV_PI;
+    float p = p_spec * pdf_spec + (1.0f - p_spec) * pdf_diff;
+    pdf = fmaxf(p, 1e-6f);
+    return brdf_eval(n, wo, wi, base, metallic, roughness);
+}
+RRH_D float brdf_pdf(f3 wo, f3 wi, f3 n, float roughness) {
+    float r = clampf(roughness, 0.04f, 1.0f);
+    f3 h = normalize(add(wo, wi));
+    float NdotH = fmaxf(dot(n, h), 1e-6f);
+    float VdotH = fmaxf(dot(wo, h), 1e-6f);
+    float NdotL = fmaxf(dot(n, wi), 0.0f);
+    float D = distribution_ggx(NdotH, r * r);
+    float pdf_spec = D * NdotH / fmaxf(4.0f * VdotH, 1e-6f);
+    float pdf_diff = NdotL * K_INV_PI;
+    return 0.5f * pdf_spec + 0.5f * pdf_diff;
+}
+RRH_D float brdf_sample_pdf(f3 wo, f3 wi, f3 n, f3 base, float metallic, float roughness) {
+    float r = clampf(roughness, 0.04f, 1.0f);
+    float p_spec = p_spec_of(n, wo, base, metallic);
+    f3 h = normalize(add(wo, wi));
+    float NdotH = fmaxf(dot(n, h), 1e-6f);
+    float VdotH = fmaxf(dot(wo, h), 1e-6f);
+    float D = distribution_ggx(NdotH, r * r);
+    float pdf_spec = D * NdotH / fmaxf(4.0f * VdotH, 1e-6f);
+    float pdf_diff = fmaxf(dot(n, wi), 0.0f) * K_INV_PI;
+    return fmaxf(p_spec * pdf_spec + (1.0f - p_spec) * pdf_diff, 1e-6f);
+}
+
+struct Mat { f3 base; float metallic, roughness; uint32_t lambert; f3 emissive; };
+RRH_D Mat load_mat(const DevScene& S, uint32_t mi) {
+    const rrh_material* m = S.mats + mi;
+    Mat r;
+    r.base = ld3(m->base_color); r.metallic = m->metallic; r.roughness = m->roughness;
+    r.lambert = (m->flags & RRH_MAT_PURE_LAMBERT) ? 1u : 0u;
+    r.emissive = ld3(m->emissive);
+    return r;
+}
+RRH_D bool is_emissive(const Mat& m) { return m.emissive.x > 0.0f || m.emissive.y > 0.0f || m.emissive.z > 0.0f; }
+RRH_D f3 mat_eval(const Mat& m, f3 n, f3 wo, f3 wi) {
+    if (m.lambert) return scale(m.base, K_INV_PI);
+    return brdf_eval(n, wo, wi, m.base, m.metallic, m.roughness);
+}
+RRH_D f3 mat_sample(const Mat& m, f3 wo, f3 n, float ux, float uy, f3& wi, float& pdf) {
+    if (m.lambert) {
+        f3 T, B; build_onb(n, T, B);
+        f3 c = cosine_hemisphere(ux, uy, pdf);
+        wi = to_world(c, T, B, n);
+        return scale(m.base, K_INV_PI);
+    }
+    return brdf_sample(wo, n, m.base, m.metallic, m.roughness, ux, uy, wi, pdf);
+}
+RRH_D float mat_sample_pdf(const Mat& m, f3 wo, f3 wi, f3 n) {
+    if (m.lambert) return fmaxf(dot(n, wi), 0.0f) * K_INV_PI;
+    return brdf_sample_pdf(wo, wi, n, m.base, m.metallic, m.roughness);
+}
+RRH_D float mat_ref_pdf(const Mat& m, f3 wo, f3 wi, f3 n) {
+    if (m.lambert) return fmaxf(dot(n, wi), 0.0f) * K_INV_PI;
+    return brdf_pdf(wo, wi, n, m.roughness);
+}
+
+// ---------------------------------------------------------------------------
+// triangle lights (new; the reference lights are point / directional only)
+// ---------------------------------------------------------------------------
+struct TriSample { f3 p, nl; float area; f3 Le; };
+RRH_D TriSample tri_light_point(const DevScene& S, uint32_t tl, float u1, float u2) {
+    const float4* L = S.tl + (size_t)tl * 4;
+    float4 q0 = L[0], q1 = L[1], q2 = L[2];
+    f3 a = xyz(q0), b = xyz(q1), c = xyz(q2);
+    float su = __fsqrt_rn(u1);
+    float b0 = 1.0f - su;
+    float b1 = u2 * su;
+    float b2 = 1.0f - b0 - b1;
+    TriSample r;
+    r.p = add(add(scale(a, b0), scale(b, b1)), scale(c, b2));
+    f3 cr = cross(sub(b, a), sub(c, a));
+    float l2 = length(cr);
+    r.area = 0.5f * l2;
+    r.nl = scale(cr, 1.0f / l2);
+    r.Le = mk(q0.w, q1.w, q2.w);
+    return r;
+}
+RRH_D uint32_t quant_uv(float u1, float u2) {
+    uint32_t a = (uint32_t)(u1 * 65535.0f + 0.5f), b = (uint32_t)(u2 * 65535.0f + 0.5f);
+    return a | (b << 16);
+}
+RRH_D void dequant_uv(uint32_t q, float& u1, float& u2) {
+    u1 = (float)(q & 0xffffu) * (1.0f / 65535.0f);
+    u2 = (float)(q >> 16) * (1.0f / 65535.0f);
+}
+RRH_D float tri_light_pdf(const DevScene& S, uint32_t tl, f3 dir, float t) {
+    const float4* L = S.tl + (size_t)tl * 4;
+    f3 a = xyz(L[0]), b = xyz(L[1]), c = xyz(L[2]);
+    f3 cr = cross(sub(b, a), sub(c, a));
+    float l2 = length(cr);
+    float area = 0.5f * l2;
+    float cosl = fabsf(dot(scale(cr, 1.0f / l2), dir));
+    float N = (float)(S.nl + S.ntl);
+    return (t * t) / (cosl * area) / N;
+}
+RRH_D float lum709(f3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+
+// ---------------------------------------------------------------------------
+// vMF lobe: GuidingGrid.slang:160-189
+// ---------------------------------------------------------------------------
+RRH_D f3 vmf_sample(f3 mu, float kappa, float u0, float u1, float& pdf) {
+    float k = fmaxf(kappa, 1.0e-3f);
+    float w = 1.0f + (1.0f / k) * log_rr(u0 + (1.0f - u0) * exp_rr(-2.0f * k));
+    w = clampf(w, -1.0f, 1.0f);
+    float s = __fsqrt_rn(fmaxf(0.0f, 1.0f - w * w));
+    float phi = K_TWO_PI * u1;
+    float sp, cp; sincos_rr(phi, &sp, &cp);
+    f3 local = mk(s * cp, s * sp, w);
+    f3 T, B; build_onb(mu, T, B);
+    f3 dir = to_world(local, T, B, mu);
+    float denom = K_TWO_PI * (1.0f - exp_rr(-2.0f * k));
+    pdf = k * exp_rr(k * (w - 1.0f)) / fmaxf(denom, 1.0e-8f);
+    return dir;
+}
+RRH_D float vmf_pdf(f3 mu, float kappa, f3 dir) {
+    float k = fmaxf(kappa, 1.0e-3f);
+    float w = clampf(dot(mu, dir), -1.0f, 1.0f);
+    float denom = K_TWO_PI * (1.0f - exp_rr(-2.0f * k));
+    return k * exp_rr(k * (w - 1.0f)) / fmaxf(denom, 1.0e-8f);
+}
+
+// wave-aggregated queue push: returns this lane's slot (valid only where pred)
+RRH_D uint32_t wave_push(uint32_t* counter, bool pred) {
+    unsigned long long mask = __ballot(pred ? 1 : 0);
+    if (mask == 0ull) return 0u;
+    uint32_t lane = __lane_id();
+    uint32_t leader = (uint32_t)(__ffsll((long long)mask) - 1);
+    uint32_t base = 0u;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, (int)leader);
+    unsigned long long lt = (lane == 0u) ? 0ull : (~0ull >> (64u - lane));
+    return base + (uint32_t)__popcll(mask & lt);
+}
+
+}  // namespace rrh

@@ -1,0 +1,523 @@
+// rrh_guide.hip -- vMF path guiding.
+//
+// RRH_GRID_HASH restates the reference's ReSTIR-PT guiding grid:
+//   cell index   GuidingGrid.slang:54-67     splat  :73-90 (int fixed-point atomics)
+//   fit          :100-137 / cs_fit_guiding restir_pt.slang:1005-1015 (one thread per cell)
+//   sample       sample_primary_dir restir_pt.slang:520-563
+//   buffers      ReSTIRPTPass.cpp:311-341 (accum + fitted, 16 B per cell each)
+// RRH_GRID_VMM is new: a res^3 dense grid with K <= 8 vMF lobes per cell fitted
+// by online EM.  E-step: responsibilities in fp32, sufficient statistics
+// (w*g, w*g*dir) accumulated with FP64 atomics.  M-step per cell: weights from
+// the responsibility mass and the closed-form mean-resultant fit of the
+// reference (GuidingGrid.slang:125-136) per lobe, so K = 1 reproduces it.
+#include "rrh_internal.h"
+#include "rrh_guide_dev.h"
+
+using namespace rrh;
+
+struct rrh_grid {
+    rrh_ctx* ctx = nullptr;
+    rrh_grid_params p{};
+    uint32_t ncell = 0;
+    DevBuf accum, fitted;              // HASH
+    DevBuf lobes, stats, fstats, valid; // VMM
+};
+
+namespace {
+
+// GuidingGrid.slang:54-67
+__device__ __forceinline__ uint32_t hash_cell(f3 p, uint32_t bsdf, f3 cam, float base, float slope, uint32_t count) {
+    float vd = length(sub(p, cam));
+    float cell = fmaxf(base + slope * vd, 1.0e-4f);
+    int qx = (int)floorf(p.x / cell), qy = (int)floorf(p.y / cell), qz = (int)floorf(p.z / cell);
+    uint32_t h = pcg_hash((uint32_t)qx * 73856093u ^ (uint32_t)qy * 19349663u ^ (uint32_t)qz * 83492791u ^
+                          bsdf * 2654435761u);
+    return h % (count > 1u ? count : 1u);
+}
+
+__global__ void k_hash_splat(const float* __restrict__ pos, const uint32_t* __restrict__ bsdf,
+                             const float* __restrict__ dir, uint32_t n, f3 cam, float base, float slope,
+                             uint32_t count, uint32_t* __restrict__ accum) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    f3 p = mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    uint32_t c = hash_cell(p, bsdf[i], cam, base, slope, count);
+    uint32_t* a = accum + (size_t)c * 4;
+    // GuidingGrid.slang:81-89: round to the nearest 2^-20, add as two's complement
+    atomicAdd(a + 0, (uint32_t)(int32_t)roundf(dir[3 * i + 0] * 1048576.0f));
+    atomicAdd(a + 1, (uint32_t)(int32_t)roundf(dir[3 * i + 1] * 1048576.0f));
+    atomicAdd(a + 2, (uint32_t)(int32_t)roundf(dir[3 * i + 2] * 1048576.0f));
+    atomicAdd(a + 3, 1u);
+}
+
+__global__ void k_hash_fit(uint4* __restrict__ accum, float4* __restrict__ fitted, uint32_t count,
+                           uint32_t min_samples, float kappa_max) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= count) return;
+    uint4 a = accum[c];
+    accum[c] = make_uint4(0, 0, 0, 0);
+    uint32_t n = a.w;
+    if (n < (min_samples > 1u ? min_samples : 1u)) {
+        fitted[c].w = -1.0f;
+        return;
+    }
+    f3 sum = scale(mk((float)(int32_t)a.x, (float)(int32_t)a.y, (float)(int32_t)a.z), 1.0f / 1048576.0f);
+    float len = length(sum);
+    float r = len / (float)n;
+    r = clampf(r, 1.0e-4f, 0.9999f);
+    f3 mu = (len > 1.0e-6f) ? divs(sum, len) : mk(0.0f, 0.0f, 1.0f);
+    float kappa = r * (3.0f - r * r) / (1.0f - r * r);
+    kappa = clampf(kappa, 0.5f, kappa_max);
+    fitted[c] = make_float4(mu.x, mu.y, mu.z, kappa);
+}
+
+// restir_pt.slang:520-563
+__global__ void k_hash_sample(const float4* __restrict__ fitted, const float* __restrict__ pos,
+                              const float* __restrict__ nrm, const uint32_t* __restrict__ bsdf,
+                              uint32_t* __restrict__ seeds, uint32_t n, f3 cam, float base, float slope,
+                              uint32_t count, float alpha_in, float* __restrict__ dirs, float* __restrict__ pdfs) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    f3 p = mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    f3 N = mk(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+    uint32_t seed = seeds[i];
+    float alpha = 0.0f;
+    bool valid = false;
+    f3 mu = mk(0, 0, 0);
+    float kappa = 0.0f;
+    if (alpha_in > 0.0f) {
+        uint32_t c = hash_cell(p, bsdf[i], cam, base, slope, count);
+        float4 f = fitted[c];
+        mu = xyz(f); kappa = f.w;
+        valid = (kappa > 0.0f) && (dot(mu, mu) > 0.25f);
+        if (valid) alpha = alpha_in;
+    }
+    f3 T, B; build_onb(N, T, B);
+    f3 wi;
+    if (rnd(seed) < alpha) {
+        float u0 = rnd(seed), u1 = rnd(seed), vp;
+        wi = vmf_sample(mu, kappa, u0, u1, vp);
+    } else {
+        float u0 = rnd(seed), u1 = rnd(seed), cp;
+        wi = to_world(cosine_hemisphere(u0, u1, cp), T, B, N);
+    }
+    float ci = dot(N, wi);
+    float pc = (ci > 0.0f) ? (ci * K_INV_PI) : 0.0f;
+    float pv = valid ? vmf_pdf(mu, kappa, wi) : 0.0f;
+    pdfs[i] = alpha * pv + (1.0f - alpha) * pc;
+    dirs[3 * i] = wi.x; dirs[3 * i + 1] = wi.y; dirs[3 * i + 2] = wi.z;
+    seeds[i] = seed;
+}
+
+// ---- VMM ----------------------------------------------------------------------
+struct VmmArgs {
+    uint32_t res, K;
+    float bmin[3], bext[3];
+};
+
+__device__ __forceinline__ void vmm_estep_one(const VmmArgs& A, const float* __restrict__ lobes, f3 p, f3 d, float w,
+                                              double* __restrict__ st) {
+    if (!(w > 0.0f)) return;
+    GuideDev G;
+    G.res = A.res; G.K = A.K;
+    for (int k = 0; k < 3; ++k) { G.bmin[k] = A.bmin[k]; G.bext[k] = A.bext[k]; }
+    uint32_t c = vmm_cell(G, p);
+    const float* lc = lobes + (size_t)c * A.K * 5;
+    float g[kMaxLobes];
+    float tot = 0.0f;
+    for (uint32_t k = 0; k < A.K; ++k) {
+        g[k] = lc[k * 5 + 4] * vmf_pdf(mk(lc[k * 5], lc[k * 5 + 1], lc[k * 5 + 2]), lc[k * 5 + 3], d);
+        tot += g[k];
+    }
+    double* sc = st + (size_t)c * A.K * 4;
+    for (uint32_t k = 0; k < A.K; ++k) {
+        float gk = (tot > 0.0f) ? g[k] / tot : 1.0f / (float)A.K;
+        double wg = (double)w * (double)gk;
+        if (wg == 0.0) continue;
+        atomicAdd(sc + k * 4 + 0, wg);
+        atomicAdd(sc + k * 4 + 1, wg * (double)d.x);
+        atomicAdd(sc + k * 4 + 2, wg * (double)d.y);
+        atomicAdd(sc + k * 4 + 3, wg * (double)d.z);
+    }
+}
+
+__global__ void k_vmm_estep(VmmArgs A, const float* __restrict__ lobes, const float* __restrict__ pos,
+                            const float* __restrict__ dir, const float* __restrict__ w, uint32_t n,
+                            double* __restrict__ st) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    vmm_estep_one(A, lobes, mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]), mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]),
+                  w[i], st);
+}
+
+// training samples: t0 (pos, w) t1 (dir, pdf) t2 (n, 0) t3 (wo, 0)
+__global__ void k_vmm_estep_train(VmmArgs A, const float* __restrict__ lobes, const float4* __restrict__ tr,
+                                  const uint32_t* __restrict__ cnt, uint32_t cap, double* __restrict__ st) {
+    uint32_t n = min(*cnt, cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float4 t0 = tr[(size_t)i * 4], t1 = tr[(size_t)i * 4 + 1];
+        vmm_estep_one(A, lobes, xyz(t0), xyz(t1), t0.w, st);
+    }
+}
+
+// running stats = decay * running + frame; M-step; frame stats cleared
+__global__ void k_vmm_mstep(uint32_t ncell, uint32_t K, float decay, float kappa_max, double min_weight,
+                            double* __restrict__ run, double* __restrict__ frame, float* __restrict__ lobes,
+                            uint32_t* __restrict__ valid) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    double* R = run + (size_t)c * K * 4;
+    double* F = frame + (size_t)c * K * 4;
+    double W = 0.0;
+    for (uint32_t j = 0; j < K * 4; ++j) {
+        R[j] = (double)decay * R[j] + F[j];
+        F[j] = 0.0;
+    }
+    for (uint32_t k = 0; k < K; ++k) W += R[k * 4];
+    if (!(W > min_weight)) return;
+    float* lc = lobes + (size_t)c * K * 5;
+    for (uint32_t k = 0; k < K; ++k) {
+        double s0 = R[k * 4], sx = R[k * 4 + 1], sy = R[k * 4 + 2], sz = R[k * 4 + 3];
+        double len = sqrt(sx * sx + sy * sy + sz * sz);
+        if (!(s0 > 0.0)) { lc[k * 5 + 4] = 0.0f; continue; }
+        double r = len / s0;
+        r = r < 1e-4 ? 1e-4 : (r > 0.9999 ? 0.9999 : r);
+        if (len > 1e-12) { lc[k * 5] = (float)(sx / len); lc[k * 5 + 1] = (float)(sy / len); lc[k * 5 + 2] = (float)(sz / len); }
+        double kap = r * (3.0 - r * r) / (1.0 - r * r);
+        kap = kap < 0.5 ? 0.5 : (kap > (double)kappa_max ? (double)kappa_max : kap);
+        lc[k * 5 + 3] = (float)kap;
+        lc[k * 5 + 4] = (float)(s0 / W);
+    }
+    valid[c] = 1u;
+}
+
+// initial lobes: cube-corner directions, kappa 2, equal weights (cells start invalid)
+__global__ void k_vmm_init(uint32_t ncell, uint32_t K, float* __restrict__ lobes, uint32_t* __restrict__ valid) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    float* lc = lobes + (size_t)c * K * 5;
+    const float s = 0.577350269f;
+    for (uint32_t k = 0; k < K; ++k) {
+        lc[k * 5 + 0] = (k & 1u) ? s : -s;
+        lc[k * 5 + 1] = (k & 2u) ? s : -s;
+        lc[k * 5 + 2] = (k & 4u) ? s : -s;
+        lc[k * 5 + 3] = 2.0f;
+        lc[k * 5 + 4] = 1.0f / (float)K;
+    }
+    valid[c] = 0u;
+}
+
+// lobes exposed to sampling: a cell is used only after its first M-step
+__global__ void k_vmm_mask(uint32_t ncell, uint32_t K, const float* __restrict__ lobes, const uint32_t* __restrict__ valid,
+                           float* __restrict__ out) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    for (uint32_t j = 0; j < K * 5; ++j) {
+        float v = lobes[(size_t)c * K * 5 + j];
+        if (!valid[c] && (j % 5u) == 4u) v = 0.0f;
+        out[(size_t)c * K * 5 + j] = v;
+    }
+}
+
+// ---- path-vertex finalisation: incident radiance along the sampled direction ----
+__global__ void k_finalize(const float4* __restrict__ rec, uint32_t nrec, uint32_t max_bounces, uint32_t stamp,
+                           const float4* __restrict__ rad, uint32_t keep_thresh, float4* __restrict__ train,
+                           uint32_t* __restrict__ cnt, uint32_t cap) {
+    for (uint32_t base = blockIdx.x * blockDim.x; base < nrec; base += gridDim.x * blockDim.x) {
+        uint32_t i = base + threadIdx.x;
+        bool take = false;
+        float4 t0 = make_float4(0, 0, 0, 0), t1 = t0, t2 = t0, t3 = t0;
+        if (i < nrec) {
+            const float4* r = rec + (size_t)i * kRecF4;
+            float4 r0 = r[0];
+            if (__float_as_uint(r0.w) == stamp && (pcg_hash(i ^ (stamp * 2654435761u)) >> 8) < keep_thresh) {
+                float4 r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5], r6 = r[6];
+                uint32_t pixel = i / max_bounces;
+                float4 Lf = rad[pixel];
+                f3 Ls = sub(sub(xyz(Lf), xyz(r4)), (r3.w > 0.5f) ? xyz(r5) : mk(0, 0, 0));
+                f3 tn = xyz(r6);
+                f3 Li = mk(tn.x > 0.0f ? Ls.x / tn.x : 0.0f, tn.y > 0.0f ? Ls.y / tn.y : 0.0f, tn.z > 0.0f ? Ls.z / tn.z : 0.0f);
+                float w = fmaxf(lum709(Li), 0.0f);
+                if (r1.w > 0.0f && w > 0.0f && isfinite(w)) {
+                    take = true;
+                    t0 = make_float4(r0.x, r0.y, r0.z, w / r1.w);
+                    t1 = make_float4(r3.x, r3.y, r3.z, r1.w);
+                    t2 = make_float4(r1.x, r1.y, r1.z, 0.0f);
+                    t3 = make_float4(r2.x, r2.y, r2.z, 0.0f);
+                }
+            }
+        }
+        uint32_t slot = wave_push(cnt, take);
+        if (take && slot < cap) {
+            train[(size_t)slot * 4] = t0; train[(size_t)slot * 4 + 1] = t1;
+            train[(size_t)slot * 4 + 2] = t2; train[(size_t)slot * 4 + 3] = t3;
+        }
+    }
+}
+
+VmmArgs vmm_args(const rrh_grid* g) {
+    VmmArgs A;
+    A.res = g->p.res; A.K = g->p.lobes;
+    for (int k = 0; k < 3; ++k) { A.bmin[k] = g->p.bounds_min[k]; A.bext[k] = g->p.bounds_max[k] - g->p.bounds_min[k]; }
+    return A;
+}
+
+int vmm_mstep(rrh_grid* g) {
+    rrh_ctx* c = g->ctx;
+    hipLaunchKernelGGL(k_vmm_mstep, dim3((g->ncell + 255) / 256), dim3(256), 0, c->stream, g->ncell, g->p.lobes,
+                       g->p.decay, g->p.kappa_max, (double)g->p.min_weight, (double*)g->stats.p, (double*)g->fstats.p,
+                       (float*)g->lobes.p, (uint32_t*)g->valid.p);
+    RRH_LAUNCH_CHECK();
+    return RRH_OK;
+}
+
+}  // namespace
+
+namespace rrh {
+
+int guide_prepare(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* grid, rrh_mlp* mlp, GuideDev& G,
+                  VertexRec& R) {
+    (void)s; (void)mlp;
+    G = GuideDev();
+    R = VertexRec();
+    if (p->guide_mode == RRH_GUIDE_VMM) {
+        RRH_ARG(grid && grid->p.kind == RRH_GRID_VMM, "guided render: RRH_GUIDE_VMM needs a VMM grid");
+        int rc = c->scratch.ensure((size_t)grid->ncell * grid->p.lobes * 5 * 4);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_vmm_mask, dim3((grid->ncell + 255) / 256), dim3(256), 0, c->stream, grid->ncell,
+                           grid->p.lobes, (const float*)grid->lobes.p, (const uint32_t*)grid->valid.p, (float*)c->scratch.p);
+        RRH_LAUNCH_CHECK();
+        G.lobes = (const float*)c->scratch.p;
+        G.res = grid->p.res;
+        G.K = grid->p.lobes;
+        for (int k = 0; k < 3; ++k) { G.bmin[k] = grid->p.bounds_min[k]; G.bext[k] = grid->p.bounds_max[k] - grid->p.bounds_min[k]; }
+    } else if (p->guide_mode == RRH_GUIDE_MLP) {
+        set_error("guided render: the learned guide is not available in this build");
+        return RRH_E_STATE;
+    }
+    if (p->record_vertices) {
+        uint32_t npix = s->cam_host.screen_width * s->cam_host.screen_height;
+        size_t nrec = (size_t)npix * p->max_bounces;
+        int rc = c->vtx.ensure(nrec * kRecF4 * 16);
+        if (rc) return rc;
+        c->stamp++;
+        if (c->stamp == 0) c->stamp = 1;
+        R.v = (float4*)c->vtx.p;
+        R.max_bounces = p->max_bounces;
+        R.stamp = c->stamp;
+        c->last_rec_bounces = p->max_bounces;
+    }
+    return RRH_OK;
+}
+
+int guide_finish(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* grid, rrh_mlp* mlp, const VertexRec& R,
+                 const float4* rad) {
+    (void)grid; (void)mlp;
+    if (!R.v) return RRH_OK;
+    uint32_t npix = s->cam_host.screen_width * s->cam_host.screen_height;
+    uint32_t nrec = npix * p->max_bounces;
+    uint32_t target = p->record_vertices;
+    uint32_t cap = target;
+    int rc = c->train.ensure((size_t)cap * 64);
+    if (rc) return rc;
+    if ((rc = c->train_cnt.ensure(4))) return rc;
+    c->train_capacity = cap;
+    // keep about `target` of the (roughly nrec/2) live vertices
+    double keep = (double)target / ((double)nrec * 0.5);
+    uint32_t thresh = keep >= 1.0 ? (1u << 24) : (uint32_t)(keep * (double)(1u << 24));
+    RRH_HIP(hipMemsetAsync(c->train_cnt.p, 0, 4, c->stream));
+    uint32_t g = (nrec + 255) / 256;
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(k_finalize, dim3(g), dim3(256), 0, c->stream, (const float4*)c->vtx.p, nrec, p->max_bounces,
+                       R.stamp, rad, thresh,
+                       (float4*)c->train.p, (uint32_t*)c->train_cnt.p, cap);
+    RRH_LAUNCH_CHECK();
+    return RRH_OK;
+}
+
+}  // namespace rrh
+
+extern "C" {
+
+int rrh_grid_create(rrh_ctx* c, const rrh_grid_params* p, rrh_grid** out) {
+    RRH_ARG(c && p && out, "rrh_grid_create: null argument");
+    *out = nullptr;
+    rrh_grid* g = new rrh_grid();
+    g->ctx = c;
+    g->p = *p;
+    int rc = RRH_OK;
+    if (p->kind == RRH_GRID_HASH) {
+        if (p->cell_count == 0 || p->cell_count > (1u << 26)) { delete g; set_error("rrh_grid_create: cell_count must be in [1, 2^26]"); return RRH_E_ARG; }
+        g->ncell = p->cell_count;
+        if (!rc) rc = g->accum.ensure((size_t)g->ncell * 16);
+        if (!rc) rc = g->fitted.ensure((size_t)g->ncell * 16);
+    } else if (p->kind == RRH_GRID_VMM) {
+        if (p->res == 0 || p->res > 128 || p->lobes == 0 || p->lobes > kMaxLobes) { delete g; set_error("rrh_grid_create: res in [1,128], lobes in [1,8]"); return RRH_E_ARG; }
+        for (int k = 0; k < 3; ++k)
+            if (!(p->bounds_max[k] > p->bounds_min[k])) { delete g; set_error("rrh_grid_create: empty bounds"); return RRH_E_ARG; }
+        g->ncell = p->res * p->res * p->res;
+        size_t nl = (size_t)g->ncell * p->lobes;
+        if (!rc) rc = g->lobes.ensure(nl * 5 * 4);
+        if (!rc) rc = g->stats.ensure(nl * 4 * 8);
+        if (!rc) rc = g->fstats.ensure(nl * 4 * 8);
+        if (!rc) rc = g->valid.ensure((size_t)g->ncell * 4);
+    } else {
+        delete g;
+        set_error("rrh_grid_create: unknown kind");
+        return RRH_E_ARG;
+    }
+    if (rc) { rrh_grid_destroy(g); return rc; }
+    rc = rrh_grid_reset(c, g);
+    if (rc) { rrh_grid_destroy(g); return rc; }
+    *out = g;
+    return RRH_OK;
+}
+
+int rrh_grid_destroy(rrh_grid* g) {
+    if (!g) return RRH_OK;
+    if (g->ctx) hipStreamSynchronize(g->ctx->stream);
+    g->accum.release(); g->fitted.release(); g->lobes.release(); g->stats.release(); g->fstats.release(); g->valid.release();
+    delete g;
+    return RRH_OK;
+}
+
+// ReSTIRPTPass.cpp:602-612 clears both buffers when the grid is dirty
+int rrh_grid_reset(rrh_ctx* c, rrh_grid* g) {
+    RRH_ARG(c && g, "rrh_grid_reset: null argument");
+    if (g->p.kind == RRH_GRID_HASH) {
+        RRH_HIP(hipMemsetAsync(g->accum.p, 0, g->accum.bytes, c->stream));
+        RRH_HIP(hipMemsetAsync(g->fitted.p, 0, g->fitted.bytes, c->stream));
+    } else {
+        RRH_HIP(hipMemsetAsync(g->stats.p, 0, g->stats.bytes, c->stream));
+        RRH_HIP(hipMemsetAsync(g->fstats.p, 0, g->fstats.bytes, c->stream));
+        hipLaunchKernelGGL(k_vmm_init, dim3((g->ncell + 255) / 256), dim3(256), 0, c->stream, g->ncell, g->p.lobes,
+                           (float*)g->lobes.p, (uint32_t*)g->valid.p);
+        RRH_LAUNCH_CHECK();
+    }
+    return RRH_OK;
+}
+
+static int upload_tmp(rrh_ctx* c, const void* src, size_t bytes, void** dst) {
+    RRH_HIP(hipMalloc(dst, bytes ? bytes : 16));
+    if (bytes) RRH_HIP(hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    return RRH_OK;
+}
+
+int rrh_grid_splat(rrh_ctx* c, rrh_grid* g, const float* pos, const uint32_t* bsdf, const float* dirs, uint32_t n,
+                   const float cam_pos[3]) {
+    RRH_ARG(c && g && pos && bsdf && dirs && cam_pos, "rrh_grid_splat: null argument");
+    RRH_ARG(g->p.kind == RRH_GRID_HASH, "rrh_grid_splat: HASH grids only");
+    if (n == 0) return RRH_OK;
+    void *dp = nullptr, *db = nullptr, *dd = nullptr;
+    int rc = upload_tmp(c, pos, (size_t)n * 12, &dp);
+    if (!rc) rc = upload_tmp(c, bsdf, (size_t)n * 4, &db);
+    if (!rc) rc = upload_tmp(c, dirs, (size_t)n * 12, &dd);
+    if (!rc) {
+        hipLaunchKernelGGL(k_hash_splat, dim3((n + 255) / 256), dim3(256), 0, c->stream, (const float*)dp,
+                           (const uint32_t*)db, (const float*)dd, n, mk(cam_pos[0], cam_pos[1], cam_pos[2]),
+                           g->p.base_cell_size, g->p.distance_slope, g->p.cell_count, (uint32_t*)g->accum.p);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) { set_error(hipGetErrorString(e)); rc = RRH_E_HIP; }
+    }
+    hipFree(dp); hipFree(db); hipFree(dd);
+    return rc;
+}
+
+int rrh_grid_fit(rrh_ctx* c, rrh_grid* g) {
+    RRH_ARG(c && g, "rrh_grid_fit: null argument");
+    if (g->p.kind == RRH_GRID_HASH) {
+        hipLaunchKernelGGL(k_hash_fit, dim3((g->ncell + 63) / 64), dim3(64), 0, c->stream, (uint4*)g->accum.p,
+                           (float4*)g->fitted.p, g->ncell, g->p.min_samples, g->p.kappa_max);
+        RRH_LAUNCH_CHECK();
+        return RRH_OK;
+    }
+    return vmm_mstep(g);
+}
+
+int rrh_grid_sample_primary(rrh_ctx* c, rrh_grid* g, const float* pos, const float* nrm, const uint32_t* bsdf,
+                            uint32_t* seeds, uint32_t n, const float cam_pos[3], float alpha, float* dirs, float* pdfs) {
+    RRH_ARG(c && g && pos && nrm && bsdf && seeds && cam_pos && dirs && pdfs, "rrh_grid_sample_primary: null argument");
+    RRH_ARG(g->p.kind == RRH_GRID_HASH, "rrh_grid_sample_primary: HASH grids only");
+    if (n == 0) return RRH_OK;
+    void *dp = nullptr, *dn = nullptr, *db = nullptr, *ds = nullptr, *dd = nullptr, *dq = nullptr;
+    int rc = upload_tmp(c, pos, (size_t)n * 12, &dp);
+    if (!rc) rc = upload_tmp(c, nrm, (size_t)n * 12, &dn);
+    if (!rc) rc = upload_tmp(c, bsdf, (size_t)n * 4, &db);
+    if (!rc) rc = upload_tmp(c, seeds, (size_t)n * 4, &ds);
+    if (!rc) rc = upload_tmp(c, nullptr, 0, &dd);
+    if (!rc) { hipFree(dd); rc = (hipMalloc(&dd, (size_t)n * 12) == hipSuccess) ? RRH_OK : RRH_E_NOMEM; }
+    if (!rc) rc = (hipMalloc(&dq, (size_t)n * 4) == hipSuccess) ? RRH_OK : RRH_E_NOMEM;
+    if (!rc) {
+        hipLaunchKernelGGL(k_hash_sample, dim3((n + 255) / 256), dim3(256), 0, c->stream, (const float4*)g->fitted.p,
+                           (const float*)dp, (const float*)dn, (const uint32_t*)db, (uint32_t*)ds, n,
+                           mk(cam_pos[0], cam_pos[1], cam_pos[2]), g->p.base_cell_size, g->p.distance_slope,
+                           g->p.cell_count, alpha, (float*)dd, (float*)dq);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(dirs, dd, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(pdfs, dq, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(seeds, ds, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) { set_error(hipGetErrorString(e)); rc = RRH_E_HIP; }
+    }
+    hipFree(dp); hipFree(dn); hipFree(db); hipFree(ds); hipFree(dd); hipFree(dq);
+    return rc;
+}
+
+int rrh_grid_em_step(rrh_ctx* c, rrh_grid* g, const float* pos, const float* dirs, const float* w, uint32_t n) {
+    RRH_ARG(c && g && pos && dirs && w, "rrh_grid_em_step: null argument");
+    RRH_ARG(g->p.kind == RRH_GRID_VMM, "rrh_grid_em_step: VMM grids only");
+    void *dp = nullptr, *dd = nullptr, *dw = nullptr;
+    int rc = upload_tmp(c, pos, (size_t)n * 12, &dp);
+    if (!rc) rc = upload_tmp(c, dirs, (size_t)n * 12, &dd);
+    if (!rc) rc = upload_tmp(c, w, (size_t)n * 4, &dw);
+    if (!rc && n) {
+        hipLaunchKernelGGL(k_vmm_estep, dim3((n + 255) / 256), dim3(256), 0, c->stream, vmm_args(g),
+                           (const float*)g->lobes.p, (const float*)dp, (const float*)dd, (const float*)dw, n,
+                           (double*)g->fstats.p);
+        if (hipGetLastError() != hipSuccess) { set_error("k_vmm_estep launch failed"); rc = RRH_E_HIP; }
+    }
+    if (!rc) rc = vmm_mstep(g);
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) { set_error("rrh_grid_em_step: sync failed"); rc = RRH_E_HIP; }
+    hipFree(dp); hipFree(dd); hipFree(dw);
+    return rc;
+}
+
+int rrh_grid_em_from_vertices(rrh_ctx* c, rrh_grid* g) {
+    RRH_ARG(c && g, "rrh_grid_em_from_vertices: null argument");
+    RRH_ARG(g->p.kind == RRH_GRID_VMM, "rrh_grid_em_from_vertices: VMM grids only");
+    RRH_ARG(c->train.p && c->train_cnt.p, "rrh_grid_em_from_vertices: no recorded vertices (render with record_vertices)");
+    uint32_t grid = (c->train_capacity + 255) / 256;
+    if (grid > 8192) grid = 8192;
+    if (grid == 0) grid = 1;
+    hipLaunchKernelGGL(k_vmm_estep_train, dim3(grid), dim3(256), 0, c->stream, vmm_args(g), (const float*)g->lobes.p,
+                       (const float4*)c->train.p, (const uint32_t*)c->train_cnt.p, c->train_capacity,
+                       (double*)g->fstats.p);
+    RRH_LAUNCH_CHECK();
+    return vmm_mstep(g);
+}
+
+int rrh_grid_download(rrh_ctx* c, const rrh_grid* g, void* a, size_t b0, void* b, size_t b1) {
+    RRH_ARG(c && g, "rrh_grid_download: null argument");
+    const DevBuf& A = (g->p.kind == RRH_GRID_HASH) ? g->fitted : g->lobes;
+    const DevBuf& B = (g->p.kind == RRH_GRID_HASH) ? g->accum : g->stats;
+    if (a) { RRH_ARG(b0 == A.bytes, "rrh_grid_download: first buffer size mismatch"); RRH_HIP(hipMemcpyAsync(a, A.p, b0, hipMemcpyDeviceToHost, c->stream)); }
+    if (b) { RRH_ARG(b1 == B.bytes, "rrh_grid_download: second buffer size mismatch"); RRH_HIP(hipMemcpyAsync(b, B.p, b1, hipMemcpyDeviceToHost, c->stream)); }
+    RRH_HIP(hipStreamSynchronize(c->stream));
+    return RRH_OK;
+}
+
+int rrh_grid_upload(rrh_ctx* c, rrh_grid* g, const void* a, size_t b0) {
+    RRH_ARG(c && g && a, "rrh_grid_upload: null argument");
+    DevBuf& A = (g->p.kind == RRH_GRID_HASH) ? g->fitted : g->lobes;
+    RRH_ARG(b0 == A.bytes, "rrh_grid_upload: size mismatch");
+    RRH_HIP(hipMemcpyAsync(A.p, a, b0, hipMemcpyHostToDevice, c->stream));
+    if (g->p.kind == RRH_GRID_VMM) {
+        std::vector<uint32_t> ones(g->ncell, 1u);
+        RRH_HIP(hipMemcpyAsync(g->valid.p, ones.data(), (size_t)g->ncell * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    RRH_HIP(hipStreamSynchronize(c->stream));
+    return RRH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,108 @@
+// rrh_guide_dev.h -- path-guiding pieces used inside the path tracer's shade kernel:
+// one-sample MIS between the BSDF and a vMF mixture (8-lobe EM grid or the learned
+// guide), and the path-vertex records that train both guides.
+#pragma once
+#include "rrh_device.h"
+
+struct rrh_ctx;
+struct rrh_scene;
+
+namespace rrh {
+
+constexpr uint32_t kMaxLobes = 8;
+
+struct GuideDev {
+    const float* lobes = nullptr;      // VMM: res^3 cells x K x (mu3, kappa, pi)
+    const float* mlp_lobes = nullptr;  // MLP: per ray-queue slot K x (mu3, kappa, pi)
+    uint32_t res = 0, K = 0;
+    float bmin[3] = {0, 0, 0}, bext[3] = {1, 1, 1};
+};
+
+// path-vertex record (7 float4 = 112 B) at index pixel * max_bounces + bounce:
+//  r0 (pos, stamp) r1 (n, pdf) r2 (wo, mat) r3 (wi, nee visible)
+//  r4 (L before this vertex, alive) r5 (NEE contribution if visible) r6 (throughput after)
+struct VertexRec {
+    float4* v = nullptr;
+    uint32_t max_bounces = 0;
+    uint32_t stamp = 0;
+};
+constexpr uint32_t kRecF4 = 7;
+
+RRH_D uint32_t vmm_cell(const GuideDev& G, f3 p) {
+    float f[3] = {(p.x - G.bmin[0]) / G.bext[0] * (float)G.res, (p.y - G.bmin[1]) / G.bext[1] * (float)G.res,
+                  (p.z - G.bmin[2]) / G.bext[2] * (float)G.res};
+    uint32_t c[3];
+    for (int k = 0; k < 3; ++k) {
+        int q = (int)floorf(f[k]);
+        q = q < 0 ? 0 : (q > (int)G.res - 1 ? (int)G.res - 1 : q);
+        c[k] = (uint32_t)q;
+    }
+    return (c[2] * G.res + c[1]) * G.res + c[0];
+}
+
+RRH_D float mixture_pdf(const float* lc, uint32_t K, f3 wi) {
+    float p = 0.0f;
+    for (uint32_t k = 0; k < K; ++k) {
+        float pk = lc[k * 5 + 4];
+        if (pk > 0.0f) p += pk * vmf_pdf(mk(lc[k * 5], lc[k * 5 + 1], lc[k * 5 + 2]), lc[k * 5 + 3], wi);
+    }
+    return p;
+}
+
+// one-sample MIS: with probability alpha sample the mixture, else the BSDF;
+// pdf = alpha * p_guide + (1 - alpha) * p_bsdf; returns f(wo, wi)
+RRH_D f3 guided_sample(const GuideDev& G, uint32_t mode, float alpha, const DevScene& S, f3 pos, f3 n, f3 wo,
+                       const Mat& m, uint32_t& seed, f3& wi, float& pdf) {
+    const float* lc = nullptr;
+    if (mode == RRH_GUIDE_VMM && G.lobes) lc = G.lobes + (size_t)vmm_cell(G, pos) * G.K * 5;
+    float tot = 0.0f;
+    if (lc)
+        for (uint32_t k = 0; k < G.K; ++k) tot += fmaxf(lc[k * 5 + 4], 0.0f);
+    float a = (tot > 0.0f) ? alpha : 0.0f;
+    float usel = rnd(seed);
+    if (usel < a) {
+        float uk = rnd(seed) * tot;
+        uint32_t pick = G.K - 1;
+        float acc = 0.0f;
+        for (uint32_t k = 0; k < G.K; ++k) {
+            acc += fmaxf(lc[k * 5 + 4], 0.0f);
+            if (uk < acc) { pick = k; break; }
+        }
+        float u0 = rnd(seed), u1 = rnd(seed), vp;
+        wi = vmf_sample(mk(lc[pick * 5], lc[pick * 5 + 1], lc[pick * 5 + 2]), lc[pick * 5 + 3], u0, u1, vp);
+    } else {
+        float ux = rnd(seed), uy = rnd(seed), bp;
+        f3 f = mat_sample(m, wo, n, ux, uy, wi, bp);
+        if (!m.lambert && bp == 1.0f && f.x == 0.0f && f.y == 0.0f && f.z == 0.0f) { pdf = 0.0f; return f; }
+    }
+    float pb = mat_sample_pdf(m, wo, wi, n);
+    float pg = (a > 0.0f) ? mixture_pdf(lc, G.K, wi) / tot : 0.0f;
+    pdf = a * pg + (1.0f - a) * pb;
+    if (dot(n, wi) <= 0.0f) { pdf = 0.0f; return mk(0, 0, 0); }
+    return mat_eval(m, n, wo, wi);
+}
+
+RRH_D void guide_record_vertex(const VertexRec& R, uint32_t pixel, uint32_t b, f3 pos, f3 n, f3 wo, f3 wi, float pdf,
+                               f3 thp_before, f3 thp_after, f3 L_pre, f3 nee_c, uint32_t mat, bool alive) {
+    float4* r = R.v + ((size_t)pixel * R.max_bounces + b) * kRecF4;
+    (void)thp_before;
+    r[0] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(R.stamp));
+    r[1] = make_float4(n.x, n.y, n.z, pdf);
+    r[2] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(mat));
+    r[3] = make_float4(wi.x, wi.y, wi.z, 0.0f);
+    r[4] = make_float4(L_pre.x, L_pre.y, L_pre.z, alive ? 1.0f : 0.0f);
+    r[5] = make_float4(nee_c.x, nee_c.y, nee_c.z, 0.0f);
+    r[6] = make_float4(thp_after.x, thp_after.y, thp_after.z, 0.0f);
+}
+RRH_D void guide_record_nee_visible(const VertexRec& R, uint32_t pixel, uint32_t b) {
+    float4* r = R.v + ((size_t)pixel * R.max_bounces + b) * kRecF4;
+    r[3].w = 1.0f;
+}
+
+// host side (rrh_guide.hip)
+int guide_prepare(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* grid, rrh_mlp* mlp, GuideDev& G,
+                  VertexRec& R);
+int guide_finish(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* grid, rrh_mlp* mlp, const VertexRec& R,
+                 const float4* rad);
+
+}  // namespace rrh

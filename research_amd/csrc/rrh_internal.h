@@ -1,0 +1,117 @@
+// rrh_internal.h -- host-side objects behind the opaque C handles.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/rrh.h"
+#include "rrh_device.h"
+
+namespace rrh {
+
+void set_error(const std::string& msg);
+
+#define RRH_HIP(call)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            ::rrh::set_error(std::string(#call) + ": " + hipGetErrorString(e_));            \
+            return RRH_E_HIP;                                                               \
+        }                                                                                   \
+    } while (0)
+#define RRH_ARG(cond, msg)                                                                  \
+    do {                                                                                    \
+        if (!(cond)) { ::rrh::set_error(msg); return RRH_E_ARG; }                           \
+    } while (0)
+#define RRH_LAUNCH_CHECK()                                                                  \
+    do {                                                                                    \
+        hipError_t e_ = hipGetLastError();                                                  \
+        if (e_ != hipSuccess) {                                                             \
+            ::rrh::set_error(std::string("kernel launch: ") + hipGetErrorString(e_));        \
+            return RRH_E_HIP;                                                               \
+        }                                                                                   \
+    } while (0)
+
+// device buffer that grows on demand
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t n);
+    void release();
+};
+
+// binary BVH produced by the builder (rrh_bvh.cpp)
+struct BvhOut {
+    std::vector<float4> nodes;   // 4 per inner node
+    std::vector<float4> tris;    // 3 per triangle, leaf order
+    uint32_t depth = 0, leaves = 0;
+};
+// tri_v: ntri x 9 floats (world space).  Returns false with a message on failure.
+bool build_bvh(const float* tri_v, uint32_t ntri, BvhOut& out, std::string& err);
+
+// procedural scene generators (rrh_scenegen.cpp)
+struct SceneData {
+    std::vector<rrh_vertex> verts;
+    std::vector<uint32_t> idx;
+    std::vector<rrh_mesh> meshes;
+    std::vector<rrh_material> mats;
+    std::vector<rrh_instance> insts;
+    std::vector<rrh_light> lights;
+    uint32_t flags = 0;
+    float eye[3] = {0, 0, 3}, center[3] = {0, 0, 0}, fov = 45.0f;
+};
+bool generate_scene(int kind, uint32_t seed, float scale, SceneData& out, std::string& err);
+
+}  // namespace rrh
+
+struct rrh_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t markers[16] = {};
+    // path-tracer wavefront state (sized for the largest frame seen)
+    rrh::DevBuf q_o[2], q_d[2], q_thp[2];
+    rrh::DevBuf hits, sh_o, sh_d, sh_c;
+    rrh::DevBuf counters;         // per-bounce queue counts (see rrh_pt.hip)
+    rrh::DevBuf vtx;              // recorded path vertices (guiding)
+    rrh::DevBuf train;            // finalised training samples (4 float4 each)
+    rrh::DevBuf train_cnt;        // device counter of training samples
+    uint32_t train_capacity = 0;
+    uint32_t stamp = 0;           // vertex-record generation
+    uint32_t last_rec_bounces = 0;
+    uint32_t last_max_bounces = 0;
+    uint32_t last_record_stride = 0;
+    uint32_t last_npix = 0;
+    rrh::DevBuf di_counter;
+    rrh::DevBuf totals;           // uint64 x 3: primary, extension, shadow (device-side sums)
+    rrh::DevBuf fetch;            // uint64 x 3: node fetches, tri fetches, rays (instrumented traversal)
+    bool profiling = false;
+    std::vector<hipEvent_t> prof_events;   // pairs around each traversal launch
+    uint32_t prof_used = 0;
+    rrh::DevBuf scratch;          // small general-purpose scratch
+};
+
+struct rrh_scene {
+    rrh_ctx* ctx = nullptr;
+    rrh::DevBuf nodes, tris, tri_ref, tri_light, tl, verts, idx, meshes, mats, insts, lights, cam;
+    uint32_t ntri = 0, nnodes = 0, nl = 0, ntl = 0, flags = 0, depth = 0, leaves = 0;
+    float build_ms = 0.0f;
+    float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+    std::vector<float> tri_world;   // flattened world triangles (host copy, 9 floats each)
+    rrh_camera cam_host{};
+    bool cam_set = false;
+    rrh::DevScene dev() const;
+};
+
+struct rrh_image {
+    rrh_ctx* ctx = nullptr;
+    uint32_t w = 0, h = 0;
+    int fmt = 0;
+    size_t bytes = 0;
+    void* p = nullptr;
+};
+
+namespace rrh {
+size_t format_bytes(int fmt);
+}

@@ -1,0 +1,468 @@
+// rrh_pt.hip -- wavefront path tracer (replaces pathtracer.slang cs_main :181-265 and
+// PathTracerPass::execute, src/passes/pathtracer_reference/PathTracerPass.cpp:170-224)
+// plus progressive accumulation (accumulate.slang:21-46, AccumulatePass.cpp:167-218).
+//
+// One frame = raygen, then per bounce: closest-hit traversal over the live ray
+// queue, shading (NEE candidate + BSDF sample + Russian roulette) that appends
+// surviving paths to the next queue and shadow rays to the shadow queue, and an
+// occlusion pass that adds unshadowed NEE.  Each path keeps the reference's
+// per-pixel RNG stream, so the draw order per pixel is the reference's:
+// jitter(2) | per bounce: [light pick (+2 for a triangle light)] [BSDF 2] [RR 1].
+// Queues are compacted with one wave-aggregated atomic per wave.
+#include "rrh_internal.h"
+#include "rrh_guide_dev.h"
+
+using namespace rrh;
+
+namespace {
+
+constexpr int kMaxBounces = 32;
+constexpr uint32_t kBlock = 256;
+
+// counters layout (uint32): [0, 64) ray-queue size per bounce, [64, 128) shadow rays per bounce
+struct Q {
+    float4 *o, *d, *thp;
+};
+
+__global__ void __launch_bounds__(64) k_raygen(DevScene S, Q out, uint32_t* __restrict__ cnt, float4* __restrict__ rad,
+                                               uint32_t frame_seed) {
+    const rrh_camera& cam = *S.cam;
+    uint32_t W = cam.screen_width, H = cam.screen_height;
+    uint32_t px = blockIdx.x * 8 + (threadIdx.x & 7u);
+    uint32_t py = blockIdx.y * 8 + (threadIdx.x >> 3);
+    bool valid = px < W && py < H;
+    f3 d = mk(0, 0, 0);
+    uint32_t seed = 0;
+    if (valid) {
+        seed = make_seed(px, py, frame_seed);
+        d = primary_dir(cam, px, py, seed);
+        rad[(size_t)py * W + px] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    }
+    uint32_t slot = wave_push(&cnt[0], valid);
+    if (valid) {
+        uint32_t pix = py * W + px;
+        out.o[slot] = make_float4(cam.position[0], cam.position[1], cam.position[2], __uint_as_float(pix));
+        out.d[slot] = make_float4(d.x, d.y, d.z, __uint_as_float(seed));
+        out.thp[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    }
+}
+
+template <bool COUNT>
+__global__ void __launch_bounds__(kBlock) k_intersect(DevScene S, Q in, const uint32_t* __restrict__ cnt, uint32_t b,
+                                                      float4* __restrict__ hits, unsigned long long* __restrict__ fetch) {
+    uint32_t n = cnt[b];
+    uint32_t nn = 0, nt = 0, nr = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float4 o = in.o[i], d = in.d[i];
+        Hit h = trace_closest<COUNT>(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, 1e9f, &nn, &nt);
+        hits[i] = make_float4(h.t, __uint_as_float(h.prim), h.b1, h.b2);
+        nr++;
+    }
+    if constexpr (COUNT) {
+        atomicAdd(fetch + 0, (unsigned long long)nn);
+        atomicAdd(fetch + 1, (unsigned long long)nt);
+        atomicAdd(fetch + 2, (unsigned long long)nr);
+    }
+}
+
+// adds this frame's per-bounce queue sizes into the context's running totals
+__global__ void k_sum_counts(const uint32_t* __restrict__ cnt, uint32_t max_bounces,
+                             unsigned long long* __restrict__ tot) {
+    if (threadIdx.x != 0) return;
+    unsigned long long e = 0, sh = 0;
+    for (uint32_t b = 1; b < max_bounces; ++b) e += cnt[b];
+    for (uint32_t b = 0; b < max_bounces; ++b) sh += cnt[64 + b];
+    tot[0] += cnt[0];
+    tot[1] += e;
+    tot[2] += sh;
+}
+
+struct NeeOut { bool shadow; f3 c; f3 dir; float tmax; };
+
+// pathtracer.slang:124-177 up to (excluding) the shadow test; triangle lights
+// use area sampling and the power heuristic against the BSDF sampling pdf.
+__device__ __forceinline__ NeeOut nee_candidate(const DevScene& S, f3 hit, f3 n, f3 wo, const Mat& m, uint32_t& seed) {
+    NeeOut r; r.shadow = false; r.c = mk(0, 0, 0); r.dir = mk(0, 0, 0); r.tmax = 0.0f;
+    uint32_t N = S.nl + S.ntl;
+    if (N == 0) return r;
+    uint32_t li = (uint32_t)(rnd(seed) * (float)N) % N;
+    if (li < S.nl) {
+        const rrh_light& L = S.lights[li];
+        f3 ldir; float dist, pl;
+        if (L.type == 0) {
+            f3 to = sub(ld3(L.position), hit);
+            dist = length(to);
+            ldir = divs(to, dist);
+            pl = dist * dist;
+        } else {
+            ldir = normalize(neg(ld3(L.direction)));
+            dist = 1e9f;
+            pl = 1.0f;
+        }
+        if (dot(ldir, n) <= 0.0f) return r;
+        f3 f = mat_eval(m, n, wo, ldir);
+        float ndl = fmaxf(dot(n, ldir), 0.0f);
+        f3 E = scale(ld3(L.emission), L.intensity);
+        float den = (L.type == 0) ? (pl + 1e-6f) : 1.0f;
+        f3 irr = divs(E, den);
+        float pb = mat_ref_pdf(m, wo, ldir, n);
+        float mis = (L.type == 0) ? 1.0f : (pl * pl) / (pl * pl + pb * pb + 1e-10f);
+        r.c = scale(scale(scale(mul(f, irr), ndl), mis), (float)N);
+        r.dir = ldir;
+        r.tmax = dist - 0.01f;
+        r.shadow = true;
+        return r;
+    }
+    float u1 = rnd(seed);
+    float u2 = rnd(seed);
+    TriSample ts = tri_light_point(S, li - S.nl, u1, u2);
+    f3 to = sub(ts.p, hit);
+    float d2 = dot(to, to);
+    float dist = __fsqrt_rn(d2);
+    f3 l = scale(to, 1.0f / dist);
+    if (dot(l, n) <= 0.0f) return r;
+    float cosl = fabsf(dot(ts.nl, l));
+    if (!(cosl > 0.0f)) return r;
+    f3 f = mat_eval(m, n, wo, l);
+    float ndl = fmaxf(dot(n, l), 0.0f);
+    float pl = d2 / (cosl * ts.area) / (float)N;
+    float pb = mat_sample_pdf(m, wo, l, n);
+    float mis = (pl * pl) / (pl * pl + pb * pb);
+    r.c = scale(scale(mul(f, ts.Le), ndl), mis / pl);
+    r.dir = l;
+    r.tmax = dist - 0.01f;
+    r.shadow = true;
+    return r;
+}
+
+struct ShadeArgs {
+    uint32_t b, max_bounces, rr_start, guide_mode;
+    float guide_alpha;
+    GuideDev guide;
+    VertexRec rec;
+};
+
+__global__ void __launch_bounds__(kBlock) k_shade(DevScene S, Q in, Q out, const float4* __restrict__ hits,
+                                                  uint32_t* __restrict__ cnt, float4* __restrict__ sh_o,
+                                                  float4* __restrict__ sh_d, float4* __restrict__ sh_c,
+                                                  float4* __restrict__ rad, ShadeArgs A) {
+    const uint32_t b = A.b;
+    const uint32_t n = cnt[b];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const bool last = (b + 1u >= A.max_bounces);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        uint32_t i = base + threadIdx.x;
+        bool valid = i < n;
+        bool push_ext = false, push_sh = false;
+        float4 no = make_float4(0, 0, 0, 0), nd = no, nt = no, so = no, sd = no, sc = no;
+        if (valid) {
+            float4 o4 = in.o[i], d4 = in.d[i], t4 = in.thp[i], h4 = hits[i];
+            uint32_t pixel = __float_as_uint(o4.w);
+            uint32_t seed = __float_as_uint(d4.w);
+            f3 thp = xyz(t4);
+            float last_pdf = t4.w;
+            f3 dir = xyz(d4);
+            uint32_t prim = __float_as_uint(h4.y);
+            float4 L4 = rad[pixel];
+            f3 L = xyz(L4);
+            if (prim == ~0u) {
+                L = add(L, mul(thp, sky(dir)));
+                rad[pixel] = make_float4(L.x, L.y, L.z, L4.w);
+            } else {
+                HitInfo hi = fetch_hit(S, prim, h4.z, h4.w);
+                Mat m = load_mat(S, hi.mat);
+                if (is_emissive(m)) {
+                    uint32_t lo = S.tri_light[prim];
+                    if (b > 0 && lo != ~0u) {
+                        float pl = tri_light_pdf(S, lo - S.nl, dir, h4.x);
+                        float w = (last_pdf * last_pdf) / (last_pdf * last_pdf + pl * pl);
+                        L = add(L, mul(thp, scale(m.emissive, w)));
+                    } else {
+                        L = add(L, mul(thp, m.emissive));
+                    }
+                    rad[pixel] = make_float4(L.x, L.y, L.z, L4.w);
+                } else {
+                    f3 wo = neg(dir);
+                    f3 nn = hi.normal;
+                    if (dot(nn, wo) < 0.0f) nn = neg(nn);
+                    NeeOut ne = nee_candidate(S, hi.pos, nn, wo, m, seed);
+                    f3 sorg = add(hi.pos, scale(nn, 1e-4f));
+                    if (ne.shadow) {
+                        push_sh = true;
+                        f3 c = mul(thp, ne.c);
+                        so = make_float4(sorg.x, sorg.y, sorg.z, ne.tmax);
+                        sd = make_float4(ne.dir.x, ne.dir.y, ne.dir.z, __uint_as_float(pixel));
+                        sc = make_float4(c.x, c.y, c.z, 0.0f);
+                    }
+                    f3 wi; float pdf; f3 f;
+                    if (A.guide_mode != RRH_GUIDE_NONE) {
+                        f = guided_sample(A.guide, A.guide_mode, A.guide_alpha, S, hi.pos, nn, wo, m, seed, wi, pdf);
+                    } else {
+                        float ux = rnd(seed);
+                        float uy = rnd(seed);
+                        f = mat_sample(m, wo, nn, ux, uy, wi, pdf);
+                    }
+                    bool alive = !(pdf < 1e-6f);
+                    f3 thp_before = thp;
+                    if (alive) {
+                        float ndw = fmaxf(dot(nn, wi), 0.0f);
+                        thp = mul(thp, divs(scale(f, ndw), pdf));
+                        if (b >= A.rr_start) {
+                            float survive = fminf(fmaxf(thp.x, fmaxf(thp.y, thp.z)), 0.95f);
+                            if (rnd(seed) > survive) alive = false;
+                            else thp = divs(thp, survive);
+                        }
+                    }
+                    if (A.rec.v) guide_record_vertex(A.rec, pixel, b, hi.pos, nn, wo, wi, pdf, thp_before,
+                                                     alive ? thp : mk(0, 0, 0), L, ne.shadow ? mul(thp_before, ne.c) : mk(0, 0, 0), hi.mat, alive);
+                    if (alive && !last) {
+                        push_ext = true;
+                        no = make_float4(sorg.x, sorg.y, sorg.z, __uint_as_float(pixel));
+                        nd = make_float4(wi.x, wi.y, wi.z, __uint_as_float(seed));
+                        nt = make_float4(thp.x, thp.y, thp.z, pdf);
+                    }
+                }
+            }
+        }
+        uint32_t es = wave_push(&cnt[b + 1], push_ext);
+        if (push_ext) { out.o[es] = no; out.d[es] = nd; out.thp[es] = nt; }
+        uint32_t ss = wave_push(&cnt[64 + b], push_sh);
+        if (push_sh) { sh_o[ss] = so; sh_d[ss] = sd; sh_c[ss] = sc; }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_shadow(DevScene S, const uint32_t* __restrict__ cnt, uint32_t b,
+                                                   const float4* __restrict__ sh_o, const float4* __restrict__ sh_d,
+                                                   const float4* __restrict__ sh_c, float4* __restrict__ rad,
+                                                   VertexRec rec) {
+    uint32_t n = cnt[64 + b];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float4 o = sh_o[i], d = sh_d[i];
+        if (!trace_occluded(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, o.w)) {
+            uint32_t pixel = __float_as_uint(d.w);
+            float4 c = sh_c[i];
+            float4 L = rad[pixel];
+            L.x = L.x + c.x; L.y = L.y + c.y; L.z = L.z + c.z;
+            rad[pixel] = L;
+            if (rec.v) guide_record_nee_visible(rec, pixel, b);
+        }
+    }
+}
+
+__global__ void k_accumulate(const float4* __restrict__ rad, float4* __restrict__ acc, uint32_t n,
+                             uint32_t camera_moved, uint32_t spp_in) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 r = rad[i];
+    if (camera_moved != 0u || spp_in == 0u) { acc[i] = r; return; }
+    float4 p = acc[i];
+    float spp = (float)spp_in;
+    float w = 1.0f / (spp + 1.0f);
+    float k = spp / (spp + 1.0f);
+    acc[i] = make_float4(p.x * k + r.x * w, p.y * k + r.y * w, p.z * k + r.z * w, p.w * k + r.w * w);
+}
+
+__global__ void k_accumulate_f64(const float4* __restrict__ rad, double* __restrict__ acc, uint32_t n,
+                                 uint32_t camera_moved, uint32_t spp_in) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 r = rad[i];
+    double* a = acc + (size_t)i * 4;
+    if (camera_moved != 0u || spp_in == 0u) {
+        a[0] = r.x; a[1] = r.y; a[2] = r.z; a[3] = r.w;
+    } else {
+        a[0] += (double)r.x; a[1] += (double)r.y; a[2] += (double)r.z; a[3] += (double)r.w;
+    }
+}
+
+uint32_t grid_for(uint32_t n) {
+    uint32_t g = (n + kBlock - 1) / kBlock;
+    return g < 1u ? 1u : (g > 4096u ? 4096u : g);
+}
+
+}  // namespace
+
+int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* grid, rrh_mlp* mlp,
+                       rrh_image* radiance) {
+    RRH_ARG(c && s && p && radiance, "rrh_pt_render: null argument");
+    RRH_ARG(s->cam_set, "rrh_pt_render: camera not set (rrh_camera_set)");
+    RRH_ARG(radiance->fmt == RRH_FMT_RGBA32F, "rrh_pt_render: radiance must be RGBA32F");
+    uint32_t W = s->cam_host.screen_width, H = s->cam_host.screen_height;
+    RRH_ARG(radiance->w == W && radiance->h == H, "rrh_pt_render: image extent differs from the camera");
+    RRH_ARG(p->max_bounces >= 1 && p->max_bounces <= kMaxBounces, "rrh_pt_render: max_bounces must be in [1, 32]");
+    RRH_ARG(p->guide_mode <= RRH_GUIDE_MLP, "rrh_pt_render: bad guide_mode");
+    uint32_t npix = W * H;
+    int rc = 0;
+    for (int k = 0; k < 2; ++k) {
+        if ((rc = c->q_o[k].ensure((size_t)npix * 16))) return rc;
+        if ((rc = c->q_d[k].ensure((size_t)npix * 16))) return rc;
+        if ((rc = c->q_thp[k].ensure((size_t)npix * 16))) return rc;
+    }
+    if ((rc = c->hits.ensure((size_t)npix * 16))) return rc;
+    if ((rc = c->sh_o.ensure((size_t)npix * 16))) return rc;
+    if ((rc = c->sh_d.ensure((size_t)npix * 16))) return rc;
+    if ((rc = c->sh_c.ensure((size_t)npix * 16))) return rc;
+    if ((rc = c->counters.ensure(128 * 4))) return rc;
+    if (!c->totals.p) {
+        if ((rc = c->totals.ensure(32))) return rc;
+        RRH_HIP(hipMemsetAsync(c->totals.p, 0, 32, c->stream));
+    }
+    if (!c->fetch.p) {
+        if ((rc = c->fetch.ensure(32))) return rc;
+        RRH_HIP(hipMemsetAsync(c->fetch.p, 0, 32, c->stream));
+    }
+    ShadeArgs A;
+    A.max_bounces = p->max_bounces;
+    A.rr_start = p->rr_start;
+    A.guide_mode = p->guide_mode;
+    A.guide_alpha = p->guide_alpha;
+    if ((rc = guide_prepare(c, s, p, grid, mlp, A.guide, A.rec))) return rc;
+    DevScene S = s->dev();
+    uint32_t* cnt = (uint32_t*)c->counters.p;
+    RRH_HIP(hipMemsetAsync(cnt, 0, 128 * 4, c->stream));
+    float4* rad = (float4*)radiance->p;
+    Q q[2];
+    for (int k = 0; k < 2; ++k) q[k] = Q{(float4*)c->q_o[k].p, (float4*)c->q_d[k].p, (float4*)c->q_thp[k].p};
+    hipLaunchKernelGGL(k_raygen, dim3((W + 7) / 8, (H + 7) / 8), dim3(64), 0, c->stream, S, q[0], cnt, rad,
+                       p->frame_seed);
+    RRH_LAUNCH_CHECK();
+    uint32_t g = grid_for(npix);
+    for (uint32_t b = 0; b < p->max_bounces; ++b) {
+        Q& qi = q[b & 1];
+        Q& qo = q[(b + 1) & 1];
+        if (c->profiling) {
+            if (c->prof_used + 2 > c->prof_events.size()) {
+                for (int k = 0; k < 64; ++k) {
+                    hipEvent_t e;
+                    RRH_HIP(hipEventCreate(&e));
+                    c->prof_events.push_back(e);
+                }
+            }
+            RRH_HIP(hipEventRecord(c->prof_events[c->prof_used], c->stream));
+        }
+        if (p->flags & RRH_PT_COUNT_FETCHES)
+            hipLaunchKernelGGL(k_intersect<true>, dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b,
+                               (float4*)c->hits.p, (unsigned long long*)c->fetch.p);
+        else
+            hipLaunchKernelGGL(k_intersect<false>, dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b,
+                               (float4*)c->hits.p, (unsigned long long*)nullptr);
+        RRH_LAUNCH_CHECK();
+        if (c->profiling) {
+            RRH_HIP(hipEventRecord(c->prof_events[c->prof_used + 1], c->stream));
+            c->prof_used += 2;
+        }
+        A.b = b;
+        hipLaunchKernelGGL(k_shade, dim3(g), dim3(kBlock), 0, c->stream, S, qi, qo, (const float4*)c->hits.p, cnt,
+                           (float4*)c->sh_o.p, (float4*)c->sh_d.p, (float4*)c->sh_c.p, rad, A);
+        RRH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_shadow, dim3(g), dim3(kBlock), 0, c->stream, S, (const uint32_t*)cnt, b,
+                           (const float4*)c->sh_o.p, (const float4*)c->sh_d.p, (const float4*)c->sh_c.p, rad, A.rec);
+        RRH_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(64), 0, c->stream, (const uint32_t*)cnt, p->max_bounces,
+                       (unsigned long long*)c->totals.p);
+    RRH_LAUNCH_CHECK();
+    c->last_max_bounces = p->max_bounces;
+    c->last_npix = npix;
+    return guide_finish(c, s, p, grid, mlp, A.rec, rad);
+}
+
+extern "C" {
+
+int rrh_pt_render(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_image* radiance) {
+    RRH_ARG(p, "rrh_pt_render: null params");
+    RRH_ARG(p->guide_mode == RRH_GUIDE_NONE, "rrh_pt_render: guided modes go through rrh_pt_render_guided");
+    return rrh_pt_render_impl(c, s, p, nullptr, nullptr, radiance);
+}
+
+int rrh_pt_render_guided(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* grid, rrh_mlp* mlp,
+                         rrh_image* radiance) {
+    return rrh_pt_render_impl(c, s, p, grid, mlp, radiance);
+}
+
+int rrh_pt_ray_counts(rrh_ctx* c, rrh_ray_counts* out) {
+    RRH_ARG(c && out, "rrh_pt_ray_counts: null argument");
+    out->primary = out->extension = out->shadow = 0;
+    if (!c->counters.p || c->last_max_bounces == 0) return RRH_OK;
+    uint32_t h[128];
+    RRH_HIP(hipMemcpyAsync(h, c->counters.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    RRH_HIP(hipStreamSynchronize(c->stream));
+    out->primary = h[0];
+    for (uint32_t b = 1; b < c->last_max_bounces; ++b) out->extension += h[b];
+    for (uint32_t b = 0; b < c->last_max_bounces; ++b) out->shadow += h[64 + b];
+    return RRH_OK;
+}
+
+int rrh_ray_totals(rrh_ctx* c, rrh_ray_counts* out, int reset) {
+    RRH_ARG(c && out, "rrh_ray_totals: null argument");
+    out->primary = out->extension = out->shadow = 0;
+    if (!c->totals.p) {
+        int rc = c->totals.ensure(32);
+        if (rc) return rc;
+        RRH_HIP(hipMemsetAsync(c->totals.p, 0, 32, c->stream));
+    }
+    unsigned long long h[3];
+    RRH_HIP(hipMemcpyAsync(h, c->totals.p, 24, hipMemcpyDeviceToHost, c->stream));
+    if (reset) RRH_HIP(hipMemsetAsync(c->totals.p, 0, 32, c->stream));
+    RRH_HIP(hipStreamSynchronize(c->stream));
+    out->primary = h[0]; out->extension = h[1]; out->shadow = h[2];
+    return RRH_OK;
+}
+
+int rrh_fetch_counts(rrh_ctx* c, uint64_t* nodes, uint64_t* tris, uint64_t* rays, int reset) {
+    RRH_ARG(c && nodes && tris && rays, "rrh_fetch_counts: null argument");
+    *nodes = *tris = *rays = 0;
+    if (!c->fetch.p) return RRH_OK;
+    unsigned long long h[3];
+    RRH_HIP(hipMemcpyAsync(h, c->fetch.p, 24, hipMemcpyDeviceToHost, c->stream));
+    if (reset) RRH_HIP(hipMemsetAsync(c->fetch.p, 0, 32, c->stream));
+    RRH_HIP(hipStreamSynchronize(c->stream));
+    *nodes = h[0]; *tris = h[1]; *rays = h[2];
+    return RRH_OK;
+}
+
+int rrh_profile_enable(rrh_ctx* c, int on) {
+    RRH_ARG(c, "rrh_profile_enable: null context");
+    c->profiling = on != 0;
+    return RRH_OK;
+}
+
+int rrh_profile_traversal(rrh_ctx* c, float* total_ms, uint32_t* launches, int reset) {
+    RRH_ARG(c && total_ms && launches, "rrh_profile_traversal: null argument");
+    *total_ms = 0.0f;
+    *launches = c->prof_used / 2;
+    if (c->prof_used) RRH_HIP(hipEventSynchronize(c->prof_events[c->prof_used - 1]));
+    double acc = 0.0;
+    for (uint32_t k = 0; k + 1 < c->prof_used; k += 2) {
+        float ms = 0.0f;
+        RRH_HIP(hipEventElapsedTime(&ms, c->prof_events[k], c->prof_events[k + 1]));
+        acc += ms;
+    }
+    *total_ms = (float)acc;
+    if (reset) c->prof_used = 0;
+    return RRH_OK;
+}
+
+int rrh_accumulate(rrh_ctx* c, const rrh_image* radiance, rrh_image* acc, uint32_t camera_moved, uint32_t spp) {
+    RRH_ARG(c && radiance && acc, "rrh_accumulate: null argument");
+    RRH_ARG(radiance->fmt == RRH_FMT_RGBA32F && acc->fmt == RRH_FMT_RGBA32F, "rrh_accumulate: RGBA32F images required");
+    RRH_ARG(radiance->w == acc->w && radiance->h == acc->h, "rrh_accumulate: extent mismatch");
+    uint32_t n = radiance->w * radiance->h;
+    hipLaunchKernelGGL(k_accumulate, dim3((n + 255) / 256), dim3(256), 0, c->stream, (const float4*)radiance->p,
+                       (float4*)acc->p, n, camera_moved, spp);
+    RRH_LAUNCH_CHECK();
+    return RRH_OK;
+}
+
+int rrh_accumulate_f64(rrh_ctx* c, const rrh_image* radiance, rrh_image* acc, uint32_t camera_moved, uint32_t spp) {
+    RRH_ARG(c && radiance && acc, "rrh_accumulate_f64: null argument");
+    RRH_ARG(radiance->fmt == RRH_FMT_RGBA32F && acc->fmt == RRH_FMT_RGBA64F, "rrh_accumulate_f64: RGBA32F -> RGBA64F");
+    RRH_ARG(radiance->w == acc->w && radiance->h == acc->h, "rrh_accumulate_f64: extent mismatch");
+    uint32_t n = radiance->w * radiance->h;
+    hipLaunchKernelGGL(k_accumulate_f64, dim3((n + 255) / 256), dim3(256), 0, c->stream, (const float4*)radiance->p,
+                       (double*)acc->p, n, camera_moved, spp);
+    RRH_LAUNCH_CHECK();
+    return RRH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,120 @@
+"""GPU ReSTIR DI and A-trous against the oracle."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from research_amd import _lib as L
+from research_amd import core
+from tests.helpers import ocam
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits_equal_frac(a, b):
+    a = np.ascontiguousarray(a).reshape(len(a), -1).view(np.uint32)
+    b = np.ascontiguousarray(b).reshape(len(b), -1).view(np.uint32)
+    return np.all(a == b, axis=1).mean()
+
+
+@pytest.mark.parametrize("kind,scale,M", [(L.GEN_CORNELL_REF, 1.0, 16), (L.GEN_CORNELL_RANDOM, 1.0, 32),
+                                          (L.GEN_HEIGHTFIELD, 0.1, 32)])
+def test_restir_di_three_stages_match_oracle(ctx, kind, scale, M):
+    sa = core.generate_scene(kind, 2, scale)
+    gs = core.Scene(ctx, sa)
+    os_ = O.Scene.from_arrays(sa)
+    W, H = 64, 40
+    cam = core.camera_look_at(sa.eye, sa.center, sa.fov, W, H)
+    gs.set_camera(cam)
+    oc = ocam(cam)
+    im = lambda fmt=L.FMT_RGBA32F: core.Image(ctx, W, H, fmt)
+    gp, gn, out = im(), im(), im()
+    res = [im(), im()]
+    uv = [im(L.FMT_R32U), im(L.FMT_R32U)]
+    o_prev = None
+    for frame in range(3):
+        cur, prev = frame & 1, 1 - (frame & 1)
+        prm = L.DIParams(frame, M, 1, 1, 5, 30)
+        oprm = O.DIParams(frame, M, 1, 1, 5, 30)
+        L.call("rrh_di_initial", ctx.h, gs.h, C.byref(prm), gp.h, gn.h, res[cur].h, uv[cur].h)
+        ogp, ogn, ors, ouv = os_.di_initial(oc, oprm)
+        assert _bits_equal_frac(gp.download().reshape(-1, 4), ogp) >= 0.999
+        assert _bits_equal_frac(res[cur].download().reshape(-1, 4), ors) >= 0.999
+        if frame > 0:
+            # temporal on the GPU's own previous reservoirs, oracle on the same inputs
+            rp, rpuv = res[prev].download().reshape(-1, 4), uv[prev].download().reshape(-1)
+            rc0, rcuv0 = res[cur].download().reshape(-1, 4), uv[cur].download().reshape(-1)
+            L.call("rrh_di_temporal", ctx.h, gs.h, C.byref(prm), gp.h, gn.h, res[cur].h, uv[cur].h, res[prev].h,
+                   uv[prev].h)
+            g_gp, g_gn = gp.download().reshape(-1, 4), gn.download().reshape(-1, 4)
+            orc, orcuv = os_.di_temporal(oc, oprm, g_gp, g_gn, rc0, rcuv0, rp, rpuv)
+            assert _bits_equal_frac(res[cur].download().reshape(-1, 4), orc) >= 0.999
+            assert (uv[cur].download().reshape(-1) == orcuv).mean() >= 0.999
+        L.call("rrh_di_spatial_shade", ctx.h, gs.h, C.byref(prm), gp.h, gn.h, res[cur].h, uv[cur].h, out.h)
+        g_out = out.download().reshape(-1, 4)
+        o_out = os_.di_spatial_shade(oc, oprm, gp.download().reshape(-1, 4), gn.download().reshape(-1, 4),
+                                     res[cur].download().reshape(-1, 4), uv[cur].download().reshape(-1))
+        assert _bits_equal_frac(g_out, o_out) >= 0.999
+        assert np.isfinite(g_out).all()
+
+
+def _ref_inputs(H, W, seed):
+    rng = np.random.default_rng(seed)
+    rad = (rng.random((H, W, 4)) * np.array([3, 2, 1, 1])).astype(np.float32)
+    pos = np.zeros((H, W, 4), np.float32)
+    yy, xx = np.mgrid[0:H, 0:W]
+    pos[..., 0] = xx * 0.02
+    pos[..., 1] = np.where(xx > W // 2, 0.3, 0.0)
+    pos[..., 2] = yy * 0.02
+    pos[..., 3] = 1.0
+    pos[rng.random((H, W)) < 0.05, 3] = -1.0  # misses
+    nrm = np.zeros((H, W, 4), np.float32)
+    nrm[..., 1] = 1.0
+    nrm[xx > W // 2] = [0.0, 0.6, 0.8, 0.0]
+    return rad, pos, nrm
+
+
+def test_atrous_reference_mode_matches_oracle(ctx):
+    H, W = 72, 136
+    rad, pos, nrm = _ref_inputs(H, W, 0)
+    imgs = [core.Image(ctx, W, H) for _ in range(5)]
+    for im, a in zip(imgs[:3], (rad, pos, nrm)):
+        im.upload(a)
+    for iters in (1, 5):
+        prm = L.AtrousParams(iters, L.ATROUS_REFERENCE, 0.25, 0.2, 1.0)
+        slot = C.c_int()
+        L.call("rrh_atrous", ctx.h, C.byref(prm), imgs[0].h, imgs[1].h, imgs[2].h, imgs[3].h, imgs[4].h,
+               C.byref(slot))
+        assert slot.value == (iters - 1) & 1
+        g = imgs[3 + slot.value].download()
+        o = rad
+        for k in range(iters):
+            o = O.atrous_ref(o, pos, nrm, 1 << k)
+        assert np.abs(g - o).max() <= 1e-3
+        assert _bits_equal_frac(g.reshape(-1, 4), o.reshape(-1, 4)) >= 0.999
+
+
+def test_atrous_config_mode_matches_oracle(ctx):
+    H, W = 64, 120
+    rad, pos, nrm = _ref_inputs(H, W, 1)
+    rad16 = rad.astype(np.float16).view(np.uint16).copy()
+    rad16[..., 3] = np.float16(0.05).view(np.uint16)
+    gb = np.zeros((H, W, 4), np.float32)
+    gb[..., :3] = nrm[..., :3]
+    gb[..., 3] = np.where(pos[..., 3] < 0, 0.0, 2.0 + pos[..., 2])
+    gb16 = gb.astype(np.float16).view(np.uint16).copy()
+    i_in, i_gb, t0, t1 = (core.Image(ctx, W, H, L.FMT_RGBA16F) for _ in range(4))
+    i_in.upload(rad16)
+    i_gb.upload(gb16)
+    prm = L.AtrousParams(5, L.ATROUS_CONFIG, 1.0, 128.0, 4.0)
+    slot = C.c_int()
+    L.call("rrh_atrous", ctx.h, C.byref(prm), i_in.h, i_gb.h, None, t0.h, t1.h, C.byref(slot))
+    g = (t0, t1)[slot.value].download()
+    o = rad16
+    for k in range(5):
+        o = O.atrous_cfg(o, gb16, 1 << k)
+    gf = g.view(np.float16).astype(np.float32)
+    of = o.view(np.float16).astype(np.float32)
+    assert np.abs(gf - of).max() <= 1e-3 * max(1.0, np.abs(of).max())
+    assert (g == o).all(axis=2).mean() >= 0.999

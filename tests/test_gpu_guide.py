@@ -1,0 +1,84 @@
+"""GPU vMF guiding: the reference hash grid (splat / fit / sample_primary_dir) and the
+8-lobe EM grid, against the oracle."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from research_amd import _lib as L
+from research_amd import core
+from tests.test_oracle import _vmf_draw
+
+pytestmark = pytest.mark.gpu
+
+
+def hash_params():
+    return L.GridParams(L.GRID_HASH, 1 << 20, 0.10, 0.02, 8, 512.0, 0, 0, L.f3((0, 0, 0)), L.f3((1, 1, 1)), 0.0, 0.0)
+
+
+def test_hash_grid_splat_fit_sample_match_oracle(ctx):
+    rng = np.random.default_rng(0)
+    n = 60000
+    pos = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    pos[: n // 2] = rng.uniform(0.2, 0.25, (n // 2, 3))  # a dense region so cells pass min_samples
+    dirs = _vmf_draw([0.2, 0.9, 0.3], 12.0, n, rng)
+    bsdf = rng.integers(0, 3, n).astype(np.uint32)
+    cam = (0.0, 0.0, 3.0)
+    g = core.Grid(ctx, hash_params())
+    L.call("rrh_grid_splat", ctx.h, g.h, L.ptr(pos), L.ptr(bsdf), L.ptr(dirs), n, C.byref(L.f3(cam)))
+    _, acc_gpu = g.download()
+    lib = O.lib()
+    acc = np.zeros((1 << 20, 4), np.uint32)
+    for i in range(n):
+        c = lib.orc_guiding_cell_index(O.fa(pos[i]), int(bsdf[i]), O.fa(cam), 0.10, 0.02, 1 << 20)
+        lib.orc_guiding_splat(O.p(acc), c, O.fa(dirs[i]))
+    assert np.array_equal(acc_gpu, acc)  # integer atomics: order-independent, bit-exact
+    L.call("rrh_grid_fit", ctx.h, g.h)
+    fit_gpu, acc_after = g.download()
+    assert (acc_after == 0).all()
+    fit = np.zeros((1 << 20, 4), np.float32)
+    touched = np.nonzero(acc[:, 3])[0]
+    for c in touched:
+        lib.orc_guiding_fit_cell(O.p(acc), O.p(fit), int(c), 8, 512.0)
+    assert np.array_equal(fit_gpu[touched].view(np.uint32), fit[touched].view(np.uint32))
+    assert (fit[touched, 3] > 0).sum() > 100
+    # sample_primary_dir on the fitted grid
+    m = 4000
+    sp = pos[:m].copy()
+    nrm = np.tile(np.float32([0, 1, 0]), (m, 1))
+    seeds = np.arange(m, dtype=np.uint32) * 7919 + 1
+    dg, pg = np.zeros((m, 3), np.float32), np.zeros(m, np.float32)
+    sg = seeds.copy()
+    L.call("rrh_grid_sample_primary", ctx.h, g.h, L.ptr(sp), L.ptr(nrm), L.ptr(bsdf[:m].copy()), L.ptr(sg), m,
+           C.byref(L.f3(cam)), 0.5, L.ptr(dg), L.ptr(pg))
+    do, po = np.zeros(3, np.float32), np.zeros(1, np.float32)
+    exact = 0
+    for i in range(m):
+        s = C.c_uint32(int(seeds[i]))
+        wi, pdf = (C.c_float * 3)(), C.c_float()
+        lib.orc_sample_primary_dir(O.p(fit_gpu), O.fa(sp[i]), O.fa(nrm[i]), int(bsdf[i]), O.fa(cam), 0.10, 0.02,
+                                   1 << 20, 1, 0.5, C.byref(s), wi, C.byref(pdf))
+        same = np.array(wi[:], np.float32).view(np.uint32).tolist() == dg[i].view(np.uint32).tolist()
+        exact += same and np.float32(pdf.value).view(np.uint32) == pg[i].view(np.uint32) and s.value == sg[i]
+    assert exact / m >= 0.999
+
+
+def test_vmm_em_step_matches_oracle(ctx):
+    rng = np.random.default_rng(3)
+    res, K = 4, 8
+    n = 40000
+    pos = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    dirs = np.concatenate([_vmf_draw([0, 0, 1], 15.0, n // 2, rng), _vmf_draw([1, 0, 0], 5.0, n - n // 2, rng)])
+    w = rng.uniform(0.1, 2.0, n).astype(np.float32)
+    prm = L.GridParams(L.GRID_VMM, 0, 0, 0, 0, 512.0, res, K, L.f3((0, 0, 0)), L.f3((1, 1, 1)), 0.0, 0.0)
+    g = core.Grid(ctx, prm)
+    lobes0, _ = g.download()
+    L.call("rrh_grid_em_step", ctx.h, g.h, L.ptr(pos), L.ptr(dirs), L.ptr(w), n)
+    lobes_gpu, stats_gpu = g.download()
+    st = np.zeros((res ** 3, K, 4), np.float64)
+    lob = lobes0.copy()
+    O.lib().orc_vmm_estep(O.p(lob), K, O.p(pos), O.p(dirs), O.p(w), n, O.fa([0, 0, 0]), O.fa([1, 1, 1]), res, O.p(st))
+    assert np.allclose(stats_gpu, st, rtol=1e-4, atol=1e-6)
+    O.lib().orc_vmm_mstep(O.p(st), res ** 3, K, 512.0, 0.0, O.p(lob))
+    assert np.allclose(lobes_gpu, lob, rtol=2e-4, atol=2e-4)
