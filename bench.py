@@ -107,6 +107,8 @@ def main() -> int:
     ap.add_argument("--max-bounces", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the 1 GiB copy kernel (for hardware-counter calibration) and exit")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU split is "
@@ -114,6 +116,14 @@ def main() -> int:
         return 2
 
     ctx = core.Context(0)
+    if a.copy_calibration:
+        g = C.c_float()
+        L.call("rrh_stream_copy", ctx.h, 1 << 30, 5, C.byref(g))
+        print(json.dumps({"copy_bytes_per_dispatch": 1 << 30, "copy_gbps": round(g.value, 1)}))
+        return 0
+    hbm = C.c_float()
+    L.call("rrh_stream_copy", ctx.h, 1 << 30, 10, C.byref(hbm))
+    print(f"[bench] measured copy bandwidth {hbm.value:.0f} GB/s", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     sa = core.generate_scene(L.GEN_HEIGHTFIELD, seed=2, scale=1.0)
     scene = core.Scene(ctx, sa)
@@ -184,7 +194,7 @@ def main() -> int:
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_ray": round(bytes_per_ray, 1), "avg_launch_ms": round(avg_launch_ms, 4),
-                     "launches": launches.value},
+                     "launches": launches.value, "measured_copy_gbs": round(hbm.value, 1)},
         "cpu_baseline": None,
     }
     if not a.no_cpu_baseline:

@@ -173,6 +173,10 @@ int rrh_ray_totals(rrh_ctx* ctx, rrh_ray_counts* out, int reset);
 int rrh_fetch_counts(rrh_ctx* ctx, uint64_t* node_fetches, uint64_t* tri_fetches, uint64_t* rays, int reset);
 /* per-launch GPU time of the closest-hit traversal kernel inside rrh_pt_render, taken
  * with HIP events on the context stream while profiling is on */
+/* plain device-to-device copy kernel (no reference counterpart): measures the achievable
+ * HBM bandwidth on this device (2 x bytes / time, GB/s) and calibrates hardware
+ * byte counters against a kernel of known traffic */
+int rrh_stream_copy(rrh_ctx* ctx, size_t bytes, int iterations, float* gbps);
 int rrh_profile_enable(rrh_ctx* ctx, int on);
 int rrh_profile_traversal(rrh_ctx* ctx, float* total_ms, uint32_t* launches, int reset);
 

@@ -99,6 +99,13 @@ _SIGS = {
     "orc_vmm_estep": (None, [P, U, P, P, P, U, FP, FP, U, P]),
     "orc_vmm_mstep": (None, [P, U, U, F, C.c_double, P]),
     "orc_vmm_pdf": (C.c_double, [P, U, FP]),
+    "orc_mlp_param_count": (U, []),
+    "orc_mlp_encode": (None, [FP, FP, FP, FP, FP, P]),
+    "orc_mlp_forward": (None, [P, P, P, P]),
+    "orc_mlp_head": (None, [P, P]),
+    "orc_mlp_pdf": (C.c_double, [P, FP]),
+    "orc_mlp_loss_grad": (C.c_double, [P, P, P, P, U, P]),
+    "orc_adam": (None, [P, P, P, P, U, U, C.c_double, C.c_double, C.c_double, C.c_double]),
 }
 
 

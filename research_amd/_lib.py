@@ -128,6 +128,7 @@ SIGNATURES = {
     "rrh_ray_totals": [P, C.POINTER(RayCounts), i32],
     "rrh_fetch_counts": [P, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), i32],
     "rrh_profile_enable": [P, i32],
+    "rrh_stream_copy": [P, sz, i32, C.POINTER(f32)],
     "rrh_profile_traversal": [P, C.POINTER(f32), C.POINTER(u32), i32],
     "rrh_accumulate": [P, P, P, u32, u32],
     "rrh_accumulate_f64": [P, P, P, u32, u32],

@@ -5,12 +5,25 @@ Memory: device buffers belong to the objects below and are released by close()
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 from dataclasses import dataclass
 
 import numpy as np
 
 from . import _lib as L
+
+_EXITING = False
+
+
+def _mark_exit() -> None:
+    # at interpreter exit objects are finalised in arbitrary order: device memory is
+    # released with the process instead of through per-object destroy calls
+    global _EXITING
+    _EXITING = True
+
+
+atexit.register(_mark_exit)
 
 
 class Context:
@@ -44,6 +57,8 @@ class Context:
             self.h = None
 
     def __del__(self):
+        if _EXITING:
+            return
         try:
             self.close()
         except Exception:
@@ -135,6 +150,8 @@ class Scene:
             self.h = None
 
     def __del__(self):
+        if _EXITING:
+            return
         try:
             self.close()
         except Exception:
@@ -181,6 +198,8 @@ class Image:
             self.h = None
 
     def __del__(self):
+        if _EXITING:
+            return
         try:
             self.close()
         except Exception:
@@ -239,6 +258,8 @@ class Grid:
             self.h = None
 
     def __del__(self):
+        if _EXITING:
+            return
         try:
             self.close()
         except Exception:

@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(TX* TY) k_atrous_cfg(int W, int H, int st, flo
             int sx = clampi(x + dx, 0, W - 1), sy = clampi(y + dy, 0, H - 1);
             var += gk[dx + 1] * gk[dy + 1] * h2f(in[(size_t)sy * W + sx].w);
         }
-    float phil = sl * __fsqrt_rn(fmaxf(0.0f, var)) + 1e-10f;
+    float phil = sl * sqrtf(fmaxf(0.0f, var)) + 1e-10f;
     int xm = x > 0 ? x - 1 : 0, xp = x < W - 1 ? x + 1 : W - 1;
     int ym = y > 0 ? y - 1 : 0, yp = y < H - 1 ? y + 1 : H - 1;
     float dzdx = (h2f(gb[(size_t)y * W + xp].w) - h2f(gb[(size_t)y * W + xm].w)) * 0.5f;

@@ -241,6 +241,32 @@ bool build_bvh(const float* tri_v, uint32_t ntri, BvhOut& out, std::string& err)
     }
     out.depth = E.max_depth;
     if (out.depth > kMaxDepth + 1) { err = "BVH deeper than the traversal stack"; return false; }
+    // structural check before anything reaches the device: links in range, every
+    // triangle referenced exactly once (the single-leaf root references its leaf twice)
+    const uint32_t nn = (uint32_t)(out.nodes.size() / 4), ntris = (uint32_t)(out.tris.size() / 3);
+    if (ntris != ntri) { err = "internal: triangle count changed during the build"; return false; }
+    std::vector<uint8_t> seen(ntri, 0);
+    for (uint32_t n = 0; n < nn; ++n) {
+        int32_t links[2];
+        std::memcpy(&links[0], &out.nodes[n * 4 + 3].x, 4);
+        std::memcpy(&links[1], &out.nodes[n * 4 + 3].y, 4);
+        for (int k = 0; k < 2; ++k) {
+            int32_t l = links[k];
+            if (l >= 0) {
+                if ((uint32_t)l >= nn || (uint32_t)l <= n) { err = "internal: bad inner link"; return false; }
+            } else {
+                uint32_t first = ((uint32_t)~l) >> 3, cnt = (((uint32_t)~l) & 7u) + 1u;
+                if ((uint64_t)first + cnt > ntris) { err = "internal: leaf out of range"; return false; }
+                if (nn == 1 && k == 1 && links[0] == links[1]) continue;
+                for (uint32_t t = first; t < first + cnt; ++t) {
+                    if (seen[t]) { err = "internal: triangle referenced twice"; return false; }
+                    seen[t] = 1;
+                }
+            }
+        }
+    }
+    for (uint32_t t = 0; t < ntri; ++t)
+        if (!seen[t]) { err = "internal: triangle not referenced"; return false; }
     return true;
 }
 

@@ -151,7 +151,7 @@ int rrh_image_create(rrh_ctx* c, uint32_t w, uint32_t h, int fmt, rrh_image** ou
 }
 int rrh_image_destroy(rrh_image* im) {
     if (!im) return RRH_OK;
-    if (im->ctx) hipStreamSynchronize(im->ctx->stream);
+    (void)hipDeviceSynchronize();  // the owning context may already be gone
     if (im->p) hipFree(im->p);
     delete im;
     return RRH_OK;
@@ -399,7 +399,7 @@ int rrh_scene_upload(rrh_ctx* c, const rrh_vertex* verts, uint32_t nv, const uin
 
 int rrh_scene_destroy(rrh_scene* s) {
     if (!s) return RRH_OK;
-    if (s->ctx) hipStreamSynchronize(s->ctx->stream);
+    (void)hipDeviceSynchronize();  // the owning context may already be gone
     DevBuf* bufs[] = {&s->nodes, &s->tris, &s->tri_ref, &s->tri_light, &s->tl, &s->verts, &s->idx,
                       &s->meshes, &s->mats, &s->insts, &s->lights, &s->cam};
     for (DevBuf* b : bufs) b->release();
@@ -459,7 +459,37 @@ __global__ void __launch_bounds__(256) k_trace_occluded(DevScene S, const float4
     occ[i] = trace_occluded(S, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w) ? 1u : 0u;
 }
 
+__global__ void __launch_bounds__(256) k_stream_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 extern "C" {
+
+int rrh_stream_copy(rrh_ctx* c, size_t bytes, int iterations, float* gbps) {
+    RRH_ARG(c && gbps && iterations > 0, "rrh_stream_copy: bad argument");
+    RRH_ARG(bytes >= 4096 && bytes % 16 == 0, "rrh_stream_copy: bytes must be a multiple of 16, >= 4096");
+    void *a = nullptr, *b = nullptr;
+    RRH_HIP(hipMalloc(&a, bytes));
+    hipError_t e = hipMalloc(&b, bytes);
+    if (e != hipSuccess) { hipFree(a); set_error(hipGetErrorString(e)); return RRH_E_NOMEM; }
+    size_t n = bytes / 16;
+    int grid = 256 * 32;
+    (void)hipMemsetAsync(a, 0, bytes, c->stream);
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, c->stream, (const float4*)a, (float4*)b, n);  // warm-up
+    (void)hipEventRecord(c->markers[14], c->stream);
+    for (int k = 0; k < iterations; ++k)
+        hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, c->stream, (const float4*)a, (float4*)b, n);
+    (void)hipEventRecord(c->markers[15], c->stream);
+    e = hipEventSynchronize(c->markers[15]);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->markers[14], c->markers[15]);
+    hipFree(a);
+    hipFree(b);
+    if (e != hipSuccess) { set_error(hipGetErrorString(e)); return RRH_E_HIP; }
+    *gbps = (float)(2.0 * (double)bytes * iterations / (ms * 1e-3) / 1e9);
+    return RRH_OK;
+}
 
 int rrh_trace_closest_device(rrh_ctx* c, const rrh_scene* s, const void* d_rays, uint32_t n, void* d_hits) {
     RRH_ARG(c && s && d_rays && d_hits, "rrh_trace_closest_device: null argument");

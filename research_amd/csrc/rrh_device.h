@@ -27,8 +27,8 @@ RRH_D f3 divs(f3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
 RRH_D f3 neg(f3 a) { return mk(-a.x, -a.y, -a.z); }
 RRH_D float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 RRH_D f3 cross(f3 a, f3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
-RRH_D float length(f3 a) { return __fsqrt_rn(dot(a, a)); }
-RRH_D f3 normalize(f3 a) { float inv = 1.0f / __fsqrt_rn(dot(a, a)); return scale(a, inv); }
+RRH_D float length(f3 a) { return sqrtf(dot(a, a)); }
+RRH_D f3 normalize(f3 a) { float inv = 1.0f / sqrtf(dot(a, a)); return scale(a, inv); }
 RRH_D float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 RRH_D float lerpf(float x, float y, float s) { return x * (1.0f - s) + y * s; }
 RRH_D float mrow(const float* M, int r, float x, float y, float z, float w) {
@@ -338,11 +338,11 @@ RRH_D HitInfo fetch_hit(const DevScene& S, uint32_t prim, float b1, float b2) {
 #define K_EPS 1e-4f
 
 RRH_D f3 cosine_hemisphere(float u0, float u1, float& pdf) {
-    float r = __fsqrt_rn(u0);
+    float r = sqrtf(u0);
     float phi = K_TWO_PI * u1;
     float s, c; sincos_rr(phi, &s, &c);
     float x = r * c, y = r * s;
-    float z = __fsqrt_rn(fmaxf(0.0f, 1.0f - u0));
+    float z = sqrtf(fmaxf(0.0f, 1.0f - u0));
     pdf = z * K_INV_PI;
     return mk(x, y, z);
 }
@@ -401,16 +401,16 @@ RRH_D f3 sample_ggx_vndf(f3 wo, float roughness, float u0, float u1) {
     f3 Vh = normalize(mk(alpha * wo.x, alpha * wo.y, wo.z));
     float lensq = Vh.x * Vh.x + Vh.y * Vh.y;
     f3 T1;
-    if (lensq > 0.0f) { float il = __fsqrt_rn(lensq); T1 = mk(-Vh.y / il, Vh.x / il, 0.0f / il); }
+    if (lensq > 0.0f) { float il = sqrtf(lensq); T1 = mk(-Vh.y / il, Vh.x / il, 0.0f / il); }
     else T1 = mk(1.0f, 0.0f, 0.0f);
     f3 T2 = cross(Vh, T1);
-    float r2 = __fsqrt_rn(u0);
+    float r2 = sqrtf(u0);
     float phi = K_TWO_PI * u1;
     float s, c; sincos_rr(phi, &s, &c);
     float t1 = r2 * c, t2 = r2 * s;
     float sv = 0.5f * (1.0f + Vh.z);
-    t2 = (1.0f - sv) * __fsqrt_rn(1.0f - t1 * t1) + sv * t2;
-    float tz = __fsqrt_rn(fmaxf(0.0f, 1.0f - t1 * t1 - t2 * t2));
+    t2 = (1.0f - sv) * sqrtf(1.0f - t1 * t1) + sv * t2;
+    float tz = sqrtf(fmaxf(0.0f, 1.0f - t1 * t1 - t2 * t2));
     f3 Nh = add(add(scale(T1, t1), scale(T2, t2)), scale(Vh, tz));
     return normalize(mk(alpha * Nh.x, alpha * Nh.y, fmaxf(0.0f, Nh.z)));
 }
@@ -509,7 +509,7 @@ RRH_D TriSample tri_light_point(const DevScene& S, uint32_t tl, float u1, float 
     const float4* L = S.tl + (size_t)tl * 4;
     float4 q0 = L[0], q1 = L[1], q2 = L[2];
     f3 a = xyz(q0), b = xyz(q1), c = xyz(q2);
-    float su = __fsqrt_rn(u1);
+    float su = sqrtf(u1);
     float b0 = 1.0f - su;
     float b1 = u2 * su;
     float b2 = 1.0f - b0 - b1;
@@ -549,7 +549,7 @@ RRH_D f3 vmf_sample(f3 mu, float kappa, float u0, float u1, float& pdf) {
     float k = fmaxf(kappa, 1.0e-3f);
     float w = 1.0f + (1.0f / k) * log_rr(u0 + (1.0f - u0) * exp_rr(-2.0f * k));
     w = clampf(w, -1.0f, 1.0f);
-    float s = __fsqrt_rn(fmaxf(0.0f, 1.0f - w * w));
+    float s = sqrtf(fmaxf(0.0f, 1.0f - w * w));
     float phi = K_TWO_PI * u1;
     float sp, cp; sincos_rr(phi, &sp, &cp);
     f3 local = mk(s * cp, s * sp, w);

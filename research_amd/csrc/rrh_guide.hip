@@ -375,7 +375,7 @@ int rrh_grid_create(rrh_ctx* c, const rrh_grid_params* p, rrh_grid** out) {
 
 int rrh_grid_destroy(rrh_grid* g) {
     if (!g) return RRH_OK;
-    if (g->ctx) hipStreamSynchronize(g->ctx->stream);
+    (void)hipDeviceSynchronize();  // the owning context may already be gone
     g->accum.release(); g->fitted.release(); g->lobes.release(); g->stats.release(); g->fstats.release(); g->valid.release();
     delete g;
     return RRH_OK;

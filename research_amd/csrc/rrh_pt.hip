@@ -118,7 +118,7 @@ __device__ __forceinline__ NeeOut nee_candidate(const DevScene& S, f3 hit, f3 n,
     TriSample ts = tri_light_point(S, li - S.nl, u1, u2);
     f3 to = sub(ts.p, hit);
     float d2 = dot(to, to);
-    float dist = __fsqrt_rn(d2);
+    float dist = sqrtf(d2);
     f3 l = scale(to, 1.0f / dist);
     if (dot(l, n) <= 0.0f) return r;
     float cosl = fabsf(dot(ts.nl, l));

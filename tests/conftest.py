@@ -17,5 +17,4 @@ def ctx():
     from research_amd import core
 
     c = core.Context(0)
-    yield c
-    c.close()
+    yield c  # released with the process: scenes and images created by tests still use it
