@@ -208,8 +208,8 @@ def atrous_ref(inp, gpos, gnrm, stride, sigma_p=0.25, sigma_n=0.2, sigma_l=1.0):
     H, W = inp.shape[:2]
     out = np.zeros_like(inp, dtype=np.float32)
     prm = AtrousParams(W, H, stride, 0, sigma_p, sigma_n, sigma_l, 0.0)
-    lib().orc_atrous_ref(C.byref(prm), p(np.ascontiguousarray(inp, np.float32)),
-                         p(np.ascontiguousarray(gpos, np.float32)), p(np.ascontiguousarray(gnrm, np.float32)), p(out))
+    a, b, c = (np.ascontiguousarray(v, np.float32) for v in (inp, gpos, gnrm))  # alive across the call
+    lib().orc_atrous_ref(C.byref(prm), p(a), p(b), p(c), p(out))
     return out
 
 
@@ -217,6 +217,6 @@ def atrous_cfg(inp16, gbuf16, stride, sigma_z=1.0, sigma_n=128.0, sigma_l=4.0):
     H, W = inp16.shape[:2]
     out = np.zeros_like(inp16, dtype=np.uint16)
     prm = AtrousParams(W, H, stride, 1, sigma_z, sigma_n, sigma_l, 0.0)
-    lib().orc_atrous_cfg(C.byref(prm), p(np.ascontiguousarray(inp16, np.uint16)),
-                         p(np.ascontiguousarray(gbuf16, np.uint16)), p(out))
+    a, b = np.ascontiguousarray(inp16, np.uint16), np.ascontiguousarray(gbuf16, np.uint16)
+    lib().orc_atrous_cfg(C.byref(prm), p(a), p(b), p(out))
     return out

@@ -205,19 +205,33 @@ RRH_D bool better(float t, uint32_t prim, float bt, uint32_t bp) { return t < bt
 // child link encoding: >= 0 inner node index; < 0 leaf = ~((first << 3) | (count - 1))
 #define RRH_LEAF_FIRST(c) ((uint32_t)(~(c)) >> 3)
 #define RRH_LEAF_COUNT(c) ((((uint32_t)(~(c))) & 7u) + 1u)
-#define RRH_STACK 64
+#define RRH_STACK 64       // private (scratch) stack: any BVH the builder accepts (depth <= 63)
+#define RRH_LDS_STACK 32   // LDS stack: BVHs of depth <= 32
 
 struct Hit { float t; uint32_t prim; float b1, b2; };
 
-// closest hit over the scene BVH; the result is independent of traversal
-// order (ties on t resolve to the smaller flattened id)
-// COUNT = true is the instrumented variant: it counts node and triangle fetches
-template <bool COUNT = false>
-RRH_D Hit trace_closest(const DevScene& S, f3 o, f3 d, float tmin, float tmax, uint32_t* n_nodes = nullptr,
-                        uint32_t* n_tris = nullptr) {
+// traversal stacks: a lane's entries in LDS are 64 words apart (one wave's lanes
+// hit 64 consecutive words, so there are no bank conflicts); the private variant
+// is the fallback for deeper trees
+struct StackLds {
+    int* p;
+    RRH_D void put(int i, int v) { p[i * 64] = v; }
+    RRH_D int get(int i) const { return p[i * 64]; }
+};
+struct StackPriv {
+    int s[RRH_STACK];
+    RRH_D void put(int i, int v) { s[i] = v; }
+    RRH_D int get(int i) const { return s[i]; }
+};
+
+// closest hit over the scene BVH; the result is independent of traversal order
+// (ties on t resolve to the smaller flattened id).  COUNT = true is the
+// instrumented variant that counts node and triangle fetches.
+template <bool COUNT, class Stk>
+RRH_D Hit trace_closest_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack,
+                          uint32_t* n_nodes = nullptr, uint32_t* n_tris = nullptr) {
     Hit h; h.t = tmax; h.prim = ~0u; h.b1 = 0.0f; h.b2 = 0.0f;
     WRay r = wray_setup(o, d);
-    int stack[RRH_STACK];
     int sp = 0;
     int node = 0;
     while (true) {
@@ -232,7 +246,7 @@ RRH_D Hit trace_closest(const DevScene& S, f3 o, f3 d, float tmin, float tmax, u
             if (hl && hr) {
                 int nearc = lk.x, farc = lk.y;
                 if (tr < tl) { nearc = lk.y; farc = lk.x; }
-                stack[sp++] = farc;
+                stack.put(sp++, farc);
                 node = nearc;
             } else if (hl) {
                 node = lk.x;
@@ -240,7 +254,7 @@ RRH_D Hit trace_closest(const DevScene& S, f3 o, f3 d, float tmin, float tmax, u
                 node = lk.y;
             } else {
                 if (sp == 0) return h;
-                node = stack[--sp];
+                node = stack.get(--sp);
             }
         }
         // leaf
@@ -256,14 +270,14 @@ RRH_D Hit trace_closest(const DevScene& S, f3 o, f3 d, float tmin, float tmax, u
             }
         }
         if (sp == 0) return h;
-        node = stack[--sp];
+        node = stack.get(--sp);
     }
 }
 
-RRH_D bool trace_occluded(const DevScene& S, f3 o, f3 d, float tmin, float tmax) {
+template <class Stk>
+RRH_D bool trace_occluded_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack) {
     if (!(tmax >= tmin)) return false;
     WRay r = wray_setup(o, d);
-    int stack[RRH_STACK];
     int sp = 0;
     int node = 0;
     while (true) {
@@ -275,7 +289,7 @@ RRH_D bool trace_occluded(const DevScene& S, f3 o, f3 d, float tmin, float tmax)
             bool hl = box_hit(a.x, a.y, a.z, a.w, c.x, c.y, r, tmin, tmax, tl);
             bool hr = box_hit(b.x, b.y, b.z, b.w, c.z, c.w, r, tmin, tmax, tr);
             if (hl && hr) {
-                stack[sp++] = lk.y;
+                stack.put(sp++, lk.y);
                 node = lk.x;
             } else if (hl) {
                 node = lk.x;
@@ -283,7 +297,7 @@ RRH_D bool trace_occluded(const DevScene& S, f3 o, f3 d, float tmin, float tmax)
                 node = lk.y;
             } else {
                 if (sp == 0) return false;
-                node = stack[--sp];
+                node = stack.get(--sp);
             }
         }
         uint32_t first = RRH_LEAF_FIRST(node), cnt = RRH_LEAF_COUNT(node);
@@ -294,8 +308,25 @@ RRH_D bool trace_occluded(const DevScene& S, f3 o, f3 d, float tmin, float tmax)
             if (wt_hit(r, xyz(p0), xyz(p1), xyz(p2), tmin, tmax, tt, b1, b2)) return true;
         }
         if (sp == 0) return false;
-        node = stack[--sp];
+        node = stack.get(--sp);
     }
+}
+
+// private-stack wrappers (any depth)
+template <bool COUNT = false>
+RRH_D Hit trace_closest(const DevScene& S, f3 o, f3 d, float tmin, float tmax, uint32_t* n_nodes = nullptr,
+                        uint32_t* n_tris = nullptr) {
+    StackPriv st;
+    return trace_closest_s<COUNT>(S, o, d, tmin, tmax, st, n_nodes, n_tris);
+}
+RRH_D bool trace_occluded(const DevScene& S, f3 o, f3 d, float tmin, float tmax) {
+    StackPriv st;
+    return trace_occluded_s(S, o, d, tmin, tmax, st);
+}
+
+// per-lane LDS stack base inside a block-wide array of (blockDim/64) x RRH_LDS_STACK x 64 ints
+RRH_D StackLds lds_stack(int* smem) {
+    return StackLds{smem + (threadIdx.x >> 6) * (RRH_LDS_STACK * 64) + (threadIdx.x & 63)};
 }
 
 // ---------------------------------------------------------------------------

@@ -24,37 +24,61 @@ struct Q {
     float4 *o, *d, *thp;
 };
 
+// one 8x8 pixel tile per wave; the queue is filled in tile order (no atomics): for an
+// image whose sides are multiples of 8 the slot is tile * 64 + lane, otherwise row-major
 __global__ void __launch_bounds__(64) k_raygen(DevScene S, Q out, uint32_t* __restrict__ cnt, float4* __restrict__ rad,
                                                uint32_t frame_seed) {
     const rrh_camera& cam = *S.cam;
     uint32_t W = cam.screen_width, H = cam.screen_height;
     uint32_t px = blockIdx.x * 8 + (threadIdx.x & 7u);
     uint32_t py = blockIdx.y * 8 + (threadIdx.x >> 3);
-    bool valid = px < W && py < H;
-    f3 d = mk(0, 0, 0);
-    uint32_t seed = 0;
-    if (valid) {
-        seed = make_seed(px, py, frame_seed);
-        d = primary_dir(cam, px, py, seed);
-        rad[(size_t)py * W + px] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-    }
-    uint32_t slot = wave_push(&cnt[0], valid);
-    if (valid) {
-        uint32_t pix = py * W + px;
-        out.o[slot] = make_float4(cam.position[0], cam.position[1], cam.position[2], __uint_as_float(pix));
-        out.d[slot] = make_float4(d.x, d.y, d.z, __uint_as_float(seed));
-        out.thp[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    }
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) cnt[0] = W * H;
+    if (px >= W || py >= H) return;
+    uint32_t seed = make_seed(px, py, frame_seed);
+    f3 d = primary_dir(cam, px, py, seed);
+    uint32_t pix = py * W + px;
+    rad[pix] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    uint32_t slot = ((W & 7u) == 0u && (H & 7u) == 0u) ? (blockIdx.y * gridDim.x + blockIdx.x) * 64u + threadIdx.x : pix;
+    out.o[slot] = make_float4(cam.position[0], cam.position[1], cam.position[2], __uint_as_float(pix));
+    out.d[slot] = make_float4(d.x, d.y, d.z, __uint_as_float(seed));
+    out.thp[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
 }
 
-template <bool COUNT>
+// block-aggregated queue append: one atomic per block; every thread of the block calls it
+__device__ __forceinline__ uint32_t block_push(uint32_t* counter, bool pred, uint32_t* s_cnt) {
+    const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63u, nw = blockDim.x >> 6;
+    unsigned long long mask = __ballot(pred ? 1 : 0);
+    unsigned long long lt = (lane == 0u) ? 0ull : (~0ull >> (64u - lane));
+    uint32_t off = (uint32_t)__popcll(mask & lt);
+    if (lane == 0) s_cnt[wid] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (uint32_t w = 0; w < nw; ++w) { uint32_t c = s_cnt[w]; s_cnt[w] = tot; tot += c; }
+        s_cnt[nw] = tot ? atomicAdd(counter, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t slot = s_cnt[nw] + s_cnt[wid] + off;
+    __syncthreads();  // s_cnt is reused by the next call
+    return slot;
+}
+
+template <bool COUNT, bool LDS>
 __global__ void __launch_bounds__(kBlock) k_intersect(DevScene S, Q in, const uint32_t* __restrict__ cnt, uint32_t b,
                                                       float4* __restrict__ hits, unsigned long long* __restrict__ fetch) {
+    __shared__ int s_stack[LDS ? (kBlock / 64) * RRH_LDS_STACK * 64 : 1];
     uint32_t n = cnt[b];
     uint32_t nn = 0, nt = 0, nr = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float4 o = in.o[i], d = in.d[i];
-        Hit h = trace_closest<COUNT>(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, 1e9f, &nn, &nt);
+        Hit h;
+        if constexpr (LDS) {
+            StackLds st = lds_stack(s_stack);
+            h = trace_closest_s<COUNT>(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, 1e9f, st, &nn, &nt);
+        } else {
+            StackPriv st;
+            h = trace_closest_s<COUNT>(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, 1e9f, st, &nn, &nt);
+        }
         hits[i] = make_float4(h.t, __uint_as_float(h.prim), h.b1, h.b2);
         nr++;
     }
@@ -150,6 +174,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, Q in, Q out, const
     const uint32_t n = cnt[b];
     const uint32_t stride = gridDim.x * blockDim.x;
     const bool last = (b + 1u >= A.max_bounces);
+    __shared__ uint32_t s_cnt[kBlock / 64 + 1];
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
         uint32_t i = base + threadIdx.x;
         bool valid = i < n;
@@ -224,21 +249,31 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, Q in, Q out, const
                 }
             }
         }
-        uint32_t es = wave_push(&cnt[b + 1], push_ext);
+        uint32_t es = block_push(&cnt[b + 1], push_ext, s_cnt);
         if (push_ext) { out.o[es] = no; out.d[es] = nd; out.thp[es] = nt; }
-        uint32_t ss = wave_push(&cnt[64 + b], push_sh);
+        uint32_t ss = block_push(&cnt[64 + b], push_sh, s_cnt);
         if (push_sh) { sh_o[ss] = so; sh_d[ss] = sd; sh_c[ss] = sc; }
     }
 }
 
+template <bool LDS>
 __global__ void __launch_bounds__(kBlock) k_shadow(DevScene S, const uint32_t* __restrict__ cnt, uint32_t b,
                                                    const float4* __restrict__ sh_o, const float4* __restrict__ sh_d,
                                                    const float4* __restrict__ sh_c, float4* __restrict__ rad,
                                                    VertexRec rec) {
+    __shared__ int s_stack[LDS ? (kBlock / 64) * RRH_LDS_STACK * 64 : 1];
     uint32_t n = cnt[64 + b];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float4 o = sh_o[i], d = sh_d[i];
-        if (!trace_occluded(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, o.w)) {
+        bool occ;
+        if constexpr (LDS) {
+            StackLds st = lds_stack(s_stack);
+            occ = trace_occluded_s(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, o.w, st);
+        } else {
+            StackPriv st;
+            occ = trace_occluded_s(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, o.w, st);
+        }
+        if (!occ) {
             uint32_t pixel = __float_as_uint(d.w);
             float4 c = sh_c[i];
             float4 L = rad[pixel];
@@ -340,12 +375,16 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
             }
             RRH_HIP(hipEventRecord(c->prof_events[c->prof_used], c->stream));
         }
-        if (p->flags & RRH_PT_COUNT_FETCHES)
-            hipLaunchKernelGGL(k_intersect<true>, dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b,
-                               (float4*)c->hits.p, (unsigned long long*)c->fetch.p);
-        else
-            hipLaunchKernelGGL(k_intersect<false>, dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b,
-                               (float4*)c->hits.p, (unsigned long long*)nullptr);
+        const bool lds = s->depth <= RRH_LDS_STACK;
+        unsigned long long* fc = (unsigned long long*)c->fetch.p;
+        float4* hp = (float4*)c->hits.p;
+        if (p->flags & RRH_PT_COUNT_FETCHES) {
+            if (lds) hipLaunchKernelGGL((k_intersect<true, true>), dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, fc);
+            else hipLaunchKernelGGL((k_intersect<true, false>), dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, fc);
+        } else {
+            if (lds) hipLaunchKernelGGL((k_intersect<false, true>), dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, fc);
+            else hipLaunchKernelGGL((k_intersect<false, false>), dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, fc);
+        }
         RRH_LAUNCH_CHECK();
         if (c->profiling) {
             RRH_HIP(hipEventRecord(c->prof_events[c->prof_used + 1], c->stream));
@@ -355,8 +394,12 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
         hipLaunchKernelGGL(k_shade, dim3(g), dim3(kBlock), 0, c->stream, S, qi, qo, (const float4*)c->hits.p, cnt,
                            (float4*)c->sh_o.p, (float4*)c->sh_d.p, (float4*)c->sh_c.p, rad, A);
         RRH_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_shadow, dim3(g), dim3(kBlock), 0, c->stream, S, (const uint32_t*)cnt, b,
-                           (const float4*)c->sh_o.p, (const float4*)c->sh_d.p, (const float4*)c->sh_c.p, rad, A.rec);
+        if (s->depth <= RRH_LDS_STACK)
+            hipLaunchKernelGGL(k_shadow<true>, dim3(g), dim3(kBlock), 0, c->stream, S, (const uint32_t*)cnt, b,
+                               (const float4*)c->sh_o.p, (const float4*)c->sh_d.p, (const float4*)c->sh_c.p, rad, A.rec);
+        else
+            hipLaunchKernelGGL(k_shadow<false>, dim3(g), dim3(kBlock), 0, c->stream, S, (const uint32_t*)cnt, b,
+                               (const float4*)c->sh_o.p, (const float4*)c->sh_d.p, (const float4*)c->sh_c.p, rad, A.rec);
         RRH_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(64), 0, c->stream, (const uint32_t*)cnt, p->max_bounces,

@@ -50,7 +50,8 @@ def test_hash_grid_splat_fit_sample_match_oracle(ctx):
     seeds = np.arange(m, dtype=np.uint32) * 7919 + 1
     dg, pg = np.zeros((m, 3), np.float32), np.zeros(m, np.float32)
     sg = seeds.copy()
-    L.call("rrh_grid_sample_primary", ctx.h, g.h, L.ptr(sp), L.ptr(nrm), L.ptr(bsdf[:m].copy()), L.ptr(sg), m,
+    sb = bsdf[:m].copy()  # keep a reference: the library reads it during the call
+    L.call("rrh_grid_sample_primary", ctx.h, g.h, L.ptr(sp), L.ptr(nrm), L.ptr(sb), L.ptr(sg), m,
            C.byref(L.f3(cam)), 0.5, L.ptr(dg), L.ptr(pg))
     do, po = np.zeros(3, np.float32), np.zeros(1, np.float32)
     exact = 0

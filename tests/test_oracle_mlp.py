@@ -59,7 +59,8 @@ def test_head_mixture_is_a_density():
     z = np.load(GOLD)
     params = np.ascontiguousarray(z["params"])
     acts, out, lobes = np.zeros(256), np.zeros(40), np.zeros(40)
-    lib.orc_mlp_forward(O.p(params), O.p(np.ascontiguousarray(z["x"][0])), O.p(acts), O.p(out))
+    x0 = np.ascontiguousarray(z["x"][0])
+    lib.orc_mlp_forward(O.p(params), O.p(x0), O.p(acts), O.p(out))
     lib.orc_mlp_head(O.p(out), O.p(lobes))
     L = lobes.reshape(8, 5)
     assert abs(L[:, 4].sum() - 1) < 1e-12 and (L[:, 3] >= 0.1).all()
