@@ -205,8 +205,8 @@ RRH_D bool better(float t, uint32_t prim, float bt, uint32_t bp) { return t < bt
 // child link encoding: >= 0 inner node index; < 0 leaf = ~((first << 3) | (count - 1))
 #define RRH_LEAF_FIRST(c) ((uint32_t)(~(c)) >> 3)
 #define RRH_LEAF_COUNT(c) ((((uint32_t)(~(c))) & 7u) + 1u)
-#define RRH_STACK 64       // private (scratch) stack: any BVH the builder accepts (depth <= 63)
-#define RRH_LDS_STACK 32   // LDS stack: BVHs of depth <= 32
+#define RRH_STACK 64       // stack entries: any BVH the builder accepts (depth <= 63)
+#define RRH_LDS_STACK 16   // entries kept in LDS; deeper ones go to a private overflow array
 
 struct Hit { float t; uint32_t prim; float b1, b2; };
 
@@ -222,6 +222,16 @@ struct StackPriv {
     int s[RRH_STACK];
     RRH_D void put(int i, int v) { s[i] = v; }
     RRH_D int get(int i) const { return s[i]; }
+};
+// the top RRH_LDS_STACK entries in LDS, the rest (rarely reached) in private memory
+struct StackHybrid {
+    int* p;
+    int ov[RRH_STACK - RRH_LDS_STACK];
+    RRH_D void put(int i, int v) {
+        if (i < RRH_LDS_STACK) p[i * 64] = v;
+        else ov[i - RRH_LDS_STACK] = v;
+    }
+    RRH_D int get(int i) const { return (i < RRH_LDS_STACK) ? p[i * 64] : ov[i - RRH_LDS_STACK]; }
 };
 
 // closest hit over the scene BVH; the result is independent of traversal order
@@ -325,8 +335,13 @@ RRH_D bool trace_occluded(const DevScene& S, f3 o, f3 d, float tmin, float tmax)
 }
 
 // per-lane LDS stack base inside a block-wide array of (blockDim/64) x RRH_LDS_STACK x 64 ints
-RRH_D StackLds lds_stack(int* smem) {
-    return StackLds{smem + (threadIdx.x >> 6) * (RRH_LDS_STACK * 64) + (threadIdx.x & 63)};
+RRH_D int* lds_stack_base(int* smem) { return smem + (threadIdx.x >> 6) * (RRH_LDS_STACK * 64) + (threadIdx.x & 63); }
+
+// XCD-aware block order: the hardware deals consecutive workgroups round-robin to the
+// 8 XCDs (each with its own L2); remapping gives every XCD one contiguous range of
+// work (an image band / queue range) so neighbouring rays share that XCD's L2
+RRH_D uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    return (nb & 7u) ? b : (b & 7u) * (nb >> 3) + (b >> 3);
 }
 
 // ---------------------------------------------------------------------------

@@ -63,22 +63,18 @@ __device__ __forceinline__ uint32_t block_push(uint32_t* counter, bool pred, uin
     return slot;
 }
 
-template <bool COUNT, bool LDS>
+template <bool COUNT>
 __global__ void __launch_bounds__(kBlock) k_intersect(DevScene S, Q in, const uint32_t* __restrict__ cnt, uint32_t b,
                                                       float4* __restrict__ hits, unsigned long long* __restrict__ fetch) {
-    __shared__ int s_stack[LDS ? (kBlock / 64) * RRH_LDS_STACK * 64 : 1];
-    uint32_t n = cnt[b];
+    __shared__ int s_stack[(kBlock / 64) * RRH_LDS_STACK * 64];
+    const uint32_t n = cnt[b];
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     uint32_t nn = 0, nt = 0, nr = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    StackHybrid st;
+    st.p = lds_stack_base(s_stack);
+    for (uint32_t i = blk * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float4 o = in.o[i], d = in.d[i];
-        Hit h;
-        if constexpr (LDS) {
-            StackLds st = lds_stack(s_stack);
-            h = trace_closest_s<COUNT>(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, 1e9f, st, &nn, &nt);
-        } else {
-            StackPriv st;
-            h = trace_closest_s<COUNT>(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, 1e9f, st, &nn, &nt);
-        }
+        Hit h = trace_closest_s<COUNT>(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, 1e9f, st, &nn, &nt);
         hits[i] = make_float4(h.t, __uint_as_float(h.prim), h.b1, h.b2);
         nr++;
     }
@@ -175,7 +171,8 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, Q in, Q out, const
     const uint32_t stride = gridDim.x * blockDim.x;
     const bool last = (b + 1u >= A.max_bounces);
     __shared__ uint32_t s_cnt[kBlock / 64 + 1];
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    for (uint32_t base = blk * blockDim.x; base < n; base += stride) {
         uint32_t i = base + threadIdx.x;
         bool valid = i < n;
         bool push_ext = false, push_sh = false;
@@ -256,24 +253,18 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, Q in, Q out, const
     }
 }
 
-template <bool LDS>
 __global__ void __launch_bounds__(kBlock) k_shadow(DevScene S, const uint32_t* __restrict__ cnt, uint32_t b,
                                                    const float4* __restrict__ sh_o, const float4* __restrict__ sh_d,
                                                    const float4* __restrict__ sh_c, float4* __restrict__ rad,
                                                    VertexRec rec) {
-    __shared__ int s_stack[LDS ? (kBlock / 64) * RRH_LDS_STACK * 64 : 1];
-    uint32_t n = cnt[64 + b];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    __shared__ int s_stack[(kBlock / 64) * RRH_LDS_STACK * 64];
+    const uint32_t n = cnt[64 + b];
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    StackHybrid st;
+    st.p = lds_stack_base(s_stack);
+    for (uint32_t i = blk * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float4 o = sh_o[i], d = sh_d[i];
-        bool occ;
-        if constexpr (LDS) {
-            StackLds st = lds_stack(s_stack);
-            occ = trace_occluded_s(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, o.w, st);
-        } else {
-            StackPriv st;
-            occ = trace_occluded_s(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, o.w, st);
-        }
-        if (!occ) {
+        if (!trace_occluded_s(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, o.w, st)) {
             uint32_t pixel = __float_as_uint(d.w);
             float4 c = sh_c[i];
             float4 L = rad[pixel];
@@ -375,16 +366,12 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
             }
             RRH_HIP(hipEventRecord(c->prof_events[c->prof_used], c->stream));
         }
-        const bool lds = s->depth <= RRH_LDS_STACK;
         unsigned long long* fc = (unsigned long long*)c->fetch.p;
         float4* hp = (float4*)c->hits.p;
-        if (p->flags & RRH_PT_COUNT_FETCHES) {
-            if (lds) hipLaunchKernelGGL((k_intersect<true, true>), dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, fc);
-            else hipLaunchKernelGGL((k_intersect<true, false>), dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, fc);
-        } else {
-            if (lds) hipLaunchKernelGGL((k_intersect<false, true>), dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, fc);
-            else hipLaunchKernelGGL((k_intersect<false, false>), dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, fc);
-        }
+        if (p->flags & RRH_PT_COUNT_FETCHES)
+            hipLaunchKernelGGL(k_intersect<true>, dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, fc);
+        else
+            hipLaunchKernelGGL(k_intersect<false>, dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, fc);
         RRH_LAUNCH_CHECK();
         if (c->profiling) {
             RRH_HIP(hipEventRecord(c->prof_events[c->prof_used + 1], c->stream));
@@ -394,12 +381,8 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
         hipLaunchKernelGGL(k_shade, dim3(g), dim3(kBlock), 0, c->stream, S, qi, qo, (const float4*)c->hits.p, cnt,
                            (float4*)c->sh_o.p, (float4*)c->sh_d.p, (float4*)c->sh_c.p, rad, A);
         RRH_LAUNCH_CHECK();
-        if (s->depth <= RRH_LDS_STACK)
-            hipLaunchKernelGGL(k_shadow<true>, dim3(g), dim3(kBlock), 0, c->stream, S, (const uint32_t*)cnt, b,
-                               (const float4*)c->sh_o.p, (const float4*)c->sh_d.p, (const float4*)c->sh_c.p, rad, A.rec);
-        else
-            hipLaunchKernelGGL(k_shadow<false>, dim3(g), dim3(kBlock), 0, c->stream, S, (const uint32_t*)cnt, b,
-                               (const float4*)c->sh_o.p, (const float4*)c->sh_d.p, (const float4*)c->sh_c.p, rad, A.rec);
+        hipLaunchKernelGGL(k_shadow, dim3(g), dim3(kBlock), 0, c->stream, S, (const uint32_t*)cnt, b,
+                           (const float4*)c->sh_o.p, (const float4*)c->sh_d.p, (const float4*)c->sh_c.p, rad, A.rec);
         RRH_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(64), 0, c->stream, (const uint32_t*)cnt, p->max_bounces,
