@@ -107,6 +107,9 @@ def main() -> int:
     ap.add_argument("--max-bounces", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--stack", choices=["private", "lds32", "hybrid16"], default="lds32",
+                    help="where the traversal stack lives (context option)")
+    ap.add_argument("--xcd-remap", type=int, default=0, help="1: contiguous work range per XCD")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB copy kernel (for hardware-counter calibration) and exit")
     a = ap.parse_args()
@@ -116,6 +119,9 @@ def main() -> int:
         return 2
 
     ctx = core.Context(0)
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAVERSAL_STACK,
+           {"private": L.STACK_PRIVATE, "lds32": L.STACK_LDS32, "hybrid16": L.STACK_HYBRID16}[a.stack])
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_XCD_REMAP, a.xcd_remap)
     if a.copy_calibration:
         g = C.c_float()
         L.call("rrh_stream_copy", ctx.h, 1 << 30, 5, C.byref(g))
@@ -164,13 +170,14 @@ def main() -> int:
     avg_launch_ms = trav_ms.value / max(launches.value, 1)
     bytes_per_launch = bytes_per_ray * pt_trace_rays / max(launches.value, 1)
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
+    # HBM bytes per k_intersect launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # summary (calibrated against the copy kernel); null when the summary is absent
     traffic = None
-    tfile = ROOT / "profiles" / "k_intersect_traffic.json"
+    tfile = ROOT / "profiles" / "heightfield_1080p_hbm_traffic.md"
     if tfile.exists():
-        try:
-            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
+        for line in tfile.read_text().splitlines():
+            if line.startswith("hbm_bytes_per_launch k_intersect:"):
+                traffic = int(float(line.split(":")[1]))
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -188,6 +195,7 @@ def main() -> int:
                    "width": W, "height": H, "spp_per_frame": 1, "max_bounces": a.max_bounces,
                    "di_candidates": 32, "di_temporal_mcap": "20*M", "di_spatial": "5 neighbours, r=30",
                    "camera": "orbit 0.5 deg/frame", "parallelism": "single GPU",
+                   "traversal_stack": a.stack, "xcd_remap": a.xcd_remap,
                    "rays_per_step": {"primary": prim // a.steps, "extension": ext // a.steps,
                                      "shadow": shadow // a.steps}},
         "roofline": {"bound": "hbm", "kernel": "k_intersect (closest-hit BVH traversal)",

@@ -173,6 +173,13 @@ int rrh_ray_totals(rrh_ctx* ctx, rrh_ray_counts* out, int reset);
 int rrh_fetch_counts(rrh_ctx* ctx, uint64_t* node_fetches, uint64_t* tri_fetches, uint64_t* rays, int reset);
 /* per-launch GPU time of the closest-hit traversal kernel inside rrh_pt_render, taken
  * with HIP events on the context stream while profiling is on */
+/* tuning options of a context (no reference counterpart) */
+#define RRH_OPT_TRAVERSAL_STACK 1   /* RRH_STACK_* : where the BVH traversal stack lives */
+#define RRH_OPT_XCD_REMAP 2         /* 1: give each XCD a contiguous range of work items */
+#define RRH_STACK_PRIVATE 0         /* per-lane private (scratch) array, any depth */
+#define RRH_STACK_LDS32 1           /* 32 entries per lane in LDS (BVH depth <= 32, else private) */
+#define RRH_STACK_HYBRID16 2        /* 16 entries in LDS + private overflow, any depth */
+int rrh_ctx_set_option(rrh_ctx* ctx, int option, int value);
 /* plain device-to-device copy kernel (no reference counterpart): measures the achievable
  * HBM bandwidth on this device (2 x bytes / time, GB/s) and calibrates hardware
  * byte counters against a kernel of known traffic */

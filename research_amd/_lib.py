@@ -36,6 +36,8 @@ FMT_RGBA32F, FMT_RGBA16F, FMT_RGBA64F, FMT_R32U = 0, 1, 2, 3
 FMT_BYTES = {FMT_RGBA32F: 16, FMT_RGBA16F: 8, FMT_RGBA64F: 32, FMT_R32U: 4}
 GUIDE_NONE, GUIDE_VMM, GUIDE_MLP = 0, 1, 2
 PT_COUNT_FETCHES = 1
+OPT_TRAVERSAL_STACK, OPT_XCD_REMAP = 1, 2
+STACK_PRIVATE, STACK_LDS32, STACK_HYBRID16 = 0, 1, 2
 GRID_HASH, GRID_VMM = 0, 1
 ATROUS_REFERENCE, ATROUS_CONFIG = 0, 1
 
@@ -129,6 +131,7 @@ SIGNATURES = {
     "rrh_fetch_counts": [P, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), i32],
     "rrh_profile_enable": [P, i32],
     "rrh_stream_copy": [P, sz, i32, C.POINTER(f32)],
+    "rrh_ctx_set_option": [P, i32, i32],
     "rrh_profile_traversal": [P, C.POINTER(f32), C.POINTER(u32), i32],
     "rrh_accumulate": [P, P, P, u32, u32],
     "rrh_accumulate_f64": [P, P, P, u32, u32],

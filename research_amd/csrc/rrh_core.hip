@@ -119,6 +119,18 @@ int rrh_device_info(rrh_ctx* c, int* cu_count, int* clock_khz, char* name, size_
     return RRH_OK;
 }
 
+int rrh_ctx_set_option(rrh_ctx* c, int option, int value) {
+    RRH_ARG(c, "rrh_ctx_set_option: null context");
+    if (option == RRH_OPT_TRAVERSAL_STACK) {
+        RRH_ARG(value >= RRH_STACK_PRIVATE && value <= RRH_STACK_HYBRID16, "rrh_ctx_set_option: bad stack kind");
+        c->stack_kind = value;
+        return RRH_OK;
+    }
+    if (option == RRH_OPT_XCD_REMAP) { c->xcd_remap = value != 0; return RRH_OK; }
+    set_error("rrh_ctx_set_option: unknown option");
+    return RRH_E_ARG;
+}
+
 int rrh_marker_record(rrh_ctx* c, int slot) {
     RRH_ARG(c && slot >= 0 && slot < 16, "rrh_marker_record: bad slot");
     RRH_HIP(hipEventRecord(c->markers[slot], c->stream));

@@ -206,7 +206,7 @@ RRH_D bool better(float t, uint32_t prim, float bt, uint32_t bp) { return t < bt
 #define RRH_LEAF_FIRST(c) ((uint32_t)(~(c)) >> 3)
 #define RRH_LEAF_COUNT(c) ((((uint32_t)(~(c))) & 7u) + 1u)
 #define RRH_STACK 64       // stack entries: any BVH the builder accepts (depth <= 63)
-#define RRH_LDS_STACK 16   // entries kept in LDS; deeper ones go to a private overflow array
+#define RRH_HYB_LDS 16     // hybrid stack: entries kept in LDS, deeper ones in a private overflow array
 
 struct Hit { float t; uint32_t prim; float b1, b2; };
 
@@ -226,13 +226,30 @@ struct StackPriv {
 // the top RRH_LDS_STACK entries in LDS, the rest (rarely reached) in private memory
 struct StackHybrid {
     int* p;
-    int ov[RRH_STACK - RRH_LDS_STACK];
+    int ov[RRH_STACK - RRH_HYB_LDS];
     RRH_D void put(int i, int v) {
-        if (i < RRH_LDS_STACK) p[i * 64] = v;
-        else ov[i - RRH_LDS_STACK] = v;
+        if (i < RRH_HYB_LDS) p[i * 64] = v;
+        else ov[i - RRH_HYB_LDS] = v;
     }
-    RRH_D int get(int i) const { return (i < RRH_LDS_STACK) ? p[i * 64] : ov[i - RRH_LDS_STACK]; }
+    RRH_D int get(int i) const { return (i < RRH_HYB_LDS) ? p[i * 64] : ov[i - RRH_HYB_LDS]; }
 };
+
+// stack kinds selectable per context (RRH_OPT_TRAVERSAL_STACK)
+template <int K> struct StackSel;
+template <> struct StackSel<RRH_STACK_PRIVATE> { using T = StackPriv; static constexpr int kLds = 1; };
+template <> struct StackSel<RRH_STACK_LDS32> { using T = StackLds; static constexpr int kLds = 32; };
+template <> struct StackSel<RRH_STACK_HYBRID16> { using T = StackHybrid; static constexpr int kLds = RRH_HYB_LDS; };
+
+// per-lane LDS base inside a block array of (blockDim/64) x entries x 64 ints
+RRH_D int* lds_stack_base(int* smem, int entries) {
+    return smem + (threadIdx.x >> 6) * (entries * 64) + (threadIdx.x & 63);
+}
+template <int K>
+RRH_D typename StackSel<K>::T make_stack(int* smem) {
+    typename StackSel<K>::T st;
+    if constexpr (K != RRH_STACK_PRIVATE) st.p = lds_stack_base(smem, StackSel<K>::kLds);
+    return st;
+}
 
 // closest hit over the scene BVH; the result is independent of traversal order
 // (ties on t resolve to the smaller flattened id).  COUNT = true is the
@@ -334,14 +351,11 @@ RRH_D bool trace_occluded(const DevScene& S, f3 o, f3 d, float tmin, float tmax)
     return trace_occluded_s(S, o, d, tmin, tmax, st);
 }
 
-// per-lane LDS stack base inside a block-wide array of (blockDim/64) x RRH_LDS_STACK x 64 ints
-RRH_D int* lds_stack_base(int* smem) { return smem + (threadIdx.x >> 6) * (RRH_LDS_STACK * 64) + (threadIdx.x & 63); }
-
 // XCD-aware block order: the hardware deals consecutive workgroups round-robin to the
 // 8 XCDs (each with its own L2); remapping gives every XCD one contiguous range of
 // work (an image band / queue range) so neighbouring rays share that XCD's L2
-RRH_D uint32_t xcd_block(uint32_t b, uint32_t nb) {
-    return (nb & 7u) ? b : (b & 7u) * (nb >> 3) + (b >> 3);
+RRH_D uint32_t xcd_block(uint32_t b, uint32_t nb, bool remap) {
+    return (!remap || (nb & 7u)) ? b : (b & 7u) * (nb >> 3) + (b >> 3);
 }
 
 // ---------------------------------------------------------------------------

@@ -54,21 +54,21 @@ __device__ __forceinline__ float4 pack(uint32_t sel, float wsum, float W, uint32
     return make_float4(__uint_as_float(sel), wsum, W, __uint_as_float(M));
 }
 
+template <int STK>
 __global__ void __launch_bounds__(64) k_di_initial(DevScene S, uint32_t frame_seed, uint32_t M, float4* __restrict__ gpos,
                                                    float4* __restrict__ gnrm, float4* __restrict__ res,
-                                                   uint32_t* __restrict__ res_uv) {
-    __shared__ int s_stack[RRH_LDS_STACK * 64];
+                                                   uint32_t* __restrict__ res_uv, bool remap) {
+    __shared__ int s_stack[StackSel<STK>::kLds * 64];
     const rrh_camera& cam = *S.cam;
     uint32_t W = cam.screen_width, H = cam.screen_height;
-    uint32_t tile = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    uint32_t tile = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y, remap);
     uint32_t px = (tile % gridDim.x) * 8 + (threadIdx.x & 7u), py = (tile / gridDim.x) * 8 + (threadIdx.x >> 3);
     if (px >= W || py >= H) return;
     size_t i = (size_t)py * W + px;
     uint32_t seed = make_seed(px, py, frame_seed);
     f3 d = primary_dir(cam, px, py, seed);
     f3 o = mk(cam.position[0], cam.position[1], cam.position[2]);
-    StackHybrid st;
-    st.p = lds_stack_base(s_stack);
+    auto st = make_stack<STK>(s_stack);
     Hit h = trace_closest_s<false>(S, o, d, 0.001f, 1e9f, st);
     res_uv[i] = 0u;
     if (h.prim == ~0u) {
@@ -112,10 +112,10 @@ __global__ void __launch_bounds__(64) k_di_initial(DevScene S, uint32_t frame_se
 __global__ void __launch_bounds__(64) k_di_temporal(DevScene S, uint32_t frame_seed, uint32_t M, const float4* __restrict__ gpos,
                                                     const float4* __restrict__ gnrm, float4* __restrict__ rc,
                                                     uint32_t* __restrict__ rc_uv, const float4* __restrict__ rp,
-                                                    const uint32_t* __restrict__ rp_uv) {
+                                                    const uint32_t* __restrict__ rp_uv, bool remap) {
     const rrh_camera& cam = *S.cam;
     uint32_t W = cam.screen_width, H = cam.screen_height;
-    uint32_t tile = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    uint32_t tile = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y, remap);
     uint32_t px = (tile % gridDim.x) * 8 + (threadIdx.x & 7u), py = (tile / gridDim.x) * 8 + (threadIdx.x >> 3);
     if (px >= W || py >= H) return;
     size_t i = (size_t)py * W + px;
@@ -150,14 +150,15 @@ __global__ void __launch_bounds__(64) k_di_temporal(DevScene S, uint32_t frame_s
     rc_uv[i] = cuv;
 }
 
+template <int STK>
 __global__ void __launch_bounds__(64) k_di_spatial_shade(DevScene S, rrh_di_params P, const float4* __restrict__ gpos,
                                                          const float4* __restrict__ gnrm, const float4* __restrict__ res,
                                                          const uint32_t* __restrict__ res_uv, float4* __restrict__ out,
-                                                         uint32_t* __restrict__ shadow_count) {
-    __shared__ int s_stack[RRH_LDS_STACK * 64];
+                                                         uint32_t* __restrict__ shadow_count, bool remap) {
+    __shared__ int s_stack[StackSel<STK>::kLds * 64];
     const rrh_camera& cam = *S.cam;
     uint32_t W = cam.screen_width, H = cam.screen_height;
-    uint32_t tile = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    uint32_t tile = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y, remap);
     uint32_t px = (tile % gridDim.x) * 8 + (threadIdx.x & 7u), py = (tile / gridDim.x) * 8 + (threadIdx.x >> 3);
     bool traced = false;
     if (px < W && py < H) {
@@ -233,8 +234,7 @@ __global__ void __launch_bounds__(64) k_di_spatial_shade(DevScene S, rrh_di_para
                     float ndl = fmaxf(dot(n, l), 0.0f);
                     if (ndl > 0.0f) {
                         traced = true;
-                        StackHybrid st;
-                        st.p = lds_stack_base(s_stack);
+                        auto st = make_stack<STK>(s_stack);
                         if (!trace_occluded_s(S, add(hp, scale(n, 1e-4f)), l, 0.001f, dist - 0.01f, st)) {
                             f3 wo = normalize(sub(ld3(cam.position), hp));
                             f3 f = mat_eval(m, n, wo, l);
@@ -271,9 +271,20 @@ int rrh_di_initial(rrh_ctx* c, rrh_scene* s, const rrh_di_params* p, rrh_image* 
     RRH_ARG(gpos->fmt == RRH_FMT_RGBA32F && gnrm->fmt == RRH_FMT_RGBA32F && res->fmt == RRH_FMT_RGBA32F &&
                 res_uv->fmt == RRH_FMT_R32U, "rrh_di_initial: formats RGBA32F x3 + R32U");
     RRH_ARG(p->num_initial_candidates <= 4096, "rrh_di_initial: num_initial_candidates above 4096");
-    hipLaunchKernelGGL(k_di_initial, dim3((W + 7) / 8, (H + 7) / 8), dim3(64), 0, c->stream, s->dev(),
-                           p->frame_seed, p->num_initial_candidates, (float4*)gpos->p, (float4*)gnrm->p, (float4*)res->p,
-                           (uint32_t*)res_uv->p);
+    const int sk = stack_for(c, s->depth);
+    dim3 grid((W + 7) / 8, (H + 7) / 8);
+    if (sk == RRH_STACK_LDS32)
+        hipLaunchKernelGGL(k_di_initial<RRH_STACK_LDS32>, grid, dim3(64), 0, c->stream, s->dev(), p->frame_seed,
+                           p->num_initial_candidates, (float4*)gpos->p, (float4*)gnrm->p, (float4*)res->p,
+                           (uint32_t*)res_uv->p, c->xcd_remap);
+    else if (sk == RRH_STACK_HYBRID16)
+        hipLaunchKernelGGL(k_di_initial<RRH_STACK_HYBRID16>, grid, dim3(64), 0, c->stream, s->dev(), p->frame_seed,
+                           p->num_initial_candidates, (float4*)gpos->p, (float4*)gnrm->p, (float4*)res->p,
+                           (uint32_t*)res_uv->p, c->xcd_remap);
+    else
+        hipLaunchKernelGGL(k_di_initial<RRH_STACK_PRIVATE>, grid, dim3(64), 0, c->stream, s->dev(), p->frame_seed,
+                           p->num_initial_candidates, (float4*)gpos->p, (float4*)gnrm->p, (float4*)res->p,
+                           (uint32_t*)res_uv->p, c->xcd_remap);
     RRH_LAUNCH_CHECK();
     return RRH_OK;
 }
@@ -289,7 +300,7 @@ int rrh_di_temporal(rrh_ctx* c, rrh_scene* s, const rrh_di_params* p, const rrh_
                 same_extent(gpos, rp_uv), "rrh_di_temporal: extent mismatch");
     hipLaunchKernelGGL(k_di_temporal, dim3((W + 7) / 8, (H + 7) / 8), dim3(64), 0, c->stream, s->dev(), p->frame_seed,
                        p->num_initial_candidates, (const float4*)gpos->p, (const float4*)gnrm->p, (float4*)rc->p,
-                       (uint32_t*)rc_uv->p, (const float4*)rp->p, (const uint32_t*)rp_uv->p);
+                       (uint32_t*)rc_uv->p, (const float4*)rp->p, (const uint32_t*)rp_uv->p, c->xcd_remap);
     RRH_LAUNCH_CHECK();
     return RRH_OK;
 }
@@ -306,9 +317,20 @@ int rrh_di_spatial_shade(rrh_ctx* c, rrh_scene* s, const rrh_di_params* p, const
     int rc = c->di_counter.ensure(16);
     if (rc) return rc;
     RRH_HIP(hipMemsetAsync(c->di_counter.p, 0, 4, c->stream));
-    hipLaunchKernelGGL(k_di_spatial_shade, dim3((W + 7) / 8, (H + 7) / 8), dim3(64), 0, c->stream, s->dev(), *p,
+    const int sk = stack_for(c, s->depth);
+    dim3 grid((W + 7) / 8, (H + 7) / 8);
+    if (sk == RRH_STACK_LDS32)
+        hipLaunchKernelGGL(k_di_spatial_shade<RRH_STACK_LDS32>, grid, dim3(64), 0, c->stream, s->dev(), *p,
                            (const float4*)gpos->p, (const float4*)gnrm->p, (const float4*)res->p,
-                           (const uint32_t*)res_uv->p, (float4*)out->p, (uint32_t*)c->di_counter.p);
+                           (const uint32_t*)res_uv->p, (float4*)out->p, (uint32_t*)c->di_counter.p, c->xcd_remap);
+    else if (sk == RRH_STACK_HYBRID16)
+        hipLaunchKernelGGL(k_di_spatial_shade<RRH_STACK_HYBRID16>, grid, dim3(64), 0, c->stream, s->dev(), *p,
+                           (const float4*)gpos->p, (const float4*)gnrm->p, (const float4*)res->p,
+                           (const uint32_t*)res_uv->p, (float4*)out->p, (uint32_t*)c->di_counter.p, c->xcd_remap);
+    else
+        hipLaunchKernelGGL(k_di_spatial_shade<RRH_STACK_PRIVATE>, grid, dim3(64), 0, c->stream, s->dev(), *p,
+                           (const float4*)gpos->p, (const float4*)gnrm->p, (const float4*)res->p,
+                           (const uint32_t*)res_uv->p, (float4*)out->p, (uint32_t*)c->di_counter.p, c->xcd_remap);
     RRH_LAUNCH_CHECK();
     if (!c->totals.p) {
         if ((rc = c->totals.ensure(32))) return rc;

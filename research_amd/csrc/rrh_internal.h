@@ -87,6 +87,8 @@ struct rrh_ctx {
     rrh::DevBuf totals;           // uint64 x 3: primary, extension, shadow (device-side sums)
     rrh::DevBuf fetch;            // uint64 x 3: node fetches, tri fetches, rays (instrumented traversal)
     bool profiling = false;
+    int stack_kind = RRH_STACK_LDS32;
+    bool xcd_remap = false;
     std::vector<hipEvent_t> prof_events;   // pairs around each traversal launch
     uint32_t prof_used = 0;
     rrh::DevBuf scratch;          // small general-purpose scratch
@@ -114,4 +116,9 @@ struct rrh_image {
 
 namespace rrh {
 size_t format_bytes(int fmt);
+// stack kind a launch uses for a scene of the given BVH depth
+inline int stack_for(const rrh_ctx* c, uint32_t depth) {
+    if (c->stack_kind == RRH_STACK_LDS32 && depth > 32) return RRH_STACK_PRIVATE;
+    return c->stack_kind;
+}
 }

@@ -63,15 +63,15 @@ __device__ __forceinline__ uint32_t block_push(uint32_t* counter, bool pred, uin
     return slot;
 }
 
-template <bool COUNT>
+template <bool COUNT, int STK>
 __global__ void __launch_bounds__(kBlock) k_intersect(DevScene S, Q in, const uint32_t* __restrict__ cnt, uint32_t b,
-                                                      float4* __restrict__ hits, unsigned long long* __restrict__ fetch) {
-    __shared__ int s_stack[(kBlock / 64) * RRH_LDS_STACK * 64];
+                                                      float4* __restrict__ hits, unsigned long long* __restrict__ fetch,
+                                                      bool remap) {
+    __shared__ int s_stack[(kBlock / 64) * StackSel<STK>::kLds * 64];
     const uint32_t n = cnt[b];
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, remap);
     uint32_t nn = 0, nt = 0, nr = 0;
-    StackHybrid st;
-    st.p = lds_stack_base(s_stack);
+    auto st = make_stack<STK>(s_stack);
     for (uint32_t i = blk * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float4 o = in.o[i], d = in.d[i];
         Hit h = trace_closest_s<COUNT>(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, 1e9f, st, &nn, &nt);
@@ -157,6 +157,7 @@ __device__ __forceinline__ NeeOut nee_candidate(const DevScene& S, f3 hit, f3 n,
 
 struct ShadeArgs {
     uint32_t b, max_bounces, rr_start, guide_mode;
+    bool remap;
     float guide_alpha;
     GuideDev guide;
     VertexRec rec;
@@ -171,7 +172,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, Q in, Q out, const
     const uint32_t stride = gridDim.x * blockDim.x;
     const bool last = (b + 1u >= A.max_bounces);
     __shared__ uint32_t s_cnt[kBlock / 64 + 1];
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, A.remap);
     for (uint32_t base = blk * blockDim.x; base < n; base += stride) {
         uint32_t i = base + threadIdx.x;
         bool valid = i < n;
@@ -253,15 +254,15 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, Q in, Q out, const
     }
 }
 
+template <int STK>
 __global__ void __launch_bounds__(kBlock) k_shadow(DevScene S, const uint32_t* __restrict__ cnt, uint32_t b,
                                                    const float4* __restrict__ sh_o, const float4* __restrict__ sh_d,
                                                    const float4* __restrict__ sh_c, float4* __restrict__ rad,
-                                                   VertexRec rec) {
-    __shared__ int s_stack[(kBlock / 64) * RRH_LDS_STACK * 64];
+                                                   VertexRec rec, bool remap) {
+    __shared__ int s_stack[(kBlock / 64) * StackSel<STK>::kLds * 64];
     const uint32_t n = cnt[64 + b];
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-    StackHybrid st;
-    st.p = lds_stack_base(s_stack);
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, remap);
+    auto st = make_stack<STK>(s_stack);
     for (uint32_t i = blk * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float4 o = sh_o[i], d = sh_d[i];
         if (!trace_occluded_s(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, o.w, st)) {
@@ -273,6 +274,26 @@ __global__ void __launch_bounds__(kBlock) k_shadow(DevScene S, const uint32_t* _
             if (rec.v) guide_record_nee_visible(rec, pixel, b);
         }
     }
+}
+
+template <int STK>
+int launch_trace(rrh_ctx* c, const DevScene& S, Q& qi, uint32_t* cnt, uint32_t b, uint32_t g, bool count) {
+    unsigned long long* fc = (unsigned long long*)c->fetch.p;
+    float4* hp = (float4*)c->hits.p;
+    if (count)
+        hipLaunchKernelGGL((k_intersect<true, STK>), dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, fc, c->xcd_remap);
+    else
+        hipLaunchKernelGGL((k_intersect<false, STK>), dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, fc, c->xcd_remap);
+    RRH_LAUNCH_CHECK();
+    return RRH_OK;
+}
+template <int STK>
+int launch_shadow(rrh_ctx* c, const DevScene& S, uint32_t* cnt, uint32_t b, uint32_t g, float4* rad, const VertexRec& rec) {
+    hipLaunchKernelGGL(k_shadow<STK>, dim3(g), dim3(kBlock), 0, c->stream, S, (const uint32_t*)cnt, b,
+                       (const float4*)c->sh_o.p, (const float4*)c->sh_d.p, (const float4*)c->sh_c.p, rad, rec,
+                       c->xcd_remap);
+    RRH_LAUNCH_CHECK();
+    return RRH_OK;
 }
 
 __global__ void k_accumulate(const float4* __restrict__ rad, float4* __restrict__ acc, uint32_t n,
@@ -342,6 +363,7 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
     A.rr_start = p->rr_start;
     A.guide_mode = p->guide_mode;
     A.guide_alpha = p->guide_alpha;
+    A.remap = c->xcd_remap;
     if ((rc = guide_prepare(c, s, p, grid, mlp, A.guide, A.rec))) return rc;
     DevScene S = s->dev();
     uint32_t* cnt = (uint32_t*)c->counters.p;
@@ -366,12 +388,12 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
             }
             RRH_HIP(hipEventRecord(c->prof_events[c->prof_used], c->stream));
         }
-        unsigned long long* fc = (unsigned long long*)c->fetch.p;
-        float4* hp = (float4*)c->hits.p;
-        if (p->flags & RRH_PT_COUNT_FETCHES)
-            hipLaunchKernelGGL(k_intersect<true>, dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, fc);
-        else
-            hipLaunchKernelGGL(k_intersect<false>, dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, fc);
+        const int sk = stack_for(c, s->depth);
+        const bool cf = (p->flags & RRH_PT_COUNT_FETCHES) != 0;
+        rc = (sk == RRH_STACK_LDS32) ? launch_trace<RRH_STACK_LDS32>(c, S, qi, cnt, b, g, cf)
+           : (sk == RRH_STACK_HYBRID16) ? launch_trace<RRH_STACK_HYBRID16>(c, S, qi, cnt, b, g, cf)
+           : launch_trace<RRH_STACK_PRIVATE>(c, S, qi, cnt, b, g, cf);
+        if (rc) return rc;
         RRH_LAUNCH_CHECK();
         if (c->profiling) {
             RRH_HIP(hipEventRecord(c->prof_events[c->prof_used + 1], c->stream));
@@ -381,9 +403,10 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
         hipLaunchKernelGGL(k_shade, dim3(g), dim3(kBlock), 0, c->stream, S, qi, qo, (const float4*)c->hits.p, cnt,
                            (float4*)c->sh_o.p, (float4*)c->sh_d.p, (float4*)c->sh_c.p, rad, A);
         RRH_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_shadow, dim3(g), dim3(kBlock), 0, c->stream, S, (const uint32_t*)cnt, b,
-                           (const float4*)c->sh_o.p, (const float4*)c->sh_d.p, (const float4*)c->sh_c.p, rad, A.rec);
-        RRH_LAUNCH_CHECK();
+        rc = (sk == RRH_STACK_LDS32) ? launch_shadow<RRH_STACK_LDS32>(c, S, cnt, b, g, rad, A.rec)
+           : (sk == RRH_STACK_HYBRID16) ? launch_shadow<RRH_STACK_HYBRID16>(c, S, cnt, b, g, rad, A.rec)
+           : launch_shadow<RRH_STACK_PRIVATE>(c, S, cnt, b, g, rad, A.rec);
+        if (rc) return rc;
     }
     hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(64), 0, c->stream, (const uint32_t*)cnt, p->max_bounces,
                        (unsigned long long*)c->totals.p);
