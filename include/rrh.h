@@ -285,6 +285,38 @@ int rrh_grid_download(rrh_ctx* ctx, const rrh_grid* g, void* fitted_or_lobes, si
                       void* accum_or_stats, size_t bytes1);
 int rrh_grid_upload(rrh_ctx* ctx, rrh_grid* g, const void* fitted_or_lobes, size_t bytes0);
 
+/* ---- learned guide (new; the reference guides with the single-lobe grid only).
+ *      x(78) = 12-frequency sin/cos encoding of the normalised position (72) ++ n ++ wo;
+ *      78 -> 64 -> 64 -> 64 -> 64 -> 40 ReLU MLP; 8 vMF lobes: mu = v/|v|,
+ *      kappa = 0.1 + softplus(d) (<= 1000), pi = softmax(e).  BF16 weights and
+ *      activations on MFMA with FP32 accumulation, FP32 master weights, Adam, dynamic
+ *      loss scaling.  Parameters: W0 (64x78), b0, W1, b1, W2, b2, W3, b3, W4 (40x64), b4. */
+#define RRH_MLP_PARAMS 20136
+typedef struct {
+    float bounds_min[3], bounds_max[3];   /* position normalisation box */
+    float lr, beta1, beta2, eps;          /* Adam (1e-3, 0.9, 0.999, 1e-8) */
+    uint32_t seed;                        /* weight initialisation */
+    float loss_scale;                     /* initial loss scale (e.g. 1024) */
+} rrh_mlp_params;
+int rrh_mlp_create(rrh_ctx* ctx, const rrh_mlp_params* p, rrh_mlp** out);
+int rrh_mlp_destroy(rrh_mlp* m);
+int rrh_mlp_set_weights(rrh_ctx* ctx, rrh_mlp* m, const float* params, uint32_t n);
+int rrh_mlp_get_weights(rrh_ctx* ctx, const rrh_mlp* m, float* params, uint32_t n);
+/* lobes: n x 8 x (mu3, kappa, pi) */
+int rrh_mlp_infer(rrh_ctx* ctx, rrh_mlp* m, const float* pos, const float* nrm, const float* wo, uint32_t n,
+                  float* lobes);
+/* mean weighted negative log-likelihood and its gradient (RRH_MLP_PARAMS floats), no update */
+int rrh_mlp_loss_grad(rrh_ctx* ctx, rrh_mlp* m, const float* pos, const float* nrm, const float* wo,
+                      const float* dirs, const float* w, uint32_t n, float* loss, float* grad);
+/* one training step (forward, backward, Adam) on host samples / on the samples recorded by
+ * the last rrh_pt_render_guided with record_vertices (at most `batch` of them) */
+int rrh_mlp_train_step(rrh_ctx* ctx, rrh_mlp* m, const float* pos, const float* nrm, const float* wo,
+                       const float* dirs, const float* w, uint32_t n, float* loss);
+int rrh_mlp_train_from_vertices(rrh_ctx* ctx, rrh_mlp* m, uint32_t batch, float* loss);
+/* one Adam step with a given (unscaled) gradient, for checking the optimiser */
+int rrh_mlp_adam_apply(rrh_ctx* ctx, rrh_mlp* m, const float* grad, uint32_t n);
+int rrh_mlp_state(rrh_ctx* ctx, const rrh_mlp* m, uint32_t* step, float* loss_scale);
+
 #ifdef __cplusplus
 }
 #endif

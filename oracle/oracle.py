@@ -130,7 +130,7 @@ def fa(v) -> C.Array:
 
 def p(a: np.ndarray) -> C.c_void_p:
     assert a.flags["C_CONTIGUOUS"]
-    return C.c_void_p(a.ctypes.data)
+    return a.ctypes.data_as(C.c_void_p)  # the pointer object keeps `a` alive (temporaries too)
 
 
 def camera_from_bytes(b: bytes) -> Camera:

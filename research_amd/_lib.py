@@ -92,6 +92,14 @@ class GridParams(C.Structure):
                 ("bounds_max", C.c_float * 3), ("decay", C.c_float), ("min_weight", C.c_float)]
 
 
+class MlpParams(C.Structure):
+    _fields_ = [("bounds_min", C.c_float * 3), ("bounds_max", C.c_float * 3), ("lr", C.c_float),
+                ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint32),
+                ("loss_scale", C.c_float)]
+
+
+MLP_PARAMS = 20136
+
 P = C.c_void_p
 PP = C.POINTER(C.c_void_p)
 u32, i32, f32, u64, sz = C.c_uint32, C.c_int, C.c_float, C.c_uint64, C.c_size_t
@@ -151,6 +159,16 @@ SIGNATURES = {
     "rrh_grid_em_from_vertices": [P, P],
     "rrh_grid_download": [P, P, P, sz, P, sz],
     "rrh_grid_upload": [P, P, P, sz],
+    "rrh_mlp_create": [P, C.POINTER(MlpParams), PP],
+    "rrh_mlp_destroy": [P],
+    "rrh_mlp_set_weights": [P, P, P, u32],
+    "rrh_mlp_get_weights": [P, P, P, u32],
+    "rrh_mlp_infer": [P, P, P, P, P, u32, P],
+    "rrh_mlp_loss_grad": [P, P, P, P, P, P, P, u32, P, P],
+    "rrh_mlp_train_step": [P, P, P, P, P, P, P, u32, P],
+    "rrh_mlp_train_from_vertices": [P, P, u32, P],
+    "rrh_mlp_adam_apply": [P, P, P, u32],
+    "rrh_mlp_state": [P, P, P, P],
 }
 
 
@@ -194,7 +212,7 @@ def ptr(a: np.ndarray | None) -> C.c_void_p | None:
     if a is None:
         return None
     assert a.flags["C_CONTIGUOUS"], "arrays passed to librrh must be C-contiguous"
-    return C.c_void_p(a.ctypes.data)
+    return a.ctypes.data_as(C.c_void_p)  # the pointer object keeps `a` alive (temporaries too)
 
 
 def f3(v) -> C.Array:

@@ -264,3 +264,79 @@ class Grid:
             self.close()
         except Exception:
             pass
+
+
+class Mlp:
+    """Learned guide: 78 -> 64x4 -> 40 BF16 MLP producing an 8-lobe vMF mixture."""
+
+    NPARAM = L.MLP_PARAMS
+
+    def __init__(self, ctx: Context, bounds_min, bounds_max, lr: float = 1e-3, beta1: float = 0.9,
+                 beta2: float = 0.999, eps: float = 1e-8, seed: int = 1, loss_scale: float = 1024.0):
+        self.ctx = ctx
+        self.p = L.MlpParams(L.f3(bounds_min), L.f3(bounds_max), lr, beta1, beta2, eps, seed, loss_scale)
+        h = C.c_void_p()
+        L.call("rrh_mlp_create", ctx.h, C.byref(self.p), C.byref(h))
+        self.h = h
+
+    def get_weights(self) -> np.ndarray:
+        w = np.zeros(self.NPARAM, np.float32)
+        L.call("rrh_mlp_get_weights", self.ctx.h, self.h, L.ptr(w), self.NPARAM)
+        return w
+
+    def set_weights(self, w: np.ndarray) -> None:
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        L.call("rrh_mlp_set_weights", self.ctx.h, self.h, L.ptr(w), len(w))
+
+    @staticmethod
+    def _v3(a) -> np.ndarray:
+        return np.ascontiguousarray(a, dtype=np.float32).reshape(-1, 3)
+
+    def infer(self, pos, nrm, wo) -> np.ndarray:
+        pos, nrm, wo = self._v3(pos), self._v3(nrm), self._v3(wo)
+        out = np.zeros((len(pos), 8, 5), np.float32)
+        L.call("rrh_mlp_infer", self.ctx.h, self.h, L.ptr(pos), L.ptr(nrm), L.ptr(wo), len(pos), L.ptr(out))
+        return out
+
+    def loss_grad(self, pos, nrm, wo, dirs, w):
+        pos, nrm, wo, dirs = self._v3(pos), self._v3(nrm), self._v3(wo), self._v3(dirs)
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        loss, g = C.c_float(), np.zeros(self.NPARAM, np.float32)
+        L.call("rrh_mlp_loss_grad", self.ctx.h, self.h, L.ptr(pos), L.ptr(nrm), L.ptr(wo), L.ptr(dirs), L.ptr(w),
+               len(w), C.byref(loss), L.ptr(g))
+        return loss.value, g
+
+    def train_step(self, pos, nrm, wo, dirs, w) -> float:
+        pos, nrm, wo, dirs = self._v3(pos), self._v3(nrm), self._v3(wo), self._v3(dirs)
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        loss = C.c_float()
+        L.call("rrh_mlp_train_step", self.ctx.h, self.h, L.ptr(pos), L.ptr(nrm), L.ptr(wo), L.ptr(dirs), L.ptr(w),
+               len(w), C.byref(loss))
+        return loss.value
+
+    def train_from_vertices(self, batch: int) -> float:
+        loss = C.c_float()
+        L.call("rrh_mlp_train_from_vertices", self.ctx.h, self.h, batch, C.byref(loss))
+        return loss.value
+
+    def adam_apply(self, grad: np.ndarray) -> None:
+        grad = np.ascontiguousarray(grad, dtype=np.float32)
+        L.call("rrh_mlp_adam_apply", self.ctx.h, self.h, L.ptr(grad), len(grad))
+
+    def state(self) -> tuple[int, float]:
+        st, sc = C.c_uint32(), C.c_float()
+        L.call("rrh_mlp_state", self.ctx.h, self.h, C.byref(st), C.byref(sc))
+        return st.value, sc.value
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            L.load().rrh_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        if _EXITING:
+            return
+        try:
+            self.close()
+        except Exception:
+            pass

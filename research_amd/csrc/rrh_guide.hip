@@ -277,7 +277,6 @@ namespace rrh {
 
 int guide_prepare(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* grid, rrh_mlp* mlp, GuideDev& G,
                   VertexRec& R) {
-    (void)s; (void)mlp;
     G = GuideDev();
     R = VertexRec();
     if (p->guide_mode == RRH_GUIDE_VMM) {
@@ -292,8 +291,12 @@ int guide_prepare(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* gr
         G.K = grid->p.lobes;
         for (int k = 0; k < 3; ++k) { G.bmin[k] = grid->p.bounds_min[k]; G.bext[k] = grid->p.bounds_max[k] - grid->p.bounds_min[k]; }
     } else if (p->guide_mode == RRH_GUIDE_MLP) {
-        set_error("guided render: the learned guide is not available in this build");
-        return RRH_E_STATE;
+        RRH_ARG(mlp, "guided render: RRH_GUIDE_MLP needs an rrh_mlp");
+        uint32_t npix = s->cam_host.screen_width * s->cam_host.screen_height;
+        int rc = c->scratch.ensure((size_t)npix * kMaxLobes * 5 * 4);
+        if (rc) return rc;
+        G.mlp_lobes = (const float*)c->scratch.p;
+        G.K = kMaxLobes;
     }
     if (p->record_vertices) {
         uint32_t npix = s->cam_host.screen_width * s->cam_host.screen_height;

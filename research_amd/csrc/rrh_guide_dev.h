@@ -51,10 +51,12 @@ RRH_D float mixture_pdf(const float* lc, uint32_t K, f3 wi) {
 
 // one-sample MIS: with probability alpha sample the mixture, else the BSDF;
 // pdf = alpha * p_guide + (1 - alpha) * p_bsdf; returns f(wo, wi)
-RRH_D f3 guided_sample(const GuideDev& G, uint32_t mode, float alpha, const DevScene& S, f3 pos, f3 n, f3 wo,
-                       const Mat& m, uint32_t& seed, f3& wi, float& pdf) {
+// slot: the ray's queue index (the learned guide's lobes are inferred per queue slot)
+RRH_D f3 guided_sample(const GuideDev& G, uint32_t mode, float alpha, const DevScene& S, uint32_t slot, f3 pos, f3 n,
+                       f3 wo, const Mat& m, uint32_t& seed, f3& wi, float& pdf) {
     const float* lc = nullptr;
     if (mode == RRH_GUIDE_VMM && G.lobes) lc = G.lobes + (size_t)vmm_cell(G, pos) * G.K * 5;
+    else if (mode == RRH_GUIDE_MLP && G.mlp_lobes) lc = G.mlp_lobes + (size_t)slot * G.K * 5;
     float tot = 0.0f;
     if (lc)
         for (uint32_t k = 0; k < G.K; ++k) tot += fmaxf(lc[k * 5 + 4], 0.0f);
@@ -104,5 +106,8 @@ int guide_prepare(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* gr
                   VertexRec& R);
 int guide_finish(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* grid, rrh_mlp* mlp, const VertexRec& R,
                  const float4* rad);
+// learned guide: lobes (8 x 5 floats) for every hit of queue b (rrh_mlp.hip)
+int mlp_infer_queue(rrh_ctx* c, rrh_mlp* m, const DevScene& S, const float4* qo, const float4* qd, const float4* hits,
+                    const uint32_t* cnt, uint32_t b, uint32_t cap, float* lobes);
 
 }  // namespace rrh

@@ -219,7 +219,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, Q in, Q out, const
                     }
                     f3 wi; float pdf; f3 f;
                     if (A.guide_mode != RRH_GUIDE_NONE) {
-                        f = guided_sample(A.guide, A.guide_mode, A.guide_alpha, S, hi.pos, nn, wo, m, seed, wi, pdf);
+                        f = guided_sample(A.guide, A.guide_mode, A.guide_alpha, S, i, hi.pos, nn, wo, m, seed, wi, pdf);
                     } else {
                         float ux = rnd(seed);
                         float uy = rnd(seed);
@@ -400,6 +400,11 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
             c->prof_used += 2;
         }
         A.b = b;
+        if (A.guide.mlp_lobes) {
+            rc = mlp_infer_queue(c, mlp, S, qi.o, qi.d, (const float4*)c->hits.p, cnt, b, npix,
+                                 const_cast<float*>(A.guide.mlp_lobes));
+            if (rc) return rc;
+        }
         hipLaunchKernelGGL(k_shade, dim3(g), dim3(kBlock), 0, c->stream, S, qi, qo, (const float4*)c->hits.p, cnt,
                            (float4*)c->sh_o.p, (float4*)c->sh_d.p, (float4*)c->sh_c.p, rad, A);
         RRH_LAUNCH_CHECK();

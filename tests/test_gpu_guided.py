@@ -1,0 +1,87 @@
+"""Guided path tracing (8-lobe EM grid and learned guide): the one-sample MIS estimator
+must stay unbiased whatever the guide has learned, so the guided mean image has to agree
+with the unguided one within their Monte Carlo error.  The guides are trained online
+from the tracer's own recorded vertices."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from research_amd import _lib as L
+from research_amd import core
+
+pytestmark = pytest.mark.gpu
+
+W = H = 32
+BOUNCES = 5
+
+
+def _scene(ctx):
+    sa = core.generate_scene(L.GEN_CORNELL_RANDOM, 1, 1.0)
+    gs = core.Scene(ctx, sa)
+    gs.set_camera(core.camera_look_at(sa.eye, sa.center, sa.fov, W, H))
+    return gs
+
+
+def _render(ctx, gs, img, seed, mode=L.GUIDE_NONE, alpha=0.0, record=0, grid=None, mlp=None):
+    p = L.PTParams(BOUNCES, seed, 3, mode, alpha, record)
+    L.call("rrh_pt_render_guided", ctx.h, gs.h, C.byref(p), grid.h if grid else None, mlp.h if mlp else None, img.h)
+    return img.download().reshape(H, W, 4)[..., :3].astype(np.float64)
+
+
+def _frames(ctx, gs, n, seed0, **kw):
+    img = core.Image(ctx, W, H)
+    blocks = []
+    for k in range(n):
+        r = _render(ctx, gs, img, seed0 + k, **kw)
+        assert np.isfinite(r).all()
+        blocks.append(r.reshape(4, 8, 4, 8, 3).mean(axis=(1, 3, 4)).ravel())  # 16 block means
+    return np.array(blocks)
+
+
+def _assert_same_mean(a, b, what):
+    ma, mb = a.mean(0), b.mean(0)
+    se = np.sqrt(a.var(0, ddof=1) / len(a) + b.var(0, ddof=1) / len(b))
+    z = np.abs(ma - mb) / np.maximum(se, 1e-12)
+    assert z.max() < 5.0, (what, z.max(), ma, mb)
+    tot_a, tot_b = a.mean(1), b.mean(1)
+    zt = abs(tot_a.mean() - tot_b.mean()) / np.sqrt(tot_a.var(ddof=1) / len(a) + tot_b.var(ddof=1) / len(b))
+    assert zt < 5.0, (what, zt)
+
+
+@pytest.fixture(scope="module")
+def unguided(ctx):
+    gs = _scene(ctx)
+    return gs, _frames(ctx, gs, 256, 1000)
+
+
+def test_vmm_guided_render_is_unbiased(ctx, unguided):
+    gs, ref = unguided
+    info = gs.info()
+    prm = L.GridParams(L.GRID_VMM, 0, 0, 0, 0, 512.0, 4, 8, L.f3(info["bounds_min"]), L.f3(info["bounds_max"]),
+                       0.0, 0.0)
+    g = core.Grid(ctx, prm)
+    img = core.Image(ctx, W, H)
+    for k in range(8):
+        _render(ctx, gs, img, 50 + k, mode=L.GUIDE_VMM, alpha=0.5 if k else 0.0, record=20000, grid=g)
+        L.call("rrh_grid_em_from_vertices", ctx.h, g.h)
+    lobes, _ = g.download()
+    assert np.isfinite(lobes).all() and (lobes[..., 3] > 0).any()
+    got = _frames(ctx, gs, 256, 5000, mode=L.GUIDE_VMM, alpha=0.5, grid=g)
+    _assert_same_mean(got, ref, "vmm")
+
+
+def test_mlp_guided_render_is_unbiased_and_training_lowers_the_loss(ctx, unguided):
+    gs, ref = unguided
+    info = gs.info()
+    m = core.Mlp(ctx, info["bounds_min"], info["bounds_max"], lr=2e-3, seed=1)
+    img = core.Image(ctx, W, H)
+    losses = []
+    for k in range(60):
+        _render(ctx, gs, img, 70 + k, mode=L.GUIDE_MLP, alpha=0.5 if k >= 20 else 0.0, record=4096, mlp=m)
+        losses.append(m.train_from_vertices(4096))
+    assert np.isfinite(losses).all()
+    assert np.mean(losses[-10:]) < np.mean(losses[:5]), (losses[:5], losses[-10:])
+    assert m.state()[0] >= 55
+    got = _frames(ctx, gs, 256, 9000, mode=L.GUIDE_MLP, alpha=0.5, mlp=m)
+    _assert_same_mean(got, ref, "mlp")

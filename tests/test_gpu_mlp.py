@@ -1,0 +1,182 @@
+"""GPU learned guide (BF16 MFMA forward / backward, Adam, loss scaling) against the
+FP64 oracle.
+
+Tolerances: the GPU rounds inputs, weights and activations to BF16 (8-bit mantissa,
+relative step 2^-8) and accumulates in FP32, so it agrees with the FP64 oracle to about
+1%, not bit for bit.  Stated bars: mixture pdf relative error median <= 1e-2 and 99th
+percentile <= 5e-2; loss relative error <= 1e-2; batch gradient cosine similarity >=
+0.999 and relative L2 error <= 3e-2 over the whole vector, <= 0.2 per layer (the input
+layer's gradient carries BF16 rounding of four backward deltas); against a numpy model
+of the kernels' BF16 rounding points (tests/mlp_bf16.py) <= 2e-2 per layer; Adam in FP32
+within 2e-6 absolute of the FP64 update."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from research_amd import core
+from tests.mlp_bf16 import bf16
+from tests.mlp_bf16 import loss_grad as bf16_loss_grad
+from tests.test_oracle import _vmf_draw
+
+pytestmark = pytest.mark.gpu
+
+BMIN, BMAX = (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)
+LAYERS = [(78, 64), (64, 64), (64, 64), (64, 64), (64, 40)]
+
+
+def _unit(a):
+    return (a / np.linalg.norm(a, axis=1, keepdims=True)).astype(np.float32)
+
+
+def _samples(n, rng):
+    pos = rng.uniform(-0.95, 0.95, (n, 3)).astype(np.float32)
+    nrm = _unit(rng.normal(size=(n, 3)))
+    wo = _unit(rng.normal(size=(n, 3)))
+    dirs = _unit(rng.normal(size=(n, 3)))
+    w = rng.uniform(0.05, 3.0, n).astype(np.float32)
+    return pos, nrm, wo, dirs, w
+
+
+def _encode(pos, nrm, wo):
+    lib = O.lib()
+    x = np.zeros((len(pos), 78))
+    row = np.zeros(78)
+    for i in range(len(pos)):
+        lib.orc_mlp_encode(O.fa(pos[i]), O.fa(BMIN), O.fa(BMAX), O.fa(nrm[i]), O.fa(wo[i]), O.p(row))
+        x[i] = row
+    return x
+
+
+def _oracle_lobes(params, x):
+    lib = O.lib()
+    P = params.astype(np.float64)
+    acts, out, lob = np.zeros(256), np.zeros(40), np.zeros(40)
+    res = np.zeros((len(x), 40))
+    for i in range(len(x)):
+        xi = np.ascontiguousarray(x[i])
+        lib.orc_mlp_forward(O.p(P), O.p(xi), O.p(acts), O.p(out))
+        lib.orc_mlp_head(O.p(out), O.p(lob))
+        res[i] = lob
+    return res.reshape(-1, 8, 5)
+
+
+def _mix_pdf(lobes, d):
+    kap = lobes[..., 3].astype(np.float64)
+    c = np.einsum("nkj,nj->nk", lobes[..., :3].astype(np.float64), d.astype(np.float64))
+    f = kap * np.exp(kap * (c - 1)) / (2 * math.pi * (1 - np.exp(-2 * kap)))
+    return (lobes[..., 4] * f).sum(1)
+
+
+def _layer_slices():
+    off, out = 0, []
+    for i, o in LAYERS:
+        out.append(slice(off, off + i * o + o))
+        off += i * o + o
+    return out
+
+
+def test_infer_matches_oracle(ctx):
+    rng = np.random.default_rng(11)
+    m = core.Mlp(ctx, BMIN, BMAX, seed=5)
+    n = 1500  # not a multiple of the 64-sample chunk
+    pos, nrm, wo, dirs, _ = _samples(n, rng)
+    lob = m.infer(pos, nrm, wo)
+    ref = _oracle_lobes(m.get_weights(), _encode(pos, nrm, wo))
+    assert np.isfinite(lob).all()
+    assert np.allclose(lob[..., 4].sum(1), 1, atol=1e-5)
+    assert np.allclose(np.linalg.norm(lob[..., :3], axis=2), 1, atol=1e-5)
+    assert np.abs(lob[..., 4] - ref[..., 4]).max() < 0.02
+    rel = np.abs(_mix_pdf(lob, dirs) - _mix_pdf(ref, dirs)) / _mix_pdf(ref, dirs)
+    assert np.median(rel) <= 1e-2 and np.quantile(rel, 0.99) <= 5e-2, (np.median(rel), np.quantile(rel, 0.99))
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("n", [1, 777, 20000])
+def test_loss_grad_matches_oracle(ctx, n):
+    rng = np.random.default_rng(100 + n)
+    m = core.Mlp(ctx, BMIN, BMAX, seed=3)
+    # pull the output layer towards concentrated lobes so the head sees kappa >> 1 too;
+    # weights rounded to BF16 so the oracle and the kernels evaluate the same network
+    wts = m.get_weights()
+    wts[-40:] = np.tile(np.float32([0.3, -0.2, 0.5, 2.0, 0.0]), 8) + rng.normal(0, 0.3, 40).astype(np.float32)
+    wts = bf16(wts).astype(np.float32)
+    m.set_weights(wts)
+    pos, nrm, wo, dirs, w = _samples(n, rng)
+    loss, g = m.loss_grad(pos, nrm, wo, dirs, w)
+    gref = np.zeros(core.Mlp.NPARAM)
+    x = _encode(pos, nrm, wo)
+    lref = O.lib().orc_mlp_loss_grad(O.p(wts.astype(np.float64)), O.p(x), O.p(dirs), O.p(w), n, O.p(gref))
+    lbf, gbf = bf16_loss_grad(wts, x, dirs, w)
+    layers = [(_rel(g[sl], gref[sl]), _rel(g[sl], gbf[sl])) for sl in _layer_slices()]
+    print(f"n={n} loss {loss} oracle {lref}; grad rel vs oracle {_rel(g, gref):.4f} vs bf16 model "
+          f"{_rel(g, gbf):.4f}; per layer {layers}")
+    assert abs(loss - lref) <= 1e-2 * abs(lref), (loss, lref)
+    # against the BF16 rounding model: what is left is FP32 accumulation order
+    assert abs(loss - lbf) <= 1e-3 * abs(lbf)
+    for k, (_, rb) in enumerate(layers):
+        assert rb <= 2e-2, (k, rb)
+    # against the FP64 oracle: BF16 deltas through four ReLU layers; a single sample
+    # has no averaging, so only the batch cases are held to the oracle per layer
+    cos = float(g @ gref / (np.linalg.norm(g) * np.linalg.norm(gref)))
+    if n > 1:
+        assert cos >= 0.999 and _rel(g, gref) <= 3e-2, (cos, _rel(g, gref))
+        for k, (ro, _) in enumerate(layers):
+            assert ro <= 0.2, (k, ro)
+    else:
+        assert cos >= 0.99
+    assert m.state()[0] == 0  # no update
+
+
+def test_adam_matches_oracle(ctx):
+    rng = np.random.default_rng(7)
+    m = core.Mlp(ctx, BMIN, BMAX, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, seed=9)
+    p = m.get_weights().astype(np.float64)
+    mo, vo = np.zeros_like(p), np.zeros_like(p)
+    for t in (1, 2, 3):
+        g = rng.normal(0, 10.0 ** rng.uniform(-4, 1), p.shape).astype(np.float32)
+        m.adam_apply(g)
+        O.lib().orc_adam(O.p(p), O.p(g.astype(np.float64)), O.p(mo), O.p(vo), len(p), t, 1e-3, 0.9, 0.999, 1e-8)
+        got = m.get_weights()
+        assert np.abs(got - p).max() <= 2e-6, (t, np.abs(got - p).max())
+        assert m.state()[0] == t
+
+
+def test_non_finite_gradient_skips_the_step_and_halves_the_scale(ctx):
+    rng = np.random.default_rng(1)
+    m = core.Mlp(ctx, BMIN, BMAX, seed=2, loss_scale=1024.0)
+    before = m.get_weights()
+    pos, nrm, wo, dirs, w = _samples(4096, rng)
+    bad = dirs.copy()
+    bad[1234] = np.nan  # one non-finite sample poisons the batch gradient
+    m.train_step(pos, nrm, wo, bad, w)
+    assert m.state() == (0, 512.0)
+    assert np.array_equal(m.get_weights(), before)
+    m.train_step(pos, nrm, wo, dirs, w)
+    assert m.state() == (1, 512.0)
+    assert not np.array_equal(m.get_weights(), before)
+
+
+def test_training_learns_a_position_dependent_distribution(ctx):
+    """Targets: vMF(kappa 30) around a direction that depends on the position.  A uniform
+    guide scores log(4 pi) = 2.53 nats; the trained mixture must score far below it."""
+    rng = np.random.default_rng(4)
+    m = core.Mlp(ctx, BMIN, BMAX, lr=3e-3, seed=4)
+
+    def batch(n):
+        pos = rng.uniform(-0.95, 0.95, (n, 3)).astype(np.float32)
+        mu = _unit(np.stack([np.sin(3 * pos[:, 0]), np.cos(3 * pos[:, 1]), 0.5 + pos[:, 2]], 1))
+        d = np.zeros((n, 3), np.float32)
+        for k in range(0, n, 512):  # one vMF draw per block of positions with a shared mean
+            d[k:k + 512] = _vmf_draw(mu[k], 30.0, min(512, n - k), rng)
+            pos[k:k + 512] = pos[k]
+        nrm = np.tile(np.float32([0, 0, 1]), (n, 1))
+        return pos, nrm, nrm, d, np.ones(n, np.float32)
+
+    losses = [m.train_step(*batch(16384)) for _ in range(400)]
+    assert math.log(4 * math.pi) - np.mean(losses[-20:]) > 1.5, (losses[0], np.mean(losses[-20:]))
+    assert np.mean(losses[-20:]) < losses[0] - 1.0
