@@ -97,6 +97,24 @@ def cpu_baseline(sa, cam, max_bounces: int, seconds: float) -> dict:
                       f"frame 0, {rays} rays in {t_used:.1f} s; ReSTIR DI not included"}
 
 
+def guiding_summary(ctx: core.Context, a) -> dict:
+    """Second half of the metric: relative MSE of the unguided tracer, the 32^3 x 8-lobe vMF grid
+    (online EM) and the learned MLP guide after the same wall time each, on the ~1M-triangle
+    indoor scene (depth 12, 4 spp per frame, MIS 0.5, MLP batch 262,144, Adam lr 1e-3, one step
+    per frame), against an FP64-accumulated reference of the same view."""
+    from research_amd import guiding_eval as G
+    r = G.evaluate(ctx, a.guide_width, a.guide_height, a.guide_seconds, a.guide_ref_spp)
+    ms = r["methods"]
+    return {"workload": f"indoor_{r['tri_count']}_tris_{a.guide_width}x{a.guide_height}_depth12_4spp_per_frame",
+            "metric": "relMSE = mean (I - R)^2 / (R^2 + 0.01) after equal wall time",
+            "seconds_per_method": a.guide_seconds, "ref_spp": a.guide_ref_spp,
+            "rel_mse": {k: v["rel_mse"] for k, v in ms.items()}, "spp": {k: v["spp"] for k, v in ms.items()},
+            "mlp_over_vmm_rel_mse": r.get("mlp_over_vmm_rel_mse"),
+            "mlp_time_to_vmm_rel_mse_ratio": r.get("mlp_time_to_vmm_rel_mse_ratio"),
+            "vmm_over_unguided_rel_mse": r.get("vmm_over_unguided_rel_mse"),
+            "mlp_loss": [ms["mlp"].get("loss_first"), ms["mlp"].get("loss_last")]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +128,11 @@ def main() -> int:
     ap.add_argument("--stack", choices=["private", "lds32", "hybrid16"], default="lds32",
                     help="where the traversal stack lives (context option)")
     ap.add_argument("--xcd-remap", type=int, default=0, help="1: contiguous work range per XCD")
+    ap.add_argument("--guide-seconds", type=float, default=4.0,
+                    help="equal-time guiding comparison: wall seconds per method (0: skip)")
+    ap.add_argument("--guide-ref-spp", type=int, default=8192, help="spp of the FP64 reference image")
+    ap.add_argument("--guide-width", type=int, default=320)
+    ap.add_argument("--guide-height", type=int, default=180)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB copy kernel (for hardware-counter calibration) and exit")
     a = ap.parse_args()
@@ -205,6 +228,9 @@ def main() -> int:
                      "launches": launches.value, "measured_copy_gbs": round(hbm.value, 1)},
         "cpu_baseline": None,
     }
+    if a.guide_seconds > 0:
+        print("[bench] equal-time guiding comparison (indoor scene) ...", file=sys.stderr, flush=True)
+        out["guiding"] = guiding_summary(ctx, a)
     if not a.no_cpu_baseline:
         cam0 = orbit_camera(sa, 0, W, H)
         print("[bench] cpu baseline (oracle) ...", file=sys.stderr, flush=True)

@@ -220,6 +220,7 @@ __global__ void k_vmm_mask(uint32_t ncell, uint32_t K, const float* __restrict__
 }
 
 // ---- path-vertex finalisation: incident radiance along the sampled direction ----
+// training sample: t0 (pos, lum(Li) / pdf)  t1 (wi, pdf)  t2 (n, lum(Li f cos) / pdf)  t3 (wo)
 __global__ void k_finalize(const float4* __restrict__ rec, uint32_t nrec, uint32_t max_bounces, uint32_t stamp,
                            const float4* __restrict__ rad, uint32_t keep_thresh, float4* __restrict__ train,
                            uint32_t* __restrict__ cnt, uint32_t cap) {
@@ -231,7 +232,7 @@ __global__ void k_finalize(const float4* __restrict__ rec, uint32_t nrec, uint32
             const float4* r = rec + (size_t)i * kRecF4;
             float4 r0 = r[0];
             if (__float_as_uint(r0.w) == stamp && (pcg_hash(i ^ (stamp * 2654435761u)) >> 8) < keep_thresh) {
-                float4 r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5], r6 = r[6];
+                float4 r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5], r6 = r[6], r7 = r[7];
                 uint32_t pixel = i / max_bounces;
                 float4 Lf = rad[pixel];
                 f3 Ls = sub(sub(xyz(Lf), xyz(r4)), (r3.w > 0.5f) ? xyz(r5) : mk(0, 0, 0));
@@ -242,7 +243,9 @@ __global__ void k_finalize(const float4* __restrict__ rec, uint32_t nrec, uint32
                     take = true;
                     t0 = make_float4(r0.x, r0.y, r0.z, w / r1.w);
                     t1 = make_float4(r3.x, r3.y, r3.z, r1.w);
-                    t2 = make_float4(r1.x, r1.y, r1.z, 0.0f);
+                    // product weight: Li f cos / pdf, the one-sample estimate of the integrand
+                    f3 bw = xyz(r7);
+                    t2 = make_float4(r1.x, r1.y, r1.z, fmaxf(lum709(mk(Li.x * bw.x, Li.y * bw.y, Li.z * bw.z)), 0.0f));
                     t3 = make_float4(r2.x, r2.y, r2.z, 0.0f);
                 }
             }

@@ -18,15 +18,16 @@ struct GuideDev {
     float bmin[3] = {0, 0, 0}, bext[3] = {1, 1, 1};
 };
 
-// path-vertex record (7 float4 = 112 B) at index pixel * max_bounces + bounce:
+// path-vertex record (8 float4 = 128 B) at index pixel * max_bounces + bounce:
 //  r0 (pos, stamp) r1 (n, pdf) r2 (wo, mat) r3 (wi, nee visible)
 //  r4 (L before this vertex, alive) r5 (NEE contribution if visible) r6 (throughput after)
+//  r7 (BSDF weight f cos / pdf before Russian roulette)
 struct VertexRec {
     float4* v = nullptr;
     uint32_t max_bounces = 0;
     uint32_t stamp = 0;
 };
-constexpr uint32_t kRecF4 = 7;
+constexpr uint32_t kRecF4 = 8;
 
 RRH_D uint32_t vmm_cell(const GuideDev& G, f3 p) {
     float f[3] = {(p.x - G.bmin[0]) / G.bext[0] * (float)G.res, (p.y - G.bmin[1]) / G.bext[1] * (float)G.res,
@@ -85,9 +86,8 @@ RRH_D f3 guided_sample(const GuideDev& G, uint32_t mode, float alpha, const DevS
 }
 
 RRH_D void guide_record_vertex(const VertexRec& R, uint32_t pixel, uint32_t b, f3 pos, f3 n, f3 wo, f3 wi, float pdf,
-                               f3 thp_before, f3 thp_after, f3 L_pre, f3 nee_c, uint32_t mat, bool alive) {
+                               f3 bsdf_w, f3 thp_after, f3 L_pre, f3 nee_c, uint32_t mat, bool alive) {
     float4* r = R.v + ((size_t)pixel * R.max_bounces + b) * kRecF4;
-    (void)thp_before;
     r[0] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(R.stamp));
     r[1] = make_float4(n.x, n.y, n.z, pdf);
     r[2] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(mat));
@@ -95,6 +95,7 @@ RRH_D void guide_record_vertex(const VertexRec& R, uint32_t pixel, uint32_t b, f
     r[4] = make_float4(L_pre.x, L_pre.y, L_pre.z, alive ? 1.0f : 0.0f);
     r[5] = make_float4(nee_c.x, nee_c.y, nee_c.z, 0.0f);
     r[6] = make_float4(thp_after.x, thp_after.y, thp_after.z, 0.0f);
+    r[7] = make_float4(bsdf_w.x, bsdf_w.y, bsdf_w.z, 0.0f);
 }
 RRH_D void guide_record_nee_visible(const VertexRec& R, uint32_t pixel, uint32_t b) {
     float4* r = R.v + ((size_t)pixel * R.max_bounces + b) * kRecF4;

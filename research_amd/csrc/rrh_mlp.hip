@@ -3,10 +3,10 @@
 // The reference has no learned guide (its guide is the single-lobe grid of
 // GuidingGrid.slang); the network and loss are defined in include/rrh.h.
 //
-// Training step = k_mlp_wsum (sum of sample weights) -> k_mlp_train (persistent blocks:
+// Training step = k_mlp_wsum (per-block sums of sample weights) -> k_mlp_train (persistent blocks:
 // per 64-sample chunk, forward through five BF16 MFMA layers, vMF-mixture NLL head,
 // backward deltas, then the block's weight-gradient tiles, kept in registers across
-// chunks) -> k_mlp_reduce (deterministic sum of the per-block partials, non-finite
+// chunks) -> k_mlp_reduce1/2 (fixed-order sum of the per-block partials, non-finite
 // check) -> k_mlp_adam (FP32 master weights) -> k_mlp_pack (BF16 copies for the next
 // step) -> k_mlp_finish (step count, dynamic loss scale).
 //
@@ -26,6 +26,7 @@ constexpr int NI = 78, KI = 96, NH = 64, NO = 40, NOP = 48, NK = 8;
 constexpr int NPARAM = RRH_MLP_PARAMS;
 constexpr int kBlockSamples = 64;   // 4 waves x 16 samples
 constexpr int kTiles = 80;          // weight-gradient tiles (16x16) over all layers
+constexpr int kWsumBlocks = 64;     // blocks of the sample-weight sum
 
 __host__ __device__ constexpr int in_of(int l) { return l == 0 ? NI : NH; }
 __host__ __device__ constexpr int out_of(int l) { return l == 4 ? NO : NH; }
@@ -58,6 +59,7 @@ struct HostSrc {  // arrays uploaded from the host (3 floats per vector)
     const float *pos, *nrm, *wo, *dir, *w;
     uint32_t n;
     __device__ uint32_t count() const { return n; }
+    __device__ float weight(uint32_t i) const { return w[i]; }
     __device__ bool get(uint32_t i, Sample& s) const {
         if (i >= n) return false;
         s.p = mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
@@ -68,38 +70,67 @@ struct HostSrc {  // arrays uploaded from the host (3 floats per vector)
         return true;
     }
 };
-struct TrainSrc {  // samples finalised from path-vertex records: t0 (pos, w) t1 (dir, pdf) t2 (n) t3 (wo)
+// samples finalised from path-vertex records: t0 (pos, w_Li) t1 (dir, pdf) t2 (n, w_product) t3 (wo).
+// The network is trained towards the product Li f cos (w_product), the ideal sampling density
+// for the integrand at the vertex; the grid trains on incident radiance (w_Li).  When more
+// than `batch` are available, the batch takes every (avail / batch)-th one, spread over the frame
+struct TrainSrc {
     const float4* tr;
     const uint32_t* cnt;
     uint32_t cap, batch;
-    __device__ uint32_t count() const { return min(min(*cnt, cap), batch); }
-    __device__ bool get(uint32_t i, Sample& s) const {
-        if (i >= count()) return false;
+    __device__ uint32_t avail() const { return min(*cnt, cap); }
+    __device__ uint32_t count() const { return min(avail(), batch); }
+    __device__ uint32_t pick(uint32_t j, uint32_t a) const { return (a > batch) ? (uint32_t)(((uint64_t)j * a) / batch) : j; }
+    __device__ float weight(uint32_t j) const { return tr[(size_t)pick(j, avail()) * 4 + 2].w; }
+    __device__ bool get(uint32_t j, Sample& s) const {
+        const uint32_t a = avail();
+        if (j >= min(a, batch)) return false;
+        const uint32_t i = pick(j, a);
         float4 t0 = tr[(size_t)i * 4], t1 = tr[(size_t)i * 4 + 1], t2 = tr[(size_t)i * 4 + 2], t3 = tr[(size_t)i * 4 + 3];
-        s.p = xyz(t0); s.w = t0.w; s.dir = xyz(t1); s.n = xyz(t2); s.wo = xyz(t3);
+        s.p = xyz(t0); s.w = t2.w; s.dir = xyz(t1); s.n = xyz(t2); s.wo = xyz(t3);  // product-weighted target
         return true;
     }
 };
 
 // ---- encoding: features 0..71 = sin/cos(2^j pi u_c), 72..74 n, 75..77 wo, 78..95 zero ----
-// 4 lanes per sample; lane quarter q writes features [24q, 24q + 24)
-__device__ __forceinline__ void encode_row(const MlpDev& M, const Sample& s, bool valid, int q, __bf16* row) {
-    if (q < 3) {
-        float pc = (q == 0) ? s.p.x : (q == 1 ? s.p.y : s.p.z);
-        float u = valid ? (pc - M.bmin[q]) / M.bext[q] : 0.0f;
-        u = clampf(u, 0.0f, 1.0f);
-        for (int j = 0; j < 12; ++j) {
-            float t = u * (float)(1 << j);               // exact scaling by 2^j
-            float tm = t - 2.0f * floorf(t * 0.5f);      // t mod 2, exact for t < 2^24
-            float sn, cs;
-            sincos_rr(3.14159265358979323846f * tm, &sn, &cs);
-            row[q * 24 + 2 * j] = (__bf16)(valid ? sn : 0.0f);
-            row[q * 24 + 2 * j + 1] = (__bf16)(valid ? cs : 0.0f);
+// The angle 2^j pi u is reduced exactly (t = 2^j u, t mod 2) and evaluated with the hardware
+// sine / cosine, whose argument is in revolutions (tm / 2 in [0, 1)); its error (~1e-6
+// absolute) is far below the BF16 rounding the feature gets next.
+__device__ __forceinline__ float unit_coord(const MlpDev& M, const Sample& s, int c) {
+    const float pc = (c == 0) ? s.p.x : (c == 1 ? s.p.y : s.p.z);
+    const float lo = (c == 0) ? M.bmin[0] : (c == 1 ? M.bmin[1] : M.bmin[2]);  // selects: no indexed
+    const float ext = (c == 0) ? M.bext[0] : (c == 1 ? M.bext[1] : M.bext[2]); // copy of the argument
+    return clampf((pc - lo) / ext, 0.0f, 1.0f);
+}
+__device__ __forceinline__ void freq_pair(float u, int j, float& sn, float& cs) {
+    const float t = u * (float)(1 << j);           // exact scaling by 2^j
+    const float rev = 0.5f * (t - 2.0f * floorf(t * 0.5f));  // (t mod 2) / 2, exact for t < 2^24
+    sn = __builtin_amdgcn_sinf(rev);
+    cs = __builtin_amdgcn_cosf(rev);
+}
+// features [fb, fb + 8) of a sample (fb a multiple of 8)
+__device__ __forceinline__ bf16x8 encode8(const MlpDev& M, const Sample& s, bool valid, int fb) {
+    bf16x8 a;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const int f = fb + 2 * p;
+        float v0 = 0.0f, v1 = 0.0f;
+        if (f < 72) {
+            freq_pair(unit_coord(M, s, f / 24), (f % 24) >> 1, v0, v1);
+        } else {
+            const int k = f - 72;  // 0..23: n (3), wo (3), zero padding
+            v0 = (k == 0) ? s.n.x : (k == 2) ? s.n.z : (k == 4) ? s.wo.y : 0.0f;
+            v1 = (k == 0) ? s.n.y : (k == 2) ? s.wo.x : (k == 4) ? s.wo.z : 0.0f;
         }
-    } else {
-        float v[6] = {s.n.x, s.n.y, s.n.z, s.wo.x, s.wo.y, s.wo.z};
-        for (int k = 0; k < 24; ++k) row[72 + k] = (__bf16)((valid && k < 6) ? v[k] : 0.0f);
+        a[2 * p] = (__bf16)(valid ? v0 : 0.0f);
+        a[2 * p + 1] = (__bf16)(valid ? v1 : 0.0f);
     }
+    return a;
+}
+// 4 lanes per sample; lane quarter q writes features [24q, 24q + 24) into the sample's row
+__device__ __forceinline__ void encode_row(const MlpDev& M, const Sample& s, bool valid, int q, __bf16* row) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) *reinterpret_cast<bf16x8*>(row + q * 24 + 8 * k) = encode8(M, s, valid, q * 24 + 8 * k);
 }
 
 // ---- one 16-sample wave through a dense layer: out = A W^T + b ----------------------
@@ -154,54 +185,68 @@ __device__ __forceinline__ void layer_bwd(const __bf16* Din, const __bf16* WT, c
 __device__ __forceinline__ float softplus_f(float d) { return d > 20.0f ? d : log_rr(1.0f + exp_rr(d)); }
 __device__ __forceinline__ float sigmoid_f(float d) { return 1.0f / (1.0f + exp_rr(-d)); }
 
-__device__ __forceinline__ void head_lobes(const float* o, float* lobes) {
-    float emax = -3.0e38f;
-    for (int k = 0; k < NK; ++k) emax = fmaxf(emax, o[k * 5 + 4]);
-    float z = 0.0f;
-    for (int k = 0; k < NK; ++k) z += exp_rr(o[k * 5 + 4] - emax);
-    for (int k = 0; k < NK; ++k) {
-        const float* v = o + k * 5;
+// The head runs on all 64 lanes: the 4 adjacent lanes of a sample each own two lobes
+// (lane quarter qd: lobes 2 qd, 2 qd + 1), so the softmax and mixture sums are two xor shuffles.
+struct Lobe { float mu[3], kap, pi; };
+__device__ __forceinline__ float group_sum4(float v) {
+    v += __shfl_xor(v, 1);
+    return v + __shfl_xor(v, 2);
+}
+__device__ __forceinline__ float group_max4(float v) {
+    v = fmaxf(v, __shfl_xor(v, 1));
+    return fmaxf(v, __shfl_xor(v, 2));
+}
+__device__ __forceinline__ void head_lobes2(const float* o, int qd, Lobe* lb) {
+    const float e0 = o[(2 * qd) * 5 + 4], e1 = o[(2 * qd + 1) * 5 + 4];
+    const float emax = group_max4(fmaxf(e0, e1));
+    const float x[2] = {exp_rr(e0 - emax), exp_rr(e1 - emax)};
+    const float z = group_sum4(x[0] + x[1]);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const float* v = o + (2 * qd + j) * 5;
         float len = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + 1e-12f);
-        lobes[k * 5 + 0] = v[0] / len;
-        lobes[k * 5 + 1] = v[1] / len;
-        lobes[k * 5 + 2] = v[2] / len;
-        lobes[k * 5 + 3] = fminf(0.1f + softplus_f(v[3]), 1000.0f);
-        lobes[k * 5 + 4] = exp_rr(v[4] - emax) / z;
+        lb[j].mu[0] = v[0] / len;
+        lb[j].mu[1] = v[1] / len;
+        lb[j].mu[2] = v[2] / len;
+        lb[j].kap = fminf(0.1f + softplus_f(v[3]), 1000.0f);
+        lb[j].pi = x[j] / z;
     }
 }
 __device__ __forceinline__ float vmf_val(float kap, float c) {
     return kap * exp_rr(kap * (c - 1.0f)) / (6.28318530717958647692f * (1.0f - exp_rr(-2.0f * kap)));
 }
-// returns the sample's loss term; writes dL/dout (times `gscale`) into g[40]
-__device__ __forceinline__ float head_grad(const float* o, f3 d, float wn, float gscale, float* g) {
-    float lob[NK * 5];
-    head_lobes(o, lob);
-    float f[NK], c[NK], q = 0.0f;
-    for (int k = 0; k < NK; ++k) {
-        c[k] = lob[k * 5] * d.x + lob[k * 5 + 1] * d.y + lob[k * 5 + 2] * d.z;
-        f[k] = vmf_val(lob[k * 5 + 3], c[k]);
-        q += lob[k * 5 + 4] * f[k];
+// NLL head of one sample on its 4 lanes: returns the loss term (on lane quarter 0, else 0)
+// and writes dL/dout (times `gscale`) of this lane's two lobes into g[10]
+__device__ __forceinline__ float head_grad2(const float* o, int qd, f3 d, float wn, float gscale, float* g) {
+    Lobe lb[2];
+    head_lobes2(o, qd, lb);
+    float c[2], f[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        c[j] = lb[j].mu[0] * d.x + lb[j].mu[1] * d.y + lb[j].mu[2] * d.z;
+        f[j] = vmf_val(lb[j].kap, c[j]);
     }
-    float loss = -wn * log_rr(q + 1e-20f);
-    float dq = -wn / (q + 1e-20f) * gscale;
-    for (int k = 0; k < NK; ++k) {
-        const float* v = o + k * 5;
-        float pik = lob[k * 5 + 4], kap = lob[k * 5 + 3];
-        g[k * 5 + 4] = dq * pik * (f[k] - q);
+    const float q = group_sum4(lb[0].pi * f[0] + lb[1].pi * f[1]);
+    const bool on = wn > 0.0f;
+    const float dq = on ? -wn / (q + 1e-20f) * gscale : 0.0f;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const float* v = o + (2 * qd + j) * 5;
+        const float pik = lb[j].pi, kap = lb[j].kap;
+        g[j * 5 + 4] = dq * pik * (f[j] - q);
         float e2 = exp_rr(-2.0f * kap);
-        float dlogf = 1.0f / kap - 2.0f * e2 / (1.0f - e2) + (c[k] - 1.0f);
+        float dlogf = 1.0f / kap - 2.0f * e2 / (1.0f - e2) + (c[j] - 1.0f);
         float raw = 0.1f + softplus_f(v[3]);
-        g[k * 5 + 3] = (raw > 1000.0f) ? 0.0f : dq * pik * f[k] * dlogf * sigmoid_f(v[3]);
+        g[j * 5 + 3] = (raw > 1000.0f) ? 0.0f : dq * pik * f[j] * dlogf * sigmoid_f(v[3]);
         float len = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + 1e-12f);
-        float mu0 = lob[k * 5], mu1 = lob[k * 5 + 1], mu2 = lob[k * 5 + 2];
-        float s = dq * pik * f[k] * kap;
+        float s = dq * pik * f[j] * kap;
         float g0 = s * d.x, g1 = s * d.y, g2 = s * d.z;
-        float mg = mu0 * g0 + mu1 * g1 + mu2 * g2;
-        g[k * 5 + 0] = (g0 - mu0 * mg) / len;
-        g[k * 5 + 1] = (g1 - mu1 * mg) / len;
-        g[k * 5 + 2] = (g2 - mu2 * mg) / len;
+        float mg = lb[j].mu[0] * g0 + lb[j].mu[1] * g1 + lb[j].mu[2] * g2;
+        g[j * 5 + 0] = (g0 - lb[j].mu[0] * mg) / len;
+        g[j * 5 + 1] = (g1 - lb[j].mu[1] * mg) / len;
+        g[j * 5 + 2] = (g2 - lb[j].mu[2] * mg) / len;
     }
-    return loss;
+    return (on && qd == 0) ? -wn * log_rr(q + 1e-20f) : 0.0f;
 }
 
 // ---- shared memory of a 4-wave block (64 samples) -------------------------------------
@@ -215,11 +260,6 @@ struct __align__(16) TrainSmem {
     __device__ float* O(int w) { return reinterpret_cast<float*>(&D[0][0]) + w * 16 * NOP; }
 };
 static_assert(4 * 16 * NOP * 4 <= 2 * kBlockSamples * 64 * 2, "raw outputs fit in D[0..1]");
-struct __align__(16) InferSmem {
-    __bf16 X[kBlockSamples * KI];
-    __bf16 H[2][kBlockSamples * 64];     // ping-pong
-    float O[4 * 16 * NOP];
-};
 
 template <class Src>
 __device__ void forward_chunk(const MlpDev& M, const Src& src, uint32_t base, __bf16* X, __bf16* const* H, float* Ow,
@@ -232,11 +272,13 @@ __device__ void forward_chunk(const MlpDev& M, const Src& src, uint32_t base, __
         bool v = src.get(base + w * 16 + sl, s);
         if (!v) { s.p = s.n = s.wo = s.dir = mk(0, 0, 0); s.w = 0.0f; }
         encode_row(M, s, v, q, X + (w * 16 + sl) * KI);
-        if (q == 0) { mine = s; valid = v; }
+        mine = s;  // every lane of the sample keeps it (the head runs on all four)
+        valid = v;
     }
     __syncthreads();
     layer_fwd<KI, 4>(X + w * 16 * KI, Wl(M, 0), M.P + off_b(0), NH, true, H[0] + w * 16 * 64, nullptr);
     __syncthreads();
+#pragma unroll
     for (int l = 1; l < 4; ++l) {
         layer_fwd<64, 4>(H[l - 1] + w * 16 * 64, Wl(M, l), M.P + off_b(l), NH, true, H[l] + w * 16 * 64, nullptr);
         __syncthreads();
@@ -258,7 +300,9 @@ __global__ void __launch_bounds__(256) k_mlp_train(MlpDev M, Src src, uint32_t n
     __shared__ TrainSmem sm;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
     __bf16* const Hs[4] = {sm.H[0], sm.H[1], sm.H[2], sm.H[3]};
-    const float inv_w = 1.0f / fmaxf(*wsum, 1e-30f);
+    float wt = 0.0f;  // total sample weight: the kWsumBlocks partials, summed in order
+    for (int b = 0; b < kWsumBlocks; ++b) wt += wsum[b];
+    const float inv_w = 1.0f / fmaxf(wt, 1e-30f);
     const float gscale = *scale_p;
     f32x4 acc[kTiles / 4];
 #pragma unroll
@@ -269,21 +313,17 @@ __global__ void __launch_bounds__(256) k_mlp_train(MlpDev M, Src src, uint32_t n
         Sample mine;
         bool valid = false;
         forward_chunk(M, src, base, sm.X, Hs, sm.O(w), mine, valid);
-        // head: 16 lanes per wave, one sample each (lane 4*sl holds sample sl after encode)
+        // head: 4 lanes per sample
         {
-            const int sl = lane >> 2;
-            float gout[NO];
-            bool act = (lane & 3) == 0;
-            float lw = 0.0f;
-            if (act) {
-                const float* o = sm.O(w) + sl * NOP;
-                float wn = valid ? mine.w * inv_w : 0.0f;
-                if (wn > 0.0f) lw = head_grad(o, mine.dir, wn, gscale, gout);
-                else for (int k = 0; k < NO; ++k) gout[k] = 0.0f;
-                __bf16* Drow = sm.D[4] + (w * 16 + sl) * 64;
-                for (int k = 0; k < 64; ++k) Drow[k] = (__bf16)(k < NO ? gout[k] : 0.0f);
-            }
-            loss_acc += lw;
+            const int sl = lane >> 2, qd = lane & 3;
+            const float wn = valid ? mine.w * inv_w : 0.0f;
+            float g[10];
+            loss_acc += head_grad2(sm.O(w) + sl * NOP, qd, mine.dir, wn, gscale, g);
+            __bf16* Drow = sm.D[4] + (w * 16 + sl) * 64;
+#pragma unroll
+            for (int k = 0; k < 10; ++k) Drow[qd * 10 + k] = (__bf16)g[k];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) Drow[NO + qd * 6 + k] = (__bf16)0.0f;
         }
         __syncthreads();
         // deltas: D3 = (D4 W4) * (H3 > 0) ... D0 = (D1 W1) * (H0 > 0)
@@ -350,51 +390,158 @@ __global__ void __launch_bounds__(256) k_mlp_train(MlpDev M, Src src, uint32_t n
     if (threadIdx.x == 0) partial_loss[blockIdx.x] = sm.red[0] + sm.red[1] + sm.red[2] + sm.red[3];
 }
 
-template <class Src>
-__global__ void __launch_bounds__(256) k_mlp_infer(MlpDev M, Src src, uint32_t nsamples, float* __restrict__ lobes) {
-    __shared__ InferSmem sm;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    __bf16* const Hs[4] = {sm.H[0], sm.H[1], sm.H[0], sm.H[1]};
-    const uint32_t n = min(src.count(), nsamples);
-    for (uint32_t base = blockIdx.x * kBlockSamples; base < n; base += gridDim.x * kBlockSamples) {
-        Sample mine;
-        bool valid = false;
-        forward_chunk(M, src, base, sm.X, Hs, sm.O + w * 16 * NOP, mine, valid);
-        const int sl = lane >> 2;
-        if ((lane & 3) == 0 && valid) {
-            float lob[NK * 5];
-            head_lobes(sm.O + w * 16 * NOP + sl * NOP, lob);
-            float* dst = lobes + (size_t)(base + w * 16 + sl) * (NK * 5);
-            for (int k = 0; k < NK * 5; ++k) dst[k] = lob[k];
-        }
-        __syncthreads();
+// Inference: every wave is independent and persistent.  It keeps all weight fragments of
+// the five layers in registers (42 x 8 BF16 per lane), so a 16-sample chunk costs no weight
+// traffic at all; layer-0 operands are encoded straight into registers, the hidden
+// activations go through the wave's own LDS tile (the D -> A operand transpose).
+struct __align__(16) InferWave {
+    __bf16 H[2][16 * 64];
+    float O[16 * NOP];
+};
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+template <int KK, int NT>
+__device__ __forceinline__ void load_frags(const __bf16* W, int K, bf16x8 (&f)[NT][KK]) {
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int kk = 0; kk < KK; ++kk) f[nt][kk] = *reinterpret_cast<const bf16x8*>(W + (nt * 16 + r) * K + kk * 32 + g * 8);
+}
+template <int NT>
+__device__ __forceinline__ void load_bias(const float* b, int nout, float (&bo)[NT]) {
+    const int r = threadIdx.x & 15;
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) bo[nt] = (nt * 16 + r < nout) ? b[nt * 16 + r] : 0.0f;
+}
+// hidden layer from the wave's LDS tile: Hout = relu(Hin W^T + b)
+__device__ __forceinline__ void hidden_reg(const __bf16* Hin, const bf16x8 (&wf)[4][2], const float (&bo)[4], __bf16* Hout) {
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    bf16x8 a0 = *reinterpret_cast<const bf16x8*>(Hin + r * 64 + g * 8);
+    bf16x8 a1 = *reinterpret_cast<const bf16x8*>(Hin + r * 64 + 32 + g * 8);
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, wf[nt][0], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, wf[nt][1], acc, 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) Hout[(4 * g + j) * 64 + nt * 16 + r] = (__bf16)fmaxf(acc[j] + bo[nt], 0.0f);
     }
 }
 
-__global__ void k_mlp_wsum_host(const float* __restrict__ w, uint32_t n, float* __restrict__ out) {
-    float v = 0.0f;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v += w[i];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    if ((threadIdx.x & 63) == 0) atomicAdd(out, v);
-}
-__global__ void k_mlp_wsum_train(const float4* __restrict__ tr, const uint32_t* __restrict__ cnt, uint32_t cap,
-                                 uint32_t batch, float* __restrict__ out) {
-    uint32_t n = min(min(*cnt, cap), batch);
-    float v = 0.0f;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v += tr[(size_t)i * 4].w;
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    if ((threadIdx.x & 63) == 0) atomicAdd(out, v);
+template <class Src>
+__global__ void __launch_bounds__(256) k_mlp_infer(MlpDev M, Src src, uint32_t nsamples, float* __restrict__ lobes) {
+    __shared__ InferWave smw[4];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    InferWave& S = smw[w];
+    bf16x8 f0[4][3], f1[4][2], f2[4][2], f3[4][2], f4[3][2];
+    load_frags<3, 4>(Wl(M, 0), KI, f0);
+    load_frags<2, 4>(Wl(M, 1), 64, f1);
+    load_frags<2, 4>(Wl(M, 2), 64, f2);
+    load_frags<2, 4>(Wl(M, 3), 64, f3);
+    load_frags<2, 3>(Wl(M, 4), 64, f4);
+    float b0[4], b1[4], b2[4], b3[4], b4[3];
+    load_bias<4>(M.P + off_b(0), NH, b0);
+    load_bias<4>(M.P + off_b(1), NH, b1);
+    load_bias<4>(M.P + off_b(2), NH, b2);
+    load_bias<4>(M.P + off_b(3), NH, b3);
+    load_bias<3>(M.P + off_b(4), NO, b4);
+    const uint32_t n = min(src.count(), nsamples);
+    const uint32_t nchunks = (n + 15) / 16;
+    for (uint32_t ch = blockIdx.x * 4 + w; ch < nchunks; ch += gridDim.x * 4) {
+        const uint32_t base = ch * 16;
+        Sample smp;
+        const bool valid = src.get(base + r, smp);  // the 4 lanes of row r hold sample r
+        if (!valid) { smp.p = smp.n = smp.wo = smp.dir = mk(0, 0, 0); smp.w = 0.0f; }
+        // layer 0: A operand (row r, features 32 kk + 8 g ..) encoded in registers
+        {
+            bf16x8 a[3];
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk) a[kk] = encode8(M, smp, valid, kk * 32 + g * 8);
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int kk = 0; kk < 3; ++kk) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kk], f0[nt][kk], acc, 0, 0, 0);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) S.H[0][(4 * g + j) * 64 + nt * 16 + r] = (__bf16)fmaxf(acc[j] + b0[nt], 0.0f);
+            }
+        }
+        wave_sync();
+        hidden_reg(S.H[0], f1, b1, S.H[1]);
+        wave_sync();
+        hidden_reg(S.H[1], f2, b2, S.H[0]);
+        wave_sync();
+        hidden_reg(S.H[0], f3, b3, S.H[1]);
+        wave_sync();
+        {
+            bf16x8 a0 = *reinterpret_cast<const bf16x8*>(S.H[1] + r * 64 + g * 8);
+            bf16x8 a1 = *reinterpret_cast<const bf16x8*>(S.H[1] + r * 64 + 32 + g * 8);
+#pragma unroll
+            for (int nt = 0; nt < 3; ++nt) {
+                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, f4[nt][0], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, f4[nt][1], acc, 0, 0, 0);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) S.O[(4 * g + j) * NOP + nt * 16 + r] = acc[j] + b4[nt];
+            }
+        }
+        wave_sync();
+        // head: 4 lanes per sample (sample sl = lane / 4; its validity sits in lane sl)
+        const int sl = lane >> 2, qd = lane & 3;
+        const bool vs = __shfl((int)valid, sl) != 0;
+        Lobe lb[2];
+        head_lobes2(S.O + sl * NOP, qd, lb);
+        if (vs) {
+            float* dst = lobes + (size_t)(base + sl) * (NK * 5) + qd * 10;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                dst[j * 5 + 0] = lb[j].mu[0];
+                dst[j * 5 + 1] = lb[j].mu[1];
+                dst[j * 5 + 2] = lb[j].mu[2];
+                dst[j * 5 + 3] = lb[j].kap;
+                dst[j * 5 + 4] = lb[j].pi;
+            }
+        }
+        wave_sync();
+    }
 }
 
-// grad = sum over blocks / loss scale (fixed order: deterministic); flags non-finite values
-__global__ void k_mlp_reduce(const float* __restrict__ partial, const float* __restrict__ partial_loss, uint32_t nb,
-                             const float* __restrict__ scale_p, float* __restrict__ grad, float* __restrict__ loss,
-                             uint32_t* __restrict__ bad) {
+// per-block sums of the sample weights (fixed grid and order: deterministic)
+template <class Src>
+__global__ void __launch_bounds__(256) k_mlp_wsum(Src src, float* __restrict__ out) {
+    __shared__ float red[4];
+    const uint32_t n = src.count();
+    float v = 0.0f;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v += src.weight(i);
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// two-stage fixed-order sum of the per-block partial gradients (deterministic):
+// stage 1 sums groups of kGroup blocks, stage 2 the groups, unscales and flags non-finite values
+constexpr uint32_t kGroup = 16;
+__global__ void k_mlp_reduce1(const float* __restrict__ partial, uint32_t nb, float* __restrict__ tmp) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x, gq = blockIdx.y;
+    if (i >= (uint32_t)NPARAM) return;
+    const uint32_t b0 = gq * kGroup, b1 = min(b0 + kGroup, nb);
+    float s = 0.0f;
+    for (uint32_t b = b0; b < b1; ++b) s += partial[(size_t)b * NPARAM + i];
+    tmp[(size_t)gq * NPARAM + i] = s;
+}
+__global__ void k_mlp_reduce2(const float* __restrict__ tmp, uint32_t ng, const float* __restrict__ partial_loss,
+                              uint32_t nb, const float* __restrict__ scale_p, float* __restrict__ grad,
+                              float* __restrict__ loss, uint32_t* __restrict__ bad) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     float inv = 1.0f / *scale_p;
     if (i < (uint32_t)NPARAM) {
         float s = 0.0f;
-        for (uint32_t b = 0; b < nb; ++b) s += partial[(size_t)b * NPARAM + i];
+        for (uint32_t g = 0; g < ng; ++g) s += tmp[(size_t)g * NPARAM + i];
         s *= inv;
         if (!isfinite(s)) atomicOr(bad, 1u);
         grad[i] = s;
@@ -471,8 +618,10 @@ uint32_t pcg_host(uint32_t s) {
 struct rrh_mlp {
     rrh_ctx* ctx = nullptr;
     rrh_mlp_params p{};
-    DevBuf P, m, v, grad, pk, partial, ploss, scal;  // scal: wsum, scale, loss (floats) + state, bad (u32)
-    uint32_t nb = 0;
+    DevBuf P, m, v, grad, pk, partial, part2, ploss, scal;
+    // scal (512 B): float [1] loss scale, [2] loss; u32 [4] Adam steps, [5] good steps, [6] overflow flag;
+    // float [16, 16 + kWsumBlocks) per-block sample-weight sums
+    uint32_t nb = 0, cu_count = 256, infer_resident = 0;
     MlpDev dev() const {
         MlpDev d;
         d.pk = (const __bf16*)pk.p;
@@ -480,7 +629,7 @@ struct rrh_mlp {
         for (int k = 0; k < 3; ++k) { d.bmin[k] = p.bounds_min[k]; d.bext[k] = p.bounds_max[k] - p.bounds_min[k]; }
         return d;
     }
-    float* wsum() const { return (float*)scal.p; }
+    float* wsum() const { return (float*)scal.p + 16; }
     float* scale() const { return (float*)scal.p + 1; }
     float* loss() const { return (float*)scal.p + 2; }
     uint32_t* state() const { return (uint32_t*)scal.p + 4; }
@@ -498,12 +647,18 @@ int mlp_pack(rrh_ctx* c, rrh_mlp* m) {
 
 template <class Src>
 int mlp_grad(rrh_ctx* c, rrh_mlp* m, const Src& src, uint32_t n) {
-    const size_t smem = 0;  // static LDS
-    hipLaunchKernelGGL(k_mlp_train<Src>, dim3(m->nb), dim3(256), smem, c->stream, m->dev(), src, n, (const float*)m->wsum(),
+    hipLaunchKernelGGL(k_mlp_wsum<Src>, dim3(kWsumBlocks), dim3(256), 0, c->stream, src, m->wsum());
+    RRH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_mlp_train<Src>, dim3(m->nb), dim3(256), 0, c->stream, m->dev(), src, n, (const float*)m->wsum(),
                        (const float*)m->scale(), (float*)m->partial.p, (float*)m->ploss.p);
     RRH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_mlp_reduce, dim3((NPARAM + 255) / 256), dim3(256), 0, c->stream, (const float*)m->partial.p,
-                       (const float*)m->ploss.p, m->nb, (const float*)m->scale(), (float*)m->grad.p, m->loss(), m->bad());
+    const uint32_t ng = (m->nb + kGroup - 1) / kGroup;
+    hipLaunchKernelGGL(k_mlp_reduce1, dim3((NPARAM + 255) / 256, ng), dim3(256), 0, c->stream,
+                       (const float*)m->partial.p, m->nb, (float*)m->part2.p);
+    RRH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_mlp_reduce2, dim3((NPARAM + 255) / 256), dim3(256), 0, c->stream, (const float*)m->part2.p,
+                       ng, (const float*)m->ploss.p, m->nb, (const float*)m->scale(), (float*)m->grad.p, m->loss(),
+                       m->bad());
     RRH_LAUNCH_CHECK();
     return RRH_OK;
 }
@@ -519,6 +674,19 @@ int mlp_update(rrh_ctx* c, rrh_mlp* m) {
     hipLaunchKernelGGL(k_mlp_finish, dim3(1), dim3(64), 0, c->stream, m->state(), m->bad(), m->scale());
     RRH_LAUNCH_CHECK();
     return RRH_OK;
+}
+
+// persistent inference grid: at most the blocks the device holds at once, fewer for small n
+template <class K>
+uint32_t infer_grid(rrh_ctx* c, rrh_mlp* m, K kernel, uint32_t n) {
+    if (m->infer_resident == 0) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+        m->infer_resident = (uint32_t)per_cu * m->cu_count;
+    }
+    (void)c;
+    const uint32_t need = (n + kBlockSamples - 1) / kBlockSamples;
+    return need < 1 ? 1 : (need < m->infer_resident ? need : m->infer_resident);
 }
 
 struct HostTmp {
@@ -557,13 +725,15 @@ int rrh_mlp_create(rrh_ctx* c, const rrh_mlp_params* p, rrh_mlp** out) {
     hipDeviceProp_t pr;
     if (hipGetDeviceProperties(&pr, c->device) == hipSuccess) cu = pr.multiProcessorCount;
     m->nb = (uint32_t)cu;
+    m->cu_count = (uint32_t)cu;
     int rc = RRH_OK;
     DevBuf* bufs[] = {&m->P, &m->m, &m->v, &m->grad};
     for (DevBuf* b : bufs) if (!rc) rc = b->ensure((size_t)NPARAM * 4);
     if (!rc) rc = m->pk.ensure((size_t)kPackElems * 2);
     if (!rc) rc = m->partial.ensure((size_t)m->nb * NPARAM * 4);
+    if (!rc) rc = m->part2.ensure((size_t)((m->nb + kGroup - 1) / kGroup) * NPARAM * 4);
     if (!rc) rc = m->ploss.ensure((size_t)m->nb * 4);
-    if (!rc) rc = m->scal.ensure(64);
+    if (!rc) rc = m->scal.ensure(512);
     if (rc) { rrh_mlp_destroy(m); return rc; }
     // Kaiming-uniform initialisation (bound sqrt(6 / fan_in)) from the seed; biases 0
     std::vector<float> P(NPARAM, 0.0f);
@@ -580,7 +750,7 @@ int rrh_mlp_create(rrh_ctx* c, const rrh_mlp_params* p, rrh_mlp** out) {
     hipError_t e = hipMemcpyAsync(m->P.p, P.data(), (size_t)NPARAM * 4, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(m->m.p, 0, (size_t)NPARAM * 4, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(m->v.p, 0, (size_t)NPARAM * 4, c->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(m->scal.p, 0, 64, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(m->scal.p, 0, 512, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(m->scal.p, sc, 16, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(m->state(), st, 8, hipMemcpyHostToDevice, c->stream);
     if (e != hipSuccess) { rrh_mlp_destroy(m); set_error(hipGetErrorString(e)); return RRH_E_HIP; }
@@ -594,7 +764,7 @@ int rrh_mlp_create(rrh_ctx* c, const rrh_mlp_params* p, rrh_mlp** out) {
 int rrh_mlp_destroy(rrh_mlp* m) {
     if (!m) return RRH_OK;
     (void)hipDeviceSynchronize();  // the owning context may already be gone
-    DevBuf* bufs[] = {&m->P, &m->m, &m->v, &m->grad, &m->pk, &m->partial, &m->ploss, &m->scal};
+    DevBuf* bufs[] = {&m->P, &m->m, &m->v, &m->grad, &m->pk, &m->partial, &m->part2, &m->ploss, &m->scal};
     for (DevBuf* b : bufs) b->release();
     delete m;
     return RRH_OK;
@@ -628,8 +798,7 @@ int rrh_mlp_infer(rrh_ctx* c, rrh_mlp* m, const float* pos, const float* nrm, co
     if (rc) return rc;
     float* dl = nullptr;
     RRH_HIP(hipMalloc(&dl, (size_t)n * NK * 5 * 4));
-    uint32_t g = (n + kBlockSamples - 1) / kBlockSamples;
-    if (g > 4096) g = 4096;
+    const uint32_t g = infer_grid(c, m, k_mlp_infer<HostSrc>, n);
     hipLaunchKernelGGL(k_mlp_infer<HostSrc>, dim3(g), dim3(256), 0, c->stream, m->dev(), src, n, dl);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(lobes, dl, (size_t)n * NK * 5 * 4, hipMemcpyDeviceToHost, c->stream);
@@ -647,9 +816,6 @@ int rrh_mlp_loss_grad(rrh_ctx* c, rrh_mlp* m, const float* pos, const float* nrm
     HostSrc src;
     int rc = upload_samples(c, pos, nrm, wo, dirs, w, n, T, src);
     if (rc) return rc;
-    RRH_HIP(hipMemsetAsync(m->wsum(), 0, 4, c->stream));
-    hipLaunchKernelGGL(k_mlp_wsum_host, dim3(64), dim3(256), 0, c->stream, src.w, n, m->wsum());
-    RRH_LAUNCH_CHECK();
     if ((rc = mlp_grad(c, m, src, n))) return rc;
     RRH_HIP(hipMemcpyAsync(grad, m->grad.p, (size_t)NPARAM * 4, hipMemcpyDeviceToHost, c->stream));
     RRH_HIP(hipMemcpyAsync(loss, m->loss(), 4, hipMemcpyDeviceToHost, c->stream));
@@ -666,9 +832,6 @@ int rrh_mlp_train_step(rrh_ctx* c, rrh_mlp* m, const float* pos, const float* nr
     HostSrc src;
     int rc = upload_samples(c, pos, nrm, wo, dirs, w, n, T, src);
     if (rc) return rc;
-    RRH_HIP(hipMemsetAsync(m->wsum(), 0, 4, c->stream));
-    hipLaunchKernelGGL(k_mlp_wsum_host, dim3(64), dim3(256), 0, c->stream, src.w, n, m->wsum());
-    RRH_LAUNCH_CHECK();
     if ((rc = mlp_grad(c, m, src, n))) return rc;
     if ((rc = mlp_update(c, m))) return rc;
     if (loss) RRH_HIP(hipMemcpyAsync(loss, m->loss(), 4, hipMemcpyDeviceToHost, c->stream));
@@ -681,9 +844,6 @@ int rrh_mlp_train_from_vertices(rrh_ctx* c, rrh_mlp* m, uint32_t batch, float* l
     RRH_ARG(c->train.p && c->train_cnt.p, "rrh_mlp_train_from_vertices: no recorded vertices (render with record_vertices)");
     RRH_ARG(batch > 0, "rrh_mlp_train_from_vertices: batch must be > 0");
     TrainSrc src{(const float4*)c->train.p, (const uint32_t*)c->train_cnt.p, c->train_capacity, batch};
-    RRH_HIP(hipMemsetAsync(m->wsum(), 0, 4, c->stream));
-    hipLaunchKernelGGL(k_mlp_wsum_train, dim3(64), dim3(256), 0, c->stream, src.tr, src.cnt, src.cap, batch, m->wsum());
-    RRH_LAUNCH_CHECK();
     int rc = mlp_grad(c, m, src, batch);
     if (rc) return rc;
     if ((rc = mlp_update(c, m))) return rc;
@@ -743,8 +903,7 @@ struct QueueSrc {  // hit points of the live ray queue (miss -> invalid)
 int mlp_infer_queue(rrh_ctx* c, rrh_mlp* m, const DevScene& S, const float4* qo, const float4* qd, const float4* hits,
                     const uint32_t* cnt, uint32_t b, uint32_t cap, float* lobes) {
     QueueSrc src{S, qo, qd, hits, cnt, b};
-    uint32_t g = (cap + kBlockSamples - 1) / kBlockSamples;
-    if (g > 2048) g = 2048;
+    const uint32_t g = infer_grid(c, m, k_mlp_infer<QueueSrc>, cap);
     hipLaunchKernelGGL(k_mlp_infer<QueueSrc>, dim3(g), dim3(256), 0, c->stream, m->dev(), src, cap, lobes);
     RRH_LAUNCH_CHECK();
     return RRH_OK;

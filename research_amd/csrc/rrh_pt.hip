@@ -227,16 +227,18 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, Q in, Q out, const
                     }
                     bool alive = !(pdf < 1e-6f);
                     f3 thp_before = thp;
+                    f3 bw = mk(0, 0, 0);
                     if (alive) {
                         float ndw = fmaxf(dot(nn, wi), 0.0f);
-                        thp = mul(thp, divs(scale(f, ndw), pdf));
+                        bw = divs(scale(f, ndw), pdf);
+                        thp = mul(thp, bw);
                         if (b >= A.rr_start) {
                             float survive = fminf(fmaxf(thp.x, fmaxf(thp.y, thp.z)), 0.95f);
                             if (rnd(seed) > survive) alive = false;
                             else thp = divs(thp, survive);
                         }
                     }
-                    if (A.rec.v) guide_record_vertex(A.rec, pixel, b, hi.pos, nn, wo, wi, pdf, thp_before,
+                    if (A.rec.v) guide_record_vertex(A.rec, pixel, b, hi.pos, nn, wo, wi, pdf, bw,
                                                      alive ? thp : mk(0, 0, 0), L, ne.shadow ? mul(thp_before, ne.c) : mk(0, 0, 0), hi.mat, alive);
                     if (alive && !last) {
                         push_ext = true;
@@ -400,13 +402,17 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
             c->prof_used += 2;
         }
         A.b = b;
-        if (A.guide.mlp_lobes) {
+        // the direction sampled at the last bounce is never traced (and its training record
+        // carries no radiance): the guide is skipped there
+        ShadeArgs Ab = A;
+        if (b + 1 >= p->max_bounces) Ab.guide_mode = RRH_GUIDE_NONE;
+        if (Ab.guide_mode == RRH_GUIDE_MLP) {
             rc = mlp_infer_queue(c, mlp, S, qi.o, qi.d, (const float4*)c->hits.p, cnt, b, npix,
                                  const_cast<float*>(A.guide.mlp_lobes));
             if (rc) return rc;
         }
         hipLaunchKernelGGL(k_shade, dim3(g), dim3(kBlock), 0, c->stream, S, qi, qo, (const float4*)c->hits.p, cnt,
-                           (float4*)c->sh_o.p, (float4*)c->sh_d.p, (float4*)c->sh_c.p, rad, A);
+                           (float4*)c->sh_o.p, (float4*)c->sh_d.p, (float4*)c->sh_c.p, rad, Ab);
         RRH_LAUNCH_CHECK();
         rc = (sk == RRH_STACK_LDS32) ? launch_shadow<RRH_STACK_LDS32>(c, S, cnt, b, g, rad, A.rec)
            : (sk == RRH_STACK_HYBRID16) ? launch_shadow<RRH_STACK_HYBRID16>(c, S, cnt, b, g, rad, A.rec)

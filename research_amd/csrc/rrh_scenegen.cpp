@@ -329,13 +329,14 @@ void heightfield(SceneData& s, uint32_t seed, float scale) {
 }
 
 // closed 10 x 10 x 4 room, tessellated walls, 0.5 x 0.5 window in the x = 0 wall,
-// random furniture boxes, lit by a directional sun through the window
+// furniture boxes (tables / cabinets with small objects on the floor and on their tops),
+// lit by a directional sun through the window.  scale 1 -> about 1.0M triangles.
 void indoor(SceneData& s, uint32_t seed, float scale) {
     Rng r(seed);
     const float X = 10.0f, Y = 4.0f, Z = 10.0f;
-    int nf = std::max(8, (int)std::lround(320.0f * scale));       // cells along 10 m
-    int nh = std::max(4, (int)std::lround(128.0f * scale));       // cells along 4 m
-    int nbox = std::max(8, (int)std::lround(20000.0f * scale));
+    int nf = std::max(8, (int)std::lround(370.0f * scale));       // cells along 10 m
+    int nh = std::max(4, (int)std::lround(148.0f * scale));       // cells along 4 m
+    int nbox = std::max(16, (int)std::lround(1500.0f * scale));
     auto lam = [&](float lo, float hi) {
         s.mats.push_back(material(r.range(lo, hi), r.range(lo, hi), r.range(lo, hi), 0.0f, 1.0f, 0, 0, 0, RRH_MAT_PURE_LAMBERT));
         return (uint32_t)s.mats.size() - 1;
@@ -376,13 +377,26 @@ void indoor(SceneData& s, uint32_t seed, float scale) {
     for (int k = 0; k < nm; ++k) lam(0.1f, 0.9f);
     std::vector<std::vector<rrh_vertex>> gv(nm);
     std::vector<std::vector<uint32_t>> gi(nm);
+    struct Top { float x0, z0, x1, z1, y; };
+    std::vector<Top> tops;
     for (int b = 0; b < nbox; ++b) {
         bool big = b < 12;
-        float sx = big ? r.range(0.6f, 2.0f) : r.range(0.05f, 0.35f);
-        float sz = big ? r.range(0.6f, 2.0f) : r.range(0.05f, 0.35f);
-        float sy = big ? r.range(0.4f, 1.8f) : r.range(0.05f, 0.5f);
-        float x = r.range(0.6f, X - 0.6f - sx), z = r.range(0.3f, Z - 0.3f - sz);
-        float y = big ? 0.0f : (r.uf() < 0.7f ? 0.0f : r.range(0.0f, 2.5f));
+        float sx = big ? r.range(0.6f, 2.0f) : r.range(0.05f, 0.3f);
+        float sz = big ? r.range(0.6f, 2.0f) : r.range(0.05f, 0.3f);
+        float sy = big ? r.range(0.4f, 1.2f) : r.range(0.05f, 0.4f);
+        float x, z, y = 0.0f;
+        if (!big && !tops.empty() && r.uf() < 0.3f) {  // on a table top
+            const Top& t = tops[r.next() % (uint32_t)tops.size()];
+            sx = std::min(sx, t.x1 - t.x0);
+            sz = std::min(sz, t.z1 - t.z0);
+            x = r.range(t.x0, t.x1 - sx);
+            z = r.range(t.z0, t.z1 - sz);
+            y = t.y;
+        } else {
+            x = r.range(0.6f, X - 0.6f - sx);
+            z = r.range(0.3f, Z - 0.3f - sz);
+        }
+        if (big) tops.push_back(Top{x, z, x + sx, z + sz, sy});
         uint32_t m = r.next() % (uint32_t)nm;
         std::vector<rrh_vertex>& v = gv[m];
         std::vector<uint32_t>& i = gi[m];
@@ -398,7 +412,7 @@ void indoor(SceneData& s, uint32_t seed, float scale) {
     std::memset(&sun, 0, sizeof(sun));
     sun.type = 1;
     sun.direction[0] = 0.8f; sun.direction[1] = -0.45f; sun.direction[2] = 0.12f;  // travels into the room
-    sun.emission[0] = 10.0f; sun.emission[1] = 9.0f; sun.emission[2] = 7.5f;
+    sun.emission[0] = 40.0f; sun.emission[1] = 36.0f; sun.emission[2] = 30.0f;
     sun.intensity = 1.0f;
     s.lights.push_back(sun);
     s.flags = 0;

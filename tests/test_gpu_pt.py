@@ -27,6 +27,7 @@ CASES = [
     ("cornell_random", L.GEN_CORNELL_RANDOM, 1, 1.0, 64, 64, 8),
     ("heightfield", L.GEN_HEIGHTFIELD, 2, 0.1, 64, 36, 8),
     ("indoor", L.GEN_INDOOR, 4, 0.1, 48, 32, 12),
+    ("indoor_full", L.GEN_INDOOR, 1, 1.0, 40, 24, 12),
 ]
 
 
