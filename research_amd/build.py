@@ -1,6 +1,7 @@
-"""Build librrh.so (HIP, gfx950) in-tree with hipcc.
+"""Build in-tree: librrh.so (HIP, gfx950, hipcc), librrh_host.so (the C++ pass classes and
+Application over the C ABI, g++) and the Python module _app (pybind11, g++).
 
-Every translation unit is compiled with -ffp-contract=off so the shading arithmetic
+Every HIP translation unit is compiled with -ffp-contract=off so the shading arithmetic
 rounds exactly as written, and with packed-FP32 instructions disabled.
 Usage: python -m research_amd.build [--jobs N]
 """
@@ -16,8 +17,13 @@ from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
+HOST = HERE / "host"
+INCLUDE = HERE.parent / "include"
 OUT = HERE / "librrh.so"
+HOST_OUT = HERE / "librrh_host.so"
 OBJ = HERE / "_build"
+HOST_SOURCES = ["render.cpp", "passes.cpp", "application.cpp"]
+CXX = os.environ.get("CXX", "g++")
 
 SOURCES = [
     "rrh_core.hip",
@@ -69,6 +75,35 @@ def build(jobs: int = 8, verbose: bool = False) -> Path:
     return OUT
 
 
+def _run(cmd: list[str], what: str) -> None:
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{what} failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+
+
+def build_host(verbose: bool = False) -> Path:
+    """C++ host layer over librrh.so (no device code: g++), then the pybind11 module."""
+    import sysconfig
+
+    import pybind11
+
+    flags = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", f"-I{HOST}"]
+    rpath = "-Wl,-rpath,$ORIGIN"
+    tmp = HOST_OUT.with_suffix(".so.tmp")
+    _run([CXX, "-shared"] + flags + [str(HOST / f) for f in HOST_SOURCES] +
+         [f"-L{HERE}", "-lrrh", rpath, "-o", str(tmp)], "host library build")
+    os.replace(tmp, HOST_OUT)
+    ext = HERE / ("_app" + sysconfig.get_config_var("EXT_SUFFIX"))
+    tmp = ext.with_suffix(".tmp")
+    _run([CXX, "-shared"] + flags + [f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}",
+                                     str(HOST / "bindings.cpp"), f"-L{HERE}", "-lrrh_host", "-lrrh", rpath,
+                                     "-o", str(tmp)], "python module build")
+    os.replace(tmp, ext)
+    if verbose:
+        print(f"built {HOST_OUT} and {ext}")
+    return ext
+
+
 def clean() -> None:
     shutil.rmtree(OBJ, ignore_errors=True)
 
@@ -79,6 +114,7 @@ if __name__ == "__main__":
     a = ap.parse_args()
     try:
         build(a.jobs, verbose=True)
+        build_host(verbose=True)
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

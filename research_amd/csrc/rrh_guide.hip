@@ -235,17 +235,25 @@ __global__ void k_finalize(const float4* __restrict__ rec, uint32_t nrec, uint32
                 float4 r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5], r6 = r[6], r7 = r[7];
                 uint32_t pixel = i / max_bounces;
                 float4 Lf = rad[pixel];
-                f3 Ls = sub(sub(xyz(Lf), xyz(r4)), (r3.w > 0.5f) ? xyz(r5) : mk(0, 0, 0));
+                const f3 nee = (r3.w > 0.5f) ? xyz(r5) : mk(0, 0, 0);
+                f3 Ls = sub(sub(xyz(Lf), xyz(r4)), nee);
+                // radiance gathered after this vertex, by subtraction: a remainder within the
+                // rounding of the sums is zero (else dividing by a small throughput inflates it)
+                const float tol = 1e-5f * (fabsf(Lf.x) + fabsf(Lf.y) + fabsf(Lf.z) + fabsf(r4.x) + fabsf(r4.y) +
+                                           fabsf(r4.z) + fabsf(nee.x) + fabsf(nee.y) + fabsf(nee.z));
+                Ls = mk(Ls.x > tol ? Ls.x : 0.0f, Ls.y > tol ? Ls.y : 0.0f, Ls.z > tol ? Ls.z : 0.0f);
                 f3 tn = xyz(r6);
                 f3 Li = mk(tn.x > 0.0f ? Ls.x / tn.x : 0.0f, tn.y > 0.0f ? Ls.y / tn.y : 0.0f, tn.z > 0.0f ? Ls.z / tn.z : 0.0f);
                 float w = fmaxf(lum709(Li), 0.0f);
-                if (r1.w > 0.0f && w > 0.0f && isfinite(w)) {
+                const float wl = (r1.w > 0.0f) ? w / r1.w : 0.0f;
+                if (wl > 0.0f && isfinite(wl)) {
                     take = true;
-                    t0 = make_float4(r0.x, r0.y, r0.z, w / r1.w);
+                    t0 = make_float4(r0.x, r0.y, r0.z, wl);
                     t1 = make_float4(r3.x, r3.y, r3.z, r1.w);
                     // product weight: Li f cos / pdf, the one-sample estimate of the integrand
                     f3 bw = xyz(r7);
-                    t2 = make_float4(r1.x, r1.y, r1.z, fmaxf(lum709(mk(Li.x * bw.x, Li.y * bw.y, Li.z * bw.z)), 0.0f));
+                    const float wp = fmaxf(lum709(mk(Li.x * bw.x, Li.y * bw.y, Li.z * bw.z)), 0.0f);
+                    t2 = make_float4(r1.x, r1.y, r1.z, isfinite(wp) ? wp : 0.0f);
                     t3 = make_float4(r2.x, r2.y, r2.z, 0.0f);
                 }
             }

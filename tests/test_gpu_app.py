@@ -1,0 +1,139 @@
+"""The headless Application and its passes on the GPU: each display mode must produce exactly
+what the C ABI calls it wraps produce on the same inputs (same kernels, same seeds), and the
+screenshot must be the tonemapped display image."""
+import ctypes as C
+import struct
+import zlib
+
+import numpy as np
+import pytest
+
+from research_amd import _app
+from research_amd import _lib as L
+from research_amd import core
+
+pytestmark = pytest.mark.gpu
+
+W, H = 64, 48
+
+
+@pytest.fixture()
+def app():
+    a = _app.Application()
+    a.set_window_size(W, H)
+    a.initialize()
+    yield a
+    a.shutdown()
+    del a
+
+
+def test_one_initialized_application_at_a_time(app):
+    other = _app.Application()
+    with pytest.raises(RuntimeError):
+        other.initialize()
+    del other
+
+
+def _direct_setup(ctx, app, kind=L.GEN_CORNELL_REF, seed=1, scale=1.0):
+    sa = core.generate_scene(kind, seed, scale)
+    gs = core.Scene(ctx, sa)
+    eye = np.float32(app.camera_position)
+    center = eye + np.float32(app.camera_forward)  # the view the Camera builds from yaw / pitch
+    return sa, gs, tuple(float(x) for x in eye), tuple(float(x) for x in center)
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def test_path_tracer_mode_equals_direct_calls(ctx, app):
+    frames = 4
+    for _ in range(frames):
+        app.step()
+    assert app.scene_name == "cornell_box" and app.accumulated_spp == frames
+    got = app.display_image()
+    sa, gs, eye, center = _direct_setup(ctx, app)
+    rad, acc = core.Image(ctx, W, H), core.Image(ctx, W, H)
+    for f in range(frames):
+        gs.set_camera(core.camera_look_at(eye, center, sa.fov, W, H, frame_index=f))
+        L.call("rrh_pt_render", ctx.h, gs.h, C.byref(L.PTParams(5, f, 3, 0, 0.0, 0)), rad.h)
+        L.call("rrh_accumulate", ctx.h, rad.h, acc.h, 1 if f == 0 else 0, f)
+    assert np.array_equal(_bits(got), _bits(acc.download()))
+    # moving the camera restarts the accumulation
+    app.set_camera_look_at((0.0, 0.2, 3.0), (0.0, 0.0, 0.0))
+    app.step()
+    assert app.accumulated_spp == 1
+
+
+def test_restir_di_and_denoise_modes_equal_direct_calls(ctx, app):
+    app.set_pass_enabled("restir_di", True)
+    di = app.restir_di_pass
+    di.num_initial_candidates, di.spatial_num_neighbors, di.spatial_radius = 32, 5, 30
+    frames = 3
+    for _ in range(frames):
+        app.step()
+    got_di = app.display_image()
+    app.set_pass_enabled("denoise", True)  # a mode switch resets the DI history
+    for _ in range(2):
+        app.step()
+    got_dn = app.display_image()
+
+    sa, gs, eye, center = _direct_setup(ctx, app)
+    im = lambda fmt=L.FMT_RGBA32F: core.Image(ctx, W, H, fmt)
+    gp, gn, out = im(), im(), im()
+    res, uv = [im(), im()], [im(L.FMT_R32U), im(L.FMT_R32U)]
+
+    def di_frames(n, first_frame_index):
+        for r in res + uv:
+            r.clear()
+        for f in range(n):
+            gs.set_camera(core.camera_look_at(eye, center, sa.fov, W, H, frame_index=first_frame_index + f))
+            cur, prev = f & 1, 1 - (f & 1)
+            prm = L.DIParams(f, 32, 1, 1, 5, 30)
+            L.call("rrh_di_initial", ctx.h, gs.h, C.byref(prm), gp.h, gn.h, res[cur].h, uv[cur].h)
+            L.call("rrh_di_temporal", ctx.h, gs.h, C.byref(prm), gp.h, gn.h, res[cur].h, uv[cur].h, res[prev].h,
+                   uv[prev].h)
+            L.call("rrh_di_spatial_shade", ctx.h, gs.h, C.byref(prm), gp.h, gn.h, res[cur].h, uv[cur].h, out.h)
+        return out.download()
+
+    assert np.array_equal(_bits(got_di), _bits(di_frames(frames, 0)))
+    di_out = di_frames(2, frames)
+    t0, t1 = im(), im()
+    slot = C.c_int()
+    prm = L.AtrousParams(5, 0, 0.25, 0.20, 1.0)
+    L.call("rrh_atrous", ctx.h, C.byref(prm), out.h, gp.h, gn.h, t0.h, t1.h, C.byref(slot))
+    ref = (t0, t1)[slot.value].download()
+    assert np.isfinite(di_out).all()
+    assert np.array_equal(_bits(got_dn), _bits(ref))
+
+
+@pytest.mark.parametrize("mode", ["vmm", "mlp"])
+def test_guiding_mode_runs_and_trains(app, mode):
+    app.load_procedural(L.GEN_CORNELL_RANDOM, 1, 1.0)
+    app.set_pass_enabled("guiding", True)
+    g = app.guiding_pass
+    g.mode = mode
+    g.mlp_batch = 4096
+    g.record_vertices = 20000
+    for _ in range(6):
+        app.step()
+    img = app.display_image()
+    assert app.scene_name == "cornell_random" and app.accumulated_spp == 6
+    assert np.isfinite(img).all() and img[..., :3].mean() > 0
+    if mode == "mlp":
+        assert np.isfinite(g.last_loss) and g.last_loss > 0
+
+
+def test_screenshot_is_the_tonemapped_display(app, tmp_path):
+    for _ in range(2):
+        app.step()
+    app.set_exposure(1.5)
+    p = tmp_path / "shot.png"
+    app.capture_screenshot(str(p))
+    data = p.read_bytes()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    n, = struct.unpack(">I", data[33:37])
+    assert data[37:41] == b"IDAT"
+    raw = np.frombuffer(zlib.decompress(data[41:41 + n]), np.uint8).reshape(H, 1 + 4 * W)[:, 1:].reshape(H, W, 4)
+    assert np.array_equal(raw, _app.tonemap_ldr(app.display_image(), 1.5))
+    assert np.array_equal(raw, app.display_ldr())

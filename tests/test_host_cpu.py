@@ -1,0 +1,89 @@
+"""The host layer's Python module without a GPU: the reference Application API surface,
+argument errors, the screenshot tonemap and the PNG writer."""
+import struct
+import zlib
+
+import numpy as np
+import pytest
+
+from research_amd import _app
+
+# src/python/bindings.cpp:28-100
+REFERENCE_API = ["initialize", "shutdown", "step", "should_close", "run_until_closed", "load_cornell", "load_gltf",
+                 "scene_name", "set_camera_look_at", "camera_position", "camera_forward", "set_pass_enabled",
+                 "set_exposure", "exposure", "set_input_enabled", "input_enabled", "capture_screenshot", "window_size"]
+
+
+def test_application_exposes_the_reference_api():
+    missing = [n for n in REFERENCE_API if not hasattr(_app.Application, n)]
+    assert not missing
+
+
+def test_application_state_before_initialize():
+    app = _app.Application()
+    try:
+        assert app.should_close()  # no device yet, like a missing window
+        assert app.window_size == (1600, 900)
+        assert app.exposure == 1.0 and app.input_enabled
+        app.set_exposure(2.0)
+        app.set_input_enabled(False)
+        assert app.exposure == 2.0 and not app.input_enabled
+        with pytest.raises(RuntimeError):
+            app.step()
+        with pytest.raises(ValueError):
+            app.set_pass_enabled("restir_pt", True)
+        with pytest.raises(ValueError):
+            app.set_pass_enabled("tonemap", True)
+        app.set_pass_enabled("restir_di", True)
+        app.set_pass_enabled("denoise", True)
+        with pytest.raises(RuntimeError):
+            app.load_gltf("scene.glb")
+        with pytest.raises(RuntimeError):  # headless: no window to close
+            app.run_until_closed()
+    finally:
+        del app
+
+
+def test_tonemap_known_points():
+    hdr = np.zeros((1, 4, 4), np.float32)
+    hdr[0, 1, :3] = 1e6   # ACES saturates at 2.51 / 2.43 > 1 -> clamped to 1
+    hdr[0, 2, :3] = 1e-4  # the curve starts flat: (0.03 v) / 0.14
+    hdr[0, 3, :3] = [0.18, 0.5, 2.0]
+    ldr = _app.tonemap_ldr(hdr, 1.0)
+    assert ldr.dtype == np.uint8 and ldr.shape == hdr.shape
+    assert (ldr[0, 0, :3] == 0).all() and (ldr[0, 1, :3] == 255).all()
+    assert (ldr[0, 2, :3] == round(255 * (1e-4 * 0.03 / 0.14) ** (1 / 2.2))).all()
+    assert (ldr[..., 3] == 255).all()
+    assert ldr[0, 3, 0] < ldr[0, 3, 1] < ldr[0, 3, 2]  # monotone
+    # exposure scales before the curve
+    assert (_app.tonemap_ldr(hdr * 2, 1.0) == _app.tonemap_ldr(hdr, 2.0)).all()
+
+
+def _read_png(path):
+    data = open(path, "rb").read()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, chunks = 8, {}
+    while pos < len(data):
+        n, = struct.unpack(">I", data[pos:pos + 4])
+        typ, body = data[pos + 4:pos + 8], data[pos + 8:pos + 8 + n]
+        crc, = struct.unpack(">I", data[pos + 8 + n:pos + 12 + n])
+        assert crc == zlib.crc32(typ + body) & 0xFFFFFFFF, typ
+        chunks.setdefault(typ, b"")
+        chunks[typ] += body
+        pos += 12 + n
+    w, h, depth, ctype = struct.unpack(">IIBB", chunks[b"IHDR"][:10])
+    raw = zlib.decompress(chunks[b"IDAT"])  # checks the Adler-32 too
+    rows = np.frombuffer(raw, np.uint8).reshape(h, 1 + 4 * w)
+    assert (rows[:, 0] == 0).all()
+    return (w, h, depth, ctype), rows[:, 1:].reshape(h, w, 4)
+
+
+@pytest.mark.parametrize("h,w", [(1, 1), (3, 5), (200, 130)])  # 200x130 spans several stored blocks
+def test_png_writer_round_trip(tmp_path, h, w):
+    rng = np.random.default_rng(h * w)
+    img = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+    p = str(tmp_path / "x.png")
+    _app.write_png(p, img)
+    hdr, back = _read_png(p)
+    assert hdr == (w, h, 8, 6)
+    assert np.array_equal(back, img)
