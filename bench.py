@@ -30,7 +30,9 @@ from research_amd import core  # noqa: E402
 
 METRIC = "Mrays/s (1080p, 1M tris); equal-time rMSE learned guide vs vMF grid"
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES, TRI_BYTES, RAY_BYTES, HIT_BYTES = 64, 48, 32, 16
+NODE_FORMATS = {"binary": 0, "wide4": 1, "wide4q": 2}
+NODE_BYTES = {"binary": 64, "wide4": 128, "wide4q": 64}
+TRI_BYTES, RAY_BYTES, HIT_BYTES = 48, 32, 16
 
 
 def orbit_camera(sa, frame: int, W: int, H: int):
@@ -128,6 +130,9 @@ def main() -> int:
     ap.add_argument("--stack", choices=["private", "lds32", "hybrid16"], default="lds32",
                     help="where the traversal stack lives (context option)")
     ap.add_argument("--xcd-remap", type=int, default=0, help="1: contiguous work range per XCD")
+    ap.add_argument("--persistent", type=int, default=0, help="1: persistent queue traversal with lane refill")
+    ap.add_argument("--bvh-nodes", choices=list(NODE_FORMATS), default="binary",
+                    help="BVH node format traversed (binary 64 B, 4-wide 128 B, quantized 4-wide 64 B)")
     ap.add_argument("--guide-seconds", type=float, default=4.0,
                     help="equal-time guiding comparison: wall seconds per method (0: skip)")
     ap.add_argument("--guide-ref-spp", type=int, default=8192, help="spp of the FP64 reference image")
@@ -145,6 +150,8 @@ def main() -> int:
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAVERSAL_STACK,
            {"private": L.STACK_PRIVATE, "lds32": L.STACK_LDS32, "hybrid16": L.STACK_HYBRID16}[a.stack])
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_XCD_REMAP, a.xcd_remap)
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_NODES, NODE_FORMATS[a.bvh_nodes])
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_PERSISTENT, a.persistent)
     if a.copy_calibration:
         g = C.c_float()
         L.call("rrh_stream_copy", ctx.h, 1 << 30, 5, C.byref(g))
@@ -169,7 +176,7 @@ def main() -> int:
     fr.run(a.warmup, L.PT_COUNT_FETCHES)
     nn, nt, nr = C.c_uint64(), C.c_uint64(), C.c_uint64()
     L.call("rrh_fetch_counts", ctx.h, C.byref(nn), C.byref(nt), C.byref(nr), 1)
-    bytes_per_ray = (nn.value * NODE_BYTES + nt.value * TRI_BYTES) / max(nr.value, 1) + RAY_BYTES + HIT_BYTES
+    bytes_per_ray = (nn.value * NODE_BYTES[a.bvh_nodes] + nt.value * TRI_BYTES) / max(nr.value, 1) + RAY_BYTES + HIT_BYTES
     print(f"[bench] traversal: {nn.value / max(nr.value, 1):.1f} nodes + {nt.value / max(nr.value, 1):.1f} tris "
           f"per ray -> {bytes_per_ray:.0f} algorithmic B/ray", file=sys.stderr, flush=True)
     ctx.synchronize()
@@ -218,7 +225,7 @@ def main() -> int:
                    "width": W, "height": H, "spp_per_frame": 1, "max_bounces": a.max_bounces,
                    "di_candidates": 32, "di_temporal_mcap": "20*M", "di_spatial": "5 neighbours, r=30",
                    "camera": "orbit 0.5 deg/frame", "parallelism": "single GPU",
-                   "traversal_stack": a.stack, "xcd_remap": a.xcd_remap,
+                   "traversal_stack": a.stack, "xcd_remap": a.xcd_remap, "bvh_nodes": a.bvh_nodes, "persistent": a.persistent,
                    "rays_per_step": {"primary": prim // a.steps, "extension": ext // a.steps,
                                      "shadow": shadow // a.steps}},
         "roofline": {"bound": "hbm", "kernel": "k_intersect (closest-hit BVH traversal)",

@@ -79,6 +79,8 @@ typedef struct {
     uint32_t tri_count, node_count, bvh_depth, leaf_count;
     uint32_t light_count;          /* reference lights + triangle lights */
     uint32_t tri_light_count;
+    uint32_t node4_count;          /* 4-wide nodes (128 B each) of the same tree */
+    uint32_t stack4;               /* traversal stack entries the 4-wide tree can need */
     uint64_t device_bytes;         /* nodes + triangles + shading records */
     float    build_ms;
     float    bounds_min[3], bounds_max[3];
@@ -176,8 +178,14 @@ int rrh_fetch_counts(rrh_ctx* ctx, uint64_t* node_fetches, uint64_t* tri_fetches
 /* tuning options of a context (no reference counterpart) */
 #define RRH_OPT_TRAVERSAL_STACK 1   /* RRH_STACK_* : where the BVH traversal stack lives */
 #define RRH_OPT_XCD_REMAP 2         /* 1: give each XCD a contiguous range of work items */
+#define RRH_OPT_PERSISTENT 4        /* 1: queue traversal by persistent waves that refill finished
+                                       lanes with new rays (binary nodes); 0 (default): grid-stride */
+#define RRH_OPT_BVH_NODES 3         /* RRH_NODES_*: the node format traversal reads (all share the leaves) */
+#define RRH_NODES_BINARY 0          /* binary, 64 B (both child boxes in fp32); default */
+#define RRH_NODES_WIDE4 1           /* 4-wide, 128 B (fp32 child boxes) */
+#define RRH_NODES_WIDE4Q 2          /* 4-wide, 64 B (8-bit child boxes, outward rounded) */
 #define RRH_STACK_PRIVATE 0         /* per-lane private (scratch) array, any depth */
-#define RRH_STACK_LDS32 1           /* 32 entries per lane in LDS (BVH depth <= 32, else private) */
+#define RRH_STACK_LDS32 1           /* 32 entries per lane in LDS (when the tree can need more: hybrid16) */
 #define RRH_STACK_HYBRID16 2        /* 16 entries in LDS + private overflow, any depth */
 int rrh_ctx_set_option(rrh_ctx* ctx, int option, int value);
 /* plain device-to-device copy kernel (no reference counterpart): measures the achievable

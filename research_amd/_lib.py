@@ -36,7 +36,8 @@ FMT_RGBA32F, FMT_RGBA16F, FMT_RGBA64F, FMT_R32U = 0, 1, 2, 3
 FMT_BYTES = {FMT_RGBA32F: 16, FMT_RGBA16F: 8, FMT_RGBA64F: 32, FMT_R32U: 4}
 GUIDE_NONE, GUIDE_VMM, GUIDE_MLP = 0, 1, 2
 PT_COUNT_FETCHES = 1
-OPT_TRAVERSAL_STACK, OPT_XCD_REMAP = 1, 2
+OPT_TRAVERSAL_STACK, OPT_XCD_REMAP, OPT_BVH_NODES, OPT_PERSISTENT = 1, 2, 3, 4
+NODES_BINARY, NODES_WIDE4, NODES_WIDE4Q = 0, 1, 2
 STACK_PRIVATE, STACK_LDS32, STACK_HYBRID16 = 0, 1, 2
 GRID_HASH, GRID_VMM = 0, 1
 ATROUS_REFERENCE, ATROUS_CONFIG = 0, 1
@@ -52,7 +53,7 @@ class Camera(C.Structure):
 class SceneInfo(C.Structure):
     _fields_ = [("tri_count", C.c_uint32), ("node_count", C.c_uint32), ("bvh_depth", C.c_uint32),
                 ("leaf_count", C.c_uint32), ("light_count", C.c_uint32), ("tri_light_count", C.c_uint32),
-                ("device_bytes", C.c_uint64), ("build_ms", C.c_float), ("bounds_min", C.c_float * 3),
+                ("node4_count", C.c_uint32), ("stack4", C.c_uint32), ("device_bytes", C.c_uint64), ("build_ms", C.c_float), ("bounds_min", C.c_float * 3),
                 ("bounds_max", C.c_float * 3)]
 
 

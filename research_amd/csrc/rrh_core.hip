@@ -39,9 +39,15 @@ size_t format_bytes(int fmt) {
 
 using namespace rrh;
 
+int rrh_scene::node_format() const { return ctx ? ctx->bvh_nodes : RRH_NODES_BINARY; }
+uint32_t rrh_scene::stack_need() const { return node_format() == RRH_NODES_BINARY ? depth : stack4; }
+
 rrh::DevScene rrh_scene::dev() const {
     DevScene d;
     d.nodes = (const float4*)nodes.p;
+    d.nodes4 = (const float4*)nodes4.p;
+    d.nodesq = (const uint4*)nodesq.p;
+    d.node_fmt = (uint32_t)node_format();
     d.tris = (const float4*)tris.p;
     d.tri_ref = (const uint2*)tri_ref.p;
     d.tri_light = (const uint32_t*)tri_light.p;
@@ -78,6 +84,10 @@ int rrh_ctx_create(int device, rrh_ctx** out) {
     RRH_HIP(hipSetDevice(device));
     rrh_ctx* c = new rrh_ctx();
     c->device = device;
+    {
+        hipDeviceProp_t pr;
+        if (hipGetDeviceProperties(&pr, device) == hipSuccess && pr.multiProcessorCount > 0) c->cu_count = pr.multiProcessorCount;
+    }
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; set_error(hipGetErrorString(e)); return RRH_E_HIP; }
     for (int i = 0; i < 16; ++i) {
@@ -127,6 +137,12 @@ int rrh_ctx_set_option(rrh_ctx* c, int option, int value) {
         return RRH_OK;
     }
     if (option == RRH_OPT_XCD_REMAP) { c->xcd_remap = value != 0; return RRH_OK; }
+    if (option == RRH_OPT_PERSISTENT) { c->persistent = value != 0; return RRH_OK; }
+    if (option == RRH_OPT_BVH_NODES) {
+        RRH_ARG(value >= RRH_NODES_BINARY && value <= RRH_NODES_WIDE4Q, "rrh_ctx_set_option: bad node format");
+        c->bvh_nodes = value;
+        return RRH_OK;
+    }
     set_error("rrh_ctx_set_option: unknown option");
     return RRH_E_ARG;
 }
@@ -385,9 +401,14 @@ int rrh_scene_upload(rrh_ctx* c, const rrh_vertex* verts, uint32_t nv, const uin
     s->nnodes = (uint32_t)(bvh.nodes.size() / 4);
     s->depth = bvh.depth;
     s->leaves = bvh.leaves;
+    s->nnodes4 = (uint32_t)(bvh.nodes4.size() / 8);
+    s->depth4 = bvh.depth4;
+    s->stack4 = bvh.stack4;
     hipStream_t st = c->stream;
     int rc = RRH_OK;
     if (!rc) rc = upload_vec(s->nodes, bvh.nodes.data(), bvh.nodes.size(), st);
+    if (!rc) rc = upload_vec(s->nodes4, bvh.nodes4.data(), bvh.nodes4.size(), st);
+    if (!rc) rc = upload_vec(s->nodesq, bvh.nodesq.data(), bvh.nodesq.size(), st);
     if (!rc) rc = upload_vec(s->tris, bvh.tris.data(), bvh.tris.size(), st);
     if (!rc) rc = upload_vec(s->tri_ref, tri_ref.data(), tri_ref.size(), st);
     if (!rc) rc = upload_vec(s->tri_light, tri_light.data(), tri_light.size(), st);
@@ -412,7 +433,7 @@ int rrh_scene_upload(rrh_ctx* c, const rrh_vertex* verts, uint32_t nv, const uin
 int rrh_scene_destroy(rrh_scene* s) {
     if (!s) return RRH_OK;
     (void)hipDeviceSynchronize();  // the owning context may already be gone
-    DevBuf* bufs[] = {&s->nodes, &s->tris, &s->tri_ref, &s->tri_light, &s->tl, &s->verts, &s->idx,
+    DevBuf* bufs[] = {&s->nodes, &s->nodes4, &s->nodesq, &s->tris, &s->tri_ref, &s->tri_light, &s->tl, &s->verts, &s->idx,
                       &s->meshes, &s->mats, &s->insts, &s->lights, &s->cam};
     for (DevBuf* b : bufs) b->release();
     delete s;
@@ -428,7 +449,10 @@ int rrh_scene_info_get(const rrh_scene* s, rrh_scene_info* o) {
     o->leaf_count = s->leaves;
     o->light_count = s->nl + s->ntl;
     o->tri_light_count = s->ntl;
-    o->device_bytes = (uint64_t)s->nnodes * 64 + (uint64_t)s->ntri * 48 + (uint64_t)s->ntri * 12 + (uint64_t)s->ntl * 64;
+    o->node4_count = s->nnodes4;
+    o->stack4 = s->stack4;
+    o->device_bytes = (uint64_t)s->nnodes * 64 + (uint64_t)s->nnodes4 * 192 + (uint64_t)s->ntri * 48 +
+                      (uint64_t)s->ntri * 12 + (uint64_t)s->ntl * 64;
     o->build_ms = s->build_ms;
     for (int k = 0; k < 3; ++k) { o->bounds_min[k] = s->bmin[k]; o->bounds_max[k] = s->bmax[k]; }
     return RRH_OK;

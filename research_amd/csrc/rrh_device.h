@@ -125,6 +125,9 @@ RRH_D f3 sky(f3 d) {
 // ---------------------------------------------------------------------------
 struct DevScene {
     const float4* nodes;       // 4 float4 per binary node (child boxes + child links)
+    const float4* nodes4;      // 8 float4 per 4-wide node (the same tree collapsed, same leaves)
+    const uint4* nodesq;       // 4 uint4 per quantized 4-wide node (same links as nodes4)
+    uint32_t node_fmt;         // RRH_NODES_*: which of the three the traversal reads
     const float4* tris;        // 3 float4 per triangle, leaf order; tris[3i].w = flattened id bits
     const uint2* tri_ref;      // per flattened id: (instance, primitive within mesh)
     const uint32_t* tri_light; // per flattened id: light index or ~0u
@@ -255,8 +258,8 @@ RRH_D typename StackSel<K>::T make_stack(int* smem) {
 // (ties on t resolve to the smaller flattened id).  COUNT = true is the
 // instrumented variant that counts node and triangle fetches.
 template <bool COUNT, class Stk>
-RRH_D Hit trace_closest_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack,
-                          uint32_t* n_nodes = nullptr, uint32_t* n_tris = nullptr) {
+RRH_D Hit trace_closest2_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack,
+                           uint32_t* n_nodes, uint32_t* n_tris) {
     Hit h; h.t = tmax; h.prim = ~0u; h.b1 = 0.0f; h.b2 = 0.0f;
     WRay r = wray_setup(o, d);
     int sp = 0;
@@ -302,7 +305,7 @@ RRH_D Hit trace_closest_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax,
 }
 
 template <class Stk>
-RRH_D bool trace_occluded_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack) {
+RRH_D bool trace_occluded2_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack) {
     if (!(tmax >= tmin)) return false;
     WRay r = wray_setup(o, d);
     int sp = 0;
@@ -337,6 +340,266 @@ RRH_D bool trace_occluded_s(const DevScene& S, f3 o, f3 d, float tmin, float tma
         if (sp == 0) return false;
         node = stack.get(--sp);
     }
+}
+
+// ---- stepwise binary traversal for persistent waves ------------------------------------------
+// One call runs the inner nodes down to the next leaf and tests that leaf (the while-while
+// body); it returns true once the ray is finished.  The state lives in registers between
+// calls, so a wave can hand a finished lane a new ray while the others keep going.
+struct TravState {
+    WRay r;
+    float tmin, t;      // t: closest hit so far (closest) / the ray's extent (occluded)
+    uint32_t prim;
+    float b1, b2;
+    int node, sp;
+};
+RRH_D void trav_begin(TravState& T, f3 o, f3 d, float tmin, float tmax) {
+    T.r = wray_setup(o, d);
+    T.tmin = tmin; T.t = tmax; T.prim = ~0u; T.b1 = 0.0f; T.b2 = 0.0f; T.node = 0; T.sp = 0;
+}
+template <bool COUNT, class Stk>
+RRH_D bool closest_step(const DevScene& S, TravState& T, Stk& stack, uint32_t& n_nodes, uint32_t& n_tris) {
+    while (T.node >= 0) {
+        const float4* n = S.nodes + (size_t)T.node * 4;
+        if constexpr (COUNT) n_nodes++;
+        float4 a = n[0], b = n[1], c = n[2];
+        int4 lk = *reinterpret_cast<const int4*>(n + 3);
+        float tl, tr;
+        bool hl = box_hit(a.x, a.y, a.z, a.w, c.x, c.y, T.r, T.tmin, T.t, tl);
+        bool hr = box_hit(b.x, b.y, b.z, b.w, c.z, c.w, T.r, T.tmin, T.t, tr);
+        if (hl && hr) {
+            int nearc = lk.x, farc = lk.y;
+            if (tr < tl) { nearc = lk.y; farc = lk.x; }
+            stack.put(T.sp++, farc);
+            T.node = nearc;
+        } else if (hl) {
+            T.node = lk.x;
+        } else if (hr) {
+            T.node = lk.y;
+        } else {
+            if (T.sp == 0) return true;
+            T.node = stack.get(--T.sp);
+        }
+    }
+    const uint32_t first = RRH_LEAF_FIRST(T.node), cnt = RRH_LEAF_COUNT(T.node);
+    if constexpr (COUNT) n_tris += cnt;
+    for (uint32_t i = 0; i < cnt; ++i) {
+        const float4* tv = S.tris + (size_t)(first + i) * 3;
+        float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
+        float tt, b1, b2;
+        uint32_t id = __float_as_uint(p0.w);
+        if (wt_hit(T.r, xyz(p0), xyz(p1), xyz(p2), T.tmin, T.t, tt, b1, b2) && better(tt, id, T.t, T.prim)) {
+            T.t = tt; T.prim = id; T.b1 = b1; T.b2 = b2;
+        }
+    }
+    if (T.sp == 0) return true;
+    T.node = stack.get(--T.sp);
+    return false;
+}
+// any hit: returns true once finished; T.prim = 0 marks "occluded"
+template <class Stk>
+RRH_D bool occluded_step(const DevScene& S, TravState& T, Stk& stack) {
+    while (T.node >= 0) {
+        const float4* n = S.nodes + (size_t)T.node * 4;
+        float4 a = n[0], b = n[1], c = n[2];
+        int4 lk = *reinterpret_cast<const int4*>(n + 3);
+        float tl, tr;
+        bool hl = box_hit(a.x, a.y, a.z, a.w, c.x, c.y, T.r, T.tmin, T.t, tl);
+        bool hr = box_hit(b.x, b.y, b.z, b.w, c.z, c.w, T.r, T.tmin, T.t, tr);
+        if (hl && hr) {
+            stack.put(T.sp++, lk.y);
+            T.node = lk.x;
+        } else if (hl) {
+            T.node = lk.x;
+        } else if (hr) {
+            T.node = lk.y;
+        } else {
+            if (T.sp == 0) return true;
+            T.node = stack.get(--T.sp);
+        }
+    }
+    const uint32_t first = RRH_LEAF_FIRST(T.node), cnt = RRH_LEAF_COUNT(T.node);
+    for (uint32_t i = 0; i < cnt; ++i) {
+        const float4* tv = S.tris + (size_t)(first + i) * 3;
+        float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
+        float tt, b1, b2;
+        if (wt_hit(T.r, xyz(p0), xyz(p1), xyz(p2), T.tmin, T.t, tt, b1, b2)) { T.prim = 0u; return true; }
+    }
+    if (T.sp == 0) return true;
+    T.node = stack.get(--T.sp);
+    return false;
+}
+
+// wave-level work distribution for persistent waves: hands the idle lanes the next indices
+// of a queue of n items (one atomic per refill); `drained` turns true (wave-uniformly) once
+// the queue has no more items for this wave
+RRH_D bool wave_refill(uint32_t* counter, uint32_t n, bool idle, bool& drained, uint32_t& my) {
+    const unsigned long long m = __ballot(idle ? 1 : 0);
+    const uint32_t k = (uint32_t)__popcll(m);
+    const uint32_t lane = __lane_id();
+    const int leader = __ffsll((long long)m) - 1;
+    uint32_t base = 0u;
+    if ((int)lane == leader) base = atomicAdd(counter, k);
+    base = __shfl(base, leader);
+    if (base + k >= n) drained = true;
+    if (!idle) return false;
+    my = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    return my < n;
+}
+
+// ---- 4-wide nodes: one 128 B fetch tests four child boxes -----------------------------------
+RRH_D float f4c(const float4& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w)); }
+RRH_D int i4c(const int4& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w)); }
+RRH_D void cswap(float& ka, int& va, float& kb, int& vb) {
+    if (kb < ka) {
+        float tk = ka; ka = kb; kb = tk;
+        int tv = va; va = vb; vb = tv;
+    }
+}
+
+// a 4-wide node as child boxes (fp32 planes) + links; Q: decoded from the quantized form
+// (origin + q * 2^(E-127) per bound, the exact fp32 expression the builder checked)
+struct Node4 { float4 xl, xh, yl, yh, zl, zh; int4 lk; };
+RRH_D float4 dequant4(float o, uint32_t q, float s) {
+    return make_float4(o + (float)(q & 255u) * s, o + (float)((q >> 8) & 255u) * s, o + (float)((q >> 16) & 255u) * s,
+                       o + (float)(q >> 24) * s);
+}
+template <bool Q>
+RRH_D void load_node4(const DevScene& S, int node, Node4& N) {
+    if constexpr (Q) {
+        const uint4* n = S.nodesq + (size_t)node * 4;
+        const uint4 a = n[0], b = n[1], c = n[2];
+        const uint2 d = *reinterpret_cast<const uint2*>(n + 3);
+        const float sx = __uint_as_float((a.w & 255u) << 23), sy = __uint_as_float(((a.w >> 8) & 255u) << 23),
+                    sz = __uint_as_float(((a.w >> 16) & 255u) << 23);
+        const float ox = __uint_as_float(a.x), oy = __uint_as_float(a.y), oz = __uint_as_float(a.z);
+        N.xl = dequant4(ox, b.x, sx); N.xh = dequant4(ox, b.y, sx);
+        N.yl = dequant4(oy, b.z, sy); N.yh = dequant4(oy, b.w, sy);
+        N.zl = dequant4(oz, c.x, sz); N.zh = dequant4(oz, c.y, sz);
+        N.lk = make_int4((int)c.z, (int)c.w, (int)d.x, (int)d.y);
+    } else {
+        const float4* n = S.nodes4 + (size_t)node * 8;
+        N.xl = n[0]; N.xh = n[1]; N.yl = n[2]; N.yh = n[3]; N.zl = n[4]; N.zh = n[5];
+        N.lk = *reinterpret_cast<const int4*>(n + 6);
+    }
+}
+
+// closest hit over the 4-wide nodes: children visited nearest first (the rest pushed far
+// to near); the same leaf tests as the binary traversal, so the same answer
+template <bool COUNT, bool Q, class Stk>
+RRH_D Hit trace_closest4_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack,
+                           uint32_t* n_nodes, uint32_t* n_tris) {
+    Hit h; h.t = tmax; h.prim = ~0u; h.b1 = 0.0f; h.b2 = 0.0f;
+    WRay r = wray_setup(o, d);
+    int sp = 0;
+    int node = 0;
+    while (true) {
+        while (node >= 0) {
+            if constexpr (COUNT) (*n_nodes)++;
+            Node4 N;
+            load_node4<Q>(S, node, N);
+            const float4 xl = N.xl, xh = N.xh, yl = N.yl, yh = N.yh, zl = N.zl, zh = N.zh;
+            const int4 lk = N.lk;
+            float k[4];
+            int c[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float tn;
+                const int l = i4c(lk, i);
+                const bool hit = (l != 0) && box_hit(f4c(xl, i), f4c(xh, i), f4c(yl, i), f4c(yh, i), f4c(zl, i),
+                                                     f4c(zh, i), r, tmin, h.t, tn);
+                k[i] = hit ? tn : INFINITY;
+                c[i] = l;
+            }
+            // sorting network on the entry distances (misses sort last)
+            cswap(k[0], c[0], k[1], c[1]);
+            cswap(k[2], c[2], k[3], c[3]);
+            cswap(k[0], c[0], k[2], c[2]);
+            cswap(k[1], c[1], k[3], c[3]);
+            cswap(k[1], c[1], k[2], c[2]);
+            if (!(k[0] < INFINITY)) {
+                if (sp == 0) return h;
+                node = stack.get(--sp);
+                continue;
+            }
+            if (k[3] < INFINITY) stack.put(sp++, c[3]);
+            if (k[2] < INFINITY) stack.put(sp++, c[2]);
+            if (k[1] < INFINITY) stack.put(sp++, c[1]);
+            node = c[0];
+        }
+        uint32_t first = RRH_LEAF_FIRST(node), cnt = RRH_LEAF_COUNT(node);
+        if constexpr (COUNT) (*n_tris) += cnt;
+        for (uint32_t i = 0; i < cnt; ++i) {
+            const float4* tv = S.tris + (size_t)(first + i) * 3;
+            float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
+            float tt, b1, b2;
+            uint32_t id = __float_as_uint(p0.w);
+            if (wt_hit(r, xyz(p0), xyz(p1), xyz(p2), tmin, h.t, tt, b1, b2) && better(tt, id, h.t, h.prim)) {
+                h.t = tt; h.prim = id; h.b1 = b1; h.b2 = b2;
+            }
+        }
+        if (sp == 0) return h;
+        node = stack.get(--sp);
+    }
+}
+
+template <bool Q, class Stk>
+RRH_D bool trace_occluded4_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack) {
+    if (!(tmax >= tmin)) return false;
+    WRay r = wray_setup(o, d);
+    int sp = 0;
+    int node = 0;
+    while (true) {
+        while (node >= 0) {
+            Node4 N;
+            load_node4<Q>(S, node, N);
+            const float4 xl = N.xl, xh = N.xh, yl = N.yl, yh = N.yh, zl = N.zl, zh = N.zh;
+            const int4 lk = N.lk;
+            int next = 0;
+            bool have = false;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float tn;
+                const int l = i4c(lk, i);
+                if ((l != 0) && box_hit(f4c(xl, i), f4c(xh, i), f4c(yl, i), f4c(yh, i), f4c(zl, i), f4c(zh, i), r,
+                                        tmin, tmax, tn)) {
+                    if (have) stack.put(sp++, next);
+                    next = l;
+                    have = true;
+                }
+            }
+            if (!have) {
+                if (sp == 0) return false;
+                node = stack.get(--sp);
+                continue;
+            }
+            node = next;
+        }
+        uint32_t first = RRH_LEAF_FIRST(node), cnt = RRH_LEAF_COUNT(node);
+        for (uint32_t i = 0; i < cnt; ++i) {
+            const float4* tv = S.tris + (size_t)(first + i) * 3;
+            float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
+            float tt, b1, b2;
+            if (wt_hit(r, xyz(p0), xyz(p1), xyz(p2), tmin, tmax, tt, b1, b2)) return true;
+        }
+        if (sp == 0) return false;
+        node = stack.get(--sp);
+    }
+}
+
+// the scene's node format (uniform across a launch) picks the traversal
+template <bool COUNT, class Stk>
+RRH_D Hit trace_closest_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack,
+                          uint32_t* n_nodes = nullptr, uint32_t* n_tris = nullptr) {
+    if (S.node_fmt == RRH_NODES_WIDE4Q) return trace_closest4_s<COUNT, true>(S, o, d, tmin, tmax, stack, n_nodes, n_tris);
+    if (S.node_fmt == RRH_NODES_WIDE4) return trace_closest4_s<COUNT, false>(S, o, d, tmin, tmax, stack, n_nodes, n_tris);
+    return trace_closest2_s<COUNT>(S, o, d, tmin, tmax, stack, n_nodes, n_tris);
+}
+template <class Stk>
+RRH_D bool trace_occluded_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack) {
+    if (S.node_fmt == RRH_NODES_WIDE4Q) return trace_occluded4_s<true>(S, o, d, tmin, tmax, stack);
+    if (S.node_fmt == RRH_NODES_WIDE4) return trace_occluded4_s<false>(S, o, d, tmin, tmax, stack);
+    return trace_occluded2_s(S, o, d, tmin, tmax, stack);
 }
 
 // private-stack wrappers (any depth)

@@ -271,7 +271,7 @@ int rrh_di_initial(rrh_ctx* c, rrh_scene* s, const rrh_di_params* p, rrh_image* 
     RRH_ARG(gpos->fmt == RRH_FMT_RGBA32F && gnrm->fmt == RRH_FMT_RGBA32F && res->fmt == RRH_FMT_RGBA32F &&
                 res_uv->fmt == RRH_FMT_R32U, "rrh_di_initial: formats RGBA32F x3 + R32U");
     RRH_ARG(p->num_initial_candidates <= 4096, "rrh_di_initial: num_initial_candidates above 4096");
-    const int sk = stack_for(c, s->depth);
+    const int sk = stack_for(c, s->stack_need());
     dim3 grid((W + 7) / 8, (H + 7) / 8);
     if (sk == RRH_STACK_LDS32)
         hipLaunchKernelGGL(k_di_initial<RRH_STACK_LDS32>, grid, dim3(64), 0, c->stream, s->dev(), p->frame_seed,
@@ -317,7 +317,7 @@ int rrh_di_spatial_shade(rrh_ctx* c, rrh_scene* s, const rrh_di_params* p, const
     int rc = c->di_counter.ensure(16);
     if (rc) return rc;
     RRH_HIP(hipMemsetAsync(c->di_counter.p, 0, 4, c->stream));
-    const int sk = stack_for(c, s->depth);
+    const int sk = stack_for(c, s->stack_need());
     dim3 grid((W + 7) / 8, (H + 7) / 8);
     if (sk == RRH_STACK_LDS32)
         hipLaunchKernelGGL(k_di_spatial_shade<RRH_STACK_LDS32>, grid, dim3(64), 0, c->stream, s->dev(), *p,

@@ -42,11 +42,15 @@ struct DevBuf {
     void release();
 };
 
-// binary BVH produced by the builder (rrh_bvh.cpp)
+// BVH produced by the builder (rrh_bvh.cpp): the binary SAH tree and its 4-wide collapse,
+// which visit the leaves in the same order and so share one triangle array
 struct BvhOut {
-    std::vector<float4> nodes;   // 4 per inner node
+    std::vector<float4> nodes;   // 4 per binary inner node
+    std::vector<float4> nodes4;  // 8 per 4-wide node
+    std::vector<float4> nodesq;  // 4 per quantized 4-wide node (16 words, see rrh_bvh.cpp)
     std::vector<float4> tris;    // 3 per triangle, leaf order
     uint32_t depth = 0, leaves = 0;
+    uint32_t depth4 = 0, stack4 = 0;  // 4-wide: depth and the traversal stack entries it can need
 };
 // tri_v: ntri x 9 floats (world space).  Returns false with a message on failure.
 bool build_bvh(const float* tri_v, uint32_t ntri, BvhOut& out, std::string& err);
@@ -68,6 +72,8 @@ bool generate_scene(int kind, uint32_t seed, float scale, SceneData& out, std::s
 
 struct rrh_ctx {
     int device = 0;
+    int cu_count = 256;            // compute units of the device (persistent grids)
+    bool persistent = false;       // RRH_OPT_PERSISTENT: ray queues traced by persistent waves
     hipStream_t stream = nullptr;
     hipEvent_t markers[16] = {};
     // path-tracer wavefront state (sized for the largest frame seen)
@@ -89,6 +95,7 @@ struct rrh_ctx {
     bool profiling = false;
     int stack_kind = RRH_STACK_LDS32;
     bool xcd_remap = false;
+    int bvh_nodes = RRH_NODES_BINARY;  // RRH_OPT_BVH_NODES (binary: fastest measured)
     std::vector<hipEvent_t> prof_events;   // pairs around each traversal launch
     uint32_t prof_used = 0;
     rrh::DevBuf scratch;          // small general-purpose scratch
@@ -96,8 +103,11 @@ struct rrh_ctx {
 
 struct rrh_scene {
     rrh_ctx* ctx = nullptr;
-    rrh::DevBuf nodes, tris, tri_ref, tri_light, tl, verts, idx, meshes, mats, insts, lights, cam;
+    rrh::DevBuf nodes, nodes4, nodesq, tris, tri_ref, tri_light, tl, verts, idx, meshes, mats, insts, lights, cam;
     uint32_t ntri = 0, nnodes = 0, nl = 0, ntl = 0, flags = 0, depth = 0, leaves = 0;
+    uint32_t nnodes4 = 0, depth4 = 0, stack4 = 0;
+    int node_format() const;       // the format the context traverses (RRH_NODES_*)
+    uint32_t stack_need() const;   // traversal stack entries the traversed tree can need
     float build_ms = 0.0f;
     float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
     std::vector<float> tri_world;   // flattened world triangles (host copy, 9 floats each)
@@ -117,8 +127,10 @@ struct rrh_image {
 namespace rrh {
 size_t format_bytes(int fmt);
 // stack kind a launch uses for a scene of the given BVH depth
-inline int stack_for(const rrh_ctx* c, uint32_t depth) {
-    if (c->stack_kind == RRH_STACK_LDS32 && depth > 32) return RRH_STACK_PRIVATE;
+// (need = entries the traversal can push: the binary depth, or the 4-wide bound); a tree
+// that may outgrow 32 LDS entries gets 16 in LDS plus a private overflow
+inline int stack_for(const rrh_ctx* c, uint32_t need) {
+    if (c->stack_kind == RRH_STACK_LDS32 && need > 32) return RRH_STACK_HYBRID16;
     return c->stack_kind;
 }
 }

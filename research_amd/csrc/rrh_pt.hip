@@ -19,7 +19,8 @@ namespace {
 constexpr int kMaxBounces = 32;
 constexpr uint32_t kBlock = 256;
 
-// counters layout (uint32): [0, 64) ray-queue size per bounce, [64, 128) shadow rays per bounce
+// counters layout (uint32): [0, 64) ray-queue size per bounce, [64, 128) shadow rays per bounce,
+// [128, 192) / [192, 256) next queue index handed out by the persistent intersect / shadow waves
 struct Q {
     float4 *o, *d, *thp;
 };
@@ -82,6 +83,96 @@ __global__ void __launch_bounds__(kBlock) k_intersect(DevScene S, Q in, const ui
         atomicAdd(fetch + 0, (unsigned long long)nn);
         atomicAdd(fetch + 1, (unsigned long long)nt);
         atomicAdd(fetch + 2, (unsigned long long)nr);
+    }
+}
+
+// Persistent variants (binary nodes): every wave loops, handing its finished lanes new rays
+// from the queue (one atomic per refill, when at least kRefill lanes are idle), so lanes
+// whose rays end early do not idle until the slowest ray of their wave is done.
+constexpr uint32_t kRefill = 16;
+
+template <bool COUNT, int STK>
+__global__ void __launch_bounds__(kBlock) k_intersect_p(DevScene S, Q in, const uint32_t* __restrict__ cnt, uint32_t b,
+                                                        float4* __restrict__ hits, uint32_t* __restrict__ next,
+                                                        unsigned long long* __restrict__ fetch) {
+    __shared__ int s_stack[(kBlock / 64) * StackSel<STK>::kLds * 64];
+    auto st = make_stack<STK>(s_stack);
+    const uint32_t n = cnt[b];
+    uint32_t nn = 0, nt = 0, nr = 0;
+    TravState T;
+    uint32_t idx = 0;
+    bool active = false, drained = false;
+    while (true) {
+        const bool idle = !active;
+        if (!drained && __popcll(__ballot(idle ? 1 : 0)) >= (int)kRefill) {
+            uint32_t my = 0;
+            if (wave_refill(next, n, idle, drained, my)) {
+                const float4 o = in.o[my], d = in.d[my];
+                trav_begin(T, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, 1e9f);
+                idx = my;
+                active = true;
+                nr++;
+            }
+        }
+        if (__ballot(active ? 1 : 0) == 0ull) {
+            if (drained) break;
+            continue;
+        }
+        if (active && closest_step<COUNT>(S, T, st, nn, nt)) {
+            hits[idx] = make_float4(T.t, __uint_as_float(T.prim), T.b1, T.b2);
+            active = false;
+        }
+    }
+    if constexpr (COUNT) {
+        atomicAdd(fetch + 0, (unsigned long long)nn);
+        atomicAdd(fetch + 1, (unsigned long long)nt);
+        atomicAdd(fetch + 2, (unsigned long long)nr);
+    }
+}
+
+template <int STK>
+__global__ void __launch_bounds__(kBlock) k_shadow_p(DevScene S, const uint32_t* __restrict__ cnt, uint32_t b,
+                                                     const float4* __restrict__ sh_o, const float4* __restrict__ sh_d,
+                                                     const float4* __restrict__ sh_c, float4* __restrict__ rad,
+                                                     VertexRec rec, uint32_t* __restrict__ next) {
+    __shared__ int s_stack[(kBlock / 64) * StackSel<STK>::kLds * 64];
+    auto st = make_stack<STK>(s_stack);
+    const uint32_t n = cnt[64 + b];
+    TravState T;
+    uint32_t idx = 0;
+    bool active = false, drained = false;
+    while (true) {
+        const bool idle = !active;
+        if (!drained && __popcll(__ballot(idle ? 1 : 0)) >= (int)kRefill) {
+            uint32_t my = 0;
+            if (wave_refill(next, n, idle, drained, my)) {
+                const float4 o = sh_o[my], d = sh_d[my];
+                trav_begin(T, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, o.w);
+                if (!(o.w >= 0.001f)) T.prim = 1u;  // empty extent: never occluded
+                idx = my;
+                active = true;
+            }
+        }
+        if (__ballot(active ? 1 : 0) == 0ull) {
+            if (drained) break;
+            continue;
+        }
+        if (active) {
+            bool done;
+            if (T.prim == 1u) done = true;  // the empty-extent ray
+            else done = occluded_step(S, T, st);
+            if (done) {
+                if (T.prim != 0u) {  // visible: add the NEE contribution
+                    const float4 d = sh_d[idx], c = sh_c[idx];
+                    const uint32_t pixel = __float_as_uint(d.w);
+                    float4 L = rad[pixel];
+                    L.x = L.x + c.x; L.y = L.y + c.y; L.z = L.z + c.z;
+                    rad[pixel] = L;
+                    if (rec.v) guide_record_nee_visible(rec, pixel, b);
+                }
+                active = false;
+            }
+        }
     }
 }
 
@@ -278,10 +369,34 @@ __global__ void __launch_bounds__(kBlock) k_shadow(DevScene S, const uint32_t* _
     }
 }
 
+// blocks the device holds at once for a kernel (persistent grids), cached per instantiation
+template <class K>
+uint32_t resident_blocks(rrh_ctx* c, K kernel) {
+    static uint32_t per_cu = 0;
+    if (per_cu == 0) {
+        int v = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, kernel, kBlock, 0) != hipSuccess || v < 1) v = 1;
+        per_cu = (uint32_t)v;
+    }
+    return per_cu * (uint32_t)c->cu_count;
+}
+
 template <int STK>
 int launch_trace(rrh_ctx* c, const DevScene& S, Q& qi, uint32_t* cnt, uint32_t b, uint32_t g, bool count) {
     unsigned long long* fc = (unsigned long long*)c->fetch.p;
     float4* hp = (float4*)c->hits.p;
+    if (c->persistent && S.node_fmt == RRH_NODES_BINARY) {
+        uint32_t* next = cnt + 128 + b;
+        if (count) {
+            const uint32_t gp = std::min(g, resident_blocks(c, k_intersect_p<true, STK>));
+            hipLaunchKernelGGL((k_intersect_p<true, STK>), dim3(gp), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, next, fc);
+        } else {
+            const uint32_t gp = std::min(g, resident_blocks(c, k_intersect_p<false, STK>));
+            hipLaunchKernelGGL((k_intersect_p<false, STK>), dim3(gp), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, next, fc);
+        }
+        RRH_LAUNCH_CHECK();
+        return RRH_OK;
+    }
     if (count)
         hipLaunchKernelGGL((k_intersect<true, STK>), dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, fc, c->xcd_remap);
     else
@@ -291,6 +406,14 @@ int launch_trace(rrh_ctx* c, const DevScene& S, Q& qi, uint32_t* cnt, uint32_t b
 }
 template <int STK>
 int launch_shadow(rrh_ctx* c, const DevScene& S, uint32_t* cnt, uint32_t b, uint32_t g, float4* rad, const VertexRec& rec) {
+    if (c->persistent && S.node_fmt == RRH_NODES_BINARY) {
+        const uint32_t gp = std::min(g, resident_blocks(c, k_shadow_p<STK>));
+        hipLaunchKernelGGL(k_shadow_p<STK>, dim3(gp), dim3(kBlock), 0, c->stream, S, (const uint32_t*)cnt, b,
+                           (const float4*)c->sh_o.p, (const float4*)c->sh_d.p, (const float4*)c->sh_c.p, rad, rec,
+                           cnt + 192 + b);
+        RRH_LAUNCH_CHECK();
+        return RRH_OK;
+    }
     hipLaunchKernelGGL(k_shadow<STK>, dim3(g), dim3(kBlock), 0, c->stream, S, (const uint32_t*)cnt, b,
                        (const float4*)c->sh_o.p, (const float4*)c->sh_d.p, (const float4*)c->sh_c.p, rad, rec,
                        c->xcd_remap);
@@ -351,7 +474,7 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
     if ((rc = c->sh_o.ensure((size_t)npix * 16))) return rc;
     if ((rc = c->sh_d.ensure((size_t)npix * 16))) return rc;
     if ((rc = c->sh_c.ensure((size_t)npix * 16))) return rc;
-    if ((rc = c->counters.ensure(128 * 4))) return rc;
+    if ((rc = c->counters.ensure(256 * 4))) return rc;
     if (!c->totals.p) {
         if ((rc = c->totals.ensure(32))) return rc;
         RRH_HIP(hipMemsetAsync(c->totals.p, 0, 32, c->stream));
@@ -369,7 +492,7 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
     if ((rc = guide_prepare(c, s, p, grid, mlp, A.guide, A.rec))) return rc;
     DevScene S = s->dev();
     uint32_t* cnt = (uint32_t*)c->counters.p;
-    RRH_HIP(hipMemsetAsync(cnt, 0, 128 * 4, c->stream));
+    RRH_HIP(hipMemsetAsync(cnt, 0, 256 * 4, c->stream));
     float4* rad = (float4*)radiance->p;
     Q q[2];
     for (int k = 0; k < 2; ++k) q[k] = Q{(float4*)c->q_o[k].p, (float4*)c->q_d[k].p, (float4*)c->q_thp[k].p};
@@ -390,7 +513,7 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
             }
             RRH_HIP(hipEventRecord(c->prof_events[c->prof_used], c->stream));
         }
-        const int sk = stack_for(c, s->depth);
+        const int sk = stack_for(c, s->stack_need());
         const bool cf = (p->flags & RRH_PT_COUNT_FETCHES) != 0;
         rc = (sk == RRH_STACK_LDS32) ? launch_trace<RRH_STACK_LDS32>(c, S, qi, cnt, b, g, cf)
            : (sk == RRH_STACK_HYBRID16) ? launch_trace<RRH_STACK_HYBRID16>(c, S, qi, cnt, b, g, cf)

@@ -23,7 +23,7 @@ def test_flattened_triangles_match_oracle(ctx, kind, scale):
     assert a.shape == b.shape
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     info = gs.info()
-    assert info["tri_count"] == sa.tri_count and info["bvh_depth"] < 64
+    assert info["tri_count"] == sa.tri_count and info["bvh_depth"] < 64 and info["stack4"] <= 64
 
 
 def test_trace_cornell_bit_exact(ctx):
@@ -74,3 +74,30 @@ def test_trace_full_size_heightfield_primary_rays(ctx):
     hit = prim_ids(o) != 0xFFFFFFFF
     rel = np.abs(g[hit, 0] - o[hit, 0]) / np.abs(o[hit, 0])
     assert rel.max() <= 1e-5
+
+
+@pytest.mark.parametrize("kind,seed,scale,lo,hi", [(L.GEN_HEIGHTFIELD, 2, 1.0, [-60, 0, -60], [60, 25, 60]),
+                                                   (L.GEN_INDOOR, 1, 1.0, [0.1, 0.1, 0.1], [9.9, 3.9, 9.9])])
+def test_node_formats_agree(ctx, kind, seed, scale, lo, hi):
+    """The three node formats of the same tree (binary, 4-wide fp32, 4-wide quantized) give
+    identical closest hits and occlusion on the full-size scenes; the 4-wide tree keeps its
+    stack bound."""
+    sa = core.generate_scene(kind, seed, scale)
+    gs = core.Scene(ctx, sa)
+    info = gs.info()
+    print(kind, {k: info[k] for k in ("tri_count", "node_count", "bvh_depth", "node4_count", "stack4")})
+    assert 0 < info["node4_count"] < info["node_count"] and info["stack4"] <= 64
+    rays = random_rays(200000, lo, hi, 9)
+    res = {}
+    try:
+        for fmt in (L.NODES_BINARY, L.NODES_WIDE4, L.NODES_WIDE4Q):
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_NODES, fmt)
+            res[fmt] = core.trace_closest(ctx, gs, rays), core.trace_occluded(ctx, gs, rays)
+    finally:
+        L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_NODES, L.NODES_BINARY)  # the default
+    g0, o0 = res[L.NODES_BINARY]
+    for fmt in (L.NODES_WIDE4, L.NODES_WIDE4Q):
+        assert np.array_equal(res[fmt][0].view(np.uint32), g0.view(np.uint32)), fmt
+        assert np.array_equal(res[fmt][1], o0), fmt
+    assert 0.05 < (prim_ids(g0) != 0xFFFFFFFF).mean()
+    assert np.array_equal(o0.astype(bool), prim_ids(g0) != 0xFFFFFFFF)
