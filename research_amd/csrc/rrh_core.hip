@@ -374,17 +374,25 @@ int rrh_scene_upload(rrh_ctx* c, const rrh_vertex* verts, uint32_t nv, const uin
             }
             tri_ref[t] = make_uint2(i, p);
             if ((flags & RRH_SCENE_TRI_LIGHTS) && em) {
-                tri_light[t] = nl + (uint32_t)(tl.size() / 4);
+                tri_light[t] = nl + (uint32_t)(tl.size() / kTlF4);
                 float4 a = {w[0], w[1], w[2], mt.emissive[0]};
                 float4 b = {w[3], w[4], w[5], mt.emissive[1]};
                 float4 cc = {w[6], w[7], w[8], mt.emissive[2]};
-                float4 d = {0, 0, 0, 0};
+                // e1 x e2, its length, area and unit normal: the kernels' fp32 expressions
+                // (cross, length = sqrtf(x*x + y*y + z*z), scale by 1 / length), same order
+                const float e1x = w[3] - w[0], e1y = w[4] - w[1], e1z = w[5] - w[2];
+                const float e2x = w[6] - w[0], e2y = w[7] - w[1], e2z = w[8] - w[2];
+                const float crx = e1y * e2z - e1z * e2y, cry = e1z * e2x - e1x * e2z, crz = e1x * e2y - e1y * e2x;
+                const float l2 = sqrtf(crx * crx + cry * cry + crz * crz);
+                const float inv = 1.0f / l2;
+                float4 d = {0.0f, 0.5f * l2, 0.0f, 0.0f};
                 std::memcpy(&d.x, &t, 4);
-                tl.push_back(a); tl.push_back(b); tl.push_back(cc); tl.push_back(d);
+                float4 nrm = {crx * inv, cry * inv, crz * inv, 0.0f};
+                tl.push_back(a); tl.push_back(b); tl.push_back(cc); tl.push_back(d); tl.push_back(nrm);
             }
         }
     }
-    s->ntl = (uint32_t)(tl.size() / 4);
+    s->ntl = (uint32_t)(tl.size() / kTlF4);
     for (int k = 0; k < 3; ++k) { s->bmin[k] = INFINITY; s->bmax[k] = -INFINITY; }
     for (size_t i = 0; i < (size_t)ntri * 3; ++i)
         for (int k = 0; k < 3; ++k) {
@@ -452,7 +460,7 @@ int rrh_scene_info_get(const rrh_scene* s, rrh_scene_info* o) {
     o->node4_count = s->nnodes4;
     o->stack4 = s->stack4;
     o->device_bytes = (uint64_t)s->nnodes * 64 + (uint64_t)s->nnodes4 * 192 + (uint64_t)s->ntri * 48 +
-                      (uint64_t)s->ntri * 12 + (uint64_t)s->ntl * 64;
+                      (uint64_t)s->ntri * 12 + (uint64_t)s->ntl * kTlF4 * 16;
     o->build_ms = s->build_ms;
     for (int k = 0; k < 3; ++k) { o->bounds_min[k] = s->bmin[k]; o->bounds_max[k] = s->bmax[k]; }
     return RRH_OK;

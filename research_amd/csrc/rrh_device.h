@@ -131,7 +131,8 @@ struct DevScene {
     const float4* tris;        // 3 float4 per triangle, leaf order; tris[3i].w = flattened id bits
     const uint2* tri_ref;      // per flattened id: (instance, primitive within mesh)
     const uint32_t* tri_light; // per flattened id: light index or ~0u
-    const float4* tl;          // per triangle light: (v0, Le.x) (v1, Le.y) (v2, Le.z) (id bits, 0, 0, 0)
+    const float4* tl;          // per triangle light, kTlF4 float4: (v0, Le.x) (v1, Le.y) (v2, Le.z)
+                               // (id bits, area, 0, 0) (unit normal, 0)
     const rrh_vertex* verts;
     const uint32_t* idx;
     const rrh_mesh* meshes;
@@ -827,10 +828,14 @@ RRH_D float mat_ref_pdf(const Mat& m, f3 wo, f3 wi, f3 n) {
 // ---------------------------------------------------------------------------
 // triangle lights (new; the reference lights are point / directional only)
 // ---------------------------------------------------------------------------
+// The area (0.5 |e1 x e2|) and unit normal ((e1 x e2) * (1 / |e1 x e2|)) are evaluated once at
+// upload, on the host, with the same fp32 operations in the same order; the records carry
+// them, so every use reads the identical values instead of recomputing them.
+constexpr uint32_t kTlF4 = 5;
 struct TriSample { f3 p, nl; float area; f3 Le; };
 RRH_D TriSample tri_light_point(const DevScene& S, uint32_t tl, float u1, float u2) {
-    const float4* L = S.tl + (size_t)tl * 4;
-    float4 q0 = L[0], q1 = L[1], q2 = L[2];
+    const float4* L = S.tl + (size_t)tl * kTlF4;
+    float4 q0 = L[0], q1 = L[1], q2 = L[2], q3 = L[3], q4 = L[4];
     f3 a = xyz(q0), b = xyz(q1), c = xyz(q2);
     float su = sqrtf(u1);
     float b0 = 1.0f - su;
@@ -838,10 +843,8 @@ RRH_D TriSample tri_light_point(const DevScene& S, uint32_t tl, float u1, float 
     float b2 = 1.0f - b0 - b1;
     TriSample r;
     r.p = add(add(scale(a, b0), scale(b, b1)), scale(c, b2));
-    f3 cr = cross(sub(b, a), sub(c, a));
-    float l2 = length(cr);
-    r.area = 0.5f * l2;
-    r.nl = scale(cr, 1.0f / l2);
+    r.area = q3.y;
+    r.nl = xyz(q4);
     r.Le = mk(q0.w, q1.w, q2.w);
     return r;
 }
@@ -854,12 +857,9 @@ RRH_D void dequant_uv(uint32_t q, float& u1, float& u2) {
     u2 = (float)(q >> 16) * (1.0f / 65535.0f);
 }
 RRH_D float tri_light_pdf(const DevScene& S, uint32_t tl, f3 dir, float t) {
-    const float4* L = S.tl + (size_t)tl * 4;
-    f3 a = xyz(L[0]), b = xyz(L[1]), c = xyz(L[2]);
-    f3 cr = cross(sub(b, a), sub(c, a));
-    float l2 = length(cr);
-    float area = 0.5f * l2;
-    float cosl = fabsf(dot(scale(cr, 1.0f / l2), dir));
+    const float4* L = S.tl + (size_t)tl * kTlF4;
+    const float area = L[3].y;
+    float cosl = fabsf(dot(xyz(L[4]), dir));
     float N = (float)(S.nl + S.ntl);
     return (t * t) / (cosl * area) / N;
 }

@@ -15,9 +15,7 @@ constexpr uint32_t kEmpty = 0xffffffffu;
 
 __device__ __forceinline__ float src_area(const DevScene& S, uint32_t li) {
     if (li < S.nl) return 1.0f;
-    const float4* L = S.tl + (size_t)(li - S.nl) * 4;
-    f3 a = xyz(L[0]), b = xyz(L[1]), c = xyz(L[2]);
-    return 0.5f * length(cross(sub(b, a), sub(c, a)));
+    return S.tl[(size_t)(li - S.nl) * kTlF4 + 3].y;
 }
 
 // restir_di.slang:152-178 (triangle lights add cos_l / d^2)
@@ -85,7 +83,7 @@ __global__ void __launch_bounds__(64) k_di_initial(DevScene S, uint32_t frame_se
     uint32_t N = S.nl + S.ntl;
     if (N == 0) { res[i] = pack(kEmpty, 0.0f, 0.0f, 0u); return; }
     uint32_t sel = kEmpty, sel_uv = 0, Mc = 0;
-    float wsum = 0.0f;
+    float wsum = 0.0f, sel_ph = 0.0f;  // sel_ph: p_hat of the kept candidate (the same value a re-evaluation gives)
     for (uint32_t k = 0; k < M; ++k) {
         uint32_t cand = (uint32_t)(rnd(seed) * (float)N) % N;
         uint32_t q = 0;
@@ -98,13 +96,10 @@ __global__ void __launch_bounds__(64) k_di_initial(DevScene S, uint32_t frame_se
         float wi = ph * (float)N * src_area(S, cand);
         wsum += wi;
         Mc++;
-        if (wsum > 0.0f && rnd(seed) < wi / wsum) { sel = cand; sel_uv = q; }
+        if (wsum > 0.0f && rnd(seed) < wi / wsum) { sel = cand; sel_uv = q; sel_ph = ph; }
     }
     float Wf = 0.0f;
-    if (sel != kEmpty) {
-        float ph = p_hat(S, sel, sel_uv, hi.pos, n);
-        Wf = (ph > 1e-10f) ? (wsum / ((float)Mc * ph)) : 0.0f;
-    }
+    if (sel != kEmpty) Wf = (sel_ph > 1e-10f) ? (wsum / ((float)Mc * sel_ph)) : 0.0f;
     res[i] = pack(sel, wsum, Wf, Mc);
     res_uv[i] = sel_uv;
 }
