@@ -131,6 +131,7 @@ def main() -> int:
                     help="where the traversal stack lives (context option)")
     ap.add_argument("--xcd-remap", type=int, default=0, help="1: contiguous work range per XCD")
     ap.add_argument("--persistent", type=int, default=0, help="1: persistent queue traversal with lane refill")
+    ap.add_argument("--speculative", type=int, default=1, help="1: speculative (leaf-parking) binary traversal")
     ap.add_argument("--bvh-nodes", choices=list(NODE_FORMATS), default="binary",
                     help="BVH node format traversed (binary 64 B, 4-wide 128 B, quantized 4-wide 64 B)")
     ap.add_argument("--guide-seconds", type=float, default=4.0,
@@ -152,6 +153,7 @@ def main() -> int:
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_XCD_REMAP, a.xcd_remap)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_NODES, NODE_FORMATS[a.bvh_nodes])
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_PERSISTENT, a.persistent)
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_SPECULATIVE, a.speculative)
     if a.copy_calibration:
         g = C.c_float()
         L.call("rrh_stream_copy", ctx.h, 1 << 30, 5, C.byref(g))
@@ -225,7 +227,7 @@ def main() -> int:
                    "width": W, "height": H, "spp_per_frame": 1, "max_bounces": a.max_bounces,
                    "di_candidates": 32, "di_temporal_mcap": "20*M", "di_spatial": "5 neighbours, r=30",
                    "camera": "orbit 0.5 deg/frame", "parallelism": "single GPU",
-                   "traversal_stack": a.stack, "xcd_remap": a.xcd_remap, "bvh_nodes": a.bvh_nodes, "persistent": a.persistent,
+                   "traversal_stack": a.stack, "xcd_remap": a.xcd_remap, "bvh_nodes": a.bvh_nodes, "persistent": a.persistent, "speculative": a.speculative,
                    "rays_per_step": {"primary": prim // a.steps, "extension": ext // a.steps,
                                      "shadow": shadow // a.steps}},
         "roofline": {"bound": "hbm", "kernel": "k_intersect (closest-hit BVH traversal)",

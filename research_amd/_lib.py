@@ -36,7 +36,7 @@ FMT_RGBA32F, FMT_RGBA16F, FMT_RGBA64F, FMT_R32U = 0, 1, 2, 3
 FMT_BYTES = {FMT_RGBA32F: 16, FMT_RGBA16F: 8, FMT_RGBA64F: 32, FMT_R32U: 4}
 GUIDE_NONE, GUIDE_VMM, GUIDE_MLP = 0, 1, 2
 PT_COUNT_FETCHES = 1
-OPT_TRAVERSAL_STACK, OPT_XCD_REMAP, OPT_BVH_NODES, OPT_PERSISTENT = 1, 2, 3, 4
+OPT_TRAVERSAL_STACK, OPT_XCD_REMAP, OPT_BVH_NODES, OPT_PERSISTENT, OPT_SPECULATIVE = 1, 2, 3, 4, 5
 NODES_BINARY, NODES_WIDE4, NODES_WIDE4Q = 0, 1, 2
 STACK_PRIVATE, STACK_LDS32, STACK_HYBRID16 = 0, 1, 2
 GRID_HASH, GRID_VMM = 0, 1

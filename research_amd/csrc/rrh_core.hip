@@ -48,6 +48,7 @@ rrh::DevScene rrh_scene::dev() const {
     d.nodes4 = (const float4*)nodes4.p;
     d.nodesq = (const uint4*)nodesq.p;
     d.node_fmt = (uint32_t)node_format();
+    d.speculative = (ctx && ctx->speculative) ? 1u : 0u;
     d.tris = (const float4*)tris.p;
     d.tri_ref = (const uint2*)tri_ref.p;
     d.tri_light = (const uint32_t*)tri_light.p;
@@ -138,6 +139,7 @@ int rrh_ctx_set_option(rrh_ctx* c, int option, int value) {
     }
     if (option == RRH_OPT_XCD_REMAP) { c->xcd_remap = value != 0; return RRH_OK; }
     if (option == RRH_OPT_PERSISTENT) { c->persistent = value != 0; return RRH_OK; }
+    if (option == RRH_OPT_SPECULATIVE) { c->speculative = value != 0; return RRH_OK; }
     if (option == RRH_OPT_BVH_NODES) {
         RRH_ARG(value >= RRH_NODES_BINARY && value <= RRH_NODES_WIDE4Q, "rrh_ctx_set_option: bad node format");
         c->bvh_nodes = value;

@@ -128,6 +128,7 @@ struct DevScene {
     const float4* nodes4;      // 8 float4 per 4-wide node (the same tree collapsed, same leaves)
     const uint4* nodesq;       // 4 uint4 per quantized 4-wide node (same links as nodes4)
     uint32_t node_fmt;         // RRH_NODES_*: which of the three the traversal reads
+    uint32_t speculative;      // 1: binary traversal postpones leaves (Aila and Laine 2009)
     const float4* tris;        // 3 float4 per triangle, leaf order; tris[3i].w = flattened id bits
     const uint2* tri_ref;      // per flattened id: (instance, primitive within mesh)
     const uint32_t* tri_light; // per flattened id: light index or ~0u
@@ -341,6 +342,106 @@ RRH_D bool trace_occluded2_s(const DevScene& S, f3 o, f3 d, float tmin, float tm
         if (sp == 0) return false;
         node = stack.get(--sp);
     }
+}
+
+// ---- speculative binary traversal (Aila and Laine 2009, "Understanding the efficiency of ray
+// traversal on GPUs", section 3.3): a lane that reaches a leaf parks it and keeps descending
+// inner nodes until every lane of the wave holds a leaf, then the parked leaves are tested
+// together, so fewer lanes idle in either loop.  Same closest-hit rule, same answer.
+constexpr int kTravDone = 0x7fffffff;  // popped from an empty stack: traversal finished
+template <bool COUNT, class Stk>
+RRH_D Hit trace_closest2spec_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack,
+                               uint32_t* n_nodes, uint32_t* n_tris) {
+    Hit h; h.t = tmax; h.prim = ~0u; h.b1 = 0.0f; h.b2 = 0.0f;
+    WRay r = wray_setup(o, d);
+    int sp = 0, node = 0, leaf = 0;  // leaf >= 0: none parked
+    while (node != kTravDone) {
+        while ((uint32_t)node < (uint32_t)kTravDone) {  // an inner node (leaf links are negative)
+            const float4* n = S.nodes + (size_t)node * 4;
+            if constexpr (COUNT) (*n_nodes)++;
+            float4 a = n[0], b = n[1], c = n[2];
+            int4 lk = *reinterpret_cast<const int4*>(n + 3);
+            float tl, tr;
+            bool hl = box_hit(a.x, a.y, a.z, a.w, c.x, c.y, r, tmin, h.t, tl);
+            bool hr = box_hit(b.x, b.y, b.z, b.w, c.z, c.w, r, tmin, h.t, tr);
+            if (hl && hr) {
+                int nearc = lk.x, farc = lk.y;
+                if (tr < tl) { nearc = lk.y; farc = lk.x; }
+                stack.put(sp++, farc);
+                node = nearc;
+            } else if (hl) {
+                node = lk.x;
+            } else if (hr) {
+                node = lk.y;
+            } else {
+                node = (sp == 0) ? kTravDone : stack.get(--sp);
+            }
+            if (node < 0 && leaf >= 0) {
+                leaf = node;
+                node = (sp == 0) ? kTravDone : stack.get(--sp);
+            }
+            if (!__any(leaf >= 0)) break;
+        }
+        while (leaf < 0) {
+            const uint32_t first = RRH_LEAF_FIRST(leaf), cnt = RRH_LEAF_COUNT(leaf);
+            if constexpr (COUNT) (*n_tris) += cnt;
+            for (uint32_t i = 0; i < cnt; ++i) {
+                const float4* tv = S.tris + (size_t)(first + i) * 3;
+                float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
+                float tt, b1, b2;
+                uint32_t id = __float_as_uint(p0.w);
+                if (wt_hit(r, xyz(p0), xyz(p1), xyz(p2), tmin, h.t, tt, b1, b2) && better(tt, id, h.t, h.prim)) {
+                    h.t = tt; h.prim = id; h.b1 = b1; h.b2 = b2;
+                }
+            }
+            leaf = node;
+            if (node < 0) node = (sp == 0) ? kTravDone : stack.get(--sp);
+        }
+    }
+    return h;
+}
+template <class Stk>
+RRH_D bool trace_occluded2spec_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack) {
+    if (!(tmax >= tmin)) return false;
+    WRay r = wray_setup(o, d);
+    int sp = 0, node = 0, leaf = 0;
+    while (node != kTravDone) {
+        while ((uint32_t)node < (uint32_t)kTravDone) {
+            const float4* n = S.nodes + (size_t)node * 4;
+            float4 a = n[0], b = n[1], c = n[2];
+            int4 lk = *reinterpret_cast<const int4*>(n + 3);
+            float tl, tr;
+            bool hl = box_hit(a.x, a.y, a.z, a.w, c.x, c.y, r, tmin, tmax, tl);
+            bool hr = box_hit(b.x, b.y, b.z, b.w, c.z, c.w, r, tmin, tmax, tr);
+            if (hl && hr) {
+                stack.put(sp++, lk.y);
+                node = lk.x;
+            } else if (hl) {
+                node = lk.x;
+            } else if (hr) {
+                node = lk.y;
+            } else {
+                node = (sp == 0) ? kTravDone : stack.get(--sp);
+            }
+            if (node < 0 && leaf >= 0) {
+                leaf = node;
+                node = (sp == 0) ? kTravDone : stack.get(--sp);
+            }
+            if (!__any(leaf >= 0)) break;
+        }
+        while (leaf < 0) {
+            const uint32_t first = RRH_LEAF_FIRST(leaf), cnt = RRH_LEAF_COUNT(leaf);
+            for (uint32_t i = 0; i < cnt; ++i) {
+                const float4* tv = S.tris + (size_t)(first + i) * 3;
+                float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
+                float tt, b1, b2;
+                if (wt_hit(r, xyz(p0), xyz(p1), xyz(p2), tmin, tmax, tt, b1, b2)) return true;
+            }
+            leaf = node;
+            if (node < 0) node = (sp == 0) ? kTravDone : stack.get(--sp);
+        }
+    }
+    return false;
 }
 
 // ---- stepwise binary traversal for persistent waves ------------------------------------------
@@ -594,12 +695,14 @@ RRH_D Hit trace_closest_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax,
                           uint32_t* n_nodes = nullptr, uint32_t* n_tris = nullptr) {
     if (S.node_fmt == RRH_NODES_WIDE4Q) return trace_closest4_s<COUNT, true>(S, o, d, tmin, tmax, stack, n_nodes, n_tris);
     if (S.node_fmt == RRH_NODES_WIDE4) return trace_closest4_s<COUNT, false>(S, o, d, tmin, tmax, stack, n_nodes, n_tris);
+    if (S.speculative) return trace_closest2spec_s<COUNT>(S, o, d, tmin, tmax, stack, n_nodes, n_tris);
     return trace_closest2_s<COUNT>(S, o, d, tmin, tmax, stack, n_nodes, n_tris);
 }
 template <class Stk>
 RRH_D bool trace_occluded_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack) {
     if (S.node_fmt == RRH_NODES_WIDE4Q) return trace_occluded4_s<true>(S, o, d, tmin, tmax, stack);
     if (S.node_fmt == RRH_NODES_WIDE4) return trace_occluded4_s<false>(S, o, d, tmin, tmax, stack);
+    if (S.speculative) return trace_occluded2spec_s(S, o, d, tmin, tmax, stack);
     return trace_occluded2_s(S, o, d, tmin, tmax, stack);
 }
 

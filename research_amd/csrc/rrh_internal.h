@@ -74,6 +74,7 @@ struct rrh_ctx {
     int device = 0;
     int cu_count = 256;            // compute units of the device (persistent grids)
     bool persistent = false;       // RRH_OPT_PERSISTENT: ray queues traced by persistent waves
+    bool speculative = true;       // RRH_OPT_SPECULATIVE: binary traversal parks leaves (default: measured faster)
     hipStream_t stream = nullptr;
     hipEvent_t markers[16] = {};
     // path-tracer wavefront state (sized for the largest frame seen)

@@ -79,7 +79,8 @@ def test_trace_full_size_heightfield_primary_rays(ctx):
 @pytest.mark.parametrize("kind,seed,scale,lo,hi", [(L.GEN_HEIGHTFIELD, 2, 1.0, [-60, 0, -60], [60, 25, 60]),
                                                    (L.GEN_INDOOR, 1, 1.0, [0.1, 0.1, 0.1], [9.9, 3.9, 9.9])])
 def test_node_formats_agree(ctx, kind, seed, scale, lo, hi):
-    """The three node formats of the same tree (binary, 4-wide fp32, 4-wide quantized) give
+    """The three node formats of the same tree (binary, also with speculative traversal; 4-wide
+    fp32; 4-wide quantized) give
     identical closest hits and occlusion on the full-size scenes; the 4-wide tree keeps its
     stack bound."""
     sa = core.generate_scene(kind, seed, scale)
@@ -89,15 +90,18 @@ def test_node_formats_agree(ctx, kind, seed, scale, lo, hi):
     assert 0 < info["node4_count"] < info["node_count"] and info["stack4"] <= 64
     rays = random_rays(200000, lo, hi, 9)
     res = {}
+    combos = [(L.NODES_BINARY, 0), (L.NODES_BINARY, 1), (L.NODES_WIDE4, 0), (L.NODES_WIDE4Q, 0)]
     try:
-        for fmt in (L.NODES_BINARY, L.NODES_WIDE4, L.NODES_WIDE4Q):
+        for fmt, spec in combos:
             L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_NODES, fmt)
-            res[fmt] = core.trace_closest(ctx, gs, rays), core.trace_occluded(ctx, gs, rays)
-    finally:
-        L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_NODES, L.NODES_BINARY)  # the default
-    g0, o0 = res[L.NODES_BINARY]
-    for fmt in (L.NODES_WIDE4, L.NODES_WIDE4Q):
-        assert np.array_equal(res[fmt][0].view(np.uint32), g0.view(np.uint32)), fmt
-        assert np.array_equal(res[fmt][1], o0), fmt
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_SPECULATIVE, spec)
+            res[(fmt, spec)] = core.trace_closest(ctx, gs, rays), core.trace_occluded(ctx, gs, rays)
+    finally:  # back to the defaults
+        L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_NODES, L.NODES_BINARY)
+        L.call("rrh_ctx_set_option", ctx.h, L.OPT_SPECULATIVE, 1)
+    g0, o0 = res[(L.NODES_BINARY, 0)]
+    for key in combos[1:]:
+        assert np.array_equal(res[key][0].view(np.uint32), g0.view(np.uint32)), key
+        assert np.array_equal(res[key][1], o0), key
     assert 0.05 < (prim_ids(g0) != 0xFFFFFFFF).mean()
     assert np.array_equal(o0.astype(bool), prim_ids(g0) != 0xFFFFFFFF)
