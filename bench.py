@@ -127,12 +127,12 @@ def main() -> int:
     ap.add_argument("--max-bounces", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--stack", choices=["private", "lds32", "hybrid16"], default="lds32",
+    ap.add_argument("--stack", choices=["private", "lds32", "hybrid16", "hybrid32"], default="lds32",
                     help="where the traversal stack lives (context option)")
     ap.add_argument("--xcd-remap", type=int, default=0, help="1: contiguous work range per XCD")
     ap.add_argument("--persistent", type=int, default=0, help="1: persistent queue traversal with lane refill")
     ap.add_argument("--speculative", type=int, default=1, help="1: speculative (leaf-parking) binary traversal")
-    ap.add_argument("--bvh-nodes", choices=list(NODE_FORMATS), default="binary",
+    ap.add_argument("--bvh-nodes", choices=list(NODE_FORMATS), default="wide4q",
                     help="BVH node format traversed (binary 64 B, 4-wide 128 B, quantized 4-wide 64 B)")
     ap.add_argument("--guide-seconds", type=float, default=4.0,
                     help="equal-time guiding comparison: wall seconds per method (0: skip)")
@@ -149,7 +149,8 @@ def main() -> int:
 
     ctx = core.Context(0)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAVERSAL_STACK,
-           {"private": L.STACK_PRIVATE, "lds32": L.STACK_LDS32, "hybrid16": L.STACK_HYBRID16}[a.stack])
+           {"private": L.STACK_PRIVATE, "lds32": L.STACK_LDS32, "hybrid16": L.STACK_HYBRID16,
+            "hybrid32": L.STACK_HYBRID32}[a.stack])
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_XCD_REMAP, a.xcd_remap)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_NODES, NODE_FORMATS[a.bvh_nodes])
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_PERSISTENT, a.persistent)

@@ -180,14 +180,15 @@ int rrh_fetch_counts(rrh_ctx* ctx, uint64_t* node_fetches, uint64_t* tri_fetches
 #define RRH_OPT_XCD_REMAP 2         /* 1: give each XCD a contiguous range of work items */
 #define RRH_OPT_PERSISTENT 4        /* 1: queue traversal by persistent waves that refill finished
                                        lanes with new rays (binary nodes); 0 (default): grid-stride */
-#define RRH_OPT_SPECULATIVE 5       /* 1 (default): binary traversal parks leaves until the whole wave has one */
+#define RRH_OPT_SPECULATIVE 5       /* 1 (default): traversal parks leaves until the whole wave has one */
 #define RRH_OPT_BVH_NODES 3         /* RRH_NODES_*: the node format traversal reads (all share the leaves) */
-#define RRH_NODES_BINARY 0          /* binary, 64 B (both child boxes in fp32); default */
+#define RRH_NODES_BINARY 0          /* binary, 64 B (both child boxes in fp32) */
 #define RRH_NODES_WIDE4 1           /* 4-wide, 128 B (fp32 child boxes) */
-#define RRH_NODES_WIDE4Q 2          /* 4-wide, 64 B (8-bit child boxes, outward rounded) */
+#define RRH_NODES_WIDE4Q 2          /* 4-wide, 64 B (8-bit child boxes, outward rounded); default */
 #define RRH_STACK_PRIVATE 0         /* per-lane private (scratch) array, any depth */
 #define RRH_STACK_LDS32 1           /* 32 entries per lane in LDS (when the tree can need more: hybrid16) */
 #define RRH_STACK_HYBRID16 2        /* 16 entries in LDS + private overflow, any depth */
+#define RRH_STACK_HYBRID32 3        /* 32 entries in LDS + private overflow, any depth */
 int rrh_ctx_set_option(rrh_ctx* ctx, int option, int value);
 /* plain device-to-device copy kernel (no reference counterpart): measures the achievable
  * HBM bandwidth on this device (2 x bytes / time, GB/s) and calibrates hardware

@@ -133,7 +133,7 @@ int rrh_device_info(rrh_ctx* c, int* cu_count, int* clock_khz, char* name, size_
 int rrh_ctx_set_option(rrh_ctx* c, int option, int value) {
     RRH_ARG(c, "rrh_ctx_set_option: null context");
     if (option == RRH_OPT_TRAVERSAL_STACK) {
-        RRH_ARG(value >= RRH_STACK_PRIVATE && value <= RRH_STACK_HYBRID16, "rrh_ctx_set_option: bad stack kind");
+        RRH_ARG(value >= RRH_STACK_PRIVATE && value <= RRH_STACK_HYBRID32, "rrh_ctx_set_option: bad stack kind");
         c->stack_kind = value;
         return RRH_OK;
     }

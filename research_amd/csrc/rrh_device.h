@@ -128,7 +128,7 @@ struct DevScene {
     const float4* nodes4;      // 8 float4 per 4-wide node (the same tree collapsed, same leaves)
     const uint4* nodesq;       // 4 uint4 per quantized 4-wide node (same links as nodes4)
     uint32_t node_fmt;         // RRH_NODES_*: which of the three the traversal reads
-    uint32_t speculative;      // 1: binary traversal postpones leaves (Aila and Laine 2009)
+    uint32_t speculative;      // 1: traversal postpones leaves (Aila and Laine 2009; binary and quantized 4-wide)
     const float4* tris;        // 3 float4 per triangle, leaf order; tris[3i].w = flattened id bits
     const uint2* tri_ref;      // per flattened id: (instance, primitive within mesh)
     const uint32_t* tri_light; // per flattened id: light index or ~0u
@@ -228,22 +228,25 @@ struct StackPriv {
     RRH_D void put(int i, int v) { s[i] = v; }
     RRH_D int get(int i) const { return s[i]; }
 };
-// the top RRH_LDS_STACK entries in LDS, the rest (rarely reached) in private memory
-struct StackHybrid {
+// the top N entries in LDS, the rest (rarely reached) in private memory
+template <int N>
+struct StackHybridN {
     int* p;
-    int ov[RRH_STACK - RRH_HYB_LDS];
+    int ov[RRH_STACK - N];
     RRH_D void put(int i, int v) {
-        if (i < RRH_HYB_LDS) p[i * 64] = v;
-        else ov[i - RRH_HYB_LDS] = v;
+        if (i < N) p[i * 64] = v;
+        else ov[i - N] = v;
     }
-    RRH_D int get(int i) const { return (i < RRH_HYB_LDS) ? p[i * 64] : ov[i - RRH_HYB_LDS]; }
+    RRH_D int get(int i) const { return (i < N) ? p[i * 64] : ov[i - N]; }
 };
+using StackHybrid = StackHybridN<RRH_HYB_LDS>;
 
 // stack kinds selectable per context (RRH_OPT_TRAVERSAL_STACK)
 template <int K> struct StackSel;
 template <> struct StackSel<RRH_STACK_PRIVATE> { using T = StackPriv; static constexpr int kLds = 1; };
 template <> struct StackSel<RRH_STACK_LDS32> { using T = StackLds; static constexpr int kLds = 32; };
 template <> struct StackSel<RRH_STACK_HYBRID16> { using T = StackHybrid; static constexpr int kLds = RRH_HYB_LDS; };
+template <> struct StackSel<RRH_STACK_HYBRID32> { using T = StackHybridN<32>; static constexpr int kLds = 32; };
 
 // per-lane LDS base inside a block array of (blockDim/64) x entries x 64 ints
 RRH_D int* lds_stack_base(int* smem, int entries) {
@@ -689,18 +692,128 @@ RRH_D bool trace_occluded4_s(const DevScene& S, f3 o, f3 d, float tmin, float tm
     }
 }
 
+// speculative (leaf-parking) traversal over the 4-wide nodes
+template <bool COUNT, bool Q, class Stk>
+RRH_D Hit trace_closest4spec_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack,
+                               uint32_t* n_nodes, uint32_t* n_tris) {
+    Hit h; h.t = tmax; h.prim = ~0u; h.b1 = 0.0f; h.b2 = 0.0f;
+    WRay r = wray_setup(o, d);
+    int sp = 0, node = 0, leaf = 0;
+    while (node != kTravDone) {
+        while ((uint32_t)node < (uint32_t)kTravDone) {
+            if constexpr (COUNT) (*n_nodes)++;
+            Node4 N;
+            load_node4<Q>(S, node, N);
+            float k[4];
+            int c[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float tn;
+                const int l = i4c(N.lk, i);
+                const bool hit = (l != 0) && box_hit(f4c(N.xl, i), f4c(N.xh, i), f4c(N.yl, i), f4c(N.yh, i),
+                                                     f4c(N.zl, i), f4c(N.zh, i), r, tmin, h.t, tn);
+                k[i] = hit ? tn : INFINITY;
+                c[i] = l;
+            }
+            cswap(k[0], c[0], k[1], c[1]);
+            cswap(k[2], c[2], k[3], c[3]);
+            cswap(k[0], c[0], k[2], c[2]);
+            cswap(k[1], c[1], k[3], c[3]);
+            cswap(k[1], c[1], k[2], c[2]);
+            if (!(k[0] < INFINITY)) {
+                node = (sp == 0) ? kTravDone : stack.get(--sp);
+            } else {
+                if (k[3] < INFINITY) stack.put(sp++, c[3]);
+                if (k[2] < INFINITY) stack.put(sp++, c[2]);
+                if (k[1] < INFINITY) stack.put(sp++, c[1]);
+                node = c[0];
+            }
+            if (node < 0 && leaf >= 0) {
+                leaf = node;
+                node = (sp == 0) ? kTravDone : stack.get(--sp);
+            }
+            if (!__any(leaf >= 0)) break;
+        }
+        while (leaf < 0) {
+            const uint32_t first = RRH_LEAF_FIRST(leaf), cnt = RRH_LEAF_COUNT(leaf);
+            if constexpr (COUNT) (*n_tris) += cnt;
+            for (uint32_t i = 0; i < cnt; ++i) {
+                const float4* tv = S.tris + (size_t)(first + i) * 3;
+                float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
+                float tt, b1, b2;
+                uint32_t id = __float_as_uint(p0.w);
+                if (wt_hit(r, xyz(p0), xyz(p1), xyz(p2), tmin, h.t, tt, b1, b2) && better(tt, id, h.t, h.prim)) {
+                    h.t = tt; h.prim = id; h.b1 = b1; h.b2 = b2;
+                }
+            }
+            leaf = node;
+            if (node < 0) node = (sp == 0) ? kTravDone : stack.get(--sp);
+        }
+    }
+    return h;
+}
+template <bool Q, class Stk>
+RRH_D bool trace_occluded4spec_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack) {
+    if (!(tmax >= tmin)) return false;
+    WRay r = wray_setup(o, d);
+    int sp = 0, node = 0, leaf = 0;
+    while (node != kTravDone) {
+        while ((uint32_t)node < (uint32_t)kTravDone) {
+            Node4 N;
+            load_node4<Q>(S, node, N);
+            int next = kTravDone;
+            bool have = false;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float tn;
+                const int l = i4c(N.lk, i);
+                if ((l != 0) && box_hit(f4c(N.xl, i), f4c(N.xh, i), f4c(N.yl, i), f4c(N.yh, i), f4c(N.zl, i),
+                                        f4c(N.zh, i), r, tmin, tmax, tn)) {
+                    if (have) stack.put(sp++, next);
+                    next = l;
+                    have = true;
+                }
+            }
+            node = have ? next : ((sp == 0) ? kTravDone : stack.get(--sp));
+            if (node < 0 && leaf >= 0) {
+                leaf = node;
+                node = (sp == 0) ? kTravDone : stack.get(--sp);
+            }
+            if (!__any(leaf >= 0)) break;
+        }
+        while (leaf < 0) {
+            const uint32_t first = RRH_LEAF_FIRST(leaf), cnt = RRH_LEAF_COUNT(leaf);
+            for (uint32_t i = 0; i < cnt; ++i) {
+                const float4* tv = S.tris + (size_t)(first + i) * 3;
+                float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
+                float tt, b1, b2;
+                if (wt_hit(r, xyz(p0), xyz(p1), xyz(p2), tmin, tmax, tt, b1, b2)) return true;
+            }
+            leaf = node;
+            if (node < 0) node = (sp == 0) ? kTravDone : stack.get(--sp);
+        }
+    }
+    return false;
+}
+
 // the scene's node format (uniform across a launch) picks the traversal
 template <bool COUNT, class Stk>
 RRH_D Hit trace_closest_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack,
                           uint32_t* n_nodes = nullptr, uint32_t* n_tris = nullptr) {
-    if (S.node_fmt == RRH_NODES_WIDE4Q) return trace_closest4_s<COUNT, true>(S, o, d, tmin, tmax, stack, n_nodes, n_tris);
+    if (S.node_fmt == RRH_NODES_WIDE4Q) {
+        if (S.speculative) return trace_closest4spec_s<COUNT, true>(S, o, d, tmin, tmax, stack, n_nodes, n_tris);
+        return trace_closest4_s<COUNT, true>(S, o, d, tmin, tmax, stack, n_nodes, n_tris);
+    }
     if (S.node_fmt == RRH_NODES_WIDE4) return trace_closest4_s<COUNT, false>(S, o, d, tmin, tmax, stack, n_nodes, n_tris);
     if (S.speculative) return trace_closest2spec_s<COUNT>(S, o, d, tmin, tmax, stack, n_nodes, n_tris);
     return trace_closest2_s<COUNT>(S, o, d, tmin, tmax, stack, n_nodes, n_tris);
 }
 template <class Stk>
 RRH_D bool trace_occluded_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack) {
-    if (S.node_fmt == RRH_NODES_WIDE4Q) return trace_occluded4_s<true>(S, o, d, tmin, tmax, stack);
+    if (S.node_fmt == RRH_NODES_WIDE4Q) {
+        if (S.speculative) return trace_occluded4spec_s<true>(S, o, d, tmin, tmax, stack);
+        return trace_occluded4_s<true>(S, o, d, tmin, tmax, stack);
+    }
     if (S.node_fmt == RRH_NODES_WIDE4) return trace_occluded4_s<false>(S, o, d, tmin, tmax, stack);
     if (S.speculative) return trace_occluded2spec_s(S, o, d, tmin, tmax, stack);
     return trace_occluded2_s(S, o, d, tmin, tmax, stack);

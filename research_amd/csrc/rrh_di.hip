@@ -276,6 +276,10 @@ int rrh_di_initial(rrh_ctx* c, rrh_scene* s, const rrh_di_params* p, rrh_image* 
         hipLaunchKernelGGL(k_di_initial<RRH_STACK_HYBRID16>, grid, dim3(64), 0, c->stream, s->dev(), p->frame_seed,
                            p->num_initial_candidates, (float4*)gpos->p, (float4*)gnrm->p, (float4*)res->p,
                            (uint32_t*)res_uv->p, c->xcd_remap);
+    else if (sk == RRH_STACK_HYBRID32)
+        hipLaunchKernelGGL(k_di_initial<RRH_STACK_HYBRID32>, grid, dim3(64), 0, c->stream, s->dev(), p->frame_seed,
+                           p->num_initial_candidates, (float4*)gpos->p, (float4*)gnrm->p, (float4*)res->p,
+                           (uint32_t*)res_uv->p, c->xcd_remap);
     else
         hipLaunchKernelGGL(k_di_initial<RRH_STACK_PRIVATE>, grid, dim3(64), 0, c->stream, s->dev(), p->frame_seed,
                            p->num_initial_candidates, (float4*)gpos->p, (float4*)gnrm->p, (float4*)res->p,
@@ -320,6 +324,10 @@ int rrh_di_spatial_shade(rrh_ctx* c, rrh_scene* s, const rrh_di_params* p, const
                            (const uint32_t*)res_uv->p, (float4*)out->p, (uint32_t*)c->di_counter.p, c->xcd_remap);
     else if (sk == RRH_STACK_HYBRID16)
         hipLaunchKernelGGL(k_di_spatial_shade<RRH_STACK_HYBRID16>, grid, dim3(64), 0, c->stream, s->dev(), *p,
+                           (const float4*)gpos->p, (const float4*)gnrm->p, (const float4*)res->p,
+                           (const uint32_t*)res_uv->p, (float4*)out->p, (uint32_t*)c->di_counter.p, c->xcd_remap);
+    else if (sk == RRH_STACK_HYBRID32)
+        hipLaunchKernelGGL(k_di_spatial_shade<RRH_STACK_HYBRID32>, grid, dim3(64), 0, c->stream, s->dev(), *p,
                            (const float4*)gpos->p, (const float4*)gnrm->p, (const float4*)res->p,
                            (const uint32_t*)res_uv->p, (float4*)out->p, (uint32_t*)c->di_counter.p, c->xcd_remap);
     else

@@ -517,6 +517,7 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
         const bool cf = (p->flags & RRH_PT_COUNT_FETCHES) != 0;
         rc = (sk == RRH_STACK_LDS32) ? launch_trace<RRH_STACK_LDS32>(c, S, qi, cnt, b, g, cf)
            : (sk == RRH_STACK_HYBRID16) ? launch_trace<RRH_STACK_HYBRID16>(c, S, qi, cnt, b, g, cf)
+           : (sk == RRH_STACK_HYBRID32) ? launch_trace<RRH_STACK_HYBRID32>(c, S, qi, cnt, b, g, cf)
            : launch_trace<RRH_STACK_PRIVATE>(c, S, qi, cnt, b, g, cf);
         if (rc) return rc;
         RRH_LAUNCH_CHECK();
@@ -539,6 +540,7 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
         RRH_LAUNCH_CHECK();
         rc = (sk == RRH_STACK_LDS32) ? launch_shadow<RRH_STACK_LDS32>(c, S, cnt, b, g, rad, A.rec)
            : (sk == RRH_STACK_HYBRID16) ? launch_shadow<RRH_STACK_HYBRID16>(c, S, cnt, b, g, rad, A.rec)
+           : (sk == RRH_STACK_HYBRID32) ? launch_shadow<RRH_STACK_HYBRID32>(c, S, cnt, b, g, rad, A.rec)
            : launch_shadow<RRH_STACK_PRIVATE>(c, S, cnt, b, g, rad, A.rec);
         if (rc) return rc;
     }

@@ -90,14 +90,14 @@ def test_node_formats_agree(ctx, kind, seed, scale, lo, hi):
     assert 0 < info["node4_count"] < info["node_count"] and info["stack4"] <= 64
     rays = random_rays(200000, lo, hi, 9)
     res = {}
-    combos = [(L.NODES_BINARY, 0), (L.NODES_BINARY, 1), (L.NODES_WIDE4, 0), (L.NODES_WIDE4Q, 0)]
+    combos = [(L.NODES_BINARY, 0), (L.NODES_BINARY, 1), (L.NODES_WIDE4, 0), (L.NODES_WIDE4Q, 0), (L.NODES_WIDE4Q, 1)]
     try:
         for fmt, spec in combos:
             L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_NODES, fmt)
             L.call("rrh_ctx_set_option", ctx.h, L.OPT_SPECULATIVE, spec)
             res[(fmt, spec)] = core.trace_closest(ctx, gs, rays), core.trace_occluded(ctx, gs, rays)
     finally:  # back to the defaults
-        L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_NODES, L.NODES_BINARY)
+        L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_NODES, L.NODES_WIDE4Q)
         L.call("rrh_ctx_set_option", ctx.h, L.OPT_SPECULATIVE, 1)
     g0, o0 = res[(L.NODES_BINARY, 0)]
     for key in combos[1:]:
