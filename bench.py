@@ -131,6 +131,8 @@ def main() -> int:
                     help="where the traversal stack lives (context option)")
     ap.add_argument("--xcd-remap", type=int, default=0, help="1: contiguous work range per XCD")
     ap.add_argument("--persistent", type=int, default=0, help="1: persistent queue traversal with lane refill")
+    ap.add_argument("--max-leaf", type=int, default=8, help="largest BVH leaf (1..8)")
+    ap.add_argument("--trav-cost", type=float, default=1.0, help="SAH traversal cost (triangle tests)")
     ap.add_argument("--speculative", type=int, default=1, help="1: speculative (leaf-parking) binary traversal")
     ap.add_argument("--bvh-nodes", choices=list(NODE_FORMATS), default="wide4q",
                     help="BVH node format traversed (binary 64 B, 4-wide 128 B, quantized 4-wide 64 B)")
@@ -155,6 +157,8 @@ def main() -> int:
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_NODES, NODE_FORMATS[a.bvh_nodes])
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_PERSISTENT, a.persistent)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_SPECULATIVE, a.speculative)
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_MAX_LEAF, a.max_leaf)
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_TRAV_COST, int(round(a.trav_cost * 100)))
     if a.copy_calibration:
         g = C.c_float()
         L.call("rrh_stream_copy", ctx.h, 1 << 30, 5, C.byref(g))
@@ -229,6 +233,7 @@ def main() -> int:
                    "di_candidates": 32, "di_temporal_mcap": "20*M", "di_spatial": "5 neighbours, r=30",
                    "camera": "orbit 0.5 deg/frame", "parallelism": "single GPU",
                    "traversal_stack": a.stack, "xcd_remap": a.xcd_remap, "bvh_nodes": a.bvh_nodes, "persistent": a.persistent, "speculative": a.speculative,
+                   "bvh_max_leaf": a.max_leaf, "bvh_trav_cost": a.trav_cost,
                    "rays_per_step": {"primary": prim // a.steps, "extension": ext // a.steps,
                                      "shadow": shadow // a.steps}},
         "roofline": {"bound": "hbm", "kernel": "k_intersect (closest-hit BVH traversal)",

@@ -181,6 +181,8 @@ int rrh_fetch_counts(rrh_ctx* ctx, uint64_t* node_fetches, uint64_t* tri_fetches
 #define RRH_OPT_PERSISTENT 4        /* 1: queue traversal by persistent waves that refill finished
                                        lanes with new rays (binary nodes); 0 (default): grid-stride */
 #define RRH_OPT_SPECULATIVE 5       /* 1 (default): traversal parks leaves until the whole wave has one */
+#define RRH_OPT_BVH_MAX_LEAF 6      /* largest BVH leaf, 1..8 (default 8); applies to later scene uploads */
+#define RRH_OPT_BVH_TRAV_COST 7     /* SAH cost of a traversal step x 100 (default 100); later uploads */
 #define RRH_OPT_BVH_NODES 3         /* RRH_NODES_*: the node format traversal reads (all share the leaves) */
 #define RRH_NODES_BINARY 0          /* binary, 64 B (both child boxes in fp32) */
 #define RRH_NODES_WIDE4 1           /* 4-wide, 128 B (fp32 child boxes) */

@@ -68,6 +68,8 @@ constexpr uint32_t kMedianDepth = 48;  // below this depth only balanced splits
 
 struct Builder {
     const float* tv;
+    uint32_t max_leaf = kMaxLeaf;  // largest leaf (<= kMaxLeaf)
+    float trav_cost = 1.0f;        // SAH cost of a traversal step, in triangle tests
     std::vector<Box> tbox;
     std::vector<float> cen;
     std::vector<uint32_t> order;
@@ -117,13 +119,13 @@ void build_rec(Builder& B, uint32_t first, uint32_t count, uint32_t depth, std::
             for (int b = 0; b < kBins - 1; ++b) {
                 lacc.grow(bb[b]); ln += bc[b];
                 if (ln == 0 || rc[b + 1] == 0) continue;
-                float cost = 1.0f + (lacc.area() * (float)ln + ra[b + 1] * (float)rc[b + 1]) * inv;
+                float cost = B.trav_cost + (lacc.area() * (float)ln + ra[b + 1] * (float)rc[b + 1]) * inv;
                 if (cost < best_cost) { best_cost = cost; best_axis = ax; best_split = b; }
             }
         }
     }
     uint32_t nl = 0;
-    if (!median && best_axis >= 0 && (best_cost < leaf_cost || count > kMaxLeaf)) {
+    if (!median && best_axis >= 0 && (best_cost < leaf_cost || count > B.max_leaf)) {
         float ext = cb.hi[best_axis] - cb.lo[best_axis];
         float k = (float)kBins / ext;
         auto mid = std::partition(B.order.begin() + first, B.order.begin() + first + count, [&](uint32_t t) {
@@ -132,7 +134,7 @@ void build_rec(Builder& B, uint32_t first, uint32_t count, uint32_t depth, std::
             return b <= best_split;
         });
         nl = (uint32_t)(mid - (B.order.begin() + first));
-    } else if (count > kMaxLeaf || (median && count > 2)) {
+    } else if (count > B.max_leaf || (median && count > 2)) {
         // object median on the longest centroid axis
         int ax = 0;
         for (int a = 1; a < 3; ++a)
@@ -314,12 +316,16 @@ struct Emitter4 {
 
 }  // namespace
 
-bool build_bvh(const float* tri_v, uint32_t ntri, BvhOut& out, std::string& err) {
+bool build_bvh(const float* tri_v, uint32_t ntri, BvhOut& out, std::string& err, uint32_t max_leaf,
+               float trav_cost) {
     out = BvhOut();
+    if (max_leaf < 1 || max_leaf > kMaxLeaf || !(trav_cost > 0.0f)) { err = "bad BVH build parameters"; return false; }
     if (ntri == 0) { err = "scene has no triangles"; return false; }
     if (ntri >= (1u << 28)) { err = "too many triangles for the 28-bit leaf index"; return false; }
     Builder B;
     B.tv = tri_v;
+    B.max_leaf = max_leaf;
+    B.trav_cost = trav_cost;
     B.tbox.resize(ntri);
     B.cen.resize((size_t)ntri * 3);
     B.order.resize(ntri);

@@ -140,6 +140,16 @@ int rrh_ctx_set_option(rrh_ctx* c, int option, int value) {
     if (option == RRH_OPT_XCD_REMAP) { c->xcd_remap = value != 0; return RRH_OK; }
     if (option == RRH_OPT_PERSISTENT) { c->persistent = value != 0; return RRH_OK; }
     if (option == RRH_OPT_SPECULATIVE) { c->speculative = value != 0; return RRH_OK; }
+    if (option == RRH_OPT_BVH_MAX_LEAF) {
+        RRH_ARG(value >= 1 && value <= 8, "rrh_ctx_set_option: BVH leaf size is 1..8");
+        c->bvh_max_leaf = (uint32_t)value;
+        return RRH_OK;
+    }
+    if (option == RRH_OPT_BVH_TRAV_COST) {
+        RRH_ARG(value >= 1 && value <= 10000, "rrh_ctx_set_option: traversal cost (x100) in [1, 10000]");
+        c->bvh_trav_cost = (uint32_t)value;
+        return RRH_OK;
+    }
     if (option == RRH_OPT_BVH_NODES) {
         RRH_ARG(value >= RRH_NODES_BINARY && value <= RRH_NODES_WIDE4Q, "rrh_ctx_set_option: bad node format");
         c->bvh_nodes = value;
@@ -403,7 +413,7 @@ int rrh_scene_upload(rrh_ctx* c, const rrh_vertex* verts, uint32_t nv, const uin
         }
     BvhOut bvh;
     std::string err;
-    if (!build_bvh(s->tri_world.data(), ntri, bvh, err)) {
+    if (!build_bvh(s->tri_world.data(), ntri, bvh, err, c->bvh_max_leaf, (float)c->bvh_trav_cost * 0.01f)) {
         delete s;
         set_error("rrh_scene_upload: " + err);
         return RRH_E_STATE;

@@ -53,7 +53,9 @@ struct BvhOut {
     uint32_t depth4 = 0, stack4 = 0;  // 4-wide: depth and the traversal stack entries it can need
 };
 // tri_v: ntri x 9 floats (world space).  Returns false with a message on failure.
-bool build_bvh(const float* tri_v, uint32_t ntri, BvhOut& out, std::string& err);
+// max_leaf: largest leaf (1..8); trav_cost: SAH cost of a traversal step relative to a triangle test
+bool build_bvh(const float* tri_v, uint32_t ntri, BvhOut& out, std::string& err, uint32_t max_leaf = 8,
+               float trav_cost = 1.0f);
 
 // procedural scene generators (rrh_scenegen.cpp)
 struct SceneData {
@@ -75,6 +77,8 @@ struct rrh_ctx {
     int cu_count = 256;            // compute units of the device (persistent grids)
     bool persistent = false;       // RRH_OPT_PERSISTENT: ray queues traced by persistent waves
     bool speculative = true;       // RRH_OPT_SPECULATIVE: traversal parks leaves (default: measured faster)
+    uint32_t bvh_max_leaf = 8;     // RRH_OPT_BVH_MAX_LEAF: applies to later scene uploads
+    uint32_t bvh_trav_cost = 100;  // RRH_OPT_BVH_TRAV_COST: SAH traversal cost x 100 (later uploads)
     hipStream_t stream = nullptr;
     hipEvent_t markers[16] = {};
     // path-tracer wavefront state (sized for the largest frame seen)
