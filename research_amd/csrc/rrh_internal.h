@@ -104,6 +104,7 @@ struct rrh_ctx {
     std::vector<hipEvent_t> prof_events;   // pairs around each traversal launch
     uint32_t prof_used = 0;
     rrh::DevBuf scratch;          // small general-purpose scratch
+    rrh::DevBuf mlp_prep;         // learned guide: per queue slot (pos, hit flag) (normal) (wo)
 };
 
 struct rrh_scene {

@@ -46,7 +46,7 @@ constexpr int kPackElems = kWTOff + 4 * 4096;
 struct MlpDev {
     const __bf16* pk;  // packed bf16 weights
     const float* P;    // fp32 master parameters (biases are read from here)
-    float bmin[3], bext[3];
+    float bmin[3], binv[3];  // position normalisation: u = (p - bmin) * binv
 };
 
 __device__ __forceinline__ const __bf16* Wl(const MlpDev& M, int l) { return M.pk + kWOff[l]; }
@@ -93,14 +93,16 @@ struct TrainSrc {
 };
 
 // ---- encoding: features 0..71 = sin/cos(2^j pi u_c), 72..74 n, 75..77 wo, 78..95 zero ----
-// The angle 2^j pi u is reduced exactly (t = 2^j u, t mod 2) and evaluated with the hardware
-// sine / cosine, whose argument is in revolutions (tm / 2 in [0, 1)); its error (~1e-6
-// absolute) is far below the BF16 rounding the feature gets next.
+// A block of 8 features holds one coordinate at four consecutive frequencies j0..j0+3: the
+// first pair comes from the hardware sine / cosine of the exactly reduced angle (argument in
+// revolutions, (2^j0 u mod 2) / 2), the next three by angle doubling (sin 2x = 2 s c,
+// cos 2x = 1 - 2 s^2); the error (~1e-5 after three doublings) is far below the BF16
+// rounding each feature gets next.
 __device__ __forceinline__ float unit_coord(const MlpDev& M, const Sample& s, int c) {
     const float pc = (c == 0) ? s.p.x : (c == 1 ? s.p.y : s.p.z);
     const float lo = (c == 0) ? M.bmin[0] : (c == 1 ? M.bmin[1] : M.bmin[2]);  // selects: no indexed
-    const float ext = (c == 0) ? M.bext[0] : (c == 1 ? M.bext[1] : M.bext[2]); // copy of the argument
-    return clampf((pc - lo) / ext, 0.0f, 1.0f);
+    const float inv = (c == 0) ? M.binv[0] : (c == 1 ? M.binv[1] : M.binv[2]); // copy of the argument
+    return clampf((pc - lo) * inv, 0.0f, 1.0f);
 }
 __device__ __forceinline__ void freq_pair(float u, int j, float& sn, float& cs) {
     const float t = u * (float)(1 << j);           // exact scaling by 2^j
@@ -111,19 +113,26 @@ __device__ __forceinline__ void freq_pair(float u, int j, float& sn, float& cs) 
 // features [fb, fb + 8) of a sample (fb a multiple of 8)
 __device__ __forceinline__ bf16x8 encode8(const MlpDev& M, const Sample& s, bool valid, int fb) {
     bf16x8 a;
+    if (fb < 72) {
+        float sn, cs;
+        freq_pair(unit_coord(M, s, fb / 24), (fb % 24) >> 1, sn, cs);
 #pragma unroll
-    for (int p = 0; p < 4; ++p) {
-        const int f = fb + 2 * p;
-        float v0 = 0.0f, v1 = 0.0f;
-        if (f < 72) {
-            freq_pair(unit_coord(M, s, f / 24), (f % 24) >> 1, v0, v1);
-        } else {
-            const int k = f - 72;  // 0..23: n (3), wo (3), zero padding
-            v0 = (k == 0) ? s.n.x : (k == 2) ? s.n.z : (k == 4) ? s.wo.y : 0.0f;
-            v1 = (k == 0) ? s.n.y : (k == 2) ? s.wo.x : (k == 4) ? s.wo.z : 0.0f;
+        for (int p = 0; p < 4; ++p) {
+            a[2 * p] = (__bf16)(valid ? sn : 0.0f);
+            a[2 * p + 1] = (__bf16)(valid ? cs : 0.0f);
+            const float s2 = 2.0f * sn * cs, c2 = 1.0f - 2.0f * sn * sn;
+            sn = s2;
+            cs = c2;
         }
-        a[2 * p] = (__bf16)(valid ? v0 : 0.0f);
-        a[2 * p + 1] = (__bf16)(valid ? v1 : 0.0f);
+    } else {
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const int k = fb - 72 + 2 * p;  // 0..23: n (3), wo (3), zero padding
+            const float v0 = (k == 0) ? s.n.x : (k == 2) ? s.n.z : (k == 4) ? s.wo.y : 0.0f;
+            const float v1 = (k == 0) ? s.n.y : (k == 2) ? s.wo.x : (k == 4) ? s.wo.z : 0.0f;
+            a[2 * p] = (__bf16)(valid ? v0 : 0.0f);
+            a[2 * p + 1] = (__bf16)(valid ? v1 : 0.0f);
+        }
     }
     return a;
 }
@@ -182,8 +191,12 @@ __device__ __forceinline__ void layer_bwd(const __bf16* Din, const __bf16* WT, c
 }
 
 // ---- head: lobes from the 40 raw outputs, and the NLL gradient -----------------------
-__device__ __forceinline__ float softplus_f(float d) { return d > 20.0f ? d : log_rr(1.0f + exp_rr(d)); }
-__device__ __forceinline__ float sigmoid_f(float d) { return 1.0f / (1.0f + exp_rr(-d)); }
+// The head uses the hardware exponential and logarithm (~1 ulp): the learned guide is held
+// to tolerances, not to bit parity with the FP64 oracle.
+__device__ __forceinline__ float hexp(float x) { return __expf(x); }
+__device__ __forceinline__ float hlog(float x) { return __logf(x); }
+__device__ __forceinline__ float softplus_f(float d) { return d > 20.0f ? d : hlog(1.0f + hexp(d)); }
+__device__ __forceinline__ float sigmoid_f(float d) { return 1.0f / (1.0f + hexp(-d)); }
 
 // The head runs on all 64 lanes: the 4 adjacent lanes of a sample each own two lobes
 // (lane quarter qd: lobes 2 qd, 2 qd + 1), so the softmax and mixture sums are two xor shuffles.
@@ -199,7 +212,7 @@ __device__ __forceinline__ float group_max4(float v) {
 __device__ __forceinline__ void head_lobes2(const float* o, int qd, Lobe* lb) {
     const float e0 = o[(2 * qd) * 5 + 4], e1 = o[(2 * qd + 1) * 5 + 4];
     const float emax = group_max4(fmaxf(e0, e1));
-    const float x[2] = {exp_rr(e0 - emax), exp_rr(e1 - emax)};
+    const float x[2] = {hexp(e0 - emax), hexp(e1 - emax)};
     const float z = group_sum4(x[0] + x[1]);
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
@@ -213,7 +226,7 @@ __device__ __forceinline__ void head_lobes2(const float* o, int qd, Lobe* lb) {
     }
 }
 __device__ __forceinline__ float vmf_val(float kap, float c) {
-    return kap * exp_rr(kap * (c - 1.0f)) / (6.28318530717958647692f * (1.0f - exp_rr(-2.0f * kap)));
+    return kap * hexp(kap * (c - 1.0f)) / (6.28318530717958647692f * (1.0f - hexp(-2.0f * kap)));
 }
 // NLL head of one sample on its 4 lanes: returns the loss term (on lane quarter 0, else 0)
 // and writes dL/dout (times `gscale`) of this lane's two lobes into g[10]
@@ -234,7 +247,7 @@ __device__ __forceinline__ float head_grad2(const float* o, int qd, f3 d, float 
         const float* v = o + (2 * qd + j) * 5;
         const float pik = lb[j].pi, kap = lb[j].kap;
         g[j * 5 + 4] = dq * pik * (f[j] - q);
-        float e2 = exp_rr(-2.0f * kap);
+        float e2 = hexp(-2.0f * kap);
         float dlogf = 1.0f / kap - 2.0f * e2 / (1.0f - e2) + (c[j] - 1.0f);
         float raw = 0.1f + softplus_f(v[3]);
         g[j * 5 + 3] = (raw > 1000.0f) ? 0.0f : dq * pik * f[j] * dlogf * sigmoid_f(v[3]);
@@ -246,7 +259,7 @@ __device__ __forceinline__ float head_grad2(const float* o, int qd, f3 d, float 
         g[j * 5 + 1] = (g1 - lb[j].mu[1] * mg) / len;
         g[j * 5 + 2] = (g2 - lb[j].mu[2] * mg) / len;
     }
-    return (on && qd == 0) ? -wn * log_rr(q + 1e-20f) : 0.0f;
+    return (on && qd == 0) ? -wn * hlog(q + 1e-20f) : 0.0f;
 }
 
 // ---- shared memory of a 4-wave block (64 samples) -------------------------------------
@@ -451,10 +464,16 @@ __global__ void __launch_bounds__(256) k_mlp_infer(MlpDev M, Src src, uint32_t n
     load_bias<3>(M.P + off_b(4), NO, b4);
     const uint32_t n = min(src.count(), nsamples);
     const uint32_t nchunks = (n + 15) / 16;
-    for (uint32_t ch = blockIdx.x * 4 + w; ch < nchunks; ch += gridDim.x * 4) {
+    const uint32_t stride = gridDim.x * 4;
+    uint32_t ch = blockIdx.x * 4 + w;
+    // the next chunk's sample is loaded while this one computes (software pipelining)
+    Sample nxt;
+    bool nxt_valid = (ch < nchunks) && src.get(ch * 16 + r, nxt);  // the 4 lanes of row r hold sample r
+    for (; ch < nchunks; ch += stride) {
         const uint32_t base = ch * 16;
-        Sample smp;
-        const bool valid = src.get(base + r, smp);  // the 4 lanes of row r hold sample r
+        Sample smp = nxt;
+        const bool valid = nxt_valid;
+        nxt_valid = (ch + stride < nchunks) && src.get((ch + stride) * 16 + r, nxt);
         if (!valid) { smp.p = smp.n = smp.wo = smp.dir = mk(0, 0, 0); smp.w = 0.0f; }
         // layer 0: A operand (row r, features 32 kk + 8 g ..) encoded in registers
         {
@@ -626,7 +645,7 @@ struct rrh_mlp {
         MlpDev d;
         d.pk = (const __bf16*)pk.p;
         d.P = (const float*)P.p;
-        for (int k = 0; k < 3; ++k) { d.bmin[k] = p.bounds_min[k]; d.bext[k] = p.bounds_max[k] - p.bounds_min[k]; }
+        for (int k = 0; k < 3; ++k) { d.bmin[k] = p.bounds_min[k]; d.binv[k] = 1.0f / (p.bounds_max[k] - p.bounds_min[k]); }
         return d;
     }
     float* wsum() const { return (float*)scal.p + 16; }
@@ -877,32 +896,57 @@ int rrh_mlp_state(rrh_ctx* c, const rrh_mlp* m, uint32_t* step, float* loss_scal
 // ---- guided rendering: per-queue-slot lobes before shading --------------------------------
 namespace rrh {
 
-struct QueueSrc {  // hit points of the live ray queue (miss -> invalid)
-    DevScene S;
-    const float4* o;
-    const float4* d;
-    const float4* hits;
+// The network's inputs for the live ray queue: hit point, normal facing the ray origin and
+// outgoing direction (as the shade kernel derives them), written by a light, high-occupancy
+// kernel so the persistent inference waves read them with one coalesced load instead of
+// the chain of dependent fetches a hit lookup takes.
+__global__ void __launch_bounds__(256) k_mlp_prep(DevScene S, const float4* __restrict__ d,
+                                                  const float4* __restrict__ hits, const uint32_t* __restrict__ cnt,
+                                                  uint32_t b, float4* __restrict__ prep) {
+    const uint32_t n = cnt[b];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4 h = hits[i];
+        const uint32_t prim = __float_as_uint(h.y);
+        float4 p0 = make_float4(0, 0, 0, 0), p1 = p0, p2 = p0;
+        if (prim != ~0u) {
+            HitInfo hi = fetch_hit(S, prim, h.z, h.w);
+            f3 wo = neg(xyz(d[i]));
+            f3 nn = hi.normal;
+            if (dot(nn, wo) < 0.0f) nn = neg(nn);
+            p0 = make_float4(hi.pos.x, hi.pos.y, hi.pos.z, 1.0f);
+            p1 = make_float4(nn.x, nn.y, nn.z, 0.0f);
+            p2 = make_float4(wo.x, wo.y, wo.z, 0.0f);
+        }
+        prep[(size_t)i * 3] = p0;
+        prep[(size_t)i * 3 + 1] = p1;
+        prep[(size_t)i * 3 + 2] = p2;
+    }
+}
+
+struct QueueSrc {  // the prepared inputs of the live ray queue (miss -> invalid)
+    const float4* prep;
     const uint32_t* cnt;
     uint32_t b;
     __device__ uint32_t count() const { return cnt[b]; }
     __device__ bool get(uint32_t i, Sample& s) const {
         if (i >= cnt[b]) return false;
-        float4 h = hits[i];
-        uint32_t prim = __float_as_uint(h.y);
-        if (prim == ~0u) return false;
-        HitInfo hi = fetch_hit(S, prim, h.z, h.w);
-        f3 dir = xyz(d[i]);
-        f3 wo = neg(dir);
-        f3 n = hi.normal;
-        if (dot(n, wo) < 0.0f) n = neg(n);
-        s.p = hi.pos; s.n = n; s.wo = wo; s.dir = mk(0, 0, 1); s.w = 0.0f;
+        const float4 p0 = prep[(size_t)i * 3], p1 = prep[(size_t)i * 3 + 1], p2 = prep[(size_t)i * 3 + 2];
+        if (!(p0.w > 0.0f)) return false;
+        s.p = xyz(p0); s.n = xyz(p1); s.wo = xyz(p2); s.dir = mk(0, 0, 1); s.w = 0.0f;
         return true;
     }
 };
 
 int mlp_infer_queue(rrh_ctx* c, rrh_mlp* m, const DevScene& S, const float4* qo, const float4* qd, const float4* hits,
                     const uint32_t* cnt, uint32_t b, uint32_t cap, float* lobes) {
-    QueueSrc src{S, qo, qd, hits, cnt, b};
+    (void)qo;
+    int rc = c->mlp_prep.ensure((size_t)cap * 48);
+    if (rc) return rc;
+    uint32_t gp = (cap + 255) / 256;
+    gp = gp < 1u ? 1u : (gp > 4096u ? 4096u : gp);
+    hipLaunchKernelGGL(k_mlp_prep, dim3(gp), dim3(256), 0, c->stream, S, qd, hits, cnt, b, (float4*)c->mlp_prep.p);
+    RRH_LAUNCH_CHECK();
+    QueueSrc src{(const float4*)c->mlp_prep.p, cnt, b};
     const uint32_t g = infer_grid(c, m, k_mlp_infer<QueueSrc>, cap);
     hipLaunchKernelGGL(k_mlp_infer<QueueSrc>, dim3(g), dim3(256), 0, c->stream, m->dev(), src, cap, lobes);
     RRH_LAUNCH_CHECK();
