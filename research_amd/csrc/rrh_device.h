@@ -1079,6 +1079,12 @@ RRH_D float tri_light_pdf(const DevScene& S, uint32_t tl, f3 dir, float t) {
     float N = (float)(S.nl + S.ntl);
     return (t * t) / (cosl * area) / N;
 }
+// floor(u N) mod N for u in [0, 1): the product rounds to at most N, so one conditional
+// subtraction replaces the integer division and gives the same index
+RRH_D uint32_t pick_index(float u, uint32_t N) {
+    const uint32_t v = (uint32_t)(u * (float)N);
+    return v >= N ? v - N : v;
+}
 RRH_D float lum709(f3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
 
 // ---------------------------------------------------------------------------

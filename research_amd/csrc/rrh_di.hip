@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(64) k_di_initial(DevScene S, uint32_t frame_se
     uint32_t sel = kEmpty, sel_uv = 0, Mc = 0;
     float wsum = 0.0f, sel_ph = 0.0f;  // sel_ph: p_hat of the kept candidate (the same value a re-evaluation gives)
     for (uint32_t k = 0; k < M; ++k) {
-        uint32_t cand = (uint32_t)(rnd(seed) * (float)N) % N;
+        uint32_t cand = pick_index(rnd(seed), N);
         uint32_t q = 0;
         if (cand >= S.nl) {
             float a = rnd(seed);

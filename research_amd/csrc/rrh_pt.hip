@@ -196,7 +196,7 @@ __device__ __forceinline__ NeeOut nee_candidate(const DevScene& S, f3 hit, f3 n,
     NeeOut r; r.shadow = false; r.c = mk(0, 0, 0); r.dir = mk(0, 0, 0); r.tmax = 0.0f;
     uint32_t N = S.nl + S.ntl;
     if (N == 0) return r;
-    uint32_t li = (uint32_t)(rnd(seed) * (float)N) % N;
+    uint32_t li = pick_index(rnd(seed), N);
     if (li < S.nl) {
         const rrh_light& L = S.lights[li];
         f3 ldir; float dist, pl;
