@@ -22,7 +22,7 @@ INCLUDE = HERE.parent / "include"
 OUT = HERE / "librrh.so"
 HOST_OUT = HERE / "librrh_host.so"
 OBJ = HERE / "_build"
-HOST_SOURCES = ["render.cpp", "passes.cpp", "application.cpp"]
+HOST_SOURCES = ["render.cpp", "passes.cpp", "application.cpp", "gltf.cpp"]
 CXX = os.environ.get("CXX", "g++")
 
 SOURCES = [

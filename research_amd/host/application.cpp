@@ -88,26 +88,48 @@ void Application::load_cornell() {
     pending_kind_ = RRH_GEN_CORNELL_REF;
     pending_seed_ = 1;
     pending_scale_ = 1.0f;
+    pending_gltf_.clear();
 }
 void Application::load_gltf(const std::string& path) {
-    throw std::runtime_error("load_gltf('" + path + "'): glTF scene I/O is not part of this build; use load_cornell "
-                             "or load_procedural");
+    // deferred like the reference's load request (Application.cpp:397-401); empty path ignored
+    if (path.empty()) return;
+    pending_gltf_ = path;
+    pending_kind_ = -1;
 }
 void Application::load_procedural(int kind, uint32_t seed, float scale) {
     if (kind < RRH_GEN_CORNELL_REF || kind > RRH_GEN_INDOOR) throw std::invalid_argument("load_procedural: bad kind");
     pending_kind_ = kind;
     pending_seed_ = seed;
     pending_scale_ = scale;
+    pending_gltf_.clear();
 }
 
 void Application::apply_scene_request() {
-    if (pending_kind_ < 0) return;
+    if (pending_kind_ < 0 && pending_gltf_.empty()) return;
     device_.wait_idle();
-    scene_.load_generated(device_, pending_kind_, pending_seed_, pending_scale_);
-    current_scene_name_ = scene_.name();
     scene::Camera cam;
-    cam.fov_y_deg = scene_.suggested_fov;
-    cam.look_at(scene_.suggested_eye, scene_.suggested_center);
+    if (!pending_gltf_.empty()) {
+        // Application::reload_scene_gltf (Application.cpp:761-793): a file that does not load
+        // falls back to the Cornell box; the camera returns to its default
+        last_scene_error_.clear();
+        try {
+            scene::GltfScene g;
+            scene::load_gltf(pending_gltf_, g);
+            const size_t slash = pending_gltf_.find_last_of('/');
+            scene_.upload(device_, g, slash == std::string::npos ? pending_gltf_ : pending_gltf_.substr(slash + 1));
+        } catch (const std::exception& e) {
+            last_scene_error_ = e.what();
+            std::fprintf(stderr, "[Scene] glTF load failed for '%s' (%s), falling back to Cornell Box\n",
+                         pending_gltf_.c_str(), e.what());
+            scene_.load_generated(device_, RRH_GEN_CORNELL_REF, 1, 1.0f);
+        }
+        pending_gltf_.clear();
+    } else {
+        scene_.load_generated(device_, pending_kind_, pending_seed_, pending_scale_);
+        cam.fov_y_deg = scene_.suggested_fov;
+        cam.look_at(scene_.suggested_eye, scene_.suggested_center);
+    }
+    current_scene_name_ = scene_.name();
     camera_ = cam;
     pending_kind_ = -1;
     camera_moved_ = true;

@@ -72,6 +72,7 @@ PYBIND11_MODULE(_app, m) {
         .def("load_procedural", &Application::load_procedural, py::arg("kind"), py::arg("seed") = 1,
              py::arg("scale") = 1.0f)
         .def_property_readonly("scene_name", &Application::scene_name)
+        .def_property_readonly("last_scene_error", &Application::last_scene_error)
         // camera
         .def(
             "set_camera_look_at",
@@ -123,6 +124,44 @@ PYBIND11_MODULE(_app, m) {
         .def_property_readonly("atrous_pass", &Application::atrous_pass, py::return_value_policy::reference_internal)
         .def_property_readonly("guiding_pass", &Application::guiding_pass, py::return_value_policy::reference_internal);
 
+    m.def(
+        "load_gltf",
+        [](const std::string& path) {
+            rrh::scene::GltfScene g;
+            {
+                py::gil_scoped_release nogil;
+                rrh::scene::load_gltf(path, g);
+            }
+            auto raw = [](const void* p, size_t n) { return py::bytes(static_cast<const char*>(p), n); };
+            py::list images;
+            for (const auto& im : g.images) {
+                py::dict di;
+                di["name"] = im.name;
+                di["mime_type"] = im.mime_type;
+                di["uri"] = im.uri;
+                di["width"] = im.width;
+                di["height"] = im.height;
+                di["bytes"] = im.bytes;
+                images.append(di);
+            }
+            py::list mat_images;
+            for (const auto& mi : g.material_images)
+                mat_images.append(py::make_tuple(mi.base_color, mi.metallic_roughness, mi.normal, mi.emissive));
+            py::dict d;
+            d["name"] = g.name;
+            d["vertices"] = raw(g.vertices.data(), g.vertices.size() * sizeof(rrh_vertex));
+            d["indices"] = raw(g.indices.data(), g.indices.size() * sizeof(uint32_t));
+            d["meshes"] = raw(g.meshes.data(), g.meshes.size() * sizeof(rrh_mesh));
+            d["materials"] = raw(g.materials.data(), g.materials.size() * sizeof(rrh_material));
+            d["instances"] = raw(g.instances.data(), g.instances.size() * sizeof(rrh_instance));
+            d["mesh_names"] = g.mesh_names;
+            d["material_names"] = g.material_names;
+            d["material_images"] = mat_images;
+            d["images"] = images;
+            d["warnings"] = g.warnings;
+            return d;
+        },
+        py::arg("path"), "Parse a .glb / .gltf file into raw scene records (bytes in the rrh_* layouts).");
     m.def(
         "tonemap_ldr",
         [](py::array_t<float, py::array::c_style | py::array::forcecast> hdr, float exposure) {

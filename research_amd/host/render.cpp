@@ -112,6 +112,23 @@ void Scene::upload(rhi::Device& d, const rrh_scene_desc& s, const std::string& n
     suggested_fov = s.fov_y_deg;
 }
 
+void Scene::upload(rhi::Device& d, const GltfScene& g, const std::string& name) {
+    rrh_scene_desc desc{};
+    desc.verts = const_cast<rrh_vertex*>(g.vertices.data());
+    desc.nv = (uint32_t)g.vertices.size();
+    desc.idx = const_cast<uint32_t*>(g.indices.data());
+    desc.ni = (uint32_t)g.indices.size();
+    desc.meshes = const_cast<rrh_mesh*>(g.meshes.data());
+    desc.nm = (uint32_t)g.meshes.size();
+    desc.mats = const_cast<rrh_material*>(g.materials.data());
+    desc.nmat = (uint32_t)g.materials.size();
+    desc.insts = const_cast<rrh_instance*>(g.instances.data());
+    desc.ninst = (uint32_t)g.instances.size();
+    desc.cam_eye[2] = 3.0f;  // the default Camera (Camera.h:55-64)
+    desc.fov_y_deg = 45.0f;
+    upload(d, desc, name);
+}
+
 void Scene::destroy() {
     if (scene_) rrh_scene_destroy(scene_);
     scene_ = nullptr;

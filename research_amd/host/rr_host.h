@@ -86,12 +86,41 @@ private:
     float yaw_ = -90.0f, pitch_ = 0.0f;
 };
 
+// glTF 2.0 import (GltfLoader::load, src/scene/GltfLoader.cpp:79-363): flat records ready for
+// rrh_scene_upload.  Images are described, not decoded (no kernel samples textures).
+struct GltfImage {
+    std::string name, mime_type, uri;
+    uint32_t width = 0, height = 0;  // from the PNG / JPEG header; 0 = unusable
+    size_t bytes = 0;
+};
+struct GltfMaterialImages {  // image index per texture slot, -1 = none
+    int32_t base_color = -1, metallic_roughness = -1, normal = -1, emissive = -1;
+};
+struct GltfScene {
+    std::string name;  // file stem
+    std::vector<rrh_vertex> vertices;
+    std::vector<uint32_t> indices;
+    std::vector<rrh_mesh> meshes;
+    std::vector<std::string> mesh_names;
+    std::vector<rrh_material> materials;
+    std::vector<std::string> material_names;
+    std::vector<GltfMaterialImages> material_images;
+    std::vector<rrh_instance> instances;  // row-major world transforms
+    std::vector<GltfImage> images;
+    std::vector<std::string> warnings;
+};
+// ".glb" -> binary container, otherwise JSON; throws std::runtime_error on malformed input
+void load_gltf(const std::string& path, GltfScene& out);
+std::string gltf_stem(const std::string& path);
+
 // Uploaded scene (flattened triangles + BVH + shading records on the device).
 class Scene {
 public:
     // procedural scenes: RRH_GEN_*; the reference Cornell box is RRH_GEN_CORNELL_REF
     void load_generated(rhi::Device& d, int kind, uint32_t seed = 1, float scale = 1.0f);
     void upload(rhi::Device& d, const rrh_scene_desc& desc, const std::string& name);
+    // Scene records from a glTF file (no lights, no scene flags: GltfLoader adds neither)
+    void upload(rhi::Device& d, const GltfScene& g, const std::string& name);
     void destroy();
     // Scene::update_camera: camera matrices for the given extent and frame
     void update_camera(const Camera& cam, uint32_t width, uint32_t height, uint32_t frame_index);
@@ -362,6 +391,8 @@ public:
     void load_gltf(const std::string& path);
     void load_procedural(int kind, uint32_t seed, float scale);
     [[nodiscard]] std::string scene_name() const { return current_scene_name_; }
+    // why the last glTF request fell back to the Cornell box ("" when it loaded)
+    [[nodiscard]] std::string last_scene_error() const { return last_scene_error_; }
 
     void set_camera_look_at(float ex, float ey, float ez, float cx, float cy, float cz, float ux, float uy, float uz);
     [[nodiscard]] std::array<float, 3> camera_position() const;
@@ -420,6 +451,7 @@ private:
     std::string current_scene_name_;
     // deferred scene load (applied at the next step, Application.cpp:957-969)
     int pending_kind_ = -1;
+    std::string pending_gltf_, last_scene_error_;
     uint32_t pending_seed_ = 1;
     float pending_scale_ = 1.0f;
     static bool s_instance_alive_;

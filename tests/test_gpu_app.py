@@ -137,3 +137,50 @@ def test_screenshot_is_the_tonemapped_display(app, tmp_path):
     raw = np.frombuffer(zlib.decompress(data[41:41 + n]), np.uint8).reshape(H, 1 + 4 * W)[:, 1:].reshape(H, W, 4)
     assert np.array_equal(raw, _app.tonemap_ldr(app.display_image(), 1.5))
     assert np.array_equal(raw, app.display_ldr())
+
+
+def test_gltf_scene_renders_like_its_records(ctx, app, tmp_path):
+    """A generated scene written as .glb and loaded by the Application renders exactly like the
+    same records uploaded through the C ABI (no lights, no flags, default camera)."""
+    from tests import glb_writer as G
+
+    src = core.generate_scene(L.GEN_CORNELL_RANDOM, 2, 1.0)
+    doc, bin_ = G.scene_to_gltf(src)
+    path = tmp_path / "cornell_random.glb"
+    G.write_glb(path, doc, bin_)
+    app.load_gltf(str(path))
+    frames = 3
+    for _ in range(frames):
+        app.step()
+    assert app.scene_name == "cornell_random.glb" and app.last_scene_error == ""
+    assert app.camera_position == (0.0, 0.0, 3.0)
+    got = app.display_image()
+
+    mats = src.mats.copy()  # what a glTF material carries: factors; ior 1.5, no flags, no textures
+    mats["ior"] = 1.5
+    mats["flags"] = 0
+    mats["sampler_idx"] = 0
+    for k in ("albedo_tex_idx", "normal_tex_idx", "metallic_roughness_tex_idx", "emissive_tex_idx"):
+        mats[k] = 0xFFFFFFFF
+    sa = core.SceneArrays(src.verts, src.idx, src.meshes, mats, src.insts, np.zeros(0, L.LIGHT), 0, (0, 0, 3),
+                          (0, 0, 0), 45.0)
+    gs = core.Scene(ctx, sa)
+    eye = np.float32(app.camera_position)
+    center = eye + np.float32(app.camera_forward)
+    rad, acc = core.Image(ctx, W, H), core.Image(ctx, W, H)
+    for f in range(frames):
+        gs.set_camera(core.camera_look_at(tuple(map(float, eye)), tuple(map(float, center)), 45.0, W, H, frame_index=f))
+        L.call("rrh_pt_render", ctx.h, gs.h, C.byref(L.PTParams(5, f, 3, 0, 0.0, 0)), rad.h)
+        L.call("rrh_accumulate", ctx.h, rad.h, acc.h, 1 if f == 0 else 0, f)
+    ref = acc.download()
+    assert ref[..., :3].max() > 0  # the emissive quad is in view
+    assert np.array_equal(_bits(got), _bits(ref))
+
+
+def test_gltf_load_failure_falls_back_to_cornell(app, tmp_path):
+    bad = tmp_path / "broken.glb"
+    bad.write_bytes(b"not a glb file at all")
+    app.load_gltf(str(bad))
+    app.step()
+    assert app.scene_name == "cornell_box" and "not a GLB" in app.last_scene_error
+    assert np.isfinite(app.display_image()).all()

@@ -36,8 +36,7 @@ def test_application_state_before_initialize():
             app.set_pass_enabled("tonemap", True)
         app.set_pass_enabled("restir_di", True)
         app.set_pass_enabled("denoise", True)
-        with pytest.raises(RuntimeError):
-            app.load_gltf("scene.glb")
+        app.load_gltf("scene.glb")  # deferred: read at the next step, like the reference's request
         with pytest.raises(RuntimeError):  # headless: no window to close
             app.run_until_closed()
     finally:
