@@ -134,6 +134,9 @@ def main() -> int:
     ap.add_argument("--max-leaf", type=int, default=8, help="largest BVH leaf (1..8)")
     ap.add_argument("--trav-cost", type=float, default=1.0, help="SAH traversal cost (triangle tests)")
     ap.add_argument("--speculative", type=int, default=1, help="1: speculative (leaf-parking) binary traversal")
+    ap.add_argument("--bvh-builder", choices=["host", "device"], default="host",
+                    help="BVH builder: binned SAH on the host, or PLOC on the GPU (same node formats)")
+    ap.add_argument("--ploc-radius", type=int, default=16, help="device builder's neighbour window (1..32)")
     ap.add_argument("--bvh-nodes", choices=list(NODE_FORMATS), default="wide4q",
                     help="BVH node format traversed (binary 64 B, 4-wide 128 B, quantized 4-wide 64 B)")
     ap.add_argument("--guide-seconds", type=float, default=4.0,
@@ -159,6 +162,9 @@ def main() -> int:
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_SPECULATIVE, a.speculative)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_MAX_LEAF, a.max_leaf)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_TRAV_COST, int(round(a.trav_cost * 100)))
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_BUILDER,
+           L.BVH_BUILD_DEVICE if a.bvh_builder == "device" else L.BVH_BUILD_HOST)
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_PLOC_RADIUS, a.ploc_radius)
     if a.copy_calibration:
         g = C.c_float()
         L.call("rrh_stream_copy", ctx.h, 1 << 30, 5, C.byref(g))
@@ -172,7 +178,9 @@ def main() -> int:
     scene = core.Scene(ctx, sa)
     info = scene.info()
     print(f"[bench] scene: {info['tri_count']} tris, {info['node_count']} BVH nodes, depth {info['bvh_depth']}, "
-          f"{info['light_count']} lights, build+upload {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+          f"{info['node4_count']} 4-wide nodes (stack bound {info['stack4']}), {info['light_count']} lights, "
+          f"{a.bvh_builder} BVH build+upload {info['build_ms']:.1f} ms, scene total {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
     W, H = a.width, a.height
     fr = Frame(ctx, scene, sa, W, H, a.max_bounces, 32)
     for f in range(a.warmup):
@@ -233,7 +241,8 @@ def main() -> int:
                    "di_candidates": 32, "di_temporal_mcap": "20*M", "di_spatial": "5 neighbours, r=30",
                    "camera": "orbit 0.5 deg/frame", "parallelism": "single GPU",
                    "traversal_stack": a.stack, "xcd_remap": a.xcd_remap, "bvh_nodes": a.bvh_nodes, "persistent": a.persistent, "speculative": a.speculative,
-                   "bvh_max_leaf": a.max_leaf, "bvh_trav_cost": a.trav_cost,
+                   "bvh_max_leaf": a.max_leaf, "bvh_trav_cost": a.trav_cost, "bvh_builder": a.bvh_builder,
+                   "bvh_build_ms": round(info["build_ms"], 1),
                    "rays_per_step": {"primary": prim // a.steps, "extension": ext // a.steps,
                                      "shadow": shadow // a.steps}},
         "roofline": {"bound": "hbm", "kernel": "k_intersect (closest-hit BVH traversal)",

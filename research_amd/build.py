@@ -33,6 +33,7 @@ SOURCES = [
     "rrh_guide.hip",
     "rrh_mlp.hip",
     "rrh_bvh.cpp",
+    "rrh_bvh_gpu.hip",
     "rrh_scenegen.cpp",
 ]
 

@@ -150,6 +150,16 @@ int rrh_ctx_set_option(rrh_ctx* c, int option, int value) {
         c->bvh_trav_cost = (uint32_t)value;
         return RRH_OK;
     }
+    if (option == RRH_OPT_BVH_BUILDER) {
+        RRH_ARG(value == RRH_BVH_BUILD_HOST || value == RRH_BVH_BUILD_DEVICE, "rrh_ctx_set_option: bad BVH builder");
+        c->bvh_builder = value;
+        return RRH_OK;
+    }
+    if (option == RRH_OPT_PLOC_RADIUS) {
+        RRH_ARG(value >= 1 && value <= 32, "rrh_ctx_set_option: PLOC radius is 1..32");
+        c->ploc_radius = (uint32_t)value;
+        return RRH_OK;
+    }
     if (option == RRH_OPT_BVH_NODES) {
         RRH_ARG(value >= RRH_NODES_BINARY && value <= RRH_NODES_WIDE4Q, "rrh_ctx_set_option: bad node format");
         c->bvh_nodes = value;
@@ -411,25 +421,35 @@ int rrh_scene_upload(rrh_ctx* c, const rrh_vertex* verts, uint32_t nv, const uin
             s->bmin[k] = std::min(s->bmin[k], s->tri_world[i * 3 + k]);
             s->bmax[k] = std::max(s->bmax[k], s->tri_world[i * 3 + k]);
         }
-    BvhOut bvh;
-    std::string err;
-    if (!build_bvh(s->tri_world.data(), ntri, bvh, err, c->bvh_max_leaf, (float)c->bvh_trav_cost * 0.01f)) {
-        delete s;
-        set_error("rrh_scene_upload: " + err);
-        return RRH_E_STATE;
-    }
-    s->nnodes = (uint32_t)(bvh.nodes.size() / 4);
-    s->depth = bvh.depth;
-    s->leaves = bvh.leaves;
-    s->nnodes4 = (uint32_t)(bvh.nodes4.size() / 8);
-    s->depth4 = bvh.depth4;
-    s->stack4 = bvh.stack4;
     hipStream_t st = c->stream;
     int rc = RRH_OK;
-    if (!rc) rc = upload_vec(s->nodes, bvh.nodes.data(), bvh.nodes.size(), st);
-    if (!rc) rc = upload_vec(s->nodes4, bvh.nodes4.data(), bvh.nodes4.size(), st);
-    if (!rc) rc = upload_vec(s->nodesq, bvh.nodesq.data(), bvh.nodesq.size(), st);
-    if (!rc) rc = upload_vec(s->tris, bvh.tris.data(), bvh.tris.size(), st);
+    if (c->bvh_builder == RRH_BVH_BUILD_DEVICE) {
+        rc = build_bvh_device(c, s, c->bvh_max_leaf, (float)c->bvh_trav_cost * 0.01f, c->ploc_radius);
+        if (rc) {
+            const std::string msg = std::string("rrh_scene_upload: ") + rrh_last_error();
+            rrh_scene_destroy(s);
+            set_error(msg);
+            return rc;
+        }
+    } else {
+        BvhOut bvh;
+        std::string err;
+        if (!build_bvh(s->tri_world.data(), ntri, bvh, err, c->bvh_max_leaf, (float)c->bvh_trav_cost * 0.01f)) {
+            delete s;
+            set_error("rrh_scene_upload: " + err);
+            return RRH_E_STATE;
+        }
+        s->nnodes = (uint32_t)(bvh.nodes.size() / 4);
+        s->depth = bvh.depth;
+        s->leaves = bvh.leaves;
+        s->nnodes4 = (uint32_t)(bvh.nodes4.size() / 8);
+        s->depth4 = bvh.depth4;
+        s->stack4 = bvh.stack4;
+        if (!rc) rc = upload_vec(s->nodes, bvh.nodes.data(), bvh.nodes.size(), st);
+        if (!rc) rc = upload_vec(s->nodes4, bvh.nodes4.data(), bvh.nodes4.size(), st);
+        if (!rc) rc = upload_vec(s->nodesq, bvh.nodesq.data(), bvh.nodesq.size(), st);
+        if (!rc) rc = upload_vec(s->tris, bvh.tris.data(), bvh.tris.size(), st);
+    }
     if (!rc) rc = upload_vec(s->tri_ref, tri_ref.data(), tri_ref.size(), st);
     if (!rc) rc = upload_vec(s->tri_light, tri_light.data(), tri_light.size(), st);
     if (!rc) rc = upload_vec(s->tl, tl.data(), tl.size(), st);

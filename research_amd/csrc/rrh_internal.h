@@ -57,6 +57,10 @@ struct BvhOut {
 bool build_bvh(const float* tri_v, uint32_t ntri, BvhOut& out, std::string& err, uint32_t max_leaf = 8,
                float trav_cost = 1.0f);
 
+// BVH build on the device (rrh_bvh_gpu.hip) from s->tri_world: fills s->nodes, nodes4, nodesq,
+// tris and the tree statistics; RRH_OK or an error code with the message set
+int build_bvh_device(rrh_ctx* c, rrh_scene* s, uint32_t max_leaf, float trav_cost, uint32_t radius);
+
 // procedural scene generators (rrh_scenegen.cpp)
 struct SceneData {
     std::vector<rrh_vertex> verts;
@@ -79,6 +83,8 @@ struct rrh_ctx {
     bool speculative = true;       // RRH_OPT_SPECULATIVE: traversal parks leaves (default: measured faster)
     uint32_t bvh_max_leaf = 8;     // RRH_OPT_BVH_MAX_LEAF: applies to later scene uploads
     uint32_t bvh_trav_cost = 100;  // RRH_OPT_BVH_TRAV_COST: SAH traversal cost x 100 (later uploads)
+    int bvh_builder = RRH_BVH_BUILD_HOST;  // RRH_OPT_BVH_BUILDER (later uploads)
+    uint32_t ploc_radius = 16;     // RRH_OPT_PLOC_RADIUS: device builder's neighbour window
     hipStream_t stream = nullptr;
     hipEvent_t markers[16] = {};
     // path-tracer wavefront state (sized for the largest frame seen)
@@ -115,6 +121,7 @@ struct rrh_scene {
     int node_format() const;       // the format the context traverses (RRH_NODES_*)
     uint32_t stack_need() const;   // traversal stack entries the traversed tree can need
     float build_ms = 0.0f;
+    uint32_t build_passes = 0;     // device builder: clustering passes
     float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
     std::vector<float> tri_world;   // flattened world triangles (host copy, 9 floats each)
     rrh_camera cam_host{};
