@@ -93,6 +93,14 @@ int rrh_scene_destroy(rrh_scene* s);
 int rrh_scene_info_get(const rrh_scene* s, rrh_scene_info* out);
 /* world-space triangles (9 floats each) in flattening order, for checking */
 int rrh_scene_triangles(const rrh_scene* s, float* out, uint32_t max_tris);
+/* moves the instances of an uploaded scene (same count, same meshes; new transforms), then
+ * refits or rebuilds its BVH on the device, updates the triangle-light records and bounds.
+ * No reference counterpart: the reference builds its acceleration structures once per load
+ * (Scene.cpp:449-507, AccelStructure.cpp:98-330); the refit plays the role of Vulkan's update
+ * build mode.  Closest hits after either mode equal a fresh upload's bit for bit. */
+#define RRH_UPDATE_REFIT 0         /* keep the tree, recompute every box bottom-up (quality drifts with motion) */
+#define RRH_UPDATE_REBUILD 1       /* new tree by the context's builder (RRH_OPT_BVH_BUILDER) */
+int rrh_scene_update_instances(rrh_ctx* ctx, rrh_scene* s, const rrh_instance* insts, uint32_t ninst, int mode);
 /* replaces Scene::update_camera (src/scene/Scene.cpp:554-585) */
 int rrh_camera_set(rrh_scene* s, const rrh_camera* cam);
 /* host restatement of Camera::look_at + rebuild_matrices (src/scene/Camera.cpp:27-36,130-146) */

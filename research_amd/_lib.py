@@ -40,6 +40,7 @@ OPT_TRAVERSAL_STACK, OPT_XCD_REMAP, OPT_BVH_NODES, OPT_PERSISTENT, OPT_SPECULATI
 OPT_BVH_MAX_LEAF, OPT_BVH_TRAV_COST = 6, 7
 OPT_BVH_BUILDER, OPT_PLOC_RADIUS = 8, 9
 BVH_BUILD_HOST, BVH_BUILD_DEVICE = 0, 1
+UPDATE_REFIT, UPDATE_REBUILD = 0, 1
 NODES_BINARY, NODES_WIDE4, NODES_WIDE4Q = 0, 1, 2
 STACK_PRIVATE, STACK_LDS32, STACK_HYBRID16, STACK_HYBRID32 = 0, 1, 2, 3
 GRID_HASH, GRID_VMM = 0, 1
@@ -120,6 +121,7 @@ SIGNATURES = {
     "rrh_scene_destroy": [P],
     "rrh_scene_info_get": [P, C.POINTER(SceneInfo)],
     "rrh_scene_triangles": [P, P, u32],
+    "rrh_scene_update_instances": [P, P, P, u32, i32],
     "rrh_camera_set": [P, C.POINTER(Camera)],
     "rrh_camera_look_at": [C.POINTER(f32 * 3), C.POINTER(f32 * 3), f32, u32, u32, f32, f32, u32, C.POINTER(Camera)],
     "rrh_scene_generate": [i32, u32, f32, C.POINTER(SceneDesc)],

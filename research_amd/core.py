@@ -140,6 +140,11 @@ class Scene:
         L.call("rrh_scene_triangles", self.h, L.ptr(out), n)
         return out
 
+    def update_instances(self, insts: np.ndarray, mode: int = L.UPDATE_REFIT) -> None:
+        """New instance transforms (same count and meshes); refit or rebuild the BVH on the device."""
+        insts = np.ascontiguousarray(insts, dtype=L.INSTANCE)
+        L.call("rrh_scene_update_instances", self.ctx.h, self.h, L.ptr(insts), len(insts), mode)
+
     def set_camera(self, cam: L.Camera) -> None:
         self.camera = cam
         L.call("rrh_camera_set", self.h, C.byref(cam))

@@ -578,10 +578,10 @@ __global__ __launch_bounds__(kBlock) void k_val_w4(const float4* __restrict__ no
 inline uint32_t grid(size_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
 struct Tmp {
-    DevBuf tw, tbox, cbound, keys, keys2, vals, vals2, nodes, cl0, cl1, nn, flags, scan, meta, bfs, cbase, icnt, ioff,
+    DevBuf tbox, cbound, keys, keys2, vals, vals2, nodes, cl0, cl1, nn, flags, scan, meta, bfs, cbase, icnt, ioff,
         rp, small, ref;
     ~Tmp() {
-        DevBuf* all[] = {&tw, &tbox, &cbound, &keys, &keys2, &vals, &vals2, &nodes, &cl0, &cl1, &nn, &flags,
+        DevBuf* all[] = {&tbox, &cbound, &keys, &keys2, &vals, &vals2, &nodes, &cl0, &cl1, &nn, &flags,
                          &scan, &meta, &bfs, &cbase, &icnt, &ioff, &rp, &small, &ref};
         for (DevBuf* b : all) b->release();
     }
@@ -609,7 +609,7 @@ int read_back(void* host, const void* dev, size_t bytes, hipStream_t st) {
 
 }  // namespace
 
-int build_bvh_device(rrh_ctx* c, rrh_scene* s, uint32_t max_leaf, float trav_cost, uint32_t radius) {
+int build_bvh_device(rrh_ctx* c, rrh_scene* s, const float* tw, uint32_t max_leaf, float trav_cost, uint32_t radius) {
     const uint32_t n = s->ntri;
     hipStream_t st = c->stream;
     if (n == 0 || n >= (1u << 28)) { set_error("BVH build: triangle count out of range"); return RRH_E_ARG; }
@@ -618,9 +618,7 @@ int build_bvh_device(rrh_ctx* c, rrh_scene* s, uint32_t max_leaf, float trav_cos
         return RRH_E_ARG;
     }
     Tmp T;
-    RRH_TRY(T.tw.ensure((size_t)n * 9 * sizeof(float)));
-    RRH_HIP(hipMemcpyAsync(T.tw.p, s->tri_world.data(), (size_t)n * 9 * sizeof(float), hipMemcpyHostToDevice, st));
-    const float* tw = (const float*)T.tw.p;
+    s->refit_ready = false;  // the refit's parent links describe the previous tree
     RRH_TRY(s->tris.ensure((size_t)n * 3 * sizeof(float4)));
     if (n == 1) {
         RRH_TRY(s->nodes.ensure(4 * sizeof(float4)));
@@ -795,6 +793,256 @@ int build_bvh_device(rrh_ctx* c, rrh_scene* s, uint32_t max_leaf, float trav_cos
         return RRH_E_STATE;
     }
     s->build_passes = passes;
+    return RRH_OK;
+}
+
+// ---- instance updates: flattening, light records, refit ----------------------------------------
+namespace {
+
+// world position row r of a row-major instance transform (the host flattening's expression)
+__device__ __forceinline__ float mrow_d(const float* M, int r, float x, float y, float z) {
+    return M[r * 4 + 0] * x + M[r * 4 + 1] * y + M[r * 4 + 2] * z + M[r * 4 + 3] * 1.0f;
+}
+
+__global__ __launch_bounds__(kBlock) void k_flatten(const uint2* __restrict__ tri_ref, const rrh_instance* __restrict__ insts,
+                                                    const rrh_mesh* __restrict__ meshes, const uint32_t* __restrict__ idx,
+                                                    const rrh_vertex* __restrict__ verts, uint32_t ntri,
+                                                    float* __restrict__ tw, uint32_t* __restrict__ bound) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (t < ntri) {
+        const uint2 r = tri_ref[t];
+        const rrh_instance& I = insts[r.x];
+        const rrh_mesh& me = meshes[I.mesh_index];
+        const uint32_t base = me.index_byte_offset / 4 + r.y * 3;
+        float* w = tw + (size_t)t * 9;
+        for (int k = 0; k < 3; ++k) {
+            const rrh_vertex& v = verts[me.vertex_base + idx[base + k]];
+            for (int a = 0; a < 3; ++a) {
+                const float x = mrow_d(I.transform, a, v.pos[0], v.pos[1], v.pos[2]);
+                w[k * 3 + a] = x;
+                lo[a] = fminf(lo[a], x);
+                hi[a] = fmaxf(hi[a], x);
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fminf(lo[k], __shfl_xor(lo[k], off));
+            hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], off));
+        }
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 3; ++k) {
+            atomicMin(&bound[k], ord_of(lo[k]));
+            atomicMax(&bound[3 + k], ord_of(hi[k]));
+        }
+}
+
+// triangle-light records from the moved triangles (the upload's fp32 expressions, same order)
+__global__ __launch_bounds__(kBlock) void k_tl_update(const float* __restrict__ tw, uint32_t ntl, float4* __restrict__ tl) {
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= ntl) return;
+    float4* L = tl + (size_t)j * kTlF4;
+    const uint32_t t = __float_as_uint(L[3].x);
+    const float* w = tw + (size_t)t * 9;
+    L[0] = make_float4(w[0], w[1], w[2], L[0].w);
+    L[1] = make_float4(w[3], w[4], w[5], L[1].w);
+    L[2] = make_float4(w[6], w[7], w[8], L[2].w);
+    const float e1x = w[3] - w[0], e1y = w[4] - w[1], e1z = w[5] - w[2];
+    const float e2x = w[6] - w[0], e2y = w[7] - w[1], e2z = w[8] - w[2];
+    const float crx = e1y * e2z - e1z * e2y, cry = e1z * e2x - e1x * e2z, crz = e1x * e2y - e1y * e2x;
+    const float l2 = sqrtf(crx * crx + cry * cry + crz * crz);
+    const float inv = 1.0f / l2;
+    L[3].y = 0.5f * l2;
+    L[4] = make_float4(crx * inv, cry * inv, crz * inv, 0.0f);
+}
+
+// leaf-ordered triangles from the moved world triangles (ids kept)
+__global__ __launch_bounds__(kBlock) void k_tris_refresh(const float* __restrict__ tw, uint32_t ntri, float4* __restrict__ tris) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= ntri) return;
+    float4* o = tris + 3 * (size_t)k;
+    const uint32_t t = __float_as_uint(o[0].w);
+    const float* v = tw + (size_t)t * 9;
+    o[0] = make_float4(v[0], v[1], v[2], __uint_as_float(t));
+    o[1] = make_float4(v[3], v[4], v[5], 0.0f);
+    o[2] = make_float4(v[6], v[7], v[8], 0.0f);
+}
+
+// box of a leaf link: the union of its triangles' vertices
+__device__ void leaf_box(const float4* __restrict__ tris, int32_t link, float lo[3], float hi[3]) {
+    const uint32_t u = ~(uint32_t)link, first = u >> 3, cnt = (u & 7u) + 1u;
+    for (int k = 0; k < 3; ++k) { lo[k] = INFINITY; hi[k] = -INFINITY; }
+    for (uint32_t i = first; i < first + cnt; ++i)
+        for (int v = 0; v < 3; ++v) {
+            const float4 p = tris[3 * (size_t)i + v];
+            lo[0] = fminf(lo[0], p.x); lo[1] = fminf(lo[1], p.y); lo[2] = fminf(lo[2], p.z);
+            hi[0] = fmaxf(hi[0], p.x); hi[1] = fmaxf(hi[1], p.y); hi[2] = fmaxf(hi[2], p.z);
+        }
+}
+
+__device__ __forceinline__ void st_f(float* p, float v) { st_u(reinterpret_cast<uint32_t*>(p), __float_as_uint(v)); }
+__device__ __forceinline__ float ld_f(const float* p) { return __uint_as_float(ld_u(reinterpret_cast<const uint32_t*>(p))); }
+
+// parent links: parent[child] = (node << shift) | slot; inner[node] = inner children
+__global__ __launch_bounds__(kBlock) void k_bin_parents(const float4* __restrict__ bin, uint32_t nn, uint32_t* __restrict__ parent,
+                                                        uint32_t* __restrict__ inner) {
+    const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+    if (n >= nn) return;
+    const float4 d = bin[4 * (size_t)n + 3];
+    const int32_t l[2] = {__float_as_int(d.x), __float_as_int(d.y)};
+    uint32_t c = 0;
+    for (uint32_t k = 0; k < 2; ++k)
+        if (l[k] >= 0) { parent[l[k]] = (n << 1) | k; ++c; }
+    inner[n] = c;
+    if (n == 0) parent[0] = kNone;
+}
+__global__ __launch_bounds__(kBlock) void k_w4_parents(const float4* __restrict__ nodes4, uint32_t n4, uint32_t* __restrict__ parent,
+                                                       uint32_t* __restrict__ inner) {
+    const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+    if (n >= n4) return;
+    const float4 L = nodes4[8 * (size_t)n + 6];
+    const int32_t l[4] = {__float_as_int(L.x), __float_as_int(L.y), __float_as_int(L.z), __float_as_int(L.w)};
+    uint32_t c = 0;
+    for (uint32_t s = 0; s < 4; ++s)
+        if (l[s] > 0) { parent[l[s]] = (n << 2) | s; ++c; }
+    inner[n] = c;
+    if (n == 0) parent[0] = kNone;
+}
+
+// binary refit, bottom-up: a node whose children are all leaves starts; the last child to
+// finish a node completes it (leaf slots from the triangles, inner slots already written)
+__global__ __launch_bounds__(kBlock) void k_bin_refit(float4* __restrict__ bin, uint32_t nn, const float4* __restrict__ tris,
+                                                      const uint32_t* __restrict__ parent, const uint32_t* __restrict__ inner,
+                                                      uint32_t* __restrict__ arrive) {
+    const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+    if (n >= nn || inner[n] != 0) return;
+    uint32_t cur = n;
+    for (;;) {
+        float* f = reinterpret_cast<float*>(bin + 4 * (size_t)cur);
+        const int32_t l[2] = {__float_as_int(f[12]), __float_as_int(f[13])};
+        // slot k: lo.x hi.x lo.y hi.y at f[4k..4k+3], lo.z hi.z at f[8 + 2k], f[9 + 2k]
+        float blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int k = 0; k < 2; ++k) {
+            float lo[3], hi[3];
+            if (l[k] < 0) {
+                leaf_box(tris, l[k], lo, hi);
+                st_f(&f[4 * k], lo[0]); st_f(&f[4 * k + 1], hi[0]); st_f(&f[4 * k + 2], lo[1]); st_f(&f[4 * k + 3], hi[1]);
+                st_f(&f[8 + 2 * k], lo[2]); st_f(&f[9 + 2 * k], hi[2]);
+            } else {
+                lo[0] = ld_f(&f[4 * k]); hi[0] = ld_f(&f[4 * k + 1]); lo[1] = ld_f(&f[4 * k + 2]); hi[1] = ld_f(&f[4 * k + 3]);
+                lo[2] = ld_f(&f[8 + 2 * k]); hi[2] = ld_f(&f[9 + 2 * k]);
+            }
+            for (int a = 0; a < 3; ++a) { blo[a] = fminf(blo[a], lo[a]); bhi[a] = fmaxf(bhi[a], hi[a]); }
+        }
+        const uint32_t p = parent[cur];
+        if (p == kNone) return;
+        const uint32_t pn = p >> 1, k = p & 1u;
+        float* g = reinterpret_cast<float*>(bin + 4 * (size_t)pn);
+        st_f(&g[4 * k], blo[0]); st_f(&g[4 * k + 1], bhi[0]); st_f(&g[4 * k + 2], blo[1]); st_f(&g[4 * k + 3], bhi[1]);
+        st_f(&g[8 + 2 * k], blo[2]); st_f(&g[9 + 2 * k], bhi[2]);
+        __threadfence();
+        if (atomicAdd(&arrive[pn], 1u) + 1u < inner[pn]) return;
+        __threadfence();
+        cur = pn;
+    }
+}
+
+// 4-wide refit: the same walk; a completed node rewrites its fp32 planes and its quantized copy
+__global__ __launch_bounds__(kBlock) void k_w4_refit(float4* __restrict__ nodes4, uint4* __restrict__ nodesq, uint32_t n4,
+                                                     const float4* __restrict__ tris, const uint32_t* __restrict__ parent,
+                                                     const uint32_t* __restrict__ inner, uint32_t* __restrict__ arrive) {
+    const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+    if (n >= n4 || inner[n] != 0) return;
+    uint32_t cur = n;
+    for (;;) {
+        float* f = reinterpret_cast<float*>(nodes4 + 8 * (size_t)cur);
+        int32_t links[4];
+        for (int s = 0; s < 4; ++s) links[s] = __float_as_int(f[24 + s]);
+        float lo[3][4] = {}, hi[3][4] = {};
+        float blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        int nch = 0;
+        for (int s = 0; s < 4; ++s) {
+            if (links[s] == 0) continue;
+            nch = s + 1;
+            float a[3], b[3];
+            if (links[s] < 0) {
+                leaf_box(tris, links[s], a, b);
+            } else {
+                for (int k = 0; k < 3; ++k) { a[k] = ld_f(&f[8 * k + s]); b[k] = ld_f(&f[8 * k + 4 + s]); }
+            }
+            for (int k = 0; k < 3; ++k) {
+                lo[k][s] = a[k];
+                hi[k][s] = b[k];
+                blo[k] = fminf(blo[k], a[k]);
+                bhi[k] = fmaxf(bhi[k], b[k]);
+            }
+        }
+        write_w4(nodes4 + 8 * (size_t)cur, nodesq + 4 * (size_t)cur, lo, hi, links, nch);
+        const uint32_t p = parent[cur];
+        if (p == kNone) return;
+        const uint32_t pn = p >> 2, s = p & 3u;
+        float* g = reinterpret_cast<float*>(nodes4 + 8 * (size_t)pn);
+        for (int k = 0; k < 3; ++k) { st_f(&g[8 * k + s], blo[k]); st_f(&g[8 * k + 4 + s], bhi[k]); }
+        __threadfence();
+        if (atomicAdd(&arrive[pn], 1u) + 1u < inner[pn]) return;
+        __threadfence();
+        cur = pn;
+    }
+}
+
+}  // namespace
+
+int flatten_device(rrh_ctx* c, rrh_scene* s) {
+    hipStream_t st = c->stream;
+    const uint32_t n = s->ntri;
+    RRH_TRY(s->twd.ensure((size_t)n * 9 * sizeof(float)));
+    DevBuf bound;
+    RRH_TRY(bound.ensure(8 * sizeof(uint32_t)));
+    k_init_bounds<<<1, 64, 0, st>>>((uint32_t*)bound.p);
+    k_flatten<<<grid(n), kBlock, 0, st>>>((const uint2*)s->tri_ref.p, (const rrh_instance*)s->insts.p,
+                                          (const rrh_mesh*)s->meshes.p, (const uint32_t*)s->idx.p,
+                                          (const rrh_vertex*)s->verts.p, n, (float*)s->twd.p, (uint32_t*)bound.p);
+    if (s->ntl) k_tl_update<<<grid(s->ntl), kBlock, 0, st>>>((const float*)s->twd.p, s->ntl, (float4*)s->tl.p);
+    RRH_LAUNCH_CHECK();
+    uint32_t b[6];
+    const int rc = read_back(b, bound.p, sizeof(b), st);
+    bound.release();
+    if (rc) return rc;
+    auto fl = [](uint32_t u) {
+        const uint32_t v = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+        float f;
+        std::memcpy(&f, &v, 4);
+        return f;
+    };
+    for (int k = 0; k < 3; ++k) { s->bmin[k] = fl(b[k]); s->bmax[k] = fl(b[3 + k]); }
+    s->tri_world_stale = true;
+    return RRH_OK;
+}
+
+int refit_bvh_device(rrh_ctx* c, rrh_scene* s) {
+    hipStream_t st = c->stream;
+    const uint32_t nb = s->nnodes, n4 = s->nnodes4;
+    k_tris_refresh<<<grid(s->ntri), kBlock, 0, st>>>((const float*)s->twd.p, s->ntri, (float4*)s->tris.p);
+    RRH_LAUNCH_CHECK();
+    if (!s->refit_ready) {
+        RRH_TRY(s->refit.ensure(((size_t)nb + n4) * 3 * sizeof(uint32_t)));
+        uint32_t* r = (uint32_t*)s->refit.p;
+        k_bin_parents<<<grid(nb), kBlock, 0, st>>>((const float4*)s->nodes.p, nb, r, r + nb);
+        k_w4_parents<<<grid(n4), kBlock, 0, st>>>((const float4*)s->nodes4.p, n4, r + 3 * (size_t)nb, r + 3 * (size_t)nb + n4);
+        RRH_LAUNCH_CHECK();
+        s->refit_ready = true;
+    }
+    uint32_t* r = (uint32_t*)s->refit.p;
+    uint32_t *bpar = r, *binn = r + nb, *barr = r + 2 * (size_t)nb;
+    uint32_t *wpar = r + 3 * (size_t)nb, *winn = wpar + n4, *warr = winn + n4;
+    RRH_HIP(hipMemsetAsync(barr, 0, (size_t)nb * 4, st));
+    RRH_HIP(hipMemsetAsync(warr, 0, (size_t)n4 * 4, st));
+    k_bin_refit<<<grid(nb), kBlock, 0, st>>>((float4*)s->nodes.p, nb, (const float4*)s->tris.p, bpar, binn, barr);
+    k_w4_refit<<<grid(n4), kBlock, 0, st>>>((float4*)s->nodes4.p, (uint4*)s->nodesq.p, n4, (const float4*)s->tris.p,
+                                            wpar, winn, warr);
+    RRH_LAUNCH_CHECK();
+    RRH_HIP(hipStreamSynchronize(st));
     return RRH_OK;
 }
 

@@ -341,6 +341,34 @@ static inline float mrow_h(const float* M, int r, float x, float y, float z, flo
 }
 
 
+// host binned-SAH build of s->tri_world, uploaded into the scene's node and triangle buffers
+static int host_build_upload(rrh_ctx* c, rrh_scene* s) {
+    BvhOut bvh;
+    std::string err;
+    if (!build_bvh(s->tri_world.data(), s->ntri, bvh, err, c->bvh_max_leaf, (float)c->bvh_trav_cost * 0.01f)) {
+        set_error("rrh_scene_upload: " + err);
+        return RRH_E_STATE;
+    }
+    s->nnodes = (uint32_t)(bvh.nodes.size() / 4);
+    s->depth = bvh.depth;
+    s->leaves = bvh.leaves;
+    s->nnodes4 = (uint32_t)(bvh.nodes4.size() / 8);
+    s->depth4 = bvh.depth4;
+    s->stack4 = bvh.stack4;
+    s->refit_ready = false;
+    hipStream_t st = c->stream;
+    int rc = RRH_OK;
+    if (!rc) rc = upload_vec(s->nodes, bvh.nodes.data(), bvh.nodes.size(), st);
+    if (!rc) rc = upload_vec(s->nodes4, bvh.nodes4.data(), bvh.nodes4.size(), st);
+    if (!rc) rc = upload_vec(s->nodesq, bvh.nodesq.data(), bvh.nodesq.size(), st);
+    if (!rc) rc = upload_vec(s->tris, bvh.tris.data(), bvh.tris.size(), st);
+    if (!rc) {
+        hipError_t e = hipStreamSynchronize(st);  // the host vectors die with this frame
+        if (e != hipSuccess) { set_error(hipGetErrorString(e)); rc = RRH_E_HIP; }
+    }
+    return rc;
+}
+
 int rrh_scene_upload(rrh_ctx* c, const rrh_vertex* verts, uint32_t nv, const uint32_t* idx, uint32_t ni,
                      const rrh_mesh* meshes, uint32_t nm, const rrh_material* mats, uint32_t nmat,
                      const rrh_instance* insts, uint32_t ninst, const rrh_light* lights, uint32_t nl,
@@ -423,32 +451,23 @@ int rrh_scene_upload(rrh_ctx* c, const rrh_vertex* verts, uint32_t nv, const uin
         }
     hipStream_t st = c->stream;
     int rc = RRH_OK;
+    s->ninst = ninst;
+    s->inst_mesh.resize(ninst);
+    for (uint32_t i = 0; i < ninst; ++i) s->inst_mesh[i] = insts[i].mesh_index;
     if (c->bvh_builder == RRH_BVH_BUILD_DEVICE) {
-        rc = build_bvh_device(c, s, c->bvh_max_leaf, (float)c->bvh_trav_cost * 0.01f, c->ploc_radius);
-        if (rc) {
-            const std::string msg = std::string("rrh_scene_upload: ") + rrh_last_error();
-            rrh_scene_destroy(s);
-            set_error(msg);
-            return rc;
-        }
+        rc = upload_vec(s->twd, s->tri_world.data(), s->tri_world.size(), st);
+        if (!rc) rc = build_bvh_device(c, s, (const float*)s->twd.p, c->bvh_max_leaf, (float)c->bvh_trav_cost * 0.01f,
+                                       c->ploc_radius);
     } else {
-        BvhOut bvh;
-        std::string err;
-        if (!build_bvh(s->tri_world.data(), ntri, bvh, err, c->bvh_max_leaf, (float)c->bvh_trav_cost * 0.01f)) {
-            delete s;
-            set_error("rrh_scene_upload: " + err);
-            return RRH_E_STATE;
-        }
-        s->nnodes = (uint32_t)(bvh.nodes.size() / 4);
-        s->depth = bvh.depth;
-        s->leaves = bvh.leaves;
-        s->nnodes4 = (uint32_t)(bvh.nodes4.size() / 8);
-        s->depth4 = bvh.depth4;
-        s->stack4 = bvh.stack4;
-        if (!rc) rc = upload_vec(s->nodes, bvh.nodes.data(), bvh.nodes.size(), st);
-        if (!rc) rc = upload_vec(s->nodes4, bvh.nodes4.data(), bvh.nodes4.size(), st);
-        if (!rc) rc = upload_vec(s->nodesq, bvh.nodesq.data(), bvh.nodesq.size(), st);
-        if (!rc) rc = upload_vec(s->tris, bvh.tris.data(), bvh.tris.size(), st);
+        rc = host_build_upload(c, s);
+    }
+    if (rc) {
+        const std::string msg = std::string(rrh_last_error()).rfind("rrh_scene_upload", 0) == 0
+                                    ? std::string(rrh_last_error())
+                                    : std::string("rrh_scene_upload: ") + rrh_last_error();
+        rrh_scene_destroy(s);
+        set_error(msg);
+        return rc;
     }
     if (!rc) rc = upload_vec(s->tri_ref, tri_ref.data(), tri_ref.size(), st);
     if (!rc) rc = upload_vec(s->tri_light, tri_light.data(), tri_light.size(), st);
@@ -474,7 +493,7 @@ int rrh_scene_destroy(rrh_scene* s) {
     if (!s) return RRH_OK;
     (void)hipDeviceSynchronize();  // the owning context may already be gone
     DevBuf* bufs[] = {&s->nodes, &s->nodes4, &s->nodesq, &s->tris, &s->tri_ref, &s->tri_light, &s->tl, &s->verts, &s->idx,
-                      &s->meshes, &s->mats, &s->insts, &s->lights, &s->cam};
+                      &s->meshes, &s->mats, &s->insts, &s->lights, &s->cam, &s->twd, &s->refit};
     for (DevBuf* b : bufs) b->release();
     delete s;
     return RRH_OK;
@@ -498,10 +517,48 @@ int rrh_scene_info_get(const rrh_scene* s, rrh_scene_info* o) {
     return RRH_OK;
 }
 
+// the host copy of the world triangles, refreshed from the device after an instance update
+static int sync_tri_world(const rrh_scene* s) {
+    if (!s->tri_world_stale) return RRH_OK;
+    RRH_HIP(hipMemcpyAsync(s->tri_world.data(), s->twd.p, (size_t)s->ntri * 36, hipMemcpyDeviceToHost, s->ctx->stream));
+    RRH_HIP(hipStreamSynchronize(s->ctx->stream));
+    s->tri_world_stale = false;
+    return RRH_OK;
+}
+
 int rrh_scene_triangles(const rrh_scene* s, float* out, uint32_t max_tris) {
     RRH_ARG(s && out, "rrh_scene_triangles: null argument");
     RRH_ARG(max_tris >= s->ntri, "rrh_scene_triangles: output too small");
+    int rc = sync_tri_world(s);
+    if (rc) return rc;
     std::memcpy(out, s->tri_world.data(), (size_t)s->ntri * 36);
+    return RRH_OK;
+}
+
+int rrh_scene_update_instances(rrh_ctx* c, rrh_scene* s, const rrh_instance* insts, uint32_t ninst, int mode) {
+    RRH_ARG(c && s && insts, "rrh_scene_update_instances: null argument");
+    RRH_ARG(s->ctx == c, "rrh_scene_update_instances: scene belongs to another context");
+    RRH_ARG(mode == RRH_UPDATE_REFIT || mode == RRH_UPDATE_REBUILD, "rrh_scene_update_instances: bad mode");
+    RRH_ARG(ninst == s->ninst, "rrh_scene_update_instances: instance count differs from the upload");
+    for (uint32_t i = 0; i < ninst; ++i)
+        RRH_ARG(insts[i].mesh_index == s->inst_mesh[i], "rrh_scene_update_instances: an instance changed its mesh");
+    auto t0 = std::chrono::steady_clock::now();
+    hipStream_t st = c->stream;
+    RRH_HIP(hipMemcpyAsync(s->insts.p, insts, (size_t)ninst * sizeof(rrh_instance), hipMemcpyHostToDevice, st));
+    int rc = flatten_device(c, s);
+    if (rc) return rc;
+    if (mode == RRH_UPDATE_REFIT) {
+        rc = refit_bvh_device(c, s);
+    } else if (c->bvh_builder == RRH_BVH_BUILD_DEVICE) {
+        rc = build_bvh_device(c, s, (const float*)s->twd.p, c->bvh_max_leaf, (float)c->bvh_trav_cost * 0.01f,
+                              c->ploc_radius);
+    } else {
+        rc = sync_tri_world(s);
+        if (!rc) rc = host_build_upload(c, s);
+    }
+    if (rc) return rc;
+    RRH_HIP(hipStreamSynchronize(st));
+    s->build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return RRH_OK;
 }
 

@@ -57,9 +57,15 @@ struct BvhOut {
 bool build_bvh(const float* tri_v, uint32_t ntri, BvhOut& out, std::string& err, uint32_t max_leaf = 8,
                float trav_cost = 1.0f);
 
-// BVH build on the device (rrh_bvh_gpu.hip) from s->tri_world: fills s->nodes, nodes4, nodesq,
-// tris and the tree statistics; RRH_OK or an error code with the message set
-int build_bvh_device(rrh_ctx* c, rrh_scene* s, uint32_t max_leaf, float trav_cost, uint32_t radius);
+// BVH build on the device (rrh_bvh_gpu.hip) from the world triangles tw (device, ntri x 9
+// floats): fills s->nodes, nodes4, nodesq, tris and the tree statistics; RRH_OK or an error
+// code with the message set
+int build_bvh_device(rrh_ctx* c, rrh_scene* s, const float* tw, uint32_t max_leaf, float trav_cost, uint32_t radius);
+// world triangles of the scene's current instances into s->twd, triangle-light records and
+// bounds updated (rrh_bvh_gpu.hip)
+int flatten_device(rrh_ctx* c, rrh_scene* s);
+// every box of the three node formats recomputed bottom-up from s->twd, topology kept
+int refit_bvh_device(rrh_ctx* c, rrh_scene* s);
 
 // procedural scene generators (rrh_scenegen.cpp)
 struct SceneData {
@@ -122,8 +128,14 @@ struct rrh_scene {
     uint32_t stack_need() const;   // traversal stack entries the traversed tree can need
     float build_ms = 0.0f;
     uint32_t build_passes = 0;     // device builder: clustering passes
+    uint32_t ninst = 0;
+    std::vector<uint32_t> inst_mesh;  // mesh index per instance (an update must keep them)
+    rrh::DevBuf twd;               // world triangles on the device (device builds, instance updates)
+    rrh::DevBuf refit;             // refit: parent links, inner-child counts, arrival counters
+    bool refit_ready = false;      // refit links describe the current tree
+    mutable bool tri_world_stale = false;  // host copy behind the device one (after an update)
     float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
-    std::vector<float> tri_world;   // flattened world triangles (host copy, 9 floats each)
+    mutable std::vector<float> tri_world;  // flattened world triangles (host copy, 9 floats each)
     rrh_camera cam_host{};
     bool cam_set = false;
     rrh::DevScene dev() const;
