@@ -12,8 +12,9 @@
 //     a symmetric key that orders all candidate pairs totally, so the smallest pair is
 //     always mutual and every pass merges); mutual pairs merge into a new node, the
 //     survivors are compacted by a prefix sum; repeat until one cluster is left;
-//  3. bottom-up leaf collapse (arrival counters): a subtree of at most max_leaf triangles
-//     becomes one leaf when its SAH cost as a leaf is not above its cost as a subtree;
+//  3. leaf collapse, decided as each node is made (its children are complete then): a
+//     subtree of at most max_leaf triangles becomes one leaf when its SAH cost as a leaf is
+//     not above its cost as a subtree;
 //  4. depth-first layout: each node walks to the root summing its left siblings' inner-node
 //     and triangle counts, which gives its binary node index and its leaf's first triangle
 //     (the host builder's emission order: parent first, left subtree before right);
@@ -81,6 +82,30 @@ __device__ __forceinline__ uint64_t spread21(uint32_t x) {
     return v;
 }
 
+// per-block min / max of three coordinates folded into bound[0..2] / bound[3..5] (ordered
+// integers) with six atomics per block; every thread of the block must call it
+__device__ void block_bounds(float lo[3], float hi[3], uint32_t* __restrict__ bound) {
+    __shared__ float part[kBlock / 64][6];
+    for (int off = 32; off > 0; off >>= 1)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fminf(lo[k], __shfl_xor(lo[k], off));
+            hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], off));
+        }
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 3; ++k) {
+            part[threadIdx.x >> 6][k] = lo[k];
+            part[threadIdx.x >> 6][3 + k] = hi[k];
+        }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        float v = part[0][threadIdx.x];
+        for (int w = 1; w < kBlock / 64; ++w)
+            v = threadIdx.x < 3 ? fminf(v, part[w][threadIdx.x]) : fmaxf(v, part[w][threadIdx.x]);
+        if (threadIdx.x < 3) atomicMin(&bound[threadIdx.x], ord_of(v));
+        else atomicMax(&bound[threadIdx.x], ord_of(v));
+    }
+}
+
 // ---- 1. boxes, Morton codes -----------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_tri_box(const float* __restrict__ tw, uint32_t n, float4* __restrict__ tbox,
                                                     uint32_t* __restrict__ cbound) {
@@ -97,16 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_tri_box(const float* __restrict__ tw
         tbox[2 * (size_t)i] = make_float4(lo[0], lo[1], lo[2], 0.0f);
         tbox[2 * (size_t)i + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
     }
-    for (int off = 32; off > 0; off >>= 1)
-        for (int k = 0; k < 3; ++k) {
-            cmin[k] = fminf(cmin[k], __shfl_xor(cmin[k], off));
-            cmax[k] = fmaxf(cmax[k], __shfl_xor(cmax[k], off));
-        }
-    if ((threadIdx.x & 63) == 0)
-        for (int k = 0; k < 3; ++k) {
-            atomicMin(&cbound[k], ord_of(cmin[k]));
-            atomicMax(&cbound[3 + k], ord_of(cmax[k]));
-        }
+    block_bounds(cmin, cmax, cbound);
 }
 
 __global__ void k_init_bounds(uint32_t* cbound) {
@@ -121,9 +137,12 @@ __global__ __launch_bounds__(kBlock) void k_morton(const float4* __restrict__ tb
     if (i >= n) return;
     const float4 lo = tbox[2 * (size_t)i], hi = tbox[2 * (size_t)i + 1];
     const float c[3] = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
+    // one scale for all axes (the largest extent): a flat scene keeps square Morton cells
+    float ext = 0.0f;
+    for (int k = 0; k < 3; ++k) ext = fmaxf(ext, float_of_ord(cbound[3 + k]) - float_of_ord(cbound[k]));
     uint64_t code = 0;
     for (int k = 0; k < 3; ++k) {
-        const float mn = float_of_ord(cbound[k]), ext = float_of_ord(cbound[3 + k]) - mn;
+        const float mn = float_of_ord(cbound[k]);
         const float t = (ext > 0.0f) ? (c[k] - mn) / ext : 0.0f;
         const uint32_t q = (t >= 0.0f) ? (uint32_t)fminf(t * 2097152.0f, 2097151.0f) : 0u;  // NaN -> 0
         code |= spread21(q) << (2 - k);
@@ -132,8 +151,27 @@ __global__ __launch_bounds__(kBlock) void k_morton(const float4* __restrict__ tb
     vals[i] = i;
 }
 
+struct Meta {  // per build node
+    uint32_t *parent, *cnt, *cost, *ninner, *mark, *first;
+};
+
+// SAH leaf collapse, decided when a node is made (its children are complete by then): a
+// subtree of at most max_leaf triangles becomes one leaf when its cost as a leaf (area x
+// triangles) is not above its cost as a subtree (area x traversal cost + children's costs)
+__device__ __forceinline__ void node_meta(Meta m, uint32_t k, float A, uint32_t L, uint32_t R, bool is_root,
+                                          uint32_t max_leaf, float trav) {
+    const uint32_t n = m.cnt[L] + m.cnt[R];
+    const float ci = A * trav + __uint_as_float(m.cost[L]) + __uint_as_float(m.cost[R]);
+    const float cl = A * (float)n;
+    const bool leaf = !is_root && n <= max_leaf && cl <= ci;
+    m.cnt[k] = n;
+    m.cost[k] = __float_as_uint(leaf ? cl : ci);
+    m.ninner[k] = leaf ? 0u : 1u + m.ninner[L] + m.ninner[R];
+    m.mark[k] = leaf ? 1u : 0u;
+}
+
 __global__ __launch_bounds__(kBlock) void k_leaves(const float4* __restrict__ tbox, const uint32_t* __restrict__ order,
-                                                   uint32_t n, PNode* __restrict__ nodes, uint32_t* __restrict__ cl) {
+                                                   uint32_t n, PNode* __restrict__ nodes, uint32_t* __restrict__ cl, Meta m) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= n) return;
     const uint32_t t = order[s];
@@ -143,6 +181,10 @@ __global__ __launch_bounds__(kBlock) void k_leaves(const float4* __restrict__ tb
     nodes[s].lo = lo;
     nodes[s].hi = hi;
     cl[s] = s;
+    m.cnt[s] = 1;
+    m.cost[s] = __float_as_uint(half_area(lo, hi));
+    m.ninner[s] = 0;
+    m.mark[s] = 1;
 }
 
 // ---- 2. PLOC ----------------------------------------------------------------------------------
@@ -205,7 +247,8 @@ __global__ __launch_bounds__(kBlock) void k_ploc_flags(const uint32_t* __restric
 __global__ __launch_bounds__(kBlock) void k_ploc_apply(PNode* __restrict__ nodes, const uint32_t* __restrict__ cl,
                                                        const uint32_t* __restrict__ nn, const uint64_t* __restrict__ sc,
                                                        uint32_t n, uint32_t next_node, uint32_t* __restrict__ cl_out,
-                                                       uint32_t* __restrict__ totals) {
+                                                       uint32_t* __restrict__ totals, Meta meta, uint32_t max_leaf,
+                                                       float trav) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const uint32_t j = nn[i];
@@ -221,6 +264,7 @@ __global__ __launch_bounds__(kBlock) void k_ploc_apply(PNode* __restrict__ nodes
         o.hi = make_float4(fmaxf(A.hi.x, B.hi.x), fmaxf(A.hi.y, B.hi.y), fmaxf(A.hi.z, B.hi.z), __uint_as_float(b));
         nodes[k] = o;
         cl_out[pos] = k;
+        node_meta(meta, k, half_area(o.lo, o.hi), a, b, n == 2, max_leaf, trav);
     } else if (survive) {
         cl_out[pos] = cl[i];
     }
@@ -230,49 +274,14 @@ __global__ __launch_bounds__(kBlock) void k_ploc_apply(PNode* __restrict__ nodes
     }
 }
 
-// ---- 3. parents, bottom-up leaf collapse ----------------------------------------------------------
+// ---- 3. parents --------------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_parents(const PNode* __restrict__ nodes, uint32_t n_leaf, uint32_t n_node,
-                                                    uint32_t* __restrict__ parent, uint32_t* __restrict__ arrive) {
+                                                    uint32_t* __restrict__ parent) {
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-    if (k < n_node) arrive[k] = 0;
     if (k < n_leaf || k >= n_node) return;
     parent[left_of(nodes[k])] = k;
     parent[right_of(nodes[k])] = k;
     if (k == n_node - 1) parent[k] = kNone;
-}
-
-struct Meta {  // per build node
-    uint32_t *parent, *arrive, *cnt, *cost, *ninner, *mark, *first;
-};
-
-__global__ __launch_bounds__(kBlock) void k_collapse(const PNode* __restrict__ nodes, uint32_t n_leaf, uint32_t root,
-                                                     Meta m, uint32_t max_leaf, float trav) {
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n_leaf) return;
-    st_u(&m.cnt[s], 1u);
-    st_u(&m.cost[s], __float_as_uint(half_area(nodes[s].lo, nodes[s].hi)));
-    st_u(&m.ninner[s], 0u);
-    st_u(&m.mark[s], 1u);
-    uint32_t c = s;
-    for (;;) {
-        const uint32_t p = m.parent[c];
-        if (p == kNone) break;
-        __threadfence();
-        if (atomicAdd(&m.arrive[p], 1u) == 0u) return;  // the sibling's thread finishes the parent
-        __threadfence();
-        const PNode P = nodes[p];
-        const uint32_t L = left_of(P), R = right_of(P);
-        const uint32_t n = ld_u(&m.cnt[L]) + ld_u(&m.cnt[R]);
-        const float A = half_area(P.lo, P.hi);
-        const float ci = A * trav + __uint_as_float(ld_u(&m.cost[L])) + __uint_as_float(ld_u(&m.cost[R]));
-        const float cl = A * (float)n;
-        const bool leaf = p != root && n <= max_leaf && cl <= ci;
-        st_u(&m.cnt[p], n);
-        st_u(&m.cost[p], __float_as_uint(leaf ? cl : ci));
-        st_u(&m.ninner[p], leaf ? 0u : 1u + ld_u(&m.ninner[L]) + ld_u(&m.ninner[R]));
-        st_u(&m.mark[p], leaf ? 1u : 0u);
-        c = p;
-    }
 }
 
 // ---- 4. depth-first layout: binary nodes and leaf-ordered triangles --------------------------------
@@ -667,9 +676,18 @@ int build_bvh_device(rrh_ctx* c, rrh_scene* s, const float* tw, uint32_t max_lea
     RRH_TRY(T.flags.ensure((size_t)n * 8));
     RRH_TRY(T.scan.ensure((size_t)n * 8));
     RRH_TRY(T.small.ensure(32 * sizeof(uint32_t)));  // [0..1] totals, [8..9] layout stats, [16] validation bits
+    RRH_TRY(T.meta.ensure((size_t)n_node * 6 * sizeof(uint32_t)));
+    Meta m;
+    uint32_t* mb = (uint32_t*)T.meta.p;
+    m.parent = mb;
+    m.cnt = mb + (size_t)n_node;
+    m.cost = mb + 2 * (size_t)n_node;
+    m.ninner = mb + 3 * (size_t)n_node;
+    m.mark = mb + 4 * (size_t)n_node;
+    m.first = mb + 5 * (size_t)n_node;
     PNode* nodes = (PNode*)T.nodes.p;
     k_leaves<<<grid(n), kBlock, 0, st>>>((const float4*)T.tbox.p, (const uint32_t*)T.vals2.p, n, nodes,
-                                         (uint32_t*)T.cl0.p);
+                                         (uint32_t*)T.cl0.p, m);
     RRH_LAUNCH_CHECK();
     uint32_t* cl = (uint32_t*)T.cl0.p;
     uint32_t* cl_next = (uint32_t*)T.cl1.p;
@@ -682,7 +700,7 @@ int build_bvh_device(rrh_ctx* c, rrh_scene* s, const float* tw, uint32_t max_lea
         RRH_PRIM(rocprim::exclusive_scan(T.rp.p, bytes, (const uint64_t*)T.flags.p, (uint64_t*)T.scan.p, (uint64_t)0,
                                          live, rocprim::plus<uint64_t>(), st));
         k_ploc_apply<<<grid(live), kBlock, 0, st>>>(nodes, cl, (const uint32_t*)T.nn.p, (const uint64_t*)T.scan.p, live,
-                                                    next_node, cl_next, (uint32_t*)T.small.p);
+                                                    next_node, cl_next, (uint32_t*)T.small.p, m, max_leaf, trav_cost);
         RRH_LAUNCH_CHECK();
         uint32_t tot[2];
         RRH_TRY(read_back(tot, T.small.p, 8, st));
@@ -698,19 +716,8 @@ int build_bvh_device(rrh_ctx* c, rrh_scene* s, const float* tw, uint32_t max_lea
     if (next_node != n_node) { set_error("BVH build: internal error (node count)"); return RRH_E_STATE; }
     T.cl0.release(); T.cl1.release(); T.nn.release(); T.flags.release(); T.scan.release(); T.vals.release();
 
-    // 3. parents and leaf collapse
-    RRH_TRY(T.meta.ensure((size_t)n_node * 7 * sizeof(uint32_t)));
-    Meta m;
-    uint32_t* mb = (uint32_t*)T.meta.p;
-    m.parent = mb;
-    m.arrive = mb + (size_t)n_node;
-    m.cnt = mb + 2 * (size_t)n_node;
-    m.cost = mb + 3 * (size_t)n_node;
-    m.ninner = mb + 4 * (size_t)n_node;
-    m.mark = mb + 5 * (size_t)n_node;
-    m.first = mb + 6 * (size_t)n_node;
-    k_parents<<<grid(n_node), kBlock, 0, st>>>(nodes, n, n_node, m.parent, m.arrive);
-    k_collapse<<<grid(n), kBlock, 0, st>>>(nodes, n, root, m, max_leaf, trav_cost);
+    // 3. parents (the leaf collapse was decided as each node was made)
+    k_parents<<<grid(n_node), kBlock, 0, st>>>(nodes, n, n_node, m.parent);
     RRH_LAUNCH_CHECK();
     uint32_t nbin = 0;
     RRH_TRY(read_back(&nbin, m.ninner + root, 4, st));
@@ -826,16 +833,7 @@ __global__ __launch_bounds__(kBlock) void k_flatten(const uint2* __restrict__ tr
             }
         }
     }
-    for (int off = 32; off > 0; off >>= 1)
-        for (int k = 0; k < 3; ++k) {
-            lo[k] = fminf(lo[k], __shfl_xor(lo[k], off));
-            hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], off));
-        }
-    if ((threadIdx.x & 63) == 0)
-        for (int k = 0; k < 3; ++k) {
-            atomicMin(&bound[k], ord_of(lo[k]));
-            atomicMax(&bound[3 + k], ord_of(hi[k]));
-        }
+    block_bounds(lo, hi, bound);
 }
 
 // triangle-light records from the moved triangles (the upload's fp32 expressions, same order)
