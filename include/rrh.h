@@ -133,11 +133,25 @@ int rrh_scene_desc_free(rrh_scene_desc* d);
 #define RRH_FMT_RGBA16F 1
 #define RRH_FMT_RGBA64F 2          /* FP64 accumulation */
 #define RRH_FMT_R32U 3
+#define RRH_FMT_RGBA8 4            /* 8-bit RGBA (tonemapped display / screenshot) */
 int rrh_image_create(rrh_ctx* ctx, uint32_t width, uint32_t height, int format, rrh_image** out);
 int rrh_image_destroy(rrh_image* img);
 int rrh_image_upload(rrh_ctx* ctx, rrh_image* img, const void* host, size_t bytes);
 int rrh_image_download(rrh_ctx* ctx, const rrh_image* img, void* host, size_t bytes);
 int rrh_image_clear(rrh_ctx* ctx, rrh_image* img);
+
+/* ---- G-buffer: replaces GBufferPass::execute + gbuffer.slang (src/passes/gbuffer/
+ *      GBufferPass.cpp:216-330): pixel-centre rays, back faces culled, depth within the
+ *      camera's near / far planes; position RGBA32F (xyz, 1), normal RGBA32F (xyz,
+ *      asfloat(material)), material id R32U; misses keep (0, 0, 0xFFFFFFFF) ------------- */
+int rrh_gbuffer(rrh_ctx* ctx, rrh_scene* s, rrh_image* position, rrh_image* normal, rrh_image* material_id);
+/* ---- screenshot tonemap: replaces the host loop of Application::capture_screenshot
+ *      (src/app/Application.cpp:853-866): ACES (Narkowicz) of hdr * exposure, clamp, gamma
+ *      1/2.2, 8 bits rounded to nearest (exactly); RGBA32F or RGBA64F in, RGBA8 out ------- */
+int rrh_tonemap(rrh_ctx* ctx, const rrh_image* hdr, float exposure, rrh_image* ldr);
+/* the 256 fp32 thresholds of that rounding (T_0 = 0; byte = largest k with T_k <= value), so
+ * host code can produce the same bytes */
+int rrh_tonemap_thresholds(float out[256]);
 
 /* ---- closest hit / occlusion: the RayQuery TraceRayInline + Proceed loop
  *      (assets/shaders/passes/pathtracer_reference/pathtracer.slang:45-75) -- */

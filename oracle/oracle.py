@@ -75,6 +75,8 @@ _SIGS = {
     "orc_trace_brute": (None, [P, P, U, P]),
     "orc_trace_bvh": (None, [P, P, U, P]),
     "orc_occluded": (C.c_int, [P, FP, FP, F, F]),
+    "orc_gbuffer": (None, [P, C.POINTER(Camera), P, P, P]),
+    "orc_tonemap": (None, [P, C.c_size_t, F, P]),
     "orc_brdf_eval": (None, [FP, FP, FP, FP, F, F, FP]),
     "orc_brdf_sample": (None, [FP, FP, FP, F, F, F, F, FP, FP, FP]),
     "orc_brdf_pdf": (F, [FP, FP, FP, F]),
@@ -107,6 +109,14 @@ _SIGS = {
     "orc_mlp_loss_grad": (C.c_double, [P, P, P, P, U, P]),
     "orc_adam": (None, [P, P, P, P, U, U, C.c_double, C.c_double, C.c_double, C.c_double]),
 }
+
+
+def tonemap(hdr: np.ndarray, exposure: float = 1.0) -> np.ndarray:
+    """Screenshot 8-bit values of an (..., 4) float32 image, Application.cpp:853-866 as written."""
+    hdr = np.ascontiguousarray(hdr, np.float32)
+    out = np.zeros(hdr.shape, np.uint8)
+    lib().orc_tonemap(p(hdr), hdr.size // 4, float(exposure), p(out))
+    return out
 
 
 def lib() -> C.CDLL:
@@ -168,6 +178,14 @@ class Scene:
         hits = np.zeros((len(rays), 4), np.float32)
         (lib().orc_trace_brute if brute else lib().orc_trace_bvh)(self.h, p(rays), len(rays), p(hits))
         return hits
+
+    def gbuffer(self, cam: Camera):
+        """(position RGBA, normal RGBA, material id) of the raster G-buffer, by pixel-centre rays."""
+        n = cam.screen_width * cam.screen_height
+        pos, nrm = np.zeros((n, 4), np.float32), np.zeros((n, 4), np.float32)
+        mat = np.zeros(n, np.uint32)
+        lib().orc_gbuffer(self.h, C.byref(cam), p(pos), p(nrm), p(mat))
+        return pos, nrm, mat
 
     def pt_render(self, cam: Camera, max_bounces: int, frame_seed: int, pixels: np.ndarray, rr_start: int = 3,
                   threads: int = 0):

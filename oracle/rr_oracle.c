@@ -447,7 +447,19 @@ static inline int better(float t, uint32_t prim, float bt, uint32_t bp) {
     return t < bt || (t == bt && prim < bp);
 }
 
+/* a triangle faces the ray when its geometric normal (v1 - v0) x (v2 - v0) points against it:
+ * the counter-clockwise front face of the reference's G-buffer raster pipeline
+ * (GBufferPass.cpp:148-151, Pipeline.h:75) */
+static int front_facing(const float* tv, v3 d) {
+    v3 p0 = vld(tv), p1 = vld(tv + 3), p2 = vld(tv + 6);
+    return vdot(vcross(vsub(p1, p0), vsub(p2, p0)), d) < 0.0f;
+}
+
+static orc_hit trace_bvh_cull(const orc_scene* s, v3 o, v3 d, float tmin, float tmax, int cull);
 static orc_hit trace_bvh_one(const orc_scene* s, v3 o, v3 d, float tmin, float tmax) {
+    return trace_bvh_cull(s, o, d, tmin, tmax, 0);
+}
+static orc_hit trace_bvh_cull(const orc_scene* s, v3 o, v3 d, float tmin, float tmax, int cull) {
     orc_hit h; h.t = tmax; h.prim = ~0u; h.b1 = h.b2 = 0.0f;
     if (s->ntri == 0) return h;
     wray r; wray_setup(o, d, &r);
@@ -460,7 +472,8 @@ static orc_hit trace_bvh_one(const orc_scene* s, v3 o, v3 d, float tmin, float t
             for (uint32_t i = nd->first; i < nd->first + nd->count; ++i) {
                 uint32_t t = s->order[i];
                 float tt, b1, b2;
-                if (wt_hit(&r, &s->tri[(size_t)t * 9], tmin, h.t, &tt, &b1, &b2) && better(tt, t, h.t, h.prim)) {
+                if (wt_hit(&r, &s->tri[(size_t)t * 9], tmin, h.t, &tt, &b1, &b2) && better(tt, t, h.t, h.prim) &&
+                    (!cull || front_facing(&s->tri[(size_t)t * 9], d))) {
                     h.t = tt; h.prim = t; h.b1 = b1; h.b2 = b2;
                 }
             }
@@ -546,6 +559,62 @@ static hitinfo fetch_hit(const orc_scene* s, orc_hit h) {
 
 static hitinfo trace(const orc_scene* s, v3 o, v3 d, float tmin, float tmax) {
     return fetch_hit(s, trace_bvh_one(s, o, d, tmin, tmax));
+}
+
+/* G-buffer of the reference's raster pass (gbuffer.slang vs_main / fs_main, GBufferPass.cpp:
+ * 216-330) by one ray through each pixel centre: front faces only, depth between the near and
+ * far planes (view depth = t * dot(dir, camera forward)); position = barycentric blend of
+ * the transformed vertices, w = 1; normal = normalize(blend of normalize(M3 n_i)), w =
+ * asfloat(material); material id; a miss keeps the clear values (0, 0, 0xFFFFFFFF). */
+void orc_gbuffer(const orc_scene* s, const orc_camera* cam, float* pos, float* nrm, uint32_t* mat) {
+    const uint32_t W = cam->screen_width, H = cam->screen_height;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t py = 0; py < (int64_t)H; ++py)
+        for (uint32_t px = 0; px < W; ++px) {
+            const size_t i = (size_t)py * W + px;
+            v3 d = center_ray_dir(cam, px, (uint32_t)py);
+            v3 o = vld(cam->position);
+            float dd = vdot(d, vld(cam->direction));
+            float* P = pos + i * 4;
+            float* N = nrm + i * 4;
+            P[0] = P[1] = P[2] = P[3] = 0.0f;
+            N[0] = N[1] = N[2] = N[3] = 0.0f;
+            mat[i] = 0xFFFFFFFFu;
+            if (!(dd > 0.0f)) continue;
+            orc_hit h = trace_bvh_cull(s, o, d, cam->near_plane / dd, cam->far_plane / dd, 1);
+            if (h.prim == ~0u) continue;
+            const orc_instance* in = &s->inst[s->tri_inst[h.prim]];
+            const orc_mesh* me = &s->m[in->mesh_index];
+            uint32_t base = me->index_byte_offset / 4 + s->tri_prim[h.prim] * 3;
+            const orc_vertex* v0 = &s->v[me->vertex_base + s->idx[base]];
+            const orc_vertex* v1 = &s->v[me->vertex_base + s->idx[base + 1]];
+            const orc_vertex* v2 = &s->v[me->vertex_base + s->idx[base + 2]];
+            float b0 = 1.0f - h.b1 - h.b2;
+            v3 w0 = xform_point(in->transform, vld(v0->pos)), w1 = xform_point(in->transform, vld(v1->pos)),
+               w2 = xform_point(in->transform, vld(v2->pos));
+            v3 n0 = vnorm(xform_dir(in->transform, vld(v0->normal))), n1 = vnorm(xform_dir(in->transform, vld(v1->normal))),
+               n2 = vnorm(xform_dir(in->transform, vld(v2->normal)));
+            v3 p = vadd(vadd(vscale(w0, b0), vscale(w1, h.b1)), vscale(w2, h.b2));
+            v3 n = vnorm(vadd(vadd(vscale(n0, b0), vscale(n1, h.b1)), vscale(n2, h.b2)));
+            vst(P, p); P[3] = 1.0f;
+            vst(N, n); N[3] = asfloat(me->material_index);
+            mat[i] = me->material_index;
+        }
+}
+
+/* the screenshot's 8-bit values, Application.cpp:853-866 as written: ACES (Narkowicz) on the
+ * exposed value, clamp to [0, 1], powf(v, 1 / 2.2), (uint8)(v * 255 + 0.5); a NaN counts as 0 */
+void orc_tonemap(const float* hdr, size_t pixels, float exposure, uint8_t* ldr) {
+    for (size_t i = 0; i < pixels; ++i) {
+        for (int c = 0; c < 3; ++c) {
+            float v = hdr[i * 4 + c] * exposure;
+            v = (v * (2.51f * v + 0.03f)) / (v * (2.43f * v + 0.59f) + 0.14f);
+            v = fminf(fmaxf(v, 0.0f), 1.0f);
+            v = powf(v, 1.0f / 2.2f);
+            ldr[i * 4 + c] = (uint8_t)(v * 255.0f + 0.5f);
+        }
+        ldr[i * 4 + 3] = 255u;
+    }
 }
 
 /* ========================================================================== */

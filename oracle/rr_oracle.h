@@ -87,6 +87,10 @@ void orc_trace_brute(const orc_scene* s, const float* rays /* n x 8: o3,tmin,d3,
                      uint32_t n, orc_hit* hits);
 /* same result through the oracle's own median-split BVH */
 void orc_trace_bvh(const orc_scene* s, const float* rays, uint32_t n, orc_hit* hits);
+/* G-buffer by pixel-centre rays, front faces only (GBufferPass.cpp:216-330, gbuffer.slang) */
+void orc_gbuffer(const orc_scene* s, const orc_camera* cam, float* pos, float* nrm, uint32_t* mat);
+/* ACES + gamma 8-bit screenshot values (Application.cpp:853-866), with powf as written */
+void orc_tonemap(const float* hdr, size_t pixels, float exposure, uint8_t* ldr);
 int  orc_occluded(const orc_scene* s, const float o[3], const float d[3], float tmin, float tmax);
 
 /* ---- BRDF: BRDF.slang:16-187 ------------------------------------------------- */

@@ -32,8 +32,8 @@ RRH_OK = 0
 MAT_PURE_LAMBERT = 1
 SCENE_TRI_LIGHTS = 1
 GEN_CORNELL_REF, GEN_CORNELL_RANDOM, GEN_HEIGHTFIELD, GEN_INDOOR = 0, 1, 2, 3
-FMT_RGBA32F, FMT_RGBA16F, FMT_RGBA64F, FMT_R32U = 0, 1, 2, 3
-FMT_BYTES = {FMT_RGBA32F: 16, FMT_RGBA16F: 8, FMT_RGBA64F: 32, FMT_R32U: 4}
+FMT_RGBA32F, FMT_RGBA16F, FMT_RGBA64F, FMT_R32U, FMT_RGBA8 = 0, 1, 2, 3, 4
+FMT_BYTES = {FMT_RGBA32F: 16, FMT_RGBA16F: 8, FMT_RGBA64F: 32, FMT_R32U: 4, FMT_RGBA8: 4}
 GUIDE_NONE, GUIDE_VMM, GUIDE_MLP = 0, 1, 2
 PT_COUNT_FETCHES = 1
 OPT_TRAVERSAL_STACK, OPT_XCD_REMAP, OPT_BVH_NODES, OPT_PERSISTENT, OPT_SPECULATIVE = 1, 2, 3, 4, 5
@@ -122,6 +122,9 @@ SIGNATURES = {
     "rrh_scene_info_get": [P, C.POINTER(SceneInfo)],
     "rrh_scene_triangles": [P, P, u32],
     "rrh_scene_update_instances": [P, P, P, u32, i32],
+    "rrh_gbuffer": [P, P, P, P, P],
+    "rrh_tonemap": [P, P, f32, P],
+    "rrh_tonemap_thresholds": [P],
     "rrh_camera_set": [P, C.POINTER(Camera)],
     "rrh_camera_look_at": [C.POINTER(f32 * 3), C.POINTER(f32 * 3), f32, u32, u32, f32, f32, u32, C.POINTER(Camera)],
     "rrh_scene_generate": [i32, u32, f32, C.POINTER(SceneDesc)],

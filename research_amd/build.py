@@ -34,6 +34,7 @@ SOURCES = [
     "rrh_mlp.hip",
     "rrh_bvh.cpp",
     "rrh_bvh_gpu.hip",
+    "rrh_gbuffer.hip",
     "rrh_scenegen.cpp",
 ]
 

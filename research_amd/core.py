@@ -181,6 +181,8 @@ class Image:
             return np.uint16, (self.h_, self.w, 4)
         if self.fmt == L.FMT_RGBA64F:
             return np.float64, (self.h_, self.w, 4)
+        if self.fmt == L.FMT_RGBA8:
+            return np.uint8, (self.h_, self.w, 4)
         return np.uint32, (self.h_, self.w)
 
     def download(self) -> np.ndarray:

@@ -31,6 +31,7 @@ size_t format_bytes(int fmt) {
     case RRH_FMT_RGBA16F: return 8;
     case RRH_FMT_RGBA64F: return 32;
     case RRH_FMT_R32U: return 4;
+    case RRH_FMT_RGBA8: return 4;
     default: return 0;
     }
 }

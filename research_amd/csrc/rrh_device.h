@@ -591,7 +591,9 @@ RRH_D void load_node4(const DevScene& S, int node, Node4& N) {
 
 // closest hit over the 4-wide nodes: children visited nearest first (the rest pushed far
 // to near); the same leaf tests as the binary traversal, so the same answer
-template <bool COUNT, bool Q, class Stk>
+// CULL: only triangles facing the ray count (geometric normal against the ray direction),
+// the front faces of the reference's G-buffer raster pipeline
+template <bool COUNT, bool Q, bool CULL = false, class Stk>
 RRH_D Hit trace_closest4_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack,
                            uint32_t* n_nodes, uint32_t* n_tris) {
     Hit h; h.t = tmax; h.prim = ~0u; h.b1 = 0.0f; h.b2 = 0.0f;
@@ -640,6 +642,10 @@ RRH_D Hit trace_closest4_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax
             float tt, b1, b2;
             uint32_t id = __float_as_uint(p0.w);
             if (wt_hit(r, xyz(p0), xyz(p1), xyz(p2), tmin, h.t, tt, b1, b2) && better(tt, id, h.t, h.prim)) {
+                if constexpr (CULL) {
+                    const f3 a = xyz(p0);
+                    if (!(dot(cross(sub(xyz(p1), a), sub(xyz(p2), a)), d) < 0.0f)) continue;
+                }
                 h.t = tt; h.prim = id; h.b1 = b1; h.b2 = b2;
             }
         }

@@ -33,11 +33,15 @@ void Application::initialize() {
 
 void Application::initialize_renderer() {
     const rhi::Extent2D e{(uint32_t)width_, (uint32_t)height_};
+    gbuffer_pass_ = renderer_.add_pass<passes::GBufferPass>();  // first, as Application.cpp:218-221
     pathtracer_pass_ = renderer_.add_pass<passes::PathTracerPass>();
     guiding_pass_ = renderer_.add_pass<passes::GuidingPass>();
     accumulate_pass_ = renderer_.add_pass<passes::AccumulatePass>();
     restir_di_pass_ = renderer_.add_pass<passes::ReSTIRDIPass>();
     atrous_pass_ = renderer_.add_pass<passes::AtrousPass>();
+    tonemap_pass_ = renderer_.add_pass<passes::TonemapPass>();  // last: the display image
+    gbuffer_pass_->initialize(device_, e);
+    tonemap_pass_->initialize(device_, e);
     pathtracer_pass_->initialize(device_, e);
     guiding_pass_->initialize(device_, e);
     accumulate_pass_->initialize(device_, e);
@@ -59,6 +63,8 @@ void Application::shutdown() {
         if (auto* q = dynamic_cast<passes::AccumulatePass*>(p.get())) q->shutdown(device_);
         if (auto* q = dynamic_cast<passes::ReSTIRDIPass*>(p.get())) q->shutdown(device_);
         if (auto* q = dynamic_cast<passes::AtrousPass*>(p.get())) q->shutdown(device_);
+        if (auto* q = dynamic_cast<passes::GBufferPass*>(p.get())) q->shutdown(device_);
+        if (auto* q = dynamic_cast<passes::TonemapPass*>(p.get())) q->shutdown(device_);
     }
     renderer_.clear();
     pathtracer_pass_ = nullptr;
@@ -66,6 +72,8 @@ void Application::shutdown() {
     accumulate_pass_ = nullptr;
     restir_di_pass_ = nullptr;
     atrous_pass_ = nullptr;
+    gbuffer_pass_ = nullptr;
+    tonemap_pass_ = nullptr;
     display_ = nullptr;
     scene_.destroy();
     device_.destroy();
@@ -192,6 +200,8 @@ void Application::step() {
     restir_di_pass_->set_enabled(use_di_);
     atrous_pass_->set_enabled(use_denoise);
     accumulate_pass_->set_input(use_guiding_ ? &guiding_pass_->output_image() : &pathtracer_pass_->output_image());
+    tonemap_pass_->set_input(current_display_image());
+    tonemap_pass_->exposure = exposure_;
 
     scene_.update_camera(camera_, (uint32_t)width_, (uint32_t)height_, frames_rendered_);
     accumulate_pass_->camera_moved = camera_moved_;
@@ -223,9 +233,13 @@ std::vector<float> Application::display_image() const {
 }
 
 std::vector<uint8_t> Application::display_ldr() const {
-    std::vector<float> hdr = display_image();
+    // the device tonemap at the current exposure (bytes identical to tonemap_ldr on the host)
+    if (!display_) throw std::runtime_error("display_ldr: no frame rendered yet");
+    rhi::Device& d = const_cast<rhi::Device&>(device_);
+    tonemap_pass_->exposure = exposure_;
+    tonemap_pass_->run(*display_);
     std::vector<uint8_t> ldr((size_t)width_ * height_ * 4);
-    tonemap_ldr(hdr.data(), (size_t)width_ * height_, exposure_, ldr.data());
+    tonemap_pass_->output_image().download(d, ldr.data(), ldr.size());
     return ldr;
 }
 
@@ -240,13 +254,16 @@ void Application::capture_screenshot(const std::string& path) {
 
 // ---- tonemap + PNG ------------------------------------------------------------------------------
 void tonemap_ldr(const float* hdr, size_t pixels, float exposure, uint8_t* ldr) {
+    // the reference's per-channel formula; the gamma and 8-bit rounding taken exactly through
+    // the library's thresholds, so these bytes equal the device tonemap's
+    float T[256];
+    check(rrh_tonemap_thresholds(T), "rrh_tonemap_thresholds");
     for (size_t i = 0; i < pixels; ++i) {
         for (int c = 0; c < 3; ++c) {
             float v = hdr[i * 4 + c] * exposure;
             v = (v * (2.51f * v + 0.03f)) / (v * (2.43f * v + 0.59f) + 0.14f);
-            v = std::min(std::max(v, 0.0f), 1.0f);
-            v = std::pow(v, 1.0f / 2.2f);
-            ldr[i * 4 + c] = (uint8_t)(v * 255.0f + 0.5f);
+            v = std::fmin(std::fmax(v, 0.0f), 1.0f);
+            ldr[i * 4 + c] = (uint8_t)(std::upper_bound(T + 1, T + 256, v) - (T + 1));
         }
         ldr[i * 4 + 3] = 255u;
     }

@@ -14,6 +14,26 @@ using namespace rrh::passes;
 PYBIND11_MODULE(_app, m) {
     m.doc() = "Headless research renderer on MI355X (HIP): the reference Application API.";
 
+    auto image_f32 = [](const rrh::rhi::Image& im, rrh::rhi::Device* d) {
+        py::array_t<float> out({(py::ssize_t)im.extent().height, (py::ssize_t)im.extent().width, (py::ssize_t)4});
+        im.download(*d, out.mutable_data(), (size_t)out.nbytes());
+        return out;
+    };
+    py::class_<GBufferPass>(m, "GBufferPass")
+        .def("is_enabled", &GBufferPass::is_enabled)
+        .def_property_readonly("name", [](const GBufferPass& p) { return std::string(p.name()); })
+        .def("position", [image_f32](const GBufferPass& p) { return image_f32(p.position_image(), p.device()); })
+        .def("normal", [image_f32](const GBufferPass& p) { return image_f32(p.normal_image(), p.device()); })
+        .def("material_id", [](const GBufferPass& p) {
+            const auto& im = p.material_id_image();
+            py::array_t<uint32_t> out({(py::ssize_t)im.extent().height, (py::ssize_t)im.extent().width});
+            im.download(*p.device(), out.mutable_data(), (size_t)out.nbytes());
+            return out;
+        });
+    py::class_<TonemapPass>(m, "TonemapPass")
+        .def_readwrite("exposure", &TonemapPass::exposure)
+        .def("is_enabled", &TonemapPass::is_enabled)
+        .def_property_readonly("name", [](const TonemapPass& p) { return std::string(p.name()); });
     py::class_<PathTracerPass>(m, "PathTracerPass")
         .def_readwrite("max_bounces", &PathTracerPass::max_bounces)
         .def_readwrite("rr_start", &PathTracerPass::rr_start)
@@ -122,7 +142,9 @@ PYBIND11_MODULE(_app, m) {
         .def_property_readonly("accumulate_pass", &Application::accumulate_pass, py::return_value_policy::reference_internal)
         .def_property_readonly("restir_di_pass", &Application::restir_di_pass, py::return_value_policy::reference_internal)
         .def_property_readonly("atrous_pass", &Application::atrous_pass, py::return_value_policy::reference_internal)
-        .def_property_readonly("guiding_pass", &Application::guiding_pass, py::return_value_policy::reference_internal);
+        .def_property_readonly("guiding_pass", &Application::guiding_pass, py::return_value_policy::reference_internal)
+        .def_property_readonly("gbuffer_pass", &Application::gbuffer_pass, py::return_value_policy::reference_internal)
+        .def_property_readonly("tonemap_pass", &Application::tonemap_pass, py::return_value_policy::reference_internal);
 
     m.def(
         "load_gltf",

@@ -19,6 +19,74 @@ static RenderPass::ResourceDesc desc(const char* name, Format f, Extent2D e, boo
     return d;
 }
 
+// ---- GBufferPass (GBufferPass.cpp:216-330) ----------------------------------------------------
+GBufferPass::~GBufferPass() {
+    position_img_.destroy();
+    normal_img_.destroy();
+    material_id_img_.destroy();
+}
+void GBufferPass::initialize(rhi::Device& device, Extent2D extent) {
+    device_ = &device;
+    initialized_ = true;
+    on_resize(extent);
+}
+void GBufferPass::shutdown(rhi::Device&) {
+    position_img_.destroy();
+    normal_img_.destroy();
+    material_id_img_.destroy();
+    initialized_ = false;
+}
+RenderPass::Reflection GBufferPass::reflect() const {
+    Reflection r;
+    r.outputs.push_back(desc("gbuffer_position", Format::RGBA32F, extent_, false));
+    r.outputs.push_back(desc("gbuffer_normal", Format::RGBA32F, extent_, false));
+    r.outputs.push_back(desc("gbuffer_material_id", Format::R32U, extent_, false));
+    return r;
+}
+void GBufferPass::on_resize(Extent2D e) {
+    if (!initialized_) return;
+    extent_ = e;
+    position_img_.create(*device_, e, Format::RGBA32F);
+    normal_img_.create(*device_, e, Format::RGBA32F);
+    material_id_img_.create(*device_, e, Format::R32U);
+}
+void GBufferPass::execute(FrameContext& fc) {
+    if (!initialized_ || !fc.scene || !fc.scene->valid()) return;
+    check(rrh_gbuffer(device_->ctx(), fc.scene->handle(), position_img_.handle(), normal_img_.handle(),
+                      material_id_img_.handle()),
+          "rrh_gbuffer");
+}
+
+// ---- TonemapPass (TonemapPass.cpp, tonemap.slang; screenshot Application.cpp:853-866) --------
+TonemapPass::~TonemapPass() { ldr_img_.destroy(); }
+void TonemapPass::initialize(rhi::Device& device, Extent2D extent) {
+    device_ = &device;
+    initialized_ = true;
+    on_resize(extent);
+}
+void TonemapPass::shutdown(rhi::Device&) {
+    ldr_img_.destroy();
+    initialized_ = false;
+}
+RenderPass::Reflection TonemapPass::reflect() const {
+    Reflection r;
+    r.outputs.push_back(desc("ldr", Format::RGBA8, extent_, false));
+    return r;
+}
+void TonemapPass::on_resize(Extent2D e) {
+    if (!initialized_) return;
+    extent_ = e;
+    ldr_img_.create(*device_, e, Format::RGBA8);
+}
+void TonemapPass::run(const Image& hdr) {
+    if (!initialized_) throw std::runtime_error("TonemapPass: not initialized");
+    check(rrh_tonemap(device_->ctx(), hdr.handle(), exposure, ldr_img_.handle()), "rrh_tonemap");
+}
+void TonemapPass::execute(FrameContext&) {
+    if (!initialized_ || !input_) return;
+    run(*input_);
+}
+
 // ---- PathTracerPass (PathTracerPass.cpp:170-224) --------------------------------------------
 PathTracerPass::~PathTracerPass() { radiance_img_.destroy(); }
 void PathTracerPass::initialize(rhi::Device& device, Extent2D extent) {

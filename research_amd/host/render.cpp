@@ -30,6 +30,7 @@ static int to_rrh(Format f) {
         case Format::RGBA16F: return RRH_FMT_RGBA16F;
         case Format::RGBA64F: return RRH_FMT_RGBA64F;
         case Format::R32U: return RRH_FMT_R32U;
+        case Format::RGBA8: return RRH_FMT_RGBA8;
         default: throw std::runtime_error("image format undefined");
     }
 }
@@ -39,6 +40,7 @@ static size_t texel_bytes(Format f) {
         case Format::RGBA16F: return 8;
         case Format::RGBA64F: return 32;
         case Format::R32U: return 4;
+        case Format::RGBA8: return 4;
         default: return 0;
     }
 }

@@ -35,7 +35,7 @@ namespace rhi {
 struct Extent2D {
     uint32_t width = 0, height = 0;
 };
-enum class Format { Undefined, RGBA32F, RGBA16F, RGBA64F, R32U };
+enum class Format { Undefined, RGBA32F, RGBA16F, RGBA64F, R32U, RGBA8 };
 
 // One HIP device and its stream (replaces rr::rhi::Device).
 class Device {
@@ -200,6 +200,57 @@ private:
 namespace passes {
 using render::FrameContext;
 using render::RenderPass;
+
+// G-buffer (GBufferPass.h:19-90): position RGBA32F (xyz, 1), normal RGBA32F (xyz,
+// asfloat(material)), material id R32U, from pixel-centre rays with back faces culled.
+class GBufferPass : public RenderPass {
+public:
+    ~GBufferPass() override;
+    void initialize(rhi::Device& device, rhi::Extent2D extent);
+    void shutdown(rhi::Device& device);
+    bool reload_shader() { return true; }
+    [[nodiscard]] const char* name() const override { return "GBufferPass"; }
+    [[nodiscard]] Reflection reflect() const override;
+    void on_resize(rhi::Extent2D new_extent) override;
+    void execute(FrameContext& fc) override;
+    [[nodiscard]] const rhi::Image& position_image() const { return position_img_; }
+    [[nodiscard]] const rhi::Image& normal_image() const { return normal_img_; }
+    [[nodiscard]] const rhi::Image& material_id_image() const { return material_id_img_; }
+    [[nodiscard]] rhi::Device* device() const { return device_; }
+
+private:
+    rhi::Device* device_ = nullptr;
+    rhi::Image position_img_, normal_img_, material_id_img_;
+    rhi::Extent2D extent_{};
+    bool initialized_ = false;
+};
+
+// Tonemap (TonemapPass / tonemap.slang, the screenshot's Application.cpp:853-866): ACES +
+// gamma 1/2.2 into an RGBA8 image, on the device.
+class TonemapPass : public RenderPass {
+public:
+    ~TonemapPass() override;
+    void initialize(rhi::Device& device, rhi::Extent2D extent);
+    void shutdown(rhi::Device& device);
+    bool reload_shader() { return true; }
+    [[nodiscard]] const char* name() const override { return "TonemapPass"; }
+    [[nodiscard]] Reflection reflect() const override;
+    void on_resize(rhi::Extent2D new_extent) override;
+    void execute(FrameContext& fc) override;
+    void set_input(const rhi::Image* hdr) { input_ = hdr; }
+    // tonemap `hdr` now (the display path calls this with the current exposure)
+    void run(const rhi::Image& hdr);
+    [[nodiscard]] const rhi::Image& output_image() const { return ldr_img_; }
+
+    float exposure = 1.0f;
+
+private:
+    rhi::Device* device_ = nullptr;
+    const rhi::Image* input_ = nullptr;
+    rhi::Image ldr_img_;
+    rhi::Extent2D extent_{};
+    bool initialized_ = false;
+};
 
 // Reference path tracer: one sample per pixel per frame into an RGBA32F radiance image.
 class PathTracerPass : public RenderPass {
@@ -423,6 +474,8 @@ public:
     passes::AccumulatePass* accumulate_pass() { return accumulate_pass_; }
     passes::ReSTIRDIPass* restir_di_pass() { return restir_di_pass_; }
     passes::AtrousPass* atrous_pass() { return atrous_pass_; }
+    passes::GBufferPass* gbuffer_pass() { return gbuffer_pass_; }
+    passes::TonemapPass* tonemap_pass() { return tonemap_pass_; }
     passes::GuidingPass* guiding_pass() { return guiding_pass_; }
 
 private:
@@ -438,6 +491,8 @@ private:
     passes::AccumulatePass* accumulate_pass_ = nullptr;
     passes::ReSTIRDIPass* restir_di_pass_ = nullptr;
     passes::AtrousPass* atrous_pass_ = nullptr;
+    passes::GBufferPass* gbuffer_pass_ = nullptr;
+    passes::TonemapPass* tonemap_pass_ = nullptr;
     passes::GuidingPass* guiding_pass_ = nullptr;
     const rhi::Image* display_ = nullptr;
 

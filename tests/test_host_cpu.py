@@ -86,3 +86,39 @@ def test_png_writer_round_trip(tmp_path, h, w):
     hdr, back = _read_png(p)
     assert hdr == (w, h, 8, 6)
     assert np.array_equal(back, img)
+
+
+def _aces_clamped(v):
+    v = np.asarray(v, np.float32)
+    a = (v * (np.float32(2.51) * v + np.float32(0.03))) / (v * (np.float32(2.43) * v + np.float32(0.59)) + np.float32(0.14))
+    return np.fmin(np.fmax(a, np.float32(0)), np.float32(1))
+
+
+def test_tonemap_thresholds_are_the_exact_rounding_points():
+    """T_k is the smallest fp32 whose v^(1/2.2) * 255 reaches k - 1/2 (checked in float64)."""
+    import ctypes as C
+    from research_amd import _lib as L
+    T = np.zeros(256, np.float32)
+    L.call("rrh_tonemap_thresholds", T.ctypes.data_as(C.c_void_p))
+    assert T[0] == 0 and (np.diff(T) > 0).all()
+    k = np.arange(1, 256)
+    t64 = T[1:].astype(np.float64)
+    below = np.nextafter(T[1:], np.float32(0)).astype(np.float64)
+    assert (t64 ** (1 / 2.2) * 255 >= k - 0.5).all()
+    assert (below ** (1 / 2.2) * 255 < k - 0.5).all()
+
+
+def test_tonemap_matches_the_reference_formula():
+    """The host tonemap against the oracle's restatement of Application.cpp:853-866 (powf):
+    equal except where powf's rounding crosses a half-integer, and there the host byte is the
+    correctly rounded one."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(5)
+    hdr = np.ones((512, 512, 4), np.float32)
+    hdr[..., :3] = np.exp(rng.uniform(-9, 4, (512, 512, 3))).astype(np.float32)
+    a, b = _app.tonemap_ldr(hdr, 1.3), O.tonemap(hdr, 1.3)
+    d = a.astype(int) - b.astype(int)
+    assert np.abs(d).max() <= 1 and (d != 0).mean() < 1e-3
+    v = _aces_clamped(hdr[..., :3] * np.float32(1.3)).astype(np.float64)
+    exact = np.floor(v ** (1 / 2.2) * 255 + 0.5)
+    assert np.array_equal(a[..., :3], exact.astype(np.uint8))
