@@ -137,6 +137,8 @@ def main() -> int:
     ap.add_argument("--bvh-builder", choices=["host", "device"], default="host",
                     help="BVH builder: binned SAH on the host, or PLOC on the GPU (same node formats)")
     ap.add_argument("--ploc-radius", type=int, default=16, help="device builder's neighbour window (1..32)")
+    ap.add_argument("--ray-sort", type=int, default=0,
+                    help="extension rays traced in key order (octant + origin Morton): key bits, 0 = off")
     ap.add_argument("--bvh-nodes", choices=list(NODE_FORMATS), default="wide4q",
                     help="BVH node format traversed (binary 64 B, 4-wide 128 B, quantized 4-wide 64 B)")
     ap.add_argument("--guide-seconds", type=float, default=4.0,
@@ -165,6 +167,7 @@ def main() -> int:
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_BUILDER,
            L.BVH_BUILD_DEVICE if a.bvh_builder == "device" else L.BVH_BUILD_HOST)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_PLOC_RADIUS, a.ploc_radius)
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_RAY_SORT, a.ray_sort)
     if a.copy_calibration:
         g = C.c_float()
         L.call("rrh_stream_copy", ctx.h, 1 << 30, 5, C.byref(g))
@@ -242,7 +245,7 @@ def main() -> int:
                    "camera": "orbit 0.5 deg/frame", "parallelism": "single GPU",
                    "traversal_stack": a.stack, "xcd_remap": a.xcd_remap, "bvh_nodes": a.bvh_nodes, "persistent": a.persistent, "speculative": a.speculative,
                    "bvh_max_leaf": a.max_leaf, "bvh_trav_cost": a.trav_cost, "bvh_builder": a.bvh_builder,
-                   "bvh_build_ms": round(info["build_ms"], 1),
+                   "bvh_build_ms": round(info["build_ms"], 1), "ray_sort_bits": a.ray_sort,
                    "rays_per_step": {"primary": prim // a.steps, "extension": ext // a.steps,
                                      "shadow": shadow // a.steps}},
         "roofline": {"bound": "hbm", "kernel": "k_intersect (closest-hit BVH traversal)",

@@ -210,6 +210,8 @@ int rrh_fetch_counts(rrh_ctx* ctx, uint64_t* node_fetches, uint64_t* tri_fetches
 #define RRH_BVH_BUILD_HOST 0        /* binned SAH on the host threads (default) */
 #define RRH_BVH_BUILD_DEVICE 1      /* PLOC clustering + SAH leaf collapse on the GPU (same formats, same hits) */
 #define RRH_OPT_PLOC_RADIUS 9       /* device builder: clusters searched on each side, 1..32 (default 16) */
+#define RRH_OPT_RAY_SORT 10         /* extension rays traced in the order of a key (direction octant, then
+                                       origin Morton code); value = key bits sorted, 0 (off) or 4..32 */
 #define RRH_NODES_BINARY 0          /* binary, 64 B (both child boxes in fp32) */
 #define RRH_NODES_WIDE4 1           /* 4-wide, 128 B (fp32 child boxes) */
 #define RRH_NODES_WIDE4Q 2          /* 4-wide, 64 B (8-bit child boxes, outward rounded); default */

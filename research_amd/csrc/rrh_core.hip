@@ -108,6 +108,7 @@ int rrh_ctx_destroy(rrh_ctx* c) {
     c->hits.release(); c->sh_o.release(); c->sh_d.release(); c->sh_c.release();
     c->counters.release(); c->vtx.release(); c->di_counter.release(); c->scratch.release(); c->mlp_prep.release();
     c->train.release(); c->train_cnt.release(); c->totals.release(); c->fetch.release();
+    c->sort_keys.release(); c->sort_perm.release(); c->sort_tmp.release();
     for (hipEvent_t e : c->prof_events) hipEventDestroy(e);
     for (int i = 0; i < 16; ++i) hipEventDestroy(c->markers[i]);
     hipStreamDestroy(c->stream);
@@ -154,6 +155,11 @@ int rrh_ctx_set_option(rrh_ctx* c, int option, int value) {
     if (option == RRH_OPT_BVH_BUILDER) {
         RRH_ARG(value == RRH_BVH_BUILD_HOST || value == RRH_BVH_BUILD_DEVICE, "rrh_ctx_set_option: bad BVH builder");
         c->bvh_builder = value;
+        return RRH_OK;
+    }
+    if (option == RRH_OPT_RAY_SORT) {
+        RRH_ARG(value == 0 || (value >= 4 && value <= 32), "rrh_ctx_set_option: ray sort bits are 0 or 4..32");
+        c->ray_sort_bits = (uint32_t)value;
         return RRH_OK;
     }
     if (option == RRH_OPT_PLOC_RADIUS) {

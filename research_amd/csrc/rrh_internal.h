@@ -91,6 +91,8 @@ struct rrh_ctx {
     uint32_t bvh_trav_cost = 100;  // RRH_OPT_BVH_TRAV_COST: SAH traversal cost x 100 (later uploads)
     int bvh_builder = RRH_BVH_BUILD_HOST;  // RRH_OPT_BVH_BUILDER (later uploads)
     uint32_t ploc_radius = 16;     // RRH_OPT_PLOC_RADIUS: device builder's neighbour window
+    uint32_t ray_sort_bits = 0;    // RRH_OPT_RAY_SORT: 0 = extension rays traced in queue order
+    rrh::DevBuf sort_keys, sort_perm, sort_tmp;  // ray sort: keys / slots (x2 each), rocPRIM scratch
     hipStream_t stream = nullptr;
     hipEvent_t markers[16] = {};
     // path-tracer wavefront state (sized for the largest frame seen)

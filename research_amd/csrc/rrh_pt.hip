@@ -9,6 +9,9 @@
 // per-pixel RNG stream, so the draw order per pixel is the reference's:
 // jitter(2) | per bounce: [light pick (+2 for a triangle light)] [BSDF 2] [RR 1].
 // Queues are compacted with one wave-aggregated atomic per wave.
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "rrh_internal.h"
 #include "rrh_guide_dev.h"
 
@@ -64,16 +67,19 @@ __device__ __forceinline__ uint32_t block_push(uint32_t* counter, bool pred, uin
     return slot;
 }
 
+// perm (optional): the queue slots in tracing order (the ray sort); results stay in slot order
 template <bool COUNT, int STK>
 __global__ void __launch_bounds__(kBlock) k_intersect(DevScene S, Q in, const uint32_t* __restrict__ cnt, uint32_t b,
                                                       float4* __restrict__ hits, unsigned long long* __restrict__ fetch,
-                                                      bool remap) {
+                                                      bool remap, const uint32_t* __restrict__ perm) {
     __shared__ int s_stack[(kBlock / 64) * StackSel<STK>::kLds * 64];
     const uint32_t n = cnt[b];
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, remap);
     uint32_t nn = 0, nt = 0, nr = 0;
     auto st = make_stack<STK>(s_stack);
-    for (uint32_t i = blk * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (uint32_t j = blk * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const uint32_t i = perm ? perm[j] : j;
+        if (i >= n) continue;  // a sorted order holds only live slots; never trust it past the queue
         float4 o = in.o[i], d = in.d[i];
         Hit h = trace_closest_s<COUNT>(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, 1e9f, st, &nn, &nt);
         hits[i] = make_float4(h.t, __uint_as_float(h.prim), h.b1, h.b2);
@@ -369,6 +375,36 @@ __global__ void __launch_bounds__(kBlock) k_shadow(DevScene S, const uint32_t* _
     }
 }
 
+// ray-sort key of each queue slot: direction octant (3 bits), then the origin's Morton code
+// (10 bits per axis of the scene box, top 29 bits); slots past the queue sort last
+__device__ __forceinline__ uint32_t spread10(uint32_t x) {
+    x &= 0x3ffu;
+    x = (x | x << 16) & 0x030000ffu;
+    x = (x | x << 8) & 0x0300f00fu;
+    x = (x | x << 4) & 0x030c30c3u;
+    x = (x | x << 2) & 0x09249249u;
+    return x;
+}
+__global__ void __launch_bounds__(kBlock) k_ray_keys(Q q, const uint32_t* __restrict__ cnt, uint32_t b, uint32_t npix,
+                                                     float3 bmin, float inv_ext, uint32_t shift,
+                                                     uint32_t* __restrict__ keys, uint32_t* __restrict__ slots) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= npix) return;
+    uint32_t key = 0xFFFFFFFFu;
+    if (i < cnt[b]) {
+        const float4 o = q.o[i], d = q.d[i];
+        const uint32_t oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+        auto cell = [&](float v, float lo) {
+            const float t = (v - lo) * inv_ext * 1024.0f;
+            return (t >= 0.0f) ? (uint32_t)fminf(t, 1023.0f) : 0u;  // NaN -> 0
+        };
+        const uint32_t m = spread10(cell(o.x, bmin.x)) << 2 | spread10(cell(o.y, bmin.y)) << 1 | spread10(cell(o.z, bmin.z));
+        key = oct << 29 | m >> 1;
+    }
+    keys[i] = key >> shift;  // the top bits only, sorted over the full 32-bit range
+    slots[i] = i;
+}
+
 // blocks the device holds at once for a kernel (persistent grids), cached per instantiation
 template <class K>
 uint32_t resident_blocks(rrh_ctx* c, K kernel) {
@@ -382,7 +418,8 @@ uint32_t resident_blocks(rrh_ctx* c, K kernel) {
 }
 
 template <int STK>
-int launch_trace(rrh_ctx* c, const DevScene& S, Q& qi, uint32_t* cnt, uint32_t b, uint32_t g, bool count) {
+int launch_trace(rrh_ctx* c, const DevScene& S, Q& qi, uint32_t* cnt, uint32_t b, uint32_t g, bool count,
+                 const uint32_t* perm) {
     unsigned long long* fc = (unsigned long long*)c->fetch.p;
     float4* hp = (float4*)c->hits.p;
     if (c->persistent && S.node_fmt == RRH_NODES_BINARY) {
@@ -398,9 +435,11 @@ int launch_trace(rrh_ctx* c, const DevScene& S, Q& qi, uint32_t* cnt, uint32_t b
         return RRH_OK;
     }
     if (count)
-        hipLaunchKernelGGL((k_intersect<true, STK>), dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, fc, c->xcd_remap);
+        hipLaunchKernelGGL((k_intersect<true, STK>), dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, fc, c->xcd_remap,
+                           perm);
     else
-        hipLaunchKernelGGL((k_intersect<false, STK>), dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, fc, c->xcd_remap);
+        hipLaunchKernelGGL((k_intersect<false, STK>), dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, fc, c->xcd_remap,
+                           perm);
     RRH_LAUNCH_CHECK();
     return RRH_OK;
 }
@@ -475,6 +514,15 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
     if ((rc = c->sh_d.ensure((size_t)npix * 16))) return rc;
     if ((rc = c->sh_c.ensure((size_t)npix * 16))) return rc;
     if ((rc = c->counters.ensure(256 * 4))) return rc;
+    if (c->ray_sort_bits) {
+        if ((rc = c->sort_keys.ensure((size_t)npix * 8))) return rc;
+        if ((rc = c->sort_perm.ensure((size_t)npix * 8))) return rc;
+        size_t tb = 0;
+        uint32_t* k = (uint32_t*)c->sort_keys.p;
+        uint32_t* v = (uint32_t*)c->sort_perm.p;
+        RRH_HIP(rocprim::radix_sort_pairs(nullptr, tb, k, k + npix, v, v + npix, npix, 0, 32, c->stream));
+        if ((rc = c->sort_tmp.ensure(tb))) return rc;
+    }
     if (!c->totals.p) {
         if ((rc = c->totals.ensure(32))) return rc;
         RRH_HIP(hipMemsetAsync(c->totals.p, 0, 32, c->stream));
@@ -503,6 +551,24 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
     for (uint32_t b = 0; b < p->max_bounces; ++b) {
         Q& qi = q[b & 1];
         Q& qo = q[(b + 1) & 1];
+        const uint32_t* perm = nullptr;
+        if (c->ray_sort_bits && b > 0) {
+            // extension rays: trace in key order (primary rays are already tile-coherent)
+            uint32_t* keys = (uint32_t*)c->sort_keys.p;
+            uint32_t* slots = (uint32_t*)c->sort_perm.p;
+            const float ext = std::max(std::max(s->bmax[0] - s->bmin[0], s->bmax[1] - s->bmin[1]), s->bmax[2] - s->bmin[2]);
+            hipLaunchKernelGGL(k_ray_keys, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, qi,
+                               (const uint32_t*)cnt, b, npix, make_float3(s->bmin[0], s->bmin[1], s->bmin[2]),
+                               ext > 0.0f ? 1.0f / ext : 0.0f, 32u - c->ray_sort_bits, keys, slots);
+            RRH_LAUNCH_CHECK();
+            // full bit range: a partial range (begin_bit > 0) returned orders that were not
+            // permutations of the slots for queues under rocPRIM's merge-sort size limit
+            size_t tb = c->sort_tmp.bytes;
+            hipError_t e = rocprim::radix_sort_pairs(c->sort_tmp.p, tb, keys, keys + npix, slots, slots + npix, npix,
+                                                     0, 32, c->stream);
+            if (e != hipSuccess) { set_error(std::string("ray sort: ") + hipGetErrorString(e)); return RRH_E_HIP; }
+            perm = slots + npix;
+        }
         if (c->profiling) {
             if (c->prof_used + 2 > c->prof_events.size()) {
                 for (int k = 0; k < 64; ++k) {
@@ -515,10 +581,10 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
         }
         const int sk = stack_for(c, s->stack_need());
         const bool cf = (p->flags & RRH_PT_COUNT_FETCHES) != 0;
-        rc = (sk == RRH_STACK_LDS32) ? launch_trace<RRH_STACK_LDS32>(c, S, qi, cnt, b, g, cf)
-           : (sk == RRH_STACK_HYBRID16) ? launch_trace<RRH_STACK_HYBRID16>(c, S, qi, cnt, b, g, cf)
-           : (sk == RRH_STACK_HYBRID32) ? launch_trace<RRH_STACK_HYBRID32>(c, S, qi, cnt, b, g, cf)
-           : launch_trace<RRH_STACK_PRIVATE>(c, S, qi, cnt, b, g, cf);
+        rc = (sk == RRH_STACK_LDS32) ? launch_trace<RRH_STACK_LDS32>(c, S, qi, cnt, b, g, cf, perm)
+           : (sk == RRH_STACK_HYBRID16) ? launch_trace<RRH_STACK_HYBRID16>(c, S, qi, cnt, b, g, cf, perm)
+           : (sk == RRH_STACK_HYBRID32) ? launch_trace<RRH_STACK_HYBRID32>(c, S, qi, cnt, b, g, cf, perm)
+           : launch_trace<RRH_STACK_PRIVATE>(c, S, qi, cnt, b, g, cf, perm);
         if (rc) return rc;
         RRH_LAUNCH_CHECK();
         if (c->profiling) {
