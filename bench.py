@@ -117,6 +117,69 @@ def guiding_summary(ctx: core.Context, a) -> dict:
             "mlp_loss": [ms["mlp"].get("loss_first"), ms["mlp"].get("loss_last")]}
 
 
+def restir_pt_summary(ctx: core.Context, scene, sa, W: int, H: int, frames: int, warmup: int) -> dict:
+    """ReSTIR PT (reconnection shift, the reference pass's defaults: 8 candidates of up to 4
+    segments, temporal and spatial reuse, hash-grid guiding fitted every frame) over ReSTIR DI's
+    G-buffer and direct image, on the same scene, extent and orbit.  Times the three ReSTIR PT
+    stages plus the grid fit with markers on the context stream; rays counted on the device."""
+    img = lambda f=L.FMT_RGBA32F: core.Image(ctx, W, H, f)
+    gp, gn, direct, out = img(), img(), img(), img()
+    dres, duv = [img(), img()], [img(L.FMT_R32U), img(L.FMT_R32U)]
+    res = [[img() for _ in range(4)], [img() for _ in range(4)]]
+    hist = [[img(), img()], [img(), img()]]
+    grid = core.Grid(ctx, L.GridParams(L.GRID_HASH, 1 << 20, 0.10, 0.02, 8, 512.0, 0, 0, L.f3((0, 0, 0)),
+                                       L.f3((1, 1, 1)), 0.0, 0.0))
+    for r in dres + duv:
+        r.clear()
+    L.call("rrh_rpt_clear", ctx.h, core.planes(res[1]), hist[1][0].h, hist[1][1].h)
+    c, s = ctx.h, scene.h
+    stage_ms = np.zeros(4)
+    ext = shadow = 0
+    prev_vp = None
+    for f in range(warmup + frames):
+        cam = orbit_camera(sa, f, W, H)
+        vp = list(cam.prev_view_proj)
+        if prev_vp is not None:
+            cam.prev_view_proj[:] = prev_vp
+        prev_vp = vp
+        scene.set_camera(cam)
+        cur, prev = f & 1, 1 - (f & 1)
+        d = L.DIParams(f, 16, 1, 1, 4, 15)
+        L.call("rrh_di_initial", c, s, C.byref(d), gp.h, gn.h, dres[cur].h, duv[cur].h)
+        L.call("rrh_di_temporal", c, s, C.byref(d), gp.h, gn.h, dres[cur].h, duv[cur].h, dres[prev].h, duv[prev].h)
+        L.call("rrh_di_spatial_shade", c, s, C.byref(d), gp.h, gn.h, dres[cur].h, duv[cur].h, direct.h)
+        p = L.RPTParams(f, 8, 4, 1, 1, 4, 12, 1, 1.0, 1e-3, 1, 0.5)
+        ctx.marker(0)
+        L.call("rrh_rpt_initial", c, s, C.byref(p), grid.h, gp.h, gn.h, core.planes(res[cur]), hist[cur][0].h,
+               hist[cur][1].h)
+        ctx.marker(1)
+        L.call("rrh_rpt_temporal", c, s, C.byref(p), gp.h, gn.h, core.planes(res[cur]), core.planes(res[prev]),
+               hist[prev][0].h, hist[prev][1].h)
+        ctx.marker(2)
+        L.call("rrh_rpt_spatial_shade", c, s, C.byref(p), grid.h, gp.h, gn.h, direct.h, core.planes(res[cur]), out.h)
+        ctx.marker(3)
+        L.call("rrh_grid_fit", c, grid.h)
+        ctx.marker(4)
+        ctx.synchronize()
+        if f >= warmup:
+            stage_ms += [ctx.elapsed_ms(k, k + 1) for k in range(4)]
+            rc = L.RayCounts()
+            L.call("rrh_rpt_ray_counts", c, C.byref(rc))
+            ext += rc.extension
+            shadow += rc.shadow
+    ms = stage_ms / frames
+    total = float(ms.sum())
+    rays = (ext + shadow) / frames
+    return {"workload": f"heightfield_{scene.info()['tri_count']}_tris_{W}x{H}_restir_pt_M8_4seg",
+            "ms_per_frame": round(total, 3),
+            "stage_ms": {"initial": round(ms[0], 3), "temporal": round(ms[1], 3), "spatial_shade": round(ms[2], 3),
+                         "grid_fit": round(ms[3], 3)},
+            "rays_per_frame": {"extension": ext // frames, "shadow_and_visibility": shadow // frames},
+            "mrays_per_s": round(rays / (total * 1e-3) / 1e6, 1) if total > 0 else None,
+            "frames": frames,
+            "note": "stage times by HIP markers on the context stream; ReSTIR DI (G-buffer, direct) not included"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +209,8 @@ def main() -> int:
     ap.add_argument("--guide-ref-spp", type=int, default=8192, help="spp of the FP64 reference image")
     ap.add_argument("--guide-width", type=int, default=320)
     ap.add_argument("--guide-height", type=int, default=180)
+    ap.add_argument("--restir-pt-frames", type=int, default=6,
+                    help="ReSTIR PT frames timed on the same scene and extent (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB copy kernel (for hardware-counter calibration) and exit")
     a = ap.parse_args()
@@ -255,6 +320,11 @@ def main() -> int:
                      "launches": launches.value, "measured_copy_gbs": round(hbm.value, 1)},
         "cpu_baseline": None,
     }
+    if a.restir_pt_frames > 0:
+        print("[bench] ReSTIR PT frames ...", file=sys.stderr, flush=True)
+        out["restir_pt"] = restir_pt_summary(ctx, scene, sa, W, H, a.restir_pt_frames, 2)
+        print(f"[bench] ReSTIR PT {out['restir_pt']['ms_per_frame']} ms/frame, "
+              f"{out['restir_pt']['mrays_per_s']} Mrays/s", file=sys.stderr, flush=True)
     if a.guide_seconds > 0:
         print("[bench] equal-time guiding comparison (indoor scene) ...", file=sys.stderr, flush=True)
         out["guiding"] = guiding_summary(ctx, a)

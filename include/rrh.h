@@ -259,6 +259,50 @@ int rrh_di_spatial_shade(rrh_ctx* ctx, rrh_scene* s, const rrh_di_params* p,
 /* shadow rays traced by the last rrh_di_spatial_shade (one per shaded pixel) */
 int rrh_di_ray_counts(rrh_ctx* ctx, rrh_ray_counts* out);
 
+/* ---- ReSTIR PT, reconnection shift: ReSTIRPTPass::execute (src/passes/restir_pt/
+ *      ReSTIRPTPass.cpp:529-813) + restir_pt.slang cs_initial :569-771, cs_temporal
+ *      :775-872, cs_spatial_shade :876-1001 (guiding fit: rrh_grid_fit) ------------------
+ * The G-buffer is ReSTIR DI's (position w < 0 on a miss, normal w = asfloat(material)).
+ * A reservoir is four RGBA32F planes (restir_pt.slang:11-15):
+ *   res[0] (reconnection position, w_sum)   res[1] (reconnection normal, asfloat(M))
+ *   res[2] (suffix radiance, W)             res[3] (prefix throughput, valid)
+ * and a history two (the frame's G-buffer position and normal, for reprojection).  Every
+ * ray culls back faces.  grid: an RRH_GRID_HASH grid that guides the first bounce and
+ * receives the splats, or NULL (cosine sampling, no splats). */
+typedef struct {
+    uint32_t frame_seed;               /* frame_count_++ (ReSTIRPTPass.cpp:561) */
+    uint32_t num_initial_candidates;   /* ReSTIRPTPass.h:88 default 8 */
+    uint32_t max_bounces;              /* path segments from the primary surface, default 4 */
+    uint32_t temporal_enable;          /* 1 */
+    uint32_t spatial_enable;           /* 1 */
+    uint32_t spatial_num_neighbors;    /* 4 */
+    uint32_t spatial_radius;           /* pixels, 12 */
+    uint32_t include_direct_lighting;  /* 1: add the direct image (ReSTIR DI's output) */
+    float    indirect_strength;        /* 1.0 */
+    float    ray_bias;                 /* 1e-3 */
+    uint32_t guiding_enable;           /* 1 */
+    float    guiding_alpha;            /* one-sample MIS probability of the lobe, .5 */
+} rrh_rpt_params;
+/* cs_initial: candidates, resampling, the kept reservoir; writes the history planes */
+int rrh_rpt_initial(rrh_ctx* ctx, rrh_scene* s, const rrh_rpt_params* p, rrh_grid* grid,
+                    const rrh_image* gbuf_pos, const rrh_image* gbuf_norm, rrh_image* const res[4],
+                    rrh_image* hist_pos, rrh_image* hist_norm);
+/* cs_temporal: reuse of the previous frame's reservoir (prev_view_proj of the camera) */
+int rrh_rpt_temporal(rrh_ctx* ctx, rrh_scene* s, const rrh_rpt_params* p, const rrh_image* gbuf_pos,
+                     const rrh_image* gbuf_norm, rrh_image* const res[4], rrh_image* const res_prev[4],
+                     const rrh_image* hist_prev_pos, const rrh_image* hist_prev_norm);
+/* cs_spatial_shade: spatial reuse, shading into out (RGBA32F), splats into the grid;
+ * direct: RGBA32F or NULL */
+int rrh_rpt_spatial_shade(rrh_ctx* ctx, rrh_scene* s, const rrh_rpt_params* p, rrh_grid* grid,
+                          const rrh_image* gbuf_pos, const rrh_image* gbuf_norm, const rrh_image* direct,
+                          rrh_image* const res[4], rrh_image* out);
+/* history reset (ReSTIRPTPass.cpp:616-663): reservoir planes zero, history position
+ * (0, 0, 0, -1), history normal zero */
+int rrh_rpt_clear(rrh_ctx* ctx, rrh_image* const res[4], rrh_image* hist_pos, rrh_image* hist_norm);
+/* rays of the last frame: extension = the initial stage's path segments, shadow = its light
+ * samples plus the temporal and spatial visibility rays */
+int rrh_rpt_ray_counts(rrh_ctx* ctx, rrh_ray_counts* out);
+
 /* ---- A-trous: AtrousPass::execute (src/passes/denoise/AtrousPass.cpp:243-306)
  *      + atrous.slang cs_main :41-106 ------------------------------------- */
 #define RRH_ATROUS_REFERENCE 0  /* RGBA32F radiance, RGBA32F position/normal G-buffer, Gaussian weights */

@@ -49,6 +49,14 @@ class DIParams(C.Structure):
                 ("_pad0", C.c_uint32), ("_pad1", C.c_uint32)]
 
 
+class RPTParams(C.Structure):
+    _fields_ = [("frame_seed", C.c_uint32), ("num_initial_candidates", C.c_uint32), ("max_bounces", C.c_uint32),
+                ("temporal_enable", C.c_uint32), ("spatial_enable", C.c_uint32),
+                ("spatial_num_neighbors", C.c_uint32), ("spatial_radius", C.c_uint32),
+                ("include_direct_lighting", C.c_uint32), ("indirect_strength", C.c_float), ("ray_bias", C.c_float),
+                ("guiding_enable", C.c_uint32), ("guiding_alpha", C.c_float)]
+
+
 class AtrousParams(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("stride", C.c_uint32), ("mode", C.c_uint32),
                 ("sigma_position", C.c_float), ("sigma_normal", C.c_float), ("sigma_luminance", C.c_float),
@@ -88,6 +96,10 @@ _SIGS = {
     "orc_di_initial": (None, [P, C.POINTER(Camera), C.POINTER(DIParams), P, P, P, P, C.c_int]),
     "orc_di_temporal": (None, [P, C.POINTER(Camera), C.POINTER(DIParams), P, P, P, P, P, P, C.c_int]),
     "orc_di_spatial_shade": (None, [P, C.POINTER(Camera), C.POINTER(DIParams), P, P, P, P, P, C.c_int]),
+    "orc_rpt_initial": (None, [P, C.POINTER(Camera), C.POINTER(RPTParams), P, F, F, U, P, P, P, P, P, C.c_int]),
+    "orc_rpt_temporal": (None, [P, C.POINTER(Camera), C.POINTER(RPTParams), P, P, P, P, P, P, C.c_int]),
+    "orc_rpt_spatial_shade": (None, [P, C.POINTER(Camera), C.POINTER(RPTParams), P, F, F, U, P, P, P, P, P,
+                                     C.c_int]),
     "orc_atrous_ref": (None, [C.POINTER(AtrousParams), P, P, P, P]),
     "orc_atrous_cfg": (None, [C.POINTER(AtrousParams), P, P, P]),
     "orc_f32_to_f16": (C.c_uint16, [F]),
@@ -213,6 +225,34 @@ class Scene:
         out = np.zeros((cam.screen_width * cam.screen_height, 4), np.float32)
         lib().orc_di_spatial_shade(self.h, C.byref(cam), C.byref(prm), p(gp), p(gn), p(rs), p(uv), p(out), threads)
         return out
+
+    # ReSTIR PT (restir_pt.slang); res: (4, W*H, 4) float32 planes; grid: (fitted or accum, base
+    # cell, slope, cell count) of the reference hash grid, or None
+    def rpt_initial(self, cam: Camera, prm: RPTParams, gp, gn, grid=None, threads: int = 0):
+        n = cam.screen_width * cam.screen_height
+        res = np.zeros((4, n, 4), np.float32)
+        hp, hn = np.zeros((n, 4), np.float32), np.zeros((n, 4), np.float32)
+        fitted, base, slope, count = grid if grid is not None else (None, 0.0, 0.0, 1)
+        lib().orc_rpt_initial(self.h, C.byref(cam), C.byref(prm), None if fitted is None else p(fitted), base, slope,
+                              count, p(gp), p(gn), p(res), p(hp), p(hn), threads)
+        return res, hp, hn
+
+    def rpt_temporal(self, cam, prm, gp, gn, res, res_prev, hp_prev, hn_prev, threads: int = 0):
+        res = np.ascontiguousarray(res, np.float32).copy()
+        lib().orc_rpt_temporal(self.h, C.byref(cam), C.byref(prm), p(gp), p(gn), p(res),
+                               p(np.ascontiguousarray(res_prev, np.float32)), p(hp_prev), p(hn_prev), threads)
+        return res
+
+    def rpt_spatial_shade(self, cam, prm, gp, gn, res, direct=None, grid=None, threads: int = 0):
+        """Returns (output, accum): accum is the grid's accumulator after the splats (a copy)."""
+        out = np.zeros((cam.screen_width * cam.screen_height, 4), np.float32)
+        accum, base, slope, count = grid if grid is not None else (None, 0.0, 0.0, 1)
+        if accum is not None:
+            accum = np.ascontiguousarray(accum, np.uint32).copy()
+        lib().orc_rpt_spatial_shade(self.h, C.byref(cam), C.byref(prm), None if accum is None else p(accum), base,
+                                    slope, count, p(gp), p(gn), None if direct is None else p(direct),
+                                    p(np.ascontiguousarray(res, np.float32)), p(out), threads)
+        return out, accum
 
     def __del__(self):
         try:

@@ -1563,3 +1563,404 @@ void orc_vmm_mstep(const double* stats, uint32_t ncell, uint32_t K, float kappa_
         }
     }
 }
+
+/* ========================================================================== */
+/* ReSTIR PT, reconnection shift -- restir_pt.slang:98-1001 (stages :569-771,
+ * :775-872, :876-1001).  Every ray culls back faces (RAY_FLAG_CULL_BACK_FACING_
+ * TRIANGLES).  1/sqrtf where the shader writes rsqrt.  Scenes whose emissive
+ * triangles are lights: light sampling covers them (area sampling, no MIS) and the
+ * reconnection vertex's own emission is left to the direct lighting pass.       */
+/* ========================================================================== */
+typedef struct { v3 rc_pos, rc_n, suffix, prefix; float w_sum, W; uint32_t M; float valid; } rpt_res;
+
+static rpt_res rpt_load(const float* res, size_t npix, size_t i) {
+    const float* p = res + i * 4; const float* n = res + (npix + i) * 4;
+    const float* r = res + (2 * npix + i) * 4; const float* e = res + (3 * npix + i) * 4;
+    rpt_res s;
+    s.rc_pos = vld(p); s.w_sum = p[3];
+    s.rc_n = vld(n); s.M = asuint(n[3]);
+    s.suffix = vld(r); s.W = r[3];
+    s.prefix = vld(e); s.valid = e[3];
+    return s;
+}
+static void rpt_store(float* res, size_t npix, size_t i, const rpt_res* s) {
+    float* p = res + i * 4; float* n = res + (npix + i) * 4;
+    float* r = res + (2 * npix + i) * 4; float* e = res + (3 * npix + i) * 4;
+    vst(p, s->rc_pos); p[3] = s->w_sum;
+    vst(n, s->rc_n); n[3] = asfloat(s->M);
+    vst(r, s->suffix); r[3] = s->W;
+    vst(e, s->prefix); e[3] = s->valid;
+}
+static void rpt_store_empty(float* res, size_t npix, size_t i) {
+    for (int k = 0; k < 4; ++k) memset(res + (k * npix + i) * 4, 0, 16);
+}
+static int hit_cull(const orc_scene* s, v3 o, v3 d, float tmin, float tmax) {
+    return trace_bvh_cull(s, o, d, tmin, tmax, 1).prim != ~0u;
+}
+
+/* sample_lights, restir_pt.slang:171-222 */
+static v3 rpt_lights(const orc_scene* s, v3 hit, v3 n, v3 wo, const matp* m, float bias, uint32_t* seed) {
+    const v3 zero = V(0, 0, 0);
+    uint32_t N = s->nl + s->ntl;
+    if (N == 0) return zero;
+    uint32_t li = (uint32_t)(orc_rand_float(seed) * (float)N) % N;
+    if (li < s->nl) {
+        const orc_light* L = &s->l[li];
+        v3 ldir; float dist, pl;
+        if (L->type == 0) {
+            v3 to = vsub(vld(L->position), hit);
+            dist = vlength(to);
+            ldir = vdivs(to, fmaxf(dist, 1e-6f));
+            pl = dist * dist;
+        } else {
+            ldir = vnorm(vneg(vld(L->direction)));
+            dist = 1e9f;
+            pl = 1.0f;
+        }
+        if (vdot(ldir, n) <= 0.0f) return zero;
+        if (hit_cull(s, vadd(hit, vscale(n, bias)), ldir, bias, dist - 0.01f)) return zero;
+        v3 f = mat_eval(m, n, wo, ldir);
+        float ndl = fmaxf(vdot(n, ldir), 0.0f);
+        v3 irr = vdivs(vscale(vld(L->emission), L->intensity), (L->type == 0) ? (pl + 1e-6f) : 1.0f);
+        float pb = mat_ref_pdf(m, wo, ldir, n);
+        float mis = (L->type == 0) ? 1.0f : (pl * pl) / (pl * pl + pb * pb + 1e-10f);
+        return vscale(vscale(vscale(vmul(f, irr), ndl), mis), (float)N);
+    }
+    float u1 = orc_rand_float(seed);
+    float u2 = orc_rand_float(seed);
+    tri_sample ts = tri_light_point(s, li - s->nl, u1, u2);
+    v3 to = vsub(ts.p, hit);
+    float d2 = vdot(to, to);
+    float dist = sqrtf(d2);
+    v3 l = vscale(to, 1.0f / dist);
+    if (vdot(l, n) <= 0.0f) return zero;
+    float cosl = fabsf(vdot(ts.nl, l));
+    if (!(cosl > 0.0f)) return zero;
+    if (hit_cull(s, vadd(hit, vscale(n, bias)), l, bias, dist - 0.01f)) return zero;
+    v3 f = mat_eval(m, n, wo, l);
+    float ndl = fmaxf(vdot(n, l), 0.0f);
+    float pl = d2 / (cosl * ts.area) / (float)N;
+    return vscale(vscale(vmul(f, ts.Le), ndl), 1.0f / pl);
+}
+
+/* evaluate_contribution / target_function, restir_pt.slang:464-489 */
+static v3 rpt_contribution(v3 pp, v3 pn, const matp* pm, v3 cam, const rpt_res* r) {
+    v3 to = vsub(r->rc_pos, pp);
+    float d2 = vdot(to, to);
+    if (d2 <= 1e-8f) return V(0, 0, 0);
+    v3 wi = vscale(to, 1.0f / sqrtf(d2));
+    float cx = fmaxf(vdot(pn, wi), 0.0f);
+    if (cx <= 0.0f) return V(0, 0, 0);
+    v3 wo = vnorm(vsub(cam, pp));
+    v3 f = mat_eval(pm, pn, wo, wi);
+    return vmul(vmul(vscale(f, cx), r->prefix), r->suffix);
+}
+static float rpt_target(v3 pp, v3 pn, const matp* pm, v3 cam, const rpt_res* r) {
+    return lum709(rpt_contribution(pp, pn, pm, cam, r));
+}
+
+/* reconnection_jacobian, restir_pt.slang:427-447 */
+static float rpt_jacobian(v3 src, v3 dst, v3 rc, v3 rn) {
+    v3 sd = vsub(rc, src), td = vsub(rc, dst);
+    float s2 = vdot(sd, sd), t2 = vdot(td, td);
+    if (s2 <= 1e-8f || t2 <= 1e-8f) return 0.0f;
+    v3 sdir = vscale(sd, 1.0f / sqrtf(s2)), tdir = vscale(td, 1.0f / sqrtf(t2));
+    float sc = fabsf(vdot(rn, vneg(sdir))), tc = fabsf(vdot(rn, vneg(tdir)));
+    if (sc <= 0.05f || tc <= 0.05f) return 0.0f;
+    return clampf((tc * s2) / (sc * t2), 0.0f, 3.0f);
+}
+
+/* trace_visibility, restir_pt.slang:449-458 */
+static int rpt_visible(const orc_scene* s, v3 o, v3 n, v3 target, float bias) {
+    v3 delta = vsub(target, o);
+    float dist = vlength(delta);
+    if (dist <= bias) return 0;
+    return !hit_cull(s, vadd(o, vscale(n, bias)), vdivs(delta, dist), bias, dist - bias);
+}
+
+/* load_surface_from_images, restir_pt.slang:285-305 */
+static int rpt_surface(const float* pos, const float* nrm, size_t i, v3* p, v3* n, uint32_t* mat) {
+    const float* a = pos + i * 4;
+    if (a[3] < 0.0f) return 0;
+    const float* b = nrm + i * 4;
+    *p = vld(a);
+    *n = vnorm(vld(b));
+    *mat = asuint(b[3]);
+    return 1;
+}
+
+void orc_rpt_initial(const orc_scene* s, const orc_camera* cam, const orc_rpt_params* p, const float* fitted,
+                     float base_cell, float slope, uint32_t cell_count, const float* gpos, const float* gnrm,
+                     float* res, float* hpos, float* hnrm, int threads) {
+    const uint32_t W = cam->screen_width, H = cam->screen_height;
+    const size_t npix = (size_t)W * H;
+    const v3 cp = vld(cam->position);
+    const int rc_emission = s->ntl == 0;
+    const float alpha = p->guiding_enable ? p->guiding_alpha : 0.0f;
+    const uint32_t max_b = p->max_bounces > 1u ? p->max_bounces : 1u;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t ii = 0; ii < (int64_t)npix; ++ii) {
+        const size_t i = (size_t)ii;
+        const uint32_t px = (uint32_t)(i % W), py = (uint32_t)(i / W);
+        v3 pp, pn; uint32_t pmi;
+        if (!rpt_surface(gpos, gnrm, i, &pp, &pn, &pmi)) {
+            float* h = hpos + i * 4; h[0] = h[1] = h[2] = 0.0f; h[3] = -1.0f;
+            memset(hnrm + i * 4, 0, 16);
+            rpt_store_empty(res, npix, i);
+            continue;
+        }
+        memcpy(hpos + i * 4, gpos + i * 4, 16);
+        memcpy(hnrm + i * 4, gnrm + i * 4, 16);
+        const matp pm = load_mat(s, pmi);
+        const float bias = p->ray_bias;
+        uint32_t seed = orc_make_seed(px, py, p->frame_seed);
+        rpt_res sel; memset(&sel, 0, sizeof(sel));
+        float w_sum = 0.0f;
+        uint32_t Mc = 0;
+        for (uint32_t c = 0; c < p->num_initial_candidates; ++c) {
+            float src_pdf, wa[3], ppa[3], pna[3], cpa[3];
+            vst(ppa, pp); vst(pna, pn); vst(cpa, cp);
+            orc_sample_primary_dir(fitted, ppa, pna, pmi, cpa, base_cell, slope, cell_count,
+                                   fitted != NULL, alpha, &seed, wa, &src_pdf);
+            const v3 wi0 = vld(wa);
+            if (src_pdf <= 1e-6f || vdot(pn, wi0) <= 0.0f) continue;
+            orc_hit h0 = trace_bvh_cull(s, vadd(pp, vscale(pn, bias)), wi0, bias, 1e9f, 1);
+            if (h0.prim == ~0u) continue;
+            hitinfo rc = fetch_hit(s, h0);
+            v3 rn = rc.normal;
+            if (vdot(rn, wi0) > 0.0f) rn = vneg(rn);
+            const matp rm = load_mat(s, rc.mat);
+            v3 suffix = rc_emission ? rm.emissive : V(0, 0, 0);
+            const v3 rwo = vneg(wi0);
+            suffix = vadd(suffix, rpt_lights(s, rc.pos, rn, rwo, &rm, bias, &seed));
+            v3 cur_p = rc.pos, cur_n = rn, cur_wo = rwo;
+            matp cur_m = rm;
+            v3 thp = V(1, 1, 1);
+            for (uint32_t b = 1; b < max_b; ++b) {
+                v3 wi; float pdf;
+                float ux = orc_rand_float(&seed), uy = orc_rand_float(&seed);
+                v3 f = mat_sample(&cur_m, cur_wo, cur_n, ux, uy, &wi, &pdf);
+                if (pdf <= 1e-6f || vdot(cur_n, wi) <= 0.0f) break;
+                float cosc = fmaxf(vdot(cur_n, wi), 0.0f);
+                thp = vmul(thp, vdivs(vscale(f, cosc), pdf));
+                if (fmaxf(thp.x, fmaxf(thp.y, thp.z)) <= 0.0f) break;
+                orc_hit h = trace_bvh_cull(s, vadd(cur_p, vscale(cur_n, bias)), wi, bias, 1e9f, 1);
+                if (h.prim == ~0u) { suffix = vadd(suffix, vmul(thp, sky(wi))); break; }
+                hitinfo hi = fetch_hit(s, h);
+                v3 hn = hi.normal;
+                if (vdot(hn, wi) > 0.0f) hn = vneg(hn);
+                const matp hm = load_mat(s, hi.mat);
+                const v3 hwo = vneg(wi);
+                suffix = vadd(suffix, vmul(thp, rpt_lights(s, hi.pos, hn, hwo, &hm, bias, &seed)));
+                cur_p = hi.pos; cur_n = hn; cur_m = hm; cur_wo = hwo;
+            }
+            rpt_res cand;
+            cand.rc_pos = rc.pos; cand.rc_n = rn; cand.suffix = suffix; cand.prefix = V(1, 1, 1);
+            cand.M = 1; cand.W = 0.0f; cand.w_sum = 0.0f; cand.valid = 1.0f;
+            float ph = rpt_target(pp, pn, &pm, cp, &cand);
+            if (ph <= 0.0f) continue;
+            v3 dl = vsub(rc.pos, pp);
+            if (vdot(dl, dl) <= 1e-8f) continue;
+            float w = ph / fmaxf(src_pdf, 1e-6f);
+            w_sum += w;
+            Mc++;
+            if (w_sum > 0.0f && orc_rand_float(&seed) < w / w_sum) sel = cand;
+        }
+        int stored = 0;
+        if (Mc > 0 && sel.valid > 0.5f) {
+            float ph = rpt_target(pp, pn, &pm, cp, &sel);
+            if (ph > 1e-10f) {
+                sel.M = Mc;
+                sel.w_sum = w_sum;
+                sel.W = w_sum / ((float)Mc * ph);
+                rpt_store(res, npix, i, &sel);
+                stored = 1;
+            }
+        }
+        if (!stored) rpt_store_empty(res, npix, i);
+    }
+}
+
+/* reproject_to_prev_pixel, restir_pt.slang:323-338 */
+static int rpt_reproject(const orc_camera* cam, v3 p, uint32_t W, uint32_t H, int* hx, int* hy) {
+    const float* M = cam->prev_view_proj;
+    float cx = mrow(M, 0, p.x, p.y, p.z, 1.0f), cy = mrow(M, 1, p.x, p.y, p.z, 1.0f),
+          cw = mrow(M, 3, p.x, p.y, p.z, 1.0f);
+    if (cw <= 1.0e-6f) return 0;
+    float nx = cx / cw, ny = cy / cw;
+    if (fabsf(nx) > 1.0f || fabsf(ny) > 1.0f) return 0;
+    float fx = (nx * 0.5f + 0.5f) * (float)W - 0.5f, fy = (ny * 0.5f + 0.5f) * (float)H - 0.5f;
+    *hx = (int)floorf(fx + 0.5f);
+    *hy = (int)floorf(fy + 0.5f);
+    return *hx >= 0 && *hy >= 0 && *hx < (int)W && *hy < (int)H;
+}
+
+void orc_rpt_temporal(const orc_scene* s, const orc_camera* cam, const orc_rpt_params* p, const float* gpos,
+                      const float* gnrm, float* res, const float* res_prev, const float* hpos_prev,
+                      const float* hnrm_prev, int threads) {
+    if (!p->temporal_enable) return;
+    const uint32_t W = cam->screen_width, H = cam->screen_height;
+    const size_t npix = (size_t)W * H;
+    const v3 cp = vld(cam->position);
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t ii = 0; ii < (int64_t)npix; ++ii) {
+        const size_t i = (size_t)ii;
+        const uint32_t px = (uint32_t)(i % W), py = (uint32_t)(i / W);
+        v3 pp, pn; uint32_t pmi;
+        int cx, cy;
+        if (!rpt_surface(gpos, gnrm, i, &pp, &pn, &pmi) || !rpt_reproject(cam, pp, W, H, &cx, &cy)) continue;
+        /* find_reprojected_history_surface, restir_pt.slang:355-401 */
+        int found = 0;
+        float best = 1e30f;
+        size_t hj = 0;
+        v3 hp = V(0, 0, 0);
+        float vdist = fmaxf(vlength(vsub(pp, cp)), 1.0f);
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int ox = -1; ox <= 1; ++ox) {
+                int qx = cx + ox, qy = cy + oy;
+                if (qx < 0 || qy < 0 || qx >= (int)W || qy >= (int)H) continue;
+                size_t j = (size_t)qy * W + (size_t)qx;
+                v3 q, qn; uint32_t qm;
+                if (!rpt_surface(hpos_prev, hnrm_prev, j, &q, &qn, &qm)) continue;
+                /* is_reprojected_history_compatible, restir_pt.slang:340-353 */
+                if (qm != pmi || vdot(pn, qn) < 0.9f) continue;
+                float plane = fabsf(vdot(vsub(q, pp), pn));
+                float dpos = vlength(vsub(q, pp));
+                if (!(plane <= 0.01f * vdist && dpos <= 0.05f * vdist)) continue;
+                float err = plane * 4.0f + dpos;
+                if (err < best) { best = err; hj = j; hp = q; found = 1; }
+            }
+        if (!found) continue;
+        rpt_res cur = rpt_load(res, npix, i);
+        rpt_res prv = rpt_load(res_prev, npix, hj);
+        if (prv.M == 0 || prv.valid < 0.5f) continue;
+        uint32_t cap = 20u * p->num_initial_candidates;
+        cap = cap < 1u ? 1u : (cap > 8192u ? 8192u : cap);
+        prv.M = prv.M < cap ? prv.M : cap;
+        float jac = rpt_jacobian(hp, pp, prv.rc_pos, prv.rc_n);
+        if (jac <= 0.0f) continue;
+        if (!rpt_visible(s, pp, pn, prv.rc_pos, p->ray_bias)) continue;
+        const matp pm = load_mat(s, pmi);
+        float ph = rpt_target(pp, pn, &pm, cp, &prv);
+        float contrib = ph * fminf(prv.W, 50.0f) * (float)prv.M * jac;
+        if (contrib <= 0.0f) continue;
+        cur.w_sum += contrib;
+        uint32_t seed = orc_make_seed(px, py, p->frame_seed + 1u);
+        if (cur.w_sum > 0.0f && orc_rand_float(&seed) < contrib / cur.w_sum) {
+            cur.rc_pos = prv.rc_pos; cur.rc_n = prv.rc_n; cur.suffix = prv.suffix; cur.prefix = prv.prefix;
+            cur.valid = 1.0f;
+        }
+        uint32_t nM = cur.M + prv.M;
+        if (nM > cap) { cur.w_sum *= (float)cap / (float)nM; cur.M = cap; }
+        else cur.M = nM;
+        if (cur.M > 0 && cur.valid > 0.5f) {
+            float py_ = rpt_target(pp, pn, &pm, cp, &cur);
+            cur.W = (py_ > 1e-10f) ? (cur.w_sum / ((float)cur.M * py_)) : 0.0f;
+            cur.W = fminf(cur.W, 50.0f);
+        }
+        rpt_store(res, npix, i, &cur);
+    }
+}
+
+static float fracf_(float x) { return x - floorf(x); }
+
+void orc_rpt_spatial_shade(const orc_scene* s, const orc_camera* cam, const orc_rpt_params* p, uint32_t* accum,
+                           float base_cell, float slope, uint32_t cell_count, const float* gpos, const float* gnrm,
+                           const float* direct, const float* res, float* out, int threads) {
+    const uint32_t W = cam->screen_width, H = cam->screen_height;
+    const size_t npix = (size_t)W * H;
+    const v3 cp = vld(cam->position);
+    float cpa[3]; vst(cpa, cp);
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t ii = 0; ii < (int64_t)npix; ++ii) {
+        const size_t i = (size_t)ii;
+        const uint32_t px = (uint32_t)(i % W), py = (uint32_t)(i / W);
+        v3 L = V(0, 0, 0);
+        if (p->include_direct_lighting != 0 && direct) L = vld(direct + i * 4);
+        v3 pp, pn; uint32_t pmi;
+        if (!rpt_surface(gpos, gnrm, i, &pp, &pn, &pmi)) {
+            if (p->include_direct_lighting == 0) L = vadd(L, sky(center_ray_dir(cam, px, py)));
+        } else {
+            const matp pm = load_mat(s, pmi);
+            if (pm.emissive.x > 0.0f || pm.emissive.y > 0.0f || pm.emissive.z > 0.0f) {
+                L = vadd(L, pm.emissive);
+            } else {
+                rpt_res r = rpt_load(res, npix, i);
+                if (p->spatial_enable && p->spatial_num_neighbors > 0 && r.valid > 0.5f) {
+                    float base_phase = fracf_((float)orc_pcg_hash(px ^ (py * 65537u)) * 2.3283064365386963e-10f);
+                    float dist_scale = fmaxf(vlength(vsub(pp, cp)), 1.0f);
+                    for (uint32_t k = 0; k < p->spatial_num_neighbors; ++k) {
+                        float u = ((float)k + 0.5f) / (float)p->spatial_num_neighbors;
+                        float angle = fracf_(base_phase + 0.61803398875f * (float)k) * K_TWO_PI;
+                        float radius = sqrtf(u) * (float)p->spatial_radius;
+                        float sa, ca; orc_sincosf(angle, &sa, &ca);
+                        int nx = (int)px + (int)(radius * ca), ny = (int)py + (int)(radius * sa);
+                        nx = nx < 0 ? 0 : (nx > (int)W - 1 ? (int)W - 1 : nx);
+                        ny = ny < 0 ? 0 : (ny > (int)H - 1 ? (int)H - 1 : ny);
+                        if (nx == (int)px && ny == (int)py) continue;
+                        size_t j = (size_t)ny * W + (size_t)nx;
+                        /* is_spatial_neighbor_compatible, restir_pt.slang:403-421 */
+                        const float* q4 = gpos + j * 4;
+                        if (q4[3] < 0.0f) continue;
+                        v3 q = vld(q4), qn = vnorm(vld(gnrm + j * 4));
+                        if (vdot(qn, pn) < 0.85f) continue;
+                        if (!(fabsf(vdot(vsub(q, pp), pn)) <= 0.02f * dist_scale)) continue;
+                        rpt_res nb = rpt_load(res, npix, j);
+                        if (nb.M == 0 || nb.valid < 0.5f) continue;
+                        float jac = rpt_jacobian(q, pp, nb.rc_pos, nb.rc_n);
+                        if (jac <= 0.0f) continue;
+                        float contrib = rpt_target(pp, pn, &pm, cp, &nb) * nb.W * (float)nb.M * jac;
+                        if (contrib <= 0.0f) continue;
+                        r.w_sum += contrib;
+                        r.M += nb.M;
+                        float accept = fracf_(base_phase + 0.754877666f * (float)(k + 1u));
+                        if (r.w_sum > 0.0f && accept < contrib / r.w_sum) {
+                            r.rc_pos = nb.rc_pos; r.rc_n = nb.rc_n; r.suffix = nb.suffix; r.prefix = nb.prefix;
+                        }
+                    }
+                    if (r.M > 0) {
+                        float ph = rpt_target(pp, pn, &pm, cp, &r);
+                        r.W = (ph > 1e-10f) ? (r.w_sum / ((float)r.M * ph)) : 0.0f;
+                    }
+                }
+                if (r.M > 0 && r.W > 0.0f && r.valid > 0.5f) {
+                    v3 ind = vscale(rpt_contribution(pp, pn, &pm, cp, &r), r.W);
+                    if (lum709(ind) > 0.0f && rpt_visible(s, pp, pn, r.rc_pos, p->ray_bias))
+                        L = vadd(L, vscale(ind, p->indirect_strength));
+                    if (p->guiding_enable && accum) {
+                        v3 to = vsub(r.rc_pos, pp);
+                        float d2 = vdot(to, to);
+                        if (d2 > 1e-8f) {
+                            float ppa[3], dir[3];
+                            vst(ppa, pp); vst(dir, vscale(to, 1.0f / sqrtf(d2)));
+                            uint32_t cell = orc_guiding_cell_index(ppa, pmi, cpa, base_cell, slope, cell_count);
+                            uint32_t* a = accum + (size_t)cell * 4;
+                            uint32_t dx = (uint32_t)(int32_t)roundf(dir[0] * 1048576.0f);
+                            uint32_t dy = (uint32_t)(int32_t)roundf(dir[1] * 1048576.0f);
+                            uint32_t dz = (uint32_t)(int32_t)roundf(dir[2] * 1048576.0f);
+#pragma omp atomic
+                            a[0] += dx;
+#pragma omp atomic
+                            a[1] += dy;
+#pragma omp atomic
+                            a[2] += dz;
+#pragma omp atomic
+                            a[3] += 1u;
+                        }
+                    }
+                }
+            }
+        }
+        float* o = out + i * 4;
+        o[0] = L.x; o[1] = L.y; o[2] = L.z; o[3] = 1.0f;
+    }
+}

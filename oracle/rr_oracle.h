@@ -139,6 +139,27 @@ void orc_di_spatial_shade(const orc_scene* s, const orc_camera* cam, const orc_d
                           const float* gbuf_pos, const float* gbuf_norm, const float* res,
                           const uint32_t* res_uv, float* out, int threads);
 
+/* ---- ReSTIR PT, reconnection shift: restir_pt.slang:98-1001 -------------------- */
+typedef struct {
+    uint32_t frame_seed, num_initial_candidates, max_bounces, temporal_enable, spatial_enable;
+    uint32_t spatial_num_neighbors, spatial_radius, include_direct_lighting;
+    float indirect_strength, ray_bias;
+    uint32_t guiding_enable;
+    float guiding_alpha;
+} orc_rpt_params;
+/* res: four planes of W*H float4 back to back (reconnection position + w_sum, normal + M,
+ * suffix + W, prefix + valid); fitted: the hash grid's lobes (4 floats per cell) or NULL;
+ * accum: its splat accumulator (4 uint32 per cell) or NULL */
+void orc_rpt_initial(const orc_scene* s, const orc_camera* cam, const orc_rpt_params* p, const float* fitted,
+                     float base_cell, float slope, uint32_t cell_count, const float* gbuf_pos,
+                     const float* gbuf_norm, float* res, float* hist_pos, float* hist_norm, int threads);
+void orc_rpt_temporal(const orc_scene* s, const orc_camera* cam, const orc_rpt_params* p, const float* gbuf_pos,
+                      const float* gbuf_norm, float* res, const float* res_prev, const float* hist_prev_pos,
+                      const float* hist_prev_norm, int threads);
+void orc_rpt_spatial_shade(const orc_scene* s, const orc_camera* cam, const orc_rpt_params* p, uint32_t* accum,
+                           float base_cell, float slope, uint32_t cell_count, const float* gbuf_pos,
+                           const float* gbuf_norm, const float* direct, const float* res, float* out, int threads);
+
 /* ---- A-trous: atrous.slang:21-106 (reference mode) and SVGF-style config mode ---- */
 typedef struct {
     uint32_t width, height, stride, mode;       /* mode 0 reference, 1 config (fp16 I/O) */

@@ -85,6 +85,14 @@ class DIParams(C.Structure):
                 ("_pad0", C.c_uint32), ("_pad1", C.c_uint32)]
 
 
+class RPTParams(C.Structure):
+    _fields_ = [("frame_seed", C.c_uint32), ("num_initial_candidates", C.c_uint32), ("max_bounces", C.c_uint32),
+                ("temporal_enable", C.c_uint32), ("spatial_enable", C.c_uint32),
+                ("spatial_num_neighbors", C.c_uint32), ("spatial_radius", C.c_uint32),
+                ("include_direct_lighting", C.c_uint32), ("indirect_strength", C.c_float), ("ray_bias", C.c_float),
+                ("guiding_enable", C.c_uint32), ("guiding_alpha", C.c_float)]
+
+
 class AtrousParams(C.Structure):
     _fields_ = [("iterations", C.c_uint32), ("mode", C.c_uint32), ("sigma_position", C.c_float),
                 ("sigma_normal", C.c_float), ("sigma_luminance", C.c_float), ("_pad", C.c_uint32 * 3)]
@@ -156,6 +164,11 @@ SIGNATURES = {
     "rrh_di_temporal": [P, P, C.POINTER(DIParams), P, P, P, P, P, P],
     "rrh_di_spatial_shade": [P, P, C.POINTER(DIParams), P, P, P, P, P],
     "rrh_di_ray_counts": [P, C.POINTER(RayCounts)],
+    "rrh_rpt_initial": [P, P, C.POINTER(RPTParams), P, P, P, P, P, P],
+    "rrh_rpt_temporal": [P, P, C.POINTER(RPTParams), P, P, P, P, P, P],
+    "rrh_rpt_spatial_shade": [P, P, C.POINTER(RPTParams), P, P, P, P, P, P],
+    "rrh_rpt_clear": [P, P, P, P],
+    "rrh_rpt_ray_counts": [P, C.POINTER(RayCounts)],
     "rrh_atrous": [P, C.POINTER(AtrousParams), P, P, P, P, P, C.POINTER(i32)],
     "rrh_atrous_prepare": [P, P, P, P, P, P, P],
     "rrh_grid_create": [P, C.POINTER(GridParams), PP],

@@ -35,6 +35,7 @@ SOURCES = [
     "rrh_bvh.cpp",
     "rrh_bvh_gpu.hip",
     "rrh_gbuffer.hip",
+    "rrh_restir_pt.hip",
     "rrh_scenegen.cpp",
 ]
 

@@ -213,6 +213,14 @@ class Image:
             pass
 
 
+def planes(images) -> C.Array:
+    """Reservoir planes as the C array of image handles the rrh_rpt_* entry points take."""
+    images = list(images)
+    if len(images) != 4:
+        raise ValueError("a ReSTIR PT reservoir has four planes")
+    return (C.c_void_p * 4)(*[im.h.value for im in images])
+
+
 def trace_closest(ctx: Context, scene: Scene, rays: np.ndarray) -> np.ndarray:
     """rays: (n, 8) float32 = origin, tmin, dir, tmax.  Returns (n, 4) float32 hits (t, prim bits, b1, b2)."""
     rays = np.ascontiguousarray(rays, dtype=np.float32)

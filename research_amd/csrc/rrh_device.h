@@ -654,7 +654,8 @@ RRH_D Hit trace_closest4_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax
     }
 }
 
-template <bool Q, class Stk>
+// CULL: only triangles facing the ray occlude (as in trace_closest4_s)
+template <bool Q, bool CULL = false, class Stk>
 RRH_D bool trace_occluded4_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack) {
     if (!(tmax >= tmin)) return false;
     WRay r = wray_setup(o, d);
@@ -691,7 +692,13 @@ RRH_D bool trace_occluded4_s(const DevScene& S, f3 o, f3 d, float tmin, float tm
             const float4* tv = S.tris + (size_t)(first + i) * 3;
             float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
             float tt, b1, b2;
-            if (wt_hit(r, xyz(p0), xyz(p1), xyz(p2), tmin, tmax, tt, b1, b2)) return true;
+            if (wt_hit(r, xyz(p0), xyz(p1), xyz(p2), tmin, tmax, tt, b1, b2)) {
+                if constexpr (CULL) {
+                    const f3 a = xyz(p0);
+                    if (!(dot(cross(sub(xyz(p1), a), sub(xyz(p2), a)), d) < 0.0f)) continue;
+                }
+                return true;
+            }
         }
         if (sp == 0) return false;
         node = stack.get(--sp);

@@ -15,25 +15,7 @@
 
 using namespace rrh;
 
-struct rrh_grid {
-    rrh_ctx* ctx = nullptr;
-    rrh_grid_params p{};
-    uint32_t ncell = 0;
-    DevBuf accum, fitted;              // HASH
-    DevBuf lobes, stats, fstats, valid; // VMM
-};
-
 namespace {
-
-// GuidingGrid.slang:54-67
-__device__ __forceinline__ uint32_t hash_cell(f3 p, uint32_t bsdf, f3 cam, float base, float slope, uint32_t count) {
-    float vd = length(sub(p, cam));
-    float cell = fmaxf(base + slope * vd, 1.0e-4f);
-    int qx = (int)floorf(p.x / cell), qy = (int)floorf(p.y / cell), qz = (int)floorf(p.z / cell);
-    uint32_t h = pcg_hash((uint32_t)qx * 73856093u ^ (uint32_t)qy * 19349663u ^ (uint32_t)qz * 83492791u ^
-                          bsdf * 2654435761u);
-    return h % (count > 1u ? count : 1u);
-}
 
 __global__ void k_hash_splat(const float* __restrict__ pos, const uint32_t* __restrict__ bsdf,
                              const float* __restrict__ dir, uint32_t n, f3 cam, float base, float slope,
@@ -41,13 +23,7 @@ __global__ void k_hash_splat(const float* __restrict__ pos, const uint32_t* __re
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     f3 p = mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
-    uint32_t c = hash_cell(p, bsdf[i], cam, base, slope, count);
-    uint32_t* a = accum + (size_t)c * 4;
-    // GuidingGrid.slang:81-89: round to the nearest 2^-20, add as two's complement
-    atomicAdd(a + 0, (uint32_t)(int32_t)roundf(dir[3 * i + 0] * 1048576.0f));
-    atomicAdd(a + 1, (uint32_t)(int32_t)roundf(dir[3 * i + 1] * 1048576.0f));
-    atomicAdd(a + 2, (uint32_t)(int32_t)roundf(dir[3 * i + 2] * 1048576.0f));
-    atomicAdd(a + 3, 1u);
+    hash_splat(accum, hash_cell(p, bsdf[i], cam, base, slope, count), mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]));
 }
 
 __global__ void k_hash_fit(uint4* __restrict__ accum, float4* __restrict__ fitted, uint32_t count,
@@ -81,30 +57,8 @@ __global__ void k_hash_sample(const float4* __restrict__ fitted, const float* __
     f3 p = mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
     f3 N = mk(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
     uint32_t seed = seeds[i];
-    float alpha = 0.0f;
-    bool valid = false;
-    f3 mu = mk(0, 0, 0);
-    float kappa = 0.0f;
-    if (alpha_in > 0.0f) {
-        uint32_t c = hash_cell(p, bsdf[i], cam, base, slope, count);
-        float4 f = fitted[c];
-        mu = xyz(f); kappa = f.w;
-        valid = (kappa > 0.0f) && (dot(mu, mu) > 0.25f);
-        if (valid) alpha = alpha_in;
-    }
-    f3 T, B; build_onb(N, T, B);
-    f3 wi;
-    if (rnd(seed) < alpha) {
-        float u0 = rnd(seed), u1 = rnd(seed), vp;
-        wi = vmf_sample(mu, kappa, u0, u1, vp);
-    } else {
-        float u0 = rnd(seed), u1 = rnd(seed), cp;
-        wi = to_world(cosine_hemisphere(u0, u1, cp), T, B, N);
-    }
-    float ci = dot(N, wi);
-    float pc = (ci > 0.0f) ? (ci * K_INV_PI) : 0.0f;
-    float pv = valid ? vmf_pdf(mu, kappa, wi) : 0.0f;
-    pdfs[i] = alpha * pv + (1.0f - alpha) * pc;
+    const HashGrid G{fitted, nullptr, base, slope, count};
+    const f3 wi = hash_primary_dir(G, p, N, bsdf[i], cam, alpha_in, seed, pdfs[i]);
     dirs[3 * i] = wi.x; dirs[3 * i + 1] = wi.y; dirs[3 * i + 2] = wi.z;
     seeds[i] = seed;
 }

@@ -11,6 +11,69 @@ namespace rrh {
 
 constexpr uint32_t kMaxLobes = 8;
 
+// ---- the reference's single-lobe hash grid (GuidingGrid.slang), shared by the grid's own
+// kernels and ReSTIR PT ---------------------------------------------------------------------
+struct HashGrid {
+    const float4* fitted = nullptr;  // per cell (mu, kappa); nullptr: no guiding
+    uint32_t* accum = nullptr;       // per cell (sum x, y, z in 2^-20 units, count)
+    float base = 0.0f, slope = 0.0f;
+    uint32_t count = 1;
+};
+
+// GuidingGrid.slang:54-67: cell edge grows with the distance to the camera; the quantised
+// position and the BSDF id hash to one of `count` cells
+RRH_D uint32_t hash_cell(f3 p, uint32_t bsdf, f3 cam, float base, float slope, uint32_t count) {
+    float vd = length(sub(p, cam));
+    float cell = fmaxf(base + slope * vd, 1.0e-4f);
+    int qx = (int)floorf(p.x / cell), qy = (int)floorf(p.y / cell), qz = (int)floorf(p.z / cell);
+    uint32_t h = pcg_hash((uint32_t)qx * 73856093u ^ (uint32_t)qy * 19349663u ^ (uint32_t)qz * 83492791u ^
+                          bsdf * 2654435761u);
+    return h % (count > 1u ? count : 1u);
+}
+
+// GuidingGrid.slang:73-90: each component rounded to the nearest 2^-20 and added as a two's
+// complement integer (order-independent sums), plus the sample count
+RRH_D void hash_splat(uint32_t* accum, uint32_t cell, f3 dir) {
+    uint32_t* a = accum + (size_t)cell * 4;
+    atomicAdd(a + 0, (uint32_t)(int32_t)roundf(dir.x * 1048576.0f));
+    atomicAdd(a + 1, (uint32_t)(int32_t)roundf(dir.y * 1048576.0f));
+    atomicAdd(a + 2, (uint32_t)(int32_t)roundf(dir.z * 1048576.0f));
+    atomicAdd(a + 3, 1u);
+}
+
+// sample_primary_dir, restir_pt.slang:520-563: one-sample MIS between the cosine hemisphere and
+// the cell's fitted lobe (GuidingGrid.slang:147-157 decides whether the lobe is usable);
+// src_pdf = alpha p_vmf + (1 - alpha) p_cos at the chosen direction
+RRH_D f3 hash_primary_dir(const HashGrid& G, f3 p, f3 N, uint32_t bsdf, f3 cam, float alpha_in, uint32_t& seed,
+                          float& src_pdf) {
+    float alpha = 0.0f;
+    bool valid = false;
+    f3 mu = mk(0.0f, 0.0f, 0.0f);
+    float kappa = 0.0f;
+    if (G.fitted && alpha_in > 0.0f) {
+        const float4 f = G.fitted[hash_cell(p, bsdf, cam, G.base, G.slope, G.count)];
+        mu = xyz(f);
+        kappa = f.w;
+        valid = (kappa > 0.0f) && (dot(mu, mu) > 0.25f);
+        if (valid) alpha = alpha_in;
+    }
+    f3 T, B;
+    build_onb(N, T, B);
+    f3 wi;
+    if (rnd(seed) < alpha) {
+        float u0 = rnd(seed), u1 = rnd(seed), vp;
+        wi = vmf_sample(mu, kappa, u0, u1, vp);
+    } else {
+        float u0 = rnd(seed), u1 = rnd(seed), cp;
+        wi = to_world(cosine_hemisphere(u0, u1, cp), T, B, N);
+    }
+    float ci = dot(N, wi);
+    float pc = (ci > 0.0f) ? (ci * K_INV_PI) : 0.0f;
+    float pv = valid ? vmf_pdf(mu, kappa, wi) : 0.0f;
+    src_pdf = alpha * pv + (1.0f - alpha) * pc;
+    return wi;
+}
+
 struct GuideDev {
     const float* lobes = nullptr;      // VMM: res^3 cells x K x (mu3, kappa, pi)
     const float* mlp_lobes = nullptr;  // MLP: per ray-queue slot K x (mu3, kappa, pi)

@@ -109,6 +109,7 @@ struct rrh_ctx {
     uint32_t last_record_stride = 0;
     uint32_t last_npix = 0;
     rrh::DevBuf di_counter;
+    rrh::DevBuf rpt_counter;      // ReSTIR PT ray counters (see rrh_restir_pt.hip)
     rrh::DevBuf totals;           // uint64 x 3: primary, extension, shadow (device-side sums)
     rrh::DevBuf fetch;            // uint64 x 3: node fetches, tri fetches, rays (instrumented traversal)
     bool profiling = false;
@@ -141,6 +142,15 @@ struct rrh_scene {
     rrh_camera cam_host{};
     bool cam_set = false;
     rrh::DevScene dev() const;
+};
+
+// guiding grid (rrh_guide.hip): the reference's hash grid or the 8-lobe EM grid
+struct rrh_grid {
+    rrh_ctx* ctx = nullptr;
+    rrh_grid_params p{};
+    uint32_t ncell = 0;
+    rrh::DevBuf accum, fitted;              // HASH
+    rrh::DevBuf lobes, stats, fstats, valid; // VMM
 };
 
 struct rrh_image {

@@ -1,0 +1,665 @@
+// rrh_restir_pt.hip -- ReSTIR PT with the reconnection shift (restir_pt.slang: reservoir
+// planes :11-25, trace_ray :98-161, sample_lights :171-222, history and reprojection :273-421,
+// reconnection_jacobian :427-447, trace_visibility :449-458, target function :464-489,
+// cs_initial :569-771, cs_temporal :775-872, cs_spatial_shade :876-1001; host order
+// ReSTIRPTPass::execute, src/passes/restir_pt/ReSTIRPTPass.cpp:529-813).
+//
+// One thread per pixel in 8x8 tiles (one wave64 per tile), every stage one kernel: the
+// initial stage traces num_initial_candidates paths of up to max_bounces segments from the
+// primary surface (extension and shadow rays in the same thread, traversal stack in LDS),
+// resamples them by their reconnected contribution and keeps one; the temporal stage
+// reprojects into the previous frame and reconnects its kept sample; the spatial stage
+// reconnects neighbours' samples, shades with one visibility ray and splats the chosen
+// direction into the guiding grid.  Every ray culls back faces (RAY_FLAG_CULL_BACK_FACING_
+// TRIANGLES in the reference): the 4-wide traversal with the culling test (the quantized
+// nodes unless the context traverses fp32 4-wide nodes).
+//
+// Deviations, each for exactness or for the triangle-light scenes:
+//  * 1/sqrtf(x) where the reference writes rsqrt(x), so host and device round alike;
+//  * in scenes whose emissive triangles are lights (RRH_SCENE_TRI_LIGHTS), light sampling
+//    covers them (area sampling, no MIS: emission is never added at a BSDF hit of the suffix)
+//    and the reconnection vertex's own emission is left to the direct lighting pass, which
+//    samples the same lights; with point / directional lights only, as in the reference, the
+//    reconnection vertex's emission is part of the suffix.
+#include "rrh_internal.h"
+#include "rrh_guide_dev.h"
+
+using namespace rrh;
+
+namespace {
+
+// four RGBA32F planes (restir_pt.slang:11-15)
+struct Res4 {
+    float4* pos;  // reconnection position, w_sum
+    float4* nrm;  // reconnection normal, asfloat(M)
+    float4* rad;  // suffix radiance, W
+    float4* pre;  // prefix throughput, valid
+};
+struct Reservoir {
+    f3 rc_pos, rc_n, suffix, prefix;
+    float w_sum, W;
+    uint32_t M;
+    float valid;
+};
+
+RRH_D Reservoir load_res(const Res4& R, size_t i) {
+    const float4 p = R.pos[i], n = R.nrm[i], r = R.rad[i], e = R.pre[i];
+    Reservoir s;
+    s.rc_pos = xyz(p); s.w_sum = p.w;
+    s.rc_n = xyz(n); s.M = __float_as_uint(n.w);
+    s.suffix = xyz(r); s.W = r.w;
+    s.prefix = xyz(e); s.valid = e.w;
+    return s;
+}
+RRH_D void store_res(const Res4& R, size_t i, const Reservoir& s) {
+    R.pos[i] = make_float4(s.rc_pos.x, s.rc_pos.y, s.rc_pos.z, s.w_sum);
+    R.nrm[i] = make_float4(s.rc_n.x, s.rc_n.y, s.rc_n.z, __uint_as_float(s.M));
+    R.rad[i] = make_float4(s.suffix.x, s.suffix.y, s.suffix.z, s.W);
+    R.pre[i] = make_float4(s.prefix.x, s.prefix.y, s.prefix.z, s.valid);
+}
+RRH_D void store_empty(const Res4& R, size_t i) {
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    R.pos[i] = z; R.nrm[i] = z; R.rad[i] = z; R.pre[i] = z;
+}
+
+// trace_ray with back faces culled: closest hit, and the any-hit form for shadow tests
+template <class Stk>
+RRH_D Hit trace_cull(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& st) {
+    if (S.node_fmt == RRH_NODES_WIDE4) return trace_closest4_s<false, false, true>(S, o, d, tmin, tmax, st, nullptr, nullptr);
+    return trace_closest4_s<false, true, true>(S, o, d, tmin, tmax, st, nullptr, nullptr);
+}
+template <class Stk>
+RRH_D bool occluded_cull(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& st) {
+    if (S.node_fmt == RRH_NODES_WIDE4) return trace_occluded4_s<false, true>(S, o, d, tmin, tmax, st);
+    return trace_occluded4_s<true, true>(S, o, d, tmin, tmax, st);
+}
+
+// sample_lights, restir_pt.slang:171-222 (+ triangle lights, see the header)
+template <class Stk>
+RRH_D f3 rpt_lights(const DevScene& S, f3 hit, f3 n, f3 wo, const Mat& m, float bias, uint32_t& seed, Stk& st,
+                    uint32_t& shadow_rays) {
+    const f3 zero = mk(0.0f, 0.0f, 0.0f);
+    const uint32_t N = S.nl + S.ntl;
+    if (N == 0) return zero;
+    const uint32_t li = pick_index(rnd(seed), N);
+    if (li < S.nl) {
+        const rrh_light& L = S.lights[li];
+        f3 ldir;
+        float dist, pl;
+        if (L.type == 0) {
+            const f3 to = sub(ld3(L.position), hit);
+            dist = length(to);
+            ldir = divs(to, fmaxf(dist, 1e-6f));
+            pl = dist * dist;
+        } else {
+            ldir = normalize(neg(ld3(L.direction)));
+            dist = 1e9f;
+            pl = 1.0f;
+        }
+        if (dot(ldir, n) <= 0.0f) return zero;
+        shadow_rays++;
+        if (occluded_cull(S, add(hit, scale(n, bias)), ldir, bias, dist - 0.01f, st)) return zero;
+        const f3 f = mat_eval(m, n, wo, ldir);
+        const float ndl = fmaxf(dot(n, ldir), 0.0f);
+        const f3 irr = divs(scale(ld3(L.emission), L.intensity), (L.type == 0) ? (pl + 1e-6f) : 1.0f);
+        const float pb = mat_ref_pdf(m, wo, ldir, n);
+        const float mis = (L.type == 0) ? 1.0f : (pl * pl) / (pl * pl + pb * pb + 1e-10f);
+        return scale(scale(scale(mul(f, irr), ndl), mis), (float)N);
+    }
+    const float u1 = rnd(seed);
+    const float u2 = rnd(seed);
+    const TriSample ts = tri_light_point(S, li - S.nl, u1, u2);
+    const f3 to = sub(ts.p, hit);
+    const float d2 = dot(to, to);
+    const float dist = sqrtf(d2);
+    const f3 l = scale(to, 1.0f / dist);
+    if (dot(l, n) <= 0.0f) return zero;
+    const float cosl = fabsf(dot(ts.nl, l));
+    if (!(cosl > 0.0f)) return zero;
+    shadow_rays++;
+    if (occluded_cull(S, add(hit, scale(n, bias)), l, bias, dist - 0.01f, st)) return zero;
+    const f3 f = mat_eval(m, n, wo, l);
+    const float ndl = fmaxf(dot(n, l), 0.0f);
+    const float pl = d2 / (cosl * ts.area) / (float)N;
+    return scale(scale(mul(f, ts.Le), ndl), 1.0f / pl);
+}
+
+// evaluate_contribution / target_function, restir_pt.slang:464-489
+RRH_D f3 contribution(f3 pp, f3 pn, const Mat& pm, f3 cam, const Reservoir& r) {
+    const f3 to = sub(r.rc_pos, pp);
+    const float d2 = dot(to, to);
+    if (d2 <= 1e-8f) return mk(0.0f, 0.0f, 0.0f);
+    const f3 wi = scale(to, 1.0f / sqrtf(d2));
+    const float cx = fmaxf(dot(pn, wi), 0.0f);
+    if (cx <= 0.0f) return mk(0.0f, 0.0f, 0.0f);
+    const f3 wo = normalize(sub(cam, pp));
+    const f3 f = mat_eval(pm, pn, wo, wi);
+    return mul(mul(scale(f, cx), r.prefix), r.suffix);
+}
+RRH_D float target(f3 pp, f3 pn, const Mat& pm, f3 cam, const Reservoir& r) {
+    return lum709(contribution(pp, pn, pm, cam, r));
+}
+
+// reconnection_jacobian, restir_pt.slang:427-447
+RRH_D float jacobian(f3 src, f3 dst, f3 rc, f3 rn) {
+    const f3 sd = sub(rc, src), td = sub(rc, dst);
+    const float s2 = dot(sd, sd), t2 = dot(td, td);
+    if (s2 <= 1e-8f || t2 <= 1e-8f) return 0.0f;
+    const f3 sdir = scale(sd, 1.0f / sqrtf(s2)), tdir = scale(td, 1.0f / sqrtf(t2));
+    const float sc = fabsf(dot(rn, neg(sdir))), tc = fabsf(dot(rn, neg(tdir)));
+    if (sc <= 0.05f || tc <= 0.05f) return 0.0f;
+    return clampf((tc * s2) / (sc * t2), 0.0f, 3.0f);
+}
+
+// trace_visibility, restir_pt.slang:449-458
+template <class Stk>
+RRH_D bool visible(const DevScene& S, f3 o, f3 n, f3 target_pos, float bias, Stk& st) {
+    const f3 delta = sub(target_pos, o);
+    const float dist = length(delta);
+    if (dist <= bias) return false;
+    return !occluded_cull(S, add(o, scale(n, bias)), divs(delta, dist), bias, dist - bias, st);
+}
+
+// load_surface_from_images, restir_pt.slang:285-305
+RRH_D bool load_surface(const float4* pos, const float4* nrm, size_t i, f3& p, f3& n, uint32_t& mat) {
+    const float4 a = pos[i];
+    if (a.w < 0.0f) return false;
+    const float4 b = nrm[i];
+    p = xyz(a);
+    n = normalize(xyz(b));
+    mat = __float_as_uint(b.w);
+    return true;
+}
+
+struct Frame {
+    const float4* gpos;
+    const float4* gnrm;
+    uint32_t W, H;
+};
+
+RRH_D bool tile_pixel(uint32_t& px, uint32_t& py, uint32_t W, uint32_t H, bool remap) {
+    const uint32_t tile = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y, remap);
+    px = (tile % gridDim.x) * 8 + (threadIdx.x & 7u);
+    py = (tile / gridDim.x) * 8 + (threadIdx.x >> 3);
+    return px < W && py < H;
+}
+
+// rays of a wave summed into one counter (one atomic per wave)
+RRH_D void count_rays(uint32_t* counter, uint32_t n) {
+    for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
+    if ((threadIdx.x & 63u) == 0 && n) atomicAdd(counter, n);
+}
+
+// ---------------------------------------------------------------------------
+// cs_initial, restir_pt.slang:569-771
+// ---------------------------------------------------------------------------
+template <int STK>
+__global__ void __launch_bounds__(64) k_rpt_initial(DevScene S, rrh_rpt_params P, HashGrid G, Frame F, Res4 R,
+                                                    float4* __restrict__ hpos, float4* __restrict__ hnrm,
+                                                    uint32_t* __restrict__ counter, bool remap) {
+    __shared__ int s_stack[StackSel<STK>::kLds * 64];
+    uint32_t px, py, rays = 0, shadow_rays = 0;
+    if (tile_pixel(px, py, F.W, F.H, remap)) {
+        const size_t i = (size_t)py * F.W + px;
+        f3 pp, pn;
+        uint32_t pmi;
+        if (!load_surface(F.gpos, F.gnrm, i, pp, pn, pmi)) {
+            hpos[i] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+            hnrm[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            store_empty(R, i);
+        } else {
+            hpos[i] = F.gpos[i];
+            hnrm[i] = F.gnrm[i];
+            const rrh_camera& cam = *S.cam;
+            const f3 cp = ld3(cam.position);
+            const Mat pm = load_mat(S, pmi);
+            const float bias = P.ray_bias;
+            const bool rc_emission = S.ntl == 0;
+            const float alpha = P.guiding_enable ? P.guiding_alpha : 0.0f;
+            const uint32_t max_b = P.max_bounces > 1u ? P.max_bounces : 1u;
+            auto st = make_stack<STK>(s_stack);
+            uint32_t seed = make_seed(px, py, P.frame_seed);
+            Reservoir sel;
+            sel.rc_pos = sel.rc_n = sel.suffix = sel.prefix = mk(0.0f, 0.0f, 0.0f);
+            sel.valid = 0.0f;
+            sel.w_sum = sel.W = 0.0f;
+            sel.M = 0;
+            float w_sum = 0.0f;
+            uint32_t Mc = 0;
+            for (uint32_t c = 0; c < P.num_initial_candidates; ++c) {
+                float src_pdf;
+                const f3 wi0 = hash_primary_dir(G, pp, pn, pmi, cp, alpha, seed, src_pdf);
+                if (src_pdf <= 1e-6f || dot(pn, wi0) <= 0.0f) continue;
+                rays++;
+                const Hit h0 = trace_cull(S, add(pp, scale(pn, bias)), wi0, bias, 1e9f, st);
+                if (h0.prim == ~0u) continue;
+                const HitInfo rc = fetch_hit(S, h0.prim, h0.b1, h0.b2);
+                f3 rn = rc.normal;
+                if (dot(rn, wi0) > 0.0f) rn = neg(rn);
+                const Mat rm = load_mat(S, rc.mat);
+                f3 suffix = rc_emission ? rm.emissive : mk(0.0f, 0.0f, 0.0f);
+                const f3 rwo = neg(wi0);
+                suffix = add(suffix, rpt_lights(S, rc.pos, rn, rwo, rm, bias, seed, st, shadow_rays));
+                f3 cur_p = rc.pos, cur_n = rn, cur_wo = rwo;
+                Mat cur_m = rm;
+                f3 thp = mk(1.0f, 1.0f, 1.0f);
+                for (uint32_t b = 1; b < max_b; ++b) {
+                    f3 wi;
+                    float pdf;
+                    const float ux = rnd(seed), uy = rnd(seed);
+                    const f3 f = mat_sample(cur_m, cur_wo, cur_n, ux, uy, wi, pdf);
+                    if (pdf <= 1e-6f || dot(cur_n, wi) <= 0.0f) break;
+                    const float cosc = fmaxf(dot(cur_n, wi), 0.0f);
+                    thp = mul(thp, divs(scale(f, cosc), pdf));
+                    if (fmaxf(thp.x, fmaxf(thp.y, thp.z)) <= 0.0f) break;
+                    rays++;
+                    const Hit h = trace_cull(S, add(cur_p, scale(cur_n, bias)), wi, bias, 1e9f, st);
+                    if (h.prim == ~0u) {
+                        suffix = add(suffix, mul(thp, sky(wi)));
+                        break;
+                    }
+                    const HitInfo hi = fetch_hit(S, h.prim, h.b1, h.b2);
+                    f3 hn = hi.normal;
+                    if (dot(hn, wi) > 0.0f) hn = neg(hn);
+                    const Mat hm = load_mat(S, hi.mat);
+                    const f3 hwo = neg(wi);
+                    suffix = add(suffix, mul(thp, rpt_lights(S, hi.pos, hn, hwo, hm, bias, seed, st, shadow_rays)));
+                    cur_p = hi.pos; cur_n = hn; cur_m = hm; cur_wo = hwo;
+                }
+                Reservoir cand;
+                cand.rc_pos = rc.pos; cand.rc_n = rn; cand.suffix = suffix; cand.prefix = mk(1.0f, 1.0f, 1.0f);
+                cand.M = 1; cand.W = 0.0f; cand.w_sum = 0.0f; cand.valid = 1.0f;
+                const float ph = target(pp, pn, pm, cp, cand);
+                if (ph <= 0.0f) continue;
+                const f3 dl = sub(rc.pos, pp);
+                if (dot(dl, dl) <= 1e-8f) continue;
+                const float w = ph / fmaxf(src_pdf, 1e-6f);
+                w_sum += w;
+                Mc++;
+                if (w_sum > 0.0f && rnd(seed) < w / w_sum) sel = cand;
+            }
+            bool stored = false;
+            if (Mc > 0 && sel.valid > 0.5f) {
+                const float ph = target(pp, pn, pm, cp, sel);
+                if (ph > 1e-10f) {
+                    sel.M = Mc;
+                    sel.w_sum = w_sum;
+                    sel.W = w_sum / ((float)Mc * ph);
+                    store_res(R, i, sel);
+                    stored = true;
+                }
+            }
+            if (!stored) store_empty(R, i);
+        }
+    }
+    count_rays(counter, rays);
+    count_rays(counter + 1, shadow_rays);
+}
+
+// ---------------------------------------------------------------------------
+// cs_temporal, restir_pt.slang:775-872 (reprojection :323-421)
+// ---------------------------------------------------------------------------
+struct History {
+    const float4* pos;
+    const float4* nrm;
+};
+
+RRH_D bool reproject(const rrh_camera& cam, f3 p, uint32_t W, uint32_t H, int& hx, int& hy) {
+    const float* M = cam.prev_view_proj;
+    const float cx = mrow(M, 0, p.x, p.y, p.z, 1.0f), cy = mrow(M, 1, p.x, p.y, p.z, 1.0f),
+                cw = mrow(M, 3, p.x, p.y, p.z, 1.0f);
+    if (cw <= 1.0e-6f) return false;
+    const float nx = cx / cw, ny = cy / cw;
+    if (fabsf(nx) > 1.0f || fabsf(ny) > 1.0f) return false;
+    const float fx = (nx * 0.5f + 0.5f) * (float)W - 0.5f, fy = (ny * 0.5f + 0.5f) * (float)H - 0.5f;
+    hx = (int)floorf(fx + 0.5f);
+    hy = (int)floorf(fy + 0.5f);
+    return hx >= 0 && hy >= 0 && hx < (int)W && hy < (int)H;
+}
+
+template <int STK>
+__global__ void __launch_bounds__(64) k_rpt_temporal(DevScene S, rrh_rpt_params P, Frame F, Res4 R, Res4 Rp,
+                                                     History Hp, uint32_t* __restrict__ counter, bool remap) {
+    __shared__ int s_stack[StackSel<STK>::kLds * 64];
+    uint32_t px, py, rays = 0;
+    if (tile_pixel(px, py, F.W, F.H, remap)) {
+        const size_t i = (size_t)py * F.W + px;
+        f3 pp, pn;
+        uint32_t pmi;
+        const rrh_camera& cam = *S.cam;
+        const f3 cp = ld3(cam.position);
+        int cx, cy;
+        if (load_surface(F.gpos, F.gnrm, i, pp, pn, pmi) && reproject(cam, pp, F.W, F.H, cx, cy)) {
+            // find_reprojected_history_surface: the most consistent of the 3x3 history pixels
+            bool found = false;
+            float best = 1e30f;
+            size_t hj = 0;
+            f3 hp = mk(0.0f, 0.0f, 0.0f);
+            const float vdist = fmaxf(length(sub(pp, cp)), 1.0f);
+            for (int oy = -1; oy <= 1; ++oy)
+                for (int ox = -1; ox <= 1; ++ox) {
+                    const int qx = cx + ox, qy = cy + oy;
+                    if (qx < 0 || qy < 0 || qx >= (int)F.W || qy >= (int)F.H) continue;
+                    const size_t j = (size_t)qy * F.W + (size_t)qx;
+                    f3 q, qn;
+                    uint32_t qm;
+                    if (!load_surface(Hp.pos, Hp.nrm, j, q, qn, qm)) continue;
+                    if (qm != pmi || dot(pn, qn) < 0.9f) continue;
+                    const float plane = fabsf(dot(sub(q, pp), pn));
+                    const float dpos = length(sub(q, pp));
+                    if (!(plane <= 0.01f * vdist && dpos <= 0.05f * vdist)) continue;
+                    const float err = plane * 4.0f + dpos;
+                    if (err < best) { best = err; hj = j; hp = q; found = true; }
+                }
+            if (found) {
+                Reservoir cur = load_res(R, i);
+                Reservoir prv = load_res(Rp, hj);
+                if (prv.M != 0 && !(prv.valid < 0.5f)) {
+                    uint32_t cap = 20u * P.num_initial_candidates;
+                    cap = cap < 1u ? 1u : (cap > 8192u ? 8192u : cap);
+                    prv.M = prv.M < cap ? prv.M : cap;
+                    const float jac = jacobian(hp, pp, prv.rc_pos, prv.rc_n);
+                    bool go = jac > 0.0f;
+                    if (go) {
+                        auto st = make_stack<STK>(s_stack);
+                        rays++;
+                        go = visible(S, pp, pn, prv.rc_pos, P.ray_bias, st);
+                    }
+                    if (go) {
+                        const Mat pm = load_mat(S, pmi);
+                        const float ph = target(pp, pn, pm, cp, prv);
+                        const float contrib = ph * fminf(prv.W, 50.0f) * (float)prv.M * jac;
+                        if (contrib > 0.0f) {
+                            cur.w_sum += contrib;
+                            uint32_t seed = make_seed(px, py, P.frame_seed + 1u);
+                            if (cur.w_sum > 0.0f && rnd(seed) < contrib / cur.w_sum) {
+                                cur.rc_pos = prv.rc_pos; cur.rc_n = prv.rc_n;
+                                cur.suffix = prv.suffix; cur.prefix = prv.prefix;
+                                cur.valid = 1.0f;
+                            }
+                            const uint32_t nM = cur.M + prv.M;
+                            if (nM > cap) {
+                                cur.w_sum *= (float)cap / (float)nM;
+                                cur.M = cap;
+                            } else {
+                                cur.M = nM;
+                            }
+                            if (cur.M > 0 && cur.valid > 0.5f) {
+                                const float py_ = target(pp, pn, pm, cp, cur);
+                                cur.W = (py_ > 1e-10f) ? (cur.w_sum / ((float)cur.M * py_)) : 0.0f;
+                                cur.W = fminf(cur.W, 50.0f);
+                            }
+                            store_res(R, i, cur);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    count_rays(counter, rays);
+}
+
+// ---------------------------------------------------------------------------
+// cs_spatial_shade, restir_pt.slang:876-1001
+// ---------------------------------------------------------------------------
+RRH_D float fracf(float x) { return x - floorf(x); }
+
+template <int STK>
+__global__ void __launch_bounds__(64) k_rpt_spatial_shade(DevScene S, rrh_rpt_params P, HashGrid G, Frame F, Res4 R,
+                                                          const float4* __restrict__ direct, float4* __restrict__ out,
+                                                          uint32_t* __restrict__ counter, bool remap) {
+    __shared__ int s_stack[StackSel<STK>::kLds * 64];
+    uint32_t px, py, rays = 0;
+    if (tile_pixel(px, py, F.W, F.H, remap)) {
+        const size_t i = (size_t)py * F.W + px;
+        const rrh_camera& cam = *S.cam;
+        const f3 cp = ld3(cam.position);
+        const bool with_direct = P.include_direct_lighting != 0 && direct != nullptr;
+        f3 L = mk(0.0f, 0.0f, 0.0f);
+        if (with_direct) L = xyz(direct[i]);
+        f3 pp, pn;
+        uint32_t pmi;
+        if (!load_surface(F.gpos, F.gnrm, i, pp, pn, pmi)) {
+            if (P.include_direct_lighting == 0) L = add(L, sky(center_dir(cam, px, py)));
+        } else {
+            const Mat pm = load_mat(S, pmi);
+            if (is_emissive(pm)) {
+                L = add(L, pm.emissive);
+            } else {
+                Reservoir res = load_res(R, i);
+                if (P.spatial_enable && P.spatial_num_neighbors > 0 && res.valid > 0.5f) {
+                    const float base_phase = fracf((float)pcg_hash(px ^ (py * 65537u)) * 2.3283064365386963e-10f);
+                    const float dist_scale = fmaxf(length(sub(pp, cp)), 1.0f);
+                    for (uint32_t k = 0; k < P.spatial_num_neighbors; ++k) {
+                        const float u = ((float)k + 0.5f) / (float)P.spatial_num_neighbors;
+                        const float angle = fracf(base_phase + 0.61803398875f * (float)k) * K_TWO_PI;
+                        const float radius = sqrtf(u) * (float)P.spatial_radius;
+                        float sa, ca;
+                        sincos_rr(angle, &sa, &ca);
+                        int nx = (int)px + (int)(radius * ca), ny = (int)py + (int)(radius * sa);
+                        nx = nx < 0 ? 0 : (nx > (int)F.W - 1 ? (int)F.W - 1 : nx);
+                        ny = ny < 0 ? 0 : (ny > (int)F.H - 1 ? (int)F.H - 1 : ny);
+                        if (nx == (int)px && ny == (int)py) continue;
+                        const size_t j = (size_t)ny * F.W + (size_t)nx;
+                        // is_spatial_neighbor_compatible, restir_pt.slang:403-421
+                        const float4 q4 = F.gpos[j];
+                        if (q4.w < 0.0f) continue;
+                        const f3 q = xyz(q4), qn = normalize(xyz(F.gnrm[j]));
+                        if (dot(qn, pn) < 0.85f) continue;
+                        if (!(fabsf(dot(sub(q, pp), pn)) <= 0.02f * dist_scale)) continue;
+                        const Reservoir nb = load_res(R, j);
+                        if (nb.M == 0 || nb.valid < 0.5f) continue;
+                        const float jac = jacobian(q, pp, nb.rc_pos, nb.rc_n);
+                        if (jac <= 0.0f) continue;
+                        const float contrib = target(pp, pn, pm, cp, nb) * nb.W * (float)nb.M * jac;
+                        if (contrib <= 0.0f) continue;
+                        res.w_sum += contrib;
+                        res.M += nb.M;
+                        const float accept = fracf(base_phase + 0.754877666f * (float)(k + 1u));
+                        if (res.w_sum > 0.0f && accept < contrib / res.w_sum) {
+                            res.rc_pos = nb.rc_pos; res.rc_n = nb.rc_n;
+                            res.suffix = nb.suffix; res.prefix = nb.prefix;
+                        }
+                    }
+                    if (res.M > 0) {
+                        const float ph = target(pp, pn, pm, cp, res);
+                        res.W = (ph > 1e-10f) ? (res.w_sum / ((float)res.M * ph)) : 0.0f;
+                    }
+                }
+                if (res.M > 0 && res.W > 0.0f && res.valid > 0.5f) {
+                    const f3 ind = scale(contribution(pp, pn, pm, cp, res), res.W);
+                    if (lum709(ind) > 0.0f) {
+                        auto st = make_stack<STK>(s_stack);
+                        rays++;
+                        if (visible(S, pp, pn, res.rc_pos, P.ray_bias, st)) L = add(L, scale(ind, P.indirect_strength));
+                    }
+                    // splat (restir_pt.slang:982-994): the direction toward the kept
+                    // reconnection vertex trains next frame's guiding
+                    if (P.guiding_enable && G.accum) {
+                        const f3 to = sub(res.rc_pos, pp);
+                        const float d2 = dot(to, to);
+                        if (d2 > 1e-8f)
+                            hash_splat(G.accum, hash_cell(pp, pmi, cp, G.base, G.slope, G.count),
+                                       scale(to, 1.0f / sqrtf(d2)));
+                    }
+                }
+            }
+        }
+        out[i] = make_float4(L.x, L.y, L.z, 1.0f);
+    }
+    count_rays(counter, rays);
+}
+
+__global__ void k_rpt_clear(Res4 R, float4* __restrict__ hpos, float4* __restrict__ hnrm, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    store_empty(R, i);
+    hpos[i] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+    hnrm[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// counters: [0] extension rays of the initial stage, [1] its shadow rays and the temporal
+// visibility rays, [2] the spatial stage's visibility rays (the primary surface comes from the
+// G-buffer, so no primary rays)
+__global__ void k_rpt_totals(const uint32_t* __restrict__ cnt, unsigned long long* __restrict__ tot) {
+    if (threadIdx.x != 0) return;
+    tot[1] += cnt[0];
+    tot[2] += cnt[1] + cnt[2];
+}
+
+// ---- host side -----------------------------------------------------------------------
+bool fits(const rrh_image* a, uint32_t W, uint32_t H) { return a && a->w == W && a->h == H && a->fmt == RRH_FMT_RGBA32F; }
+
+int res4(rrh_image* const* r, uint32_t W, uint32_t H, Res4& out, const char* who) {
+    if (!r) { set_error(std::string(who) + ": null reservoir"); return RRH_E_ARG; }
+    for (int k = 0; k < 4; ++k)
+        if (!fits(r[k], W, H)) {
+            set_error(std::string(who) + ": reservoir planes must be four RGBA32F images of the camera's extent");
+            return RRH_E_ARG;
+        }
+    out = Res4{(float4*)r[0]->p, (float4*)r[1]->p, (float4*)r[2]->p, (float4*)r[3]->p};
+    return RRH_OK;
+}
+
+int hash_grid(rrh_grid* g, HashGrid& G, const char* who) {
+    G = HashGrid{};
+    if (!g) return RRH_OK;
+    if (g->p.kind != RRH_GRID_HASH) { set_error(std::string(who) + ": guiding grid must be of kind RRH_GRID_HASH"); return RRH_E_ARG; }
+    G.fitted = (const float4*)g->fitted.p;
+    G.accum = (uint32_t*)g->accum.p;
+    G.base = g->p.base_cell_size;
+    G.slope = g->p.distance_slope;
+    G.count = g->p.cell_count;
+    return RRH_OK;
+}
+
+// shared checks (the ray counters are cleared by the initial stage)
+int prepare(rrh_ctx* c, rrh_scene* s, const rrh_rpt_params* p, const rrh_image* gpos, const rrh_image* gnrm,
+            Frame& F, const char* who) {
+    if (!(c && s && p && gpos && gnrm)) { set_error(std::string(who) + ": null argument"); return RRH_E_ARG; }
+    if (!s->cam_set) { set_error(std::string(who) + ": camera not set (rrh_camera_set)"); return RRH_E_ARG; }
+    const uint32_t W = s->cam_host.screen_width, H = s->cam_host.screen_height;
+    if (!fits(gpos, W, H) || !fits(gnrm, W, H)) {
+        set_error(std::string(who) + ": G-buffer position / normal must be RGBA32F images of the camera's extent");
+        return RRH_E_ARG;
+    }
+    if (p->num_initial_candidates > 4096 || p->max_bounces > 64 || p->spatial_num_neighbors > 64) {
+        set_error(std::string(who) + ": num_initial_candidates <= 4096, max_bounces <= 64, spatial_num_neighbors <= 64");
+        return RRH_E_RANGE;
+    }
+    if (s->stack4 > RRH_STACK) { set_error(std::string(who) + ": tree deeper than the traversal stack"); return RRH_E_STATE; }
+    int rc = c->rpt_counter.ensure(16);
+    if (rc) return rc;
+    F = Frame{(const float4*)gpos->p, (const float4*)gnrm->p, W, H};
+    return RRH_OK;
+}
+
+// culled rays traverse 4-wide nodes, so the stack must hold what the 4-wide tree can need
+int stack_kind(const rrh_ctx* c, const rrh_scene* s) { return stack_for(c, s->stack4); }
+
+template <template <int> class K, class... A>
+void launch(int sk, dim3 g, hipStream_t st, A... a) {
+    if (sk == RRH_STACK_LDS32) hipLaunchKernelGGL(K<RRH_STACK_LDS32>::fn, g, dim3(64), 0, st, a...);
+    else if (sk == RRH_STACK_HYBRID16) hipLaunchKernelGGL(K<RRH_STACK_HYBRID16>::fn, g, dim3(64), 0, st, a...);
+    else if (sk == RRH_STACK_HYBRID32) hipLaunchKernelGGL(K<RRH_STACK_HYBRID32>::fn, g, dim3(64), 0, st, a...);
+    else hipLaunchKernelGGL(K<RRH_STACK_PRIVATE>::fn, g, dim3(64), 0, st, a...);
+}
+template <int S> struct KInit { static constexpr auto fn = k_rpt_initial<S>; };
+template <int S> struct KTemp { static constexpr auto fn = k_rpt_temporal<S>; };
+template <int S> struct KSpat { static constexpr auto fn = k_rpt_spatial_shade<S>; };
+
+}  // namespace
+
+extern "C" {
+
+int rrh_rpt_initial(rrh_ctx* c, rrh_scene* s, const rrh_rpt_params* p, rrh_grid* grid, const rrh_image* gpos,
+                    const rrh_image* gnrm, rrh_image* const res[4], rrh_image* hist_pos, rrh_image* hist_norm) {
+    Frame F;
+    int rc = prepare(c, s, p, gpos, gnrm, F, "rrh_rpt_initial");
+    if (rc) return rc;
+    RRH_ARG(fits(hist_pos, F.W, F.H) && fits(hist_norm, F.W, F.H),
+            "rrh_rpt_initial: history position / normal must be RGBA32F images of the camera's extent");
+    Res4 R;
+    if ((rc = res4(res, F.W, F.H, R, "rrh_rpt_initial"))) return rc;
+    HashGrid G;
+    if ((rc = hash_grid(grid, G, "rrh_rpt_initial"))) return rc;
+    RRH_HIP(hipMemsetAsync(c->rpt_counter.p, 0, 16, c->stream));
+    launch<KInit>(stack_kind(c, s), dim3((F.W + 7) / 8, (F.H + 7) / 8), c->stream, s->dev(), *p, G, F, R,
+                  (float4*)hist_pos->p, (float4*)hist_norm->p, (uint32_t*)c->rpt_counter.p, c->xcd_remap);
+    RRH_LAUNCH_CHECK();
+    return RRH_OK;
+}
+
+int rrh_rpt_temporal(rrh_ctx* c, rrh_scene* s, const rrh_rpt_params* p, const rrh_image* gpos, const rrh_image* gnrm,
+                     rrh_image* const res[4], rrh_image* const res_prev[4], const rrh_image* hist_prev_pos,
+                     const rrh_image* hist_prev_norm) {
+    Frame F;
+    int rc = prepare(c, s, p, gpos, gnrm, F, "rrh_rpt_temporal");
+    if (rc) return rc;
+    RRH_ARG(fits(hist_prev_pos, F.W, F.H) && fits(hist_prev_norm, F.W, F.H),
+            "rrh_rpt_temporal: history position / normal must be RGBA32F images of the camera's extent");
+    Res4 R, Rp;
+    if ((rc = res4(res, F.W, F.H, R, "rrh_rpt_temporal"))) return rc;
+    if ((rc = res4(res_prev, F.W, F.H, Rp, "rrh_rpt_temporal"))) return rc;
+    if (!p->temporal_enable) return RRH_OK;
+    launch<KTemp>(stack_kind(c, s), dim3((F.W + 7) / 8, (F.H + 7) / 8), c->stream, s->dev(), *p, F, R, Rp,
+                  History{(const float4*)hist_prev_pos->p, (const float4*)hist_prev_norm->p},
+                  (uint32_t*)c->rpt_counter.p + 1, c->xcd_remap);
+    RRH_LAUNCH_CHECK();
+    return RRH_OK;
+}
+
+int rrh_rpt_spatial_shade(rrh_ctx* c, rrh_scene* s, const rrh_rpt_params* p, rrh_grid* grid, const rrh_image* gpos,
+                          const rrh_image* gnrm, const rrh_image* direct, rrh_image* const res[4], rrh_image* out) {
+    Frame F;
+    int rc = prepare(c, s, p, gpos, gnrm, F, "rrh_rpt_spatial_shade");
+    if (rc) return rc;
+    RRH_ARG(fits(out, F.W, F.H), "rrh_rpt_spatial_shade: output must be an RGBA32F image of the camera's extent");
+    RRH_ARG(!direct || fits(direct, F.W, F.H),
+            "rrh_rpt_spatial_shade: direct lighting must be an RGBA32F image of the camera's extent");
+    Res4 R;
+    if ((rc = res4(res, F.W, F.H, R, "rrh_rpt_spatial_shade"))) return rc;
+    HashGrid G;
+    if ((rc = hash_grid(grid, G, "rrh_rpt_spatial_shade"))) return rc;
+    launch<KSpat>(stack_kind(c, s), dim3((F.W + 7) / 8, (F.H + 7) / 8), c->stream, s->dev(), *p, G, F, R,
+                  direct ? (const float4*)direct->p : (const float4*)nullptr, (float4*)out->p,
+                  (uint32_t*)c->rpt_counter.p + 2, c->xcd_remap);
+    RRH_LAUNCH_CHECK();
+    if (!c->totals.p) {
+        if ((rc = c->totals.ensure(32))) return rc;
+        RRH_HIP(hipMemsetAsync(c->totals.p, 0, 32, c->stream));
+    }
+    hipLaunchKernelGGL(k_rpt_totals, dim3(1), dim3(64), 0, c->stream, (const uint32_t*)c->rpt_counter.p,
+                       (unsigned long long*)c->totals.p);
+    RRH_LAUNCH_CHECK();
+    return RRH_OK;
+}
+
+int rrh_rpt_clear(rrh_ctx* c, rrh_image* const res[4], rrh_image* hist_pos, rrh_image* hist_norm) {
+    RRH_ARG(c && res && hist_pos && hist_norm, "rrh_rpt_clear: null argument");
+    const uint32_t W = hist_pos->w, H = hist_pos->h;
+    RRH_ARG(fits(hist_pos, W, H) && fits(hist_norm, W, H), "rrh_rpt_clear: history planes must be RGBA32F, same extent");
+    Res4 R;
+    int rc = res4(res, W, H, R, "rrh_rpt_clear");
+    if (rc) return rc;
+    const uint32_t n = W * H;
+    if (n == 0) return RRH_OK;
+    hipLaunchKernelGGL(k_rpt_clear, dim3((n + 255) / 256), dim3(256), 0, c->stream, R, (float4*)hist_pos->p,
+                       (float4*)hist_norm->p, n);
+    RRH_LAUNCH_CHECK();
+    return RRH_OK;
+}
+
+int rrh_rpt_ray_counts(rrh_ctx* c, rrh_ray_counts* out) {
+    RRH_ARG(c && out, "rrh_rpt_ray_counts: null argument");
+    out->primary = out->extension = out->shadow = 0;
+    if (!c->rpt_counter.p) return RRH_OK;
+    uint32_t h[4] = {0, 0, 0, 0};
+    RRH_HIP(hipMemcpyAsync(h, c->rpt_counter.p, 16, hipMemcpyDeviceToHost, c->stream));
+    RRH_HIP(hipStreamSynchronize(c->stream));
+    out->extension = h[0];
+    out->shadow = (uint64_t)h[1] + h[2];
+    return RRH_OK;
+}
+
+}  // extern "C"

@@ -51,6 +51,25 @@ PYBIND11_MODULE(_app, m) {
         .def_readwrite("spatial_radius", &ReSTIRDIPass::spatial_radius)
         .def("reset_history", &ReSTIRDIPass::reset_history)
         .def("is_enabled", &ReSTIRDIPass::is_enabled);
+    py::class_<ReSTIRPTPass>(m, "ReSTIRPTPass")
+        .def_readwrite("include_direct_lighting", &ReSTIRPTPass::include_direct_lighting)
+        .def_readwrite("temporal_enable", &ReSTIRPTPass::temporal_enable)
+        .def_readwrite("spatial_enable", &ReSTIRPTPass::spatial_enable)
+        .def_readwrite("indirect_strength", &ReSTIRPTPass::indirect_strength)
+        .def_readwrite("ray_bias", &ReSTIRPTPass::ray_bias)
+        .def_readwrite("num_initial_candidates", &ReSTIRPTPass::num_initial_candidates)
+        .def_readwrite("max_bounces", &ReSTIRPTPass::max_bounces)
+        .def_readwrite("spatial_num_neighbors", &ReSTIRPTPass::spatial_num_neighbors)
+        .def_readwrite("spatial_radius", &ReSTIRPTPass::spatial_radius)
+        .def_readwrite("guiding_enable", &ReSTIRPTPass::guiding_enable)
+        .def_readwrite("guiding_alpha", &ReSTIRPTPass::guiding_alpha)
+        .def_readwrite("grid_base_cell_size", &ReSTIRPTPass::grid_base_cell_size)
+        .def_readwrite("grid_distance_slope", &ReSTIRPTPass::grid_distance_slope)
+        .def_readwrite("grid_min_samples", &ReSTIRPTPass::grid_min_samples)
+        .def_readwrite("grid_kappa_max", &ReSTIRPTPass::grid_kappa_max)
+        .def("reset_history", &ReSTIRPTPass::reset_history)
+        .def("is_enabled", &ReSTIRPTPass::is_enabled)
+        .def_property_readonly("name", [](const ReSTIRPTPass& p) { return std::string(p.name()); });
     py::class_<AtrousPass>(m, "AtrousPass")
         .def_readwrite("iterations", &AtrousPass::iterations)
         .def_readwrite("sigma_position", &AtrousPass::sigma_position)
@@ -141,6 +160,7 @@ PYBIND11_MODULE(_app, m) {
         .def_property_readonly("pathtracer_pass", &Application::pathtracer_pass, py::return_value_policy::reference_internal)
         .def_property_readonly("accumulate_pass", &Application::accumulate_pass, py::return_value_policy::reference_internal)
         .def_property_readonly("restir_di_pass", &Application::restir_di_pass, py::return_value_policy::reference_internal)
+        .def_property_readonly("restir_pt_pass", &Application::restir_pt_pass, py::return_value_policy::reference_internal)
         .def_property_readonly("atrous_pass", &Application::atrous_pass, py::return_value_policy::reference_internal)
         .def_property_readonly("guiding_pass", &Application::guiding_pass, py::return_value_policy::reference_internal)
         .def_property_readonly("gbuffer_pass", &Application::gbuffer_pass, py::return_value_policy::reference_internal)

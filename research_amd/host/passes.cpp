@@ -232,6 +232,123 @@ void ReSTIRDIPass::execute(FrameContext& fc) {
     first_execute_ = false;
 }
 
+// ---- ReSTIRPTPass (ReSTIRPTPass.cpp:529-813) ------------------------------------------------
+ReSTIRPTPass::~ReSTIRPTPass() {
+    destroy_images();
+    if (grid_) rrh_grid_destroy(grid_);
+    grid_ = nullptr;
+}
+void ReSTIRPTPass::destroy_images() {
+    for (auto& r : reservoir_)
+        for (Image& i : r) i.destroy();
+    for (int k = 0; k < 2; ++k) {
+        history_pos_[k].destroy();
+        history_norm_[k].destroy();
+    }
+    output_img_.destroy();
+}
+void ReSTIRPTPass::initialize(rhi::Device& device, Extent2D extent) {
+    device_ = &device;
+    extent_ = extent;
+    for (auto& r : reservoir_)
+        for (Image& i : r) i.create(device, extent, Format::RGBA32F);
+    for (int k = 0; k < 2; ++k) {
+        history_pos_[k].create(device, extent, Format::RGBA32F);
+        history_norm_[k].create(device, extent, Format::RGBA32F);
+    }
+    output_img_.create(device, extent, Format::RGBA32F);
+    reset_history();
+    initialized_ = true;
+}
+void ReSTIRPTPass::shutdown(rhi::Device&) {
+    destroy_images();
+    if (grid_) rrh_grid_destroy(grid_);
+    grid_ = nullptr;
+    initialized_ = false;
+}
+RenderPass::Reflection ReSTIRPTPass::reflect() const {
+    Reflection r;
+    r.inputs.push_back(desc("gbuffer_position", Format::RGBA32F, extent_, false));
+    r.inputs.push_back(desc("gbuffer_normal", Format::RGBA32F, extent_, false));
+    r.inputs.push_back(desc("direct_radiance", Format::RGBA32F, extent_, false));
+    for (const char* n : {"reservoir_pos", "reservoir_nrm", "reservoir_rad", "reservoir_pre", "history_primary_pos",
+                          "history_primary_norm"})
+        r.outputs.push_back(desc(n, Format::RGBA32F, extent_, true));
+    r.outputs.push_back(desc("output", Format::RGBA32F, extent_, false));
+    return r;
+}
+void ReSTIRPTPass::on_resize(Extent2D e) {
+    if (!initialized_) return;
+    destroy_images();
+    initialize(*device_, e);
+}
+void ReSTIRPTPass::execute(FrameContext& fc) {
+    if (!initialized_ || !fc.scene || !fc.scene->valid() || !gbuf_pos_ || !gbuf_norm_) return;
+    rrh_ctx* c = device_->ctx();
+    rrh_scene* s = fc.scene->handle();
+    // the guiding grid (ReSTIRPTPass.cpp:311-341): made on first use, made again when its
+    // knobs change; a reset clears the accumulator and every fitted lobe (:602-612)
+    if (guiding_enable && (!grid_ || grid_made_[0] != grid_base_cell_size || grid_made_[1] != grid_distance_slope ||
+                           grid_made_[2] != grid_kappa_max || grid_made_min_ != grid_min_samples)) {
+        if (grid_) rrh_grid_destroy(grid_);
+        grid_ = nullptr;
+        rrh_grid_params g{};
+        g.kind = RRH_GRID_HASH;
+        g.cell_count = kGridCellCount;
+        g.base_cell_size = grid_base_cell_size;
+        g.distance_slope = grid_distance_slope;
+        g.min_samples = grid_min_samples;
+        g.kappa_max = grid_kappa_max;
+        check(rrh_grid_create(c, &g, &grid_), "rrh_grid_create");
+        grid_made_[0] = grid_base_cell_size;
+        grid_made_[1] = grid_distance_slope;
+        grid_made_[2] = grid_kappa_max;
+        grid_made_min_ = grid_min_samples;
+        grid_dirty_ = true;
+    }
+    if (grid_dirty_) {
+        if (grid_) check(rrh_grid_reset(c, grid_), "rrh_grid_reset");
+        grid_dirty_ = false;
+    }
+    const uint32_t cur = reservoir_flip_, prev = 1u - reservoir_flip_;
+    rrh_image* res_cur[4];
+    rrh_image* res_prev[4];
+    for (int k = 0; k < 4; ++k) {
+        res_cur[k] = reservoir_[cur][k].handle();
+        res_prev[k] = reservoir_[prev][k].handle();
+    }
+    if (history_dirty_) {  // an explicitly invalid history (ReSTIRPTPass.cpp:616-663)
+        check(rrh_rpt_clear(c, res_prev, history_pos_[prev].handle(), history_norm_[prev].handle()), "rrh_rpt_clear");
+        history_dirty_ = false;
+    }
+    rrh_rpt_params p{};
+    p.frame_seed = frame_count_++;
+    p.num_initial_candidates = num_initial_candidates;
+    p.max_bounces = max_bounces;
+    p.temporal_enable = temporal_enable ? 1u : 0u;
+    p.spatial_enable = spatial_enable ? 1u : 0u;
+    p.spatial_num_neighbors = spatial_num_neighbors;
+    p.spatial_radius = spatial_radius;
+    p.include_direct_lighting = include_direct_lighting ? 1u : 0u;
+    p.indirect_strength = indirect_strength;
+    p.ray_bias = ray_bias;
+    p.guiding_enable = guiding_enable ? 1u : 0u;
+    p.guiding_alpha = guiding_alpha;
+    rrh_grid* g = guiding_enable ? grid_ : nullptr;
+    check(rrh_rpt_initial(c, s, &p, g, gbuf_pos_->handle(), gbuf_norm_->handle(), res_cur, history_pos_[cur].handle(),
+                          history_norm_[cur].handle()),
+          "rrh_rpt_initial");
+    check(rrh_rpt_temporal(c, s, &p, gbuf_pos_->handle(), gbuf_norm_->handle(), res_cur, res_prev,
+                           history_pos_[prev].handle(), history_norm_[prev].handle()),
+          "rrh_rpt_temporal");
+    check(rrh_rpt_spatial_shade(c, s, &p, g, gbuf_pos_->handle(), gbuf_norm_->handle(),
+                                (include_direct_lighting && direct_) ? direct_->handle() : nullptr, res_cur,
+                                output_img_.handle()),
+          "rrh_rpt_spatial_shade");
+    if (g) check(rrh_grid_fit(c, g), "rrh_grid_fit");  // lobes for the next frame (:784-810)
+    reservoir_flip_ ^= 1u;
+}
+
 // ---- AtrousPass (AtrousPass.cpp:243-306) ----------------------------------------------------
 AtrousPass::~AtrousPass() {
     ping_images_[0].destroy();

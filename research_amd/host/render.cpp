@@ -1,5 +1,6 @@
 // render.cpp -- device, images, camera, scene and renderer of the host layer (rr_host.h).
 #include <cmath>
+#include <cstring>
 #include <stdexcept>
 
 #include "rr_host.h"
@@ -134,6 +135,7 @@ void Scene::upload(rhi::Device& d, const GltfScene& g, const std::string& name) 
 void Scene::destroy() {
     if (scene_) rrh_scene_destroy(scene_);
     scene_ = nullptr;
+    have_view_proj_ = false;
 }
 
 rrh_scene_info Scene::info() const {
@@ -151,6 +153,13 @@ void Scene::update_camera(const Camera& cam, uint32_t width, uint32_t height, ui
     rrh_camera c{};
     check(rrh_camera_look_at(eye, ctr, cam.fov_y_deg, width, height, cam.near_plane, cam.far_plane, frame_index, &c),
           "rrh_camera_look_at");
+    // rrh_camera_look_at fills prev_view_proj with this view's matrix; the previous call's
+    // takes its place, and this one is kept for the next frame
+    float vp[16];
+    std::memcpy(vp, c.prev_view_proj, sizeof(vp));
+    if (have_view_proj_) std::memcpy(c.prev_view_proj, last_view_proj_, sizeof(vp));
+    std::memcpy(last_view_proj_, vp, sizeof(vp));
+    have_view_proj_ = true;
     check(rrh_camera_set(scene_, &c), "rrh_camera_set");
 }
 

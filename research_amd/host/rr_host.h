@@ -122,7 +122,8 @@ public:
     // Scene records from a glTF file (no lights, no scene flags: GltfLoader adds neither)
     void upload(rhi::Device& d, const GltfScene& g, const std::string& name);
     void destroy();
-    // Scene::update_camera: camera matrices for the given extent and frame
+    // Scene::update_camera: camera matrices for the given extent and frame; prev_view_proj is
+    // the view-projection of the previous call (Camera::update keeps it, Camera.cpp:66)
     void update_camera(const Camera& cam, uint32_t width, uint32_t height, uint32_t frame_index);
     [[nodiscard]] rrh_scene* handle() const { return scene_; }
     [[nodiscard]] const std::string& name() const { return name_; }
@@ -135,6 +136,8 @@ public:
 private:
     rrh_scene* scene_ = nullptr;
     std::string name_;
+    float last_view_proj_[16] = {};
+    bool have_view_proj_ = false;
 };
 }  // namespace scene
 
@@ -342,6 +345,66 @@ private:
     bool initialized_ = false;
 };
 
+// ReSTIR PT with the reconnection shift (ReSTIRPTPass.h:22-165): initial multi-bounce
+// candidates, temporal and spatial reuse, shading over ReSTIR DI's G-buffer and direct image,
+// first-bounce guiding by the reference's single-lobe hash grid (splat, then fit each frame).
+class ReSTIRPTPass : public RenderPass {
+public:
+    ~ReSTIRPTPass() override;
+    void initialize(rhi::Device& device, rhi::Extent2D extent);
+    void shutdown(rhi::Device& device);
+    bool reload_shader() { return true; }
+    [[nodiscard]] const char* name() const override { return "ReSTIRPTPass"; }
+    [[nodiscard]] Reflection reflect() const override;
+    void on_resize(rhi::Extent2D new_extent) override;
+    void execute(FrameContext& fc) override;
+    // G-buffer position / normal (ReSTIR DI's) and the direct lighting image
+    void set_inputs(const rhi::Image* gbuf_pos, const rhi::Image* gbuf_norm, const rhi::Image* direct) {
+        gbuf_pos_ = gbuf_pos;
+        gbuf_norm_ = gbuf_norm;
+        direct_ = direct;
+    }
+    void reset_history() {
+        reservoir_flip_ = 0;
+        frame_count_ = 0;
+        history_dirty_ = true;
+        grid_dirty_ = true;
+    }
+    [[nodiscard]] const rhi::Image& output_image() const { return output_img_; }
+    [[nodiscard]] rrh_grid* guiding_grid() const { return grid_; }
+
+    bool include_direct_lighting = true;
+    bool temporal_enable = true;
+    bool spatial_enable = true;
+    float indirect_strength = 1.0f;
+    float ray_bias = 1.0e-3f;
+    uint32_t num_initial_candidates = 8;
+    uint32_t max_bounces = 4;
+    uint32_t spatial_num_neighbors = 4;
+    uint32_t spatial_radius = 12;
+    bool guiding_enable = true;
+    float guiding_alpha = 0.5f;
+    float grid_base_cell_size = 0.10f;
+    float grid_distance_slope = 0.02f;
+    uint32_t grid_min_samples = 8;
+    float grid_kappa_max = 512.0f;
+    static constexpr uint32_t kGridCellCount = 1u << 20;
+
+private:
+    void destroy_images();
+    rhi::Device* device_ = nullptr;
+    const rhi::Image *gbuf_pos_ = nullptr, *gbuf_norm_ = nullptr, *direct_ = nullptr;
+    rhi::Image reservoir_[2][4];
+    rhi::Image history_pos_[2], history_norm_[2];
+    rhi::Image output_img_;
+    rrh_grid* grid_ = nullptr;
+    float grid_made_[3] = {0, 0, 0};  // base cell, slope, kappa of the grid as created
+    uint32_t grid_made_min_ = 0;
+    rhi::Extent2D extent_{};
+    bool initialized_ = false, history_dirty_ = true, grid_dirty_ = true;
+    uint32_t reservoir_flip_ = 0, frame_count_ = 0;
+};
+
 // Edge-aware A-trous wavelet filter over the ReSTIR DI output and its G-buffer.
 class AtrousPass : public RenderPass {
 public:
@@ -477,6 +540,7 @@ public:
     passes::GBufferPass* gbuffer_pass() { return gbuffer_pass_; }
     passes::TonemapPass* tonemap_pass() { return tonemap_pass_; }
     passes::GuidingPass* guiding_pass() { return guiding_pass_; }
+    passes::ReSTIRPTPass* restir_pt_pass() { return restir_pt_pass_; }
 
 private:
     void initialize_renderer();
@@ -494,13 +558,14 @@ private:
     passes::GBufferPass* gbuffer_pass_ = nullptr;
     passes::TonemapPass* tonemap_pass_ = nullptr;
     passes::GuidingPass* guiding_pass_ = nullptr;
+    passes::ReSTIRPTPass* restir_pt_pass_ = nullptr;
     const rhi::Image* display_ = nullptr;
 
     int width_ = 1600, height_ = 900;  // Application.h:158-159
     float exposure_ = 1.0f;
     bool input_enabled_ = true, initialized_ = false, close_requested_ = false;
-    bool use_di_ = false, use_denoise_ = false, use_guiding_ = false;
-    bool prev_use_di_ = false, prev_use_denoise_ = false, prev_use_guiding_ = false;
+    bool use_di_ = false, use_denoise_ = false, use_guiding_ = false, use_pt_ = false;
+    bool prev_use_di_ = false, prev_use_denoise_ = false, prev_use_guiding_ = false, prev_use_pt_ = false;
     bool camera_moved_ = true;
     uint32_t accumulated_spp_ = 0, frames_rendered_ = 0;
     std::string current_scene_name_;

@@ -107,6 +107,53 @@ def test_restir_di_and_denoise_modes_equal_direct_calls(ctx, app):
     assert np.array_equal(_bits(got_dn), _bits(ref))
 
 
+@pytest.mark.parametrize("with_di", [True, False])
+def test_restir_pt_mode_equals_direct_calls(ctx, app, with_di):
+    """ReSTIR PT over ReSTIR DI's G-buffer (and its direct image when DI is on), the hash grid
+    splatted and fitted every frame (ReSTIRPTPass.cpp:529-813)."""
+    if with_di:
+        app.set_pass_enabled("restir_di", True)
+    app.set_pass_enabled("restir_pt", True)
+    pt = app.restir_pt_pass
+    assert pt.name == "ReSTIRPTPass" and pt.num_initial_candidates == 8 and pt.max_bounces == 4
+    frames = 3
+    for _ in range(frames):
+        app.step()
+    assert pt.is_enabled() and app.restir_di_pass.is_enabled()
+    got = app.display_image()
+
+    sa, gs, eye, center = _direct_setup(ctx, app)
+    im = lambda fmt=L.FMT_RGBA32F: core.Image(ctx, W, H, fmt)
+    gp, gn, direct, out = im(), im(), im(), im()
+    dres, duv = [im(), im()], [im(L.FMT_R32U), im(L.FMT_R32U)]
+    res = [[im() for _ in range(4)], [im() for _ in range(4)]]
+    hist = [[im(), im()], [im(), im()]]
+    grid = core.Grid(ctx, L.GridParams(L.GRID_HASH, 1 << 20, 0.10, 0.02, 8, 512.0, 0, 0, L.f3((0, 0, 0)),
+                                       L.f3((1, 1, 1)), 0.0, 0.0))
+    for r in dres + duv:
+        r.clear()
+    L.call("rrh_rpt_clear", ctx.h, core.planes(res[1]), hist[1][0].h, hist[1][1].h)
+    for f in range(frames):
+        gs.set_camera(core.camera_look_at(eye, center, sa.fov, W, H, frame_index=f))
+        cur, prev = f & 1, 1 - (f & 1)
+        dprm = L.DIParams(f, 16, 1, 1, 4, 15)
+        L.call("rrh_di_initial", ctx.h, gs.h, C.byref(dprm), gp.h, gn.h, dres[cur].h, duv[cur].h)
+        L.call("rrh_di_temporal", ctx.h, gs.h, C.byref(dprm), gp.h, gn.h, dres[cur].h, duv[cur].h, dres[prev].h,
+               duv[prev].h)
+        L.call("rrh_di_spatial_shade", ctx.h, gs.h, C.byref(dprm), gp.h, gn.h, dres[cur].h, duv[cur].h, direct.h)
+        prm = L.RPTParams(f, 8, 4, 1, 1, 4, 12, 1 if with_di else 0, 1.0, 1e-3, 1, 0.5)
+        L.call("rrh_rpt_initial", ctx.h, gs.h, C.byref(prm), grid.h, gp.h, gn.h, core.planes(res[cur]),
+               hist[cur][0].h, hist[cur][1].h)
+        L.call("rrh_rpt_temporal", ctx.h, gs.h, C.byref(prm), gp.h, gn.h, core.planes(res[cur]),
+               core.planes(res[prev]), hist[prev][0].h, hist[prev][1].h)
+        L.call("rrh_rpt_spatial_shade", ctx.h, gs.h, C.byref(prm), grid.h, gp.h, gn.h,
+               direct.h if with_di else None, core.planes(res[cur]), out.h)
+        L.call("rrh_grid_fit", ctx.h, grid.h)
+    ref = out.download()
+    assert np.isfinite(ref).all() and ref[..., :3].mean() > 0
+    assert np.array_equal(_bits(got), _bits(ref))
+
+
 @pytest.mark.parametrize("mode", ["vmm", "mlp"])
 def test_guiding_mode_runs_and_trains(app, mode):
     app.load_procedural(L.GEN_CORNELL_RANDOM, 1, 1.0)
