@@ -593,9 +593,10 @@ RRH_D void load_node4(const DevScene& S, int node, Node4& N) {
 // to near); the same leaf tests as the binary traversal, so the same answer
 // CULL: only triangles facing the ray count (geometric normal against the ray direction),
 // the front faces of the reference's G-buffer raster pipeline
+// first_hit: return at the first triangle accepted (an occlusion query through the same code)
 template <bool COUNT, bool Q, bool CULL = false, class Stk>
 RRH_D Hit trace_closest4_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack,
-                           uint32_t* n_nodes, uint32_t* n_tris) {
+                           uint32_t* n_nodes, uint32_t* n_tris, bool first_hit = false) {
     Hit h; h.t = tmax; h.prim = ~0u; h.b1 = 0.0f; h.b2 = 0.0f;
     WRay r = wray_setup(o, d);
     int sp = 0;
@@ -647,6 +648,7 @@ RRH_D Hit trace_closest4_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax
                     if (!(dot(cross(sub(xyz(p1), a), sub(xyz(p2), a)), d) < 0.0f)) continue;
                 }
                 h.t = tt; h.prim = id; h.b1 = b1; h.b2 = b2;
+                if (first_hit) return h;
             }
         }
         if (sp == 0) return h;

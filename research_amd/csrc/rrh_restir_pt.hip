@@ -11,8 +11,8 @@
 // reprojects into the previous frame and reconnects its kept sample; the spatial stage
 // reconnects neighbours' samples, shades with one visibility ray and splats the chosen
 // direction into the guiding grid.  Every ray culls back faces (RAY_FLAG_CULL_BACK_FACING_
-// TRIANGLES in the reference): the 4-wide traversal with the culling test (the quantized
-// nodes unless the context traverses fp32 4-wide nodes).
+// TRIANGLES in the reference): the 4-wide traversal with the culling test (quantized nodes; the
+// visibility rays of the reuse stages read the fp32 4-wide nodes when the context selects them).
 //
 // Deviations, each for exactness or for the triangle-light scenes:
 //  * 1/sqrtf(x) where the reference writes rsqrt(x), so host and device round alike;
@@ -21,6 +21,8 @@
 //    and the reconnection vertex's own emission is left to the direct lighting pass, which
 //    samples the same lights; with point / directional lights only, as in the reference, the
 //    reconnection vertex's emission is part of the suffix.
+#include <algorithm>
+
 #include "rrh_internal.h"
 #include "rrh_guide_dev.h"
 
@@ -62,25 +64,23 @@ RRH_D void store_empty(const Res4& R, size_t i) {
     R.pos[i] = z; R.nrm[i] = z; R.rad[i] = z; R.pre[i] = z;
 }
 
-// trace_ray with back faces culled: closest hit, and the any-hit form for shadow tests
-template <class Stk>
-RRH_D Hit trace_cull(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& st) {
-    if (S.node_fmt == RRH_NODES_WIDE4) return trace_closest4_s<false, false, true>(S, o, d, tmin, tmax, st, nullptr, nullptr);
-    return trace_closest4_s<false, true, true>(S, o, d, tmin, tmax, st, nullptr, nullptr);
-}
+// trace_ray with back faces culled as an occlusion query (the initial stage traces through
+// trace_closest4_s with CULL directly)
 template <class Stk>
 RRH_D bool occluded_cull(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& st) {
     if (S.node_fmt == RRH_NODES_WIDE4) return trace_occluded4_s<false, true>(S, o, d, tmin, tmax, st);
     return trace_occluded4_s<true, true>(S, o, d, tmin, tmax, st);
 }
 
-// sample_lights, restir_pt.slang:171-222 (+ triangle lights, see the header)
-template <class Stk>
-RRH_D f3 rpt_lights(const DevScene& S, f3 hit, f3 n, f3 wo, const Mat& m, float bias, uint32_t& seed, Stk& st,
-                    uint32_t& shadow_rays) {
-    const f3 zero = mk(0.0f, 0.0f, 0.0f);
+// sample_lights, restir_pt.slang:171-222 (+ triangle lights, see the header), up to its shadow
+// ray: c = the contribution if the light is visible, and the shadow ray (origin, direction,
+// tmax; tmin = bias).  false: no ray is needed and the contribution is zero.  The shadow test
+// only decides whether c counts, so c is formed before the ray is traced.
+RRH_D bool light_sample(const DevScene& S, f3 hit, f3 n, f3 wo, const Mat& m, float bias, uint32_t& seed, f3& c,
+                        f3& ro, f3& rd, float& tmax) {
+    c = mk(0.0f, 0.0f, 0.0f);
     const uint32_t N = S.nl + S.ntl;
-    if (N == 0) return zero;
+    if (N == 0) return false;
     const uint32_t li = pick_index(rnd(seed), N);
     if (li < S.nl) {
         const rrh_light& L = S.lights[li];
@@ -96,15 +96,17 @@ RRH_D f3 rpt_lights(const DevScene& S, f3 hit, f3 n, f3 wo, const Mat& m, float 
             dist = 1e9f;
             pl = 1.0f;
         }
-        if (dot(ldir, n) <= 0.0f) return zero;
-        shadow_rays++;
-        if (occluded_cull(S, add(hit, scale(n, bias)), ldir, bias, dist - 0.01f, st)) return zero;
+        if (dot(ldir, n) <= 0.0f) return false;
         const f3 f = mat_eval(m, n, wo, ldir);
         const float ndl = fmaxf(dot(n, ldir), 0.0f);
         const f3 irr = divs(scale(ld3(L.emission), L.intensity), (L.type == 0) ? (pl + 1e-6f) : 1.0f);
         const float pb = mat_ref_pdf(m, wo, ldir, n);
         const float mis = (L.type == 0) ? 1.0f : (pl * pl) / (pl * pl + pb * pb + 1e-10f);
-        return scale(scale(scale(mul(f, irr), ndl), mis), (float)N);
+        c = scale(scale(scale(mul(f, irr), ndl), mis), (float)N);
+        ro = add(hit, scale(n, bias));
+        rd = ldir;
+        tmax = dist - 0.01f;
+        return true;
     }
     const float u1 = rnd(seed);
     const float u2 = rnd(seed);
@@ -113,15 +115,17 @@ RRH_D f3 rpt_lights(const DevScene& S, f3 hit, f3 n, f3 wo, const Mat& m, float 
     const float d2 = dot(to, to);
     const float dist = sqrtf(d2);
     const f3 l = scale(to, 1.0f / dist);
-    if (dot(l, n) <= 0.0f) return zero;
+    if (dot(l, n) <= 0.0f) return false;
     const float cosl = fabsf(dot(ts.nl, l));
-    if (!(cosl > 0.0f)) return zero;
-    shadow_rays++;
-    if (occluded_cull(S, add(hit, scale(n, bias)), l, bias, dist - 0.01f, st)) return zero;
+    if (!(cosl > 0.0f)) return false;
     const f3 f = mat_eval(m, n, wo, l);
     const float ndl = fmaxf(dot(n, l), 0.0f);
     const float pl = d2 / (cosl * ts.area) / (float)N;
-    return scale(scale(mul(f, ts.Le), ndl), 1.0f / pl);
+    c = scale(scale(mul(f, ts.Le), ndl), 1.0f / pl);
+    ro = add(hit, scale(n, bias));
+    rd = l;
+    tmax = dist - 0.01f;
+    return true;
 }
 
 // evaluate_contribution / target_function, restir_pt.slang:464-489
@@ -193,104 +197,214 @@ RRH_D void count_rays(uint32_t* counter, uint32_t n) {
 // ---------------------------------------------------------------------------
 // cs_initial, restir_pt.slang:569-771
 // ---------------------------------------------------------------------------
+// The candidate loop runs as a per-lane state machine around ONE traversal call: each pass of
+// the outer loop lets every lane consume the result of its ray in flight and advance to its
+// next ray (the candidate's first segment, a light sample's shadow ray, or the next path
+// segment), then the whole wave traverses together.  The waves are persistent: a lane that
+// finishes its pixel takes the next one from a counter (pixels in 8x8 tile order), so a wave's
+// lanes stay busy until the image runs out instead of waiting for the tile's longest pixel.
+// A pixel's random numbers and arithmetic follow the reference's nested loops exactly; only
+// which lanes traverse together changes.
+enum : uint32_t { kNext, kToRc, kShadow, kBounce, kExt, kFinish, kDone };
+
 template <int STK>
 __global__ void __launch_bounds__(64) k_rpt_initial(DevScene S, rrh_rpt_params P, HashGrid G, Frame F, Res4 R,
                                                     float4* __restrict__ hpos, float4* __restrict__ hnrm,
-                                                    uint32_t* __restrict__ counter, bool remap) {
+                                                    uint32_t* __restrict__ counter, uint32_t tiles_x,
+                                                    uint32_t npad) {
     __shared__ int s_stack[StackSel<STK>::kLds * 64];
-    uint32_t px, py, rays = 0, shadow_rays = 0;
-    if (tile_pixel(px, py, F.W, F.H, remap)) {
-        const size_t i = (size_t)py * F.W + px;
-        f3 pp, pn;
-        uint32_t pmi;
-        if (!load_surface(F.gpos, F.gnrm, i, pp, pn, pmi)) {
-            hpos[i] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
-            hnrm[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            store_empty(R, i);
-        } else {
-            hpos[i] = F.gpos[i];
-            hnrm[i] = F.gnrm[i];
-            const rrh_camera& cam = *S.cam;
-            const f3 cp = ld3(cam.position);
-            const Mat pm = load_mat(S, pmi);
-            const float bias = P.ray_bias;
-            const bool rc_emission = S.ntl == 0;
-            const float alpha = P.guiding_enable ? P.guiding_alpha : 0.0f;
-            const uint32_t max_b = P.max_bounces > 1u ? P.max_bounces : 1u;
-            auto st = make_stack<STK>(s_stack);
-            uint32_t seed = make_seed(px, py, P.frame_seed);
-            Reservoir sel;
-            sel.rc_pos = sel.rc_n = sel.suffix = sel.prefix = mk(0.0f, 0.0f, 0.0f);
-            sel.valid = 0.0f;
-            sel.w_sum = sel.W = 0.0f;
-            sel.M = 0;
-            float w_sum = 0.0f;
-            uint32_t Mc = 0;
-            for (uint32_t c = 0; c < P.num_initial_candidates; ++c) {
-                float src_pdf;
-                const f3 wi0 = hash_primary_dir(G, pp, pn, pmi, cp, alpha, seed, src_pdf);
-                if (src_pdf <= 1e-6f || dot(pn, wi0) <= 0.0f) continue;
-                rays++;
-                const Hit h0 = trace_cull(S, add(pp, scale(pn, bias)), wi0, bias, 1e9f, st);
-                if (h0.prim == ~0u) continue;
-                const HitInfo rc = fetch_hit(S, h0.prim, h0.b1, h0.b2);
-                f3 rn = rc.normal;
-                if (dot(rn, wi0) > 0.0f) rn = neg(rn);
-                const Mat rm = load_mat(S, rc.mat);
-                f3 suffix = rc_emission ? rm.emissive : mk(0.0f, 0.0f, 0.0f);
-                const f3 rwo = neg(wi0);
-                suffix = add(suffix, rpt_lights(S, rc.pos, rn, rwo, rm, bias, seed, st, shadow_rays));
-                f3 cur_p = rc.pos, cur_n = rn, cur_wo = rwo;
-                Mat cur_m = rm;
-                f3 thp = mk(1.0f, 1.0f, 1.0f);
-                for (uint32_t b = 1; b < max_b; ++b) {
+    uint32_t rays = 0, shadow_rays = 0;
+    uint32_t* next_pixel = counter + 3;
+    const f3 cp = ld3(S.cam->position);
+    const float bias = P.ray_bias;
+    const bool rc_emission = S.ntl == 0;
+    const float alpha = P.guiding_enable ? P.guiding_alpha : 0.0f;
+    const uint32_t max_b = P.max_bounces > 1u ? P.max_bounces : 1u;
+    // the pixel (its material is read again where needed: fewer live registers)
+    size_t i = 0;
+    f3 pp = mk(0.0f, 0.0f, 0.0f), pn = pp;
+    uint32_t pmi = 0, seed = 0;
+    // the resampling state of the pixel (the kept candidate waits in the reservoir planes) and
+    // the candidate in progress (the current vertex's material by index; its outgoing
+    // direction is -dir)
+    uint32_t state = kDone;
+    bool exhausted = false, have_sel = false;
+    float w_sum = 0.0f, src_pdf = 0.0f;
+    uint32_t Mc = 0, c = 0, b = 0, cur_mi = 0;
+    f3 dir = pp;  // the candidate's first direction, then the last segment's
+    f3 rc_pos = pp, rc_n = pp, suffix = pp, thp = pp, nee = pp, cur_p = pp, cur_n = pp;
+    // the ray in flight
+    f3 ro = pp, rd = pp;
+    float rtmax = 0.0f;
+    bool first_hit = false;
+    Hit h;
+    h.t = 0.0f; h.prim = ~0u; h.b1 = h.b2 = 0.0f;
+    auto st = make_stack<STK>(s_stack);
+    while (true) {
+        bool want_ray = false;
+        while (true) {
+            // consume the last ray's result and advance to the next ray (or the pixel's end)
+            while (state != kDone && !want_ray) {
+                switch (state) {
+                case kNext:
+                    if (c >= P.num_initial_candidates) {
+                        bool stored = false;
+                        if (Mc > 0 && have_sel) {
+                            Reservoir r;
+                            r.rc_pos = xyz(R.pos[i]); r.rc_n = xyz(R.nrm[i]); r.suffix = xyz(R.rad[i]);
+                            r.prefix = mk(1.0f, 1.0f, 1.0f);
+                            r.valid = 1.0f;
+                            const float ph = target(pp, pn, load_mat(S, pmi), cp, r);
+                            if (ph > 1e-10f) {
+                                r.M = Mc;
+                                r.w_sum = w_sum;
+                                r.W = w_sum / ((float)Mc * ph);
+                                store_res(R, i, r);
+                                stored = true;
+                            }
+                        }
+                        if (!stored) store_empty(R, i);
+                        state = kDone;
+                        break;
+                    }
+                    ++c;
+                    dir = hash_primary_dir(G, pp, pn, pmi, cp, alpha, seed, src_pdf);
+                    if (src_pdf <= 1e-6f || dot(pn, dir) <= 0.0f) break;  // dropped, not counted
+                    ro = add(pp, scale(pn, bias));
+                    rd = dir;
+                    rtmax = 1e9f;
+                    first_hit = false;
+                    rays++;
+                    state = kToRc;
+                    want_ray = true;
+                    break;
+                case kToRc: {
+                    if (h.prim == ~0u) { state = kNext; break; }  // escaped: dropped, not counted
+                    const HitInfo rc = fetch_hit(S, h.prim, h.b1, h.b2);
+                    rc_n = rc.normal;
+                    if (dot(rc_n, dir) > 0.0f) rc_n = neg(rc_n);
+                    rc_pos = rc.pos;
+                    cur_mi = rc.mat;
+                    const Mat m = load_mat(S, cur_mi);
+                    suffix = rc_emission ? m.emissive : mk(0.0f, 0.0f, 0.0f);
+                    cur_p = rc_pos; cur_n = rc_n;
+                    thp = mk(1.0f, 1.0f, 1.0f);
+                    b = 1;
+                    if (light_sample(S, cur_p, cur_n, neg(dir), m, bias, seed, nee, ro, rd, rtmax)) {
+                        first_hit = true;
+                        shadow_rays++;
+                        state = kShadow;
+                        want_ray = true;
+                    } else {
+                        suffix = add(suffix, mul(thp, nee));
+                        state = kBounce;
+                    }
+                    break;
+                }
+                case kShadow:
+                    suffix = add(suffix, mul(thp, h.prim == ~0u ? nee : mk(0.0f, 0.0f, 0.0f)));
+                    state = kBounce;
+                    break;
+                case kBounce: {
+                    if (b >= max_b) { state = kFinish; break; }
                     f3 wi;
                     float pdf;
                     const float ux = rnd(seed), uy = rnd(seed);
-                    const f3 f = mat_sample(cur_m, cur_wo, cur_n, ux, uy, wi, pdf);
-                    if (pdf <= 1e-6f || dot(cur_n, wi) <= 0.0f) break;
+                    const f3 f = mat_sample(load_mat(S, cur_mi), neg(dir), cur_n, ux, uy, wi, pdf);
+                    if (pdf <= 1e-6f || dot(cur_n, wi) <= 0.0f) { state = kFinish; break; }
                     const float cosc = fmaxf(dot(cur_n, wi), 0.0f);
                     thp = mul(thp, divs(scale(f, cosc), pdf));
-                    if (fmaxf(thp.x, fmaxf(thp.y, thp.z)) <= 0.0f) break;
+                    if (fmaxf(thp.x, fmaxf(thp.y, thp.z)) <= 0.0f) { state = kFinish; break; }
+                    ro = add(cur_p, scale(cur_n, bias));
+                    rd = wi;
+                    dir = wi;
+                    rtmax = 1e9f;
+                    first_hit = false;
                     rays++;
-                    const Hit h = trace_cull(S, add(cur_p, scale(cur_n, bias)), wi, bias, 1e9f, st);
+                    ++b;
+                    state = kExt;
+                    want_ray = true;
+                    break;
+                }
+                case kExt: {
                     if (h.prim == ~0u) {
-                        suffix = add(suffix, mul(thp, sky(wi)));
+                        suffix = add(suffix, mul(thp, sky(dir)));
+                        state = kFinish;
                         break;
                     }
                     const HitInfo hi = fetch_hit(S, h.prim, h.b1, h.b2);
                     f3 hn = hi.normal;
-                    if (dot(hn, wi) > 0.0f) hn = neg(hn);
-                    const Mat hm = load_mat(S, hi.mat);
-                    const f3 hwo = neg(wi);
-                    suffix = add(suffix, mul(thp, rpt_lights(S, hi.pos, hn, hwo, hm, bias, seed, st, shadow_rays)));
-                    cur_p = hi.pos; cur_n = hn; cur_m = hm; cur_wo = hwo;
+                    if (dot(hn, dir) > 0.0f) hn = neg(hn);
+                    cur_mi = hi.mat;
+                    cur_p = hi.pos; cur_n = hn;
+                    if (light_sample(S, cur_p, cur_n, neg(dir), load_mat(S, cur_mi), bias, seed, nee, ro, rd, rtmax)) {
+                        first_hit = true;
+                        shadow_rays++;
+                        state = kShadow;
+                        want_ray = true;
+                    } else {
+                        suffix = add(suffix, mul(thp, nee));
+                        state = kBounce;
+                    }
+                    break;
                 }
-                Reservoir cand;
-                cand.rc_pos = rc.pos; cand.rc_n = rn; cand.suffix = suffix; cand.prefix = mk(1.0f, 1.0f, 1.0f);
-                cand.M = 1; cand.W = 0.0f; cand.w_sum = 0.0f; cand.valid = 1.0f;
-                const float ph = target(pp, pn, pm, cp, cand);
-                if (ph <= 0.0f) continue;
-                const f3 dl = sub(rc.pos, pp);
-                if (dot(dl, dl) <= 1e-8f) continue;
-                const float w = ph / fmaxf(src_pdf, 1e-6f);
-                w_sum += w;
-                Mc++;
-                if (w_sum > 0.0f && rnd(seed) < w / w_sum) sel = cand;
-            }
-            bool stored = false;
-            if (Mc > 0 && sel.valid > 0.5f) {
-                const float ph = target(pp, pn, pm, cp, sel);
-                if (ph > 1e-10f) {
-                    sel.M = Mc;
-                    sel.w_sum = w_sum;
-                    sel.W = w_sum / ((float)Mc * ph);
-                    store_res(R, i, sel);
-                    stored = true;
+                case kFinish: {
+                    Reservoir cand;
+                    cand.rc_pos = rc_pos; cand.rc_n = rc_n; cand.suffix = suffix; cand.prefix = mk(1.0f, 1.0f, 1.0f);
+                    cand.M = 1; cand.W = 0.0f; cand.w_sum = 0.0f; cand.valid = 1.0f;
+                    const float ph = target(pp, pn, load_mat(S, pmi), cp, cand);
+                    const f3 dl = sub(rc_pos, pp);
+                    if (!(ph <= 0.0f) && !(dot(dl, dl) <= 1e-8f)) {
+                        const float w = ph / fmaxf(src_pdf, 1e-6f);
+                        w_sum += w;
+                        Mc++;
+                        if (w_sum > 0.0f && rnd(seed) < w / w_sum) {
+                            R.pos[i] = make_float4(rc_pos.x, rc_pos.y, rc_pos.z, 0.0f);
+                            R.nrm[i] = make_float4(rc_n.x, rc_n.y, rc_n.z, 0.0f);
+                            R.rad[i] = make_float4(suffix.x, suffix.y, suffix.z, 0.0f);
+                            have_sel = true;
+                        }
+                    }
+                    state = kNext;
+                    break;
+                }
+                default:
+                    state = kDone;
+                    break;
                 }
             }
-            if (!stored) store_empty(R, i);
+            // lanes without a pixel take the next one
+            const bool need = state == kDone && !exhausted;
+            if (!__any(need)) break;
+            const uint32_t slot = wave_push(next_pixel, need);
+            if (need) {
+                if (slot >= npad) {
+                    exhausted = true;
+                } else {
+                    const uint32_t tile = slot >> 6, l = slot & 63u;
+                    const uint32_t px = (tile % tiles_x) * 8 + (l & 7u), py = (tile / tiles_x) * 8 + (l >> 3);
+                    if (px < F.W && py < F.H) {
+                        i = (size_t)py * F.W + px;
+                        if (load_surface(F.gpos, F.gnrm, i, pp, pn, pmi)) {
+                            hpos[i] = F.gpos[i];
+                            hnrm[i] = F.gnrm[i];
+                            seed = make_seed(px, py, P.frame_seed);
+                            have_sel = false;
+                            w_sum = 0.0f;
+                            Mc = 0;
+                            c = 0;
+                            state = kNext;
+                        } else {
+                            hpos[i] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+                            hnrm[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                            store_empty(R, i);
+                        }
+                    }
+                }
+            }
         }
+        if (!__any(want_ray)) break;  // every lane is out of pixels
+        if (want_ray) h = trace_closest4_s<false, true, true>(S, ro, rd, bias, rtmax, st, nullptr, nullptr, first_hit);
     }
     count_rays(counter, rays);
     count_rays(counter + 1, shadow_rays);
@@ -500,7 +614,7 @@ __global__ void k_rpt_clear(Res4 R, float4* __restrict__ hpos, float4* __restric
 
 // counters: [0] extension rays of the initial stage, [1] its shadow rays and the temporal
 // visibility rays, [2] the spatial stage's visibility rays (the primary surface comes from the
-// G-buffer, so no primary rays)
+// G-buffer, so no primary rays), [3] the initial stage's next pixel
 __global__ void k_rpt_totals(const uint32_t* __restrict__ cnt, unsigned long long* __restrict__ tot) {
     if (threadIdx.x != 0) return;
     tot[1] += cnt[0];
@@ -584,8 +698,18 @@ int rrh_rpt_initial(rrh_ctx* c, rrh_scene* s, const rrh_rpt_params* p, rrh_grid*
     HashGrid G;
     if ((rc = hash_grid(grid, G, "rrh_rpt_initial"))) return rc;
     RRH_HIP(hipMemsetAsync(c->rpt_counter.p, 0, 16, c->stream));
-    launch<KInit>(stack_kind(c, s), dim3((F.W + 7) / 8, (F.H + 7) / 8), c->stream, s->dev(), *p, G, F, R,
-                  (float4*)hist_pos->p, (float4*)hist_norm->p, (uint32_t*)c->rpt_counter.p, c->xcd_remap);
+    // persistent waves: as many as the CUs hold at once (the occupancy the kernel's registers and
+    // LDS stack allow), never more than there are 8x8 tiles
+    const int sk = stack_kind(c, s);
+    int per_cu = 0;
+    if (sk == RRH_STACK_LDS32) RRH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, KInit<RRH_STACK_LDS32>::fn, 64, 0));
+    else if (sk == RRH_STACK_HYBRID16) RRH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, KInit<RRH_STACK_HYBRID16>::fn, 64, 0));
+    else if (sk == RRH_STACK_HYBRID32) RRH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, KInit<RRH_STACK_HYBRID32>::fn, 64, 0));
+    else RRH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, KInit<RRH_STACK_PRIVATE>::fn, 64, 0));
+    const uint32_t tiles_x = (F.W + 7) / 8, tiles = tiles_x * ((F.H + 7) / 8);
+    const uint32_t waves = std::min<uint32_t>(tiles, (uint32_t)c->cu_count * (uint32_t)std::max(per_cu, 1));
+    launch<KInit>(sk, dim3(waves), c->stream, s->dev(), *p, G, F, R, (float4*)hist_pos->p, (float4*)hist_norm->p,
+                  (uint32_t*)c->rpt_counter.p, tiles_x, tiles * 64u);
     RRH_LAUNCH_CHECK();
     return RRH_OK;
 }
