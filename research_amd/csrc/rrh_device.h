@@ -707,10 +707,11 @@ RRH_D bool trace_occluded4_s(const DevScene& S, f3 o, f3 d, float tmin, float tm
     }
 }
 
-// speculative (leaf-parking) traversal over the 4-wide nodes
-template <bool COUNT, bool Q, class Stk>
+// speculative (leaf-parking) traversal over the 4-wide nodes; CULL and first_hit as in
+// trace_closest4_s
+template <bool COUNT, bool Q, bool CULL = false, class Stk>
 RRH_D Hit trace_closest4spec_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack,
-                               uint32_t* n_nodes, uint32_t* n_tris) {
+                               uint32_t* n_nodes, uint32_t* n_tris, bool first_hit = false) {
     Hit h; h.t = tmax; h.prim = ~0u; h.b1 = 0.0f; h.b2 = 0.0f;
     WRay r = wray_setup(o, d);
     int sp = 0, node = 0, leaf = 0;
@@ -758,7 +759,12 @@ RRH_D Hit trace_closest4spec_s(const DevScene& S, f3 o, f3 d, float tmin, float 
                 float tt, b1, b2;
                 uint32_t id = __float_as_uint(p0.w);
                 if (wt_hit(r, xyz(p0), xyz(p1), xyz(p2), tmin, h.t, tt, b1, b2) && better(tt, id, h.t, h.prim)) {
+                    if constexpr (CULL) {
+                        const f3 a = xyz(p0);
+                        if (!(dot(cross(sub(xyz(p1), a), sub(xyz(p2), a)), d) < 0.0f)) continue;
+                    }
                     h.t = tt; h.prim = id; h.b1 = b1; h.b2 = b2;
+                    if (first_hit) return h;
                 }
             }
             leaf = node;

@@ -208,7 +208,7 @@ RRH_D void count_rays(uint32_t* counter, uint32_t n) {
 enum : uint32_t { kNext, kToRc, kShadow, kBounce, kExt, kFinish, kDone };
 
 template <int STK>
-__global__ void __launch_bounds__(64) k_rpt_initial(DevScene S, rrh_rpt_params P, HashGrid G, Frame F, Res4 R,
+__global__ void __launch_bounds__(64, 4) k_rpt_initial(DevScene S, rrh_rpt_params P, HashGrid G, Frame F, Res4 R,
                                                     float4* __restrict__ hpos, float4* __restrict__ hnrm,
                                                     uint32_t* __restrict__ counter, uint32_t tiles_x,
                                                     uint32_t npad) {
@@ -404,7 +404,12 @@ __global__ void __launch_bounds__(64) k_rpt_initial(DevScene S, rrh_rpt_params P
             }
         }
         if (!__any(want_ray)) break;  // every lane is out of pixels
-        if (want_ray) h = trace_closest4_s<false, true, true>(S, ro, rd, bias, rtmax, st, nullptr, nullptr, first_hit);
+        if (want_ray) {
+            if (S.speculative)
+                h = trace_closest4spec_s<false, true, true>(S, ro, rd, bias, rtmax, st, nullptr, nullptr, first_hit);
+            else
+                h = trace_closest4_s<false, true, true>(S, ro, rd, bias, rtmax, st, nullptr, nullptr, first_hit);
+        }
     }
     count_rays(counter, rays);
     count_rays(counter + 1, shadow_rays);
