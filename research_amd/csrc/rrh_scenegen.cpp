@@ -7,6 +7,7 @@
 // pcg_hash so a seed reproduces the scene on any host.
 #include <cmath>
 #include <cstring>
+#include <utility>
 
 #include "rrh_internal.h"
 
@@ -48,6 +49,29 @@ void add_quad(std::vector<rrh_vertex>& vs, std::vector<uint32_t>& is, V3 a, V3 b
     vs.push_back(make_v(d, n, 0, 1));
     uint32_t t[6] = {base, base + 1, base + 2, base, base + 2, base + 3};
     is.insert(is.end(), t, t + 6);
+}
+
+// a quad whose two triangles face its normal n: geometric normal (v1 - v0) x (v2 - v0) along
+// n, the front face of back-face culling (the G-buffer and ReSTIR PT cull back faces).  The
+// reference's add_quad keeps the corner order it is given, and several quads of its Cornell box
+// face away from their normals; the reference scene keeps that, the new generators face out.
+void add_quad_facing(std::vector<rrh_vertex>& vs, std::vector<uint32_t>& is, V3 a, V3 b, V3 c, V3 d, V3 n) {
+    const float ux = b.x - a.x, uy = b.y - a.y, uz = b.z - a.z, vx = c.x - a.x, vy = c.y - a.y, vz = c.z - a.z;
+    const float gx = uy * vz - uz * vy, gy = uz * vx - ux * vz, gz = ux * vy - uy * vx;
+    if (gx * n.x + gy * n.y + gz * n.z < 0.0f) add_quad(vs, is, a, d, c, b, n);
+    else add_quad(vs, is, a, b, c, d, n);
+}
+
+// a box whose faces all face outward
+void add_box_facing(std::vector<rrh_vertex>& vs, std::vector<uint32_t>& is, V3 mn, V3 mx) {
+    V3 a{mn.x, mn.y, mn.z}, b{mx.x, mn.y, mn.z}, c{mx.x, mn.y, mx.z}, d{mn.x, mn.y, mx.z};
+    V3 e{mn.x, mx.y, mn.z}, f{mx.x, mx.y, mn.z}, g{mx.x, mx.y, mx.z}, h{mn.x, mx.y, mx.z};
+    add_quad_facing(vs, is, d, c, b, a, v3(0, -1, 0));
+    add_quad_facing(vs, is, e, f, g, h, v3(0, 1, 0));
+    add_quad_facing(vs, is, d, h, g, c, v3(0, 0, 1));
+    add_quad_facing(vs, is, b, f, e, a, v3(0, 0, -1));
+    add_quad_facing(vs, is, a, e, h, d, v3(-1, 0, 0));
+    add_quad_facing(vs, is, c, g, f, b, v3(1, 0, 0));
 }
 
 // Scene.cpp:118-141
@@ -157,7 +181,7 @@ void cornell_random(SceneData& s, uint32_t seed) {
     };
     std::vector<rrh_vertex> v; std::vector<uint32_t> i;
     auto quad_mesh = [&](V3 a, V3 b, V3 c, V3 d, V3 n, uint32_t m) {
-        v.clear(); i.clear(); add_quad(v, i, a, b, c, d, n); add_mesh(s, v, i, m);
+        v.clear(); i.clear(); add_quad_facing(v, i, a, b, c, d, n); add_mesh(s, v, i, m);
     };
     quad_mesh(v3(-1, -1, -1), v3(1, -1, -1), v3(1, 1, -1), v3(-1, 1, -1), v3(0, 0, 1), lam());
     quad_mesh(v3(-1, -1, 1), v3(1, -1, 1), v3(1, -1, -1), v3(-1, -1, -1), v3(0, 1, 0), lam());
@@ -174,7 +198,7 @@ void cornell_random(SceneData& s, uint32_t seed) {
         float x0 = std::fmax(-0.98f, cx - 0.5f * sx), x1 = std::fmin(0.98f, cx + 0.5f * sx);
         float z0 = std::fmax(-0.98f, cz - 0.5f * sz), z1 = std::fmin(0.98f, cz + 0.5f * sz);
         v.clear(); i.clear();
-        add_box(v, i, v3(x0, -1.0f, z0), v3(x1, -1.0f + h, z1));
+        add_box_facing(v, i, v3(x0, -1.0f, z0), v3(x1, -1.0f + h, z1));
         add_mesh(s, v, i, lam());
     }
     s.flags = RRH_SCENE_TRI_LIGHTS;
@@ -346,6 +370,7 @@ void indoor(SceneData& s, uint32_t seed, float scale) {
     // plane grid: origin o, axes u (nu cells) and v (nv cells), normal n; skip cells inside the window
     auto plane = [&](V3 o, V3 u, V3 v, int nu, int nv, V3 n, uint32_t m, bool window) {
         std::vector<rrh_vertex> vs; std::vector<uint32_t> is;
+        const bool flip = (u.y * v.z - u.z * v.y) * n.x + (u.z * v.x - u.x * v.z) * n.y + (u.x * v.y - u.y * v.x) * n.z < 0.0f;
         for (int j = 0; j <= nv; ++j)
             for (int i = 0; i <= nu; ++i) {
                 float a = (float)i / nu, b = (float)j / nv;
@@ -360,6 +385,7 @@ void indoor(SceneData& s, uint32_t seed, float scale) {
                     if (zc > wz0 && zc < wz1 && yc > wy0 && yc < wy1) continue;
                 }
                 uint32_t a = j * (nu + 1) + i, b = a + 1, c = a + (nu + 1), d = c + 1;
+                if (flip) std::swap(b, c);  // triangles face the plane's normal
                 uint32_t t[6] = {a, b, d, a, d, c};
                 is.insert(is.end(), t, t + 6);
             }
@@ -401,7 +427,7 @@ void indoor(SceneData& s, uint32_t seed, float scale) {
         std::vector<rrh_vertex>& v = gv[m];
         std::vector<uint32_t>& i = gi[m];
         std::vector<rrh_vertex> bv; std::vector<uint32_t> bi;
-        add_box(bv, bi, v3(x, y, z), v3(x + sx, y + sy, z + sz));
+        add_box_facing(bv, bi, v3(x, y, z), v3(x + sx, y + sy, z + sz));
         uint32_t base = (uint32_t)v.size();
         v.insert(v.end(), bv.begin(), bv.end());
         for (uint32_t q : bi) i.push_back(base + q);

@@ -402,3 +402,79 @@ def test_em_recovers_two_lobe_mixture_and_likelihood_increases():
     big, small = lobes[order[0]], lobes[order[1]]
     assert abs(big[4] - 0.75) < 0.02 and big[:3] @ m1 > 0.99 and small[:3] @ m2 > 0.99
     assert abs(big[3] - 20) < 3 and abs(small[3] - 20) < 4
+
+
+# ---- G-buffer (GBufferPass by pixel-centre rays) ------------------------------------------
+def _pixel_centre_dirs(cam):
+    """Pixel-centre directions from the camera matrices (numpy, float64)."""
+    W, H = cam.screen_width, cam.screen_height
+    ip = np.asarray(cam.inv_proj, np.float64).reshape(4, 4)
+    iv = np.asarray(cam.inv_view, np.float64).reshape(4, 4)
+    px, py = np.meshgrid(np.arange(W) + 0.5, np.arange(H) + 0.5)
+    ndc = np.stack([px.ravel() / W * 2 - 1, py.ravel() / H * 2 - 1, np.ones(W * H), np.ones(W * H)], 1)
+    v = ndc @ ip.T
+    w = np.stack([v[:, 0], v[:, 1], -np.ones(W * H), np.zeros(W * H)], 1) @ iv.T
+    return w[:, :3] / np.linalg.norm(w[:, :3], axis=1, keepdims=True)
+
+
+@pytest.mark.parametrize("kind,eye", [(L.GEN_CORNELL_REF, None), (L.GEN_CORNELL_REF, (0.0, 0.0, -4.0)),
+                                      (L.GEN_CORNELL_RANDOM, None)])
+def test_gbuffer_is_the_nearest_front_face_on_each_pixel_ray(kind, eye):
+    """Every G-buffer surface lies on its pixel-centre ray, is the two-sided closest hit when that
+    hit faces the ray and lies beyond it when it does not (back faces culled); material ids are
+    the hit triangle's mesh material; normals are the flat walls' unit normals, w = material;
+    misses keep the clear values.  The reference Cornell box's side walls and box tops are wound
+    away from their normals (Scene.cpp:94-141 keeps the corner order), so they are culled from
+    inside; from behind the back wall (eye at z = -4) that wall is culled too; the generated
+    Cornell box faces every surface along its normal, so nothing in view is culled."""
+    sa = core.generate_scene(kind, 3, 1.0)
+    sc = O.Scene.from_arrays(sa)
+    W, H = 40, 30
+    cam = core.camera_look_at(eye or sa.eye, sa.center, sa.fov, W, H)
+    oc = ocam(cam)
+    pos, nrm, mat = sc.gbuffer(oc)
+    d = _pixel_centre_dirs(oc)
+    e = np.asarray(oc.position, np.float64)
+    rays = np.zeros((W * H, 8), np.float32)
+    rays[:, :3], rays[:, 3], rays[:, 4:7], rays[:, 7] = e, 0.0, d, 1e9
+    two = sc.trace(rays)
+    tid = prim_ids(two)
+    tri = sc.triangles().reshape(-1, 3, 3).astype(np.float64)
+    gn = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    # material of each flattened triangle: instances in order, each mesh's triangles in order
+    tri_mat = np.concatenate([np.full(sa.meshes["index_count"][m] // 3, sa.meshes["material_index"][m])
+                              for m in sa.insts["mesh_index"]])
+    hit = mat != 0xFFFFFFFF
+    assert hit.mean() > 0.15
+    assert (pos[~hit] == 0).all() and (nrm[~hit] == 0).all()
+    to = pos[hit, :3] - e
+    dist = np.linalg.norm(to, axis=1)
+    assert np.allclose(np.cross(to / dist[:, None], d[hit]), 0, atol=1e-5)  # on the pixel ray
+    assert (pos[hit, 3] == 1).all() and (nrm[hit, 3].view(np.uint32) == mat[hit]).all()
+    assert np.allclose(np.linalg.norm(nrm[hit, :3], axis=1), 1, atol=1e-6)
+    any_hit = tid != 0xFFFFFFFF
+    facing = np.zeros(W * H, bool)
+    facing[any_hit] = (gn[tid[any_hit]] * d[any_hit]).sum(1) < 0
+    # the numpy rays differ from the G-buffer's fp32 ones in the last bits: silhouette pixels
+    # may disagree on hit / miss
+    assert (~hit[any_hit & facing]).sum() <= 0.01 * W * H and hit[~any_hit].sum() <= 0.01 * W * H
+    culled = any_hit & ~facing  # the two-sided closest hit faces away
+    two_hit = hit & any_hit
+    front, back = two_hit & facing, two_hit & ~facing
+    full_dist = np.zeros(W * H)
+    full_dist[hit] = dist
+    same = front & np.isclose(full_dist, two[:, 0], rtol=1e-5)
+    assert same.sum() >= 0.98 * front.sum()  # the rest: silhouette pixels of the two ray sets
+    if same.any():
+        assert (mat[same] == tri_mat[tid[same]]).mean() > 0.98
+        n_geo = gn[tid[same]] / np.linalg.norm(gn[tid[same]], axis=1, keepdims=True)
+        assert (np.abs(np.abs((nrm[same, :3] * n_geo).sum(1)) - 1) < 1e-5).mean() > 0.98
+    # a back-facing two-sided hit: the G-buffer surface lies at or beyond it (at: the ray meets
+    # an edge shared with a front face, or a coincident front face)
+    assert (dist[back[hit]] >= two[back, 0] * (1 - 1e-5)).all()
+    if back.any():
+        assert (dist[back[hit]] > two[back, 0] * (1 + 1e-4)).mean() > 0.5
+    if kind == L.GEN_CORNELL_REF:
+        assert culled.mean() > 0.3  # culled walls: the box's far side (or nothing) shows through
+    else:
+        assert not culled.any() and front.mean() > 0.8
