@@ -303,6 +303,39 @@ int rrh_rpt_clear(rrh_ctx* ctx, rrh_image* const res[4], rrh_image* hist_pos, rr
  * samples plus the temporal and spatial visibility rays */
 int rrh_rpt_ray_counts(rrh_ctx* ctx, rrh_ray_counts* out);
 
+/* ---- ReSTIR GI, one-bounce reconnection reuse: ReSTIRGIPass::execute (src/passes/restir_gi/
+ *      ReSTIRGIPass.cpp:428-651) + restir_gi.slang cs_initial :566-684, cs_temporal :686-847,
+ *      cs_spatial_shade :849-1013 ---------------------------------------------------------
+ * G-buffer and history as ReSTIR PT's.  A reservoir is three RGBA32F planes
+ * (restir_gi.slang:204-242): res[0] (sample position, w_sum), res[1] (sample radiance, W),
+ * res[2] (sample normal, asfloat(M)); M = 0 is empty.  Cosine candidates, no guiding; every
+ * ray culls back faces. */
+typedef struct {
+    uint32_t frame_seed;               /* frame_count_++ (ReSTIRGIPass.cpp:460) */
+    uint32_t num_initial_candidates;   /* ReSTIRGIPass.h:72 default 8 */
+    uint32_t temporal_enable;          /* 1 */
+    uint32_t spatial_enable;           /* 1 */
+    uint32_t spatial_num_neighbors;    /* 4 */
+    uint32_t spatial_radius;           /* pixels, 12 */
+    uint32_t include_direct_lighting;  /* 1 */
+    float    indirect_strength;        /* 1.0 */
+    float    ray_bias;                 /* 1e-3 */
+    uint32_t _pad[3];
+} rrh_rgi_params;
+int rrh_rgi_initial(rrh_ctx* ctx, rrh_scene* s, const rrh_rgi_params* p, const rrh_image* gbuf_pos,
+                    const rrh_image* gbuf_norm, rrh_image* const res[3], rrh_image* hist_pos, rrh_image* hist_norm);
+int rrh_rgi_temporal(rrh_ctx* ctx, rrh_scene* s, const rrh_rgi_params* p, const rrh_image* gbuf_pos,
+                     const rrh_image* gbuf_norm, rrh_image* const res[3], rrh_image* const res_prev[3],
+                     const rrh_image* hist_prev_pos, const rrh_image* hist_prev_norm);
+int rrh_rgi_spatial_shade(rrh_ctx* ctx, rrh_scene* s, const rrh_rgi_params* p, const rrh_image* gbuf_pos,
+                          const rrh_image* gbuf_norm, const rrh_image* direct, rrh_image* const res[3],
+                          rrh_image* out);
+/* history reset (ReSTIRGIPass.cpp:380-400, 505-530): planes zero, history position (0, 0, 0, -1) */
+int rrh_rgi_clear(rrh_ctx* ctx, rrh_image* const res[3], rrh_image* hist_pos, rrh_image* hist_norm);
+/* rays of the last frame: extension = candidate segments, shadow = their light samples plus
+ * the temporal and spatial visibility rays */
+int rrh_rgi_ray_counts(rrh_ctx* ctx, rrh_ray_counts* out);
+
 /* ---- A-trous: AtrousPass::execute (src/passes/denoise/AtrousPass.cpp:243-306)
  *      + atrous.slang cs_main :41-106 ------------------------------------- */
 #define RRH_ATROUS_REFERENCE 0  /* RGBA32F radiance, RGBA32F position/normal G-buffer, Gaussian weights */

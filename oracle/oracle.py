@@ -57,6 +57,13 @@ class RPTParams(C.Structure):
                 ("guiding_enable", C.c_uint32), ("guiding_alpha", C.c_float)]
 
 
+class RGIParams(C.Structure):
+    _fields_ = [("frame_seed", C.c_uint32), ("num_initial_candidates", C.c_uint32), ("temporal_enable", C.c_uint32),
+                ("spatial_enable", C.c_uint32), ("spatial_num_neighbors", C.c_uint32),
+                ("spatial_radius", C.c_uint32), ("include_direct_lighting", C.c_uint32),
+                ("indirect_strength", C.c_float), ("ray_bias", C.c_float), ("_pad", C.c_uint32 * 3)]
+
+
 class AtrousParams(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("stride", C.c_uint32), ("mode", C.c_uint32),
                 ("sigma_position", C.c_float), ("sigma_normal", C.c_float), ("sigma_luminance", C.c_float),
@@ -100,6 +107,9 @@ _SIGS = {
     "orc_rpt_temporal": (None, [P, C.POINTER(Camera), C.POINTER(RPTParams), P, P, P, P, P, P, C.c_int]),
     "orc_rpt_spatial_shade": (None, [P, C.POINTER(Camera), C.POINTER(RPTParams), P, F, F, U, P, P, P, P, P,
                                      C.c_int]),
+    "orc_rgi_initial": (None, [P, C.POINTER(Camera), C.POINTER(RGIParams), P, P, P, P, P, C.c_int]),
+    "orc_rgi_temporal": (None, [P, C.POINTER(Camera), C.POINTER(RGIParams), P, P, P, P, P, P, C.c_int]),
+    "orc_rgi_spatial_shade": (None, [P, C.POINTER(Camera), C.POINTER(RGIParams), P, P, P, P, P, C.c_int]),
     "orc_atrous_ref": (None, [C.POINTER(AtrousParams), P, P, P, P]),
     "orc_atrous_cfg": (None, [C.POINTER(AtrousParams), P, P, P]),
     "orc_f32_to_f16": (C.c_uint16, [F]),
@@ -253,6 +263,27 @@ class Scene:
                                     slope, count, p(gp), p(gn), None if direct is None else p(direct),
                                     p(np.ascontiguousarray(res, np.float32)), p(out), threads)
         return out, accum
+
+    # ReSTIR GI (restir_gi.slang); res: (3, W*H, 4) float32 planes
+    def rgi_initial(self, cam: Camera, prm: RGIParams, gp, gn, threads: int = 0):
+        n = cam.screen_width * cam.screen_height
+        res = np.zeros((3, n, 4), np.float32)
+        hp, hn = np.zeros((n, 4), np.float32), np.zeros((n, 4), np.float32)
+        lib().orc_rgi_initial(self.h, C.byref(cam), C.byref(prm), p(gp), p(gn), p(res), p(hp), p(hn), threads)
+        return res, hp, hn
+
+    def rgi_temporal(self, cam, prm, gp, gn, res, res_prev, hp_prev, hn_prev, threads: int = 0):
+        res = np.ascontiguousarray(res, np.float32).copy()
+        lib().orc_rgi_temporal(self.h, C.byref(cam), C.byref(prm), p(gp), p(gn), p(res),
+                               p(np.ascontiguousarray(res_prev, np.float32)), p(hp_prev), p(hn_prev), threads)
+        return res
+
+    def rgi_spatial_shade(self, cam, prm, gp, gn, res, direct=None, threads: int = 0):
+        out = np.zeros((cam.screen_width * cam.screen_height, 4), np.float32)
+        lib().orc_rgi_spatial_shade(self.h, C.byref(cam), C.byref(prm), p(gp), p(gn),
+                                    None if direct is None else p(direct), p(np.ascontiguousarray(res, np.float32)),
+                                    p(out), threads)
+        return out
 
     def __del__(self):
         try:

@@ -1798,6 +1798,36 @@ static int rpt_reproject(const orc_camera* cam, v3 p, uint32_t W, uint32_t H, in
     return *hx >= 0 && *hy >= 0 && *hx < (int)W && *hy < (int)H;
 }
 
+/* find_reprojected_history_surface, restir_pt.slang:355-401 (restir_gi.slang:358-433) */
+static int rpt_find_history(const orc_camera* cam, const float* hpos_prev, const float* hnrm_prev, uint32_t W,
+                            uint32_t H, v3 pp, v3 pn, uint32_t pmi, v3 cp, size_t* hj_out, v3* hp_out) {
+    int cx, cy;
+    if (!rpt_reproject(cam, pp, W, H, &cx, &cy)) return 0;
+    int found = 0;
+    float best = 1e30f;
+    size_t hj = 0;
+    v3 hp = V(0, 0, 0);
+    float vdist = fmaxf(vlength(vsub(pp, cp)), 1.0f);
+    for (int oy = -1; oy <= 1; ++oy)
+        for (int ox = -1; ox <= 1; ++ox) {
+            int qx = cx + ox, qy = cy + oy;
+            if (qx < 0 || qy < 0 || qx >= (int)W || qy >= (int)H) continue;
+            size_t j = (size_t)qy * W + (size_t)qx;
+            v3 q, qn; uint32_t qm;
+            if (!rpt_surface(hpos_prev, hnrm_prev, j, &q, &qn, &qm)) continue;
+            /* is_reprojected_history_compatible, restir_pt.slang:340-353 */
+            if (qm != pmi || vdot(pn, qn) < 0.9f) continue;
+            float plane = fabsf(vdot(vsub(q, pp), pn));
+            float dpos = vlength(vsub(q, pp));
+            if (!(plane <= 0.01f * vdist && dpos <= 0.05f * vdist)) continue;
+            float err = plane * 4.0f + dpos;
+            if (err < best) { best = err; hj = j; hp = q; found = 1; }
+        }
+    *hj_out = hj;
+    *hp_out = hp;
+    return found;
+}
+
 void orc_rpt_temporal(const orc_scene* s, const orc_camera* cam, const orc_rpt_params* p, const float* gpos,
                       const float* gnrm, float* res, const float* res_prev, const float* hpos_prev,
                       const float* hnrm_prev, int threads) {
@@ -1813,30 +1843,10 @@ void orc_rpt_temporal(const orc_scene* s, const orc_camera* cam, const orc_rpt_p
         const size_t i = (size_t)ii;
         const uint32_t px = (uint32_t)(i % W), py = (uint32_t)(i / W);
         v3 pp, pn; uint32_t pmi;
-        int cx, cy;
-        if (!rpt_surface(gpos, gnrm, i, &pp, &pn, &pmi) || !rpt_reproject(cam, pp, W, H, &cx, &cy)) continue;
-        /* find_reprojected_history_surface, restir_pt.slang:355-401 */
-        int found = 0;
-        float best = 1e30f;
-        size_t hj = 0;
-        v3 hp = V(0, 0, 0);
-        float vdist = fmaxf(vlength(vsub(pp, cp)), 1.0f);
-        for (int oy = -1; oy <= 1; ++oy)
-            for (int ox = -1; ox <= 1; ++ox) {
-                int qx = cx + ox, qy = cy + oy;
-                if (qx < 0 || qy < 0 || qx >= (int)W || qy >= (int)H) continue;
-                size_t j = (size_t)qy * W + (size_t)qx;
-                v3 q, qn; uint32_t qm;
-                if (!rpt_surface(hpos_prev, hnrm_prev, j, &q, &qn, &qm)) continue;
-                /* is_reprojected_history_compatible, restir_pt.slang:340-353 */
-                if (qm != pmi || vdot(pn, qn) < 0.9f) continue;
-                float plane = fabsf(vdot(vsub(q, pp), pn));
-                float dpos = vlength(vsub(q, pp));
-                if (!(plane <= 0.01f * vdist && dpos <= 0.05f * vdist)) continue;
-                float err = plane * 4.0f + dpos;
-                if (err < best) { best = err; hj = j; hp = q; found = 1; }
-            }
-        if (!found) continue;
+        size_t hj;
+        v3 hp;
+        if (!rpt_surface(gpos, gnrm, i, &pp, &pn, &pmi) ||
+            !rpt_find_history(cam, hpos_prev, hnrm_prev, W, H, pp, pn, pmi, cp, &hj, &hp)) continue;
         rpt_res cur = rpt_load(res, npix, i);
         rpt_res prv = rpt_load(res_prev, npix, hj);
         if (prv.M == 0 || prv.valid < 0.5f) continue;
@@ -1957,6 +1967,246 @@ void orc_rpt_spatial_shade(const orc_scene* s, const orc_camera* cam, const orc_
                             a[3] += 1u;
                         }
                     }
+                }
+            }
+        }
+        float* o = out + i * 4;
+        o[0] = L.x; o[1] = L.y; o[2] = L.z; o[3] = 1.0f;
+    }
+}
+
+/* ========================================================================== */
+/* ReSTIR GI, one-bounce reconnection reuse -- restir_gi.slang:143-1013.  The
+ * helpers are ReSTIR PT's (the shaders define them identically); here the target is
+ * brdf * radiance * cos, the source pdf cos / pi toward the hit, the kept normal the
+ * interpolated one, and M = 0 marks an empty reservoir (three planes).            */
+/* ========================================================================== */
+typedef struct { v3 pos, rad, n; float w_sum, W; uint32_t M; } rgi_res;
+
+static rgi_res rgi_load(const float* res, size_t npix, size_t i) {
+    const float* p = res + i * 4; const float* r = res + (npix + i) * 4; const float* m = res + (2 * npix + i) * 4;
+    rgi_res s;
+    s.pos = vld(p); s.w_sum = p[3];
+    s.rad = vld(r); s.W = r[3];
+    s.n = vld(m); s.M = asuint(m[3]);
+    return s;
+}
+static void rgi_store(float* res, size_t npix, size_t i, const rgi_res* s) {
+    float* p = res + i * 4; float* r = res + (npix + i) * 4; float* m = res + (2 * npix + i) * 4;
+    vst(p, s->pos); p[3] = s->w_sum;
+    vst(r, s->rad); r[3] = s->W;
+    vst(m, s->n); m[3] = asfloat(s->M);
+}
+static void rgi_store_empty(float* res, size_t npix, size_t i) {
+    for (int k = 0; k < 3; ++k) memset(res + (k * npix + i) * 4, 0, 16);
+}
+/* evaluate_indirect_radiance / compute_target_function, restir_gi.slang:487-537 */
+static v3 rgi_indirect(v3 pp, v3 pn, const matp* pm, v3 sp, v3 srad, v3 cam) {
+    v3 to = vsub(sp, pp);
+    float d2 = vdot(to, to);
+    if (d2 <= 1e-8f) return V(0, 0, 0);
+    v3 wi = vscale(to, 1.0f / sqrtf(d2));
+    float cx = fmaxf(vdot(pn, wi), 0.0f);
+    if (cx <= 0.0f) return V(0, 0, 0);
+    v3 wo = vnorm(vsub(cam, pp));
+    return vscale(vmul(mat_eval(pm, pn, wo, wi), srad), cx);
+}
+static float rgi_target(v3 pp, v3 pn, const matp* pm, v3 sp, v3 srad, v3 cam) {
+    return lum709(rgi_indirect(pp, pn, pm, sp, srad, cam));
+}
+/* sample_primary_pdf, restir_gi.slang:510-522 */
+static float rgi_primary_pdf(v3 pp, v3 pn, v3 sp) {
+    v3 d = vsub(sp, pp);
+    float d2 = vdot(d, d);
+    if (d2 <= 1e-8f) return 0.0f;
+    v3 wi = vscale(d, 1.0f / sqrtf(d2));
+    float cx = fmaxf(vdot(pn, wi), 0.0f);
+    return fmaxf(cx * K_INV_PI, 0.0f);
+}
+
+void orc_rgi_initial(const orc_scene* s, const orc_camera* cam, const orc_rgi_params* p, const float* gpos,
+                     const float* gnrm, float* res, float* hpos, float* hnrm, int threads) {
+    const uint32_t W = cam->screen_width, H = cam->screen_height;
+    const size_t npix = (size_t)W * H;
+    const v3 cp = vld(cam->position);
+    const int hit_emission = s->ntl == 0;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t ii = 0; ii < (int64_t)npix; ++ii) {
+        const size_t i = (size_t)ii;
+        const uint32_t px = (uint32_t)(i % W), py = (uint32_t)(i / W);
+        v3 pp, pn; uint32_t pmi;
+        if (!rpt_surface(gpos, gnrm, i, &pp, &pn, &pmi)) {
+            float* h = hpos + i * 4; h[0] = h[1] = h[2] = 0.0f; h[3] = -1.0f;
+            memset(hnrm + i * 4, 0, 16);
+            rgi_store_empty(res, npix, i);
+            continue;
+        }
+        memcpy(hpos + i * 4, gpos + i * 4, 16);
+        memcpy(hnrm + i * 4, gnrm + i * 4, 16);
+        const matp pm = load_mat(s, pmi);
+        const float bias = p->ray_bias;
+        uint32_t seed = orc_make_seed(px, py, p->frame_seed);
+        rgi_res sel; memset(&sel, 0, sizeof(sel));
+        float w_sum = 0.0f;
+        uint32_t Mc = 0;
+        for (uint32_t c = 0; c < p->num_initial_candidates; ++c) {
+            float u0 = orc_rand_float(&seed), u1 = orc_rand_float(&seed), l[3], pdf_dir;
+            orc_sample_cosine_hemisphere(u0, u1, l, &pdf_dir);
+            v3 T, B; build_onb(pn, &T, &B);
+            v3 dir = to_world(vld(l), T, B, pn);
+            if (vdot(pn, dir) <= 0.0f) continue;
+            orc_hit h = trace_bvh_cull(s, vadd(pp, vscale(pn, bias)), dir, bias, 1e9f, 1);
+            if (h.prim == ~0u) continue;
+            hitinfo hi = fetch_hit(s, h);
+            /* evaluate_sample_radiance, restir_gi.slang:539-551 */
+            v3 wo = vneg(dir);
+            v3 n = hi.normal;
+            if (vdot(n, wo) < 0.0f) n = vneg(n);
+            const matp m = load_mat(s, hi.mat);
+            v3 srad = hit_emission ? m.emissive : V(0, 0, 0);
+            srad = vadd(srad, rpt_lights(s, hi.pos, n, wo, &m, bias, &seed));
+            float tgt = rgi_target(pp, pn, &pm, hi.pos, srad, cp);
+            if (tgt <= 0.0f) continue;
+            float pdf = rgi_primary_pdf(pp, pn, hi.pos);
+            if (pdf <= 1e-6f) continue;
+            float w = tgt / pdf;
+            w_sum += w;
+            Mc++;
+            if (w_sum > 0.0f && orc_rand_float(&seed) < w / w_sum) {
+                sel.pos = hi.pos; sel.rad = srad; sel.n = hi.normal;
+            }
+        }
+        int stored = 0;
+        if (Mc > 0) {
+            float ph = rgi_target(pp, pn, &pm, sel.pos, sel.rad, cp);
+            if (ph > 1e-10f) {
+                sel.w_sum = w_sum;
+                sel.W = w_sum / ((float)Mc * ph);
+                sel.M = Mc;
+                rgi_store(res, npix, i, &sel);
+                stored = 1;
+            }
+        }
+        if (!stored) rgi_store_empty(res, npix, i);
+    }
+}
+
+void orc_rgi_temporal(const orc_scene* s, const orc_camera* cam, const orc_rgi_params* p, const float* gpos,
+                      const float* gnrm, float* res, const float* res_prev, const float* hpos_prev,
+                      const float* hnrm_prev, int threads) {
+    if (!p->temporal_enable) return;
+    const uint32_t W = cam->screen_width, H = cam->screen_height;
+    const size_t npix = (size_t)W * H;
+    const v3 cp = vld(cam->position);
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t ii = 0; ii < (int64_t)npix; ++ii) {
+        const size_t i = (size_t)ii;
+        const uint32_t px = (uint32_t)(i % W), py = (uint32_t)(i / W);
+        v3 pp, pn; uint32_t pmi;
+        size_t hj;
+        v3 hp;
+        if (!rpt_surface(gpos, gnrm, i, &pp, &pn, &pmi) ||
+            !rpt_find_history(cam, hpos_prev, hnrm_prev, W, H, pp, pn, pmi, cp, &hj, &hp)) continue;
+        rgi_res cur = rgi_load(res, npix, i);
+        rgi_res prv = rgi_load(res_prev, npix, hj);
+        if (prv.M == 0) continue;
+        uint32_t cap = 20u * p->num_initial_candidates;
+        cap = cap < 1u ? 1u : (cap > 8192u ? 8192u : cap);
+        prv.M = prv.M < cap ? prv.M : cap;
+        float jac = rpt_jacobian(hp, pp, prv.pos, prv.n);
+        if (jac <= 0.0f) continue;
+        if (!rpt_visible(s, pp, pn, prv.pos, p->ray_bias)) continue;
+        const matp pm = load_mat(s, pmi);
+        float ph = rgi_target(pp, pn, &pm, prv.pos, prv.rad, cp);
+        float contrib = ph * fminf(prv.W, 50.0f) * (float)prv.M * jac;
+        if (contrib <= 0.0f) continue;
+        cur.w_sum += contrib;
+        uint32_t seed = orc_make_seed(px, py, p->frame_seed + 1u);
+        if (cur.w_sum > 0.0f && orc_rand_float(&seed) < contrib / cur.w_sum) {
+            cur.pos = prv.pos; cur.rad = prv.rad; cur.n = prv.n;
+        }
+        uint32_t nM = cur.M + prv.M;
+        if (nM > cap) { cur.w_sum *= (float)cap / (float)nM; cur.M = cap; }
+        else cur.M = nM;
+        float Wn = 0.0f;
+        if (cur.M > 0) {
+            float py_ = rgi_target(pp, pn, &pm, cur.pos, cur.rad, cp);
+            Wn = (py_ > 1e-10f) ? (cur.w_sum / ((float)cur.M * py_)) : 0.0f;
+            Wn = fminf(Wn, 50.0f);
+        }
+        cur.W = Wn;
+        rgi_store(res, npix, i, &cur);
+    }
+}
+
+void orc_rgi_spatial_shade(const orc_scene* s, const orc_camera* cam, const orc_rgi_params* p, const float* gpos,
+                           const float* gnrm, const float* direct, const float* res, float* out, int threads) {
+    const uint32_t W = cam->screen_width, H = cam->screen_height;
+    const size_t npix = (size_t)W * H;
+    const v3 cp = vld(cam->position);
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t ii = 0; ii < (int64_t)npix; ++ii) {
+        const size_t i = (size_t)ii;
+        const uint32_t px = (uint32_t)(i % W), py = (uint32_t)(i / W);
+        v3 L = V(0, 0, 0);
+        if (p->include_direct_lighting != 0 && direct) L = vld(direct + i * 4);
+        v3 pp, pn; uint32_t pmi;
+        if (!rpt_surface(gpos, gnrm, i, &pp, &pn, &pmi)) {
+            if (p->include_direct_lighting == 0) L = vadd(L, sky(center_ray_dir(cam, px, py)));
+        } else {
+            const matp pm = load_mat(s, pmi);
+            if (pm.emissive.x > 0.0f || pm.emissive.y > 0.0f || pm.emissive.z > 0.0f) {
+                L = vadd(L, pm.emissive);
+            } else {
+                rgi_res r = rgi_load(res, npix, i);
+                if (p->spatial_enable && p->spatial_num_neighbors > 0) {
+                    float base_phase = fracf_((float)orc_pcg_hash(px ^ (py * 65537u)) * 2.3283064365386963e-10f);
+                    float dist_scale = fmaxf(vlength(vsub(pp, cp)), 1.0f);
+                    for (uint32_t k = 0; k < p->spatial_num_neighbors; ++k) {
+                        float u = ((float)k + 0.5f) / (float)p->spatial_num_neighbors;
+                        float angle = fracf_(base_phase + 0.61803398875f * (float)k) * K_TWO_PI;
+                        float radius = sqrtf(u) * (float)p->spatial_radius;
+                        float sa, ca; orc_sincosf(angle, &sa, &ca);
+                        int nx = (int)px + (int)(radius * ca), ny = (int)py + (int)(radius * sa);
+                        nx = nx < 0 ? 0 : (nx > (int)W - 1 ? (int)W - 1 : nx);
+                        ny = ny < 0 ? 0 : (ny > (int)H - 1 ? (int)H - 1 : ny);
+                        if (nx == (int)px && ny == (int)py) continue;
+                        size_t j = (size_t)ny * W + (size_t)nx;
+                        /* is_spatial_neighbor_compatible, restir_gi.slang:435-459 */
+                        const float* q4 = gpos + j * 4;
+                        if (q4[3] < 0.0f) continue;
+                        v3 q = vld(q4), qn = vnorm(vld(gnrm + j * 4));
+                        if (vdot(qn, pn) < 0.85f) continue;
+                        if (!(fabsf(vdot(vsub(q, pp), pn)) <= 0.02f * dist_scale)) continue;
+                        rgi_res nb = rgi_load(res, npix, j);
+                        if (nb.M == 0) continue;
+                        float jac = rpt_jacobian(q, pp, nb.pos, nb.n);
+                        if (jac <= 0.0f) continue;
+                        float contrib = rgi_target(pp, pn, &pm, nb.pos, nb.rad, cp) * nb.W * (float)nb.M * jac;
+                        if (contrib <= 0.0f) continue;
+                        r.w_sum += contrib;
+                        r.M += nb.M;
+                        float accept = fracf_(base_phase + 0.754877666f * (float)(k + 1u));
+                        if (r.w_sum > 0.0f && accept < contrib / r.w_sum) { r.pos = nb.pos; r.rad = nb.rad; r.n = nb.n; }
+                    }
+                    if (r.M > 0) {
+                        float ph = rgi_target(pp, pn, &pm, r.pos, r.rad, cp);
+                        r.W = (ph > 1e-10f) ? (r.w_sum / ((float)r.M * ph)) : 0.0f;
+                    }
+                }
+                if (r.M > 0 && r.W > 0.0f) {
+                    v3 ind = vscale(rgi_indirect(pp, pn, &pm, r.pos, r.rad, cp), r.W);
+                    if (lum709(ind) > 0.0f && rpt_visible(s, pp, pn, r.pos, p->ray_bias))
+                        L = vadd(L, vscale(ind, p->indirect_strength));
                 }
             }
         }

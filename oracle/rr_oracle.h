@@ -160,6 +160,23 @@ void orc_rpt_spatial_shade(const orc_scene* s, const orc_camera* cam, const orc_
                            float base_cell, float slope, uint32_t cell_count, const float* gbuf_pos,
                            const float* gbuf_norm, const float* direct, const float* res, float* out, int threads);
 
+/* ---- ReSTIR GI: restir_gi.slang:143-1013 --------------------------------------- */
+typedef struct {
+    uint32_t frame_seed, num_initial_candidates, temporal_enable, spatial_enable;
+    uint32_t spatial_num_neighbors, spatial_radius, include_direct_lighting;
+    float indirect_strength, ray_bias;
+    uint32_t _pad[3];
+} orc_rgi_params;
+/* res: three planes of W*H float4 back to back (sample position + w_sum, radiance + W,
+ * normal + M) */
+void orc_rgi_initial(const orc_scene* s, const orc_camera* cam, const orc_rgi_params* p, const float* gbuf_pos,
+                     const float* gbuf_norm, float* res, float* hist_pos, float* hist_norm, int threads);
+void orc_rgi_temporal(const orc_scene* s, const orc_camera* cam, const orc_rgi_params* p, const float* gbuf_pos,
+                      const float* gbuf_norm, float* res, const float* res_prev, const float* hist_prev_pos,
+                      const float* hist_prev_norm, int threads);
+void orc_rgi_spatial_shade(const orc_scene* s, const orc_camera* cam, const orc_rgi_params* p, const float* gbuf_pos,
+                           const float* gbuf_norm, const float* direct, const float* res, float* out, int threads);
+
 /* ---- A-trous: atrous.slang:21-106 (reference mode) and SVGF-style config mode ---- */
 typedef struct {
     uint32_t width, height, stride, mode;       /* mode 0 reference, 1 config (fp16 I/O) */

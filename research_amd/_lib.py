@@ -93,6 +93,13 @@ class RPTParams(C.Structure):
                 ("guiding_enable", C.c_uint32), ("guiding_alpha", C.c_float)]
 
 
+class RGIParams(C.Structure):
+    _fields_ = [("frame_seed", C.c_uint32), ("num_initial_candidates", C.c_uint32), ("temporal_enable", C.c_uint32),
+                ("spatial_enable", C.c_uint32), ("spatial_num_neighbors", C.c_uint32),
+                ("spatial_radius", C.c_uint32), ("include_direct_lighting", C.c_uint32),
+                ("indirect_strength", C.c_float), ("ray_bias", C.c_float), ("_pad", C.c_uint32 * 3)]
+
+
 class AtrousParams(C.Structure):
     _fields_ = [("iterations", C.c_uint32), ("mode", C.c_uint32), ("sigma_position", C.c_float),
                 ("sigma_normal", C.c_float), ("sigma_luminance", C.c_float), ("_pad", C.c_uint32 * 3)]
@@ -169,6 +176,11 @@ SIGNATURES = {
     "rrh_rpt_spatial_shade": [P, P, C.POINTER(RPTParams), P, P, P, P, P, P],
     "rrh_rpt_clear": [P, P, P, P],
     "rrh_rpt_ray_counts": [P, C.POINTER(RayCounts)],
+    "rrh_rgi_initial": [P, P, C.POINTER(RGIParams), P, P, P, P, P],
+    "rrh_rgi_temporal": [P, P, C.POINTER(RGIParams), P, P, P, P, P, P],
+    "rrh_rgi_spatial_shade": [P, P, C.POINTER(RGIParams), P, P, P, P, P],
+    "rrh_rgi_clear": [P, P, P, P],
+    "rrh_rgi_ray_counts": [P, C.POINTER(RayCounts)],
     "rrh_atrous": [P, C.POINTER(AtrousParams), P, P, P, P, P, C.POINTER(i32)],
     "rrh_atrous_prepare": [P, P, P, P, P, P, P],
     "rrh_grid_create": [P, C.POINTER(GridParams), PP],

@@ -36,6 +36,7 @@ SOURCES = [
     "rrh_bvh_gpu.hip",
     "rrh_gbuffer.hip",
     "rrh_restir_pt.hip",
+    "rrh_restir_gi.hip",
     "rrh_scenegen.cpp",
 ]
 

@@ -214,11 +214,12 @@ class Image:
 
 
 def planes(images) -> C.Array:
-    """Reservoir planes as the C array of image handles the rrh_rpt_* entry points take."""
+    """Reservoir planes as the C array of image handles the rrh_rpt_* (four planes) and
+    rrh_rgi_* (three planes) entry points take."""
     images = list(images)
-    if len(images) != 4:
-        raise ValueError("a ReSTIR PT reservoir has four planes")
-    return (C.c_void_p * 4)(*[im.h.value for im in images])
+    if len(images) not in (3, 4):
+        raise ValueError("a ReSTIR PT reservoir has four planes, a ReSTIR GI reservoir three")
+    return (C.c_void_p * len(images))(*[im.h.value for im in images])
 
 
 def trace_closest(ctx: Context, scene: Scene, rays: np.ndarray) -> np.ndarray:

@@ -21,12 +21,12 @@
 //    and the reconnection vertex's own emission is left to the direct lighting pass, which
 //    samples the same lights; with point / directional lights only, as in the reference, the
 //    reconnection vertex's emission is part of the suffix.
-#include <algorithm>
-
-#include "rrh_internal.h"
 #include "rrh_guide_dev.h"
+#include "rrh_internal.h"
+#include "rrh_reuse.h"
 
 using namespace rrh;
+using namespace rrh::reuse;
 
 namespace {
 
@@ -64,70 +64,6 @@ RRH_D void store_empty(const Res4& R, size_t i) {
     R.pos[i] = z; R.nrm[i] = z; R.rad[i] = z; R.pre[i] = z;
 }
 
-// trace_ray with back faces culled as an occlusion query (the initial stage traces through
-// trace_closest4_s with CULL directly)
-template <class Stk>
-RRH_D bool occluded_cull(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& st) {
-    if (S.node_fmt == RRH_NODES_WIDE4) return trace_occluded4_s<false, true>(S, o, d, tmin, tmax, st);
-    return trace_occluded4_s<true, true>(S, o, d, tmin, tmax, st);
-}
-
-// sample_lights, restir_pt.slang:171-222 (+ triangle lights, see the header), up to its shadow
-// ray: c = the contribution if the light is visible, and the shadow ray (origin, direction,
-// tmax; tmin = bias).  false: no ray is needed and the contribution is zero.  The shadow test
-// only decides whether c counts, so c is formed before the ray is traced.
-RRH_D bool light_sample(const DevScene& S, f3 hit, f3 n, f3 wo, const Mat& m, float bias, uint32_t& seed, f3& c,
-                        f3& ro, f3& rd, float& tmax) {
-    c = mk(0.0f, 0.0f, 0.0f);
-    const uint32_t N = S.nl + S.ntl;
-    if (N == 0) return false;
-    const uint32_t li = pick_index(rnd(seed), N);
-    if (li < S.nl) {
-        const rrh_light& L = S.lights[li];
-        f3 ldir;
-        float dist, pl;
-        if (L.type == 0) {
-            const f3 to = sub(ld3(L.position), hit);
-            dist = length(to);
-            ldir = divs(to, fmaxf(dist, 1e-6f));
-            pl = dist * dist;
-        } else {
-            ldir = normalize(neg(ld3(L.direction)));
-            dist = 1e9f;
-            pl = 1.0f;
-        }
-        if (dot(ldir, n) <= 0.0f) return false;
-        const f3 f = mat_eval(m, n, wo, ldir);
-        const float ndl = fmaxf(dot(n, ldir), 0.0f);
-        const f3 irr = divs(scale(ld3(L.emission), L.intensity), (L.type == 0) ? (pl + 1e-6f) : 1.0f);
-        const float pb = mat_ref_pdf(m, wo, ldir, n);
-        const float mis = (L.type == 0) ? 1.0f : (pl * pl) / (pl * pl + pb * pb + 1e-10f);
-        c = scale(scale(scale(mul(f, irr), ndl), mis), (float)N);
-        ro = add(hit, scale(n, bias));
-        rd = ldir;
-        tmax = dist - 0.01f;
-        return true;
-    }
-    const float u1 = rnd(seed);
-    const float u2 = rnd(seed);
-    const TriSample ts = tri_light_point(S, li - S.nl, u1, u2);
-    const f3 to = sub(ts.p, hit);
-    const float d2 = dot(to, to);
-    const float dist = sqrtf(d2);
-    const f3 l = scale(to, 1.0f / dist);
-    if (dot(l, n) <= 0.0f) return false;
-    const float cosl = fabsf(dot(ts.nl, l));
-    if (!(cosl > 0.0f)) return false;
-    const f3 f = mat_eval(m, n, wo, l);
-    const float ndl = fmaxf(dot(n, l), 0.0f);
-    const float pl = d2 / (cosl * ts.area) / (float)N;
-    c = scale(scale(mul(f, ts.Le), ndl), 1.0f / pl);
-    ro = add(hit, scale(n, bias));
-    rd = l;
-    tmax = dist - 0.01f;
-    return true;
-}
-
 // evaluate_contribution / target_function, restir_pt.slang:464-489
 RRH_D f3 contribution(f3 pp, f3 pn, const Mat& pm, f3 cam, const Reservoir& r) {
     const f3 to = sub(r.rc_pos, pp);
@@ -142,56 +78,6 @@ RRH_D f3 contribution(f3 pp, f3 pn, const Mat& pm, f3 cam, const Reservoir& r) {
 }
 RRH_D float target(f3 pp, f3 pn, const Mat& pm, f3 cam, const Reservoir& r) {
     return lum709(contribution(pp, pn, pm, cam, r));
-}
-
-// reconnection_jacobian, restir_pt.slang:427-447
-RRH_D float jacobian(f3 src, f3 dst, f3 rc, f3 rn) {
-    const f3 sd = sub(rc, src), td = sub(rc, dst);
-    const float s2 = dot(sd, sd), t2 = dot(td, td);
-    if (s2 <= 1e-8f || t2 <= 1e-8f) return 0.0f;
-    const f3 sdir = scale(sd, 1.0f / sqrtf(s2)), tdir = scale(td, 1.0f / sqrtf(t2));
-    const float sc = fabsf(dot(rn, neg(sdir))), tc = fabsf(dot(rn, neg(tdir)));
-    if (sc <= 0.05f || tc <= 0.05f) return 0.0f;
-    return clampf((tc * s2) / (sc * t2), 0.0f, 3.0f);
-}
-
-// trace_visibility, restir_pt.slang:449-458
-template <class Stk>
-RRH_D bool visible(const DevScene& S, f3 o, f3 n, f3 target_pos, float bias, Stk& st) {
-    const f3 delta = sub(target_pos, o);
-    const float dist = length(delta);
-    if (dist <= bias) return false;
-    return !occluded_cull(S, add(o, scale(n, bias)), divs(delta, dist), bias, dist - bias, st);
-}
-
-// load_surface_from_images, restir_pt.slang:285-305
-RRH_D bool load_surface(const float4* pos, const float4* nrm, size_t i, f3& p, f3& n, uint32_t& mat) {
-    const float4 a = pos[i];
-    if (a.w < 0.0f) return false;
-    const float4 b = nrm[i];
-    p = xyz(a);
-    n = normalize(xyz(b));
-    mat = __float_as_uint(b.w);
-    return true;
-}
-
-struct Frame {
-    const float4* gpos;
-    const float4* gnrm;
-    uint32_t W, H;
-};
-
-RRH_D bool tile_pixel(uint32_t& px, uint32_t& py, uint32_t W, uint32_t H, bool remap) {
-    const uint32_t tile = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y, remap);
-    px = (tile % gridDim.x) * 8 + (threadIdx.x & 7u);
-    py = (tile / gridDim.x) * 8 + (threadIdx.x >> 3);
-    return px < W && py < H;
-}
-
-// rays of a wave summed into one counter (one atomic per wave)
-RRH_D void count_rays(uint32_t* counter, uint32_t n) {
-    for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
-    if ((threadIdx.x & 63u) == 0 && n) atomicAdd(counter, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -418,24 +304,6 @@ __global__ void __launch_bounds__(64, 4) k_rpt_initial(DevScene S, rrh_rpt_param
 // ---------------------------------------------------------------------------
 // cs_temporal, restir_pt.slang:775-872 (reprojection :323-421)
 // ---------------------------------------------------------------------------
-struct History {
-    const float4* pos;
-    const float4* nrm;
-};
-
-RRH_D bool reproject(const rrh_camera& cam, f3 p, uint32_t W, uint32_t H, int& hx, int& hy) {
-    const float* M = cam.prev_view_proj;
-    const float cx = mrow(M, 0, p.x, p.y, p.z, 1.0f), cy = mrow(M, 1, p.x, p.y, p.z, 1.0f),
-                cw = mrow(M, 3, p.x, p.y, p.z, 1.0f);
-    if (cw <= 1.0e-6f) return false;
-    const float nx = cx / cw, ny = cy / cw;
-    if (fabsf(nx) > 1.0f || fabsf(ny) > 1.0f) return false;
-    const float fx = (nx * 0.5f + 0.5f) * (float)W - 0.5f, fy = (ny * 0.5f + 0.5f) * (float)H - 0.5f;
-    hx = (int)floorf(fx + 0.5f);
-    hy = (int)floorf(fy + 0.5f);
-    return hx >= 0 && hy >= 0 && hx < (int)W && hy < (int)H;
-}
-
 template <int STK>
 __global__ void __launch_bounds__(64) k_rpt_temporal(DevScene S, rrh_rpt_params P, Frame F, Res4 R, Res4 Rp,
                                                      History Hp, uint32_t* __restrict__ counter, bool remap) {
@@ -447,29 +315,10 @@ __global__ void __launch_bounds__(64) k_rpt_temporal(DevScene S, rrh_rpt_params 
         uint32_t pmi;
         const rrh_camera& cam = *S.cam;
         const f3 cp = ld3(cam.position);
-        int cx, cy;
-        if (load_surface(F.gpos, F.gnrm, i, pp, pn, pmi) && reproject(cam, pp, F.W, F.H, cx, cy)) {
-            // find_reprojected_history_surface: the most consistent of the 3x3 history pixels
-            bool found = false;
-            float best = 1e30f;
-            size_t hj = 0;
-            f3 hp = mk(0.0f, 0.0f, 0.0f);
-            const float vdist = fmaxf(length(sub(pp, cp)), 1.0f);
-            for (int oy = -1; oy <= 1; ++oy)
-                for (int ox = -1; ox <= 1; ++ox) {
-                    const int qx = cx + ox, qy = cy + oy;
-                    if (qx < 0 || qy < 0 || qx >= (int)F.W || qy >= (int)F.H) continue;
-                    const size_t j = (size_t)qy * F.W + (size_t)qx;
-                    f3 q, qn;
-                    uint32_t qm;
-                    if (!load_surface(Hp.pos, Hp.nrm, j, q, qn, qm)) continue;
-                    if (qm != pmi || dot(pn, qn) < 0.9f) continue;
-                    const float plane = fabsf(dot(sub(q, pp), pn));
-                    const float dpos = length(sub(q, pp));
-                    if (!(plane <= 0.01f * vdist && dpos <= 0.05f * vdist)) continue;
-                    const float err = plane * 4.0f + dpos;
-                    if (err < best) { best = err; hj = j; hp = q; found = true; }
-                }
+        size_t hj;
+        f3 hp;
+        if (load_surface(F.gpos, F.gnrm, i, pp, pn, pmi)) {
+            const bool found = find_history(cam, Hp, F.W, F.H, pp, pn, pmi, cp, hj, hp);
             if (found) {
                 Reservoir cur = load_res(R, i);
                 Reservoir prv = load_res(Rp, hj);
@@ -521,8 +370,6 @@ __global__ void __launch_bounds__(64) k_rpt_temporal(DevScene S, rrh_rpt_params 
 // ---------------------------------------------------------------------------
 // cs_spatial_shade, restir_pt.slang:876-1001
 // ---------------------------------------------------------------------------
-RRH_D float fracf(float x) { return x - floorf(x); }
-
 template <int STK>
 __global__ void __launch_bounds__(64) k_rpt_spatial_shade(DevScene S, rrh_rpt_params P, HashGrid G, Frame F, Res4 R,
                                                           const float4* __restrict__ direct, float4* __restrict__ out,
@@ -547,25 +394,13 @@ __global__ void __launch_bounds__(64) k_rpt_spatial_shade(DevScene S, rrh_rpt_pa
             } else {
                 Reservoir res = load_res(R, i);
                 if (P.spatial_enable && P.spatial_num_neighbors > 0 && res.valid > 0.5f) {
-                    const float base_phase = fracf((float)pcg_hash(px ^ (py * 65537u)) * 2.3283064365386963e-10f);
+                    const float phase = spiral_phase(px, py);
                     const float dist_scale = fmaxf(length(sub(pp, cp)), 1.0f);
                     for (uint32_t k = 0; k < P.spatial_num_neighbors; ++k) {
-                        const float u = ((float)k + 0.5f) / (float)P.spatial_num_neighbors;
-                        const float angle = fracf(base_phase + 0.61803398875f * (float)k) * K_TWO_PI;
-                        const float radius = sqrtf(u) * (float)P.spatial_radius;
-                        float sa, ca;
-                        sincos_rr(angle, &sa, &ca);
-                        int nx = (int)px + (int)(radius * ca), ny = (int)py + (int)(radius * sa);
-                        nx = nx < 0 ? 0 : (nx > (int)F.W - 1 ? (int)F.W - 1 : nx);
-                        ny = ny < 0 ? 0 : (ny > (int)F.H - 1 ? (int)F.H - 1 : ny);
-                        if (nx == (int)px && ny == (int)py) continue;
-                        const size_t j = (size_t)ny * F.W + (size_t)nx;
-                        // is_spatial_neighbor_compatible, restir_pt.slang:403-421
-                        const float4 q4 = F.gpos[j];
-                        if (q4.w < 0.0f) continue;
-                        const f3 q = xyz(q4), qn = normalize(xyz(F.gnrm[j]));
-                        if (dot(qn, pn) < 0.85f) continue;
-                        if (!(fabsf(dot(sub(q, pp), pn)) <= 0.02f * dist_scale)) continue;
+                        size_t j;
+                        f3 q;
+                        if (!neighbour(k, P.spatial_num_neighbors, P.spatial_radius, phase, px, py, F.W, F.H, j)) continue;
+                        if (!neighbour_surface(F, j, pp, pn, dist_scale, q)) continue;
                         const Reservoir nb = load_res(R, j);
                         if (nb.M == 0 || nb.valid < 0.5f) continue;
                         const float jac = jacobian(q, pp, nb.rc_pos, nb.rc_n);
@@ -574,7 +409,7 @@ __global__ void __launch_bounds__(64) k_rpt_spatial_shade(DevScene S, rrh_rpt_pa
                         if (contrib <= 0.0f) continue;
                         res.w_sum += contrib;
                         res.M += nb.M;
-                        const float accept = fracf(base_phase + 0.754877666f * (float)(k + 1u));
+                        const float accept = fracf(phase + 0.754877666f * (float)(k + 1u));
                         if (res.w_sum > 0.0f && accept < contrib / res.w_sum) {
                             res.rc_pos = nb.rc_pos; res.rc_n = nb.rc_n;
                             res.suffix = nb.suffix; res.prefix = nb.prefix;
@@ -627,8 +462,6 @@ __global__ void k_rpt_totals(const uint32_t* __restrict__ cnt, unsigned long lon
 }
 
 // ---- host side -----------------------------------------------------------------------
-bool fits(const rrh_image* a, uint32_t W, uint32_t H) { return a && a->w == W && a->h == H && a->fmt == RRH_FMT_RGBA32F; }
-
 int res4(rrh_image* const* r, uint32_t W, uint32_t H, Res4& out, const char* who) {
     if (!r) { set_error(std::string(who) + ": null reservoir"); return RRH_E_ARG; }
     for (int k = 0; k < 4; ++k)
@@ -652,37 +485,15 @@ int hash_grid(rrh_grid* g, HashGrid& G, const char* who) {
     return RRH_OK;
 }
 
-// shared checks (the ray counters are cleared by the initial stage)
 int prepare(rrh_ctx* c, rrh_scene* s, const rrh_rpt_params* p, const rrh_image* gpos, const rrh_image* gnrm,
             Frame& F, const char* who) {
-    if (!(c && s && p && gpos && gnrm)) { set_error(std::string(who) + ": null argument"); return RRH_E_ARG; }
-    if (!s->cam_set) { set_error(std::string(who) + ": camera not set (rrh_camera_set)"); return RRH_E_ARG; }
-    const uint32_t W = s->cam_host.screen_width, H = s->cam_host.screen_height;
-    if (!fits(gpos, W, H) || !fits(gnrm, W, H)) {
-        set_error(std::string(who) + ": G-buffer position / normal must be RGBA32F images of the camera's extent");
-        return RRH_E_ARG;
-    }
-    if (p->num_initial_candidates > 4096 || p->max_bounces > 64 || p->spatial_num_neighbors > 64) {
-        set_error(std::string(who) + ": num_initial_candidates <= 4096, max_bounces <= 64, spatial_num_neighbors <= 64");
-        return RRH_E_RANGE;
-    }
-    if (s->stack4 > RRH_STACK) { set_error(std::string(who) + ": tree deeper than the traversal stack"); return RRH_E_STATE; }
-    int rc = c->rpt_counter.ensure(16);
+    if (!p) { set_error(std::string(who) + ": null argument"); return RRH_E_ARG; }
+    int rc = check_frame(c, s, gpos, gnrm, p->num_initial_candidates, p->spatial_num_neighbors, F, who);
     if (rc) return rc;
-    F = Frame{(const float4*)gpos->p, (const float4*)gnrm->p, W, H};
-    return RRH_OK;
+    if (p->max_bounces > 64) { set_error(std::string(who) + ": max_bounces <= 64"); return RRH_E_RANGE; }
+    return c->rpt_counter.ensure(16);
 }
 
-// culled rays traverse 4-wide nodes, so the stack must hold what the 4-wide tree can need
-int stack_kind(const rrh_ctx* c, const rrh_scene* s) { return stack_for(c, s->stack4); }
-
-template <template <int> class K, class... A>
-void launch(int sk, dim3 g, hipStream_t st, A... a) {
-    if (sk == RRH_STACK_LDS32) hipLaunchKernelGGL(K<RRH_STACK_LDS32>::fn, g, dim3(64), 0, st, a...);
-    else if (sk == RRH_STACK_HYBRID16) hipLaunchKernelGGL(K<RRH_STACK_HYBRID16>::fn, g, dim3(64), 0, st, a...);
-    else if (sk == RRH_STACK_HYBRID32) hipLaunchKernelGGL(K<RRH_STACK_HYBRID32>::fn, g, dim3(64), 0, st, a...);
-    else hipLaunchKernelGGL(K<RRH_STACK_PRIVATE>::fn, g, dim3(64), 0, st, a...);
-}
 template <int S> struct KInit { static constexpr auto fn = k_rpt_initial<S>; };
 template <int S> struct KTemp { static constexpr auto fn = k_rpt_temporal<S>; };
 template <int S> struct KSpat { static constexpr auto fn = k_rpt_spatial_shade<S>; };
@@ -706,13 +517,13 @@ int rrh_rpt_initial(rrh_ctx* c, rrh_scene* s, const rrh_rpt_params* p, rrh_grid*
     // persistent waves: as many as the CUs hold at once (the occupancy the kernel's registers and
     // LDS stack allow), never more than there are 8x8 tiles
     const int sk = stack_kind(c, s);
-    int per_cu = 0;
-    if (sk == RRH_STACK_LDS32) RRH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, KInit<RRH_STACK_LDS32>::fn, 64, 0));
-    else if (sk == RRH_STACK_HYBRID16) RRH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, KInit<RRH_STACK_HYBRID16>::fn, 64, 0));
-    else if (sk == RRH_STACK_HYBRID32) RRH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, KInit<RRH_STACK_HYBRID32>::fn, 64, 0));
-    else RRH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, KInit<RRH_STACK_PRIVATE>::fn, 64, 0));
     const uint32_t tiles_x = (F.W + 7) / 8, tiles = tiles_x * ((F.H + 7) / 8);
-    const uint32_t waves = std::min<uint32_t>(tiles, (uint32_t)c->cu_count * (uint32_t)std::max(per_cu, 1));
+    uint32_t waves = 0;
+    if (sk == RRH_STACK_LDS32) rc = persistent_waves(c, KInit<RRH_STACK_LDS32>::fn, tiles, waves);
+    else if (sk == RRH_STACK_HYBRID16) rc = persistent_waves(c, KInit<RRH_STACK_HYBRID16>::fn, tiles, waves);
+    else if (sk == RRH_STACK_HYBRID32) rc = persistent_waves(c, KInit<RRH_STACK_HYBRID32>::fn, tiles, waves);
+    else rc = persistent_waves(c, KInit<RRH_STACK_PRIVATE>::fn, tiles, waves);
+    if (rc) return rc;
     launch<KInit>(sk, dim3(waves), c->stream, s->dev(), *p, G, F, R, (float4*)hist_pos->p, (float4*)hist_norm->p,
                   (uint32_t*)c->rpt_counter.p, tiles_x, tiles * 64u);
     RRH_LAUNCH_CHECK();

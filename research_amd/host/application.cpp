@@ -38,7 +38,8 @@ void Application::initialize_renderer() {
     guiding_pass_ = renderer_.add_pass<passes::GuidingPass>();
     accumulate_pass_ = renderer_.add_pass<passes::AccumulatePass>();
     restir_di_pass_ = renderer_.add_pass<passes::ReSTIRDIPass>();
-    restir_pt_pass_ = renderer_.add_pass<passes::ReSTIRPTPass>();  // after DI: its G-buffer and direct image
+    restir_gi_pass_ = renderer_.add_pass<passes::ReSTIRGIPass>();  // after DI: its G-buffer and direct image
+    restir_pt_pass_ = renderer_.add_pass<passes::ReSTIRPTPass>();
     atrous_pass_ = renderer_.add_pass<passes::AtrousPass>();
     tonemap_pass_ = renderer_.add_pass<passes::TonemapPass>();  // last: the display image
     gbuffer_pass_->initialize(device_, e);
@@ -47,6 +48,9 @@ void Application::initialize_renderer() {
     guiding_pass_->initialize(device_, e);
     accumulate_pass_->initialize(device_, e);
     restir_di_pass_->initialize(device_, e);
+    restir_gi_pass_->initialize(device_, e);
+    restir_gi_pass_->set_inputs(&restir_di_pass_->gbuffer_position(), &restir_di_pass_->gbuffer_normal(),
+                                &restir_di_pass_->output_image());
     restir_pt_pass_->initialize(device_, e);
     restir_pt_pass_->set_inputs(&restir_di_pass_->gbuffer_position(), &restir_di_pass_->gbuffer_normal(),
                                 &restir_di_pass_->output_image());
@@ -55,6 +59,7 @@ void Application::initialize_renderer() {
     atrous_pass_->set_gbuffer(&restir_di_pass_->gbuffer_position(), &restir_di_pass_->gbuffer_normal());
     guiding_pass_->set_enabled(false);
     restir_di_pass_->set_enabled(false);
+    restir_gi_pass_->set_enabled(false);
     restir_pt_pass_->set_enabled(false);
     atrous_pass_->set_enabled(false);
 }
@@ -68,6 +73,7 @@ void Application::shutdown() {
         if (auto* q = dynamic_cast<passes::AccumulatePass*>(p.get())) q->shutdown(device_);
         if (auto* q = dynamic_cast<passes::ReSTIRDIPass*>(p.get())) q->shutdown(device_);
         if (auto* q = dynamic_cast<passes::ReSTIRPTPass*>(p.get())) q->shutdown(device_);
+        if (auto* q = dynamic_cast<passes::ReSTIRGIPass*>(p.get())) q->shutdown(device_);
         if (auto* q = dynamic_cast<passes::AtrousPass*>(p.get())) q->shutdown(device_);
         if (auto* q = dynamic_cast<passes::GBufferPass*>(p.get())) q->shutdown(device_);
         if (auto* q = dynamic_cast<passes::TonemapPass*>(p.get())) q->shutdown(device_);
@@ -78,6 +84,7 @@ void Application::shutdown() {
     accumulate_pass_ = nullptr;
     restir_di_pass_ = nullptr;
     restir_pt_pass_ = nullptr;
+    restir_gi_pass_ = nullptr;
     atrous_pass_ = nullptr;
     gbuffer_pass_ = nullptr;
     tonemap_pass_ = nullptr;
@@ -151,6 +158,7 @@ void Application::apply_scene_request() {
     accumulated_spp_ = 0;
     if (restir_di_pass_) restir_di_pass_->reset_history();
     if (restir_pt_pass_) restir_pt_pass_->reset_history();
+    if (restir_gi_pass_) restir_gi_pass_->reset_history();
     if (guiding_pass_) guiding_pass_->reset_history();
 }
 
@@ -166,14 +174,12 @@ std::array<float, 3> Application::camera_forward() const { return camera_.forwar
 void Application::set_pass_enabled(const std::string& name, bool enabled) {
     if (name == "restir_di") use_di_ = enabled;
     else if (name == "restir_pt") use_pt_ = enabled;
+    else if (name == "restir_gi") use_gi_ = enabled;
     else if (name == "denoise") use_denoise_ = enabled;
     else if (name == "guiding") use_guiding_ = enabled;
-    else if (name == "restir_gi")
-        throw std::invalid_argument("set_pass_enabled: 'restir_gi' is not part of this build (ReSTIR PT reuses "
-                                    "multi-bounce paths: 'restir_pt')");
     else
         throw std::invalid_argument("set_pass_enabled: unknown pass '" + name +
-                                    "' (expected restir_di | restir_pt | denoise | guiding)");
+                                    "' (expected restir_di | restir_gi | restir_pt | denoise | guiding)");
 }
 
 void Application::set_window_size(int width, int height) {
@@ -187,6 +193,7 @@ void Application::set_window_size(int width, int height) {
         accumulated_spp_ = 0;
         if (restir_di_pass_) restir_di_pass_->reset_history();
         if (restir_pt_pass_) restir_pt_pass_->reset_history();
+        if (restir_gi_pass_) restir_gi_pass_->reset_history();
     }
 }
 
@@ -195,16 +202,20 @@ void Application::step() {
     apply_scene_request();
     // display modes (Application.cpp:404-441): the real-time ones (DI, ReSTIR PT, optionally
     // denoised) show the frame; path tracing, guided or not, accumulates progressively
-    const bool use_realtime = use_di_ || use_pt_;
+    // ReSTIR PT and ReSTIR GI are exclusive indirect modes; PT wins (Application.cpp:420-423)
+    const bool use_gi = use_gi_ && !use_pt_;
+    const bool use_realtime = use_di_ || use_pt_ || use_gi;
     const bool use_denoise = use_denoise_ && use_realtime;
-    if (use_di_ != prev_use_di_ || use_pt_ != prev_use_pt_ || use_denoise != prev_use_denoise_ ||
-        use_guiding_ != prev_use_guiding_) {
+    if (use_di_ != prev_use_di_ || use_pt_ != prev_use_pt_ || use_gi_ != prev_use_gi_ ||
+        use_denoise != prev_use_denoise_ || use_guiding_ != prev_use_guiding_) {
         accumulated_spp_ = 0;
         camera_moved_ = true;
         restir_di_pass_->reset_history();
         restir_pt_pass_->reset_history();
+        restir_gi_pass_->reset_history();
         prev_use_di_ = use_di_;
         prev_use_pt_ = use_pt_;
+        prev_use_gi_ = use_gi_;
         prev_use_denoise_ = use_denoise;
         prev_use_guiding_ = use_guiding_;
     }
@@ -213,11 +224,14 @@ void Application::step() {
     accumulate_pass_->set_enabled(!use_realtime);
     // ReSTIR PT reads the G-buffer ReSTIR DI writes, so DI runs for it too; its direct image
     // is added only when DI is on (include_direct_lighting = use_di_, Application.cpp:424-425)
-    restir_di_pass_->set_enabled(use_di_ || use_pt_);
+    restir_di_pass_->set_enabled(use_di_ || use_pt_ || use_gi);
     restir_pt_pass_->set_enabled(use_pt_);
     restir_pt_pass_->include_direct_lighting = use_di_;
+    restir_gi_pass_->set_enabled(use_gi);
+    restir_gi_pass_->include_direct_lighting = use_di_;
     atrous_pass_->set_enabled(use_denoise);
-    atrous_pass_->set_input(use_pt_ ? &restir_pt_pass_->output_image() : &restir_di_pass_->output_image());
+    atrous_pass_->set_input(use_pt_ ? &restir_pt_pass_->output_image()
+                                    : (use_gi ? &restir_gi_pass_->output_image() : &restir_di_pass_->output_image()));
     accumulate_pass_->set_input(use_guiding_ ? &guiding_pass_->output_image() : &pathtracer_pass_->output_image());
     tonemap_pass_->set_input(current_display_image());
     tonemap_pass_->exposure = exposure_;
@@ -240,6 +254,7 @@ void Application::step() {
 
 const rhi::Image* Application::current_display_image() const {
     if (use_pt_) return (use_denoise_ && atrous_pass_) ? &atrous_pass_->output_image() : &restir_pt_pass_->output_image();
+    if (use_gi_) return (use_denoise_ && atrous_pass_) ? &atrous_pass_->output_image() : &restir_gi_pass_->output_image();
     if (use_di_) return (use_denoise_ && atrous_pass_) ? &atrous_pass_->output_image() : &restir_di_pass_->output_image();
     return &accumulate_pass_->accumulated_image();
 }

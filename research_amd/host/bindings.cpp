@@ -70,6 +70,18 @@ PYBIND11_MODULE(_app, m) {
         .def("reset_history", &ReSTIRPTPass::reset_history)
         .def("is_enabled", &ReSTIRPTPass::is_enabled)
         .def_property_readonly("name", [](const ReSTIRPTPass& p) { return std::string(p.name()); });
+    py::class_<ReSTIRGIPass>(m, "ReSTIRGIPass")
+        .def_readwrite("include_direct_lighting", &ReSTIRGIPass::include_direct_lighting)
+        .def_readwrite("temporal_enable", &ReSTIRGIPass::temporal_enable)
+        .def_readwrite("spatial_enable", &ReSTIRGIPass::spatial_enable)
+        .def_readwrite("indirect_strength", &ReSTIRGIPass::indirect_strength)
+        .def_readwrite("ray_bias", &ReSTIRGIPass::ray_bias)
+        .def_readwrite("num_initial_candidates", &ReSTIRGIPass::num_initial_candidates)
+        .def_readwrite("spatial_num_neighbors", &ReSTIRGIPass::spatial_num_neighbors)
+        .def_readwrite("spatial_radius", &ReSTIRGIPass::spatial_radius)
+        .def("reset_history", &ReSTIRGIPass::reset_history)
+        .def("is_enabled", &ReSTIRGIPass::is_enabled)
+        .def_property_readonly("name", [](const ReSTIRGIPass& p) { return std::string(p.name()); });
     py::class_<AtrousPass>(m, "AtrousPass")
         .def_readwrite("iterations", &AtrousPass::iterations)
         .def_readwrite("sigma_position", &AtrousPass::sigma_position)
@@ -161,6 +173,7 @@ PYBIND11_MODULE(_app, m) {
         .def_property_readonly("accumulate_pass", &Application::accumulate_pass, py::return_value_policy::reference_internal)
         .def_property_readonly("restir_di_pass", &Application::restir_di_pass, py::return_value_policy::reference_internal)
         .def_property_readonly("restir_pt_pass", &Application::restir_pt_pass, py::return_value_policy::reference_internal)
+        .def_property_readonly("restir_gi_pass", &Application::restir_gi_pass, py::return_value_policy::reference_internal)
         .def_property_readonly("atrous_pass", &Application::atrous_pass, py::return_value_policy::reference_internal)
         .def_property_readonly("guiding_pass", &Application::guiding_pass, py::return_value_policy::reference_internal)
         .def_property_readonly("gbuffer_pass", &Application::gbuffer_pass, py::return_value_policy::reference_internal)

@@ -349,6 +349,88 @@ void ReSTIRPTPass::execute(FrameContext& fc) {
     reservoir_flip_ ^= 1u;
 }
 
+// ---- ReSTIRGIPass (ReSTIRGIPass.cpp:428-651) ------------------------------------------------
+ReSTIRGIPass::~ReSTIRGIPass() { destroy_images(); }
+void ReSTIRGIPass::destroy_images() {
+    for (auto& r : reservoir_)
+        for (Image& i : r) i.destroy();
+    for (int k = 0; k < 2; ++k) {
+        history_pos_[k].destroy();
+        history_norm_[k].destroy();
+    }
+    output_img_.destroy();
+}
+void ReSTIRGIPass::initialize(rhi::Device& device, Extent2D extent) {
+    device_ = &device;
+    extent_ = extent;
+    for (auto& r : reservoir_)
+        for (Image& i : r) i.create(device, extent, Format::RGBA32F);
+    for (int k = 0; k < 2; ++k) {
+        history_pos_[k].create(device, extent, Format::RGBA32F);
+        history_norm_[k].create(device, extent, Format::RGBA32F);
+    }
+    output_img_.create(device, extent, Format::RGBA32F);
+    reset_history();
+    initialized_ = true;
+}
+void ReSTIRGIPass::shutdown(rhi::Device&) {
+    destroy_images();
+    initialized_ = false;
+}
+RenderPass::Reflection ReSTIRGIPass::reflect() const {
+    Reflection r;
+    r.inputs.push_back(desc("gbuffer_position", Format::RGBA32F, extent_, false));
+    r.inputs.push_back(desc("gbuffer_normal", Format::RGBA32F, extent_, false));
+    r.inputs.push_back(desc("direct_radiance", Format::RGBA32F, extent_, false));
+    for (const char* n : {"reservoir_pos", "reservoir_rad", "reservoir_meta", "history_primary_pos",
+                          "history_primary_norm"})
+        r.outputs.push_back(desc(n, Format::RGBA32F, extent_, true));
+    r.outputs.push_back(desc("output", Format::RGBA32F, extent_, false));
+    return r;
+}
+void ReSTIRGIPass::on_resize(Extent2D e) {
+    if (!initialized_) return;
+    destroy_images();
+    initialize(*device_, e);
+}
+void ReSTIRGIPass::execute(FrameContext& fc) {
+    if (!initialized_ || !fc.scene || !fc.scene->valid() || !gbuf_pos_ || !gbuf_norm_) return;
+    rrh_ctx* c = device_->ctx();
+    rrh_scene* s = fc.scene->handle();
+    const uint32_t cur = reservoir_flip_, prev = 1u - reservoir_flip_;
+    rrh_image* res_cur[3];
+    rrh_image* res_prev[3];
+    for (int k = 0; k < 3; ++k) {
+        res_cur[k] = reservoir_[cur][k].handle();
+        res_prev[k] = reservoir_[prev][k].handle();
+    }
+    if (history_dirty_) {
+        check(rrh_rgi_clear(c, res_prev, history_pos_[prev].handle(), history_norm_[prev].handle()), "rrh_rgi_clear");
+        history_dirty_ = false;
+    }
+    rrh_rgi_params p{};
+    p.frame_seed = frame_count_++;
+    p.num_initial_candidates = num_initial_candidates;
+    p.temporal_enable = temporal_enable ? 1u : 0u;
+    p.spatial_enable = spatial_enable ? 1u : 0u;
+    p.spatial_num_neighbors = spatial_num_neighbors;
+    p.spatial_radius = spatial_radius;
+    p.include_direct_lighting = include_direct_lighting ? 1u : 0u;
+    p.indirect_strength = indirect_strength;
+    p.ray_bias = ray_bias;
+    check(rrh_rgi_initial(c, s, &p, gbuf_pos_->handle(), gbuf_norm_->handle(), res_cur, history_pos_[cur].handle(),
+                          history_norm_[cur].handle()),
+          "rrh_rgi_initial");
+    check(rrh_rgi_temporal(c, s, &p, gbuf_pos_->handle(), gbuf_norm_->handle(), res_cur, res_prev,
+                           history_pos_[prev].handle(), history_norm_[prev].handle()),
+          "rrh_rgi_temporal");
+    check(rrh_rgi_spatial_shade(c, s, &p, gbuf_pos_->handle(), gbuf_norm_->handle(),
+                                (include_direct_lighting && direct_) ? direct_->handle() : nullptr, res_cur,
+                                output_img_.handle()),
+          "rrh_rgi_spatial_shade");
+    reservoir_flip_ ^= 1u;
+}
+
 // ---- AtrousPass (AtrousPass.cpp:243-306) ----------------------------------------------------
 AtrousPass::~AtrousPass() {
     ping_images_[0].destroy();

@@ -405,6 +405,51 @@ private:
     uint32_t reservoir_flip_ = 0, frame_count_ = 0;
 };
 
+// ReSTIR GI (ReSTIRGIPass.h:22-122): one-bounce indirect reuse with the reconnection shift over
+// ReSTIR DI's G-buffer and direct image; cosine candidates, temporal and spatial reuse.
+class ReSTIRGIPass : public RenderPass {
+public:
+    ~ReSTIRGIPass() override;
+    void initialize(rhi::Device& device, rhi::Extent2D extent);
+    void shutdown(rhi::Device& device);
+    bool reload_shader() { return true; }
+    [[nodiscard]] const char* name() const override { return "ReSTIRGIPass"; }
+    [[nodiscard]] Reflection reflect() const override;
+    void on_resize(rhi::Extent2D new_extent) override;
+    void execute(FrameContext& fc) override;
+    void set_inputs(const rhi::Image* gbuf_pos, const rhi::Image* gbuf_norm, const rhi::Image* direct) {
+        gbuf_pos_ = gbuf_pos;
+        gbuf_norm_ = gbuf_norm;
+        direct_ = direct;
+    }
+    void reset_history() {
+        reservoir_flip_ = 0;
+        frame_count_ = 0;
+        history_dirty_ = true;
+    }
+    [[nodiscard]] const rhi::Image& output_image() const { return output_img_; }
+
+    bool include_direct_lighting = true;
+    bool temporal_enable = true;
+    bool spatial_enable = true;
+    float indirect_strength = 1.0f;
+    float ray_bias = 1.0e-3f;
+    uint32_t num_initial_candidates = 8;
+    uint32_t spatial_num_neighbors = 4;
+    uint32_t spatial_radius = 12;
+
+private:
+    void destroy_images();
+    rhi::Device* device_ = nullptr;
+    const rhi::Image *gbuf_pos_ = nullptr, *gbuf_norm_ = nullptr, *direct_ = nullptr;
+    rhi::Image reservoir_[2][3];
+    rhi::Image history_pos_[2], history_norm_[2];
+    rhi::Image output_img_;
+    rhi::Extent2D extent_{};
+    bool initialized_ = false, history_dirty_ = true;
+    uint32_t reservoir_flip_ = 0, frame_count_ = 0;
+};
+
 // Edge-aware A-trous wavelet filter over the ReSTIR DI output and its G-buffer.
 class AtrousPass : public RenderPass {
 public:
@@ -541,6 +586,7 @@ public:
     passes::TonemapPass* tonemap_pass() { return tonemap_pass_; }
     passes::GuidingPass* guiding_pass() { return guiding_pass_; }
     passes::ReSTIRPTPass* restir_pt_pass() { return restir_pt_pass_; }
+    passes::ReSTIRGIPass* restir_gi_pass() { return restir_gi_pass_; }
 
 private:
     void initialize_renderer();
@@ -559,13 +605,15 @@ private:
     passes::TonemapPass* tonemap_pass_ = nullptr;
     passes::GuidingPass* guiding_pass_ = nullptr;
     passes::ReSTIRPTPass* restir_pt_pass_ = nullptr;
+    passes::ReSTIRGIPass* restir_gi_pass_ = nullptr;
     const rhi::Image* display_ = nullptr;
 
     int width_ = 1600, height_ = 900;  // Application.h:158-159
     float exposure_ = 1.0f;
     bool input_enabled_ = true, initialized_ = false, close_requested_ = false;
-    bool use_di_ = false, use_denoise_ = false, use_guiding_ = false, use_pt_ = false;
+    bool use_di_ = false, use_denoise_ = false, use_guiding_ = false, use_pt_ = false, use_gi_ = false;
     bool prev_use_di_ = false, prev_use_denoise_ = false, prev_use_guiding_ = false, prev_use_pt_ = false;
+    bool prev_use_gi_ = false;
     bool camera_moved_ = true;
     uint32_t accumulated_spp_ = 0, frames_rendered_ = 0;
     std::string current_scene_name_;

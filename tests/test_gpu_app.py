@@ -154,6 +154,50 @@ def test_restir_pt_mode_equals_direct_calls(ctx, app, with_di):
     assert np.array_equal(_bits(got), _bits(ref))
 
 
+def test_restir_gi_mode_equals_direct_calls(ctx, app):
+    """ReSTIR GI over ReSTIR DI's G-buffer and direct image (ReSTIRGIPass.cpp:428-651); with
+    ReSTIR PT also on, PT wins (Application.cpp:418-420)."""
+    app.set_pass_enabled("restir_di", True)
+    app.set_pass_enabled("restir_gi", True)
+    gi = app.restir_gi_pass
+    assert gi.name == "ReSTIRGIPass" and gi.num_initial_candidates == 8
+    frames = 3
+    for _ in range(frames):
+        app.step()
+    assert gi.is_enabled() and not app.restir_pt_pass.is_enabled()
+    got = app.display_image()
+
+    sa, gs, eye, center = _direct_setup(ctx, app)
+    im = lambda fmt=L.FMT_RGBA32F: core.Image(ctx, W, H, fmt)
+    gp, gn, direct, out = im(), im(), im(), im()
+    dres, duv = [im(), im()], [im(L.FMT_R32U), im(L.FMT_R32U)]
+    res = [[im() for _ in range(3)], [im() for _ in range(3)]]
+    hist = [[im(), im()], [im(), im()]]
+    for r in dres + duv:
+        r.clear()
+    L.call("rrh_rgi_clear", ctx.h, core.planes(res[1]), hist[1][0].h, hist[1][1].h)
+    for f in range(frames):
+        gs.set_camera(core.camera_look_at(eye, center, sa.fov, W, H, frame_index=f))
+        cur, prev = f & 1, 1 - (f & 1)
+        dprm = L.DIParams(f, 16, 1, 1, 4, 15)
+        L.call("rrh_di_initial", ctx.h, gs.h, C.byref(dprm), gp.h, gn.h, dres[cur].h, duv[cur].h)
+        L.call("rrh_di_temporal", ctx.h, gs.h, C.byref(dprm), gp.h, gn.h, dres[cur].h, duv[cur].h, dres[prev].h,
+               duv[prev].h)
+        L.call("rrh_di_spatial_shade", ctx.h, gs.h, C.byref(dprm), gp.h, gn.h, dres[cur].h, duv[cur].h, direct.h)
+        prm = L.RGIParams(f, 8, 1, 1, 4, 12, 1, 1.0, 1e-3)
+        L.call("rrh_rgi_initial", ctx.h, gs.h, C.byref(prm), gp.h, gn.h, core.planes(res[cur]), hist[cur][0].h,
+               hist[cur][1].h)
+        L.call("rrh_rgi_temporal", ctx.h, gs.h, C.byref(prm), gp.h, gn.h, core.planes(res[cur]),
+               core.planes(res[prev]), hist[prev][0].h, hist[prev][1].h)
+        L.call("rrh_rgi_spatial_shade", ctx.h, gs.h, C.byref(prm), gp.h, gn.h, direct.h, core.planes(res[cur]), out.h)
+    ref = out.download()
+    assert np.isfinite(ref).all() and ref[..., :3].mean() > 0
+    assert np.array_equal(_bits(got), _bits(ref))
+    app.set_pass_enabled("restir_pt", True)  # PT wins
+    app.step()
+    assert app.restir_pt_pass.is_enabled() and not gi.is_enabled()
+
+
 @pytest.mark.parametrize("mode", ["vmm", "mlp"])
 def test_guiding_mode_runs_and_trains(app, mode):
     app.load_procedural(L.GEN_CORNELL_RANDOM, 1, 1.0)

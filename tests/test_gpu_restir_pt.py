@@ -31,9 +31,10 @@ def _same_frac(a, b):
 
 
 def _res_same_frac(g, o):
-    # g, o: (4, n, 4); a pixel agrees when all four planes do
+    # g, o: (planes, n, 4); a pixel agrees when every plane does
+    assert g.shape == o.shape
     eq = np.ones(g.shape[1], bool)
-    for k in range(4):
+    for k in range(g.shape[0]):
         eq &= np.all(_bits(g[k]) == _bits(o[k]), axis=1)
     return eq.mean()
 

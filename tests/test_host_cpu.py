@@ -31,9 +31,10 @@ def test_application_state_before_initialize():
         with pytest.raises(RuntimeError):
             app.step()
         with pytest.raises(ValueError):
-            app.set_pass_enabled("restir_gi", True)
-        app.set_pass_enabled("restir_pt", True)
-        app.set_pass_enabled("restir_pt", False)
+            app.set_pass_enabled("restir", True)
+        for name in ("restir_pt", "restir_gi"):
+            app.set_pass_enabled(name, True)
+            app.set_pass_enabled(name, False)
         with pytest.raises(ValueError):
             app.set_pass_enabled("tonemap", True)
         app.set_pass_enabled("restir_di", True)

@@ -1,4 +1,5 @@
-"""Pins of the oracle's ReSTIR PT restatement (restir_pt.slang) by what the mathematics fixes:
+"""Pins of the oracle's ReSTIR PT and ReSTIR GI restatements (restir_pt.slang, restir_gi.slang) by
+what the mathematics fixes:
 
 * furnace: inside a closed box whose Lambertian walls (albedo rho) all emit Le and no light is
   sampled, every path suffix is exactly Le (continued bounces add no emission), so the
@@ -168,3 +169,46 @@ def test_temporal_reuse_with_itself_keeps_w():
     big[1][:, 3] = np.full(len(gp), 70, np.uint32).view(np.float32)
     out2 = os_.rpt_temporal(oc, prm, gp, gn, big, big, hp, hn)
     assert (out2[1][keep, 3].view(np.uint32) == 80).all()
+
+
+# ---- ReSTIR GI (restir_gi.slang): the one-bounce sibling -------------------------------------
+def _gi_prm(M, temporal=1, bias=1e-3):
+    return O.RGIParams(3, M, temporal, 0, 4, 12, 0, 1.0, bias)
+
+
+def _gi_estimate(gp, gn, res):
+    n = gn[:, :3] / np.linalg.norm(gn[:, :3], axis=1, keepdims=True)
+    to = res[0][:, :3].astype(np.float64) - gp[:, :3]
+    wi = to / np.linalg.norm(to, axis=1, keepdims=True)
+    cos = np.maximum((n * wi).sum(1), 0)
+    return RHO / np.pi * cos * res[1][:, 0] * res[1][:, 3]
+
+
+@pytest.mark.parametrize("M", [1, 8])
+def test_gi_furnace_every_reservoir_estimates_rho_le(M):
+    """ReSTIR GI takes its source pdf from the hit position (cos / pi toward it), so with a
+    default ray bias the furnace identity f cos Le W = rho Le still holds to rounding."""
+    os_, oc, gp, gn = _setup()
+    res, hp, hn = os_.rgi_initial(oc, _gi_prm(M), gp, gn)
+    Ms = res[2][:, 3].view(np.uint32)
+    valid = Ms > 0
+    assert valid.mean() > 0.99 and (Ms[valid] <= M).all()
+    assert (res[1][valid, :3] == np.float32(LE)).all()
+    assert np.allclose(_gi_estimate(gp, gn, res)[valid], RHO * LE, rtol=2e-5)
+    # the kept normal is the interpolated one: the walls' inward normals, facing the primary
+    to = res[0][valid, :3] - gp[valid, :3]
+    assert ((res[2][valid, :3] * to).sum(1) < 0).all()
+    assert np.array_equal(hp, gp) and np.array_equal(hn, gn)
+
+
+def test_gi_temporal_reuse_with_itself_keeps_w():
+    os_, oc, gp, gn = _setup()
+    prm = _gi_prm(4)
+    res, hp, hn = os_.rgi_initial(oc, prm, gp, gn)
+    out = os_.rgi_temporal(oc, prm, gp, gn, res, res, hp, hn)
+    M0, M1 = res[2][:, 3].view(np.uint32), out[2][:, 3].view(np.uint32)
+    keep = (M1 == 2 * M0) & (M0 > 0)
+    under = keep & (res[1][:, 3] < 50)
+    assert under.mean() > 0.5
+    assert np.allclose(out[1][under, 3], res[1][under, 3], rtol=1e-5)
+    assert np.allclose(out[0][under, 3], 2 * res[0][under, 3], rtol=1e-6)
