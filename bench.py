@@ -180,6 +180,62 @@ def restir_pt_summary(ctx: core.Context, scene, sa, W: int, H: int, frames: int,
             "note": "stage times by HIP markers on the context stream; ReSTIR DI (G-buffer, direct) not included"}
 
 
+def restir_gi_summary(ctx: core.Context, scene, sa, W: int, H: int, frames: int, warmup: int) -> dict:
+    """ReSTIR GI (one-bounce reconnection reuse, the reference pass's defaults: 8 cosine
+    candidates, temporal and spatial reuse) over ReSTIR DI's G-buffer and direct image, on the
+    same scene, extent and orbit; stage times by markers on the context stream."""
+    img = lambda f=L.FMT_RGBA32F: core.Image(ctx, W, H, f)
+    gp, gn, direct, out = img(), img(), img(), img()
+    dres, duv = [img(), img()], [img(L.FMT_R32U), img(L.FMT_R32U)]
+    res = [[img() for _ in range(3)], [img() for _ in range(3)]]
+    hist = [[img(), img()], [img(), img()]]
+    for r in dres + duv:
+        r.clear()
+    L.call("rrh_rgi_clear", ctx.h, core.planes(res[1]), hist[1][0].h, hist[1][1].h)
+    c, s = ctx.h, scene.h
+    stage_ms = np.zeros(3)
+    ext = shadow = 0
+    prev_vp = None
+    for f in range(warmup + frames):
+        cam = orbit_camera(sa, f, W, H)
+        vp = list(cam.prev_view_proj)
+        if prev_vp is not None:
+            cam.prev_view_proj[:] = prev_vp
+        prev_vp = vp
+        scene.set_camera(cam)
+        cur, prev = f & 1, 1 - (f & 1)
+        d = L.DIParams(f, 16, 1, 1, 4, 15)
+        L.call("rrh_di_initial", c, s, C.byref(d), gp.h, gn.h, dres[cur].h, duv[cur].h)
+        L.call("rrh_di_temporal", c, s, C.byref(d), gp.h, gn.h, dres[cur].h, duv[cur].h, dres[prev].h, duv[prev].h)
+        L.call("rrh_di_spatial_shade", c, s, C.byref(d), gp.h, gn.h, dres[cur].h, duv[cur].h, direct.h)
+        p = L.RGIParams(f, 8, 1, 1, 4, 12, 1, 1.0, 1e-3)
+        ctx.marker(0)
+        L.call("rrh_rgi_initial", c, s, C.byref(p), gp.h, gn.h, core.planes(res[cur]), hist[cur][0].h, hist[cur][1].h)
+        ctx.marker(1)
+        L.call("rrh_rgi_temporal", c, s, C.byref(p), gp.h, gn.h, core.planes(res[cur]), core.planes(res[prev]),
+               hist[prev][0].h, hist[prev][1].h)
+        ctx.marker(2)
+        L.call("rrh_rgi_spatial_shade", c, s, C.byref(p), gp.h, gn.h, direct.h, core.planes(res[cur]), out.h)
+        ctx.marker(3)
+        ctx.synchronize()
+        if f >= warmup:
+            stage_ms += [ctx.elapsed_ms(k, k + 1) for k in range(3)]
+            rc = L.RayCounts()
+            L.call("rrh_rgi_ray_counts", c, C.byref(rc))
+            ext += rc.extension
+            shadow += rc.shadow
+    ms = stage_ms / frames
+    total = float(ms.sum())
+    rays = (ext + shadow) / frames
+    return {"workload": f"heightfield_{scene.info()['tri_count']}_tris_{W}x{H}_restir_gi_M8",
+            "ms_per_frame": round(total, 3),
+            "stage_ms": {"initial": round(ms[0], 3), "temporal": round(ms[1], 3), "spatial_shade": round(ms[2], 3)},
+            "rays_per_frame": {"extension": ext // frames, "shadow_and_visibility": shadow // frames},
+            "mrays_per_s": round(rays / (total * 1e-3) / 1e6, 1) if total > 0 else None,
+            "frames": frames,
+            "note": "stage times by HIP markers on the context stream; ReSTIR DI (G-buffer, direct) not included"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,7 +266,7 @@ def main() -> int:
     ap.add_argument("--guide-width", type=int, default=320)
     ap.add_argument("--guide-height", type=int, default=180)
     ap.add_argument("--restir-pt-frames", type=int, default=6,
-                    help="ReSTIR PT frames timed on the same scene and extent (0: skip)")
+                    help="ReSTIR PT and ReSTIR GI frames timed on the same scene and extent (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB copy kernel (for hardware-counter calibration) and exit")
     a = ap.parse_args()
@@ -325,6 +381,9 @@ def main() -> int:
         out["restir_pt"] = restir_pt_summary(ctx, scene, sa, W, H, a.restir_pt_frames, 2)
         print(f"[bench] ReSTIR PT {out['restir_pt']['ms_per_frame']} ms/frame, "
               f"{out['restir_pt']['mrays_per_s']} Mrays/s", file=sys.stderr, flush=True)
+        out["restir_gi"] = restir_gi_summary(ctx, scene, sa, W, H, a.restir_pt_frames, 2)
+        print(f"[bench] ReSTIR GI {out['restir_gi']['ms_per_frame']} ms/frame, "
+              f"{out['restir_gi']['mrays_per_s']} Mrays/s", file=sys.stderr, flush=True)
     if a.guide_seconds > 0:
         print("[bench] equal-time guiding comparison (indoor scene) ...", file=sys.stderr, flush=True)
         out["guiding"] = guiding_summary(ctx, a)
