@@ -104,3 +104,42 @@ def test_restir_gi_stages_off_and_checks(ctx):
     assert rc.extension > W * H
     with pytest.raises(RuntimeError, match="three RGBA32F"):
         L.call("rrh_rgi_initial", ctx.h, gs.h, C.byref(prm), gp.h, gn.h, core.planes(res[:2] + [duv]), hp.h, hn.h)
+
+
+@pytest.mark.parametrize("W,H", [(37, 29), (8, 1), (1, 9)])
+def test_restir_gi_odd_extents_and_no_lights(ctx, W, H):
+    from tests.test_oracle_restir_pt import furnace_box
+    sa = furnace_box()
+    gs = core.Scene(ctx, sa)
+    os_ = O.Scene.from_arrays(sa)
+    im = lambda: core.Image(ctx, W, H)
+    gp, gn, out = im(), im(), im()
+    res = [[im() for _ in range(3)], [im() for _ in range(3)]]
+    hist = [[im(), im()], [im(), im()]]
+    for k in range(2):
+        L.call("rrh_rgi_clear", ctx.h, core.planes(res[k]), hist[k][0].h, hist[k][1].h)
+    dl = lambda planes: np.stack([p.download().reshape(-1, 4) for p in planes])
+    for f, cam in enumerate(_cams(sa, W, H, 2, 0.02)):
+        cur, prev = f & 1, 1 - (f & 1)
+        gs.set_camera(cam)
+        oc = ocam(cam)
+        g_gp, g_gn, _ = os_.gbuffer(oc)
+        gp.upload(g_gp)
+        gn.upload(g_gn)
+        prm, oprm = _params(f, M=3, direct=0)
+        L.call("rrh_rgi_initial", ctx.h, gs.h, C.byref(prm), gp.h, gn.h, core.planes(res[cur]), hist[cur][0].h,
+               hist[cur][1].h)
+        o_res, _, _ = os_.rgi_initial(oc, oprm, g_gp, g_gn)
+        g_res = dl(res[cur])
+        assert _res_same_frac(g_res, o_res) == 1.0
+        if f > 0:
+            g_prev = dl(res[prev])
+            g_hp, g_hn = hist[prev][0].download().reshape(-1, 4), hist[prev][1].download().reshape(-1, 4)
+            L.call("rrh_rgi_temporal", ctx.h, gs.h, C.byref(prm), gp.h, gn.h, core.planes(res[cur]),
+                   core.planes(res[prev]), hist[prev][0].h, hist[prev][1].h)
+            o_res = os_.rgi_temporal(oc, oprm, g_gp, g_gn, g_res, g_prev, g_hp, g_hn)
+            g_res = dl(res[cur])
+            assert _res_same_frac(g_res, o_res) == 1.0
+        L.call("rrh_rgi_spatial_shade", ctx.h, gs.h, C.byref(prm), gp.h, gn.h, None, core.planes(res[cur]), out.h)
+        o_out = os_.rgi_spatial_shade(oc, oprm, g_gp, g_gn, g_res)
+        assert _same_frac(out.download().reshape(-1, 4), o_out) == 1.0

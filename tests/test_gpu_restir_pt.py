@@ -198,3 +198,51 @@ def test_restir_pt_argument_checks(ctx):
     big, _ = _params(0, M=5000)
     with pytest.raises(RuntimeError, match="num_initial_candidates"):
         L.call("rrh_rpt_initial", ctx.h, gs.h, C.byref(big), None, gp.h, gn.h, core.planes(res), hp.h, hn.h)
+
+
+@pytest.mark.parametrize("W,H", [(37, 29), (8, 1), (1, 9)])
+def test_restir_pt_odd_extents_and_no_lights(ctx, W, H):
+    """Extents that are not multiples of the 8x8 tile (partial tiles in the persistent waves'
+    pixel order) and a scene without lights (the furnace box: every wall emits): all three
+    stages against the oracle, the guiding grid's splats included."""
+    from tests.test_oracle_restir_pt import furnace_box
+    sa = furnace_box()
+    gs = core.Scene(ctx, sa)
+    os_ = O.Scene.from_arrays(sa)
+    grid = _grid(ctx, 1 << 12)
+    im = lambda: core.Image(ctx, W, H)
+    gp, gn, out = im(), im(), im()
+    res = [[im() for _ in range(4)], [im() for _ in range(4)]]
+    hist = [[im(), im()], [im(), im()]]
+    for k in range(2):
+        L.call("rrh_rpt_clear", ctx.h, core.planes(res[k]), hist[k][0].h, hist[k][1].h)
+    dl = lambda planes: np.stack([p.download().reshape(-1, 4) for p in planes])
+    gridp = lambda a: (a, 0.10, 0.02, grid.ncell)
+    for f, cam in enumerate(_cams(sa, W, H, 2, 0.02)):
+        cur, prev = f & 1, 1 - (f & 1)
+        gs.set_camera(cam)
+        oc = ocam(cam)
+        g_gp, g_gn, _ = os_.gbuffer(oc)  # the raster-style G-buffer: every pixel sees a wall (w = 1)
+        gp.upload(g_gp)
+        gn.upload(g_gn)
+        prm, oprm = _params(f, M=3, bounces=2, direct=0)
+        fitted, acc0 = grid.download()
+        L.call("rrh_rpt_initial", ctx.h, gs.h, C.byref(prm), grid.h, gp.h, gn.h, core.planes(res[cur]),
+               hist[cur][0].h, hist[cur][1].h)
+        o_res, _, _ = os_.rpt_initial(oc, oprm, g_gp, g_gn, grid=gridp(fitted))
+        g_res = dl(res[cur])
+        assert _res_same_frac(g_res, o_res) == 1.0
+        if f > 0:
+            g_prev = dl(res[prev])
+            g_hp, g_hn = hist[prev][0].download().reshape(-1, 4), hist[prev][1].download().reshape(-1, 4)
+            L.call("rrh_rpt_temporal", ctx.h, gs.h, C.byref(prm), gp.h, gn.h, core.planes(res[cur]),
+                   core.planes(res[prev]), hist[prev][0].h, hist[prev][1].h)
+            o_res = os_.rpt_temporal(oc, oprm, g_gp, g_gn, g_res, g_prev, g_hp, g_hn)
+            g_res = dl(res[cur])
+            assert _res_same_frac(g_res, o_res) == 1.0
+        L.call("rrh_rpt_spatial_shade", ctx.h, gs.h, C.byref(prm), grid.h, gp.h, gn.h, None, core.planes(res[cur]),
+               out.h)
+        o_out, o_acc = os_.rpt_spatial_shade(oc, oprm, g_gp, g_gn, g_res, grid=gridp(acc0))
+        assert _same_frac(out.download().reshape(-1, 4), o_out) == 1.0
+        assert np.array_equal(grid.download()[1], o_acc)
+        L.call("rrh_grid_fit", ctx.h, grid.h)
