@@ -180,7 +180,8 @@ typedef struct {
     float    guide_alpha;          /* one-sample MIS probability of the guide */
     uint32_t record_vertices;      /* > 0: record path vertices, keep about this many as training samples */
     uint32_t flags;                /* RRH_PT_* */
-    uint32_t _pad;
+    uint32_t guide_bounces;        /* guided modes: the guide samples bounces [0, guide_bounces), the
+                                      BSDF alone after them; 0 = every bounce */
 } rrh_pt_params;
 #define RRH_PT_COUNT_FETCHES 1u    /* instrumented traversal: count node / triangle fetches (slower) */
 typedef struct { uint64_t primary, extension, shadow; } rrh_ray_counts;

@@ -72,7 +72,7 @@ class SceneDesc(C.Structure):
 class PTParams(C.Structure):
     _fields_ = [("max_bounces", C.c_uint32), ("frame_seed", C.c_uint32), ("rr_start", C.c_uint32),
                 ("guide_mode", C.c_uint32), ("guide_alpha", C.c_float), ("record_vertices", C.c_uint32),
-                ("flags", C.c_uint32), ("_pad", C.c_uint32)]
+                ("flags", C.c_uint32), ("guide_bounces", C.c_uint32)]
 
 
 class RayCounts(C.Structure):

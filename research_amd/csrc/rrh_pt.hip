@@ -595,7 +595,7 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
         // the direction sampled at the last bounce is never traced (and its training record
         // carries no radiance): the guide is skipped there
         ShadeArgs Ab = A;
-        if (b + 1 >= p->max_bounces) Ab.guide_mode = RRH_GUIDE_NONE;
+        if (b + 1 >= p->max_bounces || (p->guide_bounces != 0 && b >= p->guide_bounces)) Ab.guide_mode = RRH_GUIDE_NONE;
         if (Ab.guide_mode == RRH_GUIDE_MLP) {
             rc = mlp_infer_queue(c, mlp, S, qi.o, qi.d, (const float4*)c->hits.p, cnt, b, npix,
                                  const_cast<float*>(A.guide.mlp_lobes));

@@ -45,8 +45,8 @@ class Setup:
         self.acc = core.Image(ctx, width, height, L.FMT_RGBA64F)
 
     def render(self, seed: int, mode: int = L.GUIDE_NONE, alpha: float = 0.0, record: int = 0, grid=None,
-               mlp=None) -> None:
-        p = L.PTParams(self.bounces, seed, 3, mode, alpha, record)
+               mlp=None, guide_bounces: int = 0) -> None:
+        p = L.PTParams(self.bounces, seed, 3, mode, alpha, record, 0, guide_bounces)
         L.call("rrh_pt_render_guided", self.ctx.h, self.scene.h, C.byref(p), grid.h if grid else None,
                mlp.h if mlp else None, self.rad.h)
 
@@ -69,7 +69,7 @@ class Setup:
 def run_method(S: Setup, method: str, seconds: float, ref: np.ndarray, spp_per_frame: int = 4,
                alpha: float = 0.5, batch: int = 262144, record: int | None = None, train_steps: int = 1,
                checkpoints: int = 12, seed0: int = 0, lr: float = 1e-3, decay: float = 0.95,
-               min_weight: float = 0.0, res: int = 32) -> dict:
+               min_weight: float = 0.0, res: int = 32, guide_bounces: int = 0) -> dict:
     """Render (and train) for `seconds` of wall time; relMSE at evenly spaced checkpoints.
     The clock stops while a checkpoint image is read back and scored.  The grid keeps its EM
     statistics across frames with weight `decay` per frame (0.95: about 20 frames of memory)."""
@@ -95,7 +95,7 @@ def run_method(S: Setup, method: str, seconds: float, ref: np.ndarray, spp_per_f
         a = alpha if (mode != L.GUIDE_NONE and frame > 0) else 0.0
         for s in range(spp_per_frame):
             rec = record if (mode != L.GUIDE_NONE and s == 0) else 0
-            S.render(seed0 + spp, mode, a, rec, grid, mlp)
+            S.render(seed0 + spp, mode, a, rec, grid, mlp, guide_bounces)
             S.accumulate(spp)
             spp += 1
         if method == "vmm":
@@ -176,10 +176,12 @@ def main() -> None:
     ap.add_argument("--train-steps", type=int, default=1)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--alpha", type=float, default=0.5)
+    ap.add_argument("--guide-bounces", type=int, default=0, help="guide only the first N bounces (0: all)")
     a = ap.parse_args()
     ctx = core.Context(0)
     res = evaluate(ctx, a.width, a.height, a.seconds, a.ref_spp, tuple(a.methods.split(",")), a.scale, True,
-                   spp_per_frame=a.spp_per_frame, batch=a.batch, train_steps=a.train_steps, lr=a.lr, alpha=a.alpha)
+                   spp_per_frame=a.spp_per_frame, batch=a.batch, train_steps=a.train_steps, lr=a.lr, alpha=a.alpha,
+                   guide_bounces=a.guide_bounces)
     print(json.dumps(res), flush=True)
 
 
