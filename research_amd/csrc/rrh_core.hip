@@ -12,7 +12,7 @@ void set_error(const std::string& msg) { g_err = msg; }
 
 int DevBuf::ensure(size_t n) {
     if (n <= bytes && p) return RRH_OK;
-    if (p) { hipFree(p); p = nullptr; bytes = 0; }
+    if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
     if (n == 0) n = 16;
     hipError_t e = hipMalloc(&p, n);
     if (e != hipSuccess) { p = nullptr; set_error(std::string("hipMalloc: ") + hipGetErrorString(e)); return RRH_E_NOMEM; }
@@ -20,7 +20,7 @@ int DevBuf::ensure(size_t n) {
     return RRH_OK;
 }
 void DevBuf::release() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
 }
@@ -102,16 +102,16 @@ int rrh_ctx_create(int device, rrh_ctx** out) {
 
 int rrh_ctx_destroy(rrh_ctx* c) {
     if (!c) return RRH_OK;
-    hipSetDevice(c->device);
-    hipStreamSynchronize(c->stream);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
     for (int i = 0; i < 2; ++i) { c->q_o[i].release(); c->q_d[i].release(); c->q_thp[i].release(); }
     c->hits.release(); c->sh_o.release(); c->sh_d.release(); c->sh_c.release();
     c->counters.release(); c->vtx.release(); c->di_counter.release(); c->rpt_counter.release(); c->rgi_counter.release(); c->scratch.release(); c->mlp_prep.release();
     c->train.release(); c->train_cnt.release(); c->totals.release(); c->fetch.release();
     c->sort_keys.release(); c->sort_perm.release(); c->sort_tmp.release();
-    for (hipEvent_t e : c->prof_events) hipEventDestroy(e);
-    for (int i = 0; i < 16; ++i) hipEventDestroy(c->markers[i]);
-    hipStreamDestroy(c->stream);
+    for (hipEvent_t e : c->prof_events) (void)hipEventDestroy(e);
+    for (int i = 0; i < 16; ++i) (void)hipEventDestroy(c->markers[i]);
+    (void)hipStreamDestroy(c->stream);
     delete c;
     return RRH_OK;
 }
@@ -202,14 +202,14 @@ int rrh_image_create(rrh_ctx* c, uint32_t w, uint32_t h, int fmt, rrh_image** ou
     hipError_t e = hipMalloc(&im->p, im->bytes);
     if (e != hipSuccess) { delete im; set_error(std::string("hipMalloc: ") + hipGetErrorString(e)); return RRH_E_NOMEM; }
     e = hipMemsetAsync(im->p, 0, im->bytes, c->stream);
-    if (e != hipSuccess) { hipFree(im->p); delete im; set_error(hipGetErrorString(e)); return RRH_E_HIP; }
+    if (e != hipSuccess) { (void)hipFree(im->p); delete im; set_error(hipGetErrorString(e)); return RRH_E_HIP; }
     *out = im;
     return RRH_OK;
 }
 int rrh_image_destroy(rrh_image* im) {
     if (!im) return RRH_OK;
     (void)hipDeviceSynchronize();  // the owning context may already be gone
-    if (im->p) hipFree(im->p);
+    if (im->p) (void)hipFree(im->p);
     delete im;
     return RRH_OK;
 }
@@ -612,7 +612,7 @@ int rrh_stream_copy(rrh_ctx* c, size_t bytes, int iterations, float* gbps) {
     void *a = nullptr, *b = nullptr;
     RRH_HIP(hipMalloc(&a, bytes));
     hipError_t e = hipMalloc(&b, bytes);
-    if (e != hipSuccess) { hipFree(a); set_error(hipGetErrorString(e)); return RRH_E_NOMEM; }
+    if (e != hipSuccess) { (void)hipFree(a); set_error(hipGetErrorString(e)); return RRH_E_NOMEM; }
     size_t n = bytes / 16;
     int grid = 256 * 32;
     (void)hipMemsetAsync(a, 0, bytes, c->stream);
@@ -624,8 +624,8 @@ int rrh_stream_copy(rrh_ctx* c, size_t bytes, int iterations, float* gbps) {
     e = hipEventSynchronize(c->markers[15]);
     float ms = 0.0f;
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->markers[14], c->markers[15]);
-    hipFree(a);
-    hipFree(b);
+    (void)hipFree(a);
+    (void)hipFree(b);
     if (e != hipSuccess) { set_error(hipGetErrorString(e)); return RRH_E_HIP; }
     *gbps = (float)(2.0 * (double)bytes * iterations / (ms * 1e-3) / 1e9);
     return RRH_OK;
@@ -644,14 +644,14 @@ int rrh_trace_buffers(rrh_ctx* c, uint32_t n, void** d_rays, void** d_hits) {
     RRH_ARG(c && d_rays && d_hits && n > 0, "rrh_trace_buffers: bad argument");
     RRH_HIP(hipMalloc(d_rays, (size_t)n * 32));
     hipError_t e = hipMalloc(d_hits, (size_t)n * 16);
-    if (e != hipSuccess) { hipFree(*d_rays); set_error(hipGetErrorString(e)); return RRH_E_NOMEM; }
+    if (e != hipSuccess) { (void)hipFree(*d_rays); set_error(hipGetErrorString(e)); return RRH_E_NOMEM; }
     return RRH_OK;
 }
 int rrh_trace_buffers_free(rrh_ctx* c, void* d_rays, void* d_hits) {
     RRH_ARG(c, "null context");
-    hipStreamSynchronize(c->stream);
-    if (d_rays) hipFree(d_rays);
-    if (d_hits) hipFree(d_hits);
+    (void)hipStreamSynchronize(c->stream);
+    if (d_rays) (void)hipFree(d_rays);
+    if (d_hits) (void)hipFree(d_hits);
     return RRH_OK;
 }
 
@@ -669,8 +669,8 @@ int rrh_trace_closest(rrh_ctx* c, const rrh_scene* s, const rrh_ray* rays, uint3
     }
     if (e == hipSuccess) e = hipMemcpyAsync(hits, dh, (size_t)n * 16, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(dr);
-    hipFree(dh);
+    (void)hipFree(dr);
+    (void)hipFree(dh);
     if (e != hipSuccess) { set_error(std::string("rrh_trace_closest: ") + hipGetErrorString(e)); return RRH_E_HIP; }
     return RRH_OK;
 }
@@ -689,8 +689,8 @@ int rrh_trace_occluded(rrh_ctx* c, const rrh_scene* s, const rrh_ray* rays, uint
     }
     if (e == hipSuccess) e = hipMemcpyAsync(occ, dh, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(dr);
-    hipFree(dh);
+    (void)hipFree(dr);
+    (void)hipFree(dh);
     if (e != hipSuccess) { set_error(std::string("rrh_trace_occluded: ") + hipGetErrorString(e)); return RRH_E_HIP; }
     return RRH_OK;
 }

@@ -388,7 +388,7 @@ int rrh_grid_splat(rrh_ctx* c, rrh_grid* g, const float* pos, const uint32_t* bs
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) { set_error(hipGetErrorString(e)); rc = RRH_E_HIP; }
     }
-    hipFree(dp); hipFree(db); hipFree(dd);
+    (void)hipFree(dp); (void)hipFree(db); (void)hipFree(dd);
     return rc;
 }
 
@@ -414,7 +414,7 @@ int rrh_grid_sample_primary(rrh_ctx* c, rrh_grid* g, const float* pos, const flo
     if (!rc) rc = upload_tmp(c, bsdf, (size_t)n * 4, &db);
     if (!rc) rc = upload_tmp(c, seeds, (size_t)n * 4, &ds);
     if (!rc) rc = upload_tmp(c, nullptr, 0, &dd);
-    if (!rc) { hipFree(dd); rc = (hipMalloc(&dd, (size_t)n * 12) == hipSuccess) ? RRH_OK : RRH_E_NOMEM; }
+    if (!rc) { (void)hipFree(dd); rc = (hipMalloc(&dd, (size_t)n * 12) == hipSuccess) ? RRH_OK : RRH_E_NOMEM; }
     if (!rc) rc = (hipMalloc(&dq, (size_t)n * 4) == hipSuccess) ? RRH_OK : RRH_E_NOMEM;
     if (!rc) {
         hipLaunchKernelGGL(k_hash_sample, dim3((n + 255) / 256), dim3(256), 0, c->stream, (const float4*)g->fitted.p,
@@ -428,7 +428,7 @@ int rrh_grid_sample_primary(rrh_ctx* c, rrh_grid* g, const float* pos, const flo
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) { set_error(hipGetErrorString(e)); rc = RRH_E_HIP; }
     }
-    hipFree(dp); hipFree(dn); hipFree(db); hipFree(ds); hipFree(dd); hipFree(dq);
+    (void)hipFree(dp); (void)hipFree(dn); (void)hipFree(db); (void)hipFree(ds); (void)hipFree(dd); (void)hipFree(dq);
     return rc;
 }
 
@@ -447,7 +447,7 @@ int rrh_grid_em_step(rrh_ctx* c, rrh_grid* g, const float* pos, const float* dir
     }
     if (!rc) rc = vmm_mstep(g);
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) { set_error("rrh_grid_em_step: sync failed"); rc = RRH_E_HIP; }
-    hipFree(dp); hipFree(dd); hipFree(dw);
+    (void)hipFree(dp); (void)hipFree(dd); (void)hipFree(dw);
     return rc;
 }
 

@@ -710,7 +710,7 @@ uint32_t infer_grid(rrh_ctx* c, rrh_mlp* m, K kernel, uint32_t n) {
 
 struct HostTmp {
     void* p[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    ~HostTmp() { for (void* q : p) if (q) hipFree(q); }
+    ~HostTmp() { for (void* q : p) if (q) (void)hipFree(q); }
 };
 int upload_samples(rrh_ctx* c, const float* pos, const float* nrm, const float* wo, const float* dir, const float* w,
                    uint32_t n, HostTmp& T, HostSrc& src) {
@@ -822,7 +822,7 @@ int rrh_mlp_infer(rrh_ctx* c, rrh_mlp* m, const float* pos, const float* nrm, co
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(lobes, dl, (size_t)n * NK * 5 * 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(dl);
+    (void)hipFree(dl);
     if (e != hipSuccess) { set_error(std::string("rrh_mlp_infer: ") + hipGetErrorString(e)); return RRH_E_HIP; }
     return RRH_OK;
 }
