@@ -85,3 +85,58 @@ def test_mlp_guided_render_is_unbiased_and_training_lowers_the_loss(ctx, unguide
     assert m.state()[0] >= 55
     got = _frames(ctx, gs, 256, 9000, mode=L.GUIDE_MLP, alpha=0.5, mlp=m)
     _assert_same_mean(got, ref, "mlp")
+
+
+def _noise(ctx, gs, w, h, n, seed0, bounces, **kw):
+    """Per-sample noise: mean over pixels of the variance over n frames / (mean^2 + 0.01), and
+    the mean image."""
+    img = core.Image(ctx, w, h)
+    acc, acc2 = np.zeros((h, w, 3)), np.zeros((h, w, 3))
+    for k in range(n):
+        p = L.PTParams(bounces, seed0 + k, 3, kw.get("mode", L.GUIDE_NONE), kw.get("alpha", 0.0), 0)
+        L.call("rrh_pt_render_guided", ctx.h, gs.h, C.byref(p), kw["grid"].h if kw.get("grid") else None,
+               kw["mlp"].h if kw.get("mlp") else None, img.h)
+        r = img.download()[..., :3].astype(np.float64)
+        acc += r
+        acc2 += r * r
+    m = acc / n
+    var = (acc2 - n * m * m) / (n - 1)
+    return float(np.mean(var / (m * m + 0.01))), m
+
+
+def test_trained_guides_lower_the_noise_of_the_window_lit_room(ctx):
+    """What the trained guides gain over the tracer without them: in the indoor room lit only
+    through a small window, one-sample MIS with the online-EM vMF grid or the trained MLP gives
+    clearly less variance per sample than BSDF sampling alone (and the same mean).  Both guides
+    train online for 48 frames at 256x192 (about 0.4 M path vertices per frame for the grid, one
+    65,536-vertex Adam step per frame for the MLP); with the 64x48 evaluation extent alone they
+    see too few paths and a poorly fitted guide adds variance.  Measured: unguided 1.098, grid
+    0.743, MLP 0.864."""
+    W2, H2, B = 64, 48, 8
+    sa = core.generate_scene(L.GEN_INDOOR, 1, 0.25)
+    gs = core.Scene(ctx, sa)
+    info = gs.info()
+    grid = core.Grid(ctx, L.GridParams(L.GRID_VMM, 0, 0, 0, 0, 512.0, 32, 8, L.f3(info["bounds_min"]),
+                                       L.f3(info["bounds_max"]), 0.95, 0.0))
+    mlp = core.Mlp(ctx, info["bounds_min"], info["bounds_max"], lr=3e-3, seed=1)
+    # online training from the tracer's own vertices, at a larger extent (more paths per frame)
+    TW, TH = 256, 192
+    gs.set_camera(core.camera_look_at(sa.eye, sa.center, sa.fov, TW, TH))
+    timg = core.Image(ctx, TW, TH)
+    for k in range(48):
+        a = 0.5 if k else 0.0
+        p = L.PTParams(B, 100 + k, 3, L.GUIDE_VMM, a, 400000)
+        L.call("rrh_pt_render_guided", ctx.h, gs.h, C.byref(p), grid.h, None, timg.h)
+        L.call("rrh_grid_em_from_vertices", ctx.h, grid.h)
+        p = L.PTParams(B, 300 + k, 3, L.GUIDE_MLP, a, 65536)
+        L.call("rrh_pt_render_guided", ctx.h, gs.h, C.byref(p), None, mlp.h, timg.h)
+        mlp.train_from_vertices(65536)
+    gs.set_camera(core.camera_look_at(sa.eye, sa.center, sa.fov, W2, H2))
+    n = 256
+    v_none, m_none = _noise(ctx, gs, W2, H2, n, 10000, B)
+    v_vmm, m_vmm = _noise(ctx, gs, W2, H2, n, 20000, B, mode=L.GUIDE_VMM, alpha=0.5, grid=grid)
+    v_mlp, m_mlp = _noise(ctx, gs, W2, H2, n, 30000, B, mode=L.GUIDE_MLP, alpha=0.5, mlp=mlp)
+    print(f"per-sample noise: unguided {v_none:.3f}, vMF grid {v_vmm:.3f}, MLP {v_mlp:.3f}")
+    assert v_vmm < 0.85 * v_none and v_mlp < 0.85 * v_none
+    for m in (m_vmm, m_mlp):  # the same image (within noise): guiding changes the variance, not the mean
+        assert abs(m.mean() / m_none.mean() - 1) < 0.1
