@@ -347,8 +347,13 @@ typedef struct {
     float sigma_position;       /* reference: .25 ; config: sigma_depth 1.0 */
     float sigma_normal;         /* reference: .2  ; config: normal exponent 128 */
     float sigma_luminance;      /* reference: 1.0 ; config: sigma_color 4.0 (variance-scaled) */
-    uint32_t _pad[3];
+    uint32_t flags;             /* RRH_ATROUS_FAST_MATH; 0 = exact */
+    uint32_t _pad[2];
 } rrh_atrous_params;
+/* hardware exp / log / reciprocal / rsqrt in the per-tap weights instead of the correctly
+ * rounded forms, within 1e-3 per pixel of the exact filter (no reference
+ * counterpart; the reference's shader math is the GPU driver's approximate exp / divide) */
+#define RRH_ATROUS_FAST_MATH 1u
 /* runs `iterations` passes with stride 1<<i ping-ponging between tmp0/tmp1;
  * *out_slot receives the slot holding the result ((iterations-1)&1, AtrousPass.cpp:197-201).
  * Reference mode: gbuf_a = positions (w<0 miss), gbuf_b = normals.  Config mode: gbuf_a =

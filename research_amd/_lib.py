@@ -45,6 +45,7 @@ NODES_BINARY, NODES_WIDE4, NODES_WIDE4Q = 0, 1, 2
 STACK_PRIVATE, STACK_LDS32, STACK_HYBRID16, STACK_HYBRID32 = 0, 1, 2, 3
 GRID_HASH, GRID_VMM = 0, 1
 ATROUS_REFERENCE, ATROUS_CONFIG = 0, 1
+ATROUS_FAST_MATH = 1
 
 
 class Camera(C.Structure):
@@ -102,7 +103,8 @@ class RGIParams(C.Structure):
 
 class AtrousParams(C.Structure):
     _fields_ = [("iterations", C.c_uint32), ("mode", C.c_uint32), ("sigma_position", C.c_float),
-                ("sigma_normal", C.c_float), ("sigma_luminance", C.c_float), ("_pad", C.c_uint32 * 3)]
+                ("sigma_normal", C.c_float), ("sigma_luminance", C.c_float), ("flags", C.c_uint32),
+                ("_pad", C.c_uint32 * 2)]
 
 
 class GridParams(C.Structure):

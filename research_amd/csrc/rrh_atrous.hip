@@ -10,9 +10,11 @@
 // alpha, normal term max(0, n.n)^sigma_n, depth term against the screen-space
 // depth gradient; the variance is filtered with squared weights.
 //
-// Both are HBM streams: each output pixel reads 25 taps whose footprint is shared
-// with its neighbours, so a 2D tile of outputs per workgroup keeps the taps in L2;
-// for strides 1 and 2 the tile plus halo is staged in LDS first.
+// Each output pixel reads 25 taps whose footprint is shared with its neighbours: up to
+// stride 4 every workgroup stages one window of its stride's sub-lattice in LDS (1.56 image
+// reads per output) and the taps read LDS; at larger strides a dense tile reads its taps
+// through L2.  With the exact transcendentals the per-tap arithmetic, not the memory
+// traffic, bounds the kernels.
 #include <hip/hip_fp16.h>
 
 #include "rrh_internal.h"
@@ -29,90 +31,113 @@ __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? l
 
 struct RefPx { f3 pos; float w; f3 nrm; f3 rgb; };
 
-// ---- reference mode -----------------------------------------------------------------
-// LDS-staged variant: the (TX + 4s) x (TY + 4s) window of position/normal/radiance
-// around the tile is loaded once (clamped to the image), then all 25 taps read LDS.
-template <int S>
-__global__ void __launch_bounds__(TX* TY) k_atrous_ref_lds(int W, int H, float sps, float sns, float sls,
-                                                          const float4* __restrict__ in, const float4* __restrict__ gpos,
-                                                          const float4* __restrict__ gnrm, float4* __restrict__ out) {
-    constexpr int HW = TX + 4 * S, HH = TY + 4 * S;
-    __shared__ float4 s_pos[HH * HW];
-    __shared__ float4 s_nrm[HH * HW];
-    __shared__ float4 s_rad[HH * HW];
-    const int x0 = blockIdx.x * TX - 2 * S, y0 = blockIdx.y * TY - 2 * S;
-    for (int k = threadIdx.x; k < HW * HH; k += TX * TY) {
-        int lx = k % HW, ly = k / HW;
-        int gx = clampi(x0 + lx, 0, W - 1), gy = clampi(y0 + ly, 0, H - 1);
-        size_t g = (size_t)gy * W + gx;
-        s_pos[k] = gpos[g];
-        s_nrm[k] = gnrm[g];
-        s_rad[k] = in[g];
+// Per-tap weight arithmetic.  Exact: correctly rounded division and square root and the
+// library's own exp / log, bit-equal to the CPU restatement.  Fast: the hardware
+// approximations (v_exp_f32, v_log_f32, v_rcp_f32, v_rsq_f32, about 1 ulp each); the taps'
+// weights then differ in the last bits and the filtered pixel by far less than 1e-3.
+template <bool FAST> struct WMath;
+template <> struct WMath<false> {
+    static __device__ __forceinline__ float ex(float x) { return exp_rr(x); }
+    static __device__ __forceinline__ float div(float a, float b) { return a / b; }
+    static __device__ __forceinline__ f3 unit(f3 a) { return normalize(a); }
+    static __device__ __forceinline__ float powp(float x, float e) { return (x > 0.0f) ? exp_rr(e * log_rr(x)) : 0.0f; }
+};
+template <> struct WMath<true> {
+    static __device__ __forceinline__ float ex(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504f); }
+    static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    static __device__ __forceinline__ f3 unit(f3 a) { return scale(a, __builtin_amdgcn_rsqf(dot(a, a))); }
+    static __device__ __forceinline__ float powp(float x, float e) {
+        return (x > 0.0f) ? __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(x)) : 0.0f;
     }
-    __syncthreads();
+};
+
+// ---- sub-lattice tiling ---------------------------------------------------------------
+// A workgroup's outputs are a TX x TY tile of the stride-S sub-lattice with residue (rx, ry):
+// pixel (rx + S (lx0 + tx), ry + S (ly0 + ty)).  Every tap of those outputs lies on the same
+// sub-lattice, clamping at the image edge included (lattice index k holds the image pixel
+// clamp(rx + S k)), so the (TX + 4) x (TY + 4) lattice window is staged in LDS once (1.56
+// loads per output) and all 25 taps read LDS.  The residue varies fastest
+// over blockIdx.x, so the S x S workgroups of one lattice tile cover one dense image region.
+constexpr int LW = TX + 4, LH = TY + 4;
+constexpr int kLatMaxStride = 4;  // lattice staging up to this stride, dense tiles above
+
+struct LatTile {
+    int rx, ry, lx0, ly0;
+};
+__device__ __forceinline__ LatTile lat_tile(int S) {
+    LatTile t;
+    t.rx = (int)blockIdx.x % S;
+    t.ry = (int)blockIdx.y % S;
+    t.lx0 = ((int)blockIdx.x / S) * TX;
+    t.ly0 = ((int)blockIdx.y / S) * TY;
+    return t;
+}
+// image offset of window slot k (clamped to the image, as the taps are)
+__device__ __forceinline__ size_t lat_src(const LatTile& t, int S, int W, int H, int k) {
+    const int lx = k % LW, ly = k / LW;
+    const int gx = clampi(t.rx + S * (t.lx0 - 2 + lx), 0, W - 1);
+    const int gy = clampi(t.ry + S * (t.ly0 - 2 + ly), 0, H - 1);
+    return (size_t)gy * W + gx;
+}
+
+// ---- reference mode -----------------------------------------------------------------
+// LAT: the taps come from the staged sub-lattice window; otherwise every tap is read from
+// global memory (through L2) for a dense TX x TY tile of pixels, which is faster at strides of
+// 8 and more, where the lattice window's image reads no longer share cache lines.
+template <bool FAST, bool LAT>
+__global__ void __launch_bounds__(TX* TY) k_atrous_ref(int W, int H, int S, float sps, float sns, float sls,
+                                                      const float4* __restrict__ in, const float4* __restrict__ gpos,
+                                                      const float4* __restrict__ gnrm, float4* __restrict__ out) {
+    __shared__ float4 s_pos[LAT ? LW * LH : 1];
+    __shared__ float4 s_nrm[LAT ? LW * LH : 1];
+    __shared__ float4 s_rad[LAT ? LW * LH : 1];
+    const LatTile t = LAT ? lat_tile(S) : LatTile{0, 0, 0, 0};
+    if constexpr (LAT) {
+        for (int k = threadIdx.x; k < LW * LH; k += TX * TY) {
+            const size_t g = lat_src(t, S, W, H, k);
+            s_pos[k] = gpos[g];
+            s_nrm[k] = gnrm[g];
+            s_rad[k] = in[g];
+        }
+        __syncthreads();
+    }
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    const int x = blockIdx.x * TX + tx, y = blockIdx.y * TY + ty;
+    const int x = LAT ? t.rx + S * (t.lx0 + tx) : (int)blockIdx.x * TX + tx;
+    const int y = LAT ? t.ry + S * (t.ly0 + ty) : (int)blockIdx.y * TY + ty;
     if (x >= W || y >= H) return;
-    const int lc = (ty + 2 * S) * HW + (tx + 2 * S);
-    float4 cp = s_pos[lc], cr = s_rad[lc];
+    const int lc = (ty + 2) * LW + (tx + 2);
     size_t i = (size_t)y * W + x;
+    // tap (dx, dy) in lattice steps: its position, normal and radiance
+    auto tap = [&](int dx, int dy, float4& p, float4& n, float4& r) {
+        if constexpr (LAT) {
+            const int l = lc + dy * LW + dx;
+            p = s_pos[l]; n = s_nrm[l]; r = s_rad[l];
+        } else {
+            const size_t j = (size_t)clampi(y + dy * S, 0, H - 1) * W + clampi(x + dx * S, 0, W - 1);
+            p = gpos[j]; n = gnrm[j]; r = in[j];
+        }
+    };
+    float4 cp, cn4, cr;
+    tap(0, 0, cp, cn4, cr);
     if (cp.w < 0.0f) { out[i] = cr; return; }
-    f3 cn = normalize(xyz(s_nrm[lc]));
+    f3 cn = WMath<FAST>::unit(xyz(cn4));
     float cl = luma601(xyz(cr));
     f3 acc = mk(0, 0, 0);
     float wsum = 0.0f;
     for (int ty2 = -2; ty2 <= 2; ++ty2) {
         for (int tx2 = -2; tx2 <= 2; ++tx2) {
-            // clamp in image space, then map into the staged window
-            int sx = clampi(x + tx2 * S, 0, W - 1), sy = clampi(y + ty2 * S, 0, H - 1);
-            int l = (sy - y0) * HW + (sx - x0);
-            float4 sp = s_pos[l];
+            float4 sp, sn4, sr4;
+            tap(tx2, ty2, sp, sn4, sr4);
             if (sp.w < 0.0f) continue;
-            f3 sr = xyz(s_rad[l]);
+            f3 sr = xyz(sr4);
             float bw = (kw(tx2) * kw(ty2)) / 256.0f;
             f3 pd = sub(xyz(sp), xyz(cp));
-            float pw = exp_rr(-dot(pd, pd) / sps);
-            f3 sn = normalize(xyz(s_nrm[l]));
+            float pw = WMath<FAST>::ex(-WMath<FAST>::div(dot(pd, pd), sps));
+            f3 sn = WMath<FAST>::unit(xyz(sn4));
             float nd = fmaxf(0.0f, 1.0f - dot(cn, sn));
-            float nw = exp_rr(-nd / sns);
+            float nw = WMath<FAST>::ex(-WMath<FAST>::div(nd, sns));
             float ld = luma601(sr) - cl;
-            float lw = exp_rr(-(ld * ld) / sls);
-            float w = bw * pw * nw * lw;
-            acc = add(acc, scale(sr, w));
-            wsum += w;
-        }
-    }
-    f3 r = (wsum > 0.0f) ? divs(acc, wsum) : xyz(cr);
-    out[i] = make_float4(r.x, r.y, r.z, cr.w);
-}
-
-__global__ void __launch_bounds__(TX* TY) k_atrous_ref(int W, int H, int st, float sps, float sns, float sls,
-                                                      const float4* __restrict__ in, const float4* __restrict__ gpos,
-                                                      const float4* __restrict__ gnrm, float4* __restrict__ out) {
-    const int x = blockIdx.x * TX + (int)(threadIdx.x % TX), y = blockIdx.y * TY + (int)(threadIdx.x / TX);
-    if (x >= W || y >= H) return;
-    size_t i = (size_t)y * W + x;
-    float4 cp = gpos[i], cr = in[i];
-    if (cp.w < 0.0f) { out[i] = cr; return; }
-    f3 cn = normalize(xyz(gnrm[i]));
-    float cl = luma601(xyz(cr));
-    f3 acc = mk(0, 0, 0);
-    float wsum = 0.0f;
-    for (int ty = -2; ty <= 2; ++ty) {
-        for (int tx = -2; tx <= 2; ++tx) {
-            int sx = clampi(x + tx * st, 0, W - 1), sy = clampi(y + ty * st, 0, H - 1);
-            size_t j = (size_t)sy * W + sx;
-            float4 sp = gpos[j];
-            if (sp.w < 0.0f) continue;
-            f3 sr = xyz(in[j]);
-            float bw = (kw(tx) * kw(ty)) / 256.0f;
-            f3 pd = sub(xyz(sp), xyz(cp));
-            float pw = exp_rr(-dot(pd, pd) / sps);
-            f3 sn = normalize(xyz(gnrm[j]));
-            float nd = fmaxf(0.0f, 1.0f - dot(cn, sn));
-            float nw = exp_rr(-nd / sns);
-            float ld = luma601(sr) - cl;
-            float lw = exp_rr(-(ld * ld) / sls);
+            float lw = WMath<FAST>::ex(-WMath<FAST>::div(ld * ld, sls));
             float w = bw * pw * nw * lw;
             acc = add(acc, scale(sr, w));
             wsum += w;
@@ -128,19 +153,47 @@ __device__ __forceinline__ unsigned short f2h(float f) { return __half_as_ushort
 
 struct H4 { unsigned short x, y, z, w; };
 __device__ __forceinline__ H4 ldh(const ushort4* p, size_t i) { ushort4 v = p[i]; return H4{v.x, v.y, v.z, v.w}; }
-__device__ __forceinline__ float powpos(float x, float e) { return (x > 0.0f) ? exp_rr(e * log_rr(x)) : 0.0f; }
+__device__ __forceinline__ H4 h4(ushort4 v) { return H4{v.x, v.y, v.z, v.w}; }
 
-__global__ void __launch_bounds__(TX* TY) k_atrous_cfg(int W, int H, int st, float sz, float sn, float sl,
+// The taps come from the staged lattice window; the centre's 3x3 variance and its depth
+// gradient (stride 1, off the lattice) are read from global memory.
+template <bool FAST, bool LAT>
+__global__ void __launch_bounds__(TX* TY) k_atrous_cfg(int W, int H, int S, float sz, float sn, float sl,
                                                       const ushort4* __restrict__ in, const ushort4* __restrict__ gb,
                                                       ushort4* __restrict__ out) {
-    const int x = blockIdx.x * TX + (int)(threadIdx.x % TX), y = blockIdx.y * TY + (int)(threadIdx.x / TX);
+    __shared__ ushort4 s_gb[LAT ? LW * LH : 1];
+    __shared__ ushort4 s_in[LAT ? LW * LH : 1];
+    const LatTile t = LAT ? lat_tile(S) : LatTile{0, 0, 0, 0};
+    if constexpr (LAT) {
+        for (int k = threadIdx.x; k < LW * LH; k += TX * TY) {
+            const size_t g = lat_src(t, S, W, H, k);
+            s_gb[k] = gb[g];
+            s_in[k] = in[g];
+        }
+        __syncthreads();
+    }
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int x = LAT ? t.rx + S * (t.lx0 + tx) : (int)blockIdx.x * TX + tx;
+    const int y = LAT ? t.ry + S * (t.ly0 + ty) : (int)blockIdx.y * TY + ty;
     if (x >= W || y >= H) return;
+    const int lc = (ty + 2) * LW + (tx + 2);
     size_t i = (size_t)y * W + x;
-    H4 cg = ldh(gb, i);
+    auto tap = [&](int dx, int dy, ushort4& g, ushort4& r) {
+        if constexpr (LAT) {
+            const int l = lc + dy * LW + dx;
+            g = s_gb[l]; r = s_in[l];
+        } else {
+            const size_t j = (size_t)clampi(y + dy * S, 0, H - 1) * W + clampi(x + dx * S, 0, W - 1);
+            g = gb[j]; r = in[j];
+        }
+    };
+    ushort4 cg4, ci4;
+    tap(0, 0, cg4, ci4);
+    H4 cg = h4(cg4);
     float zp = h2f(cg.w);
-    if (!(zp > 0.0f)) { out[i] = in[i]; return; }
+    if (!(zp > 0.0f)) { out[i] = ci4; return; }
     f3 np = mk(h2f(cg.x), h2f(cg.y), h2f(cg.z));
-    H4 ci = ldh(in, i);
+    H4 ci = h4(ci4);
     float lp = lum709(mk(h2f(ci.x), h2f(ci.y), h2f(ci.z)));
     const float gk[3] = {0.25f, 0.5f, 0.25f};
     float var = 0.0f;
@@ -156,22 +209,22 @@ __global__ void __launch_bounds__(TX* TY) k_atrous_cfg(int W, int H, int st, flo
     float dzdy = (h2f(gb[(size_t)yp * W + x].w) - h2f(gb[(size_t)ym * W + x].w)) * 0.5f;
     f3 acc = mk(0, 0, 0);
     float accv = 0.0f, wsum = 0.0f;
-    for (int ty = -2; ty <= 2; ++ty) {
-        for (int tx = -2; tx <= 2; ++tx) {
-            int sx = clampi(x + tx * st, 0, W - 1), sy = clampi(y + ty * st, 0, H - 1);
-            size_t j = (size_t)sy * W + sx;
-            H4 sg = ldh(gb, j);
+    for (int ty2 = -2; ty2 <= 2; ++ty2) {
+        for (int tx2 = -2; tx2 <= 2; ++tx2) {
+            ushort4 sg4, si4;
+            tap(tx2, ty2, sg4, si4);
+            H4 sg = h4(sg4);
             float zq = h2f(sg.w);
             if (!(zq > 0.0f)) continue;
             f3 nq = mk(h2f(sg.x), h2f(sg.y), h2f(sg.z));
-            H4 si = ldh(in, j);
+            H4 si = h4(si4);
             f3 cq = mk(h2f(si.x), h2f(si.y), h2f(si.z));
             float vq = h2f(si.w);
-            float bw = (kw(tx) * kw(ty)) / 256.0f;
-            float wz = fabsf(zp - zq) / (sz * fabsf(dzdx * (float)(tx * st) + dzdy * (float)(ty * st)) + 1e-3f);
-            float wn = powpos(fmaxf(0.0f, dot(np, nq)), sn);
-            float wl = fabsf(lp - lum709(cq)) / phil;
-            float w = bw * exp_rr(0.0f - wl - wz) * wn;
+            float bw = (kw(tx2) * kw(ty2)) / 256.0f;
+            float wz = WMath<FAST>::div(fabsf(zp - zq), sz * fabsf(dzdx * (float)(tx2 * S) + dzdy * (float)(ty2 * S)) + 1e-3f);
+            float wn = WMath<FAST>::powp(fmaxf(0.0f, dot(np, nq)), sn);
+            float wl = WMath<FAST>::div(fabsf(lp - lum709(cq)), phil);
+            float w = bw * WMath<FAST>::ex(0.0f - wl - wz) * wn;
             acc = add(acc, scale(cq, w));
             accv += vq * (w * w);
             wsum += w;
@@ -182,7 +235,7 @@ __global__ void __launch_bounds__(TX* TY) k_atrous_cfg(int W, int H, int st, flo
         o.x = f2h(acc.x / wsum); o.y = f2h(acc.y / wsum); o.z = f2h(acc.z / wsum); o.w = f2h(accv / (wsum * wsum));
         out[i] = o;
     } else {
-        out[i] = in[i];
+        out[i] = ci4;
     }
 }
 
@@ -216,6 +269,12 @@ __global__ void __launch_bounds__(TX* TY) k_atrous_prepare(int W, int H, f3 cpos
     gb16[i] = g;
 }
 
+// S x S workgroups per TX x TY tile of each residue's sub-lattice
+dim3 lat_grid(uint32_t W, uint32_t H, int S) {
+    const uint32_t lw = (W + S - 1) / S, lh = (H + S - 1) / S;
+    return dim3(S * ((lw + TX - 1) / TX), S * ((lh + TY - 1) / TY));
+}
+
 }  // namespace
 
 extern "C" {
@@ -224,10 +283,12 @@ int rrh_atrous(rrh_ctx* c, const rrh_atrous_params* p, const rrh_image* input, c
                rrh_image* t0, rrh_image* t1, int* out_slot) {
     RRH_ARG(c && p && input && ga && t0 && t1 && out_slot, "rrh_atrous: null argument");
     RRH_ARG(p->mode == RRH_ATROUS_REFERENCE || p->mode == RRH_ATROUS_CONFIG, "rrh_atrous: bad mode");
+    RRH_ARG((p->flags & ~RRH_ATROUS_FAST_MATH) == 0u, "rrh_atrous: unknown flags");
+    const bool fast = (p->flags & RRH_ATROUS_FAST_MATH) != 0u;
     uint32_t W = input->w, H = input->h;
     RRH_ARG(ga->w == W && ga->h == H && t0->w == W && t0->h == H && t1->w == W && t1->h == H, "rrh_atrous: extent mismatch");
     uint32_t it = p->iterations < 1 ? 1 : (p->iterations > 7 ? 7 : p->iterations);
-    dim3 grid((W + TX - 1) / TX, (H + TY - 1) / TY), block(TX * TY);
+    const dim3 block(TX * TY), dense((W + TX - 1) / TX, (H + TY - 1) / TY);
     rrh_image* slots[2] = {t0, t1};
     if (p->mode == RRH_ATROUS_REFERENCE) {
         RRH_ARG(gbb && gbb->w == W && gbb->h == H, "rrh_atrous: reference mode needs position and normal images");
@@ -239,16 +300,12 @@ int rrh_atrous(rrh_ctx* c, const rrh_atrous_params* p, const rrh_image* input, c
         for (uint32_t k = 0; k < it; ++k) {
             const float4* src = (const float4*)(k == 0 ? input->p : slots[(k - 1) & 1]->p);
             float4* dst = (float4*)slots[k & 1]->p;
-            int st = 1 << k;
-            if (st == 1)
-                hipLaunchKernelGGL(k_atrous_ref_lds<1>, grid, block, 0, c->stream, (int)W, (int)H, sps, sns, sls, src,
-                                   (const float4*)ga->p, (const float4*)gbb->p, dst);
-            else if (st == 2)
-                hipLaunchKernelGGL(k_atrous_ref_lds<2>, grid, block, 0, c->stream, (int)W, (int)H, sps, sns, sls, src,
-                                   (const float4*)ga->p, (const float4*)gbb->p, dst);
-            else
-                hipLaunchKernelGGL(k_atrous_ref, grid, block, 0, c->stream, (int)W, (int)H, st, sps, sns, sls, src,
-                                   (const float4*)ga->p, (const float4*)gbb->p, dst);
+            const int S = 1 << k;
+            const bool lat = S <= kLatMaxStride;
+            auto kern = fast ? (lat ? k_atrous_ref<true, true> : k_atrous_ref<true, false>)
+                             : (lat ? k_atrous_ref<false, true> : k_atrous_ref<false, false>);
+            hipLaunchKernelGGL(kern, lat ? lat_grid(W, H, S) : dense, block, 0, c->stream, (int)W, (int)H, S, sps, sns, sls,
+                               src, (const float4*)ga->p, (const float4*)gbb->p, dst);
             RRH_LAUNCH_CHECK();
         }
     } else {
@@ -257,7 +314,11 @@ int rrh_atrous(rrh_ctx* c, const rrh_atrous_params* p, const rrh_image* input, c
         for (uint32_t k = 0; k < it; ++k) {
             const ushort4* src = (const ushort4*)(k == 0 ? input->p : slots[(k - 1) & 1]->p);
             ushort4* dst = (ushort4*)slots[k & 1]->p;
-            hipLaunchKernelGGL(k_atrous_cfg, grid, block, 0, c->stream, (int)W, (int)H, 1 << k, p->sigma_position,
+            const int S = 1 << k;
+            const bool lat = S <= kLatMaxStride;
+            auto kern = fast ? (lat ? k_atrous_cfg<true, true> : k_atrous_cfg<true, false>)
+                             : (lat ? k_atrous_cfg<false, true> : k_atrous_cfg<false, false>);
+            hipLaunchKernelGGL(kern, lat ? lat_grid(W, H, S) : dense, block, 0, c->stream, (int)W, (int)H, S, p->sigma_position,
                                p->sigma_normal, p->sigma_luminance, src, (const ushort4*)ga->p, dst);
             RRH_LAUNCH_CHECK();
         }

@@ -87,6 +87,7 @@ PYBIND11_MODULE(_app, m) {
         .def_readwrite("sigma_position", &AtrousPass::sigma_position)
         .def_readwrite("sigma_normal", &AtrousPass::sigma_normal)
         .def_readwrite("sigma_luminance", &AtrousPass::sigma_luminance)
+        .def_readwrite("fast_math", &AtrousPass::fast_math)
         .def("is_enabled", &AtrousPass::is_enabled);
     py::class_<GuidingPass>(m, "GuidingPass")
         .def_property(

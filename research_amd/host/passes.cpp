@@ -472,6 +472,7 @@ void AtrousPass::execute(FrameContext&) {
     p.sigma_position = sigma_position;
     p.sigma_normal = sigma_normal;
     p.sigma_luminance = sigma_luminance;
+    p.flags = fast_math ? RRH_ATROUS_FAST_MATH : 0u;
     int slot = 0;
     check(rrh_atrous(device_->ctx(), &p, input_->handle(), gbuf_pos_->handle(), gbuf_norm_->handle(),
                      ping_images_[0].handle(), ping_images_[1].handle(), &slot),

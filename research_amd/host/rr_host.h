@@ -472,6 +472,7 @@ public:
     float sigma_position = 0.25f;
     float sigma_normal = 0.20f;
     float sigma_luminance = 1.00f;
+    bool fast_math = false;  // hardware exp / reciprocal in the weights (within 1e-3 of exact)
 
 private:
     rhi::Device* device_ = nullptr;

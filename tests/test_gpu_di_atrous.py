@@ -118,3 +118,56 @@ def test_atrous_config_mode_matches_oracle(ctx):
     of = o.view(np.float16).astype(np.float32)
     assert np.abs(gf - of).max() <= 1e-3 * max(1.0, np.abs(of).max())
     assert (g == o).all(axis=2).mean() >= 0.999
+
+
+def test_atrous_fast_math_within_tolerance_of_the_exact_filter(ctx):
+    """RRH_ATROUS_FAST_MATH (hardware exp / log / reciprocal / rsqrt in the weights) against the
+    exact oracle: per pixel within 1e-3 relative (absolute below 1), plus one fp16 step of the
+    output format in config mode, at 1, 3 and 5 iterations (strides up to 16)."""
+    H, W = 72, 136
+    rad, pos, nrm = _ref_inputs(H, W, 2)
+    imgs = [core.Image(ctx, W, H) for _ in range(5)]
+    for im, a in zip(imgs[:3], (rad, pos, nrm)):
+        im.upload(a)
+    for iters in (1, 3, 5):
+        prm = L.AtrousParams(iters, L.ATROUS_REFERENCE, 0.25, 0.2, 1.0, L.ATROUS_FAST_MATH)
+        slot = C.c_int()
+        L.call("rrh_atrous", ctx.h, C.byref(prm), imgs[0].h, imgs[1].h, imgs[2].h, imgs[3].h, imgs[4].h,
+               C.byref(slot))
+        g = imgs[3 + slot.value].download()
+        o = rad
+        for k in range(iters):
+            o = O.atrous_ref(o, pos, nrm, 1 << k)
+        assert np.isfinite(g).all()
+        assert (np.abs(g - o) <= 1e-3 * np.maximum(1.0, np.abs(o))).all()
+
+    rad16 = rad.astype(np.float16).view(np.uint16).copy()
+    rad16[..., 3] = np.float16(0.05).view(np.uint16)
+    gb = np.zeros((H, W, 4), np.float32)
+    gb[..., :3] = nrm[..., :3]
+    gb[..., 3] = np.where(pos[..., 3] < 0, 0.0, 2.0 + pos[..., 2])
+    gb16 = gb.astype(np.float16).view(np.uint16).copy()
+    i_in, i_gb, t0, t1 = (core.Image(ctx, W, H, L.FMT_RGBA16F) for _ in range(4))
+    i_in.upload(rad16)
+    i_gb.upload(gb16)
+    for iters in (1, 5):
+        prm = L.AtrousParams(iters, L.ATROUS_CONFIG, 1.0, 128.0, 4.0, L.ATROUS_FAST_MATH)
+        slot = C.c_int()
+        L.call("rrh_atrous", ctx.h, C.byref(prm), i_in.h, i_gb.h, None, t0.h, t1.h, C.byref(slot))
+        g = (t0, t1)[slot.value].download()
+        o = rad16
+        for k in range(iters):
+            o = O.atrous_cfg(o, gb16, 1 << k)
+        of16 = o.view(np.float16)
+        gf = g.view(np.float16).astype(np.float32)
+        of = of16.astype(np.float32)
+        step = np.spacing(np.abs(of16)).astype(np.float32)
+        assert np.isfinite(gf).all()
+        assert (np.abs(gf - of) <= 1e-3 * np.maximum(1.0, np.abs(of)) + step).all()
+
+
+def test_atrous_rejects_unknown_flags(ctx):
+    im = [core.Image(ctx, 8, 8) for _ in range(5)]
+    prm = L.AtrousParams(1, L.ATROUS_REFERENCE, 0.25, 0.2, 1.0, 2)
+    with pytest.raises(L.RRHError, match="unknown flags"):
+        L.call("rrh_atrous", ctx.h, C.byref(prm), im[0].h, im[1].h, im[2].h, im[3].h, im[4].h, C.byref(C.c_int()))
