@@ -11,7 +11,7 @@
 // depth gradient; the variance is filtered with squared weights.
 //
 // Each output pixel reads 25 taps whose footprint is shared with its neighbours: up to
-// stride 4 every workgroup stages one window of its stride's sub-lattice in LDS (1.56 image
+// stride 8 every workgroup stages one window of its stride's sub-lattice in LDS (1.56 image
 // reads per output) and the taps read LDS; at larger strides a dense tile reads its taps
 // through L2.  With the exact transcendentals the per-tap arithmetic, not the memory
 // traffic, bounds the kernels.
@@ -59,17 +59,38 @@ template <> struct WMath<true> {
 // loads per output) and all 25 taps read LDS.  The residue varies fastest
 // over blockIdx.x, so the S x S workgroups of one lattice tile cover one dense image region.
 constexpr int LW = TX + 4, LH = TY + 4;
-constexpr int kLatMaxStride = 4;  // lattice staging up to this stride, dense tiles above
+#ifndef RRH_ATROUS_LAT_MAX
+#define RRH_ATROUS_LAT_MAX 8
+#endif
+constexpr int kLatMaxStride = RRH_ATROUS_LAT_MAX;  // lattice staging up to this stride, dense tiles above
+
+// XCD-aware workgroup order: the hardware deals linear ids round-robin to the 8 XCDs (each
+// with its own L2).  Runs of kXcdRun consecutive logical workgroups (a row of neighbouring
+// tiles, or the residues of one lattice tile, which share cache lines) are given to one XCD,
+// and the runs rotate over the XCDs so that every XCD still gets a mix of cheap and costly
+// image regions.  A last partial round of runs keeps the hardware order.
+constexpr uint32_t kXcdRun = 16;
+__device__ __forceinline__ void xcd_tile(int& bx, int& by) {
+    const uint32_t nb = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
+    const uint32_t full = nb - nb % (8u * kXcdRun);
+    uint32_t l = b;
+    if (b < full) {
+        const uint32_t x = b & 7u, k = b >> 3;
+        l = ((k / kXcdRun) * 8u + x) * kXcdRun + k % kXcdRun;
+    }
+    bx = (int)(l % gridDim.x);
+    by = (int)(l / gridDim.x);
+}
 
 struct LatTile {
     int rx, ry, lx0, ly0;
 };
-__device__ __forceinline__ LatTile lat_tile(int S) {
+__device__ __forceinline__ LatTile lat_tile(int S, int bx, int by) {
     LatTile t;
-    t.rx = (int)blockIdx.x % S;
-    t.ry = (int)blockIdx.y % S;
-    t.lx0 = ((int)blockIdx.x / S) * TX;
-    t.ly0 = ((int)blockIdx.y / S) * TY;
+    t.rx = bx % S;
+    t.ry = by % S;
+    t.lx0 = (bx / S) * TX;
+    t.ly0 = (by / S) * TY;
     return t;
 }
 // image offset of window slot k (clamped to the image, as the taps are)
@@ -82,8 +103,8 @@ __device__ __forceinline__ size_t lat_src(const LatTile& t, int S, int W, int H,
 
 // ---- reference mode -----------------------------------------------------------------
 // LAT: the taps come from the staged sub-lattice window; otherwise every tap is read from
-// global memory (through L2) for a dense TX x TY tile of pixels, which is faster at strides of
-// 8 and more, where the lattice window's image reads no longer share cache lines.
+// global memory (through L2) for a dense TX x TY tile of pixels, which is faster at stride 16,
+// where the lattice window's image reads (one pixel per 16) no longer share cache lines.
 template <bool FAST, bool LAT>
 __global__ void __launch_bounds__(TX* TY) k_atrous_ref(int W, int H, int S, float sps, float sns, float sls,
                                                       const float4* __restrict__ in, const float4* __restrict__ gpos,
@@ -91,7 +112,9 @@ __global__ void __launch_bounds__(TX* TY) k_atrous_ref(int W, int H, int S, floa
     __shared__ float4 s_pos[LAT ? LW * LH : 1];
     __shared__ float4 s_nrm[LAT ? LW * LH : 1];
     __shared__ float4 s_rad[LAT ? LW * LH : 1];
-    const LatTile t = LAT ? lat_tile(S) : LatTile{0, 0, 0, 0};
+    int bx, by;
+    xcd_tile(bx, by);
+    const LatTile t = LAT ? lat_tile(S, bx, by) : LatTile{0, 0, 0, 0};
     if constexpr (LAT) {
         for (int k = threadIdx.x; k < LW * LH; k += TX * TY) {
             const size_t g = lat_src(t, S, W, H, k);
@@ -102,8 +125,8 @@ __global__ void __launch_bounds__(TX* TY) k_atrous_ref(int W, int H, int S, floa
         __syncthreads();
     }
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    const int x = LAT ? t.rx + S * (t.lx0 + tx) : (int)blockIdx.x * TX + tx;
-    const int y = LAT ? t.ry + S * (t.ly0 + ty) : (int)blockIdx.y * TY + ty;
+    const int x = LAT ? t.rx + S * (t.lx0 + tx) : bx * TX + tx;
+    const int y = LAT ? t.ry + S * (t.ly0 + ty) : by * TY + ty;
     if (x >= W || y >= H) return;
     const int lc = (ty + 2) * LW + (tx + 2);
     size_t i = (size_t)y * W + x;
@@ -163,7 +186,9 @@ __global__ void __launch_bounds__(TX* TY) k_atrous_cfg(int W, int H, int S, floa
                                                       ushort4* __restrict__ out) {
     __shared__ ushort4 s_gb[LAT ? LW * LH : 1];
     __shared__ ushort4 s_in[LAT ? LW * LH : 1];
-    const LatTile t = LAT ? lat_tile(S) : LatTile{0, 0, 0, 0};
+    int bx, by;
+    xcd_tile(bx, by);
+    const LatTile t = LAT ? lat_tile(S, bx, by) : LatTile{0, 0, 0, 0};
     if constexpr (LAT) {
         for (int k = threadIdx.x; k < LW * LH; k += TX * TY) {
             const size_t g = lat_src(t, S, W, H, k);
@@ -173,8 +198,8 @@ __global__ void __launch_bounds__(TX* TY) k_atrous_cfg(int W, int H, int S, floa
         __syncthreads();
     }
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    const int x = LAT ? t.rx + S * (t.lx0 + tx) : (int)blockIdx.x * TX + tx;
-    const int y = LAT ? t.ry + S * (t.ly0 + ty) : (int)blockIdx.y * TY + ty;
+    const int x = LAT ? t.rx + S * (t.lx0 + tx) : bx * TX + tx;
+    const int y = LAT ? t.ry + S * (t.ly0 + ty) : by * TY + ty;
     if (x >= W || y >= H) return;
     const int lc = (ty + 2) * LW + (tx + 2);
     size_t i = (size_t)y * W + x;
