@@ -248,7 +248,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stack", choices=["private", "lds32", "hybrid16", "hybrid32"], default="lds32",
                     help="where the traversal stack lives (context option)")
-    ap.add_argument("--xcd-remap", type=int, default=0, help="1: contiguous work range per XCD")
+    ap.add_argument("--xcd-remap", type=int, default=0, help="1: runs of 16 neighbouring workgroups per XCD")
     ap.add_argument("--persistent", type=int, default=0, help="1: persistent queue traversal with lane refill")
     ap.add_argument("--max-leaf", type=int, default=8, help="largest BVH leaf (1..8)")
     ap.add_argument("--trav-cost", type=float, default=1.0, help="SAH traversal cost (triangle tests)")

@@ -200,7 +200,7 @@ int rrh_fetch_counts(rrh_ctx* ctx, uint64_t* node_fetches, uint64_t* tri_fetches
  * with HIP events on the context stream while profiling is on */
 /* tuning options of a context (no reference counterpart) */
 #define RRH_OPT_TRAVERSAL_STACK 1   /* RRH_STACK_* : where the BVH traversal stack lives */
-#define RRH_OPT_XCD_REMAP 2         /* 1: give each XCD a contiguous range of work items */
+#define RRH_OPT_XCD_REMAP 2         /* 1: give each XCD runs of 16 consecutive workgroups (rotating) */
 #define RRH_OPT_PERSISTENT 4        /* 1: queue traversal by persistent waves that refill finished
                                        lanes with new rays (binary nodes); 0 (default): grid-stride */
 #define RRH_OPT_SPECULATIVE 5       /* 1 (default): traversal parks leaves until the whole wave has one */

@@ -64,20 +64,10 @@ constexpr int LW = TX + 4, LH = TY + 4;
 #endif
 constexpr int kLatMaxStride = RRH_ATROUS_LAT_MAX;  // lattice staging up to this stride, dense tiles above
 
-// XCD-aware workgroup order: the hardware deals linear ids round-robin to the 8 XCDs (each
-// with its own L2).  Runs of kXcdRun consecutive logical workgroups (a row of neighbouring
-// tiles, or the residues of one lattice tile, which share cache lines) are given to one XCD,
-// and the runs rotate over the XCDs so that every XCD still gets a mix of cheap and costly
-// image regions.  A last partial round of runs keeps the hardware order.
-constexpr uint32_t kXcdRun = 16;
+// Workgroups in XCD runs (xcd_block): the residues of one lattice tile and neighbouring
+// tiles share cache lines, so they are kept on one XCD's L2.
 __device__ __forceinline__ void xcd_tile(int& bx, int& by) {
-    const uint32_t nb = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
-    const uint32_t full = nb - nb % (8u * kXcdRun);
-    uint32_t l = b;
-    if (b < full) {
-        const uint32_t x = b & 7u, k = b >> 3;
-        l = ((k / kXcdRun) * 8u + x) * kXcdRun + k % kXcdRun;
-    }
+    const uint32_t l = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y, true);
     bx = (int)(l % gridDim.x);
     by = (int)(l / gridDim.x);
 }

@@ -853,10 +853,17 @@ RRH_D bool trace_occluded(const DevScene& S, f3 o, f3 d, float tmin, float tmax)
 }
 
 // XCD-aware block order: the hardware deals consecutive workgroups round-robin to the
-// 8 XCDs (each with its own L2); remapping gives every XCD one contiguous range of
-// work (an image band / queue range) so neighbouring rays share that XCD's L2
+// 8 XCDs (each with its own L2).  Remapping gives each XCD runs of kXcdRun consecutive
+// logical workgroups (neighbouring tiles or queue ranges, which share cache lines) and
+// rotates the runs over the XCDs, so that every XCD still gets a mix of cheap and costly
+// image regions; a last partial round of runs keeps the hardware order.
+constexpr uint32_t kXcdRun = 16;
 RRH_D uint32_t xcd_block(uint32_t b, uint32_t nb, bool remap) {
-    return (!remap || (nb & 7u)) ? b : (b & 7u) * (nb >> 3) + (b >> 3);
+    if (!remap) return b;
+    const uint32_t full = nb - nb % (8u * kXcdRun);
+    if (b >= full) return b;
+    const uint32_t x = b & 7u, k = b >> 3;
+    return ((k / kXcdRun) * 8u + x) * kXcdRun + k % kXcdRun;
 }
 
 // ---------------------------------------------------------------------------
