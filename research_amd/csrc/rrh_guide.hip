@@ -7,9 +7,14 @@
 //   buffers      ReSTIRPTPass.cpp:311-341 (accum + fitted, 16 B per cell each)
 // RRH_GRID_VMM is new: a res^3 dense grid with K <= 8 vMF lobes per cell fitted
 // by online EM.  E-step: responsibilities in fp32, sufficient statistics
-// (w*g, w*g*dir) accumulated with FP64 atomics.  M-step per cell: weights from
+// (w*g, w*g*dir) in FP64.  The samples are sorted by cell first (rocPRIM radix sort
+// of cell keys), then one wave per cell sums its samples in registers and one lane
+// adds the 8 x 4 totals to the cell's frame statistics: no atomics, and the sums do
+// not depend on scheduling.  M-step per cell: weights from
 // the responsibility mass and the closed-form mean-resultant fit of the
 // reference (GuidingGrid.slang:125-136) per lobe, so K = 1 reproduces it.
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "rrh_internal.h"
 #include "rrh_guide_dev.h"
 
@@ -69,48 +74,131 @@ struct VmmArgs {
     float bmin[3], bext[3];
 };
 
-__device__ __forceinline__ void vmm_estep_one(const VmmArgs& A, const float* __restrict__ lobes, f3 p, f3 d, float w,
-                                              double* __restrict__ st) {
-    if (!(w > 0.0f)) return;
+// E-step sample sources: host arrays uploaded (pos, dir, w: 3 + 3 + 1 floats) or the
+// path vertices a guided render recorded (t0 = (pos, w), t1 = (dir, pdf); count on the device)
+struct ArraySrc {
+    const float *pos, *dir, *w;
+    __device__ uint32_t count(uint32_t cap) const { return cap; }
+    __device__ void get(uint32_t i, f3& p, f3& d, float& wt) const {
+        p = mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+        d = mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
+        wt = w[i];
+    }
+};
+struct TrainSrc {
+    const float4* tr;
+    const uint32_t* cnt;
+    __device__ uint32_t count(uint32_t cap) const { return min(*cnt, cap); }
+    __device__ void get(uint32_t i, f3& p, f3& d, float& wt) const {
+        const float4 t0 = tr[(size_t)i * 4], t1 = tr[(size_t)i * 4 + 1];
+        p = xyz(t0); d = xyz(t1); wt = t0.w;
+    }
+};
+
+__device__ __forceinline__ uint32_t vmm_cell_of(const VmmArgs& A, f3 p) {
     GuideDev G;
     G.res = A.res; G.K = A.K;
     for (int k = 0; k < 3; ++k) { G.bmin[k] = A.bmin[k]; G.bext[k] = A.bext[k]; }
-    uint32_t c = vmm_cell(G, p);
-    const float* lc = lobes + (size_t)c * A.K * 5;
-    float g[kMaxLobes];
-    float tot = 0.0f;
-    for (uint32_t k = 0; k < A.K; ++k) {
-        g[k] = lc[k * 5 + 4] * vmf_pdf(mk(lc[k * 5], lc[k * 5 + 1], lc[k * 5 + 2]), lc[k * 5 + 3], d);
-        tot += g[k];
-    }
-    double* sc = st + (size_t)c * A.K * 4;
-    for (uint32_t k = 0; k < A.K; ++k) {
-        float gk = (tot > 0.0f) ? g[k] / tot : 1.0f / (float)A.K;
-        double wg = (double)w * (double)gk;
-        if (wg == 0.0) continue;
-        atomicAdd(sc + k * 4 + 0, wg);
-        atomicAdd(sc + k * 4 + 1, wg * (double)d.x);
-        atomicAdd(sc + k * 4 + 2, wg * (double)d.y);
-        atomicAdd(sc + k * 4 + 3, wg * (double)d.z);
+    return vmm_cell(G, p);
+}
+
+// key = the sample's cell; samples past the count or with no weight get the key ncell and
+// sort behind every cell
+template <class Src>
+__global__ void k_em_keys(VmmArgs A, Src src, uint32_t cap, uint32_t ncell, uint32_t* __restrict__ keys,
+                          uint32_t* __restrict__ vals) {
+    const uint32_t n = src.count(cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) {
+        uint32_t key = ncell;
+        if (i < n) {
+            f3 p, d;
+            float w;
+            src.get(i, p, d, w);
+            if (w > 0.0f) key = vmm_cell_of(A, p);
+        }
+        keys[i] = key;
+        vals[i] = i;
     }
 }
 
-__global__ void k_vmm_estep(VmmArgs A, const float* __restrict__ lobes, const float* __restrict__ pos,
-                            const float* __restrict__ dir, const float* __restrict__ w, uint32_t n,
-                            double* __restrict__ st) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    vmm_estep_one(A, lobes, mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]), mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]),
-                  w[i], st);
+// start[c] = first sorted position whose key is >= c, for c in [0, ncell]; every entry is
+// written exactly once (the boundaries between consecutive keys cover [0, ncell])
+__global__ void k_em_ranges(const uint32_t* __restrict__ keys, uint32_t n, uint32_t ncell, uint32_t* __restrict__ start) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += gridDim.x * blockDim.x) {
+        const uint32_t lo = (i == 0) ? 0u : keys[i - 1] + 1u;  // cells in (keys[i-1], keys[i]] start at i
+        const uint32_t hi = (i == n) ? ncell : keys[i];
+        for (uint32_t c = lo; c <= hi; ++c) start[c] = i;
+    }
 }
 
-// training samples: t0 (pos, w) t1 (dir, pdf) t2 (n, 0) t3 (wo, 0)
-__global__ void k_vmm_estep_train(VmmArgs A, const float* __restrict__ lobes, const float4* __restrict__ tr,
-                                  const uint32_t* __restrict__ cnt, uint32_t cap, double* __restrict__ st) {
-    uint32_t n = min(*cnt, cap);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        float4 t0 = tr[(size_t)i * 4], t1 = tr[(size_t)i * 4 + 1];
-        vmm_estep_one(A, lobes, xyz(t0), xyz(t1), t0.w, st);
+// one wave per cell (grid-stride over cells): lanes stride over the cell's sorted samples,
+// keep the K x 4 sums in registers, then an xor-shuffle tree reduces them and lane 0 adds
+// them to the cell's frame statistics (cleared by the M-step)
+template <class Src>
+__global__ void __launch_bounds__(256) k_em_cells(VmmArgs A, Src src, const float* __restrict__ lobes,
+                                                  const uint32_t* __restrict__ vals, const uint32_t* __restrict__ start,
+                                                  uint32_t ncell, double* __restrict__ st) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < ncell; c += waves) {
+        const uint32_t s0 = start[c], s1 = start[c + 1];
+        if (s0 == s1) continue;
+        const float* lc = lobes + (size_t)c * A.K * 5;
+        // every loop over the lobes is unrolled to kMaxLobes with a guard, so the sums stay
+        // in registers (a loop bounded by K would index them dynamically, through scratch)
+        double acc[kMaxLobes][4];
+#pragma unroll
+        for (int k = 0; k < kMaxLobes; ++k)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[k][j] = 0.0;
+        for (uint32_t j = s0 + lane; j < s1; j += 64u) {
+            f3 p, d;
+            float w;
+            src.get(vals[j], p, d, w);
+            float g[kMaxLobes];
+            float tot = 0.0f;
+#pragma unroll
+            for (int k = 0; k < kMaxLobes; ++k) {
+                g[k] = 0.0f;
+                if (k < (int)A.K) {
+                    g[k] = lc[k * 5 + 4] * vmf_pdf(mk(lc[k * 5], lc[k * 5 + 1], lc[k * 5 + 2]), lc[k * 5 + 3], d);
+                    tot += g[k];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kMaxLobes; ++k) {
+                if (k < (int)A.K) {
+                    const float gk = (tot > 0.0f) ? g[k] / tot : 1.0f / (float)A.K;
+                    const double wg = (double)w * (double)gk;
+                    acc[k][0] += wg;
+                    acc[k][1] += wg * (double)d.x;
+                    acc[k][2] += wg * (double)d.y;
+                    acc[k][3] += wg * (double)d.z;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kMaxLobes; ++k) {
+            if (k < (int)A.K) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    double v = acc[k][j];
+#pragma unroll
+                    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+                    acc[k][j] = v;
+                }
+            }
+        }
+        if (lane == 0) {
+            double* sc = st + (size_t)c * A.K * 4;
+#pragma unroll
+            for (int k = 0; k < kMaxLobes; ++k) {
+                if (k < (int)A.K) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) sc[k * 4 + j] += acc[k][j];
+                }
+            }
+        }
     }
 }
 
@@ -225,6 +313,38 @@ VmmArgs vmm_args(const rrh_grid* g) {
     A.res = g->p.res; A.K = g->p.lobes;
     for (int k = 0; k < 3; ++k) { A.bmin[k] = g->p.bounds_min[k]; A.bext[k] = g->p.bounds_max[k] - g->p.bounds_min[k]; }
     return A;
+}
+
+// E-step over up to `cap` samples of src (the count may live on the device): cell keys,
+// a stable radix sort of (key, index), the per-cell ranges, then the per-cell sums
+template <class Src>
+int vmm_estep_sorted(rrh_grid* g, Src src, uint32_t cap) {
+    rrh_ctx* c = g->ctx;
+    if (cap == 0) return RRH_OK;
+    int rc;
+    if ((rc = g->em_keys.ensure((size_t)cap * 8))) return rc;
+    if ((rc = g->em_vals.ensure((size_t)cap * 8))) return rc;
+    if ((rc = g->em_start.ensure((size_t)(g->ncell + 1) * 4))) return rc;
+    uint32_t* k = (uint32_t*)g->em_keys.p;
+    uint32_t* v = (uint32_t*)g->em_vals.p;
+    size_t tb = 0;
+    RRH_HIP(rocprim::radix_sort_pairs(nullptr, tb, k, k + cap, v, v + cap, cap, 0, 32, c->stream));
+    if ((rc = g->em_tmp.ensure(tb > 0 ? tb : 1))) return rc;
+    const uint32_t gk = std::min((cap + 255u) / 256u, 8192u);
+    hipLaunchKernelGGL(k_em_keys<Src>, dim3(gk), dim3(256), 0, c->stream, vmm_args(g), src, cap, g->ncell, k, v);
+    RRH_LAUNCH_CHECK();
+    tb = g->em_tmp.bytes;
+    const hipError_t e = rocprim::radix_sort_pairs(g->em_tmp.p, tb, k, k + cap, v, v + cap, cap, 0, 32, c->stream);
+    if (e != hipSuccess) { set_error(std::string("EM sample sort: ") + hipGetErrorString(e)); return RRH_E_HIP; }
+    const uint32_t gr = std::min((cap + 1u + 255u) / 256u, 8192u);
+    hipLaunchKernelGGL(k_em_ranges, dim3(gr), dim3(256), 0, c->stream, (const uint32_t*)(k + cap), cap, g->ncell,
+                       (uint32_t*)g->em_start.p);
+    RRH_LAUNCH_CHECK();
+    const uint32_t gc = std::min((g->ncell + 3u) / 4u, 4096u);
+    hipLaunchKernelGGL(k_em_cells<Src>, dim3(gc), dim3(256), 0, c->stream, vmm_args(g), src, (const float*)g->lobes.p,
+                       (const uint32_t*)(v + cap), (const uint32_t*)g->em_start.p, g->ncell, (double*)g->fstats.p);
+    RRH_LAUNCH_CHECK();
+    return RRH_OK;
 }
 
 int vmm_mstep(rrh_grid* g) {
@@ -345,6 +465,7 @@ int rrh_grid_destroy(rrh_grid* g) {
     if (!g) return RRH_OK;
     (void)hipDeviceSynchronize();  // the owning context may already be gone
     g->accum.release(); g->fitted.release(); g->lobes.release(); g->stats.release(); g->fstats.release(); g->valid.release();
+    g->em_keys.release(); g->em_vals.release(); g->em_tmp.release(); g->em_start.release();
     delete g;
     return RRH_OK;
 }
@@ -439,12 +560,7 @@ int rrh_grid_em_step(rrh_ctx* c, rrh_grid* g, const float* pos, const float* dir
     int rc = upload_tmp(c, pos, (size_t)n * 12, &dp);
     if (!rc) rc = upload_tmp(c, dirs, (size_t)n * 12, &dd);
     if (!rc) rc = upload_tmp(c, w, (size_t)n * 4, &dw);
-    if (!rc && n) {
-        hipLaunchKernelGGL(k_vmm_estep, dim3((n + 255) / 256), dim3(256), 0, c->stream, vmm_args(g),
-                           (const float*)g->lobes.p, (const float*)dp, (const float*)dd, (const float*)dw, n,
-                           (double*)g->fstats.p);
-        if (hipGetLastError() != hipSuccess) { set_error("k_vmm_estep launch failed"); rc = RRH_E_HIP; }
-    }
+    if (!rc) rc = vmm_estep_sorted(g, ArraySrc{(const float*)dp, (const float*)dd, (const float*)dw}, n);
     if (!rc) rc = vmm_mstep(g);
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) { set_error("rrh_grid_em_step: sync failed"); rc = RRH_E_HIP; }
     (void)hipFree(dp); (void)hipFree(dd); (void)hipFree(dw);
@@ -455,14 +571,9 @@ int rrh_grid_em_from_vertices(rrh_ctx* c, rrh_grid* g) {
     RRH_ARG(c && g, "rrh_grid_em_from_vertices: null argument");
     RRH_ARG(g->p.kind == RRH_GRID_VMM, "rrh_grid_em_from_vertices: VMM grids only");
     RRH_ARG(c->train.p && c->train_cnt.p, "rrh_grid_em_from_vertices: no recorded vertices (render with record_vertices)");
-    uint32_t grid = (c->train_capacity + 255) / 256;
-    if (grid > 8192) grid = 8192;
-    if (grid == 0) grid = 1;
-    hipLaunchKernelGGL(k_vmm_estep_train, dim3(grid), dim3(256), 0, c->stream, vmm_args(g), (const float*)g->lobes.p,
-                       (const float4*)c->train.p, (const uint32_t*)c->train_cnt.p, c->train_capacity,
-                       (double*)g->fstats.p);
-    RRH_LAUNCH_CHECK();
-    return vmm_mstep(g);
+    const int rc = vmm_estep_sorted(g, TrainSrc{(const float4*)c->train.p, (const uint32_t*)c->train_cnt.p},
+                                    c->train_capacity);
+    return rc ? rc : vmm_mstep(g);
 }
 
 int rrh_grid_download(rrh_ctx* c, const rrh_grid* g, void* a, size_t b0, void* b, size_t b1) {

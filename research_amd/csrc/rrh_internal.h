@@ -152,6 +152,7 @@ struct rrh_grid {
     uint32_t ncell = 0;
     rrh::DevBuf accum, fitted;              // HASH
     rrh::DevBuf lobes, stats, fstats, valid; // VMM
+    rrh::DevBuf em_keys, em_vals, em_tmp, em_start;  // VMM E-step: sorted (cell, sample) pairs, cell ranges
 };
 
 struct rrh_image {
