@@ -14,6 +14,7 @@
 // the responsibility mass and the closed-form mean-resultant fit of the
 // reference (GuidingGrid.slang:125-136) per lobe, so K = 1 reproduces it.
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "rrh_internal.h"
 #include "rrh_guide_dev.h"
@@ -263,48 +264,91 @@ __global__ void k_vmm_mask(uint32_t ncell, uint32_t K, const float* __restrict__
 
 // ---- path-vertex finalisation: incident radiance along the sampled direction ----
 // training sample: t0 (pos, lum(Li) / pdf)  t1 (wi, pdf)  t2 (n, lum(Li f cos) / pdf)  t3 (wo)
-__global__ void k_finalize(const float4* __restrict__ rec, uint32_t nrec, uint32_t max_bounces, uint32_t stamp,
-                           const float4* __restrict__ rad, uint32_t keep_thresh, float4* __restrict__ train,
-                           uint32_t* __restrict__ cnt, uint32_t cap) {
-    for (uint32_t base = blockIdx.x * blockDim.x; base < nrec; base += gridDim.x * blockDim.x) {
-        uint32_t i = base + threadIdx.x;
-        bool take = false;
-        float4 t0 = make_float4(0, 0, 0, 0), t1 = t0, t2 = t0, t3 = t0;
-        if (i < nrec) {
-            const float4* r = rec + (size_t)i * kRecF4;
-            float4 r0 = r[0];
-            if (__float_as_uint(r0.w) == stamp && (pcg_hash(i ^ (stamp * 2654435761u)) >> 8) < keep_thresh) {
-                float4 r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5], r6 = r[6], r7 = r[7];
-                uint32_t pixel = i / max_bounces;
-                float4 Lf = rad[pixel];
-                const f3 nee = (r3.w > 0.5f) ? xyz(r5) : mk(0, 0, 0);
-                f3 Ls = sub(sub(xyz(Lf), xyz(r4)), nee);
-                // radiance gathered after this vertex, by subtraction: a remainder within the
-                // rounding of the sums is zero (else dividing by a small throughput inflates it)
-                const float tol = 1e-5f * (fabsf(Lf.x) + fabsf(Lf.y) + fabsf(Lf.z) + fabsf(r4.x) + fabsf(r4.y) +
-                                           fabsf(r4.z) + fabsf(nee.x) + fabsf(nee.y) + fabsf(nee.z));
-                Ls = mk(Ls.x > tol ? Ls.x : 0.0f, Ls.y > tol ? Ls.y : 0.0f, Ls.z > tol ? Ls.z : 0.0f);
-                f3 tn = xyz(r6);
-                f3 Li = mk(tn.x > 0.0f ? Ls.x / tn.x : 0.0f, tn.y > 0.0f ? Ls.y / tn.y : 0.0f, tn.z > 0.0f ? Ls.z / tn.z : 0.0f);
-                float w = fmaxf(lum709(Li), 0.0f);
-                const float wl = (r1.w > 0.0f) ? w / r1.w : 0.0f;
-                if (wl > 0.0f && isfinite(wl)) {
-                    take = true;
-                    t0 = make_float4(r0.x, r0.y, r0.z, wl);
-                    t1 = make_float4(r3.x, r3.y, r3.z, r1.w);
-                    // product weight: Li f cos / pdf, the one-sample estimate of the integrand
-                    f3 bw = xyz(r7);
-                    const float wp = fmaxf(lum709(mk(Li.x * bw.x, Li.y * bw.y, Li.z * bw.z)), 0.0f);
-                    t2 = make_float4(r1.x, r1.y, r1.z, isfinite(wp) ? wp : 0.0f);
-                    t3 = make_float4(r2.x, r2.y, r2.z, 0.0f);
-                }
-            }
+// The vertex's training sample, if it is kept (deterministic: a hash of the record index
+// thins the live vertices to about the requested count).
+__device__ __forceinline__ bool finalize_one(const float4* __restrict__ rec, uint32_t i, uint32_t max_bounces,
+                                             uint32_t stamp, const float4* __restrict__ rad, uint32_t keep_thresh,
+                                             float4& t0, float4& t1, float4& t2, float4& t3) {
+    bool take = false;
+    t0 = make_float4(0, 0, 0, 0); t1 = t0; t2 = t0; t3 = t0;
+    const float4* r = rec + (size_t)i * kRecF4;
+    float4 r0 = r[0];
+    if (__float_as_uint(r0.w) == stamp && (pcg_hash(i ^ (stamp * 2654435761u)) >> 8) < keep_thresh) {
+        float4 r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5], r6 = r[6], r7 = r[7];
+        uint32_t pixel = i / max_bounces;
+        float4 Lf = rad[pixel];
+        const f3 nee = (r3.w > 0.5f) ? xyz(r5) : mk(0, 0, 0);
+        f3 Ls = sub(sub(xyz(Lf), xyz(r4)), nee);
+        // radiance gathered after this vertex, by subtraction: a remainder within the
+        // rounding of the sums is zero (else dividing by a small throughput inflates it)
+        const float tol = 1e-5f * (fabsf(Lf.x) + fabsf(Lf.y) + fabsf(Lf.z) + fabsf(r4.x) + fabsf(r4.y) +
+                                   fabsf(r4.z) + fabsf(nee.x) + fabsf(nee.y) + fabsf(nee.z));
+        Ls = mk(Ls.x > tol ? Ls.x : 0.0f, Ls.y > tol ? Ls.y : 0.0f, Ls.z > tol ? Ls.z : 0.0f);
+        f3 tn = xyz(r6);
+        f3 Li = mk(tn.x > 0.0f ? Ls.x / tn.x : 0.0f, tn.y > 0.0f ? Ls.y / tn.y : 0.0f, tn.z > 0.0f ? Ls.z / tn.z : 0.0f);
+        float w = fmaxf(lum709(Li), 0.0f);
+        const float wl = (r1.w > 0.0f) ? w / r1.w : 0.0f;
+        if (wl > 0.0f && isfinite(wl)) {
+            take = true;
+            t0 = make_float4(r0.x, r0.y, r0.z, wl);
+            t1 = make_float4(r3.x, r3.y, r3.z, r1.w);
+            // product weight: Li f cos / pdf, the one-sample estimate of the integrand
+            f3 bw = xyz(r7);
+            const float wp = fmaxf(lum709(mk(Li.x * bw.x, Li.y * bw.y, Li.z * bw.z)), 0.0f);
+            t2 = make_float4(r1.x, r1.y, r1.z, isfinite(wp) ? wp : 0.0f);
+            t3 = make_float4(r2.x, r2.y, r2.z, 0.0f);
         }
-        uint32_t slot = wave_push(cnt, take);
+    }
+    return take;
+}
+
+// Compaction in record order, so the training set (and everything fitted from it) does not
+// depend on scheduling: k_finalize_count counts the kept vertices of each 256-record chunk,
+// an exclusive scan gives every chunk its first slot, k_finalize_write ranks the kept vertices
+// inside the chunk (wave ballots + per-wave totals in LDS) and writes them; the last chunk
+// stores the total.
+constexpr uint32_t kFinChunk = 256;
+__global__ void __launch_bounds__(kFinChunk) k_finalize_count(const float4* __restrict__ rec, uint32_t nrec,
+                                                            uint32_t max_bounces, uint32_t stamp,
+                                                            const float4* __restrict__ rad, uint32_t keep_thresh,
+                                                            uint32_t* __restrict__ chunk_cnt) {
+    const uint32_t nchunk = (nrec + kFinChunk - 1) / kFinChunk;
+    for (uint32_t ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
+        const uint32_t i = ch * kFinChunk + threadIdx.x;
+        float4 t0, t1, t2, t3;
+        const bool take = i < nrec && finalize_one(rec, i, max_bounces, stamp, rad, keep_thresh, t0, t1, t2, t3);
+        const int n = __syncthreads_count(take ? 1 : 0);
+        if (threadIdx.x == 0) chunk_cnt[ch] = (uint32_t)n;
+    }
+}
+
+__global__ void __launch_bounds__(kFinChunk) k_finalize_write(const float4* __restrict__ rec, uint32_t nrec,
+                                                            uint32_t max_bounces, uint32_t stamp,
+                                                            const float4* __restrict__ rad, uint32_t keep_thresh,
+                                                            const uint32_t* __restrict__ chunk_cnt,
+                                                            const uint32_t* __restrict__ chunk_off,
+                                                            float4* __restrict__ train, uint32_t* __restrict__ cnt,
+                                                            uint32_t cap) {
+    __shared__ uint32_t s_wave[kFinChunk / 64];
+    const uint32_t nchunk = (nrec + kFinChunk - 1) / kFinChunk;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (uint32_t ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
+        const uint32_t i = ch * kFinChunk + threadIdx.x;
+        float4 t0, t1, t2, t3;
+        const bool take = i < nrec && finalize_one(rec, i, max_bounces, stamp, rad, keep_thresh, t0, t1, t2, t3);
+        const unsigned long long m = __ballot(take ? 1 : 0);
+        if (lane == 0) s_wave[wave] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t before = 0;
+        for (uint32_t w = 0; w < wave; ++w) before += s_wave[w];
+        const unsigned long long lt = (lane == 0u) ? 0ull : (~0ull >> (64u - lane));
+        const uint32_t slot = chunk_off[ch] + before + (uint32_t)__popcll(m & lt);
         if (take && slot < cap) {
             train[(size_t)slot * 4] = t0; train[(size_t)slot * 4 + 1] = t1;
             train[(size_t)slot * 4 + 2] = t2; train[(size_t)slot * 4 + 3] = t3;
         }
+        if (ch == nchunk - 1 && threadIdx.x == 0) *cnt = chunk_off[ch] + chunk_cnt[ch];
+        __syncthreads();  // s_wave is rewritten by the next chunk
     }
 }
 
@@ -413,11 +457,27 @@ int guide_finish(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* gri
     // keep about `target` of the (roughly nrec/2) live vertices
     double keep = (double)target / ((double)nrec * 0.5);
     uint32_t thresh = keep >= 1.0 ? (1u << 24) : (uint32_t)(keep * (double)(1u << 24));
-    RRH_HIP(hipMemsetAsync(c->train_cnt.p, 0, 4, c->stream));
-    uint32_t g = (nrec + 255) / 256;
-    if (g > 8192) g = 8192;
-    hipLaunchKernelGGL(k_finalize, dim3(g), dim3(256), 0, c->stream, (const float4*)c->vtx.p, nrec, p->max_bounces,
-                       R.stamp, rad, thresh,
+    if (nrec == 0) {
+        RRH_HIP(hipMemsetAsync(c->train_cnt.p, 0, 4, c->stream));
+        return RRH_OK;
+    }
+    const uint32_t nchunk = (nrec + kFinChunk - 1) / kFinChunk;
+    if ((rc = c->fin_chunks.ensure((size_t)nchunk * 8))) return rc;
+    uint32_t* ccnt = (uint32_t*)c->fin_chunks.p;
+    uint32_t* coff = ccnt + nchunk;
+    size_t tb = 0;
+    RRH_HIP(rocprim::exclusive_scan(nullptr, tb, ccnt, coff, 0u, nchunk, rocprim::plus<uint32_t>(), c->stream));
+    if ((rc = c->fin_tmp.ensure(tb > 0 ? tb : 1))) return rc;
+    const uint32_t g = std::min(nchunk, 8192u);
+    hipLaunchKernelGGL(k_finalize_count, dim3(g), dim3(kFinChunk), 0, c->stream, (const float4*)c->vtx.p, nrec,
+                       p->max_bounces, R.stamp, rad, thresh, ccnt);
+    RRH_LAUNCH_CHECK();
+    tb = c->fin_tmp.bytes;
+    const hipError_t e = rocprim::exclusive_scan(c->fin_tmp.p, tb, ccnt, coff, 0u, nchunk, rocprim::plus<uint32_t>(),
+                                                 c->stream);
+    if (e != hipSuccess) { set_error(std::string("vertex compaction scan: ") + hipGetErrorString(e)); return RRH_E_HIP; }
+    hipLaunchKernelGGL(k_finalize_write, dim3(g), dim3(kFinChunk), 0, c->stream, (const float4*)c->vtx.p, nrec,
+                       p->max_bounces, R.stamp, rad, thresh, (const uint32_t*)ccnt, (const uint32_t*)coff,
                        (float4*)c->train.p, (uint32_t*)c->train_cnt.p, cap);
     RRH_LAUNCH_CHECK();
     return RRH_OK;

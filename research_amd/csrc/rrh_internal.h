@@ -102,6 +102,7 @@ struct rrh_ctx {
     rrh::DevBuf vtx;              // recorded path vertices (guiding)
     rrh::DevBuf train;            // finalised training samples (4 float4 each)
     rrh::DevBuf train_cnt;        // device counter of training samples
+    rrh::DevBuf fin_chunks, fin_tmp;  // record-order compaction: per-chunk counts + offsets, scan scratch
     uint32_t train_capacity = 0;
     uint32_t stamp = 0;           // vertex-record generation
     uint32_t last_rec_bounces = 0;

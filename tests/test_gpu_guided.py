@@ -87,6 +87,36 @@ def test_mlp_guided_render_is_unbiased_and_training_lowers_the_loss(ctx, unguide
     _assert_same_mean(got, ref, "mlp")
 
 
+def test_guide_training_is_deterministic(ctx):
+    """The recorded vertices are compacted in record order and the EM step sums per cell
+    without atomics, so the same frames train the same guides: two fresh contexts (the vertex
+    thinning hashes a per-context render stamp), each with a fresh VMM grid and MLP (same
+    seed), fed the same guided frames, end bit-identical."""
+    grids, mlps = [], []
+    for _ in range(2):
+        c2 = core.Context(0)
+        gs = _scene(c2)
+        info = gs.info()
+        img = core.Image(c2, W, H)
+        g = core.Grid(c2, L.GridParams(L.GRID_VMM, 0, 0, 0, 0, 512.0, 8, 8, L.f3(info["bounds_min"]),
+                                       L.f3(info["bounds_max"]), 0.9, 0.0))
+        m = core.Mlp(c2, info["bounds_min"], info["bounds_max"], lr=3e-3, seed=5)
+        for k in range(4):
+            _render(c2, gs, img, 700 + k, mode=L.GUIDE_VMM, alpha=0.5 if k else 0.0, record=2000, grid=g)
+            L.call("rrh_grid_em_from_vertices", c2.h, g.h)
+            _render(c2, gs, img, 800 + k, mode=L.GUIDE_MLP, alpha=0.5 if k else 0.0, record=1024, mlp=m)
+            m.train_from_vertices(1024)
+        grids.append(g.download())
+        mlps.append(m.get_weights())
+        for o in (m, g, img, gs):
+            o.close()
+        c2.close()
+    assert np.array_equal(grids[0][0].view(np.uint32), grids[1][0].view(np.uint32))
+    assert np.array_equal(grids[0][1].view(np.uint64), grids[1][1].view(np.uint64))
+    assert np.array_equal(mlps[0].view(np.uint32), mlps[1].view(np.uint32))
+    assert grids[0][1].any()  # the grids did see samples
+
+
 def _noise(ctx, gs, w, h, n, seed0, bounces, **kw):
     """Per-sample noise: mean over pixels of the variance over n frames / (mean^2 + 0.01), and
     the mean image."""
@@ -111,9 +141,7 @@ def test_trained_guides_lower_the_noise_of_the_window_lit_room(ctx):
     train online for 48 frames at 256x192 (about 0.4 M path vertices per frame for the grid, one
     65,536-vertex Adam step per frame for the MLP); with the 64x48 evaluation extent alone they
     see too few paths and a poorly fitted guide adds variance.  Measured: unguided 1.098, grid
-    0.743 (the EM fit is deterministic), MLP 0.864-0.947 over runs: the recorded vertices are
-    compacted with atomics, so the MLP's training batches, and with them its result, vary from
-    run to run."""
+    0.743, MLP 0.792 (training is deterministic: the same frames train the same guides)."""
     W2, H2, B = 64, 48, 8
     sa = core.generate_scene(L.GEN_INDOOR, 1, 0.25)
     gs = core.Scene(ctx, sa)
@@ -139,6 +167,6 @@ def test_trained_guides_lower_the_noise_of_the_window_lit_room(ctx):
     v_vmm, m_vmm = _noise(ctx, gs, W2, H2, n, 20000, B, mode=L.GUIDE_VMM, alpha=0.5, grid=grid)
     v_mlp, m_mlp = _noise(ctx, gs, W2, H2, n, 30000, B, mode=L.GUIDE_MLP, alpha=0.5, mlp=mlp)
     print(f"per-sample noise: unguided {v_none:.3f}, vMF grid {v_vmm:.3f}, MLP {v_mlp:.3f}")
-    assert v_vmm < 0.85 * v_none and v_mlp < 0.92 * v_none
+    assert v_vmm < 0.85 * v_none and v_mlp < 0.85 * v_none
     for m in (m_vmm, m_mlp):  # the same image (within noise): guiding changes the variance, not the mean
         assert abs(m.mean() / m_none.mean() - 1) < 0.1
