@@ -371,14 +371,16 @@ int vmm_estep_sorted(rrh_grid* g, Src src, uint32_t cap) {
     if ((rc = g->em_start.ensure((size_t)(g->ncell + 1) * 4))) return rc;
     uint32_t* k = (uint32_t*)g->em_keys.p;
     uint32_t* v = (uint32_t*)g->em_vals.p;
+    // keys are cells in [0, ncell] (ncell = past the last cell): only their low bits are sorted
+    const unsigned end_bit = 32u - (unsigned)__builtin_clz(g->ncell);
     size_t tb = 0;
-    RRH_HIP(rocprim::radix_sort_pairs(nullptr, tb, k, k + cap, v, v + cap, cap, 0, 32, c->stream));
+    RRH_HIP(rocprim::radix_sort_pairs(nullptr, tb, k, k + cap, v, v + cap, cap, 0u, end_bit, c->stream));
     if ((rc = g->em_tmp.ensure(tb > 0 ? tb : 1))) return rc;
     const uint32_t gk = std::min((cap + 255u) / 256u, 8192u);
     hipLaunchKernelGGL(k_em_keys<Src>, dim3(gk), dim3(256), 0, c->stream, vmm_args(g), src, cap, g->ncell, k, v);
     RRH_LAUNCH_CHECK();
     tb = g->em_tmp.bytes;
-    const hipError_t e = rocprim::radix_sort_pairs(g->em_tmp.p, tb, k, k + cap, v, v + cap, cap, 0, 32, c->stream);
+    const hipError_t e = rocprim::radix_sort_pairs(g->em_tmp.p, tb, k, k + cap, v, v + cap, cap, 0u, end_bit, c->stream);
     if (e != hipSuccess) { set_error(std::string("EM sample sort: ") + hipGetErrorString(e)); return RRH_E_HIP; }
     const uint32_t gr = std::min((cap + 1u + 255u) / 256u, 8192u);
     hipLaunchKernelGGL(k_em_ranges, dim3(gr), dim3(256), 0, c->stream, (const uint32_t*)(k + cap), cap, g->ncell,
