@@ -267,6 +267,8 @@ def main() -> int:
     ap.add_argument("--guide-height", type=int, default=180)
     ap.add_argument("--restir-pt-frames", type=int, default=6,
                     help="ReSTIR PT and ReSTIR GI frames timed on the same scene and extent (0: skip)")
+    ap.add_argument("--no-pass-report", action="store_true",
+                    help="skip the streaming-pass report (accumulation, ReSTIR DI temporal, A-trous, tonemap)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB copy kernel (for hardware-counter calibration) and exit")
     a = ap.parse_args()
@@ -384,6 +386,15 @@ def main() -> int:
         out["restir_gi"] = restir_gi_summary(ctx, scene, sa, W, H, a.restir_pt_frames, 2)
         print(f"[bench] ReSTIR GI {out['restir_gi']['ms_per_frame']} ms/frame, "
               f"{out['restir_gi']['mrays_per_s']} Mrays/s", file=sys.stderr, flush=True)
+    if not a.no_pass_report:
+        print("[bench] streaming passes (A-trous, accumulation, ReSTIR DI temporal, tonemap) ...", file=sys.stderr,
+              flush=True)
+        from research_amd import kernel_report
+        r = kernel_report.report(ctx, W, H, reps=10)
+        out["streaming_passes"] = {
+            "workload": f"heightfield_1164288_tris_{W}x{H}", "copy_gbs": r["copy_gbs"], "passes": r["passes"],
+            "note": "ms per call by HIP markers around 10 calls; algorithmic bytes per pixel as in "
+                    "research_amd/kernel_report.py; A-trous passes are 5-iteration frames"}
     if a.guide_seconds > 0:
         print("[bench] equal-time guiding comparison (indoor scene) ...", file=sys.stderr, flush=True)
         out["guiding"] = guiding_summary(ctx, a)
