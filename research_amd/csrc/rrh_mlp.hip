@@ -450,6 +450,13 @@ __global__ void __launch_bounds__(256) k_mlp_infer(MlpDev M, Src src, uint32_t n
     __shared__ InferWave smw[4];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
     InferWave& S = smw[w];
+    const uint32_t n = min(src.count(), nsamples);
+    const uint32_t nchunks = (n + 15) / 16;
+    const uint32_t stride = gridDim.x * 4;
+    uint32_t ch = blockIdx.x * 4 + w;
+    // the grid is sized for the queue's capacity: a wave without a chunk of the live count
+    // leaves before loading the weights (waves are independent: no block barrier follows)
+    if (ch >= nchunks) return;
     bf16x8 f0[4][3], f1[4][2], f2[4][2], f3[4][2], f4[3][2];
     load_frags<3, 4>(Wl(M, 0), KI, f0);
     load_frags<2, 4>(Wl(M, 1), 64, f1);
@@ -462,10 +469,6 @@ __global__ void __launch_bounds__(256) k_mlp_infer(MlpDev M, Src src, uint32_t n
     load_bias<4>(M.P + off_b(2), NH, b2);
     load_bias<4>(M.P + off_b(3), NH, b3);
     load_bias<3>(M.P + off_b(4), NO, b4);
-    const uint32_t n = min(src.count(), nsamples);
-    const uint32_t nchunks = (n + 15) / 16;
-    const uint32_t stride = gridDim.x * 4;
-    uint32_t ch = blockIdx.x * 4 + w;
     // the next chunk's sample is loaded while this one computes (software pipelining)
     Sample nxt;
     bool nxt_valid = (ch < nchunks) && src.get(ch * 16 + r, nxt);  // the 4 lanes of row r hold sample r
