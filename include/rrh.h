@@ -50,6 +50,40 @@ typedef struct { float view[16], inv_view[16], inv_proj[16], prev_view_proj[16];
                  float position[3]; float near_plane; float direction[3]; float far_plane;
                  uint32_t frame_index, screen_width, screen_height, _pad; } rrh_camera;            /* 304 B, GpuCameraData */
 
+/* compile-time layout checks: sizes and field offsets of src/scene/GpuScene.h:17-98, so a
+ * caller built against this header fails to compile if a record drifts from the reference */
+#if defined(__cplusplus)
+#define RRH_STATIC_ASSERT(c, m) static_assert(c, m)
+#elif defined(__STDC_VERSION__) && __STDC_VERSION__ >= 201112L
+#define RRH_STATIC_ASSERT(c, m) _Static_assert(c, m)
+#else
+#define RRH_STATIC_ASSERT(c, m)
+#endif
+RRH_STATIC_ASSERT(sizeof(rrh_vertex) == 48, "rrh_vertex must be 48 B (GpuVertex)");
+RRH_STATIC_ASSERT(offsetof(rrh_vertex, normal) == 12 && offsetof(rrh_vertex, uv) == 24 &&
+                  offsetof(rrh_vertex, tangent) == 32, "rrh_vertex field offsets (GpuVertex)");
+RRH_STATIC_ASSERT(sizeof(rrh_mesh) == 32, "rrh_mesh must be 32 B (GpuMesh)");
+RRH_STATIC_ASSERT(offsetof(rrh_mesh, index_byte_offset) == 8 && offsetof(rrh_mesh, material_index) == 16,
+                  "rrh_mesh field offsets (GpuMesh)");
+RRH_STATIC_ASSERT(sizeof(rrh_material) == 72, "rrh_material must be 72 B (GpuMaterial)");
+RRH_STATIC_ASSERT(offsetof(rrh_material, metallic) == 16 && offsetof(rrh_material, emissive) == 24 &&
+                  offsetof(rrh_material, ior) == 36 && offsetof(rrh_material, albedo_tex_idx) == 40 &&
+                  offsetof(rrh_material, sampler_idx) == 56 && offsetof(rrh_material, flags) == 60,
+                  "rrh_material field offsets (GpuMaterial)");
+RRH_STATIC_ASSERT(sizeof(rrh_instance) == 80, "rrh_instance must be 80 B (GpuInstance)");
+RRH_STATIC_ASSERT(offsetof(rrh_instance, mesh_index) == 64, "rrh_instance field offsets (GpuInstance)");
+RRH_STATIC_ASSERT(sizeof(rrh_light) == 48, "rrh_light must be 48 B (GpuLight)");
+RRH_STATIC_ASSERT(offsetof(rrh_light, type) == 12 && offsetof(rrh_light, direction) == 16 &&
+                  offsetof(rrh_light, radius) == 28 && offsetof(rrh_light, emission) == 32 &&
+                  offsetof(rrh_light, intensity) == 44, "rrh_light field offsets (GpuLight)");
+RRH_STATIC_ASSERT(sizeof(rrh_camera) == 304, "rrh_camera must be 304 B (GpuCameraData)");
+RRH_STATIC_ASSERT(offsetof(rrh_camera, inv_view) == 64 && offsetof(rrh_camera, inv_proj) == 128 &&
+                  offsetof(rrh_camera, prev_view_proj) == 192 && offsetof(rrh_camera, position) == 256 &&
+                  offsetof(rrh_camera, near_plane) == 268 && offsetof(rrh_camera, direction) == 272 &&
+                  offsetof(rrh_camera, far_plane) == 284 && offsetof(rrh_camera, frame_index) == 288 &&
+                  offsetof(rrh_camera, screen_width) == 292 && offsetof(rrh_camera, screen_height) == 296,
+                  "rrh_camera field offsets (GpuCameraData)");
+
 #define RRH_MAT_PURE_LAMBERT 1u   /* f = albedo/pi, cosine sampling (no GGX lobe) */
 #define RRH_SCENE_TRI_LIGHTS 1u   /* emissive triangles become NEE / ReSTIR lights, MIS on BSDF hits */
 

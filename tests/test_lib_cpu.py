@@ -72,3 +72,26 @@ def test_camera_rejects_bad_arguments():
         core.camera_look_at((0, 0, 3), (0, 0, 0), 45.0, 0, 10)
     with pytest.raises(L.RRHError):
         core.generate_scene(99)
+
+
+@pytest.mark.parametrize("compiler,std", [("gcc", "-std=c11"), ("g++", "-std=c++17")])
+def test_header_layout_checks_compile(tmp_path, compiler, std):
+    """include/rrh.h carries static size / offset checks of the GpuScene.h records; a caller
+    compiling against it in C11 or C++ builds, and a drifted layout does not."""
+    import subprocess
+    from pathlib import Path
+
+    inc = Path(__file__).resolve().parents[1] / "include"
+    ok = tmp_path / "ok.c"
+    ok.write_text('#include "rrh.h"\nint main(void) { return (int)sizeof(rrh_camera) - 304; }\n')
+    lang = ["-x", "c++"] if compiler == "g++" else []
+    r = subprocess.run([compiler, std, "-Wall", "-Werror", *lang, f"-I{inc}", "-c", str(ok), "-o",
+                        str(tmp_path / "ok.o")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    # a record whose layout drifts (an extra field in GpuLight) must not compile
+    bad = tmp_path / "rrh.h"
+    bad.write_text((inc / "rrh.h").read_text().replace("float emission[3]; float intensity; } rrh_light;",
+                                                        "float emission[3]; float intensity; float x; } rrh_light;"))
+    r = subprocess.run([compiler, std, *lang, f"-I{tmp_path}", "-c", str(ok), "-o", str(tmp_path / "bad.o")],
+                       capture_output=True, text=True)
+    assert r.returncode != 0 and "rrh_light must be 48 B" in r.stderr
