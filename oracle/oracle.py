@@ -97,6 +97,13 @@ _SIGS = {
     "orc_brdf_pdf": (F, [FP, FP, FP, F]),
     "orc_sample_cosine_hemisphere": (None, [F, F, FP, FP]),
     "orc_build_onb": (None, [FP, FP, FP]),
+    "orc_fresnel_schlick": (None, [F, FP, FP]),
+    "orc_distribution_ggx": (F, [F, F]),
+    "orc_geometry_smith_ggx": (F, [F, F, F]),
+    "orc_sample_ggx_vndf_n": (None, [FP, F, P, U, P]),
+    "orc_brdf_eval_n": (None, [FP, FP, P, U, FP, F, F, P]),
+    "orc_brdf_pdf_n": (None, [FP, FP, P, U, F, P]),
+    "orc_brdf_sample_n": (None, [FP, FP, FP, F, F, P, U, P, P, P]),
     "orc_pt_render": (None, [P, C.POINTER(Camera), C.POINTER(PTParams), P, U, P, C.POINTER(RayCounts), C.c_int]),
     "orc_accumulate": (None, [P, P, C.c_size_t, U, U]),
     "orc_accumulate_f64": (None, [P, P, C.c_size_t, U, U]),

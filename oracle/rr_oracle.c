@@ -772,6 +772,38 @@ float orc_brdf_pdf(const float wo[3], const float wi[3], const float n[3], float
     return brdf_pdf(vld(wo), vld(wi), vld(n), roughness);
 }
 
+/* the BRDF.slang pieces one at a time and over arrays, so tests can integrate and histogram
+ * them against closed forms (no reference counterpart: checking entry points only) */
+void orc_fresnel_schlick(float cos_theta, const float F0[3], float out[3]) {
+    vst(out, fresnel_schlick(cos_theta, vld(F0)));
+}
+float orc_distribution_ggx(float NdotH, float alpha) { return distribution_ggx(NdotH, alpha); }
+float orc_geometry_smith_ggx(float NdotV, float NdotL, float alpha) {
+    return geometry_smith_ggx(NdotV, NdotL, alpha);
+}
+void orc_sample_ggx_vndf_n(const float wo_local[3], float roughness, const float* u, uint32_t count, float* h) {
+    v3 wol = vld(wo_local);
+    for (uint32_t i = 0; i < count; ++i) vst(h + 3 * i, sample_ggx_vndf(wol, roughness, u[2 * i], u[2 * i + 1]));
+}
+void orc_brdf_eval_n(const float n[3], const float wo[3], const float* wi, uint32_t count, const float base[3],
+                     float metallic, float roughness, float* out) {
+    for (uint32_t i = 0; i < count; ++i)
+        vst(out + 3 * i, brdf_eval(vld(n), vld(wo), vld(wi + 3 * i), vld(base), metallic, roughness));
+}
+void orc_brdf_pdf_n(const float n[3], const float wo[3], const float* wi, uint32_t count, float roughness,
+                    float* out) {
+    for (uint32_t i = 0; i < count; ++i) out[i] = brdf_pdf(vld(wo), vld(wi + 3 * i), vld(n), roughness);
+}
+void orc_brdf_sample_n(const float n[3], const float wo[3], const float base[3], float metallic, float roughness,
+                       const float* u, uint32_t count, float* wi, float* pdf, float* f) {
+    for (uint32_t i = 0; i < count; ++i) {
+        v3 w;
+        vst(f + 3 * i, brdf_sample(vld(wo), vld(n), vld(base), metallic, roughness, u[2 * i], u[2 * i + 1], &w,
+                                   pdf + i));
+        vst(wi + 3 * i, w);
+    }
+}
+
 /* the sampling pdf of brdf_sample evaluated at an arbitrary wi (world space);
  * used only for the MIS weights of triangle lights */
 static float brdf_sample_pdf(v3 wo, v3 wi, v3 n, v3 base, float metallic, float roughness) {

@@ -101,6 +101,17 @@ void  orc_brdf_sample(const float wo[3], const float n[3], const float base[3], 
 float orc_brdf_pdf(const float wo[3], const float wi[3], const float n[3], float roughness);
 void  orc_sample_cosine_hemisphere(float u0, float u1, float out[3], float* pdf);
 void  orc_build_onb(const float n[3], float T[3], float B[3]);
+/* BRDF.slang pieces, and array forms for quadrature / histogram checks */
+void  orc_fresnel_schlick(float cos_theta, const float F0[3], float out[3]);
+float orc_distribution_ggx(float NdotH, float alpha);
+float orc_geometry_smith_ggx(float NdotV, float NdotL, float alpha);
+void  orc_sample_ggx_vndf_n(const float wo_local[3], float roughness, const float* u, uint32_t count, float* h);
+void  orc_brdf_eval_n(const float n[3], const float wo[3], const float* wi, uint32_t count, const float base[3],
+                      float metallic, float roughness, float* out);
+void  orc_brdf_pdf_n(const float n[3], const float wo[3], const float* wi, uint32_t count, float roughness,
+                     float* out);
+void  orc_brdf_sample_n(const float n[3], const float wo[3], const float base[3], float metallic, float roughness,
+                        const float* u, uint32_t count, float* wi, float* pdf, float* f);
 
 /* ---- path tracer: pathtracer.slang:181-265 ----------------------------------- */
 typedef struct {
