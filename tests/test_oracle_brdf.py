@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+from tests.bsdf64 import cook_torrance as _ct, d_ggx, g1_schlick, g1_smith_exact, normalize
 
 F32 = np.float32
 
@@ -28,33 +29,9 @@ def fa(v):
     return O.fa(v)
 
 
-# ---- float64 definitions from the papers ------------------------------------------------
-def d_ggx(cos_h, a):
-    """Walter 2007 eq. 33 (tangent form); zero below the surface."""
-    c = np.clip(cos_h, 1e-300, 1.0)
-    tan2 = (1.0 - c * c) / (c * c)
-    d = a * a / (np.pi * c ** 4 * (a * a + tan2) ** 2)
-    return np.where(cos_h > 0, d, 0.0)
-
-
-def g1_smith_exact(cos_v, a):
-    c = np.clip(cos_v, 1e-300, 1.0)
-    tan2 = (1.0 - c * c) / (c * c)
-    return 2.0 / (1.0 + np.sqrt(1.0 + a * a * tan2))
-
-
-def g1_schlick(x, a):
-    k = a / 2.0
-    return x / (x * (1.0 - k) + k)
-
-
 def sph(theta, phi):
     st = np.sin(theta)
     return np.stack([st * np.cos(phi), st * np.sin(phi), np.cos(theta)], -1)
-
-
-def normalize(v):
-    return v / np.linalg.norm(v, axis=-1, keepdims=True)
 
 
 def vndf_density(h, wo, a):
@@ -118,17 +95,7 @@ def test_smith_schlick_ggx_closed_form():
 
 # ---- brdf_eval ---------------------------------------------------------------------------
 def cook_torrance(n, wo, wi, base, metallic, rough):
-    """float64 metallic-roughness BRDF: (1 - F(n.wo))(1 - m) base / pi + D G F(v.h) / (4 n.v n.l)."""
-    r = min(max(rough, 0.04), 1.0)
-    a = r * r
-    base = np.asarray(base, np.float64)
-    F0 = 0.04 * (1 - metallic) + base * metallic
-    fres = lambda c: F0 + (1 - F0) * np.clip(1 - c, 0, 1) ** 5
-    h = normalize(wo + wi)
-    nv, nl = max(n @ wo, 0.0), max(n @ wi, 0.0)
-    spec = d_ggx(np.array(max(n @ h, 0.0)), a) * g1_schlick(nv, a) * g1_schlick(nl, a) * fres(max(wo @ h, 0.0))
-    spec = spec / max(4 * nv * nl, 1e-4)
-    return (1 - fres(nv)) * (1 - metallic) * base / np.pi + spec
+    return _ct(n, wo, wi, base, np.float64(metallic), np.float64(rough))
 
 
 def eval_oracle(n, wo, wis, base, m, r):
