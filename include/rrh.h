@@ -247,6 +247,10 @@ int rrh_fetch_counts(rrh_ctx* ctx, uint64_t* node_fetches, uint64_t* tri_fetches
 #define RRH_OPT_PLOC_RADIUS 9       /* device builder: clusters searched on each side, 1..32 (default 16) */
 #define RRH_OPT_RAY_SORT 10         /* extension rays traced in the order of a key (direction octant, then
                                        origin Morton code); value = key bits sorted, 0 (off) or 4..32 */
+#define RRH_OPT_RESTIR_CULL 11      /* ReSTIR PT / GI rays: 0 (default) two-sided, as the reference's instances
+                                       (VK_GEOMETRY_INSTANCE_TRIANGLE_FACING_CULL_DISABLE_BIT_KHR,
+                                       src/rhi/AccelStructure.cpp:90) override the shaders' cull flag;
+                                       1: back faces culled, as RAY_FLAG_CULL_BACK_FACING_TRIANGLES alone */
 #define RRH_NODES_BINARY 0          /* binary, 64 B (both child boxes in fp32) */
 #define RRH_NODES_WIDE4 1           /* 4-wide, 128 B (fp32 child boxes) */
 #define RRH_NODES_WIDE4Q 2          /* 4-wide, 64 B (8-bit child boxes, outward rounded); default */
@@ -301,9 +305,9 @@ int rrh_di_ray_counts(rrh_ctx* ctx, rrh_ray_counts* out);
  * A reservoir is four RGBA32F planes (restir_pt.slang:11-15):
  *   res[0] (reconnection position, w_sum)   res[1] (reconnection normal, asfloat(M))
  *   res[2] (suffix radiance, W)             res[3] (prefix throughput, valid)
- * and a history two (the frame's G-buffer position and normal, for reprojection).  Every
- * ray culls back faces.  grid: an RRH_GRID_HASH grid that guides the first bounce and
- * receives the splats, or NULL (cosine sampling, no splats). */
+ * and a history two (the frame's G-buffer position and normal, for reprojection).  Rays are
+ * two-sided unless RRH_OPT_RESTIR_CULL is set.  grid: an RRH_GRID_HASH grid that guides the
+ * first bounce and receives the splats, or NULL (cosine sampling, no splats). */
 typedef struct {
     uint32_t frame_seed;               /* frame_count_++ (ReSTIRPTPass.cpp:561) */
     uint32_t num_initial_candidates;   /* ReSTIRPTPass.h:88 default 8 */
@@ -343,8 +347,8 @@ int rrh_rpt_ray_counts(rrh_ctx* ctx, rrh_ray_counts* out);
  *      cs_spatial_shade :849-1013 ---------------------------------------------------------
  * G-buffer and history as ReSTIR PT's.  A reservoir is three RGBA32F planes
  * (restir_gi.slang:204-242): res[0] (sample position, w_sum), res[1] (sample radiance, W),
- * res[2] (sample normal, asfloat(M)); M = 0 is empty.  Cosine candidates, no guiding; every
- * ray culls back faces. */
+ * res[2] (sample normal, asfloat(M)); M = 0 is empty.  Cosine candidates, no guiding; rays
+ * are two-sided unless RRH_OPT_RESTIR_CULL is set. */
 typedef struct {
     uint32_t frame_seed;               /* frame_count_++ (ReSTIRGIPass.cpp:460) */
     uint32_t num_initial_candidates;   /* ReSTIRGIPass.h:72 default 8 */

@@ -114,11 +114,13 @@ _SIGS = {
     "orc_rpt_temporal": (None, [P, C.POINTER(Camera), C.POINTER(RPTParams), P, P, P, P, P, P, C.c_int]),
     "orc_rpt_spatial_shade": (None, [P, C.POINTER(Camera), C.POINTER(RPTParams), P, F, F, U, P, P, P, P, P,
                                      C.c_int]),
+    "orc_set_restir_cull": (None, [C.c_int]),
     "orc_rgi_initial": (None, [P, C.POINTER(Camera), C.POINTER(RGIParams), P, P, P, P, P, C.c_int]),
     "orc_rgi_temporal": (None, [P, C.POINTER(Camera), C.POINTER(RGIParams), P, P, P, P, P, P, C.c_int]),
     "orc_rgi_spatial_shade": (None, [P, C.POINTER(Camera), C.POINTER(RGIParams), P, P, P, P, P, C.c_int]),
     "orc_atrous_ref": (None, [C.POINTER(AtrousParams), P, P, P, P]),
     "orc_atrous_cfg": (None, [C.POINTER(AtrousParams), P, P, P]),
+    "orc_atrous_prepare": (None, [U, U, FP, FP, P, P, P, P, P]),
     "orc_f32_to_f16": (C.c_uint16, [F]),
     "orc_f16_to_f32": (F, [C.c_uint16]),
     "orc_guiding_cell_index": (U, [FP, U, FP, F, F, U]),
@@ -307,6 +309,15 @@ def atrous_ref(inp, gpos, gnrm, stride, sigma_p=0.25, sigma_n=0.2, sigma_l=1.0):
     a, b, c = (np.ascontiguousarray(v, np.float32) for v in (inp, gpos, gnrm))  # alive across the call
     lib().orc_atrous_ref(C.byref(prm), p(a), p(b), p(c), p(out))
     return out
+
+
+def atrous_prepare(rad, gpos, gnrm, cam_pos, cam_dir):
+    """(rad16, gbuf16) as uint16 (H, W, 4) from RGBA32F radiance and G-buffer (H, W, 4)."""
+    H, W = rad.shape[:2]
+    r16, g16 = np.zeros((H, W, 4), np.uint16), np.zeros((H, W, 4), np.uint16)
+    a, b, c = (np.ascontiguousarray(v, np.float32) for v in (rad, gpos, gnrm))
+    lib().orc_atrous_prepare(W, H, fa(cam_pos), fa(cam_dir), p(a), p(b), p(c), p(r16), p(g16))
+    return r16, g16
 
 
 def atrous_cfg(inp16, gbuf16, stride, sigma_z=1.0, sigma_n=128.0, sigma_l=4.0):

@@ -1363,6 +1363,43 @@ float orc_f16_to_f32(uint16_t h) {
     return asfloat(sign | ((e + 112u) << 23) | (m << 13));
 }
 
+/* config-mode inputs from a rendered frame (the library's rrh_atrous_prepare; no reference
+ * counterpart: the reference filters RGBA32F radiance with its fp32 G-buffer): fp16 radiance
+ * with the population variance of the luminance (Rec.709) over the valid pixels of the
+ * clamped 3x3 window in alpha, and fp16 (normal, linear depth), the depth being the hit
+ * point's distance along the camera's forward direction (0 on a miss). */
+void orc_atrous_prepare(uint32_t W, uint32_t H, const float cpos[3], const float cdir[3], const float* rad,
+                        const float* gpos, const float* gnrm, uint16_t* rad16, uint16_t* gb16) {
+    v3 cp = vld(cpos), cd = vld(cdir);
+#pragma omp parallel for schedule(static)
+    for (int64_t y = 0; y < (int64_t)H; ++y) {
+        for (int64_t x = 0; x < (int64_t)W; ++x) {
+            size_t i = (size_t)y * W + (size_t)x;
+            float m1 = 0.0f, m2 = 0.0f, n = 0.0f;
+            for (int dy = -1; dy <= 1; ++dy) {
+                for (int dx = -1; dx <= 1; ++dx) {
+                    int64_t sx = x + dx, sy = y + dy;
+                    sx = sx < 0 ? 0 : (sx > (int64_t)W - 1 ? (int64_t)W - 1 : sx);
+                    sy = sy < 0 ? 0 : (sy > (int64_t)H - 1 ? (int64_t)H - 1 : sy);
+                    size_t j = (size_t)sy * W + (size_t)sx;
+                    if (gpos[j * 4 + 3] < 0.0f) continue;
+                    float l = lum709(vld(rad + j * 4));
+                    m1 += l; m2 += l * l; n += 1.0f;
+                }
+            }
+            float var = 0.0f;
+            if (n > 0.0f) { m1 = m1 / n; m2 = m2 / n; var = fmaxf(0.0f, m2 - m1 * m1); }
+            const float* c = rad + i * 4;
+            rad16[i * 4] = orc_f32_to_f16(c[0]); rad16[i * 4 + 1] = orc_f32_to_f16(c[1]);
+            rad16[i * 4 + 2] = orc_f32_to_f16(c[2]); rad16[i * 4 + 3] = orc_f32_to_f16(var);
+            const float* gp = gpos + i * 4; const float* gn = gnrm + i * 4;
+            float z = (gp[3] < 0.0f) ? 0.0f : vdot(vsub(vld(gp), cp), cd);
+            gb16[i * 4] = orc_f32_to_f16(gn[0]); gb16[i * 4 + 1] = orc_f32_to_f16(gn[1]);
+            gb16[i * 4 + 2] = orc_f32_to_f16(gn[2]); gb16[i * 4 + 3] = orc_f32_to_f16(z);
+        }
+    }
+}
+
 /* config mode: SVGF edge-stopping (Schied et al. 2017, eqs. 3-5) on fp16
  * buffers.  in.a carries the luminance variance; gbuf = (n.xyz, linear depth). */
 static float powi_pos(float x, float e) { return (x > 0.0f) ? orc_expf(e * orc_logf(x)) : 0.0f; }
@@ -1598,11 +1635,17 @@ void orc_vmm_mstep(const double* stats, uint32_t ncell, uint32_t K, float kappa_
 
 /* ========================================================================== */
 /* ReSTIR PT, reconnection shift -- restir_pt.slang:98-1001 (stages :569-771,
- * :775-872, :876-1001).  Every ray culls back faces (RAY_FLAG_CULL_BACK_FACING_
- * TRIANGLES).  1/sqrtf where the shader writes rsqrt.  Scenes whose emissive
- * triangles are lights: light sampling covers them (area sampling, no MIS) and the
- * reconnection vertex's own emission is left to the direct lighting pass.       */
+ * :775-872, :876-1001).  Rays are two-sided: the shaders ask for
+ * RAY_FLAG_CULL_BACK_FACING_TRIANGLES (restir_pt.slang:104,111, restir_gi.slang:75,82),
+ * but every TLAS instance carries VK_GEOMETRY_INSTANCE_TRIANGLE_FACING_CULL_DISABLE_BIT_KHR
+ * (AccelStructure.cpp:90, Scene.cpp:490), which switches that culling off.
+ * orc_set_restir_cull(1) restates what the ray flag alone would do.  1/sqrtf where
+ * the shader writes rsqrt.  Scenes whose emissive triangles are lights: light sampling
+ * covers them (area sampling, no MIS) and the reconnection vertex's own emission is
+ * left to the direct lighting pass.                                             */
 /* ========================================================================== */
+static int g_restir_cull = 0;
+void orc_set_restir_cull(int on) { g_restir_cull = on ? 1 : 0; }
 typedef struct { v3 rc_pos, rc_n, suffix, prefix; float w_sum, W; uint32_t M; float valid; } rpt_res;
 
 static rpt_res rpt_load(const float* res, size_t npix, size_t i) {
@@ -1627,7 +1670,7 @@ static void rpt_store_empty(float* res, size_t npix, size_t i) {
     for (int k = 0; k < 4; ++k) memset(res + (k * npix + i) * 4, 0, 16);
 }
 static int hit_cull(const orc_scene* s, v3 o, v3 d, float tmin, float tmax) {
-    return trace_bvh_cull(s, o, d, tmin, tmax, 1).prim != ~0u;
+    return trace_bvh_cull(s, o, d, tmin, tmax, g_restir_cull).prim != ~0u;
 }
 
 /* sample_lights, restir_pt.slang:171-222 */
@@ -1759,7 +1802,7 @@ void orc_rpt_initial(const orc_scene* s, const orc_camera* cam, const orc_rpt_pa
                                    fitted != NULL, alpha, &seed, wa, &src_pdf);
             const v3 wi0 = vld(wa);
             if (src_pdf <= 1e-6f || vdot(pn, wi0) <= 0.0f) continue;
-            orc_hit h0 = trace_bvh_cull(s, vadd(pp, vscale(pn, bias)), wi0, bias, 1e9f, 1);
+            orc_hit h0 = trace_bvh_cull(s, vadd(pp, vscale(pn, bias)), wi0, bias, 1e9f, g_restir_cull);
             if (h0.prim == ~0u) continue;
             hitinfo rc = fetch_hit(s, h0);
             v3 rn = rc.normal;
@@ -1779,7 +1822,7 @@ void orc_rpt_initial(const orc_scene* s, const orc_camera* cam, const orc_rpt_pa
                 float cosc = fmaxf(vdot(cur_n, wi), 0.0f);
                 thp = vmul(thp, vdivs(vscale(f, cosc), pdf));
                 if (fmaxf(thp.x, fmaxf(thp.y, thp.z)) <= 0.0f) break;
-                orc_hit h = trace_bvh_cull(s, vadd(cur_p, vscale(cur_n, bias)), wi, bias, 1e9f, 1);
+                orc_hit h = trace_bvh_cull(s, vadd(cur_p, vscale(cur_n, bias)), wi, bias, 1e9f, g_restir_cull);
                 if (h.prim == ~0u) { suffix = vadd(suffix, vmul(thp, sky(wi))); break; }
                 hitinfo hi = fetch_hit(s, h);
                 v3 hn = hi.normal;
@@ -2090,7 +2133,7 @@ void orc_rgi_initial(const orc_scene* s, const orc_camera* cam, const orc_rgi_pa
             v3 T, B; build_onb(pn, &T, &B);
             v3 dir = to_world(vld(l), T, B, pn);
             if (vdot(pn, dir) <= 0.0f) continue;
-            orc_hit h = trace_bvh_cull(s, vadd(pp, vscale(pn, bias)), dir, bias, 1e9f, 1);
+            orc_hit h = trace_bvh_cull(s, vadd(pp, vscale(pn, bias)), dir, bias, 1e9f, g_restir_cull);
             if (h.prim == ~0u) continue;
             hitinfo hi = fetch_hit(s, h);
             /* evaluate_sample_radiance, restir_gi.slang:539-551 */

@@ -151,6 +151,9 @@ void orc_di_spatial_shade(const orc_scene* s, const orc_camera* cam, const orc_d
                           const uint32_t* res_uv, float* out, int threads);
 
 /* ---- ReSTIR PT, reconnection shift: restir_pt.slang:98-1001 -------------------- */
+/* ReSTIR PT and GI rays are two-sided by default, as the reference's CULL_DISABLE instances
+ * trace them; 1 culls back faces as RAY_FLAG_CULL_BACK_FACING_TRIANGLES alone would */
+void orc_set_restir_cull(int on);
 typedef struct {
     uint32_t frame_seed, num_initial_candidates, max_bounces, temporal_enable, spatial_enable;
     uint32_t spatial_num_neighbors, spatial_radius, include_direct_lighting;
@@ -199,6 +202,10 @@ void orc_atrous_ref(const orc_atrous_params* p, const float* in, const float* gp
 /* config mode: in/out RGBA16F (rgb + luminance variance in a), gbuf RGBA16F (n.xyz, linear depth; depth<=0 = miss) */
 void orc_atrous_cfg(const orc_atrous_params* p, const uint16_t* in, const uint16_t* gbuf,
                     uint16_t* out);
+/* config-mode inputs (fp16 radiance + 3x3 luminance variance, fp16 normal + linear depth) from a
+ * rendered frame, as rrh_atrous_prepare */
+void orc_atrous_prepare(uint32_t W, uint32_t H, const float cpos[3], const float cdir[3], const float* rad,
+                        const float* gpos, const float* gnrm, uint16_t* rad16, uint16_t* gb16);
 uint16_t orc_f32_to_f16(float f);
 float    orc_f16_to_f32(uint16_t h);
 

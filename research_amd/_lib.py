@@ -38,7 +38,7 @@ GUIDE_NONE, GUIDE_VMM, GUIDE_MLP = 0, 1, 2
 PT_COUNT_FETCHES = 1
 OPT_TRAVERSAL_STACK, OPT_XCD_REMAP, OPT_BVH_NODES, OPT_PERSISTENT, OPT_SPECULATIVE = 1, 2, 3, 4, 5
 OPT_BVH_MAX_LEAF, OPT_BVH_TRAV_COST = 6, 7
-OPT_BVH_BUILDER, OPT_PLOC_RADIUS, OPT_RAY_SORT = 8, 9, 10
+OPT_BVH_BUILDER, OPT_PLOC_RADIUS, OPT_RAY_SORT, OPT_RESTIR_CULL = 8, 9, 10, 11
 BVH_BUILD_HOST, BVH_BUILD_DEVICE = 0, 1
 UPDATE_REFIT, UPDATE_REBUILD = 0, 1
 NODES_BINARY, NODES_WIDE4, NODES_WIDE4Q = 0, 1, 2

@@ -140,6 +140,7 @@ int rrh_ctx_set_option(rrh_ctx* c, int option, int value) {
         return RRH_OK;
     }
     if (option == RRH_OPT_XCD_REMAP) { c->xcd_remap = value != 0; return RRH_OK; }
+    if (option == RRH_OPT_RESTIR_CULL) { c->restir_cull = value != 0; return RRH_OK; }
     if (option == RRH_OPT_PERSISTENT) { c->persistent = value != 0; return RRH_OK; }
     if (option == RRH_OPT_SPECULATIVE) { c->speculative = value != 0; return RRH_OK; }
     if (option == RRH_OPT_BVH_MAX_LEAF) {

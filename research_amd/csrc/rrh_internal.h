@@ -117,6 +117,7 @@ struct rrh_ctx {
     bool profiling = false;
     int stack_kind = RRH_STACK_LDS32;
     bool xcd_remap = false;
+    bool restir_cull = false;      // RRH_OPT_RESTIR_CULL: ReSTIR PT / GI rays cull back faces
     int bvh_nodes = RRH_NODES_WIDE4Q;  // RRH_OPT_BVH_NODES (quantized 4-wide: fastest measured)
     std::vector<hipEvent_t> prof_events;   // pairs around each traversal launch
     uint32_t prof_used = 0;

@@ -5,8 +5,9 @@
 // src/passes/restir_gi/ReSTIRGIPass.cpp:428-651).
 //
 // The single-bounce sibling of ReSTIR PT (rrh_restir_pt.hip) and built the same way: one thread
-// per pixel, the initial stage a per-lane state machine around one culled traversal call in
-// persistent waves (candidate segment, then its light sample's shadow ray), the reuse stages
+// per pixel, the initial stage a per-lane state machine around one traversal call in persistent
+// waves (candidate segment, then its light sample's shadow ray; rays two-sided unless
+// RRH_OPT_RESTIR_CULL selects the culling kernels, as in rrh_restir_pt.hip), the reuse stages
 // one pass each with one visibility ray.  The shared pieces (light sampling, Jacobian,
 // visibility, history search, neighbour spiral) are in rrh_reuse.h.  Differences from ReSTIR
 // PT that the reference has and that are kept: cosine candidates (no guiding, no selection
@@ -79,11 +80,13 @@ RRH_D float primary_pdf(f3 pp, f3 pn, f3 sp) {
 // ---------------------------------------------------------------------------
 enum : uint32_t { gNext, gToHit, gShadow, gFinish, gDone };
 
-template <int STK>
+template <int KS>
 __global__ void __launch_bounds__(64, 4) k_rgi_initial(DevScene S, rrh_rgi_params P, Frame F, Res3 R,
                                                        float4* __restrict__ hpos, float4* __restrict__ hnrm,
                                                        uint32_t* __restrict__ counter, uint32_t tiles_x,
                                                        uint32_t npad) {
+    constexpr int STK = KS & ~kCull;
+    constexpr bool CULL = (KS & kCull) != 0;
     __shared__ int s_stack[StackSel<STK>::kLds * 64];
     uint32_t rays = 0, shadow_rays = 0;
     uint32_t* next_pixel = counter + 3;
@@ -222,9 +225,9 @@ __global__ void __launch_bounds__(64, 4) k_rgi_initial(DevScene S, rrh_rgi_param
         if (!__any(want_ray)) break;
         if (want_ray) {
             if (S.speculative)
-                h = trace_closest4spec_s<false, true, true>(S, ro, rd, bias, rtmax, st, nullptr, nullptr, first_hit);
+                h = trace_closest4spec_s<false, true, CULL>(S, ro, rd, bias, rtmax, st, nullptr, nullptr, first_hit);
             else
-                h = trace_closest4_s<false, true, true>(S, ro, rd, bias, rtmax, st, nullptr, nullptr, first_hit);
+                h = trace_closest4_s<false, true, CULL>(S, ro, rd, bias, rtmax, st, nullptr, nullptr, first_hit);
         }
     }
     count_rays(counter, rays);
@@ -234,9 +237,11 @@ __global__ void __launch_bounds__(64, 4) k_rgi_initial(DevScene S, rrh_rgi_param
 // ---------------------------------------------------------------------------
 // cs_temporal, restir_gi.slang:686-847
 // ---------------------------------------------------------------------------
-template <int STK>
+template <int KS>
 __global__ void __launch_bounds__(64) k_rgi_temporal(DevScene S, rrh_rgi_params P, Frame F, Res3 R, Res3 Rp,
                                                      History Hp, uint32_t* __restrict__ counter, bool remap) {
+    constexpr int STK = KS & ~kCull;
+    constexpr bool CULL = (KS & kCull) != 0;
     __shared__ int s_stack[StackSel<STK>::kLds * 64];
     uint32_t px, py, rays = 0;
     if (tile_pixel(px, py, F.W, F.H, remap)) {
@@ -259,7 +264,7 @@ __global__ void __launch_bounds__(64) k_rgi_temporal(DevScene S, rrh_rgi_params 
                 if (go) {
                     auto st = make_stack<STK>(s_stack);
                     rays++;
-                    go = visible(S, pp, pn, prv.pos, P.ray_bias, st);
+                    go = visible<CULL>(S, pp, pn, prv.pos, P.ray_bias, st);
                 }
                 if (go) {
                     const Mat pm = load_mat(S, pmi);
@@ -297,10 +302,12 @@ __global__ void __launch_bounds__(64) k_rgi_temporal(DevScene S, rrh_rgi_params 
 // ---------------------------------------------------------------------------
 // cs_spatial_shade, restir_gi.slang:849-1013
 // ---------------------------------------------------------------------------
-template <int STK>
+template <int KS>
 __global__ void __launch_bounds__(64) k_rgi_spatial_shade(DevScene S, rrh_rgi_params P, Frame F, Res3 R,
                                                           const float4* __restrict__ direct, float4* __restrict__ out,
                                                           uint32_t* __restrict__ counter, bool remap) {
+    constexpr int STK = KS & ~kCull;
+    constexpr bool CULL = (KS & kCull) != 0;
     __shared__ int s_stack[StackSel<STK>::kLds * 64];
     uint32_t px, py, rays = 0;
     if (tile_pixel(px, py, F.W, F.H, remap)) {
@@ -350,7 +357,7 @@ __global__ void __launch_bounds__(64) k_rgi_spatial_shade(DevScene S, rrh_rgi_pa
                     if (lum709(ind) > 0.0f) {
                         auto st = make_stack<STK>(s_stack);
                         rays++;
-                        if (visible(S, pp, pn, res.pos, P.ray_bias, st)) L = add(L, scale(ind, P.indirect_strength));
+                        if (visible<CULL>(S, pp, pn, res.pos, P.ray_bias, st)) L = add(L, scale(ind, P.indirect_strength));
                     }
                 }
             }
@@ -414,13 +421,10 @@ int rrh_rgi_initial(rrh_ctx* c, rrh_scene* s, const rrh_rgi_params* p, const rrh
     Res3 R;
     if ((rc = res3(res, F.W, F.H, R, "rrh_rgi_initial"))) return rc;
     RRH_HIP(hipMemsetAsync(c->rgi_counter.p, 0, 16, c->stream));
-    const int sk = stack_kind(c, s);
+    const int sk = kernel_kind(c, s);
     const uint32_t tiles_x = (F.W + 7) / 8, tiles = tiles_x * ((F.H + 7) / 8);
     uint32_t waves = 0;
-    if (sk == RRH_STACK_LDS32) rc = persistent_waves(c, KInit<RRH_STACK_LDS32>::fn, tiles, waves);
-    else if (sk == RRH_STACK_HYBRID16) rc = persistent_waves(c, KInit<RRH_STACK_HYBRID16>::fn, tiles, waves);
-    else if (sk == RRH_STACK_HYBRID32) rc = persistent_waves(c, KInit<RRH_STACK_HYBRID32>::fn, tiles, waves);
-    else rc = persistent_waves(c, KInit<RRH_STACK_PRIVATE>::fn, tiles, waves);
+    rc = persistent_waves<KInit>(c, sk, tiles, waves);
     if (rc) return rc;
     launch<KInit>(sk, dim3(waves), c->stream, s->dev(), *p, F, R, (float4*)hist_pos->p, (float4*)hist_norm->p,
                   (uint32_t*)c->rgi_counter.p, tiles_x, tiles * 64u);
@@ -440,7 +444,7 @@ int rrh_rgi_temporal(rrh_ctx* c, rrh_scene* s, const rrh_rgi_params* p, const rr
     if ((rc = res3(res, F.W, F.H, R, "rrh_rgi_temporal"))) return rc;
     if ((rc = res3(res_prev, F.W, F.H, Rp, "rrh_rgi_temporal"))) return rc;
     if (!p->temporal_enable) return RRH_OK;
-    launch<KTemp>(stack_kind(c, s), dim3((F.W + 7) / 8, (F.H + 7) / 8), c->stream, s->dev(), *p, F, R, Rp,
+    launch<KTemp>(kernel_kind(c, s), dim3((F.W + 7) / 8, (F.H + 7) / 8), c->stream, s->dev(), *p, F, R, Rp,
                   History{(const float4*)hist_prev_pos->p, (const float4*)hist_prev_norm->p},
                   (uint32_t*)c->rgi_counter.p + 1, c->xcd_remap);
     RRH_LAUNCH_CHECK();
@@ -457,7 +461,7 @@ int rrh_rgi_spatial_shade(rrh_ctx* c, rrh_scene* s, const rrh_rgi_params* p, con
             "rrh_rgi_spatial_shade: direct lighting must be an RGBA32F image of the camera's extent");
     Res3 R;
     if ((rc = res3(res, F.W, F.H, R, "rrh_rgi_spatial_shade"))) return rc;
-    launch<KSpat>(stack_kind(c, s), dim3((F.W + 7) / 8, (F.H + 7) / 8), c->stream, s->dev(), *p, F, R,
+    launch<KSpat>(kernel_kind(c, s), dim3((F.W + 7) / 8, (F.H + 7) / 8), c->stream, s->dev(), *p, F, R,
                   direct ? (const float4*)direct->p : (const float4*)nullptr, (float4*)out->p,
                   (uint32_t*)c->rgi_counter.p + 2, c->xcd_remap);
     RRH_LAUNCH_CHECK();

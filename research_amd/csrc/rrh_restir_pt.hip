@@ -10,9 +10,11 @@
 // resamples them by their reconnected contribution and keeps one; the temporal stage
 // reprojects into the previous frame and reconnects its kept sample; the spatial stage
 // reconnects neighbours' samples, shades with one visibility ray and splats the chosen
-// direction into the guiding grid.  Every ray culls back faces (RAY_FLAG_CULL_BACK_FACING_
-// TRIANGLES in the reference): the 4-wide traversal with the culling test (quantized nodes; the
-// visibility rays of the reuse stages read the fp32 4-wide nodes when the context selects them).
+// direction into the guiding grid.  Rays are two-sided: the shader's RAY_FLAG_CULL_BACK_FACING_
+// TRIANGLES is overridden by the reference's CULL_DISABLE instances (AccelStructure.cpp:90);
+// RRH_OPT_RESTIR_CULL selects kernels that cull (a template flag, so each kernel keeps one
+// traversal).  Rays traverse the 4-wide nodes (quantized; the visibility rays of the reuse
+// stages read the fp32 4-wide nodes when the context selects them).
 //
 // Deviations, each for exactness or for the triangle-light scenes:
 //  * 1/sqrtf(x) where the reference writes rsqrt(x), so host and device round alike;
@@ -93,11 +95,13 @@ RRH_D float target(f3 pp, f3 pn, const Mat& pm, f3 cam, const Reservoir& r) {
 // which lanes traverse together changes.
 enum : uint32_t { kNext, kToRc, kShadow, kBounce, kExt, kFinish, kDone };
 
-template <int STK>
+template <int KS>
 __global__ void __launch_bounds__(64, 4) k_rpt_initial(DevScene S, rrh_rpt_params P, HashGrid G, Frame F, Res4 R,
                                                     float4* __restrict__ hpos, float4* __restrict__ hnrm,
                                                     uint32_t* __restrict__ counter, uint32_t tiles_x,
                                                     uint32_t npad) {
+    constexpr int STK = KS & ~kCull;
+    constexpr bool CULL = (KS & kCull) != 0;
     __shared__ int s_stack[StackSel<STK>::kLds * 64];
     uint32_t rays = 0, shadow_rays = 0;
     uint32_t* next_pixel = counter + 3;
@@ -292,9 +296,9 @@ __global__ void __launch_bounds__(64, 4) k_rpt_initial(DevScene S, rrh_rpt_param
         if (!__any(want_ray)) break;  // every lane is out of pixels
         if (want_ray) {
             if (S.speculative)
-                h = trace_closest4spec_s<false, true, true>(S, ro, rd, bias, rtmax, st, nullptr, nullptr, first_hit);
+                h = trace_closest4spec_s<false, true, CULL>(S, ro, rd, bias, rtmax, st, nullptr, nullptr, first_hit);
             else
-                h = trace_closest4_s<false, true, true>(S, ro, rd, bias, rtmax, st, nullptr, nullptr, first_hit);
+                h = trace_closest4_s<false, true, CULL>(S, ro, rd, bias, rtmax, st, nullptr, nullptr, first_hit);
         }
     }
     count_rays(counter, rays);
@@ -304,9 +308,11 @@ __global__ void __launch_bounds__(64, 4) k_rpt_initial(DevScene S, rrh_rpt_param
 // ---------------------------------------------------------------------------
 // cs_temporal, restir_pt.slang:775-872 (reprojection :323-421)
 // ---------------------------------------------------------------------------
-template <int STK>
+template <int KS>
 __global__ void __launch_bounds__(64) k_rpt_temporal(DevScene S, rrh_rpt_params P, Frame F, Res4 R, Res4 Rp,
                                                      History Hp, uint32_t* __restrict__ counter, bool remap) {
+    constexpr int STK = KS & ~kCull;
+    constexpr bool CULL = (KS & kCull) != 0;
     __shared__ int s_stack[StackSel<STK>::kLds * 64];
     uint32_t px, py, rays = 0;
     if (tile_pixel(px, py, F.W, F.H, remap)) {
@@ -331,7 +337,7 @@ __global__ void __launch_bounds__(64) k_rpt_temporal(DevScene S, rrh_rpt_params 
                     if (go) {
                         auto st = make_stack<STK>(s_stack);
                         rays++;
-                        go = visible(S, pp, pn, prv.rc_pos, P.ray_bias, st);
+                        go = visible<CULL>(S, pp, pn, prv.rc_pos, P.ray_bias, st);
                     }
                     if (go) {
                         const Mat pm = load_mat(S, pmi);
@@ -370,10 +376,12 @@ __global__ void __launch_bounds__(64) k_rpt_temporal(DevScene S, rrh_rpt_params 
 // ---------------------------------------------------------------------------
 // cs_spatial_shade, restir_pt.slang:876-1001
 // ---------------------------------------------------------------------------
-template <int STK>
+template <int KS>
 __global__ void __launch_bounds__(64) k_rpt_spatial_shade(DevScene S, rrh_rpt_params P, HashGrid G, Frame F, Res4 R,
                                                           const float4* __restrict__ direct, float4* __restrict__ out,
                                                           uint32_t* __restrict__ counter, bool remap) {
+    constexpr int STK = KS & ~kCull;
+    constexpr bool CULL = (KS & kCull) != 0;
     __shared__ int s_stack[StackSel<STK>::kLds * 64];
     uint32_t px, py, rays = 0;
     if (tile_pixel(px, py, F.W, F.H, remap)) {
@@ -425,7 +433,7 @@ __global__ void __launch_bounds__(64) k_rpt_spatial_shade(DevScene S, rrh_rpt_pa
                     if (lum709(ind) > 0.0f) {
                         auto st = make_stack<STK>(s_stack);
                         rays++;
-                        if (visible(S, pp, pn, res.rc_pos, P.ray_bias, st)) L = add(L, scale(ind, P.indirect_strength));
+                        if (visible<CULL>(S, pp, pn, res.rc_pos, P.ray_bias, st)) L = add(L, scale(ind, P.indirect_strength));
                     }
                     // splat (restir_pt.slang:982-994): the direction toward the kept
                     // reconnection vertex trains next frame's guiding
@@ -516,13 +524,10 @@ int rrh_rpt_initial(rrh_ctx* c, rrh_scene* s, const rrh_rpt_params* p, rrh_grid*
     RRH_HIP(hipMemsetAsync(c->rpt_counter.p, 0, 16, c->stream));
     // persistent waves: as many as the CUs hold at once (the occupancy the kernel's registers and
     // LDS stack allow), never more than there are 8x8 tiles
-    const int sk = stack_kind(c, s);
+    const int sk = kernel_kind(c, s);
     const uint32_t tiles_x = (F.W + 7) / 8, tiles = tiles_x * ((F.H + 7) / 8);
     uint32_t waves = 0;
-    if (sk == RRH_STACK_LDS32) rc = persistent_waves(c, KInit<RRH_STACK_LDS32>::fn, tiles, waves);
-    else if (sk == RRH_STACK_HYBRID16) rc = persistent_waves(c, KInit<RRH_STACK_HYBRID16>::fn, tiles, waves);
-    else if (sk == RRH_STACK_HYBRID32) rc = persistent_waves(c, KInit<RRH_STACK_HYBRID32>::fn, tiles, waves);
-    else rc = persistent_waves(c, KInit<RRH_STACK_PRIVATE>::fn, tiles, waves);
+    rc = persistent_waves<KInit>(c, sk, tiles, waves);
     if (rc) return rc;
     launch<KInit>(sk, dim3(waves), c->stream, s->dev(), *p, G, F, R, (float4*)hist_pos->p, (float4*)hist_norm->p,
                   (uint32_t*)c->rpt_counter.p, tiles_x, tiles * 64u);
@@ -542,7 +547,7 @@ int rrh_rpt_temporal(rrh_ctx* c, rrh_scene* s, const rrh_rpt_params* p, const rr
     if ((rc = res4(res, F.W, F.H, R, "rrh_rpt_temporal"))) return rc;
     if ((rc = res4(res_prev, F.W, F.H, Rp, "rrh_rpt_temporal"))) return rc;
     if (!p->temporal_enable) return RRH_OK;
-    launch<KTemp>(stack_kind(c, s), dim3((F.W + 7) / 8, (F.H + 7) / 8), c->stream, s->dev(), *p, F, R, Rp,
+    launch<KTemp>(kernel_kind(c, s), dim3((F.W + 7) / 8, (F.H + 7) / 8), c->stream, s->dev(), *p, F, R, Rp,
                   History{(const float4*)hist_prev_pos->p, (const float4*)hist_prev_norm->p},
                   (uint32_t*)c->rpt_counter.p + 1, c->xcd_remap);
     RRH_LAUNCH_CHECK();
@@ -561,7 +566,7 @@ int rrh_rpt_spatial_shade(rrh_ctx* c, rrh_scene* s, const rrh_rpt_params* p, rrh
     if ((rc = res4(res, F.W, F.H, R, "rrh_rpt_spatial_shade"))) return rc;
     HashGrid G;
     if ((rc = hash_grid(grid, G, "rrh_rpt_spatial_shade"))) return rc;
-    launch<KSpat>(stack_kind(c, s), dim3((F.W + 7) / 8, (F.H + 7) / 8), c->stream, s->dev(), *p, G, F, R,
+    launch<KSpat>(kernel_kind(c, s), dim3((F.W + 7) / 8, (F.H + 7) / 8), c->stream, s->dev(), *p, G, F, R,
                   direct ? (const float4*)direct->p : (const float4*)nullptr, (float4*)out->p,
                   (uint32_t*)c->rpt_counter.p + 2, c->xcd_remap);
     RRH_LAUNCH_CHECK();

@@ -12,12 +12,17 @@
 namespace rrh {
 namespace reuse {
 
-// trace_ray (RAY_FLAG_CULL_BACK_FACING_TRIANGLES) as an occlusion query: only triangles facing
-// the ray occlude
-template <class Stk>
-RRH_D bool occluded_cull(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& st) {
-    if (S.node_fmt == RRH_NODES_WIDE4) return trace_occluded4_s<false, true>(S, o, d, tmin, tmax, st);
-    return trace_occluded4_s<true, true>(S, o, d, tmin, tmax, st);
+// Kernel template flag: rays cull back faces (RRH_OPT_RESTIR_CULL).  The shaders ask for
+// RAY_FLAG_CULL_BACK_FACING_TRIANGLES (restir_pt.slang:104,111, restir_gi.slang:75,82), but the
+// reference's TLAS instances carry TRIANGLE_FACING_CULL_DISABLE (AccelStructure.cpp:90), so by
+// default every ray is two-sided; the flag rides in the kernels' stack-kind template argument.
+constexpr int kCull = 16;
+
+// trace_ray as an occlusion query (CULL: only triangles facing the ray occlude)
+template <bool CULL, class Stk>
+RRH_D bool occluded(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& st) {
+    if (S.node_fmt == RRH_NODES_WIDE4) return trace_occluded4_s<false, CULL>(S, o, d, tmin, tmax, st);
+    return trace_occluded4_s<true, CULL>(S, o, d, tmin, tmax, st);
 }
 
 // sample_lights (restir_pt.slang:171-222, restir_gi.slang:143-193) up to its shadow ray: c = the
@@ -91,12 +96,12 @@ RRH_D float jacobian(f3 src, f3 dst, f3 rc, f3 rn) {
 }
 
 // trace_visibility (restir_pt.slang:449-458, restir_gi.slang:553-564)
-template <class Stk>
+template <bool CULL, class Stk>
 RRH_D bool visible(const DevScene& S, f3 o, f3 n, f3 target_pos, float bias, Stk& st) {
     const f3 delta = sub(target_pos, o);
     const float dist = length(delta);
     if (dist <= bias) return false;
-    return !occluded_cull(S, add(o, scale(n, bias)), divs(delta, dist), bias, dist - bias, st);
+    return !occluded<CULL>(S, add(o, scale(n, bias)), divs(delta, dist), bias, dist - bias, st);
 }
 
 // load_surface_from_images (restir_pt.slang:285-305, restir_gi.slang:253-276): w < 0 is a miss
@@ -233,24 +238,44 @@ inline int check_frame(rrh_ctx* c, rrh_scene* s, const rrh_image* gpos, const rr
     return RRH_OK;
 }
 
-// culled rays traverse 4-wide nodes, so the stack must hold what the 4-wide tree can need
-inline int stack_kind(const rrh_ctx* c, const rrh_scene* s) { return stack_for(c, s->stack4); }
+// the kernels' template argument: stack kind (these rays traverse 4-wide nodes, so the stack
+// must hold what the 4-wide tree can need) plus kCull when back faces are culled
+inline int kernel_kind(const rrh_ctx* c, const rrh_scene* s) {
+    return stack_for(c, s->stack4) | (c->restir_cull ? kCull : 0);
+}
+
+// calls f with the kernel K<kind>::fn for a run-time kind
+template <template <int> class K, class Fn>
+inline int with_kernel(int kind, Fn&& f) {
+    switch (kind) {
+        case RRH_STACK_LDS32: return f(K<RRH_STACK_LDS32>::fn);
+        case RRH_STACK_HYBRID16: return f(K<RRH_STACK_HYBRID16>::fn);
+        case RRH_STACK_HYBRID32: return f(K<RRH_STACK_HYBRID32>::fn);
+        case RRH_STACK_PRIVATE | kCull: return f(K<RRH_STACK_PRIVATE | kCull>::fn);
+        case RRH_STACK_LDS32 | kCull: return f(K<RRH_STACK_LDS32 | kCull>::fn);
+        case RRH_STACK_HYBRID16 | kCull: return f(K<RRH_STACK_HYBRID16 | kCull>::fn);
+        case RRH_STACK_HYBRID32 | kCull: return f(K<RRH_STACK_HYBRID32 | kCull>::fn);
+        default: return f(K<RRH_STACK_PRIVATE>::fn);
+    }
+}
 
 template <template <int> class K, class... A>
-void launch(int sk, dim3 g, hipStream_t st, A... a) {
-    if (sk == RRH_STACK_LDS32) hipLaunchKernelGGL(K<RRH_STACK_LDS32>::fn, g, dim3(64), 0, st, a...);
-    else if (sk == RRH_STACK_HYBRID16) hipLaunchKernelGGL(K<RRH_STACK_HYBRID16>::fn, g, dim3(64), 0, st, a...);
-    else if (sk == RRH_STACK_HYBRID32) hipLaunchKernelGGL(K<RRH_STACK_HYBRID32>::fn, g, dim3(64), 0, st, a...);
-    else hipLaunchKernelGGL(K<RRH_STACK_PRIVATE>::fn, g, dim3(64), 0, st, a...);
+void launch(int kind, dim3 g, hipStream_t st, A... a) {
+    with_kernel<K>(kind, [&](auto fn) {
+        hipLaunchKernelGGL(fn, g, dim3(64), 0, st, a...);
+        return 0;
+    });
 }
 
 // persistent waves for a kernel: as many as the CUs hold at once, never more than `tiles`
-template <class Kernel>
-inline int persistent_waves(const rrh_ctx* c, Kernel k, uint32_t tiles, uint32_t& waves) {
-    int per_cu = 0;
-    RRH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64, 0));
-    waves = std::min<uint32_t>(tiles, (uint32_t)c->cu_count * (uint32_t)std::max(per_cu, 1));
-    return RRH_OK;
+template <template <int> class K>
+inline int persistent_waves(const rrh_ctx* c, int kind, uint32_t tiles, uint32_t& waves) {
+    return with_kernel<K>(kind, [&](auto fn) {
+        int per_cu = 0;
+        RRH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, 0));
+        waves = std::min<uint32_t>(tiles, (uint32_t)c->cu_count * (uint32_t)std::max(per_cu, 1));
+        return RRH_OK;
+    });
 }
 
 }  // namespace reuse

@@ -44,3 +44,43 @@ def random_rays(n: int, lo, hi, seed: int = 0) -> np.ndarray:
 
 def prim_ids(h: np.ndarray) -> np.ndarray:
     return np.ascontiguousarray(h[:, 1]).view(np.uint32)
+
+
+class restir_cull:
+    """ReSTIR PT / GI rays cull back faces in both the library and the oracle while active
+    (default: two-sided, as the reference's CULL_DISABLE instances trace)."""
+
+    def __init__(self, ctx, on: int):
+        self.ctx, self.on = ctx, int(on)
+
+    def __enter__(self):
+        L.call("rrh_ctx_set_option", self.ctx.h, L.OPT_RESTIR_CULL, self.on)
+        O.lib().orc_set_restir_cull(self.on)
+        return self
+
+    def __exit__(self, *exc):
+        L.call("rrh_ctx_set_option", self.ctx.h, L.OPT_RESTIR_CULL, 0)
+        O.lib().orc_set_restir_cull(0)
+        return False
+
+
+def cornell_sun_mr():
+    """The reference Cornell box with metallic-roughness materials and, beside its point light, a
+    directional light entering through the open front: the path tracer's directional NEE uses
+    brdf_pdf for its MIS weight (pathtracer.slang:146-150,171-173) and ReSTIR DI takes its
+    directional branches (restir_di.slang:169-173, 444-448)."""
+    sa = core.generate_scene(L.GEN_CORNELL_REF)
+    mats = sa.mats.copy()
+    mats["metallic"] = [0.0, 0.7, 0.3, 0.0]
+    mats["roughness"] = [0.6, 0.25, 0.45, 1.0]
+    sun = np.zeros(1, L.LIGHT)
+    sun["type"], sun["direction"], sun["emission"], sun["intensity"] = 1, (-0.25, -0.35, -1.0), (3.0, 3.0, 2.5), 1.0
+    sa.mats = mats
+    sa.lights = np.concatenate([sa.lights, sun])
+    assert not (mats["flags"] & L.MAT_PURE_LAMBERT).any()
+    return sa
+
+
+def scene_arrays(kind, seed=1, scale=1.0):
+    """A generated scene, or a test scene when `kind` is a function returning SceneArrays."""
+    return kind() if callable(kind) else core.generate_scene(kind, seed, scale)

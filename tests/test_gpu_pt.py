@@ -8,7 +8,7 @@ import pytest
 from oracle import oracle as O
 from research_amd import _lib as L
 from research_amd import core
-from tests.helpers import ocam
+from tests.helpers import cornell_sun_mr, ocam, scene_arrays
 
 pytestmark = pytest.mark.gpu
 
@@ -28,12 +28,13 @@ CASES = [
     ("heightfield", L.GEN_HEIGHTFIELD, 2, 0.1, 64, 36, 8),
     ("indoor", L.GEN_INDOOR, 4, 0.1, 48, 32, 12),
     ("indoor_full", L.GEN_INDOOR, 1, 1.0, 40, 24, 12),
+    ("cornell_sun_mr", cornell_sun_mr, 1, 1.0, 64, 48, 8),
 ]
 
 
 @pytest.mark.parametrize("name,kind,seed,scale,W,H,depth", CASES)
 def test_pt_matches_oracle(ctx, name, kind, seed, scale, W, H, depth):
-    sa = core.generate_scene(kind, seed, scale)
+    sa = scene_arrays(kind, seed, scale)
     gs = core.Scene(ctx, sa)
     os_ = O.Scene.from_arrays(sa)
     cam = core.camera_look_at(sa.eye, sa.center, sa.fov, W, H)
@@ -41,12 +42,9 @@ def test_pt_matches_oracle(ctx, name, kind, seed, scale, W, H, depth):
     for frame in (0, 7):
         g, gc = render_gpu(ctx, gs, cam, depth, frame)
         o, oc = os_.pt_render(ocam(cam), depth, frame, np.arange(W * H, dtype=np.uint32))
-        exact = np.all(g.view(np.uint32) == o.view(np.uint32), axis=1).mean()
-        # identical arithmetic: every pixel is expected bit-exact; allow 0.1% for rare
-        # divergences and bound their size
-        assert exact >= 0.999, (name, frame, exact)
-        rel = np.abs(g - o).max(1) / np.maximum(np.abs(o).max(1), 1e-3)
-        assert np.quantile(rel, 0.999) < 1e-4
+        # identical arithmetic in the same order: every pixel bit for bit
+        diff = np.flatnonzero(~np.all(g.view(np.uint32) == o.view(np.uint32), axis=1))
+        assert diff.size == 0, (name, frame, diff[:10], g[diff[:3]], o[diff[:3]])
         assert gc == tuple(int(x) for x in oc), (gc, oc)
         assert np.isfinite(g).all() and (g[:, 3] == 1.0).all()
 

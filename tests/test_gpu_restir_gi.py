@@ -9,7 +9,7 @@ import pytest
 from oracle import oracle as O
 from research_amd import _lib as L
 from research_amd import core
-from tests.helpers import ocam
+from tests.helpers import ocam, restir_cull
 from tests.test_gpu_restir_pt import CASES, _bits, _cams, _res_same_frac, _same_frac
 
 pytestmark = pytest.mark.gpu
@@ -21,7 +21,13 @@ def _params(frame, M=6, temporal=1, spatial=1, direct=1):
 
 
 @pytest.mark.parametrize("kind,seed,scale,step,W,H", CASES)
-def test_restir_gi_stages_match_oracle(ctx, kind, seed, scale, step, W, H):
+@pytest.mark.parametrize("cull", [0, 1])
+def test_restir_gi_stages_match_oracle(ctx, kind, seed, scale, step, W, H, cull):
+    with restir_cull(ctx, cull):
+        _restir_gi_stages_match_oracle(ctx, kind, seed, scale, step, W, H)
+
+
+def _restir_gi_stages_match_oracle(ctx, kind, seed, scale, step, W, H):
     sa = core.generate_scene(kind, seed, scale)
     gs = core.Scene(ctx, sa)
     os_ = O.Scene.from_arrays(sa)

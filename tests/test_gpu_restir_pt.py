@@ -11,7 +11,7 @@ import pytest
 from oracle import oracle as O
 from research_amd import _lib as L
 from research_amd import core
-from tests.helpers import ocam
+from tests.helpers import ocam, restir_cull
 
 pytestmark = pytest.mark.gpu
 
@@ -70,7 +70,13 @@ CASES = [
 
 
 @pytest.mark.parametrize("kind,seed,scale,step,W,H", CASES)
-def test_restir_pt_stages_match_oracle(ctx, kind, seed, scale, step, W, H):
+@pytest.mark.parametrize("cull", [0, 1])
+def test_restir_pt_stages_match_oracle(ctx, kind, seed, scale, step, W, H, cull):
+    with restir_cull(ctx, cull):
+        _restir_pt_stages_match_oracle(ctx, kind, seed, scale, step, W, H)
+
+
+def _restir_pt_stages_match_oracle(ctx, kind, seed, scale, step, W, H):
     sa = core.generate_scene(kind, seed, scale)
     gs = core.Scene(ctx, sa)
     os_ = O.Scene.from_arrays(sa)

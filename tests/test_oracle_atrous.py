@@ -147,3 +147,39 @@ def test_config_mode_miss_tap_and_miss_centre():
     assert np.array_equal(out[cy, cx, :3], rad[cy, cx, :3])
     assert out[cy, cx, 3] == rad[cy, cx, 3]     # w_c^2 var / w_c^2
     assert np.array_equal(out[3, 3], rad[3, 3])
+
+
+def test_prepare_variance_depth_and_packing():
+    """rrh_atrous_prepare's restatement: alpha = population variance of Rec.709 luminance over the
+    valid pixels of the clamped 3x3 window (edge pixels repeat at the borders), depth = (p - eye) .
+    forward (0 on a miss), everything rounded once to fp16."""
+    rng = np.random.default_rng(4)
+    H, W = 23, 31
+    rad = rng.uniform(0.0, 2.0, (H, W, 4)).astype(np.float32)
+    gpos = np.zeros((H, W, 4), np.float32)
+    gpos[..., :3] = rng.uniform(-3, 3, (H, W, 3))
+    gpos[..., 3] = 1.0
+    gpos[rng.random((H, W)) < 0.2, 3] = -1.0
+    gpos[0:3, 0:3, 3] = -1.0                   # a corner whose window is all misses
+    gnrm = rng.normal(size=(H, W, 4)).astype(np.float32)
+    eye = np.array([0.5, -0.2, 6.0])
+    fwd = np.array([0.1, 0.2, -1.0]) / np.linalg.norm([0.1, 0.2, -1.0])
+    r16, g16 = O.atrous_prepare(rad, gpos, gnrm, eye, fwd)
+    r16, g16 = r16.view(np.float16), g16.view(np.float16)
+    lum = rad[..., :3].astype(np.float64) @ np.array([0.2126, 0.7152, 0.0722])
+    want_var = np.zeros((H, W))
+    for y in range(H):
+        for x in range(W):
+            ls = [lum[min(max(y + dy, 0), H - 1), min(max(x + dx, 0), W - 1)]
+                  for dy in (-1, 0, 1) for dx in (-1, 0, 1)
+                  if gpos[min(max(y + dy, 0), H - 1), min(max(x + dx, 0), W - 1), 3] >= 0]
+            want_var[y, x] = np.var(ls) if ls else 0.0
+    got_var = r16[..., 3].astype(np.float64)
+    assert np.all(np.abs(got_var - want_var) <= want_var * 2.0 ** -10 + 2e-6)
+    assert (got_var[0:2, 0:2] == 0).all()
+    assert np.array_equal(r16[..., :3], rad[..., :3].astype(np.float16))     # one RNE rounding
+    assert np.array_equal(g16[..., :3], gnrm[..., :3].astype(np.float16))
+    z = np.where(gpos[..., 3] >= 0, (gpos[..., :3].astype(np.float64) - eye) @ fwd, 0.0)
+    got_z = g16[..., 3].astype(np.float64)
+    assert np.all(np.abs(got_z - z) <= np.abs(z) * 2.0 ** -10 + 1e-6)
+    assert (got_z[gpos[..., 3] < 0] == 0).all()

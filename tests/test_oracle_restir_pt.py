@@ -24,8 +24,9 @@ from tests.helpers import ocam
 RHO, LE = 0.5, 1.0
 
 
-def furnace_box(rho=RHO, le=LE):
-    """Cube [-1, 1]^3 of 12 triangles facing inward, one pure-Lambert emissive material."""
+def furnace_box(rho=RHO, le=LE, outward=False):
+    """Cube [-1, 1]^3 of 12 triangles facing inward (or wound outward: geometric normals out,
+    vertex normals still in), one pure-Lambert emissive material."""
     tris = []
     for a in range(3):
         b, c = (a + 1) % 3, (a + 2) % 3
@@ -39,7 +40,7 @@ def furnace_box(rho=RHO, le=LE):
             inward[a] = -s
             for t in ((q[0], q[1], q[2]), (q[0], q[2], q[3])):
                 t = np.array(t)
-                if np.dot(np.cross(t[1] - t[0], t[2] - t[0]), inward) < 0:
+                if (np.dot(np.cross(t[1] - t[0], t[2] - t[0]), inward) < 0) != outward:
                     t = t[[0, 2, 1]]
                 tris.append((t, inward))
     v = np.zeros(len(tris) * 3, L.VERTEX)
@@ -59,6 +60,15 @@ def furnace_box(rho=RHO, le=LE):
     return core.SceneArrays(v, idx, mesh, mat, inst, np.zeros(0, L.LIGHT), 0, (0.3, 0.2, 0.9), (0.0, 0.0, 0.0), 60.0)
 
 
+@pytest.fixture()
+def culled():
+    """Rays cull back faces (RAY_FLAG_CULL_BACK_FACING_TRIANGLES without the instances'
+    CULL_DISABLE): at ray bias 0 only culling keeps a ray from re-hitting the wall it leaves."""
+    O.lib().orc_set_restir_cull(1)
+    yield
+    O.lib().orc_set_restir_cull(0)
+
+
 def _setup(W=48, H=36):
     sa = furnace_box()
     os_ = O.Scene.from_arrays(sa)
@@ -70,9 +80,9 @@ def _setup(W=48, H=36):
 
 
 def _prm(M, bounces, guiding=0, temporal=1, bias=0.0):
-    """bias 0: the walls face inward and every ray culls back faces, so no ray can hit the
-    surface it leaves, and the sampled direction is exactly the one toward the hit (with a bias
-    the offset origin tilts it by about bias / distance, and rays leak past corners)."""
+    """bias 0 (with culled rays): the walls face inward, so no ray can hit the surface it
+    leaves, and the sampled direction is exactly the one toward the hit (with a bias the offset
+    origin tilts it by about bias / distance, and rays leak past corners)."""
     return O.RPTParams(3, M, bounces, temporal, 0, 4, 12, 0, 1.0, bias, guiding, 0.5)
 
 
@@ -86,7 +96,7 @@ def _estimate(gp, gn, res):
 
 
 @pytest.mark.parametrize("M,bounces", [(1, 1), (8, 1), (8, 3)])
-def test_furnace_every_reservoir_estimates_rho_le(M, bounces):
+def test_furnace_every_reservoir_estimates_rho_le(culled, M, bounces):
     os_, oc, gp, gn = _setup()
     res, hp, hn = os_.rpt_initial(oc, _prm(M, bounces), gp, gn)
     assert (res[3][:, 3] == 1).all()
@@ -106,6 +116,41 @@ def test_ray_bias_leaks_only_at_corners():
     assert near.all()
 
 
+@pytest.mark.parametrize("M,bounces", [(1, 1), (8, 3)])
+def test_two_sided_rays_ignore_winding(M, bounces):
+    """The reference traces two-sided (CULL_DISABLE instances, AccelStructure.cpp:90): a box
+    wound outward gives the same reservoirs as one wound inward, up to the rounding of the
+    swapped barycentrics, and the furnace estimate rho Le holds away from the corners (a bias
+    of 1e-3 tilts a direction by about bias / distance).  Culled rays leave the outward-wound
+    box through its walls: every candidate escapes and is dropped before it is counted."""
+    prm = _prm(M, bounces, bias=1e-3)
+    os_in, oc, gp, gn = _setup()
+    os_out = O.Scene.from_arrays(furnace_box(outward=True))   # same surfaces, G-buffer of the inward box
+    res_in, _, _ = os_in.rpt_initial(oc, prm, gp, gn)
+    res_out, _, _ = os_out.rpt_initial(oc, prm, gp, gn)
+    ok = (res_in[3][:, 3] == 1) & (res_out[3][:, 3] == 1)
+    assert ok.mean() > 0.99
+    assert (res_in[1][ok, 3].view(np.uint32) == res_out[1][ok, 3].view(np.uint32)).mean() > 0.99
+    for k in range(3):
+        assert np.allclose(res_in[k][ok], res_out[k][ok], rtol=1e-4, atol=1e-4)
+    # the offset origin moves cos toward the hit by about bias / (distance cos): keep reconnections
+    # at distance > .5 and cos > .2 (relative error < 1e-2)
+    to = res_out[0][:, :3] - gp[:, :3]
+    dist = np.linalg.norm(to, axis=1)
+    cos = (gn[:, :3] * to).sum(1) / np.maximum(dist, 1e-9)
+    far = ok & (dist > 0.5) & (cos > 0.2)
+    assert far.mean() > 0.5
+    d = np.abs(_estimate(gp, gn, res_out)[far] / (RHO * LE) - 1)
+    # one candidate: every pixel within 1e-2; eight: W also carries the others' (grazing) tilt
+    assert (d.max() < 1e-2) if M == 1 else (np.median(d) < 2e-3 and np.quantile(d, 0.9) < 6e-3), d
+    O.lib().orc_set_restir_cull(1)
+    try:
+        res_cull, _, _ = os_out.rpt_initial(oc, prm, gp, gn)
+    finally:
+        O.lib().orc_set_restir_cull(0)
+    assert (res_cull[3][:, 3] == 0).all() and (res_cull[1][:, 3].view(np.uint32) == 0).all()
+
+
 def _fib_sphere(n):
     i = np.arange(n) + 0.5
     z = 1 - 2 * i / n
@@ -114,7 +159,7 @@ def _fib_sphere(n):
     return np.stack([r * np.cos(phi), r * np.sin(phi), z], 1)
 
 
-def test_furnace_guided_sampling_matches_its_closed_form():
+def test_furnace_guided_sampling_matches_its_closed_form(culled):
     """One-sample MIS with a vMF lobe: candidates whose direction falls below the surface are
     dropped before they are counted in M (restir_pt.slang:618-619, 731-732), so the estimate
     is conditioned on acceptance: E = rho Le (1 - (1 - P)^M) / P per pixel, with P = alpha P_vmf(n . w
