@@ -222,6 +222,16 @@ typedef struct { uint64_t primary, extension, shadow; } rrh_ray_counts;
 int rrh_pt_render(rrh_ctx* ctx, rrh_scene* s, const rrh_pt_params* p, rrh_image* radiance);
 int rrh_pt_render_guided(rrh_ctx* ctx, rrh_scene* s, const rrh_pt_params* p, rrh_grid* grid,
                          rrh_mlp* mlp, rrh_image* radiance);
+/* rrh_pt_render without any device allocation.  No reference counterpart: for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The ray queues, hits, shadow queue and counters live in one file-scope device
+ * array per GPU (42 MB, held until the process ends) instead of context buffers; the result and
+ * the ray counts equal rrh_pt_render's bit for bit.  Limits: at most 262,144 pixels (512x512;
+ * larger extents return RRH_E_RANGE), unguided (guide_mode RRH_GUIDE_NONE, record_vertices 0),
+ * flags 0, RRH_OPT_RAY_SORT off, no traversal profiling.  Every call on a GPU shares that GPU's
+ * array (the last call's contents stay in it): calls that overlap on one GPU, from any context
+ * or stream, race and must be ordered by the caller. */
+int rrh_pt_render_noalloc(rrh_ctx* ctx, rrh_scene* s, const rrh_pt_params* p, rrh_image* radiance);
 /* ray counts of the last render on this context (reads back device counters) */
 int rrh_pt_ray_counts(rrh_ctx* ctx, rrh_ray_counts* out);
 /* rays traced since the last reset by every render / ReSTIR DI call on this context,

@@ -159,6 +159,7 @@ SIGNATURES = {
     "rrh_copy_to_device": [P, P, P, sz],
     "rrh_copy_to_host": [P, P, P, sz],
     "rrh_pt_render": [P, P, C.POINTER(PTParams), P],
+    "rrh_pt_render_noalloc": [P, P, C.POINTER(PTParams), P],
     "rrh_pt_render_guided": [P, P, C.POINTER(PTParams), P, P, P],
     "rrh_pt_ray_counts": [P, C.POINTER(RayCounts)],
     "rrh_ray_totals": [P, C.POINTER(RayCounts), i32],

@@ -96,6 +96,14 @@ int rrh_ctx_create(int device, rrh_ctx** out) {
         e = hipEventCreate(&c->markers[i]);
         if (e != hipSuccess) { set_error(hipGetErrorString(e)); return RRH_E_HIP; }
     }
+    // device-side ray totals and fetch counters exist from the start, so no frame allocates them
+    int rc = c->totals.ensure(32);
+    if (!rc) rc = c->fetch.ensure(32);
+    if (!rc) rc = pt_scratch_init(device, &c->noalloc_scratch);
+    if (rc) { rrh_ctx_destroy(c); return rc; }
+    e = hipMemsetAsync(c->totals.p, 0, 32, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->fetch.p, 0, 32, c->stream);
+    if (e != hipSuccess) { set_error(hipGetErrorString(e)); rrh_ctx_destroy(c); return RRH_E_HIP; }
     *out = c;
     return RRH_OK;
 }

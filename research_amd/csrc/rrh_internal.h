@@ -66,6 +66,8 @@ int build_bvh_device(rrh_ctx* c, rrh_scene* s, const float* tw, uint32_t max_lea
 int flatten_device(rrh_ctx* c, rrh_scene* s);
 // every box of the three node formats recomputed bottom-up from s->twd, topology kept
 int refit_bvh_device(rrh_ctx* c, rrh_scene* s);
+// this device's address of rrh_pt_render_noalloc's scratch array (device must be current)
+int pt_scratch_init(int device, void** out);
 
 // procedural scene generators (rrh_scenegen.cpp)
 struct SceneData {
@@ -107,6 +109,8 @@ struct rrh_ctx {
     uint32_t stamp = 0;           // vertex-record generation
     uint32_t last_rec_bounces = 0;
     uint32_t last_max_bounces = 0;
+    const uint32_t* last_counters = nullptr;  // per-bounce counters of the last path-traced frame
+    void* noalloc_scratch = nullptr;         // this device's rrh_pt_render_noalloc scratch (hipGetSymbolAddress)
     uint32_t last_record_stride = 0;
     uint32_t last_npix = 0;
     rrh::DevBuf di_counter;

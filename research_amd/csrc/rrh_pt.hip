@@ -10,6 +10,7 @@
 // jitter(2) | per bounce: [light pick (+2 for a triangle light)] [BSDF 2] [RR 1].
 // Queues are compacted with one wave-aggregated atomic per wave.
 #include <cstring>
+#include <mutex>
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include "rrh_internal.h"
@@ -27,6 +28,31 @@ constexpr uint32_t kBlock = 256;
 struct Q {
     float4 *o, *d, *thp;
 };
+
+// the frame's device scratch: ray queues (ping-pong), hits, shadow queue, counters
+struct Bufs {
+    Q q[2];
+    float4 *hits, *sh_o, *sh_d, *sh_c;
+    uint32_t* cnt;
+};
+
+// Scratch of rrh_pt_render_noalloc: the same ten float4 planes and the counters, for frames of
+// up to kNoAllocMaxPixels, in a file-scope device array (each GPU holds its own copy until the
+// process ends).  Host code uses the per-device address from hipGetSymbolAddress, never the
+// array's name.
+constexpr uint32_t kNoAllocMaxPixels = 512u * 512u;
+constexpr size_t kNoAllocPlane = (size_t)kNoAllocMaxPixels * 16u;
+constexpr size_t kNoAllocBytes = 10u * kNoAllocPlane + 256u * 4u;
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {};
+
+__global__ void k_zero_u32(uint32_t* __restrict__ p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
 
 // one 8x8 pixel tile per wave; the queue is filled in tile order (no atomics): for an
 // image whose sides are multiples of 8 the slot is tile * 64 + lane, otherwise row-major
@@ -419,9 +445,8 @@ uint32_t resident_blocks(rrh_ctx* c, K kernel) {
 
 template <int STK>
 int launch_trace(rrh_ctx* c, const DevScene& S, Q& qi, uint32_t* cnt, uint32_t b, uint32_t g, bool count,
-                 const uint32_t* perm) {
+                 const uint32_t* perm, float4* hp) {
     unsigned long long* fc = (unsigned long long*)c->fetch.p;
-    float4* hp = (float4*)c->hits.p;
     if (c->persistent && S.node_fmt == RRH_NODES_BINARY) {
         uint32_t* next = cnt + 128 + b;
         if (count) {
@@ -444,17 +469,18 @@ int launch_trace(rrh_ctx* c, const DevScene& S, Q& qi, uint32_t* cnt, uint32_t b
     return RRH_OK;
 }
 template <int STK>
-int launch_shadow(rrh_ctx* c, const DevScene& S, uint32_t* cnt, uint32_t b, uint32_t g, float4* rad, const VertexRec& rec) {
+int launch_shadow(rrh_ctx* c, const DevScene& S, uint32_t* cnt, uint32_t b, uint32_t g, float4* rad, const VertexRec& rec,
+                  const Bufs& B) {
     if (c->persistent && S.node_fmt == RRH_NODES_BINARY) {
         const uint32_t gp = std::min(g, resident_blocks(c, k_shadow_p<STK>));
         hipLaunchKernelGGL(k_shadow_p<STK>, dim3(gp), dim3(kBlock), 0, c->stream, S, (const uint32_t*)cnt, b,
-                           (const float4*)c->sh_o.p, (const float4*)c->sh_d.p, (const float4*)c->sh_c.p, rad, rec,
+                           (const float4*)B.sh_o, (const float4*)B.sh_d, (const float4*)B.sh_c, rad, rec,
                            cnt + 192 + b);
         RRH_LAUNCH_CHECK();
         return RRH_OK;
     }
     hipLaunchKernelGGL(k_shadow<STK>, dim3(g), dim3(kBlock), 0, c->stream, S, (const uint32_t*)cnt, b,
-                       (const float4*)c->sh_o.p, (const float4*)c->sh_d.p, (const float4*)c->sh_c.p, rad, rec,
+                       (const float4*)B.sh_o, (const float4*)B.sh_d, (const float4*)B.sh_c, rad, rec,
                        c->xcd_remap);
     RRH_LAUNCH_CHECK();
     return RRH_OK;
@@ -491,18 +517,8 @@ uint32_t grid_for(uint32_t n) {
     return g < 1u ? 1u : (g > 4096u ? 4096u : g);
 }
 
-}  // namespace
-
-int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* grid, rrh_mlp* mlp,
-                       rrh_image* radiance) {
-    RRH_ARG(c && s && p && radiance, "rrh_pt_render: null argument");
-    RRH_ARG(s->cam_set, "rrh_pt_render: camera not set (rrh_camera_set)");
-    RRH_ARG(radiance->fmt == RRH_FMT_RGBA32F, "rrh_pt_render: radiance must be RGBA32F");
-    uint32_t W = s->cam_host.screen_width, H = s->cam_host.screen_height;
-    RRH_ARG(radiance->w == W && radiance->h == H, "rrh_pt_render: image extent differs from the camera");
-    RRH_ARG(p->max_bounces >= 1 && p->max_bounces <= kMaxBounces, "rrh_pt_render: max_bounces must be in [1, 32]");
-    RRH_ARG(p->guide_mode <= RRH_GUIDE_MLP, "rrh_pt_render: bad guide_mode");
-    uint32_t npix = W * H;
+// the context's frame buffers, allocated on first use and grown with the extent
+int context_bufs(rrh_ctx* c, uint32_t npix, Bufs& B) {
     int rc = 0;
     for (int k = 0; k < 2; ++k) {
         if ((rc = c->q_o[k].ensure((size_t)npix * 16))) return rc;
@@ -523,14 +539,34 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
         RRH_HIP(rocprim::radix_sort_pairs(nullptr, tb, k, k + npix, v, v + npix, npix, 0, 32, c->stream));
         if ((rc = c->sort_tmp.ensure(tb))) return rc;
     }
-    if (!c->totals.p) {
-        if ((rc = c->totals.ensure(32))) return rc;
-        RRH_HIP(hipMemsetAsync(c->totals.p, 0, 32, c->stream));
-    }
-    if (!c->fetch.p) {
-        if ((rc = c->fetch.ensure(32))) return rc;
-        RRH_HIP(hipMemsetAsync(c->fetch.p, 0, 32, c->stream));
-    }
+    for (int k = 0; k < 2; ++k) B.q[k] = Q{(float4*)c->q_o[k].p, (float4*)c->q_d[k].p, (float4*)c->q_thp[k].p};
+    B.hits = (float4*)c->hits.p;
+    B.sh_o = (float4*)c->sh_o.p;
+    B.sh_d = (float4*)c->sh_d.p;
+    B.sh_c = (float4*)c->sh_c.p;
+    B.cnt = (uint32_t*)c->counters.p;
+    return RRH_OK;
+}
+
+}  // namespace
+
+// fixed == nullptr: the context's buffers (allocated on first use, grown with the extent);
+// otherwise the caller's buffers, and nothing is allocated: unguided frames only, no ray sort,
+// no fetch counting, no profiling events, counters cleared by a kernel
+static int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* grid, rrh_mlp* mlp,
+                              rrh_image* radiance, const Bufs* fixed = nullptr) {
+    RRH_ARG(c && s && p && radiance, "rrh_pt_render: null argument");
+    RRH_ARG(s->cam_set, "rrh_pt_render: camera not set (rrh_camera_set)");
+    RRH_ARG(radiance->fmt == RRH_FMT_RGBA32F, "rrh_pt_render: radiance must be RGBA32F");
+    uint32_t W = s->cam_host.screen_width, H = s->cam_host.screen_height;
+    RRH_ARG(radiance->w == W && radiance->h == H, "rrh_pt_render: image extent differs from the camera");
+    RRH_ARG(p->max_bounces >= 1 && p->max_bounces <= kMaxBounces, "rrh_pt_render: max_bounces must be in [1, 32]");
+    RRH_ARG(p->guide_mode <= RRH_GUIDE_MLP, "rrh_pt_render: bad guide_mode");
+    uint32_t npix = W * H;
+    int rc = 0;
+    Bufs Bf;
+    if (fixed) Bf = *fixed;
+    else if ((rc = context_bufs(c, npix, Bf))) return rc;
     ShadeArgs A;
     A.max_bounces = p->max_bounces;
     A.rr_start = p->rr_start;
@@ -539,11 +575,15 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
     A.remap = c->xcd_remap;
     if ((rc = guide_prepare(c, s, p, grid, mlp, A.guide, A.rec))) return rc;
     DevScene S = s->dev();
-    uint32_t* cnt = (uint32_t*)c->counters.p;
-    RRH_HIP(hipMemsetAsync(cnt, 0, 256 * 4, c->stream));
+    uint32_t* cnt = Bf.cnt;
+    if (fixed) {
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, c->stream, cnt, 256u);
+        RRH_LAUNCH_CHECK();
+    } else {
+        RRH_HIP(hipMemsetAsync(cnt, 0, 256 * 4, c->stream));
+    }
     float4* rad = (float4*)radiance->p;
-    Q q[2];
-    for (int k = 0; k < 2; ++k) q[k] = Q{(float4*)c->q_o[k].p, (float4*)c->q_d[k].p, (float4*)c->q_thp[k].p};
+    Q* q = Bf.q;
     hipLaunchKernelGGL(k_raygen, dim3((W + 7) / 8, (H + 7) / 8), dim3(64), 0, c->stream, S, q[0], cnt, rad,
                        p->frame_seed);
     RRH_LAUNCH_CHECK();
@@ -569,7 +609,8 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
             if (e != hipSuccess) { set_error(std::string("ray sort: ") + hipGetErrorString(e)); return RRH_E_HIP; }
             perm = slots + npix;
         }
-        if (c->profiling) {
+        const bool prof = c->profiling && !fixed;
+        if (prof) {
             if (c->prof_used + 2 > c->prof_events.size()) {
                 for (int k = 0; k < 64; ++k) {
                     hipEvent_t e;
@@ -581,13 +622,13 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
         }
         const int sk = stack_for(c, s->stack_need());
         const bool cf = (p->flags & RRH_PT_COUNT_FETCHES) != 0;
-        rc = (sk == RRH_STACK_LDS32) ? launch_trace<RRH_STACK_LDS32>(c, S, qi, cnt, b, g, cf, perm)
-           : (sk == RRH_STACK_HYBRID16) ? launch_trace<RRH_STACK_HYBRID16>(c, S, qi, cnt, b, g, cf, perm)
-           : (sk == RRH_STACK_HYBRID32) ? launch_trace<RRH_STACK_HYBRID32>(c, S, qi, cnt, b, g, cf, perm)
-           : launch_trace<RRH_STACK_PRIVATE>(c, S, qi, cnt, b, g, cf, perm);
+        rc = (sk == RRH_STACK_LDS32) ? launch_trace<RRH_STACK_LDS32>(c, S, qi, cnt, b, g, cf, perm, Bf.hits)
+           : (sk == RRH_STACK_HYBRID16) ? launch_trace<RRH_STACK_HYBRID16>(c, S, qi, cnt, b, g, cf, perm, Bf.hits)
+           : (sk == RRH_STACK_HYBRID32) ? launch_trace<RRH_STACK_HYBRID32>(c, S, qi, cnt, b, g, cf, perm, Bf.hits)
+           : launch_trace<RRH_STACK_PRIVATE>(c, S, qi, cnt, b, g, cf, perm, Bf.hits);
         if (rc) return rc;
         RRH_LAUNCH_CHECK();
-        if (c->profiling) {
+        if (prof) {
             RRH_HIP(hipEventRecord(c->prof_events[c->prof_used + 1], c->stream));
             c->prof_used += 2;
         }
@@ -597,17 +638,17 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
         ShadeArgs Ab = A;
         if (b + 1 >= p->max_bounces || (p->guide_bounces != 0 && b >= p->guide_bounces)) Ab.guide_mode = RRH_GUIDE_NONE;
         if (Ab.guide_mode == RRH_GUIDE_MLP) {
-            rc = mlp_infer_queue(c, mlp, S, qi.o, qi.d, (const float4*)c->hits.p, cnt, b, npix,
+            rc = mlp_infer_queue(c, mlp, S, qi.o, qi.d, (const float4*)Bf.hits, cnt, b, npix,
                                  const_cast<float*>(A.guide.mlp_lobes));
             if (rc) return rc;
         }
-        hipLaunchKernelGGL(k_shade, dim3(g), dim3(kBlock), 0, c->stream, S, qi, qo, (const float4*)c->hits.p, cnt,
-                           (float4*)c->sh_o.p, (float4*)c->sh_d.p, (float4*)c->sh_c.p, rad, Ab);
+        hipLaunchKernelGGL(k_shade, dim3(g), dim3(kBlock), 0, c->stream, S, qi, qo, (const float4*)Bf.hits, cnt,
+                           Bf.sh_o, Bf.sh_d, Bf.sh_c, rad, Ab);
         RRH_LAUNCH_CHECK();
-        rc = (sk == RRH_STACK_LDS32) ? launch_shadow<RRH_STACK_LDS32>(c, S, cnt, b, g, rad, A.rec)
-           : (sk == RRH_STACK_HYBRID16) ? launch_shadow<RRH_STACK_HYBRID16>(c, S, cnt, b, g, rad, A.rec)
-           : (sk == RRH_STACK_HYBRID32) ? launch_shadow<RRH_STACK_HYBRID32>(c, S, cnt, b, g, rad, A.rec)
-           : launch_shadow<RRH_STACK_PRIVATE>(c, S, cnt, b, g, rad, A.rec);
+        rc = (sk == RRH_STACK_LDS32) ? launch_shadow<RRH_STACK_LDS32>(c, S, cnt, b, g, rad, A.rec, Bf)
+           : (sk == RRH_STACK_HYBRID16) ? launch_shadow<RRH_STACK_HYBRID16>(c, S, cnt, b, g, rad, A.rec, Bf)
+           : (sk == RRH_STACK_HYBRID32) ? launch_shadow<RRH_STACK_HYBRID32>(c, S, cnt, b, g, rad, A.rec, Bf)
+           : launch_shadow<RRH_STACK_PRIVATE>(c, S, cnt, b, g, rad, A.rec, Bf);
         if (rc) return rc;
     }
     hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(64), 0, c->stream, (const uint32_t*)cnt, p->max_bounces,
@@ -615,10 +656,67 @@ int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_gri
     RRH_LAUNCH_CHECK();
     c->last_max_bounces = p->max_bounces;
     c->last_npix = npix;
+    c->last_counters = cnt;
     return guide_finish(c, s, p, grid, mlp, A.rec, rad);
 }
 
+namespace rrh {
+
+// resolves this device's copy of the no-alloc scratch array once per device (the device must be
+// current); called by rrh_ctx_create
+int pt_scratch_init(int device, void** out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) {
+        set_error("rrh_ctx_create: device index above the no-alloc scratch table");
+        return RRH_E_RANGE;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[device]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return RRH_E_HIP;
+        }
+        g_scratch_addr[device] = ptr;
+    }
+    *out = g_scratch_addr[device];
+    return RRH_OK;
+}
+
+}  // namespace rrh
+
 extern "C" {
+
+int rrh_pt_render_noalloc(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_image* radiance) {
+    RRH_ARG(c && s && p && radiance, "rrh_pt_render_noalloc: null argument");
+    RRH_ARG(c->noalloc_scratch, "rrh_pt_render_noalloc: the context has no scratch address");
+    RRH_ARG(s->cam_set, "rrh_pt_render_noalloc: camera not set (rrh_camera_set)");
+    RRH_ARG(p->guide_mode == RRH_GUIDE_NONE && p->record_vertices == 0,
+            "rrh_pt_render_noalloc: unguided frames only (guide_mode NONE, record_vertices 0)");
+    RRH_ARG(p->flags == 0, "rrh_pt_render_noalloc: flags must be 0 (no fetch counting)");
+    RRH_ARG(c->ray_sort_bits == 0, "rrh_pt_render_noalloc: ray sorting (RRH_OPT_RAY_SORT) needs allocated keys");
+    const uint64_t npix = (uint64_t)s->cam_host.screen_width * s->cam_host.screen_height;
+    if (npix > kNoAllocMaxPixels) {
+        set_error("rrh_pt_render_noalloc: " + std::to_string(s->cam_host.screen_width) + "x" +
+                  std::to_string(s->cam_host.screen_height) + " is above the no-alloc limit of " +
+                  std::to_string(kNoAllocMaxPixels) + " pixels (512x512)");
+        return RRH_E_RANGE;
+    }
+    // the planes are carved from the array for the largest extent, so their offsets never change
+    unsigned char* base = (unsigned char*)c->noalloc_scratch;
+    float4* plane[10];
+    for (int k = 0; k < 10; ++k) plane[k] = (float4*)(base + (size_t)k * kNoAllocPlane);
+    Bufs B;
+    B.q[0] = Q{plane[0], plane[1], plane[2]};
+    B.q[1] = Q{plane[3], plane[4], plane[5]};
+    B.hits = plane[6];
+    B.sh_o = plane[7];
+    B.sh_d = plane[8];
+    B.sh_c = plane[9];
+    B.cnt = (uint32_t*)(base + 10u * kNoAllocPlane);
+    return rrh_pt_render_impl(c, s, p, nullptr, nullptr, radiance, &B);
+}
 
 int rrh_pt_render(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_image* radiance) {
     RRH_ARG(p, "rrh_pt_render: null params");
@@ -634,9 +732,9 @@ int rrh_pt_render_guided(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_g
 int rrh_pt_ray_counts(rrh_ctx* c, rrh_ray_counts* out) {
     RRH_ARG(c && out, "rrh_pt_ray_counts: null argument");
     out->primary = out->extension = out->shadow = 0;
-    if (!c->counters.p || c->last_max_bounces == 0) return RRH_OK;
+    if (!c->last_counters || c->last_max_bounces == 0) return RRH_OK;
     uint32_t h[128];
-    RRH_HIP(hipMemcpyAsync(h, c->counters.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    RRH_HIP(hipMemcpyAsync(h, c->last_counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     RRH_HIP(hipStreamSynchronize(c->stream));
     out->primary = h[0];
     for (uint32_t b = 1; b < c->last_max_bounces; ++b) out->extension += h[b];
