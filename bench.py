@@ -12,11 +12,16 @@ Prints one JSON line (see the contract in the repository notes).
 from __future__ import annotations
 
 import argparse
+import csv
 import ctypes as C
 import json
 import math
 import os
+import re
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 from pathlib import Path
 
@@ -74,29 +79,57 @@ def ray_totals(ctx, reset=True):
     return rc.primary, rc.extension, rc.shadow
 
 
-def cpu_baseline(sa, cam, max_bounces: int, seconds: float) -> dict:
-    """The oracle's path tracer on host cores over a bounded pixel sample of frame 0."""
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sa, cams, max_bounces: int, seconds: float, di_m: int) -> dict:
+    """The oracle on the host cores this process may run on, for the bench's frame: ReSTIR DI's
+    three stages over a whole frame (frame 1 of the orbit, its history from frame 0), and the path
+    tracer on a bounded random pixel sample of the same frame, extrapolated to all pixels.  Rays
+    are the oracle's own counts (primary, extension, shadow of both passes, as the GPU counts)."""
     from oracle import oracle as O
 
-    threads = min(os.cpu_count() or 1, 16)
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     osc = O.Scene.from_arrays(sa)
-    oc = O.camera_from_bytes(bytes(cam))
-    W, H = cam.screen_width, cam.screen_height
+    W, H = cams[1].screen_width, cams[1].screen_height
+    # ReSTIR DI, whole frame
+    oc0, oc1 = O.camera_from_bytes(bytes(cams[0])), O.camera_from_bytes(bytes(cams[1]))
+    d0, d1 = O.DIParams(0, di_m, 1, 1, 5, 30), O.DIParams(1, di_m, 1, 1, 5, 30)
+    _, _, rp, rpuv = osc.di_initial(oc0, d0, threads=threads)
+    t0 = time.perf_counter()
+    gp, gn, rs, uv = osc.di_initial(oc1, d1, threads=threads)
+    rs, uv = osc.di_temporal(oc1, d1, gp, gn, rs, uv, rp, rpuv, threads=threads)
+    osc.di_spatial_shade(oc1, d1, gp, gn, rs, uv, threads=threads)
+    t_di = time.perf_counter() - t0
+    di_rays = W * H + int(O.lib().orc_di_last_shadow_rays())
+    # path tracer, random pixels of the same frame until the time budget is spent
     rng = np.random.default_rng(0)
     n = 4096
-    t_used, rays, px_done = 0.0, 0, 0
-    while t_used < seconds:
+    t_pt, pt_rays, px_done = 0.0, 0, 0
+    while t_pt < max(seconds - t_di, 1.0):
         pix = rng.choice(W * H, size=n, replace=False).astype(np.uint32)
         t0 = time.perf_counter()
-        _, (p, e, s) = osc.pt_render(oc, max_bounces, 0, pix, threads=threads)
-        dt = time.perf_counter() - t0
-        t_used += dt
-        rays += p + e + s
+        _, (p, e, s) = osc.pt_render(oc1, max_bounces, 1, pix, threads=threads)
+        t_pt += time.perf_counter() - t0
+        pt_rays += p + e + s
         px_done += n
         n = int(min(W * H // 4, max(4096, n * 2)))
-    return {"value": round(rays / t_used / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"oracle path tracer (1 spp, max_bounces {max_bounces}) on {px_done} random pixels of "
-                      f"frame 0, {rays} rays in {t_used:.1f} s; ReSTIR DI not included"}
+    scale = W * H / px_done
+    frame_rays = pt_rays * scale + di_rays
+    frame_s = t_pt * scale + t_di
+    return {"value": round(frame_rays / frame_s / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "sample": f"oracle, frame 1 of the orbit at {W}x{H}: ReSTIR DI ({di_m} candidates, temporal, 5 "
+                      f"neighbours) over the whole frame, {di_rays} rays in {t_di:.2f} s; path tracer (1 spp, "
+                      f"max_bounces {max_bounces}) on {px_done} random pixels, {pt_rays} rays in {t_pt:.1f} s, "
+                      f"extrapolated to all pixels; value = rays of one whole frame / its CPU time"}
 
 
 def guiding_summary(ctx: core.Context, a) -> dict:
@@ -236,6 +269,106 @@ def restir_gi_summary(ctx: core.Context, scene, sa, W: int, H: int, frames: int,
             "note": "stage times by HIP markers on the context stream; ReSTIR DI (G-buffer, direct) not included"}
 
 
+# the options a traffic probe must share with the timed run (same kernels, same tree)
+PROBE_OPTS = ["stack", "xcd_remap", "persistent", "max_leaf", "trav_cost", "speculative", "bvh_builder",
+              "ploc_radius", "ray_sort", "bvh_nodes", "width", "height", "max_bounces"]
+COPY_BYTES = 1 << 30
+
+
+def configure(ctx, a) -> None:
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAVERSAL_STACK,
+           {"private": L.STACK_PRIVATE, "lds32": L.STACK_LDS32, "hybrid16": L.STACK_HYBRID16,
+            "hybrid32": L.STACK_HYBRID32}[a.stack])
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_XCD_REMAP, a.xcd_remap)
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_NODES, NODE_FORMATS[a.bvh_nodes])
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_PERSISTENT, a.persistent)
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_SPECULATIVE, a.speculative)
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_MAX_LEAF, a.max_leaf)
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_TRAV_COST, int(round(a.trav_cost * 100)))
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_BUILDER,
+           L.BVH_BUILD_DEVICE if a.bvh_builder == "device" else L.BVH_BUILD_HOST)
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_PLOC_RADIUS, a.ploc_radius)
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_RAY_SORT, a.ray_sort)
+
+
+def traffic_probe(a) -> int:
+    """Run under rocprofv3 --pmc by measure_traffic: the copy kernel of known traffic (1 GiB in,
+    1 GiB out per dispatch), then four frames of the bench's workload."""
+    ctx = core.Context(0)
+    configure(ctx, a)
+    g = C.c_float()
+    L.call("rrh_stream_copy", ctx.h, COPY_BYTES, 2, C.byref(g))
+    sa = core.generate_scene(L.GEN_HEIGHTFIELD, seed=2, scale=1.0)
+    scene = core.Scene(ctx, sa)
+    fr = Frame(ctx, scene, sa, a.width, a.height, a.max_bounces, 32)
+    for f in range(4):
+        fr.run(f)
+    ctx.synchronize()
+    print("[probe] done", flush=True)
+    return 0
+
+
+def _pmc_values(directory: str) -> dict:
+    """{kernel name: [counter value per dispatch]} from rocprofv3's counter_collection csv."""
+    vals: dict = {}
+    for f in Path(directory).rglob("*counter_collection.csv"):
+        with open(f, newline="") as fh:
+            for row in csv.DictReader(fh):
+                vals.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+    return vals
+
+
+def _mean_of(vals: dict, pattern: str):
+    xs = [v for k, vs in vals.items() if re.search(pattern, k) for v in vs]
+    return sum(xs) / len(xs) if xs else None
+
+
+def measure_traffic(a) -> dict:
+    """HBM bytes per dispatch of this run's kernels: two rocprofv3 --pmc passes over the traffic
+    probe (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same counter
+    reads for the copy kernel in the same pass."""
+    exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(exe):
+        return {"note": "rocprofv3 not found: traffic not measured"}
+    extra = []
+    for k in PROBE_OPTS:
+        extra += ["--" + k.replace("_", "-"), str(getattr(a, k))]
+    tmp = tempfile.mkdtemp(prefix="rrh_traffic_", dir="/tmp")
+    per = {}
+    try:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(tmp, counter)
+            cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "probe", "--",
+                   sys.executable, str(ROOT / "bench.py"), "--traffic-probe"] + extra
+            print(f"[bench] traffic pass {counter} ...", file=sys.stderr, flush=True)
+            try:
+                r = subprocess.run(cmd, cwd="/tmp", env={**os.environ, "TMPDIR": "/tmp"}, capture_output=True,
+                                   text=True, timeout=240)
+            except subprocess.TimeoutExpired:
+                return {"note": f"rocprofv3 {counter} pass timed out: traffic not measured"}
+            vals = _pmc_values(d)
+            if "[probe] done" not in r.stdout or not vals:
+                return {"note": f"rocprofv3 {counter} pass failed (rc {r.returncode}): {r.stderr.strip()[-300:]}"}
+            per[counter] = vals
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    kib = COPY_BYTES / 1024.0
+    cf, cw = _mean_of(per["FETCH_SIZE"], r"k_stream_copy"), _mean_of(per["WRITE_SIZE"], r"k_stream_copy")
+    if not cf or not cw:
+        return {"note": "copy kernel missing from the counter passes: traffic not measured"}
+    fs, ws = kib / cf, kib / cw
+    by_kernel = {}
+    for name in sorted(set(per["FETCH_SIZE"]) & set(per["WRITE_SIZE"])):
+        short = re.sub(r"\(anonymous namespace\)::|rrh::|^void\s+", "", name).split("(")[0]
+        f_, w_ = _mean_of(per["FETCH_SIZE"], re.escape(name)), _mean_of(per["WRITE_SIZE"], re.escape(name))
+        by_kernel[short] = round((f_ * fs + w_ * ws) * 1024.0 / 1e6, 1)
+    fi, wi = _mean_of(per["FETCH_SIZE"], r"k_intersect<false"), _mean_of(per["WRITE_SIZE"], r"k_intersect<false")
+    return {"bytes": int((fi * fs + wi * ws) * 1024.0) if fi is not None and wi is not None else None,
+            "fetch_scale": round(fs, 4), "write_scale": round(ws, 4), "mb_per_dispatch": by_kernel,
+            "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes of this run over 4 frames (KiB per "
+                      "dispatch), scaled by the same counters for a 1 GiB copy kernel"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -271,26 +404,19 @@ def main() -> int:
                     help="skip the streaming-pass report (accumulation, ReSTIR DI temporal, A-trous, tonemap)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB copy kernel (for hardware-counter calibration) and exit")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes that measure HBM bytes per dispatch")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU split is "
               "described in the design notes, not implemented", file=sys.stderr)
         return 2
 
+    if a.traffic_probe:
+        return traffic_probe(a)
     ctx = core.Context(0)
-    L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAVERSAL_STACK,
-           {"private": L.STACK_PRIVATE, "lds32": L.STACK_LDS32, "hybrid16": L.STACK_HYBRID16,
-            "hybrid32": L.STACK_HYBRID32}[a.stack])
-    L.call("rrh_ctx_set_option", ctx.h, L.OPT_XCD_REMAP, a.xcd_remap)
-    L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_NODES, NODE_FORMATS[a.bvh_nodes])
-    L.call("rrh_ctx_set_option", ctx.h, L.OPT_PERSISTENT, a.persistent)
-    L.call("rrh_ctx_set_option", ctx.h, L.OPT_SPECULATIVE, a.speculative)
-    L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_MAX_LEAF, a.max_leaf)
-    L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_TRAV_COST, int(round(a.trav_cost * 100)))
-    L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_BUILDER,
-           L.BVH_BUILD_DEVICE if a.bvh_builder == "device" else L.BVH_BUILD_HOST)
-    L.call("rrh_ctx_set_option", ctx.h, L.OPT_PLOC_RADIUS, a.ploc_radius)
-    L.call("rrh_ctx_set_option", ctx.h, L.OPT_RAY_SORT, a.ray_sort)
+    configure(ctx, a)
     if a.copy_calibration:
         g = C.c_float()
         L.call("rrh_stream_copy", ctx.h, 1 << 30, 5, C.byref(g))
@@ -341,14 +467,6 @@ def main() -> int:
     avg_launch_ms = trav_ms.value / max(launches.value, 1)
     bytes_per_launch = bytes_per_ray * pt_trace_rays / max(launches.value, 1)
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
-    # HBM bytes per k_intersect launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    # summary (calibrated against the copy kernel); null when the summary is absent
-    traffic = None
-    tfile = ROOT / "profiles" / "heightfield_1080p_hbm_traffic.md"
-    if tfile.exists():
-        for line in tfile.read_text().splitlines():
-            if line.startswith("hbm_bytes_per_launch k_intersect:"):
-                traffic = int(float(line.split(":")[1]))
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -373,11 +491,15 @@ def main() -> int:
                                      "shadow": shadow // a.steps}},
         "roofline": {"bound": "hbm", "kernel": "k_intersect (closest-hit BVH traversal)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_ray": round(bytes_per_ray, 1), "avg_launch_ms": round(avg_launch_ms, 4),
                      "launches": launches.value, "measured_copy_gbs": round(hbm.value, 1)},
         "cpu_baseline": None,
     }
+    if not a.no_traffic:
+        t = measure_traffic(a)
+        out["roofline"]["traffic"] = t.get("bytes")
+        out["roofline"]["traffic_detail"] = t
     if a.restir_pt_frames > 0:
         print("[bench] ReSTIR PT frames ...", file=sys.stderr, flush=True)
         out["restir_pt"] = restir_pt_summary(ctx, scene, sa, W, H, a.restir_pt_frames, 2)
@@ -393,15 +515,16 @@ def main() -> int:
         r = kernel_report.report(ctx, W, H, reps=10)
         out["streaming_passes"] = {
             "workload": f"heightfield_1164288_tris_{W}x{H}", "copy_gbs": r["copy_gbs"], "passes": r["passes"],
-            "note": "ms per call by HIP markers around 10 calls; algorithmic bytes per pixel as in "
-                    "research_amd/kernel_report.py; A-trous passes are 5-iteration frames"}
+            "note": "ms per call by HIP markers around each of 10 calls, each after 1 GiB written elsewhere "
+                    "(inputs start in HBM); warm_ms: 10 back-to-back calls (cache resident); algorithmic bytes "
+                    "per pixel as in research_amd/kernel_report.py; A-trous passes are 5-iteration frames"}
     if a.guide_seconds > 0:
         print("[bench] equal-time guiding comparison (indoor scene) ...", file=sys.stderr, flush=True)
         out["guiding"] = guiding_summary(ctx, a)
     if not a.no_cpu_baseline:
-        cam0 = orbit_camera(sa, 0, W, H)
         print("[bench] cpu baseline (oracle) ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(sa, cam0, a.max_bounces, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(sa, [orbit_camera(sa, f, W, H) for f in (0, 1)], a.max_bounces,
+                                           a.cpu_seconds, 32)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -110,6 +110,7 @@ _SIGS = {
     "orc_di_initial": (None, [P, C.POINTER(Camera), C.POINTER(DIParams), P, P, P, P, C.c_int]),
     "orc_di_temporal": (None, [P, C.POINTER(Camera), C.POINTER(DIParams), P, P, P, P, P, P, C.c_int]),
     "orc_di_spatial_shade": (None, [P, C.POINTER(Camera), C.POINTER(DIParams), P, P, P, P, P, C.c_int]),
+    "orc_di_last_shadow_rays": (C.c_uint64, []),
     "orc_rpt_initial": (None, [P, C.POINTER(Camera), C.POINTER(RPTParams), P, F, F, U, P, P, P, P, P, C.c_int]),
     "orc_rpt_temporal": (None, [P, C.POINTER(Camera), C.POINTER(RPTParams), P, P, P, P, P, P, C.c_int]),
     "orc_rpt_spatial_shade": (None, [P, C.POINTER(Camera), C.POINTER(RPTParams), P, F, F, U, P, P, P, P, P,

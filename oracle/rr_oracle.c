@@ -1184,15 +1184,19 @@ void orc_di_temporal(const orc_scene* s, const orc_camera* cam, const orc_di_par
     }
 }
 
+static uint64_t g_di_shadow_rays = 0;
+uint64_t orc_di_last_shadow_rays(void) { return g_di_shadow_rays; }
+
 void orc_di_spatial_shade(const orc_scene* s, const orc_camera* cam, const orc_di_params* p,
                           const float* gpos, const float* gnrm, const float* res,
                           const uint32_t* res_uv, float* out, int threads) {
     uint32_t W = cam->screen_width, H = cam->screen_height;
     uint32_t N = s->nl + s->ntl;
+    uint64_t shadow_rays = 0;
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
 #endif
-#pragma omp parallel for schedule(dynamic, 64)
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : shadow_rays)
     for (int64_t i = 0; i < (int64_t)W * H; ++i) {
         uint32_t px = (uint32_t)(i % W), py = (uint32_t)(i / W);
         float* o = out + i * 4;
@@ -1266,6 +1270,7 @@ void orc_di_spatial_shade(const orc_scene* s, const orc_camera* cam, const orc_d
             }
             float ndl = fmaxf(vdot(n, l), 0.0f);
             if (ndl > 0.0f) {
+                shadow_rays++;
                 if (!occluded_bvh(s, vadd(hp, vscale(n, 1e-4f)), l, 0.001f, dist - 0.01f)) {
                     v3 wo = vnorm(vsub(vld(cam->position), hp));
                     v3 f = mat_eval(&m, n, wo, l);
@@ -1275,6 +1280,7 @@ void orc_di_spatial_shade(const orc_scene* s, const orc_camera* cam, const orc_d
         }
         o[0] = outc.x; o[1] = outc.y; o[2] = outc.z; o[3] = 1.0f;
     }
+    g_di_shadow_rays = shadow_rays;
 }
 
 /* ========================================================================== */
@@ -1401,12 +1407,24 @@ void orc_atrous_prepare(uint32_t W, uint32_t H, const float cpos[3], const float
 }
 
 /* config mode: SVGF edge-stopping (Schied et al. 2017, eqs. 3-5) on fp16
- * buffers.  in.a carries the luminance variance; gbuf = (n.xyz, linear depth). */
+ * buffers.  in.a carries the luminance variance; gbuf = (n.xyz, linear depth).
+ * The weights' arithmetic is this build's own (the reference has no config mode), chosen for
+ * few operations per tap: max(0, n.n)^sigma_n by repeated squaring when sigma_n is 2^k
+ * (k <= 10; otherwise exp(sigma_n log x)); the luminance term |dl| x (1 / phi_l) with the
+ * reciprocal per pixel; the depth term |dz| x (1 / (|gx tx + gy ty| + 1e-3)) with
+ * gx = sigma_z dz/dx S, gy = sigma_z dz/dy S per pixel and the 25 reciprocals per pixel (taps
+ * o and -o share one value exactly). */
 static float powi_pos(float x, float e) { return (x > 0.0f) ? orc_expf(e * orc_logf(x)) : 0.0f; }
+static int pow2_exponent(float e) {
+    for (int k = 0; k <= 10; ++k)
+        if (e == (float)(1 << k)) return k;
+    return -1;
+}
 
 void orc_atrous_cfg(const orc_atrous_params* p, const uint16_t* in, const uint16_t* gbuf,
                     uint16_t* out) {
     int W = (int)p->width, H = (int)p->height, st = (int)p->stride;
+    const int nsq = pow2_exponent(p->sigma_normal);
     const float gk[3] = {0.25f, 0.5f, 0.25f};
 #pragma omp parallel for schedule(static)
     for (int y = 0; y < H; ++y) {
@@ -1429,11 +1447,17 @@ void orc_atrous_cfg(const orc_atrous_params* p, const uint16_t* in, const uint16
                 }
             }
             float phil = p->sigma_luminance * sqrtf(fmaxf(0.0f, var)) + 1e-10f;
+            float inv_phil = 1.0f / phil;
             /* screen-space depth gradient by central differences */
             int xm = x > 0 ? x - 1 : 0, xp = x < W - 1 ? x + 1 : W - 1;
             int ym = y > 0 ? y - 1 : 0, yp = y < H - 1 ? y + 1 : H - 1;
             float dzdx = (orc_f16_to_f32(gbuf[((size_t)y * W + xp) * 4 + 3]) - orc_f16_to_f32(gbuf[((size_t)y * W + xm) * 4 + 3])) * 0.5f;
             float dzdy = (orc_f16_to_f32(gbuf[((size_t)yp * W + x) * 4 + 3]) - orc_f16_to_f32(gbuf[((size_t)ym * W + x) * 4 + 3])) * 0.5f;
+            float gx = p->sigma_position * dzdx * (float)st, gy = p->sigma_position * dzdy * (float)st;
+            float inv_den[5][5];
+            for (int ty = -2; ty <= 2; ++ty)
+                for (int tx = -2; tx <= 2; ++tx)
+                    inv_den[ty + 2][tx + 2] = 1.0f / (fabsf(gx * (float)tx + gy * (float)ty) + 1e-3f);
             v3 acc = V(0, 0, 0);
             float accv = 0.0f, wsum = 0.0f;
             for (int ty = -2; ty <= 2; ++ty) {
@@ -1449,9 +1473,11 @@ void orc_atrous_cfg(const orc_atrous_params* p, const uint16_t* in, const uint16
                     v3 cq = V(orc_f16_to_f32(in[j * 4]), orc_f16_to_f32(in[j * 4 + 1]), orc_f16_to_f32(in[j * 4 + 2]));
                     float vq = orc_f16_to_f32(in[j * 4 + 3]);
                     float bw = (kernel_weight(tx) * kernel_weight(ty)) / 256.0f;
-                    float wz = fabsf(zp - zq) / (p->sigma_position * fabsf(dzdx * (float)(tx * st) + dzdy * (float)(ty * st)) + 1e-3f);
-                    float wn = powi_pos(fmaxf(0.0f, vdot(np, nq)), p->sigma_normal);
-                    float wl = fabsf(lp - lum709(cq)) / phil;
+                    float wz = fabsf(zp - zq) * inv_den[ty + 2][tx + 2];
+                    float wn = fmaxf(0.0f, vdot(np, nq));
+                    if (nsq >= 0) { for (int k = 0; k < nsq; ++k) wn = wn * wn; }
+                    else wn = powi_pos(wn, p->sigma_normal);
+                    float wl = fabsf(lp - lum709(cq)) * inv_phil;
                     float w = bw * orc_expf(0.0f - wl - wz) * wn;
                     acc = vadd(acc, vscale(cq, w));
                     accv += vq * (w * w);

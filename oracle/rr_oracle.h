@@ -149,6 +149,8 @@ void orc_di_temporal(const orc_scene* s, const orc_camera* cam, const orc_di_par
 void orc_di_spatial_shade(const orc_scene* s, const orc_camera* cam, const orc_di_params* p,
                           const float* gbuf_pos, const float* gbuf_norm, const float* res,
                           const uint32_t* res_uv, float* out, int threads);
+/* shadow rays traced by the last orc_di_spatial_shade call (one per shaded pixel whose sample faces it) */
+uint64_t orc_di_last_shadow_rays(void);
 
 /* ---- ReSTIR PT, reconnection shift: restir_pt.slang:98-1001 -------------------- */
 /* ReSTIR PT and GI rays are two-sided by default, as the reference's CULL_DISABLE instances

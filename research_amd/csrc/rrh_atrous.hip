@@ -14,7 +14,8 @@
 // stride 8 every workgroup stages one window of its stride's sub-lattice in LDS (1.56 image
 // reads per output) and the taps read LDS; at larger strides a dense tile reads its taps
 // through L2.  With the exact transcendentals the per-tap arithmetic, not the memory
-// traffic, bounds the kernels.
+// traffic, bounds the kernels; config mode therefore keeps one exp per tap (its weights'
+// arithmetic, which has no reference counterpart, is defined for that, as in the oracle).
 #include <hip/hip_fp16.h>
 
 #include "rrh_internal.h"
@@ -39,12 +40,14 @@ template <bool FAST> struct WMath;
 template <> struct WMath<false> {
     static __device__ __forceinline__ float ex(float x) { return exp_rr(x); }
     static __device__ __forceinline__ float div(float a, float b) { return a / b; }
+    static __device__ __forceinline__ float rcp(float b) { return 1.0f / b; }
     static __device__ __forceinline__ f3 unit(f3 a) { return normalize(a); }
     static __device__ __forceinline__ float powp(float x, float e) { return (x > 0.0f) ? exp_rr(e * log_rr(x)) : 0.0f; }
 };
 template <> struct WMath<true> {
     static __device__ __forceinline__ float ex(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504f); }
     static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    static __device__ __forceinline__ float rcp(float b) { return __builtin_amdgcn_rcpf(b); }
     static __device__ __forceinline__ f3 unit(f3 a) { return scale(a, __builtin_amdgcn_rsqf(dot(a, a))); }
     static __device__ __forceinline__ float powp(float x, float e) {
         return (x > 0.0f) ? __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(x)) : 0.0f;
@@ -170,8 +173,11 @@ __device__ __forceinline__ H4 h4(ushort4 v) { return H4{v.x, v.y, v.z, v.w}; }
 
 // The taps come from the staged lattice window; the centre's 3x3 variance and its depth
 // gradient (stride 1, off the lattice) are read from global memory.
+// Per tap: one exp.  The normal weight max(0, n.n)^sn is nsq squarings when sn = 2^nsq (nsq < 0:
+// exp(sn log x)); the luminance and depth terms multiply by reciprocals formed once per pixel
+// (the depth term's 13: tap -o shares tap o's denominator exactly).
 template <bool FAST, bool LAT>
-__global__ void __launch_bounds__(TX* TY) k_atrous_cfg(int W, int H, int S, float sz, float sn, float sl,
+__global__ void __launch_bounds__(TX* TY) k_atrous_cfg(int W, int H, int S, float sz, float sn, int nsq, float sl,
                                                       const ushort4* __restrict__ in, const ushort4* __restrict__ gb,
                                                       ushort4* __restrict__ out) {
     __shared__ ushort4 s_gb[LAT ? LW * LH : 1];
@@ -218,13 +224,21 @@ __global__ void __launch_bounds__(TX* TY) k_atrous_cfg(int W, int H, int S, floa
             var += gk[dx + 1] * gk[dy + 1] * h2f(in[(size_t)sy * W + sx].w);
         }
     float phil = sl * sqrtf(fmaxf(0.0f, var)) + 1e-10f;
+    const float inv_phil = WMath<FAST>::rcp(phil);
     int xm = x > 0 ? x - 1 : 0, xp = x < W - 1 ? x + 1 : W - 1;
     int ym = y > 0 ? y - 1 : 0, yp = y < H - 1 ? y + 1 : H - 1;
     float dzdx = (h2f(gb[(size_t)y * W + xp].w) - h2f(gb[(size_t)y * W + xm].w)) * 0.5f;
     float dzdy = (h2f(gb[(size_t)yp * W + x].w) - h2f(gb[(size_t)ym * W + x].w)) * 0.5f;
+    const float gx = sz * dzdx * (float)S, gy = sz * dzdy * (float)S;
+    float rden[13];  // taps k = 5 (ty + 2) + (tx + 2) <= 12; tap 24 - k mirrors tap k
+#pragma unroll
+    for (int k = 0; k < 13; ++k)
+        rden[k] = WMath<FAST>::rcp(fabsf(gx * (float)(k % 5 - 2) + gy * (float)(k / 5 - 2)) + 1e-3f);
     f3 acc = mk(0, 0, 0);
     float accv = 0.0f, wsum = 0.0f;
+#pragma unroll
     for (int ty2 = -2; ty2 <= 2; ++ty2) {
+#pragma unroll
         for (int tx2 = -2; tx2 <= 2; ++tx2) {
             ushort4 sg4, si4;
             tap(tx2, ty2, sg4, si4);
@@ -236,9 +250,15 @@ __global__ void __launch_bounds__(TX* TY) k_atrous_cfg(int W, int H, int S, floa
             f3 cq = mk(h2f(si.x), h2f(si.y), h2f(si.z));
             float vq = h2f(si.w);
             float bw = (kw(tx2) * kw(ty2)) / 256.0f;
-            float wz = WMath<FAST>::div(fabsf(zp - zq), sz * fabsf(dzdx * (float)(tx2 * S) + dzdy * (float)(ty2 * S)) + 1e-3f);
-            float wn = WMath<FAST>::powp(fmaxf(0.0f, dot(np, nq)), sn);
-            float wl = WMath<FAST>::div(fabsf(lp - lum709(cq)), phil);
+            const int k = 5 * (ty2 + 2) + (tx2 + 2);
+            float wz = fabsf(zp - zq) * rden[k <= 12 ? k : 24 - k];
+            float wn = fmaxf(0.0f, dot(np, nq));
+            if (nsq >= 0) {
+                for (int q = 0; q < nsq; ++q) wn = wn * wn;
+            } else {
+                wn = WMath<FAST>::powp(wn, sn);
+            }
+            float wl = fabsf(lp - lum709(cq)) * inv_phil;
             float w = bw * WMath<FAST>::ex(0.0f - wl - wz) * wn;
             acc = add(acc, scale(cq, w));
             accv += vq * (w * w);
@@ -326,6 +346,9 @@ int rrh_atrous(rrh_ctx* c, const rrh_atrous_params* p, const rrh_image* input, c
     } else {
         RRH_ARG(input->fmt == RRH_FMT_RGBA16F && ga->fmt == RRH_FMT_RGBA16F && t0->fmt == RRH_FMT_RGBA16F &&
                     t1->fmt == RRH_FMT_RGBA16F, "rrh_atrous: config mode uses RGBA16F");
+        int nsq = -1;  // sigma_normal = 2^nsq: the normal weight by squaring
+        for (int k = 0; k <= 10; ++k)
+            if (p->sigma_normal == (float)(1 << k)) { nsq = k; break; }
         for (uint32_t k = 0; k < it; ++k) {
             const ushort4* src = (const ushort4*)(k == 0 ? input->p : slots[(k - 1) & 1]->p);
             ushort4* dst = (ushort4*)slots[k & 1]->p;
@@ -334,7 +357,7 @@ int rrh_atrous(rrh_ctx* c, const rrh_atrous_params* p, const rrh_image* input, c
             auto kern = fast ? (lat ? k_atrous_cfg<true, true> : k_atrous_cfg<true, false>)
                              : (lat ? k_atrous_cfg<false, true> : k_atrous_cfg<false, false>);
             hipLaunchKernelGGL(kern, lat ? lat_grid(W, H, S) : dense, block, 0, c->stream, (int)W, (int)H, S, p->sigma_position,
-                               p->sigma_normal, p->sigma_luminance, src, (const ushort4*)ga->p, dst);
+                               p->sigma_normal, nsq, p->sigma_luminance, src, (const ushort4*)ga->p, dst);
             RRH_LAUNCH_CHECK();
         }
     }

