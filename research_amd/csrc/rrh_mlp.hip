@@ -41,7 +41,9 @@ static_assert(off_w(5) == NPARAM, "parameter count");
 // bf16 device copies: W_l [Opad][Kpad] (forward B operand), WT_l [64][64] (backward B operand)
 constexpr int kWOff[5] = {0, 64 * 96, 64 * 96 + 4096, 64 * 96 + 8192, 64 * 96 + 12288};
 constexpr int kWTOff = 64 * 96 + 12288 + 48 * 64;  // WT1..WT4, 4096 each
-constexpr int kPackElems = kWTOff + 4 * 4096;
+// inference copies of layers 1..4 with permuted columns (and, for layer 4, rows): see k_mlp_infer
+constexpr int kWIOff = kWTOff + 4 * 4096;           // WP1..WP3 [64][64], WP4 [48][64]
+constexpr int kPackElems = kWIOff + 3 * 4096 + 48 * 64;
 
 struct MlpDev {
     const __bf16* pk;  // packed bf16 weights
@@ -51,6 +53,19 @@ struct MlpDev {
 
 __device__ __forceinline__ const __bf16* Wl(const MlpDev& M, int l) { return M.pk + kWOff[l]; }
 __device__ __forceinline__ const __bf16* WTl(const MlpDev& M, int l) { return M.pk + kWTOff + (l - 1) * 4096; }
+__device__ __forceinline__ const __bf16* WPl(const MlpDev& M, int l) { return M.pk + kWIOff + (l - 1) * 4096; }
+
+// Inference runs each layer transposed, D = W X^T: a lane (r, g = lane / 16) of output tile nt
+// holds features 16 nt + 4 g + j (j < 4) of sample r.  Tiles 2 kk and 2 kk + 1 together hold
+// what the next layer's B operand needs at "virtual" columns 32 kk + 8 g + i, so the inference
+// copies of layers 1..4 store column v = feature perm_k(v), and layer 4's rows are ordered so
+// that lane g of sample r receives the raw outputs of lobes 2g and 2g + 1 (perm_out4).
+__host__ __device__ constexpr int perm_k(int v) {
+    return 32 * (v >> 5) + ((v & 7) < 4 ? 4 * ((v >> 3) & 3) + (v & 7) : 16 + 4 * ((v >> 3) & 3) + (v & 7) - 4);
+}
+__host__ __device__ constexpr int perm_out4(int o) {  // permuted row of layer 4 -> raw output, -1 = padding
+    return (4 * (o >> 4) + (o & 3)) < 10 ? 10 * ((o >> 2) & 3) + 4 * (o >> 4) + (o & 3) : -1;
+}
 
 // ---- inputs --------------------------------------------------------------------------
 struct Sample { f3 p, n, wo, dir; float w; };
@@ -403,19 +418,13 @@ __global__ void __launch_bounds__(256) k_mlp_train(MlpDev M, Src src, uint32_t n
     if (threadIdx.x == 0) partial_loss[blockIdx.x] = sm.red[0] + sm.red[1] + sm.red[2] + sm.red[3];
 }
 
-// Inference: every wave is independent and persistent.  It keeps all weight fragments of
-// the five layers in registers (42 x 8 BF16 per lane), so a 16-sample chunk costs no weight
-// traffic at all; layer-0 operands are encoded straight into registers, the hidden
-// activations go through the wave's own LDS tile (the D -> A operand transpose).
-struct __align__(16) InferWave {
-    __bf16 H[2][16 * 64];
-    float O[16 * NOP];
-};
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
+// Inference: every wave is independent and persistent and keeps all weight fragments of the
+// five layers in registers (42 x 8 BF16 per lane), so a 16-sample chunk costs no weight traffic.
+// The layers run transposed (D = W X^T, see perm_k): each layer's output fragments are the next
+// layer's input fragments after bias, ReLU and BF16 rounding, in registers, with four independent
+// accumulators per layer; layer-0 operands are encoded straight into registers and the head runs
+// on the four lanes (r, r + 16, r + 32, r + 48) that hold a sample's lobes.  The biases sit in LDS
+// (layer 4's in permuted order).  No activation passes through LDS and no wave waits on another.
 template <int KK, int NT>
 __device__ __forceinline__ void load_frags(const __bf16* W, int K, bf16x8 (&f)[NT][KK]) {
     const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
@@ -424,111 +433,119 @@ __device__ __forceinline__ void load_frags(const __bf16* W, int K, bf16x8 (&f)[N
 #pragma unroll
         for (int kk = 0; kk < KK; ++kk) f[nt][kk] = *reinterpret_cast<const bf16x8*>(W + (nt * 16 + r) * K + kk * 32 + g * 8);
 }
-template <int NT>
-__device__ __forceinline__ void load_bias(const float* b, int nout, float (&bo)[NT]) {
-    const int r = threadIdx.x & 15;
+__device__ __forceinline__ bf16x8 relu_bf16x8(const f32x4& lo, const f32x4& hi) {
+    bf16x8 b;
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) bo[nt] = (nt * 16 + r < nout) ? b[nt * 16 + r] : 0.0f;
+    for (int j = 0; j < 4; ++j) {
+        b[j] = (__bf16)fmaxf(lo[j], 0.0f);
+        b[4 + j] = (__bf16)fmaxf(hi[j], 0.0f);
+    }
+    return b;
 }
-// hidden layer from the wave's LDS tile: Hout = relu(Hin W^T + b)
-__device__ __forceinline__ void hidden_reg(const __bf16* Hin, const bf16x8 (&wf)[4][2], const float (&bo)[4], __bf16* Hout) {
-    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-    bf16x8 a0 = *reinterpret_cast<const bf16x8*>(Hin + r * 64 + g * 8);
-    bf16x8 a1 = *reinterpret_cast<const bf16x8*>(Hin + r * 64 + 32 + g * 8);
+// hidden layer: four 16-feature tiles of one 16-sample tile, bias-initialised accumulators
+__device__ __forceinline__ void hidden_t(const bf16x8 (&wf)[4][2], const float* bias, bf16x8 (&h)[2]) {
+    const int g = (threadIdx.x & 63) >> 4;
+    f32x4 d[4];
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
-        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, wf[nt][0], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, wf[nt][1], acc, 0, 0, 0);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) Hout[(4 * g + j) * 64 + nt * 16 + r] = (__bf16)fmaxf(acc[j] + bo[nt], 0.0f);
+        d[nt] = *reinterpret_cast<const f32x4*>(bias + nt * 16 + 4 * g);
+        d[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[nt][0], h[0], d[nt], 0, 0, 0);
+        d[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[nt][1], h[1], d[nt], 0, 0, 0);
     }
+    h[0] = relu_bf16x8(d[0], d[1]);
+    h[1] = relu_bf16x8(d[2], d[3]);
+}
+__device__ __forceinline__ float group_sum4t(float v) {
+    v += __shfl_xor(v, 16);
+    return v + __shfl_xor(v, 32);
+}
+__device__ __forceinline__ float group_max4t(float v) {
+    v = fmaxf(v, __shfl_xor(v, 16));
+    return fmaxf(v, __shfl_xor(v, 32));
 }
 
 template <class Src>
 __global__ void __launch_bounds__(256) k_mlp_infer(MlpDev M, Src src, uint32_t nsamples, float* __restrict__ lobes) {
-    __shared__ InferWave smw[4];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-    InferWave& S = smw[w];
+    __shared__ __align__(16) float s_bias[4 * 64 + 48];
+    for (int k = threadIdx.x; k < 4 * 64 + 48; k += 256) {
+        float v;
+        if (k < 256) {
+            v = M.P[off_b(k >> 6) + (k & 63)];
+        } else {
+            const int raw = perm_out4(k - 256);
+            v = raw >= 0 ? M.P[off_b(4) + raw] : 0.0f;
+        }
+        s_bias[k] = v;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
     const uint32_t n = min(src.count(), nsamples);
     const uint32_t nchunks = (n + 15) / 16;
     const uint32_t stride = gridDim.x * 4;
-    uint32_t ch = blockIdx.x * 4 + w;
+    uint32_t ch = blockIdx.x * 4 + (threadIdx.x >> 6);
     // the grid is sized for the queue's capacity: a wave without a chunk of the live count
-    // leaves before loading the weights (waves are independent: no block barrier follows)
+    // leaves before loading the weights (no block barrier follows)
     if (ch >= nchunks) return;
     bf16x8 f0[4][3], f1[4][2], f2[4][2], f3[4][2], f4[3][2];
     load_frags<3, 4>(Wl(M, 0), KI, f0);
-    load_frags<2, 4>(Wl(M, 1), 64, f1);
-    load_frags<2, 4>(Wl(M, 2), 64, f2);
-    load_frags<2, 4>(Wl(M, 3), 64, f3);
-    load_frags<2, 3>(Wl(M, 4), 64, f4);
-    float b0[4], b1[4], b2[4], b3[4], b4[3];
-    load_bias<4>(M.P + off_b(0), NH, b0);
-    load_bias<4>(M.P + off_b(1), NH, b1);
-    load_bias<4>(M.P + off_b(2), NH, b2);
-    load_bias<4>(M.P + off_b(3), NH, b3);
-    load_bias<3>(M.P + off_b(4), NO, b4);
+    load_frags<2, 4>(WPl(M, 1), 64, f1);
+    load_frags<2, 4>(WPl(M, 2), 64, f2);
+    load_frags<2, 4>(WPl(M, 3), 64, f3);
+    load_frags<2, 3>(WPl(M, 4), 64, f4);
     // the next chunk's sample is loaded while this one computes (software pipelining)
     Sample nxt;
-    bool nxt_valid = (ch < nchunks) && src.get(ch * 16 + r, nxt);  // the 4 lanes of row r hold sample r
+    bool nxt_valid = src.get(ch * 16 + r, nxt);  // the 4 lanes (r, g = 0..3) hold sample r
     for (; ch < nchunks; ch += stride) {
         const uint32_t base = ch * 16;
         Sample smp = nxt;
         const bool valid = nxt_valid;
         nxt_valid = (ch + stride < nchunks) && src.get((ch + stride) * 16 + r, nxt);
         if (!valid) { smp.p = smp.n = smp.wo = smp.dir = mk(0, 0, 0); smp.w = 0.0f; }
-        // layer 0: A operand (row r, features 32 kk + 8 g ..) encoded in registers
-        {
-            bf16x8 a[3];
+        bf16x8 h[2];
+        {   // layer 0: B operand = features 32 kk + 8 g .. of sample r, encoded in registers
+            bf16x8 x[3];
 #pragma unroll
-            for (int kk = 0; kk < 3; ++kk) a[kk] = encode8(M, smp, valid, kk * 32 + g * 8);
+            for (int kk = 0; kk < 3; ++kk) x[kk] = encode8(M, smp, valid, kk * 32 + g * 8);
+            f32x4 d[4];
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
-                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                d[nt] = *reinterpret_cast<const f32x4*>(s_bias + nt * 16 + 4 * g);
 #pragma unroll
-                for (int kk = 0; kk < 3; ++kk) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kk], f0[nt][kk], acc, 0, 0, 0);
-#pragma unroll
-                for (int j = 0; j < 4; ++j) S.H[0][(4 * g + j) * 64 + nt * 16 + r] = (__bf16)fmaxf(acc[j] + b0[nt], 0.0f);
+                for (int kk = 0; kk < 3; ++kk) d[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f0[nt][kk], x[kk], d[nt], 0, 0, 0);
             }
+            h[0] = relu_bf16x8(d[0], d[1]);
+            h[1] = relu_bf16x8(d[2], d[3]);
         }
-        wave_sync();
-        hidden_reg(S.H[0], f1, b1, S.H[1]);
-        wave_sync();
-        hidden_reg(S.H[1], f2, b2, S.H[0]);
-        wave_sync();
-        hidden_reg(S.H[0], f3, b3, S.H[1]);
-        wave_sync();
-        {
-            bf16x8 a0 = *reinterpret_cast<const bf16x8*>(S.H[1] + r * 64 + g * 8);
-            bf16x8 a1 = *reinterpret_cast<const bf16x8*>(S.H[1] + r * 64 + 32 + g * 8);
+        hidden_t(f1, s_bias + 64, h);
+        hidden_t(f2, s_bias + 128, h);
+        hidden_t(f3, s_bias + 192, h);
+        f32x4 e[3];
 #pragma unroll
-            for (int nt = 0; nt < 3; ++nt) {
-                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, f4[nt][0], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, f4[nt][1], acc, 0, 0, 0);
-#pragma unroll
-                for (int j = 0; j < 4; ++j) S.O[(4 * g + j) * NOP + nt * 16 + r] = acc[j] + b4[nt];
-            }
+        for (int nt = 0; nt < 3; ++nt) {
+            e[nt] = *reinterpret_cast<const f32x4*>(s_bias + 256 + nt * 16 + 4 * g);
+            e[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f4[nt][0], h[0], e[nt], 0, 0, 0);
+            e[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f4[nt][1], h[1], e[nt], 0, 0, 0);
         }
-        wave_sync();
-        // head: 4 lanes per sample (sample sl = lane / 4; its validity sits in lane sl)
-        const int sl = lane >> 2, qd = lane & 3;
-        const bool vs = __shfl((int)valid, sl) != 0;
-        Lobe lb[2];
-        head_lobes2(S.O + sl * NOP, qd, lb);
-        if (vs) {
-            float* dst = lobes + (size_t)(base + sl) * (NK * 5) + qd * 10;
+        // head: this lane holds the raw outputs of lobes 2g (o[0..4]) and 2g + 1 (o[5..9]) of sample r
+        float o[10];
+#pragma unroll
+        for (int q = 0; q < 10; ++q) o[q] = e[q >> 2][q & 3];
+        const float emax = group_max4t(fmaxf(o[4], o[9]));
+        const float x0 = hexp(o[4] - emax), x1 = hexp(o[9] - emax);
+        const float z = group_sum4t(x0 + x1);
+        if (valid) {
+            float* dst = lobes + (size_t)(base + r) * (NK * 5) + g * 10;
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                dst[j * 5 + 0] = lb[j].mu[0];
-                dst[j * 5 + 1] = lb[j].mu[1];
-                dst[j * 5 + 2] = lb[j].mu[2];
-                dst[j * 5 + 3] = lb[j].kap;
-                dst[j * 5 + 4] = lb[j].pi;
+                const float* v = o + 5 * j;
+                const float len = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + 1e-12f);
+                dst[j * 5 + 0] = v[0] / len;
+                dst[j * 5 + 1] = v[1] / len;
+                dst[j * 5 + 2] = v[2] / len;
+                dst[j * 5 + 3] = fminf(0.1f + softplus_f(v[3]), 1000.0f);
+                dst[j * 5 + 4] = (j == 0 ? x0 : x1) / z;
             }
         }
-        wave_sync();
     }
 }
 
@@ -608,7 +625,8 @@ __global__ void k_mlp_finish(uint32_t* __restrict__ state, uint32_t* __restrict_
     *bad = 0;
 }
 
-// bf16 copies: W_l [Opad][Kpad] and WT_l [64][64] (l >= 1), zero padded
+// bf16 copies: W_l [Opad][Kpad], WT_l [64][64] (l >= 1) and the permuted inference copies WP_l
+// (l >= 1), zero padded
 __global__ void k_mlp_pack(const float* __restrict__ P, __bf16* __restrict__ pk) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= kPackElems) return;
@@ -620,11 +638,19 @@ __global__ void k_mlp_pack(const float* __restrict__ P, __bf16* __restrict__ pk)
         int kp = (l == 0) ? KI : 64;
         int o = local / kp, i = local % kp;
         if (o < out_of(l) && i < in_of(l)) val = P[off_w(l) + o * in_of(l) + i];
-    } else {
+    } else if (e < kWIOff) {
         int local = e - kWTOff;
         int l = 1 + local / 4096;
         int i = (local % 4096) / 64, o = local % 64;
         if (o < out_of(l) && i < in_of(l)) val = P[off_w(l) + o * in_of(l) + i];
+    } else {
+        int local = e - kWIOff;
+        int l = 1 + local / 4096;
+        if (l > 4) l = 4;
+        local -= (l - 1) * 4096;
+        int row = local / 64, v = local % 64;
+        int o = (l == 4) ? perm_out4(row) : row;
+        if (o >= 0 && o < out_of(l)) val = P[off_w(l) + o * in_of(l) + perm_k(v)];
     }
     pk[e] = (__bf16)val;
 }

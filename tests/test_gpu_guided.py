@@ -138,10 +138,11 @@ def test_trained_guides_lower_the_noise_of_the_window_lit_room(ctx):
     """What the trained guides gain over the tracer without them: in the indoor room lit only
     through a small window, one-sample MIS with the online-EM vMF grid or the trained MLP gives
     clearly less variance per sample than BSDF sampling alone (and the same mean).  Both guides
-    train online for 48 frames at 256x192 (about 0.4 M path vertices per frame for the grid, one
-    65,536-vertex Adam step per frame for the MLP); with the 64x48 evaluation extent alone they
-    see too few paths and a poorly fitted guide adds variance.  Measured: unguided 1.098, grid
-    0.743, MLP 0.792 (training is deterministic: the same frames train the same guides)."""
+    train online for 160 frames at 256x192 (about 0.4 M path vertices per frame for the grid, one
+    65,536-vertex Adam step per frame at lr 3e-3 for the MLP); with the 64x48 evaluation extent
+    alone they see too few paths and a poorly fitted guide adds variance.  The MLP needs the
+    steps: over initialisation seeds 1-6 its noise was 0.85-1.09 after 48 frames, 0.60-0.85 after
+    96 and 0.48-0.57 after 160 (unguided 1.098)."""
     W2, H2, B = 64, 48, 8
     sa = core.generate_scene(L.GEN_INDOOR, 1, 0.25)
     gs = core.Scene(ctx, sa)
@@ -153,7 +154,7 @@ def test_trained_guides_lower_the_noise_of_the_window_lit_room(ctx):
     TW, TH = 256, 192
     gs.set_camera(core.camera_look_at(sa.eye, sa.center, sa.fov, TW, TH))
     timg = core.Image(ctx, TW, TH)
-    for k in range(48):
+    for k in range(160):
         a = 0.5 if k else 0.0
         p = L.PTParams(B, 100 + k, 3, L.GUIDE_VMM, a, 400000)
         L.call("rrh_pt_render_guided", ctx.h, gs.h, C.byref(p), grid.h, None, timg.h)
@@ -167,6 +168,6 @@ def test_trained_guides_lower_the_noise_of_the_window_lit_room(ctx):
     v_vmm, m_vmm = _noise(ctx, gs, W2, H2, n, 20000, B, mode=L.GUIDE_VMM, alpha=0.5, grid=grid)
     v_mlp, m_mlp = _noise(ctx, gs, W2, H2, n, 30000, B, mode=L.GUIDE_MLP, alpha=0.5, mlp=mlp)
     print(f"per-sample noise: unguided {v_none:.3f}, vMF grid {v_vmm:.3f}, MLP {v_mlp:.3f}")
-    assert v_vmm < 0.85 * v_none and v_mlp < 0.85 * v_none
+    assert v_vmm < 0.85 * v_none and v_mlp < 0.7 * v_none
     for m in (m_vmm, m_mlp):  # the same image (within noise): guiding changes the variance, not the mean
         assert abs(m.mean() / m_none.mean() - 1) < 0.1
