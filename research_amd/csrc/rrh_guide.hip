@@ -267,7 +267,8 @@ __global__ void k_vmm_mask(uint32_t ncell, uint32_t K, const float* __restrict__
 // The vertex's training sample, if it is kept (deterministic: a hash of the record index
 // thins the live vertices to about the requested count).
 __device__ __forceinline__ bool finalize_one(const float4* __restrict__ rec, uint32_t i, uint32_t max_bounces,
-                                             uint32_t stamp, const float4* __restrict__ rad, uint32_t keep_thresh,
+                                             uint32_t stamp, uint32_t shift, uint32_t phase,
+                                             const float4* __restrict__ rad, uint32_t keep_thresh,
                                              float4& t0, float4& t1, float4& t2, float4& t3) {
     bool take = false;
     t0 = make_float4(0, 0, 0, 0); t1 = t0; t2 = t0; t3 = t0;
@@ -275,7 +276,7 @@ __device__ __forceinline__ bool finalize_one(const float4* __restrict__ rec, uin
     float4 r0 = r[0];
     if (__float_as_uint(r0.w) == stamp && (pcg_hash(i ^ (stamp * 2654435761u)) >> 8) < keep_thresh) {
         float4 r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5], r6 = r[6], r7 = r[7];
-        uint32_t pixel = i / max_bounces;
+        uint32_t pixel = ((i / max_bounces) << shift) + phase;
         float4 Lf = rad[pixel];
         const f3 nee = (r3.w > 0.5f) ? xyz(r5) : mk(0, 0, 0);
         f3 Ls = sub(sub(xyz(Lf), xyz(r4)), nee);
@@ -309,22 +310,23 @@ __device__ __forceinline__ bool finalize_one(const float4* __restrict__ rec, uin
 // stores the total.
 constexpr uint32_t kFinChunk = 256;
 __global__ void __launch_bounds__(kFinChunk) k_finalize_count(const float4* __restrict__ rec, uint32_t nrec,
-                                                            uint32_t max_bounces, uint32_t stamp,
-                                                            const float4* __restrict__ rad, uint32_t keep_thresh,
-                                                            uint32_t* __restrict__ chunk_cnt) {
+                                                            uint32_t max_bounces, uint32_t stamp, uint32_t shift,
+                                                            uint32_t phase, const float4* __restrict__ rad,
+                                                            uint32_t keep_thresh, uint32_t* __restrict__ chunk_cnt) {
     const uint32_t nchunk = (nrec + kFinChunk - 1) / kFinChunk;
     for (uint32_t ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
         const uint32_t i = ch * kFinChunk + threadIdx.x;
         float4 t0, t1, t2, t3;
-        const bool take = i < nrec && finalize_one(rec, i, max_bounces, stamp, rad, keep_thresh, t0, t1, t2, t3);
+        const bool take = i < nrec && finalize_one(rec, i, max_bounces, stamp, shift, phase, rad, keep_thresh, t0, t1, t2, t3);
         const int n = __syncthreads_count(take ? 1 : 0);
         if (threadIdx.x == 0) chunk_cnt[ch] = (uint32_t)n;
     }
 }
 
 __global__ void __launch_bounds__(kFinChunk) k_finalize_write(const float4* __restrict__ rec, uint32_t nrec,
-                                                            uint32_t max_bounces, uint32_t stamp,
-                                                            const float4* __restrict__ rad, uint32_t keep_thresh,
+                                                            uint32_t max_bounces, uint32_t stamp, uint32_t shift,
+                                                            uint32_t phase, const float4* __restrict__ rad,
+                                                            uint32_t keep_thresh,
                                                             const uint32_t* __restrict__ chunk_cnt,
                                                             const uint32_t* __restrict__ chunk_off,
                                                             float4* __restrict__ train, uint32_t* __restrict__ cnt,
@@ -335,7 +337,7 @@ __global__ void __launch_bounds__(kFinChunk) k_finalize_write(const float4* __re
     for (uint32_t ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
         const uint32_t i = ch * kFinChunk + threadIdx.x;
         float4 t0, t1, t2, t3;
-        const bool take = i < nrec && finalize_one(rec, i, max_bounces, stamp, rad, keep_thresh, t0, t1, t2, t3);
+        const bool take = i < nrec && finalize_one(rec, i, max_bounces, stamp, shift, phase, rad, keep_thresh, t0, t1, t2, t3);
         const unsigned long long m = __ballot(take ? 1 : 0);
         if (lane == 0) s_wave[wave] = (uint32_t)__popcll(m);
         __syncthreads();
@@ -424,9 +426,9 @@ int guide_prepare(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* gr
     } else if (p->guide_mode == RRH_GUIDE_MLP) {
         RRH_ARG(mlp, "guided render: RRH_GUIDE_MLP needs an rrh_mlp");
         uint32_t npix = s->cam_host.screen_width * s->cam_host.screen_height;
-        int rc = c->scratch.ensure((size_t)npix * kMaxLobes * 5 * 4);
+        int rc = c->scratch.ensure((size_t)npix * kMlpLobeHalves * 2);
         if (rc) return rc;
-        G.mlp_lobes = (const float*)c->scratch.p;
+        G.mlp_lobes = (const uint16_t*)c->scratch.p;
         G.K = kMaxLobes;
     }
     if (p->record_vertices) {
@@ -439,6 +441,11 @@ int guide_prepare(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* gr
         R.v = (float4*)c->vtx.p;
         R.max_bounces = p->max_bounces;
         R.stamp = c->stamp;
+        // record a pixel lattice holding about twice the requested vertices (about half of the
+        // recorded vertices are live), so the hash thinning still picks the training set
+        const double want = 4.0 * (double)p->record_vertices;
+        while (R.shift < 16 && (double)(npix >> (R.shift + 1)) * p->max_bounces >= want) ++R.shift;
+        R.phase = c->stamp & ((1u << R.shift) - 1u);
         c->last_rec_bounces = p->max_bounces;
     }
     return RRH_OK;
@@ -449,7 +456,7 @@ int guide_finish(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* gri
     (void)grid; (void)mlp;
     if (!R.v) return RRH_OK;
     uint32_t npix = s->cam_host.screen_width * s->cam_host.screen_height;
-    uint32_t nrec = npix * p->max_bounces;
+    uint32_t nrec = ((npix + (1u << R.shift) - 1u) >> R.shift) * p->max_bounces;  // the recorded lattice
     uint32_t target = p->record_vertices;
     uint32_t cap = target;
     int rc = c->train.ensure((size_t)cap * 64);
@@ -472,14 +479,14 @@ int guide_finish(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* gri
     if ((rc = c->fin_tmp.ensure(tb > 0 ? tb : 1))) return rc;
     const uint32_t g = std::min(nchunk, 8192u);
     hipLaunchKernelGGL(k_finalize_count, dim3(g), dim3(kFinChunk), 0, c->stream, (const float4*)c->vtx.p, nrec,
-                       p->max_bounces, R.stamp, rad, thresh, ccnt);
+                       p->max_bounces, R.stamp, R.shift, R.phase, rad, thresh, ccnt);
     RRH_LAUNCH_CHECK();
     tb = c->fin_tmp.bytes;
     const hipError_t e = rocprim::exclusive_scan(c->fin_tmp.p, tb, ccnt, coff, 0u, nchunk, rocprim::plus<uint32_t>(),
                                                  c->stream);
     if (e != hipSuccess) { set_error(std::string("vertex compaction scan: ") + hipGetErrorString(e)); return RRH_E_HIP; }
     hipLaunchKernelGGL(k_finalize_write, dim3(g), dim3(kFinChunk), 0, c->stream, (const float4*)c->vtx.p, nrec,
-                       p->max_bounces, R.stamp, rad, thresh, (const uint32_t*)ccnt, (const uint32_t*)coff,
+                       p->max_bounces, R.stamp, R.shift, R.phase, rad, thresh, (const uint32_t*)ccnt, (const uint32_t*)coff,
                        (float4*)c->train.p, (uint32_t*)c->train_cnt.p, cap);
     RRH_LAUNCH_CHECK();
     return RRH_OK;

@@ -2,6 +2,8 @@
 // one-sample MIS between the BSDF and a vMF mixture (8-lobe EM grid or the learned
 // guide), and the path-vertex records that train both guides.
 #pragma once
+#include <hip/hip_fp16.h>
+
 #include "rrh_device.h"
 
 struct rrh_ctx;
@@ -75,13 +77,15 @@ RRH_D f3 hash_primary_dir(const HashGrid& G, f3 p, f3 N, uint32_t bsdf, f3 cam, 
 }
 
 struct GuideDev {
-    const float* lobes = nullptr;      // VMM: res^3 cells x K x (mu3, kappa, pi)
-    const float* mlp_lobes = nullptr;  // MLP: per ray-queue slot K x (mu3, kappa, pi)
+    const float* lobes = nullptr;         // VMM: res^3 cells x K x (mu3, kappa, pi)
+    const uint16_t* mlp_lobes = nullptr;  // MLP: per ray-queue slot kMlpLobeHalves fp16 (LobesH16)
     uint32_t res = 0, K = 0;
     float bmin[3] = {0, 0, 0}, bext[3] = {1, 1, 1};
 };
 
-// path-vertex record (8 float4 = 128 B) at index pixel * max_bounces + bounce:
+// path-vertex record (8 float4 = 128 B) at index (pixel >> shift) * max_bounces + bounce, for the
+// pixels with pixel mod 2^shift = phase (whole paths of a pixel lattice; the phase moves every
+// recorded frame):
 //  r0 (pos, stamp) r1 (n, pdf) r2 (wo, mat) r3 (wi, nee visible)
 //  r4 (L before this vertex, alive) r5 (NEE contribution if visible) r6 (throughput after)
 //  r7 (BSDF weight f cos / pdf before Russian roulette)
@@ -89,6 +93,7 @@ struct VertexRec {
     float4* v = nullptr;
     uint32_t max_bounces = 0;
     uint32_t stamp = 0;
+    uint32_t shift = 0, phase = 0;
 };
 constexpr uint32_t kRecF4 = 8;
 
@@ -104,53 +109,89 @@ RRH_D uint32_t vmm_cell(const GuideDev& G, f3 p) {
     return (c[2] * G.res + c[1]) * G.res + c[0];
 }
 
-RRH_D float mixture_pdf(const float* lc, uint32_t K, f3 wi) {
+// lobe accessors of the one-sample MIS below: fp32 (mu3, kappa, pi) records (the EM grid's cell),
+// or the learned guide's fp16 record of one queue slot: 8 x (mu.xyz, kappa) then the 8 weights.
+// A fp16 mu is renormalised wherever it is read, so sampling and the pdf use the same unit
+// directions and the estimate stays unbiased with the rounded lobes.
+struct LobesF32 {
+    const float* p;
+    RRH_D float pi(uint32_t k) const { return p[k * 5 + 4]; }
+    RRH_D f3 mu(uint32_t k) const { return mk(p[k * 5], p[k * 5 + 1], p[k * 5 + 2]); }
+    RRH_D float kappa(uint32_t k) const { return p[k * 5 + 3]; }
+};
+constexpr uint32_t kMlpLobeHalves = 40;
+RRH_D float half_bits_to_float(uint32_t h) { return __half2float(__ushort_as_half((unsigned short)h)); }
+struct LobesH16 {
+    const uint32_t* p;  // 20 words: (mu.x, mu.y), (mu.z, kappa) per lobe, then the weights in pairs
+    RRH_D float pi(uint32_t k) const {
+        const uint32_t w = p[16 + (k >> 1)];
+        return half_bits_to_float((k & 1u) ? (w >> 16) : (w & 0xffffu));
+    }
+    RRH_D f3 mu(uint32_t k) const {
+        const uint32_t a = p[2 * k], b = p[2 * k + 1];
+        return normalize(mk(half_bits_to_float(a & 0xffffu), half_bits_to_float(a >> 16), half_bits_to_float(b & 0xffffu)));
+    }
+    RRH_D float kappa(uint32_t k) const { return half_bits_to_float(p[2 * k + 1] >> 16); }
+};
+
+template <class Lobes>
+RRH_D float mixture_pdf(const Lobes& lc, uint32_t K, f3 wi) {
     float p = 0.0f;
     for (uint32_t k = 0; k < K; ++k) {
-        float pk = lc[k * 5 + 4];
-        if (pk > 0.0f) p += pk * vmf_pdf(mk(lc[k * 5], lc[k * 5 + 1], lc[k * 5 + 2]), lc[k * 5 + 3], wi);
+        float pk = lc.pi(k);
+        if (pk > 0.0f) p += pk * vmf_pdf(lc.mu(k), lc.kappa(k), wi);
     }
     return p;
 }
 
-// one-sample MIS: with probability alpha sample the mixture, else the BSDF;
-// pdf = alpha * p_guide + (1 - alpha) * p_bsdf; returns f(wo, wi)
-// slot: the ray's queue index (the learned guide's lobes are inferred per queue slot)
-RRH_D f3 guided_sample(const GuideDev& G, uint32_t mode, float alpha, const DevScene& S, uint32_t slot, f3 pos, f3 n,
-                       f3 wo, const Mat& m, uint32_t& seed, f3& wi, float& pdf) {
-    const float* lc = nullptr;
-    if (mode == RRH_GUIDE_VMM && G.lobes) lc = G.lobes + (size_t)vmm_cell(G, pos) * G.K * 5;
-    else if (mode == RRH_GUIDE_MLP && G.mlp_lobes) lc = G.mlp_lobes + (size_t)slot * G.K * 5;
+// one-sample MIS with the mixture lc (K lobes; K = 0: no guide): with probability alpha sample
+// the mixture, else the BSDF; pdf = alpha * p_guide + (1 - alpha) * p_bsdf; returns f(wo, wi)
+template <class Lobes>
+RRH_D f3 mixture_guided_sample(const Lobes& lc, uint32_t K, float alpha, f3 n, f3 wo, const Mat& m, uint32_t& seed,
+                               f3& wi, float& pdf) {
     float tot = 0.0f;
-    if (lc)
-        for (uint32_t k = 0; k < G.K; ++k) tot += fmaxf(lc[k * 5 + 4], 0.0f);
+    for (uint32_t k = 0; k < K; ++k) tot += fmaxf(lc.pi(k), 0.0f);
     float a = (tot > 0.0f) ? alpha : 0.0f;
     float usel = rnd(seed);
     if (usel < a) {
         float uk = rnd(seed) * tot;
-        uint32_t pick = G.K - 1;
+        uint32_t pick = K - 1;
         float acc = 0.0f;
-        for (uint32_t k = 0; k < G.K; ++k) {
-            acc += fmaxf(lc[k * 5 + 4], 0.0f);
+        for (uint32_t k = 0; k < K; ++k) {
+            acc += fmaxf(lc.pi(k), 0.0f);
             if (uk < acc) { pick = k; break; }
         }
         float u0 = rnd(seed), u1 = rnd(seed), vp;
-        wi = vmf_sample(mk(lc[pick * 5], lc[pick * 5 + 1], lc[pick * 5 + 2]), lc[pick * 5 + 3], u0, u1, vp);
+        wi = vmf_sample(lc.mu(pick), lc.kappa(pick), u0, u1, vp);
     } else {
         float ux = rnd(seed), uy = rnd(seed), bp;
         f3 f = mat_sample(m, wo, n, ux, uy, wi, bp);
         if (!m.lambert && bp == 1.0f && f.x == 0.0f && f.y == 0.0f && f.z == 0.0f) { pdf = 0.0f; return f; }
     }
     float pb = mat_sample_pdf(m, wo, wi, n);
-    float pg = (a > 0.0f) ? mixture_pdf(lc, G.K, wi) / tot : 0.0f;
+    float pg = (a > 0.0f) ? mixture_pdf(lc, K, wi) / tot : 0.0f;
     pdf = a * pg + (1.0f - a) * pb;
     if (dot(n, wi) <= 0.0f) { pdf = 0.0f; return mk(0, 0, 0); }
     return mat_eval(m, n, wo, wi);
 }
 
+// slot: the ray's queue index (the learned guide's lobes are inferred per queue slot)
+RRH_D f3 guided_sample(const GuideDev& G, uint32_t mode, float alpha, const DevScene& S, uint32_t slot, f3 pos, f3 n,
+                       f3 wo, const Mat& m, uint32_t& seed, f3& wi, float& pdf) {
+    (void)S;
+    if (mode == RRH_GUIDE_MLP && G.mlp_lobes)
+        return mixture_guided_sample(LobesH16{reinterpret_cast<const uint32_t*>(G.mlp_lobes + (size_t)slot * kMlpLobeHalves)},
+                                     kMaxLobes, alpha, n, wo, m, seed, wi, pdf);
+    const bool vmm = mode == RRH_GUIDE_VMM && G.lobes;
+    return mixture_guided_sample(LobesF32{vmm ? G.lobes + (size_t)vmm_cell(G, pos) * G.K * 5 : nullptr}, vmm ? G.K : 0u,
+                                 alpha, n, wo, m, seed, wi, pdf);
+}
+
+RRH_D bool guide_records(const VertexRec& R, uint32_t pixel) { return (pixel & ((1u << R.shift) - 1u)) == R.phase; }
 RRH_D void guide_record_vertex(const VertexRec& R, uint32_t pixel, uint32_t b, f3 pos, f3 n, f3 wo, f3 wi, float pdf,
                                f3 bsdf_w, f3 thp_after, f3 L_pre, f3 nee_c, uint32_t mat, bool alive) {
-    float4* r = R.v + ((size_t)pixel * R.max_bounces + b) * kRecF4;
+    if (!guide_records(R, pixel)) return;
+    float4* r = R.v + ((size_t)(pixel >> R.shift) * R.max_bounces + b) * kRecF4;
     r[0] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(R.stamp));
     r[1] = make_float4(n.x, n.y, n.z, pdf);
     r[2] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(mat));
@@ -161,7 +202,8 @@ RRH_D void guide_record_vertex(const VertexRec& R, uint32_t pixel, uint32_t b, f
     r[7] = make_float4(bsdf_w.x, bsdf_w.y, bsdf_w.z, 0.0f);
 }
 RRH_D void guide_record_nee_visible(const VertexRec& R, uint32_t pixel, uint32_t b) {
-    float4* r = R.v + ((size_t)pixel * R.max_bounces + b) * kRecF4;
+    if (!guide_records(R, pixel)) return;
+    float4* r = R.v + ((size_t)(pixel >> R.shift) * R.max_bounces + b) * kRecF4;
     r[3].w = 1.0f;
 }
 
@@ -170,8 +212,8 @@ int guide_prepare(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* gr
                   VertexRec& R);
 int guide_finish(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* grid, rrh_mlp* mlp, const VertexRec& R,
                  const float4* rad);
-// learned guide: lobes (8 x 5 floats) for every hit of queue b (rrh_mlp.hip)
+// learned guide: the fp16 lobe record (kMlpLobeHalves) of every hit of queue b (rrh_mlp.hip)
 int mlp_infer_queue(rrh_ctx* c, rrh_mlp* m, const DevScene& S, const float4* qo, const float4* qd, const float4* hits,
-                    const uint32_t* cnt, uint32_t b, uint32_t cap, float* lobes);
+                    const uint32_t* cnt, uint32_t b, uint32_t cap, uint16_t* lobes);
 
 }  // namespace rrh

@@ -41,9 +41,15 @@ static_assert(off_w(5) == NPARAM, "parameter count");
 // bf16 device copies: W_l [Opad][Kpad] (forward B operand), WT_l [64][64] (backward B operand)
 constexpr int kWOff[5] = {0, 64 * 96, 64 * 96 + 4096, 64 * 96 + 8192, 64 * 96 + 12288};
 constexpr int kWTOff = 64 * 96 + 12288 + 48 * 64;  // WT1..WT4, 4096 each
-// inference copies of layers 1..4 with permuted columns (and, for layer 4, rows): see k_mlp_infer
-constexpr int kWIOff = kWTOff + 4 * 4096;           // WP1..WP3 [64][64], WP4 [48][64]
-constexpr int kPackElems = kWIOff + 3 * 4096 + 48 * 64;
+// inference copy: the 42 MFMA A fragments of the five layers (layer 0: 4 x 3, layers 1-3: 4 x 2,
+// layer 4: 3 x 2), each 64 lanes x 8 BF16 in lane order (1 KB, one ds_read_b128 per lane), layers
+// 1..4 with permuted columns and layer 4 with permuted rows: see k_mlp_infer
+constexpr int kFrags = 42;
+constexpr int kFWOff = kWTOff + 4 * 4096;
+constexpr int kPackElems = kFWOff + kFrags * 512;
+__host__ __device__ constexpr int frag_index(int l, int nt, int kk) {
+    return l == 0 ? nt * 3 + kk : (l < 4 ? 12 + (l - 1) * 8 + nt * 2 + kk : 36 + nt * 2 + kk);
+}
 
 struct MlpDev {
     const __bf16* pk;  // packed bf16 weights
@@ -53,12 +59,11 @@ struct MlpDev {
 
 __device__ __forceinline__ const __bf16* Wl(const MlpDev& M, int l) { return M.pk + kWOff[l]; }
 __device__ __forceinline__ const __bf16* WTl(const MlpDev& M, int l) { return M.pk + kWTOff + (l - 1) * 4096; }
-__device__ __forceinline__ const __bf16* WPl(const MlpDev& M, int l) { return M.pk + kWIOff + (l - 1) * 4096; }
 
 // Inference runs each layer transposed, D = W X^T: a lane (r, g = lane / 16) of output tile nt
 // holds features 16 nt + 4 g + j (j < 4) of sample r.  Tiles 2 kk and 2 kk + 1 together hold
 // what the next layer's B operand needs at "virtual" columns 32 kk + 8 g + i, so the inference
-// copies of layers 1..4 store column v = feature perm_k(v), and layer 4's rows are ordered so
+// fragments of layers 1..4 hold column v = feature perm_k(v), and layer 4's rows are ordered so
 // that lane g of sample r receives the raw outputs of lobes 2g and 2g + 1 (perm_out4).
 __host__ __device__ constexpr int perm_k(int v) {
     return 32 * (v >> 5) + ((v & 7) < 4 ? 4 * ((v >> 3) & 3) + (v & 7) : 16 + 4 * ((v >> 3) & 3) + (v & 7) - 4);
@@ -418,21 +423,13 @@ __global__ void __launch_bounds__(256) k_mlp_train(MlpDev M, Src src, uint32_t n
     if (threadIdx.x == 0) partial_loss[blockIdx.x] = sm.red[0] + sm.red[1] + sm.red[2] + sm.red[3];
 }
 
-// Inference: every wave is independent and persistent and keeps all weight fragments of the
-// five layers in registers (42 x 8 BF16 per lane), so a 16-sample chunk costs no weight traffic.
-// The layers run transposed (D = W X^T, see perm_k): each layer's output fragments are the next
-// layer's input fragments after bias, ReLU and BF16 rounding, in registers, with four independent
-// accumulators per layer; layer-0 operands are encoded straight into registers and the head runs
-// on the four lanes (r, r + 16, r + 32, r + 48) that hold a sample's lobes.  The biases sit in LDS
-// (layer 4's in permuted order).  No activation passes through LDS and no wave waits on another.
-template <int KK, int NT>
-__device__ __forceinline__ void load_frags(const __bf16* W, int K, bf16x8 (&f)[NT][KK]) {
-    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt)
-#pragma unroll
-        for (int kk = 0; kk < KK; ++kk) f[nt][kk] = *reinterpret_cast<const bf16x8*>(W + (nt * 16 + r) * K + kk * 32 + g * 8);
-}
+// Inference: persistent waves.  The block stages the 42 weight fragments (42 KB, fragment-major:
+// one conflict-free ds_read_b128 per lane and fragment) and the biases (layer 4's in permuted
+// order) in LDS once; each wave then takes 32 samples per step as two 16-sample tiles, and every
+// fragment read feeds both tiles' MFMAs.  The layers run transposed (D = W X^T, see perm_k):
+// each layer's output fragments become the next layer's input fragments after bias, ReLU and
+// BF16 rounding, in registers; layer-0 operands are encoded straight into registers and the head
+// runs on the four lanes (r, r + 16, r + 32, r + 48) that hold a sample's lobes.
 __device__ __forceinline__ bf16x8 relu_bf16x8(const f32x4& lo, const f32x4& hi) {
     bf16x8 b;
 #pragma unroll
@@ -442,19 +439,6 @@ __device__ __forceinline__ bf16x8 relu_bf16x8(const f32x4& lo, const f32x4& hi) 
     }
     return b;
 }
-// hidden layer: four 16-feature tiles of one 16-sample tile, bias-initialised accumulators
-__device__ __forceinline__ void hidden_t(const bf16x8 (&wf)[4][2], const float* bias, bf16x8 (&h)[2]) {
-    const int g = (threadIdx.x & 63) >> 4;
-    f32x4 d[4];
-#pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-        d[nt] = *reinterpret_cast<const f32x4*>(bias + nt * 16 + 4 * g);
-        d[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[nt][0], h[0], d[nt], 0, 0, 0);
-        d[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[nt][1], h[1], d[nt], 0, 0, 0);
-    }
-    h[0] = relu_bf16x8(d[0], d[1]);
-    h[1] = relu_bf16x8(d[2], d[3]);
-}
 __device__ __forceinline__ float group_sum4t(float v) {
     v += __shfl_xor(v, 16);
     return v + __shfl_xor(v, 32);
@@ -463,11 +447,83 @@ __device__ __forceinline__ float group_max4t(float v) {
     v = fmaxf(v, __shfl_xor(v, 16));
     return fmaxf(v, __shfl_xor(v, 32));
 }
+__device__ __forceinline__ bf16x8 frag(const bf16x8* s_w, int f) { return s_w[f * 64 + (threadIdx.x & 63)]; }
 
-template <class Src>
-__global__ void __launch_bounds__(256) k_mlp_infer(MlpDev M, Src src, uint32_t nsamples, float* __restrict__ lobes) {
+// hidden layer l (1..3) of both tiles: h[t] in, h[t] out
+__device__ __forceinline__ void hidden2(const bf16x8* s_w, const float* bias, int l, bf16x8 (&h)[2][2]) {
+    const int g = (threadIdx.x & 63) >> 4;
+    f32x4 d[2][4];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        const f32x4 b = *reinterpret_cast<const f32x4*>(bias + nt * 16 + 4 * g);
+        d[0][nt] = b;
+        d[1][nt] = b;
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+            const bf16x8 a = frag(s_w, frag_index(l, nt, kk));
+            d[0][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, h[0][kk], d[0][nt], 0, 0, 0);
+            d[1][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, h[1][kk], d[1][nt], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        h[t][0] = relu_bf16x8(d[t][0], d[t][1]);
+        h[t][1] = relu_bf16x8(d[t][2], d[t][3]);
+    }
+}
+// head of one tile: this lane holds the raw outputs of lobes 2g (o[0..4]) and 2g + 1 (o[5..9])
+// of sample `si`; when the sample is valid, writes them as (mu3, kappa, pi) floats (H16 false:
+// 40 floats per sample) or as the fp16 record LobesH16 reads (8 x (mu3, kappa), then
+// the 8 weights)
+template <bool H16>
+__device__ __forceinline__ void head_store_t(const f32x4 (&e)[3], bool valid, void* out, size_t si) {
+    const int g = (threadIdx.x & 63) >> 4;
+    float o[10];
+#pragma unroll
+    for (int q = 0; q < 10; ++q) o[q] = e[q >> 2][q & 3];
+    const float emax = group_max4t(fmaxf(o[4], o[9]));
+    const float x0 = hexp(o[4] - emax), x1 = hexp(o[9] - emax);
+    const float z = group_sum4t(x0 + x1);
+    if (!valid) return;
+    float lb[10];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const float* v = o + 5 * j;
+        const float len = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + 1e-12f);
+        lb[j * 5 + 0] = v[0] / len;
+        lb[j * 5 + 1] = v[1] / len;
+        lb[j * 5 + 2] = v[2] / len;
+        lb[j * 5 + 3] = fminf(0.1f + softplus_f(v[3]), 1000.0f);
+        lb[j * 5 + 4] = (j == 0 ? x0 : x1) / z;
+    }
+    if constexpr (H16) {
+        uint16_t* rec = reinterpret_cast<uint16_t*>(out) + si * kMlpLobeHalves;
+        auto hb = [](float f) { return (uint32_t)__half_as_ushort(__float2half_rn(f)); };
+        uint4 q;
+        q.x = hb(lb[0]) | hb(lb[1]) << 16;
+        q.y = hb(lb[2]) | hb(lb[3]) << 16;
+        q.z = hb(lb[5]) | hb(lb[6]) << 16;
+        q.w = hb(lb[7]) | hb(lb[8]) << 16;
+        *reinterpret_cast<uint4*>(rec + 8 * g) = q;
+        *reinterpret_cast<uint32_t*>(rec + 32 + 2 * g) = hb(lb[4]) | hb(lb[9]) << 16;
+    } else {
+        float* dst = reinterpret_cast<float*>(out) + si * (NK * 5) + g * 10;
+#pragma unroll
+        for (int q = 0; q < 10; ++q) dst[q] = lb[q];
+    }
+}
+
+// 8 waves per block share one copy of the fragments (44 KB of LDS): three blocks per CU, and the
+// registers (about 110 per lane) allow four waves per SIMD
+constexpr int kInferWaves = 8;
+template <class Src, bool H16>
+__global__ void __launch_bounds__(64 * kInferWaves, 4) k_mlp_infer(MlpDev M, Src src, uint32_t nsamples,
+                                                               void* __restrict__ lobes) {
+    __shared__ bf16x8 s_w[kFrags * 64];
     __shared__ __align__(16) float s_bias[4 * 64 + 48];
-    for (int k = threadIdx.x; k < 4 * 64 + 48; k += 256) {
+    const uint4* gw = reinterpret_cast<const uint4*>(M.pk + kFWOff);
+    for (int k = threadIdx.x; k < kFrags * 64; k += 64 * kInferWaves) reinterpret_cast<uint4*>(s_w)[k] = gw[k];
+    for (int k = threadIdx.x; k < 4 * 64 + 48; k += 64 * kInferWaves) {
         float v;
         if (k < 256) {
             v = M.P[off_b(k >> 6) + (k & 63)];
@@ -480,72 +536,72 @@ __global__ void __launch_bounds__(256) k_mlp_infer(MlpDev M, Src src, uint32_t n
     __syncthreads();
     const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
     const uint32_t n = min(src.count(), nsamples);
-    const uint32_t nchunks = (n + 15) / 16;
-    const uint32_t stride = gridDim.x * 4;
-    uint32_t ch = blockIdx.x * 4 + (threadIdx.x >> 6);
-    // the grid is sized for the queue's capacity: a wave without a chunk of the live count
-    // leaves before loading the weights (no block barrier follows)
-    if (ch >= nchunks) return;
-    bf16x8 f0[4][3], f1[4][2], f2[4][2], f3[4][2], f4[3][2];
-    load_frags<3, 4>(Wl(M, 0), KI, f0);
-    load_frags<2, 4>(WPl(M, 1), 64, f1);
-    load_frags<2, 4>(WPl(M, 2), 64, f2);
-    load_frags<2, 4>(WPl(M, 3), 64, f3);
-    load_frags<2, 3>(WPl(M, 4), 64, f4);
-    // the next chunk's sample is loaded while this one computes (software pipelining)
-    Sample nxt;
-    bool nxt_valid = src.get(ch * 16 + r, nxt);  // the 4 lanes (r, g = 0..3) hold sample r
-    for (; ch < nchunks; ch += stride) {
-        const uint32_t base = ch * 16;
-        Sample smp = nxt;
-        const bool valid = nxt_valid;
-        nxt_valid = (ch + stride < nchunks) && src.get((ch + stride) * 16 + r, nxt);
-        if (!valid) { smp.p = smp.n = smp.wo = smp.dir = mk(0, 0, 0); smp.w = 0.0f; }
-        bf16x8 h[2];
-        {   // layer 0: B operand = features 32 kk + 8 g .. of sample r, encoded in registers
-            bf16x8 x[3];
+    const uint32_t nchunks = (n + 31) / 32;
+    const uint32_t stride = gridDim.x * kInferWaves;
+    uint32_t ch = blockIdx.x * kInferWaves + (threadIdx.x >> 6);
+    // the next step's samples are loaded while this one computes (software pipelining); lanes
+    // (r, g = 0..3) of tile t hold sample 32 ch + 16 t + r
+    Sample nxt[2];
+    bool nv[2];
 #pragma unroll
-            for (int kk = 0; kk < 3; ++kk) x[kk] = encode8(M, smp, valid, kk * 32 + g * 8);
-            f32x4 d[4];
+    for (int t = 0; t < 2; ++t) nv[t] = ch < nchunks && src.get(ch * 32 + t * 16 + r, nxt[t]);
+    for (; ch < nchunks; ch += stride) {
+        // the weight fragments stay in LDS: keep the compiler from hoisting their reads into
+        // registers across steps (that would cost the occupancy this layout is for)
+        asm volatile("" ::: "memory");
+        const uint32_t base = ch * 32;
+        Sample smp[2] = {nxt[0], nxt[1]};
+        const bool valid[2] = {nv[0], nv[1]};
+        const bool more = ch + stride < nchunks;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) nv[t] = more && src.get((ch + stride) * 32 + t * 16 + r, nxt[t]);
+        bf16x8 h[2][2];
+        {   // layer 0: B operand = features 32 kk + 8 g .. of sample r, encoded in registers
+            bf16x8 x[2][3];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                if (!valid[t]) { smp[t].p = smp[t].n = smp[t].wo = smp[t].dir = mk(0, 0, 0); smp[t].w = 0.0f; }
+#pragma unroll
+                for (int kk = 0; kk < 3; ++kk) x[t][kk] = encode8(M, smp[t], valid[t], kk * 32 + g * 8);
+            }
+            f32x4 d[2][4];
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
-                d[nt] = *reinterpret_cast<const f32x4*>(s_bias + nt * 16 + 4 * g);
+                const f32x4 b = *reinterpret_cast<const f32x4*>(s_bias + nt * 16 + 4 * g);
+                d[0][nt] = b;
+                d[1][nt] = b;
 #pragma unroll
-                for (int kk = 0; kk < 3; ++kk) d[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f0[nt][kk], x[kk], d[nt], 0, 0, 0);
+                for (int kk = 0; kk < 3; ++kk) {
+                    const bf16x8 a = frag(s_w, frag_index(0, nt, kk));
+                    d[0][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x[0][kk], d[0][nt], 0, 0, 0);
+                    d[1][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x[1][kk], d[1][nt], 0, 0, 0);
+                }
             }
-            h[0] = relu_bf16x8(d[0], d[1]);
-            h[1] = relu_bf16x8(d[2], d[3]);
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                h[t][0] = relu_bf16x8(d[t][0], d[t][1]);
+                h[t][1] = relu_bf16x8(d[t][2], d[t][3]);
+            }
         }
-        hidden_t(f1, s_bias + 64, h);
-        hidden_t(f2, s_bias + 128, h);
-        hidden_t(f3, s_bias + 192, h);
-        f32x4 e[3];
+        hidden2(s_w, s_bias + 64, 1, h);
+        hidden2(s_w, s_bias + 128, 2, h);
+        hidden2(s_w, s_bias + 192, 3, h);
+        f32x4 e[2][3];
 #pragma unroll
         for (int nt = 0; nt < 3; ++nt) {
-            e[nt] = *reinterpret_cast<const f32x4*>(s_bias + 256 + nt * 16 + 4 * g);
-            e[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f4[nt][0], h[0], e[nt], 0, 0, 0);
-            e[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f4[nt][1], h[1], e[nt], 0, 0, 0);
-        }
-        // head: this lane holds the raw outputs of lobes 2g (o[0..4]) and 2g + 1 (o[5..9]) of sample r
-        float o[10];
+            const f32x4 b = *reinterpret_cast<const f32x4*>(s_bias + 256 + nt * 16 + 4 * g);
+            e[0][nt] = b;
+            e[1][nt] = b;
 #pragma unroll
-        for (int q = 0; q < 10; ++q) o[q] = e[q >> 2][q & 3];
-        const float emax = group_max4t(fmaxf(o[4], o[9]));
-        const float x0 = hexp(o[4] - emax), x1 = hexp(o[9] - emax);
-        const float z = group_sum4t(x0 + x1);
-        if (valid) {
-            float* dst = lobes + (size_t)(base + r) * (NK * 5) + g * 10;
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const float* v = o + 5 * j;
-                const float len = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + 1e-12f);
-                dst[j * 5 + 0] = v[0] / len;
-                dst[j * 5 + 1] = v[1] / len;
-                dst[j * 5 + 2] = v[2] / len;
-                dst[j * 5 + 3] = fminf(0.1f + softplus_f(v[3]), 1000.0f);
-                dst[j * 5 + 4] = (j == 0 ? x0 : x1) / z;
+            for (int kk = 0; kk < 2; ++kk) {
+                const bf16x8 a = frag(s_w, frag_index(4, nt, kk));
+                e[0][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, h[0][kk], e[0][nt], 0, 0, 0);
+                e[1][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, h[1][kk], e[1][nt], 0, 0, 0);
             }
         }
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+            head_store_t<H16>(e[t], valid[t], lobes, (size_t)(base + t * 16 + r));
     }
 }
 
@@ -625,8 +681,7 @@ __global__ void k_mlp_finish(uint32_t* __restrict__ state, uint32_t* __restrict_
     *bad = 0;
 }
 
-// bf16 copies: W_l [Opad][Kpad], WT_l [64][64] (l >= 1) and the permuted inference copies WP_l
-// (l >= 1), zero padded
+// bf16 copies: W_l [Opad][Kpad], WT_l [64][64] (l >= 1) and the inference fragments, zero padded
 __global__ void k_mlp_pack(const float* __restrict__ P, __bf16* __restrict__ pk) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= kPackElems) return;
@@ -638,19 +693,24 @@ __global__ void k_mlp_pack(const float* __restrict__ P, __bf16* __restrict__ pk)
         int kp = (l == 0) ? KI : 64;
         int o = local / kp, i = local % kp;
         if (o < out_of(l) && i < in_of(l)) val = P[off_w(l) + o * in_of(l) + i];
-    } else if (e < kWIOff) {
+    } else if (e < kFWOff) {
         int local = e - kWTOff;
         int l = 1 + local / 4096;
         int i = (local % 4096) / 64, o = local % 64;
         if (o < out_of(l) && i < in_of(l)) val = P[off_w(l) + o * in_of(l) + i];
     } else {
-        int local = e - kWIOff;
-        int l = 1 + local / 4096;
-        if (l > 4) l = 4;
-        local -= (l - 1) * 4096;
-        int row = local / 64, v = local % 64;
-        int o = (l == 4) ? perm_out4(row) : row;
-        if (o >= 0 && o < out_of(l)) val = P[off_w(l) + o * in_of(l) + perm_k(v)];
+        const int local = e - kFWOff;
+        const int f = local / 512, lane = (local % 512) / 8, i = local % 8;
+        const int l = f < 12 ? 0 : (f < 36 ? 1 + (f - 12) / 8 : 4);
+        const int fl = f < 12 ? f : (f < 36 ? (f - 12) % 8 : f - 36);
+        const int nt = l == 0 ? fl / 3 : fl / 2, kk = l == 0 ? fl % 3 : fl % 2;
+        const int row = nt * 16 + (lane & 15), col = kk * 32 + (lane >> 4) * 8 + i;
+        if (l == 0) {
+            if (row < NH && col < NI) val = P[off_w(0) + row * NI + col];
+        } else {
+            const int o = (l == 4) ? perm_out4(row) : row;
+            if (o >= 0 && o < out_of(l)) val = P[off_w(l) + o * in_of(l) + perm_k(col)];
+        }
     }
     pk[e] = (__bf16)val;
 }
@@ -729,11 +789,12 @@ template <class K>
 uint32_t infer_grid(rrh_ctx* c, rrh_mlp* m, K kernel, uint32_t n) {
     if (m->infer_resident == 0) {
         int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64 * kInferWaves, 0) != hipSuccess || per_cu < 1)
+            per_cu = 1;
         m->infer_resident = (uint32_t)per_cu * m->cu_count;
     }
     (void)c;
-    const uint32_t need = (n + kBlockSamples - 1) / kBlockSamples;
+    const uint32_t need = (n + kInferWaves * 32 - 1) / (kInferWaves * 32);  // waves of 32-sample steps
     return need < 1 ? 1 : (need < m->infer_resident ? need : m->infer_resident);
 }
 
@@ -846,8 +907,9 @@ int rrh_mlp_infer(rrh_ctx* c, rrh_mlp* m, const float* pos, const float* nrm, co
     if (rc) return rc;
     float* dl = nullptr;
     RRH_HIP(hipMalloc(&dl, (size_t)n * NK * 5 * 4));
-    const uint32_t g = infer_grid(c, m, k_mlp_infer<HostSrc>, n);
-    hipLaunchKernelGGL(k_mlp_infer<HostSrc>, dim3(g), dim3(256), 0, c->stream, m->dev(), src, n, dl);
+    const uint32_t g = infer_grid(c, m, k_mlp_infer<HostSrc, false>, n);
+    hipLaunchKernelGGL((k_mlp_infer<HostSrc, false>), dim3(g), dim3(64 * kInferWaves), 0, c->stream, m->dev(), src, n,
+                       (void*)dl);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(lobes, dl, (size_t)n * NK * 5 * 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -967,7 +1029,7 @@ struct QueueSrc {  // the prepared inputs of the live ray queue (miss -> invalid
 };
 
 int mlp_infer_queue(rrh_ctx* c, rrh_mlp* m, const DevScene& S, const float4* qo, const float4* qd, const float4* hits,
-                    const uint32_t* cnt, uint32_t b, uint32_t cap, float* lobes) {
+                    const uint32_t* cnt, uint32_t b, uint32_t cap, uint16_t* lobes) {
     (void)qo;
     int rc = c->mlp_prep.ensure((size_t)cap * 48);
     if (rc) return rc;
@@ -976,8 +1038,9 @@ int mlp_infer_queue(rrh_ctx* c, rrh_mlp* m, const DevScene& S, const float4* qo,
     hipLaunchKernelGGL(k_mlp_prep, dim3(gp), dim3(256), 0, c->stream, S, qd, hits, cnt, b, (float4*)c->mlp_prep.p);
     RRH_LAUNCH_CHECK();
     QueueSrc src{(const float4*)c->mlp_prep.p, cnt, b};
-    const uint32_t g = infer_grid(c, m, k_mlp_infer<QueueSrc>, cap);
-    hipLaunchKernelGGL(k_mlp_infer<QueueSrc>, dim3(g), dim3(256), 0, c->stream, m->dev(), src, cap, lobes);
+    const uint32_t g = infer_grid(c, m, k_mlp_infer<QueueSrc, true>, cap);
+    hipLaunchKernelGGL((k_mlp_infer<QueueSrc, true>), dim3(g), dim3(64 * kInferWaves), 0, c->stream, m->dev(), src, cap,
+                       (void*)lobes);
     RRH_LAUNCH_CHECK();
     return RRH_OK;
 }

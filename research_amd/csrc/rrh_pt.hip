@@ -639,7 +639,7 @@ static int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, 
         if (b + 1 >= p->max_bounces || (p->guide_bounces != 0 && b >= p->guide_bounces)) Ab.guide_mode = RRH_GUIDE_NONE;
         if (Ab.guide_mode == RRH_GUIDE_MLP) {
             rc = mlp_infer_queue(c, mlp, S, qi.o, qi.d, (const float4*)Bf.hits, cnt, b, npix,
-                                 const_cast<float*>(A.guide.mlp_lobes));
+                                 const_cast<uint16_t*>(A.guide.mlp_lobes));
             if (rc) return rc;
         }
         hipLaunchKernelGGL(k_shade, dim3(g), dim3(kBlock), 0, c->stream, S, qi, qo, (const float4*)Bf.hits, cnt,

@@ -5,9 +5,16 @@ training for the same wall-clock budget, scored against an FP64-accumulated
 high-spp reference of the same view.
 
 relMSE = mean over pixels of mean_c (I_c - R_c)^2 / (R_c^2 + 0.01)   (eps as in
-Rousselle et al. 2011 and Mueller et al. 2017).
+Rousselle et al. 2011 and Mueller et al. 2017).  The reference's own noise adds Var(R) to
+E[(I - R)^2]; with the reference built from batches, Var(R) is estimated per pixel from the
+spread of the batch means, and relMSE_unbiased = mean (I - R)^2 - Var(R) over the same
+denominator is reported beside the raw figure.
 
 Run:  python -m research_amd.guiding_eval --width 480 --height 270 --ref-spp 65536 --seconds 20
+At 1920x1080 a 65,536-spp reference is built in parts (one process each), then combined:
+      python -m research_amd.guiding_eval --width 1920 --height 1080 --ref-spp 16384 --ref-part 0 --ref-out DIR
+      ... --ref-part 1, 2, 3 ...
+      python -m research_amd.guiding_eval --width 1920 --height 1080 --ref-in DIR --seconds 120
 """
 from __future__ import annotations
 
@@ -27,6 +34,13 @@ EPS = 0.01
 
 def rel_mse(img: np.ndarray, ref: np.ndarray) -> float:
     return float(np.mean((img - ref) ** 2 / (ref ** 2 + EPS)))
+
+
+def rel_mse_unbiased(img: np.ndarray, ref: np.ndarray, ref_var: np.ndarray | None) -> float | None:
+    """relMSE with the reference's own variance removed: E[(I - R)^2] = E[(I - mu)^2] + Var(R)."""
+    if ref_var is None:
+        return None
+    return float(np.mean(((img - ref) ** 2 - ref_var) / (ref ** 2 + EPS)))
 
 
 class Setup:
@@ -56,20 +70,34 @@ class Setup:
     def mean_image(self, spp: int) -> np.ndarray:
         return self.acc.download()[..., :3] / float(spp)
 
-    def reference(self, spp: int, seed0: int = 1 << 24, progress: bool = False) -> np.ndarray:
-        for k in range(spp):
-            self.render(seed0 + k)
-            self.accumulate(k)
-            if progress and (k + 1) % 4096 == 0:
-                self.ctx.synchronize()
-                print(f"reference: {k + 1}/{spp} spp", file=sys.stderr, flush=True)
-        return self.mean_image(spp)
+    def reference(self, spp: int, seed0: int = 1 << 24, progress: bool = False, batch: int = 256):
+        """(mean, variance of the mean) of `spp` unguided samples per pixel, FP64-accumulated on the
+        device in batches of `batch` spp; the variance of the mean comes from the spread of the
+        batch means (Var(mean) = Var(batch mean) / batches)."""
+        batch = min(batch, spp)
+        nb = spp // batch
+        assert nb * batch == spp, "spp must be a multiple of the batch"
+        s1 = np.zeros((self.H, self.W, 3))
+        s2 = np.zeros((self.H, self.W, 3))
+        for b in range(nb):
+            for k in range(batch):
+                self.render(seed0 + b * batch + k)
+                self.accumulate(k)
+            m = self.mean_image(batch)
+            s1 += m
+            s2 += m * m
+            if progress and ((b + 1) * batch) % 1024 == 0:
+                print(f"reference: {(b + 1) * batch}/{spp} spp", file=sys.stderr, flush=True)
+        mean = s1 / nb
+        var = (s2 / nb - mean * mean) * (nb / max(nb - 1, 1)) / nb if nb > 1 else None
+        return mean, (np.maximum(var, 0.0) if var is not None else None)
 
 
-def run_method(S: Setup, method: str, seconds: float, ref: np.ndarray, spp_per_frame: int = 4,
+def run_method(S: Setup, method: str, seconds: float, ref: np.ndarray, ref_var: np.ndarray | None = None,
+               spp_per_frame: int = 4,
                alpha: float = 0.5, batch: int = 262144, record: int | None = None, train_steps: int = 1,
                checkpoints: int = 12, seed0: int = 0, lr: float = 1e-3, decay: float = 0.95,
-               min_weight: float = 0.0, res: int = 32, guide_bounces: int = 0) -> dict:
+               min_weight: float = 0.0, res: int = 32, guide_bounces: int = 0, progress: bool = False) -> dict:
     """Render (and train) for `seconds` of wall time; relMSE at evenly spaced checkpoints.
     The clock stops while a checkpoint image is read back and scored.  The grid keeps its EM
     statistics across frames with weight `decay` per frame (0.95: about 20 frames of memory)."""
@@ -108,22 +136,28 @@ def run_method(S: Setup, method: str, seconds: float, ref: np.ndarray, spp_per_f
         now = elapsed + (time.perf_counter() - t0)
         if now >= next_cp or now >= seconds:
             elapsed = now
-            curve.append((round(now, 4), spp, rel_mse(S.mean_image(spp), ref)))
+            img = S.mean_image(spp)
+            curve.append((round(now, 4), spp, rel_mse(img, ref), rel_mse_unbiased(img, ref, ref_var)))
+            if progress:
+                print(f"{method}: {now:.1f} s, {spp} spp, relMSE {curve[-1][2]:.4g}", file=sys.stderr, flush=True)
             next_cp += seconds / checkpoints
             if now >= seconds:
                 break
             t0 = time.perf_counter()
     out = {"method": method, "seconds": round(elapsed, 3), "spp": spp, "frames": frame, "rel_mse": curve[-1][2],
-           "curve": curve}
+           "rel_mse_unbiased": curve[-1][3], "curve": curve}
     if losses:
         out["loss_first"], out["loss_last"] = float(losses[0]), float(np.mean(losses[-5:]))
     return out
 
 
-def time_to_reach(curve, target: float) -> float | None:
+def time_to_reach(curve, target: float, col: int = 2) -> float | None:
     """First time at which a relMSE curve reaches `target` (linear in log relMSE between checkpoints)."""
     prev = None
-    for t, _, e in curve:
+    for row in curve:
+        t, e = row[0], row[col]
+        if e is None or e <= 0:
+            return None
         if e <= target:
             if prev is None:
                 return t
@@ -136,28 +170,51 @@ def time_to_reach(curve, target: float) -> float | None:
     return None
 
 
+def load_reference(paths: list[str]) -> tuple:
+    """Combine reference parts (equal spp each): mean of the means; the variance of the combined
+    mean is the mean of the parts' variances over the number of parts."""
+    means, vars_, spp = [], [], 0
+    for p in paths:
+        d = np.load(p)
+        means.append(d["mean"].astype(np.float64))
+        vars_.append(d["var"].astype(np.float64))
+        spp += int(d["spp"])
+    n = len(means)
+    return sum(means) / n, sum(vars_) / (n * n), spp
+
+
 def evaluate(ctx: core.Context, width: int = 320, height: int = 180, seconds: float = 4.0, ref_spp: int = 8192,
-             methods=("none", "vmm", "mlp"), scale: float = 1.0, progress: bool = False, **kw) -> dict:
+             methods=("none", "vmm", "mlp"), scale: float = 1.0, progress: bool = False, ref_in: list | None = None,
+             **kw) -> dict:
     S = Setup(ctx, width, height, scale=scale)
     t = time.perf_counter()
-    ref = S.reference(ref_spp, progress=progress)
+    if ref_in:
+        ref, ref_var, ref_spp = load_reference(ref_in)
+    else:
+        ref, ref_var = S.reference(ref_spp, progress=progress)
     ctx.synchronize()
     ref_s = time.perf_counter() - t
     res = {"scene": "indoor_10x10x4_window_sun", "tri_count": S.tri_count, "width": width, "height": height,
            "max_bounces": S.bounces, "ref_spp": ref_spp, "ref_seconds": round(ref_s, 2),
            "ref_mean": float(ref.mean()), "ref_lit_fraction": float((ref.sum(-1) > 0).mean()), "seconds": seconds,
+           "ref_rel_var": None if ref_var is None else float(np.mean(ref_var / (ref ** 2 + EPS))),
            "methods": {}}
     for m in methods:
-        res["methods"][m] = run_method(S, m, seconds, ref, **kw)
+        res["methods"][m] = run_method(S, m, seconds, ref, ref_var, progress=progress, **kw)
         if progress:
             r = res["methods"][m]
-            print(f"{m}: relMSE {r['rel_mse']:.5f} at {r['spp']} spp in {r['seconds']} s", file=sys.stderr, flush=True)
+            print(f"{m}: relMSE {r['rel_mse']:.5g} (unbiased {r['rel_mse_unbiased']}) at {r['spp']} spp in "
+                  f"{r['seconds']} s", file=sys.stderr, flush=True)
     ms = res["methods"]
     if "vmm" in ms and "mlp" in ms:
         res["mlp_over_vmm_rel_mse"] = ms["mlp"]["rel_mse"] / ms["vmm"]["rel_mse"]
         tv = ms["vmm"]["seconds"]
         tm = time_to_reach(ms["mlp"]["curve"], ms["vmm"]["rel_mse"])
         res["mlp_time_to_vmm_rel_mse_ratio"] = None if tm is None else tm / tv
+        if ms["vmm"].get("rel_mse_unbiased"):
+            res["mlp_over_vmm_rel_mse_unbiased"] = ms["mlp"]["rel_mse_unbiased"] / ms["vmm"]["rel_mse_unbiased"]
+            tm = time_to_reach(ms["mlp"]["curve"], ms["vmm"]["rel_mse_unbiased"], col=3)
+            res["mlp_time_to_vmm_rel_mse_unbiased_ratio"] = None if tm is None else tm / tv
     if "none" in ms and "vmm" in ms:
         res["vmm_over_unguided_rel_mse"] = ms["vmm"]["rel_mse"] / ms["none"]["rel_mse"]
     return res
@@ -177,11 +234,32 @@ def main() -> None:
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--alpha", type=float, default=0.5)
     ap.add_argument("--guide-bounces", type=int, default=0, help="guide only the first N bounces (0: all)")
+    ap.add_argument("--ref-part", type=int, default=-1,
+                    help="render reference part k (seeds 2^24 + k * ref_spp ..) into --ref-out and exit")
+    ap.add_argument("--ref-out", default="", help="directory for reference parts")
+    ap.add_argument("--ref-in", default="", help="directory of reference parts to combine (instead of rendering)")
     a = ap.parse_args()
     ctx = core.Context(0)
+    if a.ref_part >= 0:
+        from pathlib import Path
+        S = Setup(ctx, a.width, a.height, scale=a.scale)
+        t = time.perf_counter()
+        mean, var = S.reference(a.ref_spp, seed0=(1 << 24) + a.ref_part * a.ref_spp, progress=True)
+        Path(a.ref_out).mkdir(parents=True, exist_ok=True)
+        out = Path(a.ref_out) / f"ref_{a.width}x{a.height}_part{a.ref_part}.npz"
+        np.savez(out, mean=mean.astype(np.float32), var=var.astype(np.float32), spp=a.ref_spp)
+        print(json.dumps({"part": a.ref_part, "spp": a.ref_spp, "seconds": round(time.perf_counter() - t, 1),
+                          "mean": float(mean.mean()), "file": str(out)}), flush=True)
+        return
+    ref_in = None
+    if a.ref_in:
+        from pathlib import Path
+        ref_in = sorted(str(p) for p in Path(a.ref_in).glob(f"ref_{a.width}x{a.height}_part*.npz"))
+        if not ref_in:
+            raise SystemExit(f"no reference parts for {a.width}x{a.height} in {a.ref_in}")
     res = evaluate(ctx, a.width, a.height, a.seconds, a.ref_spp, tuple(a.methods.split(",")), a.scale, True,
-                   spp_per_frame=a.spp_per_frame, batch=a.batch, train_steps=a.train_steps, lr=a.lr, alpha=a.alpha,
-                   guide_bounces=a.guide_bounces)
+                   ref_in=ref_in, spp_per_frame=a.spp_per_frame, batch=a.batch, train_steps=a.train_steps, lr=a.lr,
+                   alpha=a.alpha, guide_bounces=a.guide_bounces)
     print(json.dumps(res), flush=True)
 
 
