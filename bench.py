@@ -270,7 +270,7 @@ def restir_gi_summary(ctx: core.Context, scene, sa, W: int, H: int, frames: int,
 
 
 # the options a traffic probe must share with the timed run (same kernels, same tree)
-PROBE_OPTS = ["stack", "xcd_remap", "persistent", "max_leaf", "trav_cost", "speculative", "bvh_builder",
+PROBE_OPTS = ["stack", "top_nodes", "xcd_remap", "persistent", "max_leaf", "trav_cost", "speculative", "bvh_builder",
               "ploc_radius", "ray_sort", "bvh_nodes", "width", "height", "max_bounces"]
 COPY_BYTES = 1 << 30
 
@@ -278,7 +278,8 @@ COPY_BYTES = 1 << 30
 def configure(ctx, a) -> None:
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAVERSAL_STACK,
            {"private": L.STACK_PRIVATE, "lds32": L.STACK_LDS32, "hybrid16": L.STACK_HYBRID16,
-            "hybrid32": L.STACK_HYBRID32}[a.stack])
+            "hybrid32": L.STACK_HYBRID32, "none": L.STACK_NONE}[a.stack])
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAV_TOP_NODES, a.top_nodes)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_XCD_REMAP, a.xcd_remap)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_NODES, NODE_FORMATS[a.bvh_nodes])
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_PERSISTENT, a.persistent)
@@ -379,8 +380,10 @@ def main() -> int:
     ap.add_argument("--max-bounces", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--stack", choices=["private", "lds32", "hybrid16", "hybrid32"], default="lds32",
-                    help="where the traversal stack lives (context option)")
+    ap.add_argument("--stack", choices=["private", "lds32", "hybrid16", "hybrid32", "none"], default="lds32",
+                    help="where the traversal stack lives (context option; none: stackless)")
+    ap.add_argument("--top-nodes", type=int, default=0,
+                    help="top-of-tree quantized nodes each workgroup reads from LDS (0..512)")
     ap.add_argument("--xcd-remap", type=int, default=0, help="1: runs of 16 neighbouring workgroups per XCD")
     ap.add_argument("--persistent", type=int, default=0, help="1: persistent queue traversal with lane refill")
     ap.add_argument("--max-leaf", type=int, default=8, help="largest BVH leaf (1..8)")
@@ -484,7 +487,7 @@ def main() -> int:
                    "width": W, "height": H, "spp_per_frame": 1, "max_bounces": a.max_bounces,
                    "di_candidates": 32, "di_temporal_mcap": "20*M", "di_spatial": "5 neighbours, r=30",
                    "camera": "orbit 0.5 deg/frame", "parallelism": "single GPU",
-                   "traversal_stack": a.stack, "xcd_remap": a.xcd_remap, "bvh_nodes": a.bvh_nodes, "persistent": a.persistent, "speculative": a.speculative,
+                   "traversal_stack": a.stack, "top_nodes_in_lds": a.top_nodes, "xcd_remap": a.xcd_remap, "bvh_nodes": a.bvh_nodes, "persistent": a.persistent, "speculative": a.speculative,
                    "bvh_max_leaf": a.max_leaf, "bvh_trav_cost": a.trav_cost, "bvh_builder": a.bvh_builder,
                    "bvh_build_ms": round(info["build_ms"], 1), "ray_sort_bits": a.ray_sort,
                    "rays_per_step": {"primary": prim // a.steps, "extension": ext // a.steps,

@@ -118,6 +118,7 @@ typedef struct {
     uint64_t device_bytes;         /* nodes + triangles + shading records */
     float    build_ms;
     float    bounds_min[3], bounds_max[3];
+    uint32_t bvh_depth4;           /* levels of the 4-wide tree (the stackless traversal takes up to 31) */
 } rrh_scene_info;
 int rrh_scene_upload(rrh_ctx* ctx, const rrh_vertex* verts, uint32_t nv, const uint32_t* idx,
                      uint32_t ni, const rrh_mesh* meshes, uint32_t nm, const rrh_material* mats,
@@ -261,6 +262,10 @@ int rrh_fetch_counts(rrh_ctx* ctx, uint64_t* node_fetches, uint64_t* tri_fetches
                                        (VK_GEOMETRY_INSTANCE_TRIANGLE_FACING_CULL_DISABLE_BIT_KHR,
                                        src/rhi/AccelStructure.cpp:90) override the shaders' cull flag;
                                        1: back faces culled, as RAY_FLAG_CULL_BACK_FACING_TRIANGLES alone */
+#define RRH_OPT_TRAV_TOP_NODES 12   /* 0 (default) or 1..512: the path tracer's ray-queue kernels copy the
+                                       first n quantized 4-wide nodes (the top of the tree: the nodes are
+                                       numbered breadth-first there) into LDS per workgroup and read
+                                       them from there */
 #define RRH_NODES_BINARY 0          /* binary, 64 B (both child boxes in fp32) */
 #define RRH_NODES_WIDE4 1           /* 4-wide, 128 B (fp32 child boxes) */
 #define RRH_NODES_WIDE4Q 2          /* 4-wide, 64 B (8-bit child boxes, outward rounded); default */
@@ -268,6 +273,10 @@ int rrh_fetch_counts(rrh_ctx* ctx, uint64_t* node_fetches, uint64_t* tri_fetches
 #define RRH_STACK_LDS32 1           /* 32 entries per lane in LDS (when the tree can need more: hybrid16) */
 #define RRH_STACK_HYBRID16 2        /* 16 entries in LDS + private overflow, any depth */
 #define RRH_STACK_HYBRID32 3        /* 32 entries in LDS + private overflow, any depth */
+#define RRH_STACK_NONE 4            /* stackless: the path tracer's ray queues are traced with 8 bits of
+                                       trail per level in registers and parent links to climb back
+                                       (4-wide trees up to 31 levels deep; other trees, and every
+                                       other kernel, use the LDS stack RRH_STACK_LDS32 would give) */
 int rrh_ctx_set_option(rrh_ctx* ctx, int option, int value);
 /* plain device-to-device copy kernel (no reference counterpart): measures the achievable
  * HBM bandwidth on this device (2 x bytes / time, GB/s) and calibrates hardware

@@ -39,10 +39,11 @@ PT_COUNT_FETCHES = 1
 OPT_TRAVERSAL_STACK, OPT_XCD_REMAP, OPT_BVH_NODES, OPT_PERSISTENT, OPT_SPECULATIVE = 1, 2, 3, 4, 5
 OPT_BVH_MAX_LEAF, OPT_BVH_TRAV_COST = 6, 7
 OPT_BVH_BUILDER, OPT_PLOC_RADIUS, OPT_RAY_SORT, OPT_RESTIR_CULL = 8, 9, 10, 11
+OPT_TRAV_TOP_NODES = 12
 BVH_BUILD_HOST, BVH_BUILD_DEVICE = 0, 1
 UPDATE_REFIT, UPDATE_REBUILD = 0, 1
 NODES_BINARY, NODES_WIDE4, NODES_WIDE4Q = 0, 1, 2
-STACK_PRIVATE, STACK_LDS32, STACK_HYBRID16, STACK_HYBRID32 = 0, 1, 2, 3
+STACK_PRIVATE, STACK_LDS32, STACK_HYBRID16, STACK_HYBRID32, STACK_NONE = 0, 1, 2, 3, 4
 GRID_HASH, GRID_VMM = 0, 1
 ATROUS_REFERENCE, ATROUS_CONFIG = 0, 1
 ATROUS_FAST_MATH = 1
@@ -59,7 +60,7 @@ class SceneInfo(C.Structure):
     _fields_ = [("tri_count", C.c_uint32), ("node_count", C.c_uint32), ("bvh_depth", C.c_uint32),
                 ("leaf_count", C.c_uint32), ("light_count", C.c_uint32), ("tri_light_count", C.c_uint32),
                 ("node4_count", C.c_uint32), ("stack4", C.c_uint32), ("device_bytes", C.c_uint64), ("build_ms", C.c_float), ("bounds_min", C.c_float * 3),
-                ("bounds_max", C.c_float * 3)]
+                ("bounds_max", C.c_float * 3), ("bvh_depth4", C.c_uint32)]
 
 
 class SceneDesc(C.Structure):

@@ -314,6 +314,41 @@ struct Emitter4 {
     }
 };
 
+// Renumbers the 4-wide nodes so that the first `top` nodes of a breadth-first walk (children
+// in slot order) come first, and the rest keep their depth-first emission order.  Every
+// prefix of the result then holds the ancestors of its nodes (the top of the tree, which the
+// traversal kernels can stage in LDS), and links still point forward.  Closest hits do not
+// depend on node order.
+void reorder_top_bfs(BvhOut& out, uint32_t top) {
+    const uint32_t n4 = (uint32_t)(out.nodes4.size() / 8);
+    auto links = [&](uint32_t n, int32_t l[4]) { std::memcpy(l, &out.nodes4[(size_t)n * 8 + 6], 16); };
+    std::vector<uint32_t> order{0};
+    for (size_t i = 0; i < order.size() && order.size() < top; ++i) {
+        int32_t l[4];
+        links(order[i], l);
+        for (int s = 0; s < 4 && order.size() < top; ++s)
+            if (l[s] > 0) order.push_back((uint32_t)l[s]);
+    }
+    std::vector<uint32_t> id(n4, ~0u);
+    for (uint32_t i = 0; i < (uint32_t)order.size(); ++i) id[order[i]] = i;
+    uint32_t next = (uint32_t)order.size();
+    for (uint32_t n = 0; n < n4; ++n)
+        if (id[n] == ~0u) id[n] = next++;
+    std::vector<float4> n4o(out.nodes4.size()), nqo(out.nodesq.size());
+    for (uint32_t n = 0; n < n4; ++n) {
+        int32_t l[4];
+        links(n, l);
+        for (int s = 0; s < 4; ++s)
+            if (l[s] > 0) l[s] = (int32_t)id[l[s]];
+        std::memcpy(&n4o[(size_t)id[n] * 8], &out.nodes4[(size_t)n * 8], 128);
+        std::memcpy(&n4o[(size_t)id[n] * 8 + 6], l, 16);
+        std::memcpy(&nqo[(size_t)id[n] * 4], &out.nodesq[(size_t)n * 4], 64);
+        std::memcpy(reinterpret_cast<uint32_t*>(&nqo[(size_t)id[n] * 4]) + 10, l, 16);
+    }
+    out.nodes4.swap(n4o);
+    out.nodesq.swap(nqo);
+}
+
 }  // namespace
 
 bool build_bvh(const float* tri_v, uint32_t ntri, BvhOut& out, std::string& err, uint32_t max_leaf,
@@ -390,6 +425,7 @@ bool build_bvh(const float* tri_v, uint32_t ntri, BvhOut& out, std::string& err,
         E4.emit(0, 0, st);
         out.depth4 = E4.max_depth;
         out.stack4 = st;
+        reorder_top_bfs(out, kTopBfsNodes);
     }
     if (out.stack4 > 64) { err = "4-wide BVH needs more than 64 traversal stack entries"; return false; }
     // structural check before anything reaches the device: links in range, every

@@ -638,7 +638,7 @@ int build_bvh_device(rrh_ctx* c, rrh_scene* s, const float* tw, uint32_t max_lea
         RRH_LAUNCH_CHECK();
         s->nnodes = 1; s->depth = 1; s->leaves = 1; s->nnodes4 = 1; s->depth4 = 1; s->stack4 = 0;
         RRH_HIP(hipStreamSynchronize(st));
-        return RRH_OK;
+        return tree_links(c, s, 1);
     }
     const uint32_t n_node = 2 * n - 1, root = n_node - 1;
 
@@ -800,7 +800,7 @@ int build_bvh_device(rrh_ctx* c, rrh_scene* s, const float* tw, uint32_t max_lea
         return RRH_E_STATE;
     }
     s->build_passes = passes;
-    return RRH_OK;
+    return tree_links(c, s, n4);  // every node is numbered breadth-first
 }
 
 // ---- instance updates: flattening, light records, refit ----------------------------------------
@@ -906,6 +906,18 @@ __global__ __launch_bounds__(kBlock) void k_w4_parents(const float4* __restrict_
         if (l[s] > 0) { parent[l[s]] = (n << 2) | s; ++c; }
     inner[n] = c;
     if (n == 0) parent[0] = kNone;
+}
+
+// parent of every 4-wide node (the root's is itself, never read)
+__global__ __launch_bounds__(kBlock) void k_w4_parent_index(const float4* __restrict__ nodes4, uint32_t n4,
+                                                            uint32_t* __restrict__ parent) {
+    const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+    if (n >= n4) return;
+    const float4 L = nodes4[8 * (size_t)n + 6];
+    const int32_t l[4] = {__float_as_int(L.x), __float_as_int(L.y), __float_as_int(L.z), __float_as_int(L.w)};
+    for (uint32_t k = 0; k < 4; ++k)
+        if (l[k] > 0 && (uint32_t)l[k] < n4) parent[l[k]] = n;
+    if (n == 0) parent[0] = 0;
 }
 
 // binary refit, bottom-up: a node whose children are all leaves starts; the last child to
@@ -1015,6 +1027,15 @@ int flatten_device(rrh_ctx* c, rrh_scene* s) {
     };
     for (int k = 0; k < 3; ++k) { s->bmin[k] = fl(b[k]); s->bmax[k] = fl(b[3 + k]); }
     s->tri_world_stale = true;
+    return RRH_OK;
+}
+
+int tree_links(rrh_ctx* c, rrh_scene* s, uint32_t top4) {
+    const uint32_t n4 = s->nnodes4;
+    RRH_TRY(s->parent4.ensure((size_t)n4 * sizeof(uint32_t)));
+    k_w4_parent_index<<<grid(n4), kBlock, 0, c->stream>>>((const float4*)s->nodes4.p, n4, (uint32_t*)s->parent4.p);
+    RRH_LAUNCH_CHECK();
+    s->top4 = std::min(top4, n4);
     return RRH_OK;
 }
 

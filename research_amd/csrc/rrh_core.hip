@@ -48,6 +48,7 @@ rrh::DevScene rrh_scene::dev() const {
     d.nodes = (const float4*)nodes.p;
     d.nodes4 = (const float4*)nodes4.p;
     d.nodesq = (const uint4*)nodesq.p;
+    d.parent4 = (const uint32_t*)parent4.p;
     d.node_fmt = (uint32_t)node_format();
     d.speculative = (ctx && ctx->speculative) ? 1u : 0u;
     d.tris = (const float4*)tris.p;
@@ -143,8 +144,13 @@ int rrh_device_info(rrh_ctx* c, int* cu_count, int* clock_khz, char* name, size_
 int rrh_ctx_set_option(rrh_ctx* c, int option, int value) {
     RRH_ARG(c, "rrh_ctx_set_option: null context");
     if (option == RRH_OPT_TRAVERSAL_STACK) {
-        RRH_ARG(value >= RRH_STACK_PRIVATE && value <= RRH_STACK_HYBRID32, "rrh_ctx_set_option: bad stack kind");
+        RRH_ARG(value >= RRH_STACK_PRIVATE && value <= RRH_STACK_NONE, "rrh_ctx_set_option: bad stack kind");
         c->stack_kind = value;
+        return RRH_OK;
+    }
+    if (option == RRH_OPT_TRAV_TOP_NODES) {
+        RRH_ARG(value >= 0 && value <= 512, "rrh_ctx_set_option: top nodes out of 0..512");
+        c->top_nodes = (uint32_t)value;
         return RRH_OK;
     }
     if (option == RRH_OPT_XCD_REMAP) { c->xcd_remap = value != 0; return RRH_OK; }
@@ -378,6 +384,7 @@ static int host_build_upload(rrh_ctx* c, rrh_scene* s) {
     if (!rc) rc = upload_vec(s->nodes4, bvh.nodes4.data(), bvh.nodes4.size(), st);
     if (!rc) rc = upload_vec(s->nodesq, bvh.nodesq.data(), bvh.nodesq.size(), st);
     if (!rc) rc = upload_vec(s->tris, bvh.tris.data(), bvh.tris.size(), st);
+    if (!rc) rc = tree_links(c, s, kTopBfsNodes);
     if (!rc) {
         hipError_t e = hipStreamSynchronize(st);  // the host vectors die with this frame
         if (e != hipSuccess) { set_error(hipGetErrorString(e)); rc = RRH_E_HIP; }
@@ -530,6 +537,7 @@ int rrh_scene_info_get(const rrh_scene* s, rrh_scene_info* o) {
                       (uint64_t)s->ntri * 12 + (uint64_t)s->ntl * kTlF4 * 16;
     o->build_ms = s->build_ms;
     for (int k = 0; k < 3; ++k) { o->bounds_min[k] = s->bmin[k]; o->bounds_max[k] = s->bmax[k]; }
+    o->bvh_depth4 = s->depth4;
     return RRH_OK;
 }
 
@@ -592,20 +600,51 @@ int rrh_camera_set(rrh_scene* s, const rrh_camera* cam) {
 // ---------------------------------------------------------------------------
 // ray-query kernels (parity probe and timing)
 // ---------------------------------------------------------------------------
+// batch ray queries: private stack (STK = RRH_STACK_PRIVATE) or stackless (RRH_STACK_NONE,
+// 4-wide trees), TOP: the first top_n quantized nodes read from LDS (dynamic shared memory)
+extern __shared__ uint4 s_top_nodes[];
+template <int STK, bool TOP>
 __global__ void __launch_bounds__(256) k_trace_closest(DevScene S, const float4* __restrict__ rays, uint32_t n,
-                                                      float4* __restrict__ hits) {
+                                                      float4* __restrict__ hits, uint32_t top_n) {
+    const auto top = stage_top<TOP>(S, s_top_nodes, top_n);
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float4 a = rays[2 * i], b = rays[2 * i + 1];
-    Hit h = trace_closest(S, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w);
+    typename StackSel<STK>::T st;
+    Hit h = trace_closest_s<false>(S, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w, st, nullptr, nullptr, top);
     hits[i] = make_float4(h.t, __uint_as_float(h.prim), h.b1, h.b2);
 }
+template <int STK, bool TOP>
 __global__ void __launch_bounds__(256) k_trace_occluded(DevScene S, const float4* __restrict__ rays, uint32_t n,
-                                                       uint32_t* __restrict__ occ) {
+                                                       uint32_t* __restrict__ occ, uint32_t top_n) {
+    const auto top = stage_top<TOP>(S, s_top_nodes, top_n);
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float4 a = rays[2 * i], b = rays[2 * i + 1];
-    occ[i] = trace_occluded(S, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w) ? 1u : 0u;
+    typename StackSel<STK>::T st;
+    occ[i] = trace_occluded_s(S, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w, st, top) ? 1u : 0u;
+}
+// launches a batch query with the context's traversal options: stackless when RRH_STACK_NONE
+// is set and the tree is 4-wide and at most 31 levels deep, the top of the tree in LDS when
+// RRH_OPT_TRAV_TOP_NODES is set on quantized nodes
+template <bool OCC>
+hipError_t launch_query(const rrh_ctx* c, const rrh_scene* s, const void* rays, uint32_t n, void* out) {
+    const int fmt = s->node_format();
+    const bool sl = c->stack_kind == RRH_STACK_NONE && fmt != RRH_NODES_BINARY && s->depth4 <= 31;
+    const uint32_t top_n = fmt == RRH_NODES_WIDE4Q ? std::min(c->top_nodes, s->top4) : 0u;
+    const size_t lds = (size_t)top_n * 4 * sizeof(uint4);
+    const dim3 g((n + 255) / 256), b(256);
+    const DevScene S = s->dev();
+    const float4* r = (const float4*)rays;
+    auto go = [&](auto kc, auto ko) {
+        if constexpr (OCC) hipLaunchKernelGGL(ko, g, b, lds, c->stream, S, r, n, (uint32_t*)out, top_n);
+        else hipLaunchKernelGGL(kc, g, b, lds, c->stream, S, r, n, (float4*)out, top_n);
+    };
+    if (sl && top_n) go(k_trace_closest<RRH_STACK_NONE, true>, k_trace_occluded<RRH_STACK_NONE, true>);
+    else if (sl) go(k_trace_closest<RRH_STACK_NONE, false>, k_trace_occluded<RRH_STACK_NONE, false>);
+    else if (top_n) go(k_trace_closest<RRH_STACK_PRIVATE, true>, k_trace_occluded<RRH_STACK_PRIVATE, true>);
+    else go(k_trace_closest<RRH_STACK_PRIVATE, false>, k_trace_occluded<RRH_STACK_PRIVATE, false>);
+    return hipGetLastError();
 }
 
 // grid-stride copy, four independent 16 B nontemporal loads in flight per lane before their
@@ -655,9 +694,7 @@ int rrh_stream_copy(rrh_ctx* c, size_t bytes, int iterations, float* gbps) {
 int rrh_trace_closest_device(rrh_ctx* c, const rrh_scene* s, const void* d_rays, uint32_t n, void* d_hits) {
     RRH_ARG(c && s && d_rays && d_hits, "rrh_trace_closest_device: null argument");
     if (n == 0) return RRH_OK;
-    hipLaunchKernelGGL(k_trace_closest, dim3((n + 255) / 256), dim3(256), 0, c->stream, s->dev(),
-                       (const float4*)d_rays, n, (float4*)d_hits);
-    RRH_LAUNCH_CHECK();
+    RRH_HIP(launch_query<false>(c, s, d_rays, n, d_hits));
     return RRH_OK;
 }
 
@@ -684,9 +721,7 @@ int rrh_trace_closest(rrh_ctx* c, const rrh_scene* s, const rrh_ray* rays, uint3
     if (rc) return rc;
     hipError_t e = hipMemcpyAsync(dr, rays, (size_t)n * 32, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_trace_closest, dim3((n + 255) / 256), dim3(256), 0, c->stream, s->dev(),
-                           (const float4*)dr, n, (float4*)dh);
-        e = hipGetLastError();
+        e = launch_query<false>(c, s, dr, n, dh);
     }
     if (e == hipSuccess) e = hipMemcpyAsync(hits, dh, (size_t)n * 16, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -704,9 +739,7 @@ int rrh_trace_occluded(rrh_ctx* c, const rrh_scene* s, const rrh_ray* rays, uint
     if (rc) return rc;
     hipError_t e = hipMemcpyAsync(dr, rays, (size_t)n * 32, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_trace_occluded, dim3((n + 255) / 256), dim3(256), 0, c->stream, s->dev(),
-                           (const float4*)dr, n, (uint32_t*)dh);
-        e = hipGetLastError();
+        e = launch_query<true>(c, s, dr, n, dh);
     }
     if (e == hipSuccess) e = hipMemcpyAsync(occ, dh, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

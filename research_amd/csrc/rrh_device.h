@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "../../include/rrh.h"
 
 #define RRH_D __device__ __forceinline__
@@ -127,6 +129,7 @@ struct DevScene {
     const float4* nodes;       // 4 float4 per binary node (child boxes + child links)
     const float4* nodes4;      // 8 float4 per 4-wide node (the same tree collapsed, same leaves)
     const uint4* nodesq;       // 4 uint4 per quantized 4-wide node (same links as nodes4)
+    const uint32_t* parent4;   // parent of each 4-wide node (stackless traversal)
     uint32_t node_fmt;         // RRH_NODES_*: which of the three the traversal reads
     uint32_t speculative;      // 1: traversal postpones leaves (Aila and Laine 2009; binary and quantized 4-wide)
     const float4* tris;        // 3 float4 per triangle, leaf order; tris[3i].w = flattened id bits
@@ -218,8 +221,17 @@ struct Hit { float t; uint32_t prim; float b1, b2; };
 // traversal stacks: a lane's entries in LDS are 64 words apart (one wave's lanes
 // hit 64 consecutive words, so there are no bank conflicts); the private variant
 // is the fallback for deeper trees
+// LDS pointers carry their address space in the type: a pointer that may point at LDS or at
+// private memory (the hybrid stack's two halves) would otherwise be merged into one generic
+// pointer, and every access would become a flat instruction instead of a ds_ one
+typedef __attribute__((address_space(3))) int lds_int;
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) const u32x4 lds_u32x4;
+typedef __attribute__((address_space(3))) const u32x2 lds_u32x2;
+typedef __attribute__((address_space(3))) const int lds_cint;
 struct StackLds {
-    int* p;
+    lds_int* p;
     RRH_D void put(int i, int v) { p[i * 64] = v; }
     RRH_D int get(int i) const { return p[i * 64]; }
 };
@@ -231,7 +243,7 @@ struct StackPriv {
 // the top N entries in LDS, the rest (rarely reached) in private memory
 template <int N>
 struct StackHybridN {
-    int* p;
+    lds_int* p;
     int ov[RRH_STACK - N];
     RRH_D void put(int i, int v) {
         if (i < N) p[i * 64] = v;
@@ -247,15 +259,17 @@ template <> struct StackSel<RRH_STACK_PRIVATE> { using T = StackPriv; static con
 template <> struct StackSel<RRH_STACK_LDS32> { using T = StackLds; static constexpr int kLds = 32; };
 template <> struct StackSel<RRH_STACK_HYBRID16> { using T = StackHybrid; static constexpr int kLds = RRH_HYB_LDS; };
 template <> struct StackSel<RRH_STACK_HYBRID32> { using T = StackHybridN<32>; static constexpr int kLds = 32; };
+struct StackNone {};  // stackless traversal (trace_closest4sl_s): no stack memory
+template <> struct StackSel<RRH_STACK_NONE> { using T = StackNone; static constexpr int kLds = 1; };
 
 // per-lane LDS base inside a block array of (blockDim/64) x entries x 64 ints
-RRH_D int* lds_stack_base(int* smem, int entries) {
-    return smem + (threadIdx.x >> 6) * (entries * 64) + (threadIdx.x & 63);
+RRH_D lds_int* lds_stack_base(int* smem, int entries) {
+    return (lds_int*)(smem + (threadIdx.x >> 6) * (entries * 64) + (threadIdx.x & 63));
 }
 template <int K>
 RRH_D typename StackSel<K>::T make_stack(int* smem) {
     typename StackSel<K>::T st;
-    if constexpr (K != RRH_STACK_PRIVATE) st.p = lds_stack_base(smem, StackSel<K>::kLds);
+    if constexpr (K != RRH_STACK_PRIVATE && K != RRH_STACK_NONE) st.p = lds_stack_base(smem, StackSel<K>::kLds);
     return st;
 }
 
@@ -569,23 +583,75 @@ RRH_D float4 dequant4(float o, uint32_t q, float s) {
     return make_float4(o + (float)(q & 255u) * s, o + (float)((q >> 8) & 255u) * s, o + (float)((q >> 16) & 255u) * s,
                        o + (float)(q >> 24) * s);
 }
+// quantized node words: origin (3 floats) and per-axis exponents, then the children's
+// 8-bit bounds, then the four links
+RRH_D void decode_q(const uint4& a, const uint4& b, const uint4& c, const uint2& d, Node4& N) {
+    const float sx = __uint_as_float((a.w & 255u) << 23), sy = __uint_as_float(((a.w >> 8) & 255u) << 23),
+                sz = __uint_as_float(((a.w >> 16) & 255u) << 23);
+    const float ox = __uint_as_float(a.x), oy = __uint_as_float(a.y), oz = __uint_as_float(a.z);
+    N.xl = dequant4(ox, b.x, sx); N.xh = dequant4(ox, b.y, sx);
+    N.yl = dequant4(oy, b.z, sy); N.yh = dequant4(oy, b.w, sy);
+    N.zl = dequant4(oz, c.x, sz); N.zh = dequant4(oz, c.y, sz);
+    N.lk = make_int4((int)c.z, (int)c.w, (int)d.x, (int)d.y);
+}
 template <bool Q>
 RRH_D void load_node4(const DevScene& S, int node, Node4& N) {
     if constexpr (Q) {
         const uint4* n = S.nodesq + (size_t)node * 4;
-        const uint4 a = n[0], b = n[1], c = n[2];
-        const uint2 d = *reinterpret_cast<const uint2*>(n + 3);
-        const float sx = __uint_as_float((a.w & 255u) << 23), sy = __uint_as_float(((a.w >> 8) & 255u) << 23),
-                    sz = __uint_as_float(((a.w >> 16) & 255u) << 23);
-        const float ox = __uint_as_float(a.x), oy = __uint_as_float(a.y), oz = __uint_as_float(a.z);
-        N.xl = dequant4(ox, b.x, sx); N.xh = dequant4(ox, b.y, sx);
-        N.yl = dequant4(oy, b.z, sy); N.yh = dequant4(oy, b.w, sy);
-        N.zl = dequant4(oz, c.x, sz); N.zh = dequant4(oz, c.y, sz);
-        N.lk = make_int4((int)c.z, (int)c.w, (int)d.x, (int)d.y);
+        decode_q(n[0], n[1], n[2], *reinterpret_cast<const uint2*>(n + 3), N);
     } else {
         const float4* n = S.nodes4 + (size_t)node * 8;
         N.xl = n[0]; N.xh = n[1]; N.yl = n[2]; N.yh = n[3]; N.zl = n[4]; N.zh = n[5];
         N.lk = *reinterpret_cast<const int4*>(n + 6);
+    }
+}
+
+// Top of the tree in LDS: a workgroup copies the first n quantized nodes into LDS and reads
+// them from there.  The nodes are numbered breadth-first at the top of the tree, so those n
+// are the root and the levels below it.
+struct TopNone {
+    static constexpr bool kOn = false;
+};
+struct TopLds {
+    static constexpr bool kOn = true;
+    lds_u32x4* p;  // 4 x 16 B per node
+    uint32_t n;
+};
+template <bool Q, class Top>
+RRH_D void load_node4(const DevScene& S, const Top& top, int node, Node4& N) {
+    if constexpr (Q && Top::kOn) {
+        if ((uint32_t)node < top.n) {
+            lds_u32x4* n = top.p + (size_t)node * 4;
+            const u32x4 a = n[0], b = n[1], c = n[2];
+            const u32x2 d = *(lds_u32x2*)(n + 3);
+            decode_q(make_uint4(a.x, a.y, a.z, a.w), make_uint4(b.x, b.y, b.z, b.w), make_uint4(c.x, c.y, c.z, c.w),
+                     make_uint2(d.x, d.y), N);
+            return;
+        }
+    }
+    load_node4<Q>(S, node, N);
+}
+// one child link of a 4-wide node
+template <bool Q, class Top>
+RRH_D int node_link(const DevScene& S, const Top& top, int node, uint32_t slot) {
+    if constexpr (Q) {
+        if constexpr (Top::kOn) {
+            if ((uint32_t)node < top.n) return ((lds_cint*)top.p)[(size_t)node * 16 + 10 + slot];
+        }
+        return reinterpret_cast<const int*>(S.nodesq)[(size_t)node * 16 + 10 + slot];
+    } else {
+        return reinterpret_cast<const int*>(S.nodes4)[(size_t)node * 32 + 24 + slot];
+    }
+}
+// stages the top n quantized nodes in `lds` (every thread of the block calls it)
+template <bool TOP>
+RRH_D auto stage_top(const DevScene& S, uint4* lds, uint32_t n) {
+    if constexpr (TOP) {
+        for (uint32_t i = threadIdx.x; i < n * 4; i += blockDim.x) lds[i] = S.nodesq[i];
+        __syncthreads();
+        return TopLds{(lds_u32x4*)lds, n};
+    } else {
+        return TopNone{};
     }
 }
 
@@ -709,9 +775,9 @@ RRH_D bool trace_occluded4_s(const DevScene& S, f3 o, f3 d, float tmin, float tm
 
 // speculative (leaf-parking) traversal over the 4-wide nodes; CULL and first_hit as in
 // trace_closest4_s
-template <bool COUNT, bool Q, bool CULL = false, class Stk>
+template <bool COUNT, bool Q, bool CULL = false, class Stk, class Top = TopNone>
 RRH_D Hit trace_closest4spec_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack,
-                               uint32_t* n_nodes, uint32_t* n_tris, bool first_hit = false) {
+                               uint32_t* n_nodes, uint32_t* n_tris, bool first_hit = false, const Top& top = Top()) {
     Hit h; h.t = tmax; h.prim = ~0u; h.b1 = 0.0f; h.b2 = 0.0f;
     WRay r = wray_setup(o, d);
     int sp = 0, node = 0, leaf = 0;
@@ -719,7 +785,7 @@ RRH_D Hit trace_closest4spec_s(const DevScene& S, f3 o, f3 d, float tmin, float 
         while ((uint32_t)node < (uint32_t)kTravDone) {
             if constexpr (COUNT) (*n_nodes)++;
             Node4 N;
-            load_node4<Q>(S, node, N);
+            load_node4<Q>(S, top, node, N);
             float k[4];
             int c[4];
 #pragma unroll
@@ -773,15 +839,16 @@ RRH_D Hit trace_closest4spec_s(const DevScene& S, f3 o, f3 d, float tmin, float 
     }
     return h;
 }
-template <bool Q, class Stk>
-RRH_D bool trace_occluded4spec_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack) {
+template <bool Q, class Stk, class Top = TopNone>
+RRH_D bool trace_occluded4spec_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack,
+                                 const Top& top = Top()) {
     if (!(tmax >= tmin)) return false;
     WRay r = wray_setup(o, d);
     int sp = 0, node = 0, leaf = 0;
     while (node != kTravDone) {
         while ((uint32_t)node < (uint32_t)kTravDone) {
             Node4 N;
-            load_node4<Q>(S, node, N);
+            load_node4<Q>(S, top, node, N);
             int next = kTravDone;
             bool have = false;
 #pragma unroll
@@ -817,27 +884,204 @@ RRH_D bool trace_occluded4spec_s(const DevScene& S, f3 o, f3 d, float tmin, floa
     return false;
 }
 
-// the scene's node format (uniform across a launch) picks the traversal
-template <bool COUNT, class Stk>
-RRH_D Hit trace_closest_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack,
-                          uint32_t* n_nodes = nullptr, uint32_t* n_tris = nullptr) {
-    if (S.node_fmt == RRH_NODES_WIDE4Q) {
-        if (S.speculative) return trace_closest4spec_s<COUNT, true>(S, o, d, tmin, tmax, stack, n_nodes, n_tris);
-        return trace_closest4_s<COUNT, true>(S, o, d, tmin, tmax, stack, n_nodes, n_tris);
+// ---------------------------------------------------------------------------
+// stackless traversal of the 4-wide nodes
+// ---------------------------------------------------------------------------
+// In place of a stack of node links, each level of the current path keeps 8 bits in
+// registers: the number of that node's hit children still to visit (bits 0-1) and their
+// slots in visit order (bits 2-7).  When a level runs out the traversal climbs to the parent
+// (DevScene::parent4) and drops the level; a pop reads the next slot's link from the node.
+// 32 levels in four 64-bit words: trees up to 31 levels deep (the root's level plus one
+// spare).  The leaf parking of the speculative traversal is kept; hits are the same as any
+// other traversal's (closest t, ties to the smaller id, conservative box tests).
+struct Trail {
+    uint64_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+    RRH_D uint32_t level() const { return (uint32_t)w0 & 255u; }
+    RRH_D void set_level(uint32_t v) { w0 = (w0 & ~255ull) | v; }
+    RRH_D void push(uint32_t v) {
+        w3 = (w3 << 8) | (w2 >> 56);
+        w2 = (w2 << 8) | (w1 >> 56);
+        w1 = (w1 << 8) | (w0 >> 56);
+        w0 = (w0 << 8) | v;
     }
-    if (S.node_fmt == RRH_NODES_WIDE4) return trace_closest4_s<COUNT, false>(S, o, d, tmin, tmax, stack, n_nodes, n_tris);
-    if (S.speculative) return trace_closest2spec_s<COUNT>(S, o, d, tmin, tmax, stack, n_nodes, n_tris);
-    return trace_closest2_s<COUNT>(S, o, d, tmin, tmax, stack, n_nodes, n_tris);
+    RRH_D void pop() {
+        w0 = (w0 >> 8) | (w1 << 56);
+        w1 = (w1 >> 8) | (w2 << 56);
+        w2 = (w2 >> 8) | (w3 << 56);
+        w3 >>= 8;
+    }
+};
+// next node to visit: the next slot of the deepest level with children left (climbing to it),
+// or kTravDone at the end
+template <bool Q, class Top>
+RRH_D int trail_next(const DevScene& S, const Top& top, int& cur, Trail& tr) {
+    while (true) {
+        const uint32_t b = tr.level();
+        const uint32_t left = b & 3u;
+        if (left) {
+            tr.set_level(((b >> 4) << 2) | (left - 1u));
+            return node_link<Q>(S, top, cur, (b >> 2) & 3u);
+        }
+        if (cur == 0) return kTravDone;
+        cur = (int)S.parent4[cur];
+        tr.pop();
+    }
 }
-template <class Stk>
-RRH_D bool trace_occluded_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack) {
-    if (S.node_fmt == RRH_NODES_WIDE4Q) {
-        if (S.speculative) return trace_occluded4spec_s<true>(S, o, d, tmin, tmax, stack);
-        return trace_occluded4_s<true>(S, o, d, tmin, tmax, stack);
+
+template <bool COUNT, bool Q, bool CULL = false, class Top = TopNone>
+RRH_D Hit trace_closest4sl_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, uint32_t* n_nodes,
+                             uint32_t* n_tris, bool first_hit = false, const Top& top = Top()) {
+    Hit h; h.t = tmax; h.prim = ~0u; h.b1 = 0.0f; h.b2 = 0.0f;
+    WRay r = wray_setup(o, d);
+    Trail tr;
+    int cur = 0, node = 0, leaf = 0;
+    while (node != kTravDone) {
+        while ((uint32_t)node < (uint32_t)kTravDone) {
+            if constexpr (COUNT) (*n_nodes)++;
+            Node4 N;
+            load_node4<Q>(S, top, node, N);
+            float k[4];
+            int c[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float tn;
+                const int l = i4c(N.lk, i);
+                const bool hit = (l != 0) && box_hit(f4c(N.xl, i), f4c(N.xh, i), f4c(N.yl, i), f4c(N.yh, i),
+                                                     f4c(N.zl, i), f4c(N.zh, i), r, tmin, h.t, tn);
+                k[i] = hit ? tn : INFINITY;
+                c[i] = i;  // slots, sorted by entry distance with their keys
+            }
+            cswap(k[0], c[0], k[1], c[1]);
+            cswap(k[2], c[2], k[3], c[3]);
+            cswap(k[0], c[0], k[2], c[2]);
+            cswap(k[1], c[1], k[3], c[3]);
+            cswap(k[1], c[1], k[2], c[2]);
+            if (!(k[0] < INFINITY)) {
+                node = trail_next<Q>(S, top, cur, tr);
+            } else {
+                const uint32_t left = (k[1] < INFINITY ? 1u : 0u) + (k[2] < INFINITY ? 1u : 0u) + (k[3] < INFINITY ? 1u : 0u);
+                tr.push(left | (uint32_t)c[1] << 2 | (uint32_t)c[2] << 4 | (uint32_t)c[3] << 6);
+                cur = node;
+                node = i4c(N.lk, c[0]);
+            }
+            if (node < 0 && leaf >= 0) {
+                leaf = node;
+                node = trail_next<Q>(S, top, cur, tr);
+            }
+            if (!__any(leaf >= 0)) break;
+        }
+        while (leaf < 0) {
+            const uint32_t first = RRH_LEAF_FIRST(leaf), cnt = RRH_LEAF_COUNT(leaf);
+            if constexpr (COUNT) (*n_tris) += cnt;
+            for (uint32_t i = 0; i < cnt; ++i) {
+                const float4* tv = S.tris + (size_t)(first + i) * 3;
+                float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
+                float tt, b1, b2;
+                uint32_t id = __float_as_uint(p0.w);
+                if (wt_hit(r, xyz(p0), xyz(p1), xyz(p2), tmin, h.t, tt, b1, b2) && better(tt, id, h.t, h.prim)) {
+                    if constexpr (CULL) {
+                        const f3 a = xyz(p0);
+                        if (!(dot(cross(sub(xyz(p1), a), sub(xyz(p2), a)), d) < 0.0f)) continue;
+                    }
+                    h.t = tt; h.prim = id; h.b1 = b1; h.b2 = b2;
+                    if (first_hit) return h;
+                }
+            }
+            leaf = node;
+            if (node < 0) node = trail_next<Q>(S, top, cur, tr);
+        }
     }
-    if (S.node_fmt == RRH_NODES_WIDE4) return trace_occluded4_s<false>(S, o, d, tmin, tmax, stack);
-    if (S.speculative) return trace_occluded2spec_s(S, o, d, tmin, tmax, stack);
-    return trace_occluded2_s(S, o, d, tmin, tmax, stack);
+    return h;
+}
+template <bool Q, class Top = TopNone>
+RRH_D bool trace_occluded4sl_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, const Top& top = Top()) {
+    if (!(tmax >= tmin)) return false;
+    WRay r = wray_setup(o, d);
+    Trail tr;
+    int cur = 0, node = 0, leaf = 0;
+    while (node != kTravDone) {
+        while ((uint32_t)node < (uint32_t)kTravDone) {
+            Node4 N;
+            load_node4<Q>(S, top, node, N);
+            int first = -1;
+            uint32_t lv = 0, left = 0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float tn;
+                const int l = i4c(N.lk, i);
+                if ((l != 0) && box_hit(f4c(N.xl, i), f4c(N.xh, i), f4c(N.yl, i), f4c(N.yh, i), f4c(N.zl, i),
+                                        f4c(N.zh, i), r, tmin, tmax, tn)) {
+                    if (first < 0) {
+                        first = i;
+                    } else {
+                        lv |= (uint32_t)i << (2 + 2 * left);
+                        ++left;
+                    }
+                }
+            }
+            if (first < 0) {
+                node = trail_next<Q>(S, top, cur, tr);
+            } else {
+                tr.push(lv | left);
+                cur = node;
+                node = i4c(N.lk, first);
+            }
+            if (node < 0 && leaf >= 0) {
+                leaf = node;
+                node = trail_next<Q>(S, top, cur, tr);
+            }
+            if (!__any(leaf >= 0)) break;
+        }
+        while (leaf < 0) {
+            const uint32_t first = RRH_LEAF_FIRST(leaf), cnt = RRH_LEAF_COUNT(leaf);
+            for (uint32_t i = 0; i < cnt; ++i) {
+                const float4* tv = S.tris + (size_t)(first + i) * 3;
+                float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
+                float tt, b1, b2;
+                if (wt_hit(r, xyz(p0), xyz(p1), xyz(p2), tmin, tmax, tt, b1, b2)) return true;
+            }
+            leaf = node;
+            if (node < 0) node = trail_next<Q>(S, top, cur, tr);
+        }
+    }
+    return false;
+}
+
+// the scene's node format (uniform across a launch) picks the traversal; a StackNone stack
+// selects the stackless one (the caller has checked that the tree is 4-wide and shallow
+// enough for it)
+template <bool COUNT, class Stk, class Top = TopNone>
+RRH_D Hit trace_closest_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack,
+                          uint32_t* n_nodes = nullptr, uint32_t* n_tris = nullptr, const Top& top = Top()) {
+    if constexpr (std::is_same<Stk, StackNone>::value) {
+        if (S.node_fmt == RRH_NODES_WIDE4Q)
+            return trace_closest4sl_s<COUNT, true>(S, o, d, tmin, tmax, n_nodes, n_tris, false, top);
+        return trace_closest4sl_s<COUNT, false>(S, o, d, tmin, tmax, n_nodes, n_tris);
+    } else {
+        if (S.node_fmt == RRH_NODES_WIDE4Q) {
+            if (S.speculative)
+                return trace_closest4spec_s<COUNT, true>(S, o, d, tmin, tmax, stack, n_nodes, n_tris, false, top);
+            return trace_closest4_s<COUNT, true>(S, o, d, tmin, tmax, stack, n_nodes, n_tris);
+        }
+        if (S.node_fmt == RRH_NODES_WIDE4) return trace_closest4_s<COUNT, false>(S, o, d, tmin, tmax, stack, n_nodes, n_tris);
+        if (S.speculative) return trace_closest2spec_s<COUNT>(S, o, d, tmin, tmax, stack, n_nodes, n_tris);
+        return trace_closest2_s<COUNT>(S, o, d, tmin, tmax, stack, n_nodes, n_tris);
+    }
+}
+template <class Stk, class Top = TopNone>
+RRH_D bool trace_occluded_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack, const Top& top = Top()) {
+    if constexpr (std::is_same<Stk, StackNone>::value) {
+        if (S.node_fmt == RRH_NODES_WIDE4Q) return trace_occluded4sl_s<true>(S, o, d, tmin, tmax, top);
+        return trace_occluded4sl_s<false>(S, o, d, tmin, tmax);
+    } else {
+        if (S.node_fmt == RRH_NODES_WIDE4Q) {
+            if (S.speculative) return trace_occluded4spec_s<true>(S, o, d, tmin, tmax, stack, top);
+            return trace_occluded4_s<true>(S, o, d, tmin, tmax, stack);
+        }
+        if (S.node_fmt == RRH_NODES_WIDE4) return trace_occluded4_s<false>(S, o, d, tmin, tmax, stack);
+        if (S.speculative) return trace_occluded2spec_s(S, o, d, tmin, tmax, stack);
+        return trace_occluded2_s(S, o, d, tmin, tmax, stack);
+    }
 }
 
 // private-stack wrappers (any depth)

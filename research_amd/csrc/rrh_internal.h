@@ -52,6 +52,9 @@ struct BvhOut {
     uint32_t depth = 0, leaves = 0;
     uint32_t depth4 = 0, stack4 = 0;  // 4-wide: depth and the traversal stack entries it can need
 };
+// the host builder numbers the first kTopBfsNodes 4-wide nodes breadth-first (the device
+// builder numbers all of them so): any prefix of that many nodes is the top of the tree
+constexpr uint32_t kTopBfsNodes = 1024;
 // tri_v: ntri x 9 floats (world space).  Returns false with a message on failure.
 // max_leaf: largest leaf (1..8); trav_cost: SAH cost of a traversal step relative to a triangle test
 bool build_bvh(const float* tri_v, uint32_t ntri, BvhOut& out, std::string& err, uint32_t max_leaf = 8,
@@ -66,6 +69,9 @@ int build_bvh_device(rrh_ctx* c, rrh_scene* s, const float* tw, uint32_t max_lea
 int flatten_device(rrh_ctx* c, rrh_scene* s);
 // every box of the three node formats recomputed bottom-up from s->twd, topology kept
 int refit_bvh_device(rrh_ctx* c, rrh_scene* s);
+// after a build: the parent of every 4-wide node (s->parent4) and the breadth-first prefix
+// length s->top4 = min(top4, node count)
+int tree_links(rrh_ctx* c, rrh_scene* s, uint32_t top4);
 // this device's address of rrh_pt_render_noalloc's scratch array (device must be current)
 int pt_scratch_init(int device, void** out);
 
@@ -120,6 +126,7 @@ struct rrh_ctx {
     rrh::DevBuf fetch;            // uint64 x 3: node fetches, tri fetches, rays (instrumented traversal)
     bool profiling = false;
     int stack_kind = RRH_STACK_LDS32;
+    uint32_t top_nodes = 0;        // RRH_OPT_TRAV_TOP_NODES: top-of-tree nodes staged in LDS
     bool xcd_remap = false;
     bool restir_cull = false;      // RRH_OPT_RESTIR_CULL: ReSTIR PT / GI rays cull back faces
     int bvh_nodes = RRH_NODES_WIDE4Q;  // RRH_OPT_BVH_NODES (quantized 4-wide: fastest measured)
@@ -134,6 +141,8 @@ struct rrh_scene {
     rrh::DevBuf nodes, nodes4, nodesq, tris, tri_ref, tri_light, tl, verts, idx, meshes, mats, insts, lights, cam;
     uint32_t ntri = 0, nnodes = 0, nl = 0, ntl = 0, flags = 0, depth = 0, leaves = 0;
     uint32_t nnodes4 = 0, depth4 = 0, stack4 = 0;
+    rrh::DevBuf parent4;          // parent of each 4-wide node (the stackless traversal climbs them)
+    uint32_t top4 = 0;             // leading 4-wide nodes in breadth-first order (see kTopBfsNodes)
     int node_format() const;       // the format the context traverses (RRH_NODES_*)
     uint32_t stack_need() const;   // traversal stack entries the traversed tree can need
     float build_ms = 0.0f;
@@ -175,8 +184,11 @@ size_t format_bytes(int fmt);
 // (need = entries the traversal can push: the binary depth, or the 4-wide bound); a tree
 // that may outgrow 32 LDS entries gets 16 in LDS plus a private overflow (measured faster
 // than 32 + overflow: the smaller LDS footprint keeps more waves resident)
+// (RRH_STACK_NONE: only the path tracer's queue kernels run stackless; the rest get the
+// LDS stack that kind would give)
 inline int stack_for(const rrh_ctx* c, uint32_t need) {
-    if (c->stack_kind == RRH_STACK_LDS32 && need > 32) return RRH_STACK_HYBRID16;
+    if ((c->stack_kind == RRH_STACK_LDS32 || c->stack_kind == RRH_STACK_NONE) && need > 32) return RRH_STACK_HYBRID16;
+    if (c->stack_kind == RRH_STACK_NONE) return RRH_STACK_LDS32;
     return c->stack_kind;
 }
 }

@@ -93,12 +93,17 @@ __device__ __forceinline__ uint32_t block_push(uint32_t* counter, bool pred, uin
     return slot;
 }
 
+// the top of the tree staged in LDS (TOP): dynamic shared memory of top_n x 64 B
+extern __shared__ uint4 s_top[];
+
 // perm (optional): the queue slots in tracing order (the ray sort); results stay in slot order
-template <bool COUNT, int STK>
+// TOP: the first top_n quantized nodes are read from LDS
+template <bool COUNT, int STK, bool TOP>
 __global__ void __launch_bounds__(kBlock) k_intersect(DevScene S, Q in, const uint32_t* __restrict__ cnt, uint32_t b,
                                                       float4* __restrict__ hits, unsigned long long* __restrict__ fetch,
-                                                      bool remap, const uint32_t* __restrict__ perm) {
+                                                      bool remap, const uint32_t* __restrict__ perm, uint32_t top_n) {
     __shared__ int s_stack[(kBlock / 64) * StackSel<STK>::kLds * 64];
+    const auto top = stage_top<TOP>(S, s_top, top_n);
     const uint32_t n = cnt[b];
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, remap);
     uint32_t nn = 0, nt = 0, nr = 0;
@@ -107,7 +112,7 @@ __global__ void __launch_bounds__(kBlock) k_intersect(DevScene S, Q in, const ui
         const uint32_t i = perm ? perm[j] : j;
         if (i >= n) continue;  // a sorted order holds only live slots; never trust it past the queue
         float4 o = in.o[i], d = in.d[i];
-        Hit h = trace_closest_s<COUNT>(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, 1e9f, st, &nn, &nt);
+        Hit h = trace_closest_s<COUNT>(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, 1e9f, st, &nn, &nt, top);
         hits[i] = make_float4(h.t, __uint_as_float(h.prim), h.b1, h.b2);
         nr++;
     }
@@ -379,18 +384,19 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, Q in, Q out, const
     }
 }
 
-template <int STK>
+template <int STK, bool TOP>
 __global__ void __launch_bounds__(kBlock) k_shadow(DevScene S, const uint32_t* __restrict__ cnt, uint32_t b,
                                                    const float4* __restrict__ sh_o, const float4* __restrict__ sh_d,
                                                    const float4* __restrict__ sh_c, float4* __restrict__ rad,
-                                                   VertexRec rec, bool remap) {
+                                                   VertexRec rec, bool remap, uint32_t top_n) {
     __shared__ int s_stack[(kBlock / 64) * StackSel<STK>::kLds * 64];
+    const auto top = stage_top<TOP>(S, s_top, top_n);
     const uint32_t n = cnt[64 + b];
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, remap);
     auto st = make_stack<STK>(s_stack);
     for (uint32_t i = blk * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float4 o = sh_o[i], d = sh_d[i];
-        if (!trace_occluded_s(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, o.w, st)) {
+        if (!trace_occluded_s(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, o.w, st, top)) {
             uint32_t pixel = __float_as_uint(d.w);
             float4 c = sh_c[i];
             float4 L = rad[pixel];
@@ -445,8 +451,9 @@ uint32_t resident_blocks(rrh_ctx* c, K kernel) {
 
 template <int STK>
 int launch_trace(rrh_ctx* c, const DevScene& S, Q& qi, uint32_t* cnt, uint32_t b, uint32_t g, bool count,
-                 const uint32_t* perm, float4* hp) {
+                 const uint32_t* perm, float4* hp, uint32_t top_n) {
     unsigned long long* fc = (unsigned long long*)c->fetch.p;
+    if constexpr (STK != RRH_STACK_NONE) {  // the stackless traversal is 4-wide only
     if (c->persistent && S.node_fmt == RRH_NODES_BINARY) {
         uint32_t* next = cnt + 128 + b;
         if (count) {
@@ -459,18 +466,27 @@ int launch_trace(rrh_ctx* c, const DevScene& S, Q& qi, uint32_t* cnt, uint32_t b
         RRH_LAUNCH_CHECK();
         return RRH_OK;
     }
-    if (count)
-        hipLaunchKernelGGL((k_intersect<true, STK>), dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, fc, c->xcd_remap,
-                           perm);
+    }
+    const size_t lds = (size_t)top_n * 4 * sizeof(uint4);
+    if (count && top_n)
+        hipLaunchKernelGGL((k_intersect<true, STK, true>), dim3(g), dim3(kBlock), lds, c->stream, S, qi, cnt, b, hp, fc,
+                           c->xcd_remap, perm, top_n);
+    else if (count)
+        hipLaunchKernelGGL((k_intersect<true, STK, false>), dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, fc,
+                           c->xcd_remap, perm, 0u);
+    else if (top_n)
+        hipLaunchKernelGGL((k_intersect<false, STK, true>), dim3(g), dim3(kBlock), lds, c->stream, S, qi, cnt, b, hp, fc,
+                           c->xcd_remap, perm, top_n);
     else
-        hipLaunchKernelGGL((k_intersect<false, STK>), dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, fc, c->xcd_remap,
-                           perm);
+        hipLaunchKernelGGL((k_intersect<false, STK, false>), dim3(g), dim3(kBlock), 0, c->stream, S, qi, cnt, b, hp, fc,
+                           c->xcd_remap, perm, 0u);
     RRH_LAUNCH_CHECK();
     return RRH_OK;
 }
 template <int STK>
 int launch_shadow(rrh_ctx* c, const DevScene& S, uint32_t* cnt, uint32_t b, uint32_t g, float4* rad, const VertexRec& rec,
-                  const Bufs& B) {
+                  const Bufs& B, uint32_t top_n) {
+    if constexpr (STK != RRH_STACK_NONE) {
     if (c->persistent && S.node_fmt == RRH_NODES_BINARY) {
         const uint32_t gp = std::min(g, resident_blocks(c, k_shadow_p<STK>));
         hipLaunchKernelGGL(k_shadow_p<STK>, dim3(gp), dim3(kBlock), 0, c->stream, S, (const uint32_t*)cnt, b,
@@ -479,9 +495,15 @@ int launch_shadow(rrh_ctx* c, const DevScene& S, uint32_t* cnt, uint32_t b, uint
         RRH_LAUNCH_CHECK();
         return RRH_OK;
     }
-    hipLaunchKernelGGL(k_shadow<STK>, dim3(g), dim3(kBlock), 0, c->stream, S, (const uint32_t*)cnt, b,
-                       (const float4*)B.sh_o, (const float4*)B.sh_d, (const float4*)B.sh_c, rad, rec,
-                       c->xcd_remap);
+    }
+    if (top_n)
+        hipLaunchKernelGGL((k_shadow<STK, true>), dim3(g), dim3(kBlock), (size_t)top_n * 4 * sizeof(uint4), c->stream, S,
+                           (const uint32_t*)cnt, b, (const float4*)B.sh_o, (const float4*)B.sh_d, (const float4*)B.sh_c,
+                           rad, rec, c->xcd_remap, top_n);
+    else
+        hipLaunchKernelGGL((k_shadow<STK, false>), dim3(g), dim3(kBlock), 0, c->stream, S, (const uint32_t*)cnt, b,
+                           (const float4*)B.sh_o, (const float4*)B.sh_d, (const float4*)B.sh_c, rad, rec,
+                           c->xcd_remap, 0u);
     RRH_LAUNCH_CHECK();
     return RRH_OK;
 }
@@ -510,6 +532,14 @@ __global__ void k_accumulate_f64(const float4* __restrict__ rad, double* __restr
     } else {
         a[0] += (double)r.x; a[1] += (double)r.y; a[2] += (double)r.z; a[3] += (double)r.w;
     }
+}
+
+// stack kind of the ray-queue kernels: RRH_STACK_NONE runs stackless on a 4-wide tree of at
+// most 31 levels (the trail's reach), other kinds as stack_for gives them
+int queue_stack(const rrh_ctx* c, const rrh_scene* s) {
+    const int fmt = s->node_format();
+    if (c->stack_kind == RRH_STACK_NONE && fmt != RRH_NODES_BINARY && s->depth4 <= 31) return RRH_STACK_NONE;
+    return stack_for(c, s->stack_need());
 }
 
 uint32_t grid_for(uint32_t n) {
@@ -588,6 +618,8 @@ static int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, 
                        p->frame_seed);
     RRH_LAUNCH_CHECK();
     uint32_t g = grid_for(npix);
+    const int sk = queue_stack(c, s);
+    const uint32_t top_n = (s->node_format() == RRH_NODES_WIDE4Q) ? std::min(c->top_nodes, s->top4) : 0u;
     for (uint32_t b = 0; b < p->max_bounces; ++b) {
         Q& qi = q[b & 1];
         Q& qo = q[(b + 1) & 1];
@@ -620,12 +652,12 @@ static int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, 
             }
             RRH_HIP(hipEventRecord(c->prof_events[c->prof_used], c->stream));
         }
-        const int sk = stack_for(c, s->stack_need());
         const bool cf = (p->flags & RRH_PT_COUNT_FETCHES) != 0;
-        rc = (sk == RRH_STACK_LDS32) ? launch_trace<RRH_STACK_LDS32>(c, S, qi, cnt, b, g, cf, perm, Bf.hits)
-           : (sk == RRH_STACK_HYBRID16) ? launch_trace<RRH_STACK_HYBRID16>(c, S, qi, cnt, b, g, cf, perm, Bf.hits)
-           : (sk == RRH_STACK_HYBRID32) ? launch_trace<RRH_STACK_HYBRID32>(c, S, qi, cnt, b, g, cf, perm, Bf.hits)
-           : launch_trace<RRH_STACK_PRIVATE>(c, S, qi, cnt, b, g, cf, perm, Bf.hits);
+        rc = (sk == RRH_STACK_LDS32) ? launch_trace<RRH_STACK_LDS32>(c, S, qi, cnt, b, g, cf, perm, Bf.hits, top_n)
+           : (sk == RRH_STACK_HYBRID16) ? launch_trace<RRH_STACK_HYBRID16>(c, S, qi, cnt, b, g, cf, perm, Bf.hits, top_n)
+           : (sk == RRH_STACK_HYBRID32) ? launch_trace<RRH_STACK_HYBRID32>(c, S, qi, cnt, b, g, cf, perm, Bf.hits, top_n)
+           : (sk == RRH_STACK_NONE) ? launch_trace<RRH_STACK_NONE>(c, S, qi, cnt, b, g, cf, perm, Bf.hits, top_n)
+           : launch_trace<RRH_STACK_PRIVATE>(c, S, qi, cnt, b, g, cf, perm, Bf.hits, top_n);
         if (rc) return rc;
         RRH_LAUNCH_CHECK();
         if (prof) {
@@ -645,10 +677,11 @@ static int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, 
         hipLaunchKernelGGL(k_shade, dim3(g), dim3(kBlock), 0, c->stream, S, qi, qo, (const float4*)Bf.hits, cnt,
                            Bf.sh_o, Bf.sh_d, Bf.sh_c, rad, Ab);
         RRH_LAUNCH_CHECK();
-        rc = (sk == RRH_STACK_LDS32) ? launch_shadow<RRH_STACK_LDS32>(c, S, cnt, b, g, rad, A.rec, Bf)
-           : (sk == RRH_STACK_HYBRID16) ? launch_shadow<RRH_STACK_HYBRID16>(c, S, cnt, b, g, rad, A.rec, Bf)
-           : (sk == RRH_STACK_HYBRID32) ? launch_shadow<RRH_STACK_HYBRID32>(c, S, cnt, b, g, rad, A.rec, Bf)
-           : launch_shadow<RRH_STACK_PRIVATE>(c, S, cnt, b, g, rad, A.rec, Bf);
+        rc = (sk == RRH_STACK_LDS32) ? launch_shadow<RRH_STACK_LDS32>(c, S, cnt, b, g, rad, A.rec, Bf, top_n)
+           : (sk == RRH_STACK_HYBRID16) ? launch_shadow<RRH_STACK_HYBRID16>(c, S, cnt, b, g, rad, A.rec, Bf, top_n)
+           : (sk == RRH_STACK_HYBRID32) ? launch_shadow<RRH_STACK_HYBRID32>(c, S, cnt, b, g, rad, A.rec, Bf, top_n)
+           : (sk == RRH_STACK_NONE) ? launch_shadow<RRH_STACK_NONE>(c, S, cnt, b, g, rad, A.rec, Bf, top_n)
+           : launch_shadow<RRH_STACK_PRIVATE>(c, S, cnt, b, g, rad, A.rec, Bf, top_n);
         if (rc) return rc;
     }
     hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(64), 0, c->stream, (const uint32_t*)cnt, p->max_bounces,

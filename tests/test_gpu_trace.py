@@ -80,7 +80,7 @@ def test_trace_full_size_heightfield_primary_rays(ctx):
                                                    (L.GEN_INDOOR, 1, 1.0, [0.1, 0.1, 0.1], [9.9, 3.9, 9.9])])
 def test_node_formats_agree(ctx, kind, seed, scale, lo, hi):
     """The three node formats of the same tree (binary, also with speculative traversal; 4-wide
-    fp32; 4-wide quantized) give
+    fp32; 4-wide quantized), the stackless traversal and the top of the tree read from LDS give
     identical closest hits and occlusion on the full-size scenes; the 4-wide tree keeps its
     stack bound."""
     sa = core.generate_scene(kind, seed, scale)
@@ -88,18 +88,30 @@ def test_node_formats_agree(ctx, kind, seed, scale, lo, hi):
     info = gs.info()
     print(kind, {k: info[k] for k in ("tri_count", "node_count", "bvh_depth", "node4_count", "stack4")})
     assert 0 < info["node4_count"] < info["node_count"] and info["stack4"] <= 64
+    assert info["bvh_depth4"] <= 31     # the stackless traversal's reach
     rays = random_rays(200000, lo, hi, 9)
     res = {}
-    combos = [(L.NODES_BINARY, 0), (L.NODES_BINARY, 1), (L.NODES_WIDE4, 0), (L.NODES_WIDE4Q, 0), (L.NODES_WIDE4Q, 1)]
+    P = L.STACK_PRIVATE
+    # (node format, speculative, stack kind, top-of-tree nodes in LDS); the stackless traversal
+    # and the LDS top only apply to 4-wide trees (the top to quantized nodes)
+    combos = [(L.NODES_BINARY, 0, P, 0), (L.NODES_BINARY, 1, P, 0), (L.NODES_WIDE4, 0, P, 0),
+              (L.NODES_WIDE4Q, 0, P, 0), (L.NODES_WIDE4Q, 1, P, 0), (L.NODES_WIDE4Q, 1, P, 85),
+              (L.NODES_WIDE4Q, 1, P, 512), (L.NODES_WIDE4, 1, L.STACK_NONE, 0),
+              (L.NODES_WIDE4Q, 1, L.STACK_NONE, 0), (L.NODES_WIDE4Q, 1, L.STACK_NONE, 21),
+              (L.NODES_WIDE4Q, 1, L.STACK_NONE, 512)]
     try:
-        for fmt, spec in combos:
+        for fmt, spec, stack, top in combos:
             L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_NODES, fmt)
             L.call("rrh_ctx_set_option", ctx.h, L.OPT_SPECULATIVE, spec)
-            res[(fmt, spec)] = core.trace_closest(ctx, gs, rays), core.trace_occluded(ctx, gs, rays)
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAVERSAL_STACK, stack)
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAV_TOP_NODES, top)
+            res[(fmt, spec, stack, top)] = core.trace_closest(ctx, gs, rays), core.trace_occluded(ctx, gs, rays)
     finally:  # back to the defaults
         L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_NODES, L.NODES_WIDE4Q)
         L.call("rrh_ctx_set_option", ctx.h, L.OPT_SPECULATIVE, 1)
-    g0, o0 = res[(L.NODES_BINARY, 0)]
+        L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAVERSAL_STACK, L.STACK_LDS32)
+        L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAV_TOP_NODES, 0)
+    g0, o0 = res[combos[0]]
     for key in combos[1:]:
         assert np.array_equal(res[key][0].view(np.uint32), g0.view(np.uint32)), key
         assert np.array_equal(res[key][1], o0), key
