@@ -270,7 +270,7 @@ def restir_gi_summary(ctx: core.Context, scene, sa, W: int, H: int, frames: int,
 
 
 # the options a traffic probe must share with the timed run (same kernels, same tree)
-PROBE_OPTS = ["stack", "top_nodes", "xcd_remap", "persistent", "max_leaf", "trav_cost", "speculative", "bvh_builder",
+PROBE_OPTS = ["stack", "top_nodes", "trace_pairs", "xcd_remap", "persistent", "max_leaf", "trav_cost", "speculative", "bvh_builder",
               "ploc_radius", "ray_sort", "bvh_nodes", "width", "height", "max_bounces"]
 COPY_BYTES = 1 << 30
 
@@ -280,6 +280,7 @@ def configure(ctx, a) -> None:
            {"private": L.STACK_PRIVATE, "lds32": L.STACK_LDS32, "hybrid16": L.STACK_HYBRID16,
             "hybrid32": L.STACK_HYBRID32, "none": L.STACK_NONE}[a.stack])
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAV_TOP_NODES, a.top_nodes)
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRACE_PAIRS, a.trace_pairs)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_XCD_REMAP, a.xcd_remap)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_NODES, NODE_FORMATS[a.bvh_nodes])
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_PERSISTENT, a.persistent)
@@ -363,7 +364,9 @@ def measure_traffic(a) -> dict:
         short = re.sub(r"\(anonymous namespace\)::|rrh::|^void\s+", "", name).split("(")[0]
         f_, w_ = _mean_of(per["FETCH_SIZE"], re.escape(name)), _mean_of(per["WRITE_SIZE"], re.escape(name))
         by_kernel[short] = round((f_ * fs + w_ * ws) * 1024.0 / 1e6, 1)
-    fi, wi = _mean_of(per["FETCH_SIZE"], r"k_intersect<false"), _mean_of(per["WRITE_SIZE"], r"k_intersect<false")
+    # per launch of the roofline's kernels: every traversal launch of the path tracer
+    group = r"k_intersect<false|k_trace_pair<|k_shadow<"
+    fi, wi = _mean_of(per["FETCH_SIZE"], group), _mean_of(per["WRITE_SIZE"], group)
     return {"bytes": int((fi * fs + wi * ws) * 1024.0) if fi is not None and wi is not None else None,
             "fetch_scale": round(fs, 4), "write_scale": round(ws, 4), "mb_per_dispatch": by_kernel,
             "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes of this run over 4 frames (KiB per "
@@ -382,6 +385,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stack", choices=["private", "lds32", "hybrid16", "hybrid32", "none"], default="lds32",
                     help="where the traversal stack lives (context option; none: stackless)")
+    ap.add_argument("--trace-pairs", type=int, default=1,
+                    help="1: extension rays of bounce b traced in one launch with the shadow rays of b - 1")
     ap.add_argument("--top-nodes", type=int, default=0,
                     help="top-of-tree quantized nodes each workgroup reads from LDS (0..512)")
     ap.add_argument("--xcd-remap", type=int, default=0, help="1: runs of 16 neighbouring workgroups per XCD")
@@ -447,10 +452,18 @@ def main() -> int:
     nn, nt, nr = C.c_uint64(), C.c_uint64(), C.c_uint64()
     L.call("rrh_fetch_counts", ctx.h, C.byref(nn), C.byref(nt), C.byref(nr), 1)
     bytes_per_ray = (nn.value * NODE_BYTES[a.bvh_nodes] + nt.value * TRI_BYTES) / max(nr.value, 1) + RAY_BYTES + HIT_BYTES
+    sn, st, sr = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    L.call("rrh_fetch_counts_shadow", ctx.h, C.byref(sn), C.byref(st), C.byref(sr), 1)
+    # a shadow ray reads its origin and direction records; its contribution record and pixel are
+    # touched only when it is visible, and are left out (a lower bound)
+    bytes_per_shadow = ((sn.value * NODE_BYTES[a.bvh_nodes] + st.value * TRI_BYTES) / sr.value + RAY_BYTES
+                        if sr.value else None)
     print(f"[bench] traversal: {nn.value / max(nr.value, 1):.1f} nodes + {nt.value / max(nr.value, 1):.1f} tris "
-          f"per ray -> {bytes_per_ray:.0f} algorithmic B/ray", file=sys.stderr, flush=True)
+          f"per ray -> {bytes_per_ray:.0f} algorithmic B/ray; shadow rays "
+          f"{sn.value / max(sr.value, 1):.1f} nodes + {st.value / max(sr.value, 1):.1f} tris", file=sys.stderr, flush=True)
     ctx.synchronize()
     ray_totals(ctx, reset=True)
+    L.call("rrh_pt_traversal_rays", ctx.h, C.byref(C.c_uint64()), C.byref(C.c_uint64()), 1)
     L.call("rrh_profile_enable", ctx.h, 1)
     L.call("rrh_profile_traversal", ctx.h, C.byref(C.c_float()), C.byref(C.c_uint32()), 1)
     ctx.synchronize()
@@ -463,12 +476,17 @@ def main() -> int:
     prim, ext, shadow = ray_totals(ctx, reset=True)
     trav_ms, launches = C.c_float(), C.c_uint32()
     L.call("rrh_profile_traversal", ctx.h, C.byref(trav_ms), C.byref(launches), 1)
-    # closest-hit rays of the path tracer (DI primary rays use the same traversal inside cs_initial)
-    pt_trace_rays = prim - a.steps * W * H + ext
+    # the path tracer's own closest-hit and shadow rays (the ReSTIR DI pass traces its primary
+    # and shadow rays inside its own kernels, which the traversal events do not cover)
+    pt_closest, pt_shadow = C.c_uint64(), C.c_uint64()
+    L.call("rrh_pt_traversal_rays", ctx.h, C.byref(pt_closest), C.byref(pt_shadow), 1)
     rays = prim + ext + shadow
     value = rays / dt / 1e6
     avg_launch_ms = trav_ms.value / max(launches.value, 1)
-    bytes_per_launch = bytes_per_ray * pt_trace_rays / max(launches.value, 1)
+    # every timed traversal launch: closest hit (bounce 0), paired closest hit + shadow, shadow
+    # (the last bounce); algorithmic bytes of all their rays over all their time
+    trav_bytes = bytes_per_ray * pt_closest.value + (bytes_per_shadow or 0.0) * pt_shadow.value
+    bytes_per_launch = trav_bytes / max(launches.value, 1)
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
     out = {
         "metric": METRIC,
@@ -487,15 +505,22 @@ def main() -> int:
                    "width": W, "height": H, "spp_per_frame": 1, "max_bounces": a.max_bounces,
                    "di_candidates": 32, "di_temporal_mcap": "20*M", "di_spatial": "5 neighbours, r=30",
                    "camera": "orbit 0.5 deg/frame", "parallelism": "single GPU",
-                   "traversal_stack": a.stack, "top_nodes_in_lds": a.top_nodes, "xcd_remap": a.xcd_remap, "bvh_nodes": a.bvh_nodes, "persistent": a.persistent, "speculative": a.speculative,
+                   "traversal_stack": a.stack, "top_nodes_in_lds": a.top_nodes, "trace_pairs": a.trace_pairs, "xcd_remap": a.xcd_remap, "bvh_nodes": a.bvh_nodes, "persistent": a.persistent, "speculative": a.speculative,
                    "bvh_max_leaf": a.max_leaf, "bvh_trav_cost": a.trav_cost, "bvh_builder": a.bvh_builder,
                    "bvh_build_ms": round(info["build_ms"], 1), "ray_sort_bits": a.ray_sort,
                    "rays_per_step": {"primary": prim // a.steps, "extension": ext // a.steps,
                                      "shadow": shadow // a.steps}},
-        "roofline": {"bound": "hbm", "kernel": "k_intersect (closest-hit BVH traversal)",
+        "roofline": {"bound": "hbm",
+                     "kernel": ("path tracer BVH traversal launches: k_intersect (bounce 0), k_trace_pair (extension "
+                                "rays of bounce b with the shadow rays of b-1), k_shadow (last bounce)"
+                                if a.trace_pairs else "path tracer BVH traversal launches: k_intersect, k_shadow"),
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_ray": round(bytes_per_ray, 1), "avg_launch_ms": round(avg_launch_ms, 4),
+                     "bytes_per_ray": round(bytes_per_ray, 1),
+                     "bytes_per_shadow_ray": round(bytes_per_shadow, 1) if bytes_per_shadow else None,
+                     "rays_per_launch": {"closest": pt_closest.value // max(launches.value, 1),
+                                         "shadow": pt_shadow.value // max(launches.value, 1)},
+                     "avg_launch_ms": round(avg_launch_ms, 4),
                      "launches": launches.value, "measured_copy_gbs": round(hbm.value, 1)},
         "cpu_baseline": None,
     }

@@ -241,8 +241,15 @@ int rrh_ray_totals(rrh_ctx* ctx, rrh_ray_counts* out, int reset);
 /* node (64 B) and triangle (48 B) fetches of closest-hit traversal counted by renders
  * with RRH_PT_COUNT_FETCHES since the last reset, and the rays they cover */
 int rrh_fetch_counts(rrh_ctx* ctx, uint64_t* node_fetches, uint64_t* tri_fetches, uint64_t* rays, int reset);
-/* per-launch GPU time of the closest-hit traversal kernel inside rrh_pt_render, taken
- * with HIP events on the context stream while profiling is on */
+/* the same for the path tracer's shadow rays (occlusion traversal; counted for quantized 4-wide
+ * trees with the speculative or the stackless traversal) */
+int rrh_fetch_counts_shadow(rrh_ctx* ctx, uint64_t* node_fetches, uint64_t* tri_fetches, uint64_t* rays, int reset);
+/* closest-hit and shadow rays traced by rrh_pt_render* on this context since the last reset
+ * (the path tracer's own rays: rrh_ray_totals also counts the ReSTIR DI pass) */
+int rrh_pt_traversal_rays(rrh_ctx* ctx, uint64_t* closest, uint64_t* shadow, int reset);
+/* GPU time of the path tracer's traversal launches (closest hit, paired closest hit + shadow,
+ * shadow) inside rrh_pt_render, taken with HIP events on the context stream while profiling
+ * is on */
 /* tuning options of a context (no reference counterpart) */
 #define RRH_OPT_TRAVERSAL_STACK 1   /* RRH_STACK_* : where the BVH traversal stack lives */
 #define RRH_OPT_XCD_REMAP 2         /* 1: give each XCD runs of 16 consecutive workgroups (rotating) */
@@ -266,6 +273,9 @@ int rrh_fetch_counts(rrh_ctx* ctx, uint64_t* node_fetches, uint64_t* tri_fetches
                                        first n quantized 4-wide nodes (the top of the tree: the nodes are
                                        numbered breadth-first there) into LDS per workgroup and read
                                        them from there */
+#define RRH_OPT_TRACE_PAIRS 13      /* 1 (default): the path tracer traces each bounce's extension rays and the
+                                       previous bounce's shadow rays in one launch (same results; not with
+                                       RRH_PT_COUNT_FETCHES or RRH_OPT_TRAV_TOP_NODES); 0: one launch each */
 #define RRH_NODES_BINARY 0          /* binary, 64 B (both child boxes in fp32) */
 #define RRH_NODES_WIDE4 1           /* 4-wide, 128 B (fp32 child boxes) */
 #define RRH_NODES_WIDE4Q 2          /* 4-wide, 64 B (8-bit child boxes, outward rounded); default */

@@ -39,7 +39,8 @@ PT_COUNT_FETCHES = 1
 OPT_TRAVERSAL_STACK, OPT_XCD_REMAP, OPT_BVH_NODES, OPT_PERSISTENT, OPT_SPECULATIVE = 1, 2, 3, 4, 5
 OPT_BVH_MAX_LEAF, OPT_BVH_TRAV_COST = 6, 7
 OPT_BVH_BUILDER, OPT_PLOC_RADIUS, OPT_RAY_SORT, OPT_RESTIR_CULL = 8, 9, 10, 11
-OPT_TRAV_TOP_NODES = 12
+OPT_TRAV_TOP_NODES, OPT_TRACE_PAIRS = 12, 13
+TRACE_PAIRS_DEFAULT = 1  # a new context's RRH_OPT_TRACE_PAIRS
 BVH_BUILD_HOST, BVH_BUILD_DEVICE = 0, 1
 UPDATE_REFIT, UPDATE_REBUILD = 0, 1
 NODES_BINARY, NODES_WIDE4, NODES_WIDE4Q = 0, 1, 2
@@ -165,6 +166,8 @@ SIGNATURES = {
     "rrh_pt_ray_counts": [P, C.POINTER(RayCounts)],
     "rrh_ray_totals": [P, C.POINTER(RayCounts), i32],
     "rrh_fetch_counts": [P, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), i32],
+    "rrh_fetch_counts_shadow": [P, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), i32],
+    "rrh_pt_traversal_rays": [P, C.POINTER(u64), C.POINTER(u64), i32],
     "rrh_profile_enable": [P, i32],
     "rrh_stream_copy": [P, sz, i32, C.POINTER(f32)],
     "rrh_ctx_set_option": [P, i32, i32],

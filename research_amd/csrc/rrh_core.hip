@@ -98,12 +98,14 @@ int rrh_ctx_create(int device, rrh_ctx** out) {
         if (e != hipSuccess) { set_error(hipGetErrorString(e)); return RRH_E_HIP; }
     }
     // device-side ray totals and fetch counters exist from the start, so no frame allocates them
-    int rc = c->totals.ensure(32);
-    if (!rc) rc = c->fetch.ensure(32);
+    // (totals: primary, extension, shadow of every pass, then the path tracer's closest-hit and
+    // shadow rays; fetch: closest-hit nodes, triangles, rays, then the same for shadow rays)
+    int rc = c->totals.ensure(64);
+    if (!rc) rc = c->fetch.ensure(64);
     if (!rc) rc = pt_scratch_init(device, &c->noalloc_scratch);
     if (rc) { rrh_ctx_destroy(c); return rc; }
-    e = hipMemsetAsync(c->totals.p, 0, 32, c->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(c->fetch.p, 0, 32, c->stream);
+    e = hipMemsetAsync(c->totals.p, 0, 64, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->fetch.p, 0, 64, c->stream);
     if (e != hipSuccess) { set_error(hipGetErrorString(e)); rrh_ctx_destroy(c); return RRH_E_HIP; }
     *out = c;
     return RRH_OK;
@@ -155,6 +157,7 @@ int rrh_ctx_set_option(rrh_ctx* c, int option, int value) {
     }
     if (option == RRH_OPT_XCD_REMAP) { c->xcd_remap = value != 0; return RRH_OK; }
     if (option == RRH_OPT_RESTIR_CULL) { c->restir_cull = value != 0; return RRH_OK; }
+    if (option == RRH_OPT_TRACE_PAIRS) { c->trace_pairs = value != 0; return RRH_OK; }
     if (option == RRH_OPT_PERSISTENT) { c->persistent = value != 0; return RRH_OK; }
     if (option == RRH_OPT_SPECULATIVE) { c->speculative = value != 0; return RRH_OK; }
     if (option == RRH_OPT_BVH_MAX_LEAF) {

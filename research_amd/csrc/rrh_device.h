@@ -839,14 +839,15 @@ RRH_D Hit trace_closest4spec_s(const DevScene& S, f3 o, f3 d, float tmin, float 
     }
     return h;
 }
-template <bool Q, class Stk, class Top = TopNone>
+template <bool Q, bool COUNT = false, class Stk, class Top = TopNone>
 RRH_D bool trace_occluded4spec_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack,
-                                 const Top& top = Top()) {
+                                 const Top& top = Top(), uint32_t* n_nodes = nullptr, uint32_t* n_tris = nullptr) {
     if (!(tmax >= tmin)) return false;
     WRay r = wray_setup(o, d);
     int sp = 0, node = 0, leaf = 0;
     while (node != kTravDone) {
         while ((uint32_t)node < (uint32_t)kTravDone) {
+            if constexpr (COUNT) (*n_nodes)++;
             Node4 N;
             load_node4<Q>(S, top, node, N);
             int next = kTravDone;
@@ -871,6 +872,7 @@ RRH_D bool trace_occluded4spec_s(const DevScene& S, f3 o, f3 d, float tmin, floa
         }
         while (leaf < 0) {
             const uint32_t first = RRH_LEAF_FIRST(leaf), cnt = RRH_LEAF_COUNT(leaf);
+            if constexpr (COUNT) (*n_tris) += cnt;
             for (uint32_t i = 0; i < cnt; ++i) {
                 const float4* tv = S.tris + (size_t)(first + i) * 3;
                 float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
@@ -993,14 +995,16 @@ RRH_D Hit trace_closest4sl_s(const DevScene& S, f3 o, f3 d, float tmin, float tm
     }
     return h;
 }
-template <bool Q, class Top = TopNone>
-RRH_D bool trace_occluded4sl_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, const Top& top = Top()) {
+template <bool Q, bool COUNT = false, class Top = TopNone>
+RRH_D bool trace_occluded4sl_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, const Top& top = Top(),
+                               uint32_t* n_nodes = nullptr, uint32_t* n_tris = nullptr) {
     if (!(tmax >= tmin)) return false;
     WRay r = wray_setup(o, d);
     Trail tr;
     int cur = 0, node = 0, leaf = 0;
     while (node != kTravDone) {
         while ((uint32_t)node < (uint32_t)kTravDone) {
+            if constexpr (COUNT) (*n_nodes)++;
             Node4 N;
             load_node4<Q>(S, top, node, N);
             int first = -1;
@@ -1034,6 +1038,7 @@ RRH_D bool trace_occluded4sl_s(const DevScene& S, f3 o, f3 d, float tmin, float 
         }
         while (leaf < 0) {
             const uint32_t first = RRH_LEAF_FIRST(leaf), cnt = RRH_LEAF_COUNT(leaf);
+            if constexpr (COUNT) (*n_tris) += cnt;
             for (uint32_t i = 0; i < cnt; ++i) {
                 const float4* tv = S.tris + (size_t)(first + i) * 3;
                 float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
@@ -1068,14 +1073,17 @@ RRH_D Hit trace_closest_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax,
         return trace_closest2_s<COUNT>(S, o, d, tmin, tmax, stack, n_nodes, n_tris);
     }
 }
-template <class Stk, class Top = TopNone>
-RRH_D bool trace_occluded_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack, const Top& top = Top()) {
+// COUNT: node and triangle fetches, counted by the stackless and the speculative quantized
+// traversals (the bench's configuration); the other paths leave the counters alone
+template <bool COUNT = false, class Stk, class Top = TopNone>
+RRH_D bool trace_occluded_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax, Stk& stack, const Top& top = Top(),
+                            uint32_t* n_nodes = nullptr, uint32_t* n_tris = nullptr) {
     if constexpr (std::is_same<Stk, StackNone>::value) {
-        if (S.node_fmt == RRH_NODES_WIDE4Q) return trace_occluded4sl_s<true>(S, o, d, tmin, tmax, top);
-        return trace_occluded4sl_s<false>(S, o, d, tmin, tmax);
+        if (S.node_fmt == RRH_NODES_WIDE4Q) return trace_occluded4sl_s<true, COUNT>(S, o, d, tmin, tmax, top, n_nodes, n_tris);
+        return trace_occluded4sl_s<false, COUNT>(S, o, d, tmin, tmax, TopNone(), n_nodes, n_tris);
     } else {
         if (S.node_fmt == RRH_NODES_WIDE4Q) {
-            if (S.speculative) return trace_occluded4spec_s<true>(S, o, d, tmin, tmax, stack, top);
+            if (S.speculative) return trace_occluded4spec_s<true, COUNT>(S, o, d, tmin, tmax, stack, top, n_nodes, n_tris);
             return trace_occluded4_s<true>(S, o, d, tmin, tmax, stack);
         }
         if (S.node_fmt == RRH_NODES_WIDE4) return trace_occluded4_s<false>(S, o, d, tmin, tmax, stack);

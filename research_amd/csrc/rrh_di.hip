@@ -335,10 +335,6 @@ int rrh_di_spatial_shade(rrh_ctx* c, rrh_scene* s, const rrh_di_params* p, const
                            (const float4*)gpos->p, (const float4*)gnrm->p, (const float4*)res->p,
                            (const uint32_t*)res_uv->p, (float4*)out->p, (uint32_t*)c->di_counter.p, c->xcd_remap);
     RRH_LAUNCH_CHECK();
-    if (!c->totals.p) {
-        if ((rc = c->totals.ensure(32))) return rc;
-        RRH_HIP(hipMemsetAsync(c->totals.p, 0, 32, c->stream));
-    }
     hipLaunchKernelGGL(k_di_totals, dim3(1), dim3(64), 0, c->stream, (const uint32_t*)c->di_counter.p, W * H,
                        (unsigned long long*)c->totals.p);
     RRH_LAUNCH_CHECK();

@@ -127,6 +127,7 @@ struct rrh_ctx {
     bool profiling = false;
     int stack_kind = RRH_STACK_LDS32;
     uint32_t top_nodes = 0;        // RRH_OPT_TRAV_TOP_NODES: top-of-tree nodes staged in LDS
+    bool trace_pairs = true;       // RRH_OPT_TRACE_PAIRS: extension rays of bounce b with shadow rays of b - 1
     bool xcd_remap = false;
     bool restir_cull = false;      // RRH_OPT_RESTIR_CULL: ReSTIR PT / GI rays cull back faces
     int bvh_nodes = RRH_NODES_WIDE4Q;  // RRH_OPT_BVH_NODES (quantized 4-wide: fastest measured)

@@ -102,19 +102,13 @@ __device__ __forceinline__ uint32_t block_push(uint32_t* counter, bool pred, uin
 // the top of the tree staged in LDS (TOP): dynamic shared memory of top_n x 64 B
 extern __shared__ uint4 s_top[];
 
+// closest hits of the extension queue's first n slots, workgroup blk of nblk (one wave each)
 // perm (optional): the queue slots in tracing order (the ray sort); results stay in slot order
-// TOP: the first top_n quantized nodes are read from LDS
-template <bool COUNT, int STK, bool TOP>
-__global__ void __launch_bounds__(kTraceBlock) k_intersect(DevScene S, Q in, const uint32_t* __restrict__ cnt, uint32_t b,
-                                                           float4* __restrict__ hits, unsigned long long* __restrict__ fetch,
-                                                           bool remap, const uint32_t* __restrict__ perm, uint32_t top_n) {
-    __shared__ int s_stack[(kTraceBlock / 64) * StackSel<STK>::kLds * 64];
-    const auto top = stage_top<TOP>(S, s_top, top_n);
-    const uint32_t n = cnt[b];
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, remap);
-    uint32_t nn = 0, nt = 0, nr = 0;
-    auto st = make_stack<STK>(s_stack);
-    for (uint32_t j = blk * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+template <bool COUNT, class Stk, class Top>
+__device__ __forceinline__ void trace_ext_part(const DevScene& S, const Q& in, uint32_t n, uint32_t blk, uint32_t nblk,
+                                               float4* __restrict__ hits, const uint32_t* __restrict__ perm, Stk& st,
+                                               const Top& top, uint32_t& nn, uint32_t& nt, uint32_t& nr) {
+    for (uint32_t j = blk * kTraceBlock + threadIdx.x; j < n; j += nblk * kTraceBlock) {
         const uint32_t i = perm ? perm[j] : j;
         if (i >= n) continue;  // a sorted order holds only live slots; never trust it past the queue
         float4 o = in.o[i], d = in.d[i];
@@ -122,6 +116,38 @@ __global__ void __launch_bounds__(kTraceBlock) k_intersect(DevScene S, Q in, con
         hits[i] = make_float4(h.t, __uint_as_float(h.prim), h.b1, h.b2);
         nr++;
     }
+}
+// shadow rays of bounce b (the queue's first n): a visible one adds its contribution to its pixel
+template <bool COUNT, class Stk, class Top>
+__device__ __forceinline__ void trace_shadow_part(const DevScene& S, uint32_t n, uint32_t b, uint32_t blk, uint32_t nblk,
+                                                  const float4* __restrict__ sh_o, const float4* __restrict__ sh_d,
+                                                  const float4* __restrict__ sh_c, float4* __restrict__ rad,
+                                                  const VertexRec& rec, Stk& st, const Top& top, uint32_t& nn,
+                                                  uint32_t& nt, uint32_t& nr) {
+    for (uint32_t i = blk * kTraceBlock + threadIdx.x; i < n; i += nblk * kTraceBlock) {
+        float4 o = sh_o[i], d = sh_d[i];
+        nr++;
+        if (!trace_occluded_s<COUNT>(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, o.w, st, top, &nn, &nt)) {
+            uint32_t pixel = __float_as_uint(d.w);
+            float4 c = sh_c[i];
+            float4 L = rad[pixel];
+            L.x = L.x + c.x; L.y = L.y + c.y; L.z = L.z + c.z;
+            rad[pixel] = L;
+            if (rec.v) guide_record_nee_visible(rec, pixel, b);
+        }
+    }
+}
+
+// TOP: the first top_n quantized nodes are read from LDS
+template <bool COUNT, int STK, bool TOP>
+__global__ void __launch_bounds__(kTraceBlock) k_intersect(DevScene S, Q in, const uint32_t* __restrict__ cnt, uint32_t b,
+                                                           float4* __restrict__ hits, unsigned long long* __restrict__ fetch,
+                                                           bool remap, const uint32_t* __restrict__ perm, uint32_t top_n) {
+    __shared__ int s_stack[(kTraceBlock / 64) * StackSel<STK>::kLds * 64];
+    const auto top = stage_top<TOP>(S, s_top, top_n);
+    uint32_t nn = 0, nt = 0, nr = 0;
+    auto st = make_stack<STK>(s_stack);
+    trace_ext_part<COUNT>(S, in, cnt[b], xcd_block(blockIdx.x, gridDim.x, remap), gridDim.x, hits, perm, st, top, nn, nt, nr);
     if constexpr (COUNT) {
         atomicAdd(fetch + 0, (unsigned long long)nn);
         atomicAdd(fetch + 1, (unsigned long long)nt);
@@ -229,6 +255,8 @@ __global__ void k_sum_counts(const uint32_t* __restrict__ cnt, uint32_t max_boun
     tot[0] += cnt[0];
     tot[1] += e;
     tot[2] += sh;
+    tot[3] += cnt[0] + e;  // the path tracer's own closest-hit and shadow rays (the DI pass adds to [0] and [2])
+    tot[4] += sh;
 }
 
 struct NeeOut { bool shadow; f3 c; f3 dir; float tmax; };
@@ -390,27 +418,57 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, Q in, Q out, const
     }
 }
 
-template <int STK, bool TOP>
+// COUNT: node and triangle fetches and rays into fetch[3..5]
+template <int STK, bool TOP, bool COUNT>
 __global__ void __launch_bounds__(kTraceBlock) k_shadow(DevScene S, const uint32_t* __restrict__ cnt, uint32_t b,
                                                         const float4* __restrict__ sh_o, const float4* __restrict__ sh_d,
                                                         const float4* __restrict__ sh_c, float4* __restrict__ rad,
-                                                        VertexRec rec, bool remap, uint32_t top_n) {
+                                                        VertexRec rec, bool remap, uint32_t top_n,
+                                                        unsigned long long* __restrict__ fetch) {
     __shared__ int s_stack[(kTraceBlock / 64) * StackSel<STK>::kLds * 64];
     const auto top = stage_top<TOP>(S, s_top, top_n);
-    const uint32_t n = cnt[64 + b];
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, remap);
     auto st = make_stack<STK>(s_stack);
-    for (uint32_t i = blk * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        float4 o = sh_o[i], d = sh_d[i];
-        if (!trace_occluded_s(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, o.w, st, top)) {
-            uint32_t pixel = __float_as_uint(d.w);
-            float4 c = sh_c[i];
-            float4 L = rad[pixel];
-            L.x = L.x + c.x; L.y = L.y + c.y; L.z = L.z + c.z;
-            rad[pixel] = L;
-            if (rec.v) guide_record_nee_visible(rec, pixel, b);
-        }
+    uint32_t nn = 0, nt = 0, nr = 0;
+    trace_shadow_part<COUNT>(S, cnt[64 + b], b, xcd_block(blockIdx.x, gridDim.x, remap), gridDim.x, sh_o, sh_d, sh_c, rad,
+                             rec, st, top, nn, nt, nr);
+    if constexpr (COUNT) {
+        atomicAdd(fetch + 3, (unsigned long long)nn);
+        atomicAdd(fetch + 4, (unsigned long long)nt);
+        atomicAdd(fetch + 5, (unsigned long long)nr);
     }
+}
+
+// The extension rays of bounce b and the shadow rays of bounce b - 1 in one launch: both
+// queues depend only on the shading pass before them, and neither kernel's tail (its
+// slowest waves) then runs alone on the chip.  Workgroups [0, g_sh) trace the shadow
+// queue, the rest the extension queue.
+template <int STK>
+__global__ void __launch_bounds__(kTraceBlock) k_trace_pair(DevScene S, Q in, const uint32_t* __restrict__ cnt, uint32_t b,
+                                                            float4* __restrict__ hits, const uint32_t* __restrict__ perm,
+                                                            uint32_t g_sh, const float4* __restrict__ sh_o,
+                                                            const float4* __restrict__ sh_d, const float4* __restrict__ sh_c,
+                                                            float4* __restrict__ rad, VertexRec rec, bool remap) {
+    __shared__ int s_stack[(kTraceBlock / 64) * StackSel<STK>::kLds * 64];
+    auto st = make_stack<STK>(s_stack);
+    const TopNone top;
+    uint32_t nn = 0, nt = 0, nr = 0;
+    if (blockIdx.x < g_sh) {
+        trace_shadow_part<false>(S, cnt[64 + b - 1], b - 1, xcd_block(blockIdx.x, g_sh, remap), g_sh, sh_o, sh_d, sh_c,
+                                 rad, rec, st, top, nn, nt, nr);
+    } else {
+        const uint32_t ng = gridDim.x - g_sh;
+        trace_ext_part<false>(S, in, cnt[b], xcd_block(blockIdx.x - g_sh, ng, remap), ng, hits, perm, st, top, nn, nt, nr);
+    }
+}
+
+template <int STK>
+int launch_pair(rrh_ctx* c, const DevScene& S, Q& qi, uint32_t* cnt, uint32_t b, uint32_t gt, const uint32_t* perm,
+                float4* rad, const VertexRec& rec, const Bufs& B) {
+    hipLaunchKernelGGL(k_trace_pair<STK>, dim3(2 * gt), dim3(kTraceBlock), 0, c->stream, S, qi, (const uint32_t*)cnt, b,
+                       B.hits, perm, gt, (const float4*)B.sh_o, (const float4*)B.sh_d, (const float4*)B.sh_c, rad, rec,
+                       c->xcd_remap);
+    RRH_LAUNCH_CHECK();
+    return RRH_OK;
 }
 
 // ray-sort key of each queue slot: direction octant (3 bits), then the origin's Morton code
@@ -491,7 +549,7 @@ int launch_trace(rrh_ctx* c, const DevScene& S, Q& qi, uint32_t* cnt, uint32_t b
 }
 template <int STK>
 int launch_shadow(rrh_ctx* c, const DevScene& S, uint32_t* cnt, uint32_t b, uint32_t g, uint32_t gt, float4* rad,
-                  const VertexRec& rec, const Bufs& B, uint32_t top_n) {
+                  const VertexRec& rec, const Bufs& B, uint32_t top_n, bool count) {
     if constexpr (STK != RRH_STACK_NONE) {
     if (c->persistent && S.node_fmt == RRH_NODES_BINARY) {
         const uint32_t gp = std::min(g, resident_blocks(c, k_shadow_p<STK>));
@@ -502,14 +560,17 @@ int launch_shadow(rrh_ctx* c, const DevScene& S, uint32_t* cnt, uint32_t b, uint
         return RRH_OK;
     }
     }
-    if (top_n)
-        hipLaunchKernelGGL((k_shadow<STK, true>), dim3(gt), dim3(kTraceBlock), (size_t)top_n * 4 * sizeof(uint4), c->stream, S,
-                           (const uint32_t*)cnt, b, (const float4*)B.sh_o, (const float4*)B.sh_d, (const float4*)B.sh_c,
-                           rad, rec, c->xcd_remap, top_n);
-    else
-        hipLaunchKernelGGL((k_shadow<STK, false>), dim3(gt), dim3(kTraceBlock), 0, c->stream, S, (const uint32_t*)cnt, b,
-                           (const float4*)B.sh_o, (const float4*)B.sh_d, (const float4*)B.sh_c, rad, rec,
-                           c->xcd_remap, 0u);
+    unsigned long long* fc = (unsigned long long*)c->fetch.p;
+    const size_t lds = (size_t)top_n * 4 * sizeof(uint4);
+    auto go = [&](auto k) {
+        hipLaunchKernelGGL(k, dim3(gt), dim3(kTraceBlock), lds, c->stream, S, (const uint32_t*)cnt, b,
+                           (const float4*)B.sh_o, (const float4*)B.sh_d, (const float4*)B.sh_c, rad, rec, c->xcd_remap,
+                           top_n, fc);
+    };
+    if (top_n && count) go(k_shadow<STK, true, true>);
+    else if (top_n) go(k_shadow<STK, true, false>);
+    else if (count) go(k_shadow<STK, false, true>);
+    else go(k_shadow<STK, false, false>);
     RRH_LAUNCH_CHECK();
     return RRH_OK;
 }
@@ -627,6 +688,30 @@ static int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, 
     const uint32_t gt = std::min((npix + kTraceBlock - 1) / kTraceBlock, kTraceGridCap);
     const int sk = queue_stack(c, s);
     const uint32_t top_n = (s->node_format() == RRH_NODES_WIDE4Q) ? std::min(c->top_nodes, s->top4) : 0u;
+    const bool cf = (p->flags & RRH_PT_COUNT_FETCHES) != 0;
+    // paired launches: not with fetch counting (which counts the closest-hit kernel alone),
+    // the LDS top of the tree, or the persistent binary kernels
+    const bool pairs = c->trace_pairs && !cf && top_n == 0 && !(c->persistent && S.node_fmt == RRH_NODES_BINARY);
+    // profiling: an event pair around every traversal launch (closest hit, paired, shadow)
+    const bool prof = c->profiling && !fixed;
+    auto prof_begin = [&]() -> int {
+        if (!prof) return RRH_OK;
+        if (c->prof_used + 2 > c->prof_events.size()) {
+            for (int k = 0; k < 64; ++k) {
+                hipEvent_t e;
+                RRH_HIP(hipEventCreate(&e));
+                c->prof_events.push_back(e);
+            }
+        }
+        RRH_HIP(hipEventRecord(c->prof_events[c->prof_used], c->stream));
+        return RRH_OK;
+    };
+    auto prof_end = [&]() -> int {
+        if (!prof) return RRH_OK;
+        RRH_HIP(hipEventRecord(c->prof_events[c->prof_used + 1], c->stream));
+        c->prof_used += 2;
+        return RRH_OK;
+    };
     for (uint32_t b = 0; b < p->max_bounces; ++b) {
         Q& qi = q[b & 1];
         Q& qo = q[(b + 1) & 1];
@@ -648,29 +733,22 @@ static int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, 
             if (e != hipSuccess) { set_error(std::string("ray sort: ") + hipGetErrorString(e)); return RRH_E_HIP; }
             perm = slots + npix;
         }
-        const bool prof = c->profiling && !fixed;
-        if (prof) {
-            if (c->prof_used + 2 > c->prof_events.size()) {
-                for (int k = 0; k < 64; ++k) {
-                    hipEvent_t e;
-                    RRH_HIP(hipEventCreate(&e));
-                    c->prof_events.push_back(e);
-                }
-            }
-            RRH_HIP(hipEventRecord(c->prof_events[c->prof_used], c->stream));
-        }
-        const bool cf = (p->flags & RRH_PT_COUNT_FETCHES) != 0;
-        rc = (sk == RRH_STACK_LDS32) ? launch_trace<RRH_STACK_LDS32>(c, S, qi, cnt, b, g, gt, cf, perm, Bf.hits, top_n)
-           : (sk == RRH_STACK_HYBRID16) ? launch_trace<RRH_STACK_HYBRID16>(c, S, qi, cnt, b, g, gt, cf, perm, Bf.hits, top_n)
-           : (sk == RRH_STACK_HYBRID32) ? launch_trace<RRH_STACK_HYBRID32>(c, S, qi, cnt, b, g, gt, cf, perm, Bf.hits, top_n)
-           : (sk == RRH_STACK_NONE) ? launch_trace<RRH_STACK_NONE>(c, S, qi, cnt, b, g, gt, cf, perm, Bf.hits, top_n)
-           : launch_trace<RRH_STACK_PRIVATE>(c, S, qi, cnt, b, g, gt, cf, perm, Bf.hits, top_n);
+        if ((rc = prof_begin())) return rc;
+        if (pairs && b > 0)  // this bounce's extension rays with the previous bounce's shadow rays
+            rc = (sk == RRH_STACK_LDS32) ? launch_pair<RRH_STACK_LDS32>(c, S, qi, cnt, b, gt, perm, rad, A.rec, Bf)
+               : (sk == RRH_STACK_HYBRID16) ? launch_pair<RRH_STACK_HYBRID16>(c, S, qi, cnt, b, gt, perm, rad, A.rec, Bf)
+               : (sk == RRH_STACK_HYBRID32) ? launch_pair<RRH_STACK_HYBRID32>(c, S, qi, cnt, b, gt, perm, rad, A.rec, Bf)
+               : (sk == RRH_STACK_NONE) ? launch_pair<RRH_STACK_NONE>(c, S, qi, cnt, b, gt, perm, rad, A.rec, Bf)
+               : launch_pair<RRH_STACK_PRIVATE>(c, S, qi, cnt, b, gt, perm, rad, A.rec, Bf);
+        else
+            rc = (sk == RRH_STACK_LDS32) ? launch_trace<RRH_STACK_LDS32>(c, S, qi, cnt, b, g, gt, cf, perm, Bf.hits, top_n)
+               : (sk == RRH_STACK_HYBRID16) ? launch_trace<RRH_STACK_HYBRID16>(c, S, qi, cnt, b, g, gt, cf, perm, Bf.hits, top_n)
+               : (sk == RRH_STACK_HYBRID32) ? launch_trace<RRH_STACK_HYBRID32>(c, S, qi, cnt, b, g, gt, cf, perm, Bf.hits, top_n)
+               : (sk == RRH_STACK_NONE) ? launch_trace<RRH_STACK_NONE>(c, S, qi, cnt, b, g, gt, cf, perm, Bf.hits, top_n)
+               : launch_trace<RRH_STACK_PRIVATE>(c, S, qi, cnt, b, g, gt, cf, perm, Bf.hits, top_n);
         if (rc) return rc;
         RRH_LAUNCH_CHECK();
-        if (prof) {
-            RRH_HIP(hipEventRecord(c->prof_events[c->prof_used + 1], c->stream));
-            c->prof_used += 2;
-        }
+        if ((rc = prof_end())) return rc;
         A.b = b;
         // the direction sampled at the last bounce is never traced (and its training record
         // carries no radiance): the guide is skipped there
@@ -684,12 +762,15 @@ static int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, 
         hipLaunchKernelGGL(k_shade, dim3(g), dim3(kBlock), 0, c->stream, S, qi, qo, (const float4*)Bf.hits, cnt,
                            Bf.sh_o, Bf.sh_d, Bf.sh_c, rad, Ab);
         RRH_LAUNCH_CHECK();
-        rc = (sk == RRH_STACK_LDS32) ? launch_shadow<RRH_STACK_LDS32>(c, S, cnt, b, g, gt, rad, A.rec, Bf, top_n)
-           : (sk == RRH_STACK_HYBRID16) ? launch_shadow<RRH_STACK_HYBRID16>(c, S, cnt, b, g, gt, rad, A.rec, Bf, top_n)
-           : (sk == RRH_STACK_HYBRID32) ? launch_shadow<RRH_STACK_HYBRID32>(c, S, cnt, b, g, gt, rad, A.rec, Bf, top_n)
-           : (sk == RRH_STACK_NONE) ? launch_shadow<RRH_STACK_NONE>(c, S, cnt, b, g, gt, rad, A.rec, Bf, top_n)
-           : launch_shadow<RRH_STACK_PRIVATE>(c, S, cnt, b, g, gt, rad, A.rec, Bf, top_n);
+        if (pairs && b + 1 < p->max_bounces) continue;  // traced with the next bounce's extension rays
+        if ((rc = prof_begin())) return rc;
+        rc = (sk == RRH_STACK_LDS32) ? launch_shadow<RRH_STACK_LDS32>(c, S, cnt, b, g, gt, rad, A.rec, Bf, top_n, cf)
+           : (sk == RRH_STACK_HYBRID16) ? launch_shadow<RRH_STACK_HYBRID16>(c, S, cnt, b, g, gt, rad, A.rec, Bf, top_n, cf)
+           : (sk == RRH_STACK_HYBRID32) ? launch_shadow<RRH_STACK_HYBRID32>(c, S, cnt, b, g, gt, rad, A.rec, Bf, top_n, cf)
+           : (sk == RRH_STACK_NONE) ? launch_shadow<RRH_STACK_NONE>(c, S, cnt, b, g, gt, rad, A.rec, Bf, top_n, cf)
+           : launch_shadow<RRH_STACK_PRIVATE>(c, S, cnt, b, g, gt, rad, A.rec, Bf, top_n, cf);
         if (rc) return rc;
+        if ((rc = prof_end())) return rc;
     }
     hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(64), 0, c->stream, (const uint32_t*)cnt, p->max_bounces,
                        (unsigned long long*)c->totals.p);
@@ -785,14 +866,9 @@ int rrh_pt_ray_counts(rrh_ctx* c, rrh_ray_counts* out) {
 int rrh_ray_totals(rrh_ctx* c, rrh_ray_counts* out, int reset) {
     RRH_ARG(c && out, "rrh_ray_totals: null argument");
     out->primary = out->extension = out->shadow = 0;
-    if (!c->totals.p) {
-        int rc = c->totals.ensure(32);
-        if (rc) return rc;
-        RRH_HIP(hipMemsetAsync(c->totals.p, 0, 32, c->stream));
-    }
     unsigned long long h[3];
     RRH_HIP(hipMemcpyAsync(h, c->totals.p, 24, hipMemcpyDeviceToHost, c->stream));
-    if (reset) RRH_HIP(hipMemsetAsync(c->totals.p, 0, 32, c->stream));
+    if (reset) RRH_HIP(hipMemsetAsync(c->totals.p, 0, 24, c->stream));
     RRH_HIP(hipStreamSynchronize(c->stream));
     out->primary = h[0]; out->extension = h[1]; out->shadow = h[2];
     return RRH_OK;
@@ -804,9 +880,31 @@ int rrh_fetch_counts(rrh_ctx* c, uint64_t* nodes, uint64_t* tris, uint64_t* rays
     if (!c->fetch.p) return RRH_OK;
     unsigned long long h[3];
     RRH_HIP(hipMemcpyAsync(h, c->fetch.p, 24, hipMemcpyDeviceToHost, c->stream));
-    if (reset) RRH_HIP(hipMemsetAsync(c->fetch.p, 0, 32, c->stream));
+    if (reset) RRH_HIP(hipMemsetAsync(c->fetch.p, 0, 24, c->stream));
     RRH_HIP(hipStreamSynchronize(c->stream));
     *nodes = h[0]; *tris = h[1]; *rays = h[2];
+    return RRH_OK;
+}
+
+int rrh_fetch_counts_shadow(rrh_ctx* c, uint64_t* nodes, uint64_t* tris, uint64_t* rays, int reset) {
+    RRH_ARG(c && nodes && tris && rays, "rrh_fetch_counts_shadow: null argument");
+    *nodes = *tris = *rays = 0;
+    if (!c->fetch.p) return RRH_OK;
+    unsigned long long h[3];
+    RRH_HIP(hipMemcpyAsync(h, (unsigned long long*)c->fetch.p + 3, 24, hipMemcpyDeviceToHost, c->stream));
+    if (reset) RRH_HIP(hipMemsetAsync((unsigned long long*)c->fetch.p + 3, 0, 24, c->stream));
+    RRH_HIP(hipStreamSynchronize(c->stream));
+    *nodes = h[0]; *tris = h[1]; *rays = h[2];
+    return RRH_OK;
+}
+
+int rrh_pt_traversal_rays(rrh_ctx* c, uint64_t* closest, uint64_t* shadow, int reset) {
+    RRH_ARG(c && closest && shadow, "rrh_pt_traversal_rays: null argument");
+    unsigned long long h[2];
+    RRH_HIP(hipMemcpyAsync(h, (unsigned long long*)c->totals.p + 3, 16, hipMemcpyDeviceToHost, c->stream));
+    if (reset) RRH_HIP(hipMemsetAsync((unsigned long long*)c->totals.p + 3, 0, 16, c->stream));
+    RRH_HIP(hipStreamSynchronize(c->stream));
+    *closest = h[0]; *shadow = h[1];
     return RRH_OK;
 }
 

@@ -109,26 +109,31 @@ def test_ray_sort_keeps_the_image(ctx, bits):
         assert np.array_equal(g.view(np.uint32), r.view(np.uint32))
 
 
-@pytest.mark.parametrize("stack,top", [(L.STACK_NONE, 0), (L.STACK_NONE, 85), (L.STACK_LDS32, 85),
-                                       (L.STACK_LDS32, 512), (L.STACK_HYBRID16, 21), (L.STACK_PRIVATE, 341)])
-def test_traversal_variants_keep_the_image(ctx, stack, top):
-    """The stackless traversal (RRH_STACK_NONE) and the top of the tree read from LDS
-    (RRH_OPT_TRAV_TOP_NODES) change only how the ray queues are traced: images and ray counts
-    equal the default frame bit for bit, on the indoor scene (depth 12) and the heightfield."""
+@pytest.mark.parametrize("stack,top,pairs", [(L.STACK_NONE, 0, 0), (L.STACK_NONE, 85, 0), (L.STACK_LDS32, 85, 0),
+                                             (L.STACK_LDS32, 512, 0), (L.STACK_HYBRID16, 21, 0), (L.STACK_PRIVATE, 341, 0),
+                                             (L.STACK_LDS32, 0, 1), (L.STACK_NONE, 0, 1), (L.STACK_PRIVATE, 0, 1)])
+def test_traversal_variants_keep_the_image(ctx, stack, top, pairs):
+    """The stackless traversal (RRH_STACK_NONE), the top of the tree read from LDS
+    (RRH_OPT_TRAV_TOP_NODES) and paired extension + shadow launches (RRH_OPT_TRACE_PAIRS)
+    change only how the ray queues are traced: images and ray counts equal those of one launch
+    per queue with the LDS stack, bit for bit, on the indoor scene (depth 12) and the heightfield."""
     for kind, seed, scale, W, H, depth in ((L.GEN_HEIGHTFIELD, 2, 0.25, 160, 90, 6), (L.GEN_INDOOR, 1, 0.3, 96, 64, 12)):
         sa = core.generate_scene(kind, seed, scale)
         gs = core.Scene(ctx, sa)
         assert gs.info()["bvh_depth4"] <= 31
         cam = core.camera_look_at(sa.eye, sa.center, sa.fov, W, H)
         gs.set_camera(cam)
-        ref = [render_gpu(ctx, gs, cam, depth, f) for f in (0, 3)]
         try:
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRACE_PAIRS, 0)
+            ref = [render_gpu(ctx, gs, cam, depth, f) for f in (0, 3)]
             L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAVERSAL_STACK, stack)
             L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAV_TOP_NODES, top)
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRACE_PAIRS, pairs)
             got = [render_gpu(ctx, gs, cam, depth, f) for f in (0, 3)]
         finally:
             L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAVERSAL_STACK, L.STACK_LDS32)
             L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAV_TOP_NODES, 0)
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRACE_PAIRS, L.TRACE_PAIRS_DEFAULT)
         for (g, gc), (r, rc) in zip(got, ref):
             assert gc == rc
             assert np.array_equal(g.view(np.uint32), r.view(np.uint32))
