@@ -50,33 +50,59 @@ def orbit_camera(sa, frame: int, W: int, H: int):
 
 
 class Frame:
-    def __init__(self, ctx, scene, sa, W, H, max_bounces, di_m):
+    """One step: the path tracer (1 spp) plus accumulation, and the three ReSTIR DI stages.
+    di_stream: the DI pass runs on a second context (its own HIP stream and its own copy of the
+    scene), so that its kernels overlap the path tracer's on the GPU; otherwise after it on the
+    same stream."""
+
+    def __init__(self, ctx, scene, sa, W, H, max_bounces, di_m, di_ctx=None):
         self.ctx, self.scene, self.sa, self.W, self.H = ctx, scene, sa, W, H
         self.max_bounces, self.di_m = max_bounces, di_m
-        img = lambda f=L.FMT_RGBA32F: core.Image(ctx, W, H, f)
-        self.rad, self.acc, self.gp, self.gn, self.out = img(), img(), img(), img(), img()
-        self.res = [img(), img()]
-        self.uv = [img(L.FMT_R32U), img(L.FMT_R32U)]
+        if di_ctx is not None:
+            self.dctx = di_ctx
+            self.dscene = core.Scene(di_ctx, sa)
+        else:
+            self.dctx, self.dscene = ctx, scene
+        img = lambda c, f=L.FMT_RGBA32F: core.Image(c, W, H, f)
+        self.rad, self.acc = img(ctx), img(ctx)
+        self.gp, self.gn, self.out = img(self.dctx), img(self.dctx), img(self.dctx)
+        self.res = [img(self.dctx), img(self.dctx)]
+        self.uv = [img(self.dctx, L.FMT_R32U), img(self.dctx, L.FMT_R32U)]
+
+    def contexts(self):
+        return [self.ctx] if self.dctx is self.ctx else [self.ctx, self.dctx]
+
+    def synchronize(self) -> None:
+        for c in self.contexts():
+            c.synchronize()
 
     def run(self, f: int, flags: int = 0) -> None:
         c, s = self.ctx.h, self.scene.h
-        self.scene.set_camera(orbit_camera(self.sa, f, self.W, self.H))
+        cam = orbit_camera(self.sa, f, self.W, self.H)
+        self.scene.set_camera(cam)
+        if self.dscene is not self.scene:
+            self.dscene.set_camera(cam)
         p = L.PTParams(self.max_bounces, f, 3, L.GUIDE_NONE, 0.0, 0, flags)
         L.call("rrh_pt_render", c, s, C.byref(p), self.rad.h)
         L.call("rrh_accumulate", c, self.rad.h, self.acc.h, 1, 0)  # orbiting camera: reset every frame
         cur, prev = f & 1, 1 - (f & 1)
         d = L.DIParams(f, self.di_m, 1, 1, 5, 30)
-        L.call("rrh_di_initial", c, s, C.byref(d), self.gp.h, self.gn.h, self.res[cur].h, self.uv[cur].h)
-        L.call("rrh_di_temporal", c, s, C.byref(d), self.gp.h, self.gn.h, self.res[cur].h, self.uv[cur].h,
+        dc, ds = self.dctx.h, self.dscene.h
+        L.call("rrh_di_initial", dc, ds, C.byref(d), self.gp.h, self.gn.h, self.res[cur].h, self.uv[cur].h)
+        L.call("rrh_di_temporal", dc, ds, C.byref(d), self.gp.h, self.gn.h, self.res[cur].h, self.uv[cur].h,
                self.res[prev].h, self.uv[prev].h)
-        L.call("rrh_di_spatial_shade", c, s, C.byref(d), self.gp.h, self.gn.h, self.res[cur].h, self.uv[cur].h,
+        L.call("rrh_di_spatial_shade", dc, ds, C.byref(d), self.gp.h, self.gn.h, self.res[cur].h, self.uv[cur].h,
                self.out.h)
 
 
-def ray_totals(ctx, reset=True):
-    rc = L.RayCounts()
-    L.call("rrh_ray_totals", ctx.h, C.byref(rc), 1 if reset else 0)
-    return rc.primary, rc.extension, rc.shadow
+def ray_totals(ctxs, reset=True):
+    """rays traced since the last reset, summed over the frame's contexts"""
+    tot = [0, 0, 0]
+    for ctx in ctxs:
+        rc = L.RayCounts()
+        L.call("rrh_ray_totals", ctx.h, C.byref(rc), 1 if reset else 0)
+        tot = [tot[0] + rc.primary, tot[1] + rc.extension, tot[2] + rc.shadow]
+    return tuple(tot)
 
 
 def cpu_model() -> str:
@@ -270,7 +296,7 @@ def restir_gi_summary(ctx: core.Context, scene, sa, W: int, H: int, frames: int,
 
 
 # the options a traffic probe must share with the timed run (same kernels, same tree)
-PROBE_OPTS = ["stack", "top_nodes", "trace_pairs", "xcd_remap", "persistent", "max_leaf", "trav_cost", "speculative", "bvh_builder",
+PROBE_OPTS = ["stack", "top_nodes", "trace_pairs", "di_stream", "xcd_remap", "persistent", "max_leaf", "trav_cost", "speculative", "bvh_builder",
               "ploc_radius", "ray_sort", "bvh_nodes", "width", "height", "max_bounces"]
 COPY_BYTES = 1 << 30
 
@@ -293,6 +319,15 @@ def configure(ctx, a) -> None:
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_RAY_SORT, a.ray_sort)
 
 
+def di_context(a):
+    """the ReSTIR DI pass's own context (its own stream) when --di-stream is set"""
+    if not a.di_stream:
+        return None
+    dctx = core.Context(0)
+    configure(dctx, a)
+    return dctx
+
+
 def traffic_probe(a) -> int:
     """Run under rocprofv3 --pmc by measure_traffic: the copy kernel of known traffic (1 GiB in,
     1 GiB out per dispatch), then four frames of the bench's workload."""
@@ -302,10 +337,10 @@ def traffic_probe(a) -> int:
     L.call("rrh_stream_copy", ctx.h, COPY_BYTES, 2, C.byref(g))
     sa = core.generate_scene(L.GEN_HEIGHTFIELD, seed=2, scale=1.0)
     scene = core.Scene(ctx, sa)
-    fr = Frame(ctx, scene, sa, a.width, a.height, a.max_bounces, 32)
+    fr = Frame(ctx, scene, sa, a.width, a.height, a.max_bounces, 32, di_context(a))
     for f in range(4):
         fr.run(f)
-    ctx.synchronize()
+    fr.synchronize()
     print("[probe] done", flush=True)
     return 0
 
@@ -385,6 +420,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stack", choices=["private", "lds32", "hybrid16", "hybrid32", "none"], default="lds32",
                     help="where the traversal stack lives (context option; none: stackless)")
+    ap.add_argument("--di-stream", type=int, default=1,
+                    help="1: the ReSTIR DI pass on a second context (its own stream), overlapping the path tracer")
     ap.add_argument("--trace-pairs", type=int, default=1,
                     help="1: extension rays of bounce b traced in one launch with the shadow rays of b - 1")
     ap.add_argument("--top-nodes", type=int, default=0,
@@ -442,10 +479,10 @@ def main() -> int:
           f"{a.bvh_builder} BVH build+upload {info['build_ms']:.1f} ms, scene total {time.perf_counter() - t0:.1f} s",
           file=sys.stderr, flush=True)
     W, H = a.width, a.height
-    fr = Frame(ctx, scene, sa, W, H, a.max_bounces, 32)
+    fr = Frame(ctx, scene, sa, W, H, a.max_bounces, 32, di_context(a))
     for f in range(a.warmup):
         fr.run(f)
-    ctx.synchronize()
+    fr.synchronize()
     # instrumented traversal on one extra frame: algorithmic bytes per closest-hit ray
     L.call("rrh_fetch_counts", ctx.h, C.byref(C.c_uint64()), C.byref(C.c_uint64()), C.byref(C.c_uint64()), 1)
     fr.run(a.warmup, L.PT_COUNT_FETCHES)
@@ -461,19 +498,19 @@ def main() -> int:
     print(f"[bench] traversal: {nn.value / max(nr.value, 1):.1f} nodes + {nt.value / max(nr.value, 1):.1f} tris "
           f"per ray -> {bytes_per_ray:.0f} algorithmic B/ray; shadow rays "
           f"{sn.value / max(sr.value, 1):.1f} nodes + {st.value / max(sr.value, 1):.1f} tris", file=sys.stderr, flush=True)
-    ctx.synchronize()
-    ray_totals(ctx, reset=True)
+    fr.synchronize()
+    ray_totals(fr.contexts(), reset=True)
     L.call("rrh_pt_traversal_rays", ctx.h, C.byref(C.c_uint64()), C.byref(C.c_uint64()), 1)
     L.call("rrh_profile_enable", ctx.h, 1)
     L.call("rrh_profile_traversal", ctx.h, C.byref(C.c_float()), C.byref(C.c_uint32()), 1)
-    ctx.synchronize()
+    fr.synchronize()
     t0 = time.perf_counter()
     for k in range(a.steps):
         fr.run(a.warmup + 1 + k)
-    ctx.synchronize()
+    fr.synchronize()
     dt = time.perf_counter() - t0
     L.call("rrh_profile_enable", ctx.h, 0)
-    prim, ext, shadow = ray_totals(ctx, reset=True)
+    prim, ext, shadow = ray_totals(fr.contexts(), reset=True)
     trav_ms, launches = C.c_float(), C.c_uint32()
     L.call("rrh_profile_traversal", ctx.h, C.byref(trav_ms), C.byref(launches), 1)
     # the path tracer's own closest-hit and shadow rays (the ReSTIR DI pass traces its primary
@@ -505,7 +542,7 @@ def main() -> int:
                    "width": W, "height": H, "spp_per_frame": 1, "max_bounces": a.max_bounces,
                    "di_candidates": 32, "di_temporal_mcap": "20*M", "di_spatial": "5 neighbours, r=30",
                    "camera": "orbit 0.5 deg/frame", "parallelism": "single GPU",
-                   "traversal_stack": a.stack, "top_nodes_in_lds": a.top_nodes, "trace_pairs": a.trace_pairs, "xcd_remap": a.xcd_remap, "bvh_nodes": a.bvh_nodes, "persistent": a.persistent, "speculative": a.speculative,
+                   "traversal_stack": a.stack, "top_nodes_in_lds": a.top_nodes, "trace_pairs": a.trace_pairs, "di_pass_stream": "own context, overlapping the path tracer" if a.di_stream else "after the path tracer, same stream", "xcd_remap": a.xcd_remap, "bvh_nodes": a.bvh_nodes, "persistent": a.persistent, "speculative": a.speculative,
                    "bvh_max_leaf": a.max_leaf, "bvh_trav_cost": a.trav_cost, "bvh_builder": a.bvh_builder,
                    "bvh_build_ms": round(info["build_ms"], 1), "ray_sort_bits": a.ray_sort,
                    "rays_per_step": {"primary": prim // a.steps, "extension": ext // a.steps,

@@ -117,7 +117,7 @@ int rrh_ctx_destroy(rrh_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     for (int i = 0; i < 2; ++i) { c->q_o[i].release(); c->q_d[i].release(); c->q_thp[i].release(); }
     c->hits.release(); c->sh_o.release(); c->sh_d.release(); c->sh_c.release();
-    c->counters.release(); c->vtx.release(); c->di_counter.release(); c->rpt_counter.release(); c->rgi_counter.release(); c->scratch.release(); c->mlp_prep.release();
+    c->counters.release(); c->vtx.release(); c->di_counter.release(); c->di_pack.release(); c->rpt_counter.release(); c->rgi_counter.release(); c->scratch.release(); c->mlp_prep.release();
     c->train.release(); c->train_cnt.release(); c->fin_chunks.release(); c->fin_tmp.release(); c->totals.release(); c->fetch.release();
     c->sort_keys.release(); c->sort_perm.release(); c->sort_tmp.release();
     for (hipEvent_t e : c->prof_events) (void)hipEventDestroy(e);

@@ -145,11 +145,26 @@ __global__ void __launch_bounds__(64) k_di_temporal(DevScene S, uint32_t frame_s
     rc_uv[i] = cuv;
 }
 
+// What a spatial neighbour reads of a pixel, in one 16-byte record: selected light (kEmpty for
+// a miss, an empty reservoir or M = 0), W, M and the sample point; one cache line per
+// neighbour instead of one in each of the position, reservoir and sample planes
+__global__ void __launch_bounds__(256) k_di_pack(const float4* __restrict__ gpos, const float4* __restrict__ res,
+                                                 const uint32_t* __restrict__ res_uv, uint4* __restrict__ pk, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float4 r = res[i];
+    uint32_t sel = __float_as_uint(r.x);
+    const uint32_t M = __float_as_uint(r.w);
+    if (gpos[i].w < 0.0f || M == 0u) sel = kEmpty;
+    pk[i] = make_uint4(sel, __float_as_uint(r.z), M, res_uv[i]);
+}
+
 template <int STK>
 __global__ void __launch_bounds__(64) k_di_spatial_shade(DevScene S, rrh_di_params P, const float4* __restrict__ gpos,
                                                          const float4* __restrict__ gnrm, const float4* __restrict__ res,
-                                                         const uint32_t* __restrict__ res_uv, float4* __restrict__ out,
-                                                         uint32_t* __restrict__ shadow_count, bool remap) {
+                                                         const uint32_t* __restrict__ res_uv, const uint4* __restrict__ pk,
+                                                         float4* __restrict__ out, uint32_t* __restrict__ shadow_count,
+                                                         bool remap) {
     __shared__ int s_stack[StackSel<STK>::kLds * 64];
     const rrh_camera& cam = *S.cam;
     uint32_t W = cam.screen_width, H = cam.screen_height;
@@ -185,12 +200,11 @@ __global__ void __launch_bounds__(64) k_di_spatial_shade(DevScene S, rrh_di_para
                         nx = nx < 0 ? 0 : (nx > (int)W - 1 ? (int)W - 1 : nx);
                         ny = ny < 0 ? 0 : (ny > (int)H - 1 ? (int)H - 1 : ny);
                         size_t j = (size_t)ny * W + (size_t)nx;
-                        if (gpos[j].w < 0.0f) continue;
-                        float4 nr = res[j];
-                        uint32_t nsel = __float_as_uint(nr.x), nM = __float_as_uint(nr.w), nuv = res_uv[j];
-                        if (nsel == kEmpty || nM == 0u) continue;
+                        const uint4 nr = pk[j];
+                        const uint32_t nsel = nr.x, nM = nr.z, nuv = nr.w;
+                        if (nsel == kEmpty) continue;  // a miss, an empty reservoir or M = 0
                         float ph = p_hat(S, nsel, nuv, hp, n);
-                        float contrib = ph * nr.z * (float)nM;
+                        float contrib = ph * __uint_as_float(nr.y) * (float)nM;
                         rw += contrib;
                         rM += nM;
                         if (contrib > 0.0f && rnd(seed) < contrib / rw) { rsel = nsel; ruv = nuv; }
@@ -316,24 +330,23 @@ int rrh_di_spatial_shade(rrh_ctx* c, rrh_scene* s, const rrh_di_params* p, const
     int rc = c->di_counter.ensure(16);
     if (rc) return rc;
     RRH_HIP(hipMemsetAsync(c->di_counter.p, 0, 4, c->stream));
+    const uint32_t npix = W * H;
+    if ((rc = c->di_pack.ensure((size_t)npix * sizeof(uint4)))) return rc;
+    const uint4* pk = (const uint4*)c->di_pack.p;
+    hipLaunchKernelGGL(k_di_pack, dim3((npix + 255) / 256), dim3(256), 0, c->stream, (const float4*)gpos->p,
+                       (const float4*)res->p, (const uint32_t*)res_uv->p, (uint4*)c->di_pack.p, npix);
+    RRH_LAUNCH_CHECK();
     const int sk = stack_for(c, s->stack_need());
     dim3 grid((W + 7) / 8, (H + 7) / 8);
-    if (sk == RRH_STACK_LDS32)
-        hipLaunchKernelGGL(k_di_spatial_shade<RRH_STACK_LDS32>, grid, dim3(64), 0, c->stream, s->dev(), *p,
-                           (const float4*)gpos->p, (const float4*)gnrm->p, (const float4*)res->p,
-                           (const uint32_t*)res_uv->p, (float4*)out->p, (uint32_t*)c->di_counter.p, c->xcd_remap);
-    else if (sk == RRH_STACK_HYBRID16)
-        hipLaunchKernelGGL(k_di_spatial_shade<RRH_STACK_HYBRID16>, grid, dim3(64), 0, c->stream, s->dev(), *p,
-                           (const float4*)gpos->p, (const float4*)gnrm->p, (const float4*)res->p,
-                           (const uint32_t*)res_uv->p, (float4*)out->p, (uint32_t*)c->di_counter.p, c->xcd_remap);
-    else if (sk == RRH_STACK_HYBRID32)
-        hipLaunchKernelGGL(k_di_spatial_shade<RRH_STACK_HYBRID32>, grid, dim3(64), 0, c->stream, s->dev(), *p,
-                           (const float4*)gpos->p, (const float4*)gnrm->p, (const float4*)res->p,
-                           (const uint32_t*)res_uv->p, (float4*)out->p, (uint32_t*)c->di_counter.p, c->xcd_remap);
-    else
-        hipLaunchKernelGGL(k_di_spatial_shade<RRH_STACK_PRIVATE>, grid, dim3(64), 0, c->stream, s->dev(), *p,
-                           (const float4*)gpos->p, (const float4*)gnrm->p, (const float4*)res->p,
-                           (const uint32_t*)res_uv->p, (float4*)out->p, (uint32_t*)c->di_counter.p, c->xcd_remap);
+    auto go = [&](auto k) {
+        hipLaunchKernelGGL(k, grid, dim3(64), 0, c->stream, s->dev(), *p, (const float4*)gpos->p, (const float4*)gnrm->p,
+                           (const float4*)res->p, (const uint32_t*)res_uv->p, pk, (float4*)out->p,
+                           (uint32_t*)c->di_counter.p, c->xcd_remap);
+    };
+    if (sk == RRH_STACK_LDS32) go(k_di_spatial_shade<RRH_STACK_LDS32>);
+    else if (sk == RRH_STACK_HYBRID16) go(k_di_spatial_shade<RRH_STACK_HYBRID16>);
+    else if (sk == RRH_STACK_HYBRID32) go(k_di_spatial_shade<RRH_STACK_HYBRID32>);
+    else go(k_di_spatial_shade<RRH_STACK_PRIVATE>);
     RRH_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_di_totals, dim3(1), dim3(64), 0, c->stream, (const uint32_t*)c->di_counter.p, W * H,
                        (unsigned long long*)c->totals.p);

@@ -120,6 +120,7 @@ struct rrh_ctx {
     uint32_t last_record_stride = 0;
     uint32_t last_npix = 0;
     rrh::DevBuf di_counter;
+    rrh::DevBuf di_pack;          // ReSTIR DI spatial pass: one 16-byte neighbour record per pixel
     rrh::DevBuf rpt_counter;      // ReSTIR PT ray counters (see rrh_restir_pt.hip)
     rrh::DevBuf rgi_counter;      // ReSTIR GI ray counters (see rrh_restir_gi.hip)
     rrh::DevBuf totals;           // uint64 x 3: primary, extension, shadow (device-side sums)
