@@ -441,8 +441,9 @@ int guide_prepare(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* gr
         R.v = (float4*)c->vtx.p;
         R.max_bounces = p->max_bounces;
         R.stamp = c->stamp;
-        // record a pixel lattice holding about twice the requested vertices (about half of the
-        // recorded vertices are live), so the hash thinning still picks the training set
+        // record a pixel lattice with at least four vertex slots per requested vertex (paths end
+        // early, so only part of the slots hold vertices), so the hash thinning still picks
+        // the training set
         const double want = 4.0 * (double)p->record_vertices;
         while (R.shift < 16 && (double)(npix >> (R.shift + 1)) * p->max_bounces >= want) ++R.shift;
         R.phase = c->stamp & ((1u << R.shift) - 1u);

@@ -88,7 +88,7 @@ def test_restir_di_and_atrous_full_1080p(ctx, heightfield):
         _assert_same(g_out, o_out, "shaded output")
         assert np.isfinite(g_out).all()
 
-    # config 3 on this frame: fp16 radiance + luminance variance, fp16 normal + linear depth
+    # config-mode A-trous inputs from this frame: fp16 radiance + luminance variance, fp16 normal + linear depth
     r16, g16 = im(L.FMT_RGBA16F), im(L.FMT_RGBA16F)
     L.call("rrh_atrous_prepare", ctx.h, gs.h, out.h, gp.h, gn.h, r16.h, g16.h)
     o_r16, o_g16 = O.atrous_prepare(g_out.reshape(H, W, 4), g_gp.reshape(H, W, 4), g_gn.reshape(H, W, 4),

@@ -353,6 +353,32 @@ def test_guiding_splat_fixed_point_wraps_like_int32():
     assert acc2[0, :3].view(np.int32).tolist() == [1, -2, 3]  # round half away from zero
 
 
+def test_guiding_cell_index_quantises_positions():
+    """What fixes the cell index without the hash itself (which stays unpinned): positions in
+    one cell of side base + slope x distance share an index, a cell boundary, another BSDF id
+    or a larger cell (farther camera) changes it, and the index stays below the cell count."""
+    cam0 = O.fa([0.0, 0.0, 0.0])
+    idx = lambda pos, bsdf=0, cam=cam0, base=0.1, slope=0.0, n=1 << 20: lib.orc_guiding_cell_index(
+        O.fa(pos), bsdf, cam, base, slope, n)
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        q = rng.integers(-50, 50, 3)                       # a cell of side 0.1
+        inside = [(q + rng.uniform(0.05, 0.95, 3)) * 0.1 for _ in range(4)]
+        ids = {idx(p.tolist()) for p in inside}
+        assert len(ids) == 1
+        # one step across each face lands elsewhere (a collision in 2^20 cells is ~1e-6)
+        for k in range(3):
+            e = np.zeros(3); e[k] = 0.1
+            assert idx((inside[0] + e).tolist()) not in ids
+        assert idx(inside[0].tolist(), bsdf=1) not in ids
+    # cell side grows with distance: at slope 1 from 9.95 m away the side is 10.05, so points
+    # 1 m apart share a cell that a 0.1 m grid splits
+    far = O.fa([0.0, 0.0, -9.95])
+    assert idx([0.2, 0.2, 0.0], cam=far, slope=1.0) == idx([1.2, 1.2, 0.0], cam=far, slope=1.0)
+    assert idx([0.2, 0.2, 0.0]) != idx([1.2, 1.2, 0.0])
+    assert max(idx(rng.uniform(-100, 100, 3).tolist(), n=1000) for _ in range(500)) < 1000
+
+
 def test_guiding_fit_below_min_samples_marks_invalid():
     acc = np.zeros((2, 4), np.uint32)
     fit = np.ones((2, 4), np.float32)
