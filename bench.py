@@ -23,6 +23,7 @@ import subprocess
 import sys
 import tempfile
 import time
+import types
 from pathlib import Path
 
 import numpy as np
@@ -55,7 +56,7 @@ class Frame:
     scene), so that its kernels overlap the path tracer's on the GPU; otherwise after it on the
     same stream."""
 
-    def __init__(self, ctx, scene, sa, W, H, max_bounces, di_m, di_ctx=None):
+    def __init__(self, ctx, scene, sa, W, H, max_bounces, di_m, di_ctx=None, band_ctxs=()):
         self.ctx, self.scene, self.sa, self.W, self.H = ctx, scene, sa, W, H
         self.max_bounces, self.di_m = max_bounces, di_m
         if di_ctx is not None:
@@ -63,27 +64,41 @@ class Frame:
             self.dscene = core.Scene(di_ctx, sa)
         else:
             self.dctx, self.dscene = ctx, scene
+        # band_ctxs: the path tracer's image split into 1 + len(band_ctxs) row bands, each traced
+        # on its own context (its own stream and copy of the scene); band 0 on ctx
+        self.bands = [(ctx, scene)] + [(c, core.Scene(c, sa)) for c in band_ctxs]
+        n = len(self.bands)
+        self.rows = [(H * k // n, H * (k + 1) // n) for k in range(n)]
         img = lambda c, f=L.FMT_RGBA32F: core.Image(c, W, H, f)
         self.rad, self.acc = img(ctx), img(ctx)
         self.gp, self.gn, self.out = img(self.dctx), img(self.dctx), img(self.dctx)
         self.res = [img(self.dctx), img(self.dctx)]
         self.uv = [img(self.dctx, L.FMT_R32U), img(self.dctx, L.FMT_R32U)]
 
+    def pt_contexts(self):
+        return [c for c, _ in self.bands]
+
     def contexts(self):
-        return [self.ctx] if self.dctx is self.ctx else [self.ctx, self.dctx]
+        return self.pt_contexts() + ([] if self.dctx is self.ctx else [self.dctx])
 
     def synchronize(self) -> None:
         for c in self.contexts():
             c.synchronize()
 
     def run(self, f: int, flags: int = 0) -> None:
-        c, s = self.ctx.h, self.scene.h
+        c = self.ctx.h
         cam = orbit_camera(self.sa, f, self.W, self.H)
-        self.scene.set_camera(cam)
+        for _, sc in self.bands:
+            sc.set_camera(cam)
         if self.dscene is not self.scene:
             self.dscene.set_camera(cam)
-        p = L.PTParams(self.max_bounces, f, 3, L.GUIDE_NONE, 0.0, 0, flags)
-        L.call("rrh_pt_render", c, s, C.byref(p), self.rad.h)
+        whole = len(self.bands) == 1
+        for (bc, bs), (r0, r1) in zip(self.bands, self.rows):
+            p = L.PTParams(self.max_bounces, f, 3, L.GUIDE_NONE, 0.0, 0, flags, 0, 0 if whole else r0,
+                           0 if whole else r1)
+            L.call("rrh_pt_render", bc.h, bs.h, C.byref(p), self.rad.h)
+        for bc, _ in self.bands[1:]:
+            L.call("rrh_ctx_join", c, bc.h)  # the accumulation reads every band
         L.call("rrh_accumulate", c, self.rad.h, self.acc.h, 1, 0)  # orbiting camera: reset every frame
         cur, prev = f & 1, 1 - (f & 1)
         d = L.DIParams(f, self.di_m, 1, 1, 5, 30)
@@ -296,7 +311,7 @@ def restir_gi_summary(ctx: core.Context, scene, sa, W: int, H: int, frames: int,
 
 
 # the options a traffic probe must share with the timed run (same kernels, same tree)
-PROBE_OPTS = ["stack", "top_nodes", "trace_pairs", "di_stream", "xcd_remap", "persistent", "max_leaf", "trav_cost", "speculative", "bvh_builder",
+PROBE_OPTS = ["stack", "top_nodes", "trace_pairs", "di_stream", "pt_bands", "xcd_remap", "persistent", "max_leaf", "trav_cost", "speculative", "bvh_builder",
               "ploc_radius", "ray_sort", "bvh_nodes", "width", "height", "max_bounces"]
 COPY_BYTES = 1 << 30
 
@@ -328,6 +343,16 @@ def di_context(a):
     return dctx
 
 
+def band_contexts(a):
+    """contexts for the path tracer's row bands after the first (--pt-bands)"""
+    out = []
+    for _ in range(max(a.pt_bands, 1) - 1):
+        bc = core.Context(0)
+        configure(bc, a)
+        out.append(bc)
+    return out
+
+
 def traffic_probe(a) -> int:
     """Run under rocprofv3 --pmc by measure_traffic: the copy kernel of known traffic (1 GiB in,
     1 GiB out per dispatch), then four frames of the bench's workload."""
@@ -337,7 +362,7 @@ def traffic_probe(a) -> int:
     L.call("rrh_stream_copy", ctx.h, COPY_BYTES, 2, C.byref(g))
     sa = core.generate_scene(L.GEN_HEIGHTFIELD, seed=2, scale=1.0)
     scene = core.Scene(ctx, sa)
-    fr = Frame(ctx, scene, sa, a.width, a.height, a.max_bounces, 32, di_context(a))
+    fr = Frame(ctx, scene, sa, a.width, a.height, a.max_bounces, 32, di_context(a), band_contexts(a))
     for f in range(4):
         fr.run(f)
     fr.synchronize()
@@ -420,6 +445,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stack", choices=["private", "lds32", "hybrid16", "hybrid32", "none"], default="lds32",
                     help="where the traversal stack lives (context option; none: stackless)")
+    ap.add_argument("--pt-bands", type=int, default=3,
+                    help="the path tracer's image in this many row bands, each on its own context and stream "
+                         "(with the DI pass's, 3 bands fill the 4 hardware queues a process gets)")
     ap.add_argument("--di-stream", type=int, default=1,
                     help="1: the ReSTIR DI pass on a second context (its own stream), overlapping the path tracer")
     ap.add_argument("--trace-pairs", type=int, default=1,
@@ -479,18 +507,29 @@ def main() -> int:
           f"{a.bvh_builder} BVH build+upload {info['build_ms']:.1f} ms, scene total {time.perf_counter() - t0:.1f} s",
           file=sys.stderr, flush=True)
     W, H = a.width, a.height
-    fr = Frame(ctx, scene, sa, W, H, a.max_bounces, 32, di_context(a))
+    fr = Frame(ctx, scene, sa, W, H, a.max_bounces, 32, di_context(a), band_contexts(a))
     for f in range(a.warmup):
         fr.run(f)
     fr.synchronize()
     # instrumented traversal on one extra frame: algorithmic bytes per closest-hit ray
-    L.call("rrh_fetch_counts", ctx.h, C.byref(C.c_uint64()), C.byref(C.c_uint64()), C.byref(C.c_uint64()), 1)
+    ptc = fr.pt_contexts()
+
+    def summed(fn, k, reset=1):
+        """a counter query summed over the path tracer's contexts"""
+        tot = [0] * k
+        for c in ptc:
+            v = [C.c_uint64() for _ in range(k)]
+            L.call(fn, c.h, *[C.byref(x) for x in v], reset)
+            tot = [t + x.value for t, x in zip(tot, v)]
+        return [types.SimpleNamespace(value=t) for t in tot]
+
+    summed("rrh_fetch_counts", 3)
+    summed("rrh_fetch_counts_shadow", 3)
     fr.run(a.warmup, L.PT_COUNT_FETCHES)
-    nn, nt, nr = C.c_uint64(), C.c_uint64(), C.c_uint64()
-    L.call("rrh_fetch_counts", ctx.h, C.byref(nn), C.byref(nt), C.byref(nr), 1)
+    fr.synchronize()
+    nn, nt, nr = summed("rrh_fetch_counts", 3)
     bytes_per_ray = (nn.value * NODE_BYTES[a.bvh_nodes] + nt.value * TRI_BYTES) / max(nr.value, 1) + RAY_BYTES + HIT_BYTES
-    sn, st, sr = C.c_uint64(), C.c_uint64(), C.c_uint64()
-    L.call("rrh_fetch_counts_shadow", ctx.h, C.byref(sn), C.byref(st), C.byref(sr), 1)
+    sn, st, sr = summed("rrh_fetch_counts_shadow", 3)
     # a shadow ray reads its origin and direction records; its contribution record and pixel are
     # touched only when it is visible, and are left out (a lower bound)
     bytes_per_shadow = ((sn.value * NODE_BYTES[a.bvh_nodes] + st.value * TRI_BYTES) / sr.value + RAY_BYTES
@@ -500,23 +539,27 @@ def main() -> int:
           f"{sn.value / max(sr.value, 1):.1f} nodes + {st.value / max(sr.value, 1):.1f} tris", file=sys.stderr, flush=True)
     fr.synchronize()
     ray_totals(fr.contexts(), reset=True)
-    L.call("rrh_pt_traversal_rays", ctx.h, C.byref(C.c_uint64()), C.byref(C.c_uint64()), 1)
-    L.call("rrh_profile_enable", ctx.h, 1)
-    L.call("rrh_profile_traversal", ctx.h, C.byref(C.c_float()), C.byref(C.c_uint32()), 1)
+    summed("rrh_pt_traversal_rays", 2)
+    for c in ptc:
+        L.call("rrh_profile_enable", c.h, 1)
+        L.call("rrh_profile_traversal", c.h, C.byref(C.c_float()), C.byref(C.c_uint32()), 1)
     fr.synchronize()
     t0 = time.perf_counter()
     for k in range(a.steps):
         fr.run(a.warmup + 1 + k)
     fr.synchronize()
     dt = time.perf_counter() - t0
-    L.call("rrh_profile_enable", ctx.h, 0)
     prim, ext, shadow = ray_totals(fr.contexts(), reset=True)
-    trav_ms, launches = C.c_float(), C.c_uint32()
-    L.call("rrh_profile_traversal", ctx.h, C.byref(trav_ms), C.byref(launches), 1)
+    trav_ms, launches = types.SimpleNamespace(value=0.0), types.SimpleNamespace(value=0)
+    for c in ptc:
+        L.call("rrh_profile_enable", c.h, 0)
+        ms, nl = C.c_float(), C.c_uint32()
+        L.call("rrh_profile_traversal", c.h, C.byref(ms), C.byref(nl), 1)
+        trav_ms.value += ms.value
+        launches.value += nl.value
     # the path tracer's own closest-hit and shadow rays (the ReSTIR DI pass traces its primary
     # and shadow rays inside its own kernels, which the traversal events do not cover)
-    pt_closest, pt_shadow = C.c_uint64(), C.c_uint64()
-    L.call("rrh_pt_traversal_rays", ctx.h, C.byref(pt_closest), C.byref(pt_shadow), 1)
+    pt_closest, pt_shadow = summed("rrh_pt_traversal_rays", 2)
     rays = prim + ext + shadow
     value = rays / dt / 1e6
     avg_launch_ms = trav_ms.value / max(launches.value, 1)
@@ -542,7 +585,7 @@ def main() -> int:
                    "width": W, "height": H, "spp_per_frame": 1, "max_bounces": a.max_bounces,
                    "di_candidates": 32, "di_temporal_mcap": "20*M", "di_spatial": "5 neighbours, r=30",
                    "camera": "orbit 0.5 deg/frame", "parallelism": "single GPU",
-                   "traversal_stack": a.stack, "top_nodes_in_lds": a.top_nodes, "trace_pairs": a.trace_pairs, "di_pass_stream": "own context, overlapping the path tracer" if a.di_stream else "after the path tracer, same stream", "xcd_remap": a.xcd_remap, "bvh_nodes": a.bvh_nodes, "persistent": a.persistent, "speculative": a.speculative,
+                   "traversal_stack": a.stack, "top_nodes_in_lds": a.top_nodes, "trace_pairs": a.trace_pairs, "di_pass_stream": "own context, overlapping the path tracer" if a.di_stream else "after the path tracer, same stream", "pt_row_bands": a.pt_bands, "xcd_remap": a.xcd_remap, "bvh_nodes": a.bvh_nodes, "persistent": a.persistent, "speculative": a.speculative,
                    "bvh_max_leaf": a.max_leaf, "bvh_trav_cost": a.trav_cost, "bvh_builder": a.bvh_builder,
                    "bvh_build_ms": round(info["build_ms"], 1), "ray_sort_bits": a.ray_sort,
                    "rays_per_step": {"primary": prim // a.steps, "extension": ext // a.steps,

@@ -99,6 +99,11 @@ typedef struct rrh_mlp rrh_mlp;       /* learned guide */
 int rrh_ctx_create(int device, rrh_ctx** out);
 int rrh_ctx_destroy(rrh_ctx* ctx);
 int rrh_ctx_synchronize(rrh_ctx* ctx);
+/* no reference counterpart (the reference records every pass into one queue): work enqueued on
+ * `waiter` after this call starts only after everything enqueued on `signaler` before it has
+ * finished (an event recorded on the signaler's stream, waited for on the waiter's; the host
+ * does not block).  Both contexts on one device. */
+int rrh_ctx_join(rrh_ctx* waiter, rrh_ctx* signaler);
 const char* rrh_last_error(void);
 /* device properties used for roofline arithmetic: CU count, clock, name */
 int rrh_device_info(rrh_ctx* ctx, int* cu_count, int* clock_khz, char* name, size_t name_len);
@@ -217,6 +222,11 @@ typedef struct {
     uint32_t flags;                /* RRH_PT_* */
     uint32_t guide_bounces;        /* guided modes: the guide samples bounces [0, guide_bounces), the
                                       BSDF alone after them; 0 = every bounce */
+    uint32_t row_begin, row_end;   /* unguided frames: trace only the image rows [row_begin, row_end)
+                                      (0, 0 = every row); the other rows of the radiance image are left
+                                      as they are.  Every pixel's path depends only on its own seed, so
+                                      bands rendered by separate calls (or contexts) equal one call
+                                      over the whole frame bit for bit */
 } rrh_pt_params;
 #define RRH_PT_COUNT_FETCHES 1u    /* instrumented traversal: count node / triangle fetches (slower) */
 typedef struct { uint64_t primary, extension, shadow; } rrh_ray_counts;

@@ -75,7 +75,8 @@ class SceneDesc(C.Structure):
 class PTParams(C.Structure):
     _fields_ = [("max_bounces", C.c_uint32), ("frame_seed", C.c_uint32), ("rr_start", C.c_uint32),
                 ("guide_mode", C.c_uint32), ("guide_alpha", C.c_float), ("record_vertices", C.c_uint32),
-                ("flags", C.c_uint32), ("guide_bounces", C.c_uint32)]
+                ("flags", C.c_uint32), ("guide_bounces", C.c_uint32), ("row_begin", C.c_uint32),
+                ("row_end", C.c_uint32)]
 
 
 class RayCounts(C.Structure):
@@ -133,6 +134,7 @@ SIGNATURES = {
     "rrh_ctx_create": [i32, PP],
     "rrh_ctx_destroy": [P],
     "rrh_ctx_synchronize": [P],
+    "rrh_ctx_join": [P, P],
     "rrh_device_info": [P, C.POINTER(i32), C.POINTER(i32), C.c_char_p, sz],
     "rrh_marker_record": [P, i32],
     "rrh_marker_elapsed": [P, i32, i32, C.POINTER(f32)],

@@ -97,6 +97,8 @@ int rrh_ctx_create(int device, rrh_ctx** out) {
         e = hipEventCreate(&c->markers[i]);
         if (e != hipSuccess) { set_error(hipGetErrorString(e)); return RRH_E_HIP; }
     }
+    e = hipEventCreateWithFlags(&c->join_event, hipEventDisableTiming);
+    if (e != hipSuccess) { set_error(hipGetErrorString(e)); return RRH_E_HIP; }
     // device-side ray totals and fetch counters exist from the start, so no frame allocates them
     // (totals: primary, extension, shadow of every pass, then the path tracer's closest-hit and
     // shadow rays; fetch: closest-hit nodes, triangles, rays, then the same for shadow rays)
@@ -122,6 +124,7 @@ int rrh_ctx_destroy(rrh_ctx* c) {
     c->sort_keys.release(); c->sort_perm.release(); c->sort_tmp.release();
     for (hipEvent_t e : c->prof_events) (void)hipEventDestroy(e);
     for (int i = 0; i < 16; ++i) (void)hipEventDestroy(c->markers[i]);
+    if (c->join_event) (void)hipEventDestroy(c->join_event);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return RRH_OK;
@@ -130,6 +133,16 @@ int rrh_ctx_destroy(rrh_ctx* c) {
 int rrh_ctx_synchronize(rrh_ctx* c) {
     RRH_ARG(c, "null context");
     RRH_HIP(hipStreamSynchronize(c->stream));
+    return RRH_OK;
+}
+
+int rrh_ctx_join(rrh_ctx* waiter, rrh_ctx* signaler) {
+    RRH_ARG(waiter && signaler, "rrh_ctx_join: null context");
+    RRH_ARG(waiter->device == signaler->device, "rrh_ctx_join: contexts on different devices");
+    if (waiter == signaler) return RRH_OK;
+    // the wait takes the event's state now; recording it again later does not affect it
+    RRH_HIP(hipEventRecord(signaler->join_event, signaler->stream));
+    RRH_HIP(hipStreamWaitEvent(waiter->stream, signaler->join_event, 0));
     return RRH_OK;
 }
 

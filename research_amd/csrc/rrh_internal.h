@@ -103,6 +103,7 @@ struct rrh_ctx {
     rrh::DevBuf sort_keys, sort_perm, sort_tmp;  // ray sort: keys / slots (x2 each), rocPRIM scratch
     hipStream_t stream = nullptr;
     hipEvent_t markers[16] = {};
+    hipEvent_t join_event = nullptr;  // rrh_ctx_join: recorded on this context's stream
     // path-tracer wavefront state (sized for the largest frame seen)
     rrh::DevBuf q_o[2], q_d[2], q_thp[2];
     rrh::DevBuf hits, sh_o, sh_d, sh_c;

@@ -62,19 +62,22 @@ __global__ void k_zero_u32(uint32_t* __restrict__ p, uint32_t n) {
 
 // one 8x8 pixel tile per wave; the queue is filled in tile order (no atomics): for an
 // image whose sides are multiples of 8 the slot is tile * 64 + lane, otherwise row-major
+// the primary rays of image rows [r0, r0 + rows), 8x8 tiles in queue order
 __global__ void __launch_bounds__(64) k_raygen(DevScene S, Q out, uint32_t* __restrict__ cnt, float4* __restrict__ rad,
-                                               uint32_t frame_seed) {
+                                               uint32_t frame_seed, uint32_t r0, uint32_t rows) {
     const rrh_camera& cam = *S.cam;
-    uint32_t W = cam.screen_width, H = cam.screen_height;
+    uint32_t W = cam.screen_width;
     uint32_t px = blockIdx.x * 8 + (threadIdx.x & 7u);
-    uint32_t py = blockIdx.y * 8 + (threadIdx.x >> 3);
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) cnt[0] = W * H;
-    if (px >= W || py >= H) return;
+    uint32_t ly = blockIdx.y * 8 + (threadIdx.x >> 3);  // row within the band
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) cnt[0] = W * rows;
+    if (px >= W || ly >= rows) return;
+    const uint32_t py = r0 + ly;
     uint32_t seed = make_seed(px, py, frame_seed);
     f3 d = primary_dir(cam, px, py, seed);
     uint32_t pix = py * W + px;
     rad[pix] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-    uint32_t slot = ((W & 7u) == 0u && (H & 7u) == 0u) ? (blockIdx.y * gridDim.x + blockIdx.x) * 64u + threadIdx.x : pix;
+    uint32_t slot = ((W & 7u) == 0u && (rows & 7u) == 0u) ? (blockIdx.y * gridDim.x + blockIdx.x) * 64u + threadIdx.x
+                                                         : ly * W + px;
     out.o[slot] = make_float4(cam.position[0], cam.position[1], cam.position[2], __uint_as_float(pix));
     out.d[slot] = make_float4(d.x, d.y, d.z, __uint_as_float(seed));
     out.thp[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
@@ -659,7 +662,13 @@ static int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, 
     RRH_ARG(radiance->w == W && radiance->h == H, "rrh_pt_render: image extent differs from the camera");
     RRH_ARG(p->max_bounces >= 1 && p->max_bounces <= kMaxBounces, "rrh_pt_render: max_bounces must be in [1, 32]");
     RRH_ARG(p->guide_mode <= RRH_GUIDE_MLP, "rrh_pt_render: bad guide_mode");
-    uint32_t npix = W * H;
+    const bool band = p->row_begin != 0 || p->row_end != 0;
+    const uint32_t r0 = band ? p->row_begin : 0u, r1 = band ? p->row_end : H;
+    RRH_ARG(r0 < r1 && r1 <= H, "rrh_pt_render: row band outside the image (row_begin < row_end <= height)");
+    RRH_ARG(!band || (p->guide_mode == RRH_GUIDE_NONE && p->record_vertices == 0),
+            "rrh_pt_render: a row band is for unguided frames without vertex recording");
+    // every per-ray buffer is sized for the band's pixels
+    uint32_t npix = W * (r1 - r0);
     int rc = 0;
     Bufs Bf;
     if (fixed) Bf = *fixed;
@@ -681,8 +690,8 @@ static int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, 
     }
     float4* rad = (float4*)radiance->p;
     Q* q = Bf.q;
-    hipLaunchKernelGGL(k_raygen, dim3((W + 7) / 8, (H + 7) / 8), dim3(64), 0, c->stream, S, q[0], cnt, rad,
-                       p->frame_seed);
+    hipLaunchKernelGGL(k_raygen, dim3((W + 7) / 8, (r1 - r0 + 7) / 8), dim3(64), 0, c->stream, S, q[0], cnt, rad,
+                       p->frame_seed, r0, r1 - r0);
     RRH_LAUNCH_CHECK();
     uint32_t g = grid_for(npix);
     const uint32_t gt = std::min((npix + kTraceBlock - 1) / kTraceBlock, kTraceGridCap);

@@ -144,3 +144,46 @@ def test_traversal_options_are_checked(ctx):
         L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAVERSAL_STACK, 5)
     with pytest.raises(L.RRHError, match="top nodes"):
         L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAV_TOP_NODES, 513)
+
+
+def test_row_bands_equal_the_whole_frame(ctx):
+    """Rows rendered in bands, by one context or by two whose streams are joined, equal one
+    call over the whole frame bit for bit; the ray counts add up (row_begin / row_end)."""
+    sa = core.generate_scene(L.GEN_HEIGHTFIELD, 2, 0.25)
+    W, H = 160, 90
+    gs = core.Scene(ctx, sa)
+    cam = core.camera_look_at(sa.eye, sa.center, sa.fov, W, H)
+    gs.set_camera(cam)
+    ctx2 = core.Context(0)
+    gs2 = core.Scene(ctx2, sa)
+    gs2.set_camera(cam)
+    for frame in (0, 5):
+        full, counts = render_gpu(ctx, gs, cam, 6, frame)
+        img = core.Image(ctx, W, H)
+        tot = [0, 0, 0]
+        for r0, r1 in ((0, 37), (37, 64), (64, 90)):     # ragged bands, not multiples of 8
+            p = L.PTParams(6, frame, 3, 0, 0.0, 0, 0, 0, r0, r1)
+            L.call("rrh_pt_render", ctx.h, gs.h, C.byref(p), img.h)
+            rc = L.RayCounts()
+            L.call("rrh_pt_ray_counts", ctx.h, C.byref(rc))
+            tot = [tot[0] + rc.primary, tot[1] + rc.extension, tot[2] + rc.shadow]
+        assert tuple(tot) == counts
+        assert np.array_equal(img.download().reshape(-1, 4).view(np.uint32), full.view(np.uint32))
+        # two contexts: the top half on ctx2, joined into ctx before the download
+        img2 = core.Image(ctx, W, H)
+        L.call("rrh_pt_render", ctx2.h, gs2.h, C.byref(L.PTParams(6, frame, 3, 0, 0.0, 0, 0, 0, 0, 48)), img2.h)
+        L.call("rrh_pt_render", ctx.h, gs.h, C.byref(L.PTParams(6, frame, 3, 0, 0.0, 0, 0, 0, 48, 90)), img2.h)
+        L.call("rrh_ctx_join", ctx.h, ctx2.h)
+        assert np.array_equal(img2.download().reshape(-1, 4).view(np.uint32), full.view(np.uint32))
+
+
+def test_row_band_arguments_are_checked(ctx):
+    sa = core.generate_scene(L.GEN_CORNELL_REF)
+    gs = core.Scene(ctx, sa)
+    gs.set_camera(core.camera_look_at(sa.eye, sa.center, sa.fov, 32, 32))
+    img = core.Image(ctx, 32, 32)
+    for r0, r1 in ((8, 8), (9, 4), (0, 33)):
+        with pytest.raises(L.RRHError, match="row band"):
+            L.call("rrh_pt_render", ctx.h, gs.h, C.byref(L.PTParams(5, 0, 3, 0, 0.0, 0, 0, 0, r0, r1)), img.h)
+    with pytest.raises(L.RRHError, match="unguided"):
+        L.call("rrh_pt_render", ctx.h, gs.h, C.byref(L.PTParams(5, 0, 3, 0, 0.0, 1000, 0, 0, 0, 16)), img.h)
