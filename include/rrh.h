@@ -304,6 +304,13 @@ int rrh_ctx_set_option(rrh_ctx* ctx, int option, int value);
 int rrh_stream_copy(rrh_ctx* ctx, size_t bytes, int iterations, float* gbps);
 int rrh_profile_enable(rrh_ctx* ctx, int on);
 int rrh_profile_traversal(rrh_ctx* ctx, float* total_ms, uint32_t* launches, int reset);
+/* time origin for rrh_profile_spans: an event recorded on this context's stream (no reference
+ * counterpart; measurement only) */
+int rrh_profile_mark(rrh_ctx* ctx);
+/* [begin, end] in ms after `origin`'s mark of each traversal launch profiled on `ctx` since the
+ * last reset of rrh_profile_traversal (call before that reset); cap: slots in each array */
+int rrh_profile_spans(rrh_ctx* ctx, const rrh_ctx* origin, float* begin_ms, float* end_ms, uint32_t cap,
+                      uint32_t* n);
 
 /* ---- accumulate: AccumulatePass::execute (src/passes/accumulate/
  *      AccumulatePass.cpp:167-218) + accumulate.slang:21-46 -------------- */

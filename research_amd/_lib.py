@@ -174,6 +174,8 @@ SIGNATURES = {
     "rrh_stream_copy": [P, sz, i32, C.POINTER(f32)],
     "rrh_ctx_set_option": [P, i32, i32],
     "rrh_profile_traversal": [P, C.POINTER(f32), C.POINTER(u32), i32],
+    "rrh_profile_mark": [P],
+    "rrh_profile_spans": [P, P, C.POINTER(f32), C.POINTER(f32), u32, C.POINTER(u32)],
     "rrh_accumulate": [P, P, P, u32, u32],
     "rrh_accumulate_f64": [P, P, P, u32, u32],
     "rrh_di_initial": [P, P, C.POINTER(DIParams), P, P, P, P],

@@ -123,6 +123,7 @@ int rrh_ctx_destroy(rrh_ctx* c) {
     c->train.release(); c->train_cnt.release(); c->fin_chunks.release(); c->fin_tmp.release(); c->totals.release(); c->fetch.release();
     c->sort_keys.release(); c->sort_perm.release(); c->sort_tmp.release();
     for (hipEvent_t e : c->prof_events) (void)hipEventDestroy(e);
+    if (c->prof_mark) (void)hipEventDestroy(c->prof_mark);
     for (int i = 0; i < 16; ++i) (void)hipEventDestroy(c->markers[i]);
     if (c->join_event) (void)hipEventDestroy(c->join_event);
     (void)hipStreamDestroy(c->stream);

@@ -135,6 +135,7 @@ struct rrh_ctx {
     int bvh_nodes = RRH_NODES_WIDE4Q;  // RRH_OPT_BVH_NODES (quantized 4-wide: fastest measured)
     std::vector<hipEvent_t> prof_events;   // pairs around each traversal launch
     uint32_t prof_used = 0;
+    hipEvent_t prof_mark = nullptr;       // rrh_profile_mark: time origin of rrh_profile_spans
     rrh::DevBuf scratch;          // small general-purpose scratch
     rrh::DevBuf mlp_prep;         // learned guide: per queue slot (pos, hit flag) (normal) (wo)
 };

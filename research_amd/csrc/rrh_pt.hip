@@ -939,6 +939,32 @@ int rrh_profile_traversal(rrh_ctx* c, float* total_ms, uint32_t* launches, int r
     return RRH_OK;
 }
 
+int rrh_profile_mark(rrh_ctx* c) {
+    RRH_ARG(c, "rrh_profile_mark: null context");
+    if (!c->prof_mark) RRH_HIP(hipEventCreate(&c->prof_mark));
+    RRH_HIP(hipEventRecord(c->prof_mark, c->stream));
+    return RRH_OK;
+}
+
+// [begin, end] of every profiled traversal launch of `c` in ms after `origin`'s mark; events of
+// one device share a clock, so spans of contexts on different streams compare directly
+int rrh_profile_spans(rrh_ctx* c, const rrh_ctx* origin, float* begin_ms, float* end_ms, uint32_t cap,
+                      uint32_t* n) {
+    RRH_ARG(c && origin && begin_ms && end_ms && n, "rrh_profile_spans: null argument");
+    RRH_ARG(origin->prof_mark, "rrh_profile_spans: the origin context has no mark (rrh_profile_mark)");
+    RRH_ARG(origin->device == c->device, "rrh_profile_spans: contexts on different devices");
+    const uint32_t k = c->prof_used / 2;
+    *n = k;
+    RRH_ARG(cap >= k, "rrh_profile_spans: fewer slots than profiled launches");
+    if (c->prof_used) RRH_HIP(hipEventSynchronize(c->prof_events[c->prof_used - 1]));
+    RRH_HIP(hipEventSynchronize(origin->prof_mark));
+    for (uint32_t i = 0; i < k; ++i) {
+        RRH_HIP(hipEventElapsedTime(&begin_ms[i], origin->prof_mark, c->prof_events[2 * i]));
+        RRH_HIP(hipEventElapsedTime(&end_ms[i], origin->prof_mark, c->prof_events[2 * i + 1]));
+    }
+    return RRH_OK;
+}
+
 int rrh_accumulate(rrh_ctx* c, const rrh_image* radiance, rrh_image* acc, uint32_t camera_moved, uint32_t spp) {
     RRH_ARG(c && radiance && acc, "rrh_accumulate: null argument");
     RRH_ARG(radiance->fmt == RRH_FMT_RGBA32F && acc->fmt == RRH_FMT_RGBA32F, "rrh_accumulate: RGBA32F images required");
