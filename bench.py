@@ -326,7 +326,7 @@ def restir_gi_summary(ctx: core.Context, scene, sa, W: int, H: int, frames: int,
 
 
 # the options a traffic probe must share with the timed run (same kernels, same tree)
-PROBE_OPTS = ["stack", "top_nodes", "trace_pairs", "di_stream", "pt_bands", "xcd_remap", "persistent", "max_leaf", "trav_cost", "speculative", "bvh_builder",
+PROBE_OPTS = ["stack", "top_nodes", "trace_pairs", "di_stream", "pt_bands", "xcd_remap", "persistent", "refill", "max_leaf", "trav_cost", "speculative", "bvh_builder",
               "ploc_radius", "ray_sort", "bvh_nodes", "width", "height", "max_bounces"]
 COPY_BYTES = 1 << 30
 
@@ -340,6 +340,7 @@ def configure(ctx, a) -> None:
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_XCD_REMAP, a.xcd_remap)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_NODES, NODE_FORMATS[a.bvh_nodes])
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_PERSISTENT, a.persistent)
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_REFILL, a.refill)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_SPECULATIVE, a.speculative)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_MAX_LEAF, a.max_leaf)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_TRAV_COST, int(round(a.trav_cost * 100)))
@@ -470,7 +471,9 @@ def main() -> int:
     ap.add_argument("--top-nodes", type=int, default=0,
                     help="top-of-tree quantized nodes each workgroup reads from LDS (0..512)")
     ap.add_argument("--xcd-remap", type=int, default=0, help="1: runs of 16 neighbouring workgroups per XCD")
-    ap.add_argument("--persistent", type=int, default=0, help="1: persistent queue traversal with lane refill")
+    ap.add_argument("--persistent", type=int, default=0,
+                    help="1: persistent queue traversal with lane refill; 2..64: at most that many waves per CU")
+    ap.add_argument("--refill", type=int, default=16, help="persistent waves take new rays at this many idle lanes")
     ap.add_argument("--max-leaf", type=int, default=8, help="largest BVH leaf (1..8)")
     ap.add_argument("--trav-cost", type=float, default=1.0, help="SAH traversal cost (triangle tests)")
     ap.add_argument("--speculative", type=int, default=1, help="1: speculative (leaf-parking) binary traversal")
@@ -610,7 +613,7 @@ def main() -> int:
                    "width": W, "height": H, "spp_per_frame": 1, "max_bounces": a.max_bounces,
                    "di_candidates": 32, "di_temporal_mcap": "20*M", "di_spatial": "5 neighbours, r=30",
                    "camera": "orbit 0.5 deg/frame", "parallelism": "single GPU",
-                   "traversal_stack": a.stack, "top_nodes_in_lds": a.top_nodes, "trace_pairs": a.trace_pairs, "di_pass_stream": "own context, overlapping the path tracer" if a.di_stream else "after the path tracer, same stream", "pt_row_bands": a.pt_bands, "xcd_remap": a.xcd_remap, "bvh_nodes": a.bvh_nodes, "persistent": a.persistent, "speculative": a.speculative,
+                   "traversal_stack": a.stack, "top_nodes_in_lds": a.top_nodes, "trace_pairs": a.trace_pairs, "di_pass_stream": "own context, overlapping the path tracer" if a.di_stream else "after the path tracer, same stream", "pt_row_bands": a.pt_bands, "xcd_remap": a.xcd_remap, "bvh_nodes": a.bvh_nodes, "persistent": a.persistent, "refill": a.refill, "speculative": a.speculative,
                    "bvh_max_leaf": a.max_leaf, "bvh_trav_cost": a.trav_cost, "bvh_builder": a.bvh_builder,
                    "bvh_build_ms": round(info["build_ms"], 1), "ray_sort_bits": a.ray_sort,
                    "rays_per_step": {"primary": prim // a.steps, "extension": ext // a.steps,

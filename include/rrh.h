@@ -263,8 +263,13 @@ int rrh_pt_traversal_rays(rrh_ctx* ctx, uint64_t* closest, uint64_t* shadow, int
 /* tuning options of a context (no reference counterpart) */
 #define RRH_OPT_TRAVERSAL_STACK 1   /* RRH_STACK_* : where the BVH traversal stack lives */
 #define RRH_OPT_XCD_REMAP 2         /* 1: give each XCD runs of 16 consecutive workgroups (rotating) */
-#define RRH_OPT_PERSISTENT 4        /* 1: queue traversal by persistent waves that refill finished
-                                       lanes with new rays (binary nodes); 0 (default): grid-stride */
+#define RRH_OPT_PERSISTENT 4        /* 1: the path tracer's ray queues traced by persistent waves that
+                                       hand finished lanes new rays (binary nodes, and quantized 4-wide
+                                       nodes with an LDS or private stack, where one launch drains a
+                                       bounce's shadow and extension rays); 2..64: the same with at most
+                                       that many waves per CU; 0 (default): one ray per lane */
+#define RRH_OPT_REFILL 14           /* persistent 4-wide traversal: a wave takes new rays once this many
+                                       of its 64 lanes are idle, 1..64 (default 16) */
 #define RRH_OPT_SPECULATIVE 5       /* 1 (default): traversal parks leaves until the whole wave has one */
 #define RRH_OPT_BVH_MAX_LEAF 6      /* largest BVH leaf, 1..8 (default 8); applies to later scene uploads */
 #define RRH_OPT_BVH_TRAV_COST 7     /* SAH cost of a traversal step x 100 (default 100); later uploads */

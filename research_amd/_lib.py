@@ -39,7 +39,7 @@ PT_COUNT_FETCHES = 1
 OPT_TRAVERSAL_STACK, OPT_XCD_REMAP, OPT_BVH_NODES, OPT_PERSISTENT, OPT_SPECULATIVE = 1, 2, 3, 4, 5
 OPT_BVH_MAX_LEAF, OPT_BVH_TRAV_COST = 6, 7
 OPT_BVH_BUILDER, OPT_PLOC_RADIUS, OPT_RAY_SORT, OPT_RESTIR_CULL = 8, 9, 10, 11
-OPT_TRAV_TOP_NODES, OPT_TRACE_PAIRS = 12, 13
+OPT_TRAV_TOP_NODES, OPT_TRACE_PAIRS, OPT_REFILL = 12, 13, 14
 TRACE_PAIRS_DEFAULT = 1  # a new context's RRH_OPT_TRACE_PAIRS
 BVH_BUILD_HOST, BVH_BUILD_DEVICE = 0, 1
 UPDATE_REFIT, UPDATE_REBUILD = 0, 1

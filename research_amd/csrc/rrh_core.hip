@@ -172,7 +172,17 @@ int rrh_ctx_set_option(rrh_ctx* c, int option, int value) {
     if (option == RRH_OPT_XCD_REMAP) { c->xcd_remap = value != 0; return RRH_OK; }
     if (option == RRH_OPT_RESTIR_CULL) { c->restir_cull = value != 0; return RRH_OK; }
     if (option == RRH_OPT_TRACE_PAIRS) { c->trace_pairs = value != 0; return RRH_OK; }
-    if (option == RRH_OPT_PERSISTENT) { c->persistent = value != 0; return RRH_OK; }
+    if (option == RRH_OPT_PERSISTENT) {
+        RRH_ARG(value >= 0 && value <= 64, "rrh_ctx_set_option: persistent waves per CU is 0 (off), 1 (all the CU holds) or 2..64");
+        c->persistent = value != 0;
+        c->persistent_waves = value > 1 ? (uint32_t)value : 1u;
+        return RRH_OK;
+    }
+    if (option == RRH_OPT_REFILL) {
+        RRH_ARG(value >= 1 && value <= 64, "rrh_ctx_set_option: refill threshold is 1..64 idle lanes");
+        c->refill = (uint32_t)value;
+        return RRH_OK;
+    }
     if (option == RRH_OPT_SPECULATIVE) { c->speculative = value != 0; return RRH_OK; }
     if (option == RRH_OPT_BVH_MAX_LEAF) {
         RRH_ARG(value >= 1 && value <= 8, "rrh_ctx_set_option: BVH leaf size is 1..8");

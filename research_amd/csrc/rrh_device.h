@@ -579,9 +579,11 @@ RRH_D void cswap(float& ka, int& va, float& kb, int& vb) {
 // a 4-wide node as child boxes (fp32 planes) + links; Q: decoded from the quantized form
 // (origin + q * 2^(E-127) per bound, the exact fp32 expression the builder checked)
 struct Node4 { float4 xl, xh, yl, yh, zl, zh; int4 lk; };
+// q * s is exact (an 8-bit integer times a power of two), so the fused form rounds exactly like
+// o + q * s with a separate multiply and add
 RRH_D float4 dequant4(float o, uint32_t q, float s) {
-    return make_float4(o + (float)(q & 255u) * s, o + (float)((q >> 8) & 255u) * s, o + (float)((q >> 16) & 255u) * s,
-                       o + (float)(q >> 24) * s);
+    return make_float4(__builtin_fmaf((float)(q & 255u), s, o), __builtin_fmaf((float)((q >> 8) & 255u), s, o),
+                       __builtin_fmaf((float)((q >> 16) & 255u), s, o), __builtin_fmaf((float)(q >> 24), s, o));
 }
 // quantized node words: origin (3 floats) and per-axis exponents, then the children's
 // 8-bit bounds, then the four links
@@ -680,8 +682,8 @@ RRH_D Hit trace_closest4_s(const DevScene& S, f3 o, f3 d, float tmin, float tmax
             for (int i = 0; i < 4; ++i) {
                 float tn;
                 const int l = i4c(lk, i);
-                const bool hit = (l != 0) && box_hit(f4c(xl, i), f4c(xh, i), f4c(yl, i), f4c(yh, i), f4c(zl, i),
-                                                     f4c(zh, i), r, tmin, h.t, tn);
+                const bool hit = box_hit(f4c(xl, i), f4c(xh, i), f4c(yl, i), f4c(yh, i), f4c(zl, i),
+                                                     f4c(zh, i), r, tmin, h.t, tn) & (l != 0);
                 k[i] = hit ? tn : INFINITY;
                 c[i] = l;
             }
@@ -741,8 +743,8 @@ RRH_D bool trace_occluded4_s(const DevScene& S, f3 o, f3 d, float tmin, float tm
             for (int i = 0; i < 4; ++i) {
                 float tn;
                 const int l = i4c(lk, i);
-                if ((l != 0) && box_hit(f4c(xl, i), f4c(xh, i), f4c(yl, i), f4c(yh, i), f4c(zl, i), f4c(zh, i), r,
-                                        tmin, tmax, tn)) {
+                if (box_hit(f4c(xl, i), f4c(xh, i), f4c(yl, i), f4c(yh, i), f4c(zl, i), f4c(zh, i), r,
+                                        tmin, tmax, tn) & (l != 0)) {
                     if (have) stack.put(sp++, next);
                     next = l;
                     have = true;
@@ -792,8 +794,8 @@ RRH_D Hit trace_closest4spec_s(const DevScene& S, f3 o, f3 d, float tmin, float 
             for (int i = 0; i < 4; ++i) {
                 float tn;
                 const int l = i4c(N.lk, i);
-                const bool hit = (l != 0) && box_hit(f4c(N.xl, i), f4c(N.xh, i), f4c(N.yl, i), f4c(N.yh, i),
-                                                     f4c(N.zl, i), f4c(N.zh, i), r, tmin, h.t, tn);
+                const bool hit = box_hit(f4c(N.xl, i), f4c(N.xh, i), f4c(N.yl, i), f4c(N.yh, i),
+                                                     f4c(N.zl, i), f4c(N.zh, i), r, tmin, h.t, tn) & (l != 0);
                 k[i] = hit ? tn : INFINITY;
                 c[i] = l;
             }
@@ -856,8 +858,8 @@ RRH_D bool trace_occluded4spec_s(const DevScene& S, f3 o, f3 d, float tmin, floa
             for (int i = 0; i < 4; ++i) {
                 float tn;
                 const int l = i4c(N.lk, i);
-                if ((l != 0) && box_hit(f4c(N.xl, i), f4c(N.xh, i), f4c(N.yl, i), f4c(N.yh, i), f4c(N.zl, i),
-                                        f4c(N.zh, i), r, tmin, tmax, tn)) {
+                if (box_hit(f4c(N.xl, i), f4c(N.xh, i), f4c(N.yl, i), f4c(N.yh, i), f4c(N.zl, i),
+                                        f4c(N.zh, i), r, tmin, tmax, tn) & (l != 0)) {
                     if (have) stack.put(sp++, next);
                     next = l;
                     have = true;
@@ -948,8 +950,8 @@ RRH_D Hit trace_closest4sl_s(const DevScene& S, f3 o, f3 d, float tmin, float tm
             for (int i = 0; i < 4; ++i) {
                 float tn;
                 const int l = i4c(N.lk, i);
-                const bool hit = (l != 0) && box_hit(f4c(N.xl, i), f4c(N.xh, i), f4c(N.yl, i), f4c(N.yh, i),
-                                                     f4c(N.zl, i), f4c(N.zh, i), r, tmin, h.t, tn);
+                const bool hit = box_hit(f4c(N.xl, i), f4c(N.xh, i), f4c(N.yl, i), f4c(N.yh, i),
+                                                     f4c(N.zl, i), f4c(N.zh, i), r, tmin, h.t, tn) & (l != 0);
                 k[i] = hit ? tn : INFINITY;
                 c[i] = i;  // slots, sorted by entry distance with their keys
             }
@@ -1013,8 +1015,8 @@ RRH_D bool trace_occluded4sl_s(const DevScene& S, f3 o, f3 d, float tmin, float 
             for (int i = 0; i < 4; ++i) {
                 float tn;
                 const int l = i4c(N.lk, i);
-                if ((l != 0) && box_hit(f4c(N.xl, i), f4c(N.xh, i), f4c(N.yl, i), f4c(N.yh, i), f4c(N.zl, i),
-                                        f4c(N.zh, i), r, tmin, tmax, tn)) {
+                if (box_hit(f4c(N.xl, i), f4c(N.xh, i), f4c(N.yl, i), f4c(N.yh, i), f4c(N.zl, i),
+                                        f4c(N.zh, i), r, tmin, tmax, tn) & (l != 0)) {
                     if (first < 0) {
                         first = i;
                     } else {

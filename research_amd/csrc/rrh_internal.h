@@ -94,6 +94,8 @@ struct rrh_ctx {
     int device = 0;
     int cu_count = 256;            // compute units of the device (persistent grids)
     bool persistent = false;       // RRH_OPT_PERSISTENT: ray queues traced by persistent waves
+    uint32_t persistent_waves = 1; // RRH_OPT_PERSISTENT's value: 1 = as many waves as the CUs hold, N > 1 caps them
+    uint32_t refill = 16;          // RRH_OPT_REFILL: idle lanes at which a persistent wave takes new rays
     bool speculative = true;       // RRH_OPT_SPECULATIVE: traversal parks leaves (default: measured faster)
     uint32_t bvh_max_leaf = 8;     // RRH_OPT_BVH_MAX_LEAF: applies to later scene uploads
     uint32_t bvh_trav_cost = 100;  // RRH_OPT_BVH_TRAV_COST: SAH traversal cost x 100 (later uploads)

@@ -30,7 +30,8 @@ constexpr uint32_t kTraceBlock = 64;
 constexpr uint32_t kTraceGridCap = 32768;
 
 // counters layout (uint32): [0, 64) ray-queue size per bounce, [64, 128) shadow rays per bounce,
-// [128, 192) / [192, 256) next queue index handed out by the persistent intersect / shadow waves
+// [128, 192) / [192, 256) next queue slot handed out by the persistent waves of bounce b's
+// closest-hit launch (with its paired shadow queue) / of a launch of bounce b's shadow rays alone
 struct Q {
     float4 *o, *d, *thp;
 };
@@ -474,6 +475,188 @@ int launch_pair(rrh_ctx* c, const DevScene& S, Q& qi, uint32_t* cnt, uint32_t b,
     return RRH_OK;
 }
 
+// Persistent traversal over the quantized 4-wide tree.  One launch drains a virtual queue: the
+// shadow rays of bounce b_sh (slots [0, n_sh)), then the extension rays of bounce b_ext (slots
+// [n_sh, n_sh + n_ext)); kNoQueue leaves a part out.  A wave keeps its lanes busy: whenever at
+// least `refill` of them are idle (at the start all 64) it hands them the next slots with one
+// atomic, so a lane whose ray ends early takes another instead of idling until the slowest ray
+// of its wave is done.  Lanes descend speculatively (a lane that reaches a leaf parks it until
+// every lane holds one or is done) and test the parked leaves together; a shadow ray stops at its
+// first hit.  The closest hit does not depend on the visiting order and a shadow ray's answer only
+// on the leaves its fixed extent reaches, and every pixel has at most one shadow ray per bounce,
+// so hit records and images equal those of k_trace_pair bit for bit.
+// Exit: `refill` is 1..64, so a wave whose lanes are all idle always asks for slots, and once the
+// queue has none left (`drained`, wave-uniform) it ends as soon as its last lane finishes.
+constexpr uint32_t kNoQueue = ~0u;
+template <int STK>
+__global__ void __launch_bounds__(kTraceBlock) k_trace_queue_p(DevScene S, Q in, const uint32_t* __restrict__ cnt,
+                                                               uint32_t b_ext, uint32_t b_sh, float4* __restrict__ hits,
+                                                               const float4* __restrict__ sh_o,
+                                                               const float4* __restrict__ sh_d,
+                                                               const float4* __restrict__ sh_c, float4* __restrict__ rad,
+                                                               VertexRec rec, uint32_t* __restrict__ next,
+                                                               uint32_t refill) {
+    __shared__ int s_stack[StackSel<STK>::kLds * 64];
+    auto st = make_stack<STK>(s_stack);
+    const uint32_t n_sh = (b_sh != kNoQueue) ? cnt[64 + b_sh] : 0u;
+    const uint32_t n = n_sh + ((b_ext != kNoQueue) ? cnt[b_ext] : 0u);
+    const float tmin = 0.001f;
+    WRay r = wray_setup(mk(0, 0, 0), mk(0, 0, 1));
+    float t = 0.0f, b1 = 0.0f, b2 = 0.0f;
+    uint32_t prim = ~0u, slot = 0u;
+    int node = kTravDone, leaf = 0, sp = 0;
+    bool active = false, drained = false, shadow = false;
+    while (true) {
+        const bool idle = !active;
+        if (!drained && __popcll(__ballot(idle ? 1 : 0)) >= (int)refill) {
+            uint32_t my = 0;
+            if (wave_refill(next, n, idle, drained, my)) {
+                slot = my;
+                active = true;
+                shadow = my < n_sh;
+                float4 o, d;
+                if (shadow) {
+                    o = sh_o[my];
+                    d = sh_d[my];
+                    t = o.w;
+                } else {
+                    o = in.o[my - n_sh];
+                    d = in.d[my - n_sh];
+                    t = 1e9f;
+                }
+                r = wray_setup(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z));
+                prim = ~0u;
+                b1 = 0.0f;
+                b2 = 0.0f;
+                sp = 0;
+                leaf = 0;
+                node = (shadow && !(t >= tmin)) ? kTravDone : 0;  // an empty extent: never occluded
+            }
+        }
+        if (__ballot(active ? 1 : 0) == 0ull) {
+            if (drained) break;
+            continue;
+        }
+        if (!active) continue;
+        // speculative descent: inner nodes until every lane in the loop holds a leaf
+        while ((uint32_t)node < (uint32_t)kTravDone) {
+            Node4 N;
+            load_node4<true>(S, node, N);
+            float k[4];
+            int c[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float tn;
+                const int l = i4c(N.lk, i);
+                const bool hit = box_hit(f4c(N.xl, i), f4c(N.xh, i), f4c(N.yl, i), f4c(N.yh, i),
+                                                     f4c(N.zl, i), f4c(N.zh, i), r, tmin, t, tn) & (l != 0);
+                k[i] = hit ? tn : INFINITY;
+                c[i] = l;
+            }
+            cswap(k[0], c[0], k[1], c[1]);
+            cswap(k[2], c[2], k[3], c[3]);
+            cswap(k[0], c[0], k[2], c[2]);
+            cswap(k[1], c[1], k[3], c[3]);
+            cswap(k[1], c[1], k[2], c[2]);
+            if (!(k[0] < INFINITY)) {
+                node = (sp == 0) ? kTravDone : st.get(--sp);
+            } else {
+                if (k[3] < INFINITY) st.put(sp++, c[3]);
+                if (k[2] < INFINITY) st.put(sp++, c[2]);
+                if (k[1] < INFINITY) st.put(sp++, c[1]);
+                node = c[0];
+            }
+            if (node < 0 && leaf >= 0) {
+                leaf = node;
+                node = (sp == 0) ? kTravDone : st.get(--sp);
+            }
+            if (!__any(leaf >= 0)) break;
+        }
+        // the parked leaf, and a second one the descent stopped at
+        while (leaf < 0) {
+            const uint32_t first = RRH_LEAF_FIRST(leaf), nt = RRH_LEAF_COUNT(leaf);
+            bool occluded = false;
+            for (uint32_t i = 0; i < nt; ++i) {
+                const float4* tv = S.tris + (size_t)(first + i) * 3;
+                const float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
+                float tt, u, v;
+                const uint32_t id = __float_as_uint(p0.w);
+                if (wt_hit(r, xyz(p0), xyz(p1), xyz(p2), tmin, t, tt, u, v)) {
+                    if (shadow) {
+                        occluded = true;
+                        break;
+                    }
+                    if (better(tt, id, t, prim)) {
+                        t = tt;
+                        prim = id;
+                        b1 = u;
+                        b2 = v;
+                    }
+                }
+            }
+            if (occluded) {
+                prim = 0u;
+                node = kTravDone;
+                leaf = 0;
+                break;
+            }
+            leaf = node;
+            if (node < 0) node = (sp == 0) ? kTravDone : st.get(--sp);
+        }
+        if (node == kTravDone && leaf >= 0) {  // this lane's ray is finished
+            if (!shadow) {
+                hits[slot - n_sh] = make_float4(t, __uint_as_float(prim), b1, b2);
+            } else if (prim != 0u) {  // visible: the NEE contribution
+                const uint32_t pixel = __float_as_uint(sh_d[slot].w);
+                const float4 cc = sh_c[slot];
+                float4 L = rad[pixel];
+                L.x = L.x + cc.x; L.y = L.y + cc.y; L.z = L.z + cc.z;
+                rad[pixel] = L;
+                if (rec.v) guide_record_nee_visible(rec, pixel, b_sh);
+            }
+            active = false;
+        }
+    }
+}
+
+// one-wave workgroups of k_trace_queue_p<STK> a CU holds at once (cached per stack kind)
+template <int STK>
+uint32_t queue_p_waves_per_cu() {
+    static uint32_t per_cu = 0;
+    if (per_cu == 0) {
+        int v = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_trace_queue_p<STK>, kTraceBlock, 0) != hipSuccess || v < 1)
+            v = 1;
+        per_cu = (uint32_t)v;
+    }
+    return per_cu;
+}
+
+// the persistent launch of one virtual queue (see k_trace_queue_p); next: its slot counter
+// (zeroed with the frame's counters); RRH_OPT_PERSISTENT above 1 caps the waves per CU
+template <int STK>
+int launch_queue_p(rrh_ctx* c, const DevScene& S, const Q& qi, uint32_t* cnt, uint32_t b_ext, uint32_t b_sh,
+                   uint32_t* next, float4* rad, const VertexRec& rec, const Bufs& B) {
+    uint32_t w = queue_p_waves_per_cu<STK>();
+    if (c->persistent_waves > 1) w = std::min(w, c->persistent_waves);
+    const uint32_t g = std::max(w, 1u) * (uint32_t)c->cu_count;
+    hipLaunchKernelGGL(k_trace_queue_p<STK>, dim3(g), dim3(kTraceBlock), 0, c->stream, S, qi, (const uint32_t*)cnt,
+                       b_ext, b_sh, B.hits, (const float4*)B.sh_o, (const float4*)B.sh_d, (const float4*)B.sh_c, rad,
+                       rec, next, c->refill);
+    RRH_LAUNCH_CHECK();
+    return RRH_OK;
+}
+
+int launch_queue_p_sel(int sk, rrh_ctx* c, const DevScene& S, const Q& qi, uint32_t* cnt, uint32_t b_ext,
+                       uint32_t b_sh, uint32_t* next, float4* rad, const VertexRec& rec, const Bufs& B) {
+    switch (sk) {
+        case RRH_STACK_LDS32: return launch_queue_p<RRH_STACK_LDS32>(c, S, qi, cnt, b_ext, b_sh, next, rad, rec, B);
+        case RRH_STACK_HYBRID16: return launch_queue_p<RRH_STACK_HYBRID16>(c, S, qi, cnt, b_ext, b_sh, next, rad, rec, B);
+        case RRH_STACK_HYBRID32: return launch_queue_p<RRH_STACK_HYBRID32>(c, S, qi, cnt, b_ext, b_sh, next, rad, rec, B);
+        default: return launch_queue_p<RRH_STACK_PRIVATE>(c, S, qi, cnt, b_ext, b_sh, next, rad, rec, B);
+    }
+}
+
 // ray-sort key of each queue slot: direction octant (3 bits), then the origin's Morton code
 // (10 bits per axis of the scene box, top 29 bits); slots past the queue sort last
 __device__ __forceinline__ uint32_t spread10(uint32_t x) {
@@ -701,6 +884,11 @@ static int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, 
     // paired launches: not with fetch counting (which counts the closest-hit kernel alone),
     // the LDS top of the tree, or the persistent binary kernels
     const bool pairs = c->trace_pairs && !cf && top_n == 0 && !(c->persistent && S.node_fmt == RRH_NODES_BINARY);
+    // persistent waves over the quantized 4-wide tree: each launch drains its queues (the paired
+    // shadow queue first); not with fetch counting, the LDS top of the tree, the stackless
+    // traversal or the ray sort
+    const bool pq = c->persistent && S.node_fmt == RRH_NODES_WIDE4Q && !cf && top_n == 0 && sk != RRH_STACK_NONE &&
+                    c->ray_sort_bits == 0;
     // profiling: an event pair around every traversal launch (closest hit, paired, shadow)
     const bool prof = c->profiling && !fixed;
     auto prof_begin = [&]() -> int {
@@ -743,7 +931,10 @@ static int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, 
             perm = slots + npix;
         }
         if ((rc = prof_begin())) return rc;
-        if (pairs && b > 0)  // this bounce's extension rays with the previous bounce's shadow rays
+        if (pq)
+            rc = launch_queue_p_sel(sk, c, S, qi, cnt, b, (pairs && b > 0) ? b - 1 : kNoQueue, cnt + 128 + b, rad,
+                                    A.rec, Bf);
+        else if (pairs && b > 0)  // this bounce's extension rays with the previous bounce's shadow rays
             rc = (sk == RRH_STACK_LDS32) ? launch_pair<RRH_STACK_LDS32>(c, S, qi, cnt, b, gt, perm, rad, A.rec, Bf)
                : (sk == RRH_STACK_HYBRID16) ? launch_pair<RRH_STACK_HYBRID16>(c, S, qi, cnt, b, gt, perm, rad, A.rec, Bf)
                : (sk == RRH_STACK_HYBRID32) ? launch_pair<RRH_STACK_HYBRID32>(c, S, qi, cnt, b, gt, perm, rad, A.rec, Bf)
@@ -773,7 +964,8 @@ static int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, 
         RRH_LAUNCH_CHECK();
         if (pairs && b + 1 < p->max_bounces) continue;  // traced with the next bounce's extension rays
         if ((rc = prof_begin())) return rc;
-        rc = (sk == RRH_STACK_LDS32) ? launch_shadow<RRH_STACK_LDS32>(c, S, cnt, b, g, gt, rad, A.rec, Bf, top_n, cf)
+        rc = pq ? launch_queue_p_sel(sk, c, S, qi, cnt, kNoQueue, b, cnt + 192 + b, rad, A.rec, Bf)
+           : (sk == RRH_STACK_LDS32) ? launch_shadow<RRH_STACK_LDS32>(c, S, cnt, b, g, gt, rad, A.rec, Bf, top_n, cf)
            : (sk == RRH_STACK_HYBRID16) ? launch_shadow<RRH_STACK_HYBRID16>(c, S, cnt, b, g, gt, rad, A.rec, Bf, top_n, cf)
            : (sk == RRH_STACK_HYBRID32) ? launch_shadow<RRH_STACK_HYBRID32>(c, S, cnt, b, g, gt, rad, A.rec, Bf, top_n, cf)
            : (sk == RRH_STACK_NONE) ? launch_shadow<RRH_STACK_NONE>(c, S, cnt, b, g, gt, rad, A.rec, Bf, top_n, cf)

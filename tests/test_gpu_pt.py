@@ -139,11 +139,47 @@ def test_traversal_variants_keep_the_image(ctx, stack, top, pairs):
             assert np.array_equal(g.view(np.uint32), r.view(np.uint32))
 
 
+@pytest.mark.parametrize("stack,pairs,waves,refill", [(L.STACK_LDS32, 1, 1, 16), (L.STACK_LDS32, 0, 1, 64),
+                                                      (L.STACK_PRIVATE, 1, 1, 1), (L.STACK_HYBRID32, 1, 2, 8),
+                                                      (L.STACK_LDS32, 1, 3, 32)])
+def test_persistent_queue_traversal_keeps_the_image(ctx, stack, pairs, waves, refill):
+    """Persistent waves over the quantized 4-wide tree (RRH_OPT_PERSISTENT, RRH_OPT_REFILL): each
+    launch drains its queues, a wave handing its idle lanes new rays; images and ray counts equal
+    those of one launch per queue, bit for bit, with paired and separate launches, every stack
+    kind, a capped grid and refill thresholds from 1 to 64 lanes."""
+    for kind, seed, scale, W, H, depth in ((L.GEN_HEIGHTFIELD, 2, 0.25, 160, 90, 6), (L.GEN_INDOOR, 1, 0.3, 96, 64, 12)):
+        sa = core.generate_scene(kind, seed, scale)
+        gs = core.Scene(ctx, sa)
+        cam = core.camera_look_at(sa.eye, sa.center, sa.fov, W, H)
+        gs.set_camera(cam)
+        try:
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRACE_PAIRS, 0)
+            ref = [render_gpu(ctx, gs, cam, depth, f) for f in (0, 3)]
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAVERSAL_STACK, stack)
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRACE_PAIRS, pairs)
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_PERSISTENT, waves)
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_REFILL, refill)
+            got = [render_gpu(ctx, gs, cam, depth, f) for f in (0, 3)]
+        finally:
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAVERSAL_STACK, L.STACK_LDS32)
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRACE_PAIRS, L.TRACE_PAIRS_DEFAULT)
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_PERSISTENT, 0)
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_REFILL, 16)
+        for (g, gc), (r, rc) in zip(got, ref):
+            assert gc == rc
+            assert np.array_equal(g.view(np.uint32), r.view(np.uint32))
+
+
 def test_traversal_options_are_checked(ctx):
     with pytest.raises(L.RRHError, match="stack kind"):
         L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAVERSAL_STACK, 5)
     with pytest.raises(L.RRHError, match="top nodes"):
         L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAV_TOP_NODES, 513)
+    with pytest.raises(L.RRHError, match="persistent waves"):
+        L.call("rrh_ctx_set_option", ctx.h, L.OPT_PERSISTENT, 65)
+    for bad in (0, 65):
+        with pytest.raises(L.RRHError, match="refill threshold"):
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_REFILL, bad)
 
 
 def test_row_bands_equal_the_whole_frame(ctx):
