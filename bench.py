@@ -93,13 +93,17 @@ class Frame:
         if self.dscene is not self.scene:
             self.dscene.set_camera(cam)
         whole = len(self.bands) == 1
+        # each band is traced and accumulated (orbiting camera: reset every frame) on its own stream,
+        # so its next frame's rays start only after its rows were accumulated, and no stream waits
+        # for another
         for (bc, bs), (r0, r1) in zip(self.bands, self.rows):
             p = L.PTParams(self.max_bounces, f, 3, L.GUIDE_NONE, 0.0, 0, flags, 0, 0 if whole else r0,
                            0 if whole else r1)
             L.call("rrh_pt_render", bc.h, bs.h, C.byref(p), self.rad.h)
-        for bc, _ in self.bands[1:]:
-            L.call("rrh_ctx_join", c, bc.h)  # the accumulation reads every band
-        L.call("rrh_accumulate", c, self.rad.h, self.acc.h, 1, 0)  # orbiting camera: reset every frame
+            if whole:
+                L.call("rrh_accumulate", bc.h, self.rad.h, self.acc.h, 1, 0)
+            else:
+                L.call("rrh_accumulate_rows", bc.h, self.rad.h, self.acc.h, 1, 0, r0, r1)
         cur, prev = f & 1, 1 - (f & 1)
         d = L.DIParams(f, self.di_m, 1, 1, 5, 30)
         dc, ds = self.dctx.h, self.dscene.h

@@ -321,6 +321,11 @@ int rrh_profile_spans(rrh_ctx* ctx, const rrh_ctx* origin, float* begin_ms, floa
  *      AccumulatePass.cpp:167-218) + accumulate.slang:21-46 -------------- */
 int rrh_accumulate(rrh_ctx* ctx, const rrh_image* radiance, rrh_image* accumulated,
                    uint32_t camera_moved, uint32_t accumulated_spp);
+/* rrh_accumulate over image rows [row_begin, row_end) only (no reference counterpart: the row
+ * band of a path-traced frame, rrh_pt_params.row_begin / row_end, accumulated on the stream of
+ * the context that traced it, so no other stream has to wait for the band) */
+int rrh_accumulate_rows(rrh_ctx* ctx, const rrh_image* radiance, rrh_image* accumulated,
+                        uint32_t camera_moved, uint32_t accumulated_spp, uint32_t row_begin, uint32_t row_end);
 /* FP64 running sum (RGBA64F); mean = sum / spp */
 int rrh_accumulate_f64(rrh_ctx* ctx, const rrh_image* radiance, rrh_image* accum_sum,
                        uint32_t camera_moved, uint32_t accumulated_spp);

@@ -178,6 +178,7 @@ SIGNATURES = {
     "rrh_profile_spans": [P, P, C.POINTER(f32), C.POINTER(f32), u32, C.POINTER(u32)],
     "rrh_accumulate": [P, P, P, u32, u32],
     "rrh_accumulate_f64": [P, P, P, u32, u32],
+    "rrh_accumulate_rows": [P, P, P, u32, u32, u32, u32],
     "rrh_di_initial": [P, P, C.POINTER(DIParams), P, P, P, P],
     "rrh_di_temporal": [P, P, C.POINTER(DIParams), P, P, P, P, P, P],
     "rrh_di_spatial_shade": [P, P, C.POINTER(DIParams), P, P, P, P, P],

@@ -1168,6 +1168,20 @@ int rrh_accumulate(rrh_ctx* c, const rrh_image* radiance, rrh_image* acc, uint32
     return RRH_OK;
 }
 
+int rrh_accumulate_rows(rrh_ctx* c, const rrh_image* radiance, rrh_image* acc, uint32_t camera_moved, uint32_t spp,
+                        uint32_t row_begin, uint32_t row_end) {
+    RRH_ARG(c && radiance && acc, "rrh_accumulate_rows: null argument");
+    RRH_ARG(radiance->fmt == RRH_FMT_RGBA32F && acc->fmt == RRH_FMT_RGBA32F, "rrh_accumulate_rows: RGBA32F images required");
+    RRH_ARG(radiance->w == acc->w && radiance->h == acc->h, "rrh_accumulate_rows: extent mismatch");
+    RRH_ARG(row_begin < row_end && row_end <= radiance->h, "rrh_accumulate_rows: rows outside the image (row_begin < row_end <= height)");
+    const size_t first = (size_t)row_begin * radiance->w;
+    const uint32_t n = (row_end - row_begin) * radiance->w;
+    hipLaunchKernelGGL(k_accumulate, dim3((n + 255) / 256), dim3(256), 0, c->stream,
+                       (const float4*)radiance->p + first, (float4*)acc->p + first, n, camera_moved, spp);
+    RRH_LAUNCH_CHECK();
+    return RRH_OK;
+}
+
 int rrh_accumulate_f64(rrh_ctx* c, const rrh_image* radiance, rrh_image* acc, uint32_t camera_moved, uint32_t spp) {
     RRH_ARG(c && radiance && acc, "rrh_accumulate_f64: null argument");
     RRH_ARG(radiance->fmt == RRH_FMT_RGBA32F && acc->fmt == RRH_FMT_RGBA64F, "rrh_accumulate_f64: RGBA32F -> RGBA64F");

@@ -213,6 +213,26 @@ def test_row_bands_equal_the_whole_frame(ctx):
         assert np.array_equal(img2.download().reshape(-1, 4).view(np.uint32), full.view(np.uint32))
 
 
+def test_accumulate_rows_equals_the_whole_image(ctx):
+    """rrh_accumulate_rows over ragged row bands, each on its own context, equals rrh_accumulate
+    over the whole image bit for bit (a running mean after a reset, then two more samples)."""
+    W, H = 40, 27
+    rng = np.random.default_rng(3)
+    ctx2 = core.Context(0)
+    full_acc, band_acc = core.Image(ctx, W, H), core.Image(ctx, W, H)
+    for spp in range(3):
+        rad = core.Image(ctx, W, H)
+        rad.upload(rng.random((H, W, 4), dtype=np.float32))
+        L.call("rrh_accumulate", ctx.h, rad.h, full_acc.h, 0, spp)
+        for cc, (r0, r1) in ((ctx2, (0, 11)), (ctx, (11, 12)), (ctx2, (12, 27))):
+            L.call("rrh_accumulate_rows", cc.h, rad.h, band_acc.h, 0, spp, r0, r1)
+        L.call("rrh_ctx_join", ctx.h, ctx2.h)
+        assert np.array_equal(band_acc.download().view(np.uint32), full_acc.download().view(np.uint32))
+    for r0, r1 in ((5, 5), (9, 4), (0, 28)):
+        with pytest.raises(L.RRHError, match="rows outside"):
+            L.call("rrh_accumulate_rows", ctx.h, rad.h, band_acc.h, 0, 0, r0, r1)
+
+
 def test_row_band_arguments_are_checked(ctx):
     sa = core.generate_scene(L.GEN_CORNELL_REF)
     gs = core.Scene(ctx, sa)
