@@ -330,7 +330,7 @@ def restir_gi_summary(ctx: core.Context, scene, sa, W: int, H: int, frames: int,
 
 
 # the options a traffic probe must share with the timed run (same kernels, same tree)
-PROBE_OPTS = ["stack", "top_nodes", "trace_pairs", "di_stream", "pt_bands", "xcd_remap", "persistent", "refill", "max_leaf", "trav_cost", "speculative", "bvh_builder",
+PROBE_OPTS = ["stack", "top_nodes", "trace_pairs", "di_stream", "pt_bands", "xcd_remap", "persistent", "refill", "suspend", "max_leaf", "trav_cost", "speculative", "bvh_builder",
               "ploc_radius", "ray_sort", "bvh_nodes", "width", "height", "max_bounces"]
 COPY_BYTES = 1 << 30
 
@@ -345,6 +345,7 @@ def configure(ctx, a) -> None:
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_NODES, NODE_FORMATS[a.bvh_nodes])
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_PERSISTENT, a.persistent)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_REFILL, a.refill)
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_SUSPEND_VISITS, a.suspend)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_SPECULATIVE, a.speculative)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_MAX_LEAF, a.max_leaf)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_TRAV_COST, int(round(a.trav_cost * 100)))
@@ -478,6 +479,8 @@ def main() -> int:
     ap.add_argument("--persistent", type=int, default=0,
                     help="1: persistent queue traversal with lane refill; 2..64: at most that many waves per CU")
     ap.add_argument("--refill", type=int, default=16, help="persistent waves take new rays at this many idle lanes")
+    ap.add_argument("--suspend", type=int, default=L.SUSPEND_VISITS_DEFAULT,
+                    help="node visits after which a ray is suspended and resumed by a second launch (0: off)")
     ap.add_argument("--max-leaf", type=int, default=8, help="largest BVH leaf (1..8)")
     ap.add_argument("--trav-cost", type=float, default=1.0, help="SAH traversal cost (triangle tests)")
     ap.add_argument("--speculative", type=int, default=1, help="1: speculative (leaf-parking) binary traversal")
@@ -547,18 +550,25 @@ def main() -> int:
 
     summed("rrh_fetch_counts", 3)
     summed("rrh_fetch_counts_shadow", 3)
+    summed("rrh_fetch_wave_max", 4)
     fr.run(a.warmup, L.PT_COUNT_FETCHES)
     fr.synchronize()
     nn, nt, nr = summed("rrh_fetch_counts", 3)
+    wn, wt, wsn, wst = summed("rrh_fetch_wave_max", 4)
     bytes_per_ray = (nn.value * NODE_BYTES[a.bvh_nodes] + nt.value * TRI_BYTES) / max(nr.value, 1) + RAY_BYTES + HIT_BYTES
     sn, st, sr = summed("rrh_fetch_counts_shadow", 3)
     # a shadow ray reads its origin and direction records; its contribution record and pixel are
     # touched only when it is visible, and are left out (a lower bound)
     bytes_per_shadow = ((sn.value * NODE_BYTES[a.bvh_nodes] + st.value * TRI_BYTES) / sr.value + RAY_BYTES
                         if sr.value else None)
+    # the lane utilisation the spread of ray lengths alone allows: a wave lasts as long as its
+    # longest ray, so sum over rays / (64 x the longest per wave), in node visits
+    lane_bound = {"closest": round(nn.value / wn.value, 3) if wn.value else None,
+                  "shadow": round(sn.value / wsn.value, 3) if wsn.value else None}
     print(f"[bench] traversal: {nn.value / max(nr.value, 1):.1f} nodes + {nt.value / max(nr.value, 1):.1f} tris "
           f"per ray -> {bytes_per_ray:.0f} algorithmic B/ray; shadow rays "
-          f"{sn.value / max(sr.value, 1):.1f} nodes + {st.value / max(sr.value, 1):.1f} tris", file=sys.stderr, flush=True)
+          f"{sn.value / max(sr.value, 1):.1f} nodes + {st.value / max(sr.value, 1):.1f} tris; lane bound from "
+          f"ray lengths {lane_bound}", file=sys.stderr, flush=True)
     fr.synchronize()
     ray_totals(fr.contexts(), reset=True)
     summed("rrh_pt_traversal_rays", 2)
@@ -617,7 +627,7 @@ def main() -> int:
                    "width": W, "height": H, "spp_per_frame": 1, "max_bounces": a.max_bounces,
                    "di_candidates": 32, "di_temporal_mcap": "20*M", "di_spatial": "5 neighbours, r=30",
                    "camera": "orbit 0.5 deg/frame", "parallelism": "single GPU",
-                   "traversal_stack": a.stack, "top_nodes_in_lds": a.top_nodes, "trace_pairs": a.trace_pairs, "di_pass_stream": "own context, overlapping the path tracer" if a.di_stream else "after the path tracer, same stream", "pt_row_bands": a.pt_bands, "xcd_remap": a.xcd_remap, "bvh_nodes": a.bvh_nodes, "persistent": a.persistent, "refill": a.refill, "speculative": a.speculative,
+                   "traversal_stack": a.stack, "top_nodes_in_lds": a.top_nodes, "trace_pairs": a.trace_pairs, "di_pass_stream": "own context, overlapping the path tracer" if a.di_stream else "after the path tracer, same stream", "pt_row_bands": a.pt_bands, "xcd_remap": a.xcd_remap, "bvh_nodes": a.bvh_nodes, "persistent": a.persistent, "refill": a.refill, "suspend_visits": a.suspend, "speculative": a.speculative,
                    "bvh_max_leaf": a.max_leaf, "bvh_trav_cost": a.trav_cost, "bvh_builder": a.bvh_builder,
                    "bvh_build_ms": round(info["build_ms"], 1), "ray_sort_bits": a.ray_sort,
                    "rays_per_step": {"primary": prim // a.steps, "extension": ext // a.steps,
@@ -634,6 +644,7 @@ def main() -> int:
                                          "shadow": pt_shadow.value // max(launches.value, 1)},
                      "avg_launch_ms": round(avg_launch_ms, 4),
                      "launches": launches.value, "measured_copy_gbs": round(hbm.value, 1),
+                     "lane_bound_from_ray_lengths": lane_bound,
                      "concurrent": {"launch_union_ms": round(union_ms, 3), "achieved": round(achieved_chip, 1),
                                     "frac": round(achieved_chip / HBM_PEAK_GBS, 4),
                                     "frame_share": round(union_ms / (dt * 1e3), 3),

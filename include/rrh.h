@@ -254,6 +254,11 @@ int rrh_fetch_counts(rrh_ctx* ctx, uint64_t* node_fetches, uint64_t* tri_fetches
 /* the same for the path tracer's shadow rays (occlusion traversal; counted for quantized 4-wide
  * trees with the speculative or the stackless traversal) */
 int rrh_fetch_counts_shadow(rrh_ctx* ctx, uint64_t* node_fetches, uint64_t* tri_fetches, uint64_t* rays, int reset);
+/* the same frames, per wave: the sum over waves of 64 x the largest per-ray node (and triangle)
+ * count of the wave; node_fetches / closest_nodes bounds the lane utilisation the spread of ray
+ * lengths allows (no reference counterpart; measurement only) */
+int rrh_fetch_wave_max(rrh_ctx* ctx, uint64_t* closest_nodes, uint64_t* closest_tris, uint64_t* shadow_nodes,
+                       uint64_t* shadow_tris, int reset);
 /* closest-hit and shadow rays traced by rrh_pt_render* on this context since the last reset
  * (the path tracer's own rays: rrh_ray_totals also counts the ReSTIR DI pass) */
 int rrh_pt_traversal_rays(rrh_ctx* ctx, uint64_t* closest, uint64_t* shadow, int reset);
@@ -270,6 +275,10 @@ int rrh_pt_traversal_rays(rrh_ctx* ctx, uint64_t* closest, uint64_t* shadow, int
                                        that many waves per CU; 0 (default): one ray per lane */
 #define RRH_OPT_REFILL 14           /* persistent 4-wide traversal: a wave takes new rays once this many
                                        of its 64 lanes are idle, 1..64 (default 16) */
+#define RRH_OPT_SUSPEND_VISITS 15   /* 1..4096: on the quantized 4-wide tree (speculative, LDS or private
+                                       stack) the path tracer's rays stop after this many node visits
+                                       and a second launch resumes them, compacted into full waves;
+                                       0: every ray runs to its end in one launch */
 #define RRH_OPT_SPECULATIVE 5       /* 1 (default): traversal parks leaves until the whole wave has one */
 #define RRH_OPT_BVH_MAX_LEAF 6      /* largest BVH leaf, 1..8 (default 8); applies to later scene uploads */
 #define RRH_OPT_BVH_TRAV_COST 7     /* SAH cost of a traversal step x 100 (default 100); later uploads */

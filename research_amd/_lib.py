@@ -39,8 +39,9 @@ PT_COUNT_FETCHES = 1
 OPT_TRAVERSAL_STACK, OPT_XCD_REMAP, OPT_BVH_NODES, OPT_PERSISTENT, OPT_SPECULATIVE = 1, 2, 3, 4, 5
 OPT_BVH_MAX_LEAF, OPT_BVH_TRAV_COST = 6, 7
 OPT_BVH_BUILDER, OPT_PLOC_RADIUS, OPT_RAY_SORT, OPT_RESTIR_CULL = 8, 9, 10, 11
-OPT_TRAV_TOP_NODES, OPT_TRACE_PAIRS, OPT_REFILL = 12, 13, 14
+OPT_TRAV_TOP_NODES, OPT_TRACE_PAIRS, OPT_REFILL, OPT_SUSPEND_VISITS = 12, 13, 14, 15
 TRACE_PAIRS_DEFAULT = 1  # a new context's RRH_OPT_TRACE_PAIRS
+SUSPEND_VISITS_DEFAULT = 0  # a new context's RRH_OPT_SUSPEND_VISITS
 BVH_BUILD_HOST, BVH_BUILD_DEVICE = 0, 1
 UPDATE_REFIT, UPDATE_REBUILD = 0, 1
 NODES_BINARY, NODES_WIDE4, NODES_WIDE4Q = 0, 1, 2
@@ -170,6 +171,7 @@ SIGNATURES = {
     "rrh_fetch_counts": [P, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), i32],
     "rrh_fetch_counts_shadow": [P, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), i32],
     "rrh_pt_traversal_rays": [P, C.POINTER(u64), C.POINTER(u64), i32],
+    "rrh_fetch_wave_max": [P, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), i32],
     "rrh_profile_enable": [P, i32],
     "rrh_stream_copy": [P, sz, i32, C.POINTER(f32)],
     "rrh_ctx_set_option": [P, i32, i32],

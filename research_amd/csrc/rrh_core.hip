@@ -101,13 +101,14 @@ int rrh_ctx_create(int device, rrh_ctx** out) {
     if (e != hipSuccess) { set_error(hipGetErrorString(e)); return RRH_E_HIP; }
     // device-side ray totals and fetch counters exist from the start, so no frame allocates them
     // (totals: primary, extension, shadow of every pass, then the path tracer's closest-hit and
-    // shadow rays; fetch: closest-hit nodes, triangles, rays, then the same for shadow rays)
+    // shadow rays; fetch: closest-hit nodes, triangles, rays, then the same for shadow rays, then
+    // the per-wave maxima of rrh_fetch_wave_max)
     int rc = c->totals.ensure(64);
-    if (!rc) rc = c->fetch.ensure(64);
+    if (!rc) rc = c->fetch.ensure(128);
     if (!rc) rc = pt_scratch_init(device, &c->noalloc_scratch);
     if (rc) { rrh_ctx_destroy(c); return rc; }
     e = hipMemsetAsync(c->totals.p, 0, 64, c->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(c->fetch.p, 0, 64, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->fetch.p, 0, 128, c->stream);
     if (e != hipSuccess) { set_error(hipGetErrorString(e)); rrh_ctx_destroy(c); return RRH_E_HIP; }
     *out = c;
     return RRH_OK;
@@ -121,7 +122,7 @@ int rrh_ctx_destroy(rrh_ctx* c) {
     c->hits.release(); c->sh_o.release(); c->sh_d.release(); c->sh_c.release();
     c->counters.release(); c->vtx.release(); c->di_counter.release(); c->di_pack.release(); c->rpt_counter.release(); c->rgi_counter.release(); c->scratch.release(); c->mlp_prep.release();
     c->train.release(); c->train_cnt.release(); c->fin_chunks.release(); c->fin_tmp.release(); c->totals.release(); c->fetch.release();
-    c->sort_keys.release(); c->sort_perm.release(); c->sort_tmp.release();
+    c->sort_keys.release(); c->sort_perm.release(); c->sort_tmp.release(); c->susp.release();
     for (hipEvent_t e : c->prof_events) (void)hipEventDestroy(e);
     if (c->prof_mark) (void)hipEventDestroy(c->prof_mark);
     for (int i = 0; i < 16; ++i) (void)hipEventDestroy(c->markers[i]);
@@ -176,6 +177,11 @@ int rrh_ctx_set_option(rrh_ctx* c, int option, int value) {
         RRH_ARG(value >= 0 && value <= 64, "rrh_ctx_set_option: persistent waves per CU is 0 (off), 1 (all the CU holds) or 2..64");
         c->persistent = value != 0;
         c->persistent_waves = value > 1 ? (uint32_t)value : 1u;
+        return RRH_OK;
+    }
+    if (option == RRH_OPT_SUSPEND_VISITS) {
+        RRH_ARG(value >= 0 && value <= 4096, "rrh_ctx_set_option: suspend visits is 0 (off) or 1..4096 node visits");
+        c->suspend_visits = (uint32_t)value;
         return RRH_OK;
     }
     if (option == RRH_OPT_REFILL) {

@@ -96,6 +96,8 @@ struct rrh_ctx {
     bool persistent = false;       // RRH_OPT_PERSISTENT: ray queues traced by persistent waves
     uint32_t persistent_waves = 1; // RRH_OPT_PERSISTENT's value: 1 = as many waves as the CUs hold, N > 1 caps them
     uint32_t refill = 16;          // RRH_OPT_REFILL: idle lanes at which a persistent wave takes new rays
+    uint32_t suspend_visits = 0;   // RRH_OPT_SUSPEND_VISITS: node visits after which a ray is resumed later (0: off)
+    rrh::DevBuf susp;              // continuation records of suspended rays
     bool speculative = true;       // RRH_OPT_SPECULATIVE: traversal parks leaves (default: measured faster)
     uint32_t bvh_max_leaf = 8;     // RRH_OPT_BVH_MAX_LEAF: applies to later scene uploads
     uint32_t bvh_trav_cost = 100;  // RRH_OPT_BVH_TRAV_COST: SAH traversal cost x 100 (later uploads)

@@ -31,7 +31,8 @@ constexpr uint32_t kTraceGridCap = 32768;
 
 // counters layout (uint32): [0, 64) ray-queue size per bounce, [64, 128) shadow rays per bounce,
 // [128, 192) / [192, 256) next queue slot handed out by the persistent waves of bounce b's
-// closest-hit launch (with its paired shadow queue) / of a launch of bounce b's shadow rays alone
+// closest-hit launch (with its paired shadow queue) / of a launch of bounce b's shadow rays alone,
+// or, in the suspend mode, the rays those launches suspended
 struct Q {
     float4 *o, *d, *thp;
 };
@@ -142,6 +143,20 @@ __device__ __forceinline__ void trace_shadow_part(const DevScene& S, uint32_t n,
     }
 }
 
+// per wave of the instrumented kernels: 64 x the largest per-lane node and triangle counts into
+// w[0], w[1] (one ray per lane at the grid sizes used), so sum / (64 x max) over the waves bounds
+// the lane utilisation that the spread of ray lengths alone allows
+__device__ __forceinline__ void wave_count_max(uint32_t nn, uint32_t nt, unsigned long long* w) {
+    for (int off = 32; off > 0; off >>= 1) {
+        nn = max(nn, (uint32_t)__shfl_xor((int)nn, off));
+        nt = max(nt, (uint32_t)__shfl_xor((int)nt, off));
+    }
+    if ((threadIdx.x & 63u) == 0u) {
+        atomicAdd(w + 0, 64ull * nn);
+        atomicAdd(w + 1, 64ull * nt);
+    }
+}
+
 // TOP: the first top_n quantized nodes are read from LDS
 template <bool COUNT, int STK, bool TOP>
 __global__ void __launch_bounds__(kTraceBlock) k_intersect(DevScene S, Q in, const uint32_t* __restrict__ cnt, uint32_t b,
@@ -156,6 +171,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_intersect(DevScene S, Q in, con
         atomicAdd(fetch + 0, (unsigned long long)nn);
         atomicAdd(fetch + 1, (unsigned long long)nt);
         atomicAdd(fetch + 2, (unsigned long long)nr);
+        wave_count_max(nn, nt, fetch + 6);
     }
 }
 
@@ -439,6 +455,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow(DevScene S, const uint32
         atomicAdd(fetch + 3, (unsigned long long)nn);
         atomicAdd(fetch + 4, (unsigned long long)nt);
         atomicAdd(fetch + 5, (unsigned long long)nr);
+        wave_count_max(nn, nt, fetch + 8);
     }
 }
 
@@ -488,6 +505,137 @@ int launch_pair(rrh_ctx* c, const DevScene& S, Q& qi, uint32_t* cnt, uint32_t b,
 // Exit: `refill` is 1..64, so a wave whose lanes are all idle always asks for slots, and once the
 // queue has none left (`drained`, wave-uniform) it ends as soon as its last lane finishes.
 constexpr uint32_t kNoQueue = ~0u;
+
+// One ray of the speculative traversal over the quantized 4-wide tree, resumable.  From where
+// it stands (node, sp, stack, hit so far) it runs outer steps (a speculative descent until every
+// lane in it holds a leaf, then the parked leaves' tests) until the ray is finished (true) or
+// until, at the start of a step, it has visited `budget` nodes in this call (false; node, sp and
+// the stack then hold where to continue, with no leaf parked).  SHADOW: a fixed extent t, the
+// children pushed in link order and the first hit ends the ray (prim = 0 marks it); otherwise
+// the nearest child first and the nearest hit kept (ties to the smaller id).  Neither answer
+// depends on the visiting order or on where a ray is interrupted.
+template <bool SHADOW, class Stk>
+__device__ __forceinline__ bool trace4q_steps(const DevScene& S, const WRay& r, float& t, uint32_t& prim, float& b1,
+                                              float& b2, int& node, int& sp, Stk& st, uint32_t budget) {
+    const float tmin = 0.001f;
+    int leaf = 0;
+    uint32_t visits = 0;
+    while (node != kTravDone) {
+        if (visits >= budget) return false;
+        while ((uint32_t)node < (uint32_t)kTravDone) {
+            ++visits;
+            Node4 N;
+            load_node4<true>(S, node, N);
+            if constexpr (SHADOW) {
+                int next = kTravDone;
+                bool have = false;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    float tn;
+                    const int l = i4c(N.lk, i);
+                    if (box_hit(f4c(N.xl, i), f4c(N.xh, i), f4c(N.yl, i), f4c(N.yh, i), f4c(N.zl, i), f4c(N.zh, i), r,
+                                tmin, t, tn) & (l != 0)) {
+                        if (have) st.put(sp++, next);
+                        next = l;
+                        have = true;
+                    }
+                }
+                node = have ? next : ((sp == 0) ? kTravDone : st.get(--sp));
+            } else {
+                float k[4];
+                int c[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    float tn;
+                    const int l = i4c(N.lk, i);
+                    const bool hit = box_hit(f4c(N.xl, i), f4c(N.xh, i), f4c(N.yl, i), f4c(N.yh, i), f4c(N.zl, i),
+                                             f4c(N.zh, i), r, tmin, t, tn) & (l != 0);
+                    k[i] = hit ? tn : INFINITY;
+                    c[i] = l;
+                }
+                cswap(k[0], c[0], k[1], c[1]);
+                cswap(k[2], c[2], k[3], c[3]);
+                cswap(k[0], c[0], k[2], c[2]);
+                cswap(k[1], c[1], k[3], c[3]);
+                cswap(k[1], c[1], k[2], c[2]);
+                if (!(k[0] < INFINITY)) {
+                    node = (sp == 0) ? kTravDone : st.get(--sp);
+                } else {
+                    if (k[3] < INFINITY) st.put(sp++, c[3]);
+                    if (k[2] < INFINITY) st.put(sp++, c[2]);
+                    if (k[1] < INFINITY) st.put(sp++, c[1]);
+                    node = c[0];
+                }
+            }
+            if (node < 0 && leaf >= 0) {
+                leaf = node;
+                node = (sp == 0) ? kTravDone : st.get(--sp);
+            }
+            if (!__any(leaf >= 0)) break;
+        }
+        // the parked leaf, and a second one the descent stopped at
+        while (leaf < 0) {
+            const uint32_t first = RRH_LEAF_FIRST(leaf), nt = RRH_LEAF_COUNT(leaf);
+            for (uint32_t i = 0; i < nt; ++i) {
+                const float4* tv = S.tris + (size_t)(first + i) * 3;
+                const float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
+                float tt, u, v;
+                if (wt_hit(r, xyz(p0), xyz(p1), xyz(p2), tmin, t, tt, u, v)) {
+                    if constexpr (SHADOW) {
+                        prim = 0u;
+                        node = kTravDone;
+                        return true;
+                    } else {
+                        const uint32_t id = __float_as_uint(p0.w);
+                        if (better(tt, id, t, prim)) {
+                            t = tt;
+                            prim = id;
+                            b1 = u;
+                            b2 = v;
+                        }
+                    }
+                }
+            }
+            leaf = node;
+            if (node < 0) node = (sp == 0) ? kTravDone : st.get(--sp);
+        }
+    }
+    return true;
+}
+template <class Stk>
+__device__ __forceinline__ bool trace4q_steps(bool shadow, const DevScene& S, const WRay& r, float& t, uint32_t& prim,
+                                              float& b1, float& b2, int& node, int& sp, Stk& st, uint32_t budget) {
+    return shadow ? trace4q_steps<true>(S, r, t, prim, b1, b2, node, sp, st, budget)
+                  : trace4q_steps<false>(S, r, t, prim, b1, b2, node, sp, st, budget);
+}
+
+// a finished ray's result: an extension ray's hit record, or a visible shadow ray's NEE
+// contribution added to its pixel (every pixel has at most one shadow ray per bounce)
+__device__ __forceinline__ void finish_ray(bool shadow, uint32_t q, float t, uint32_t prim, float b1, float b2,
+                                           float4* __restrict__ hits, const float4* __restrict__ sh_d,
+                                           const float4* __restrict__ sh_c, float4* __restrict__ rad,
+                                           const VertexRec& rec, uint32_t b_sh) {
+    if (!shadow) {
+        hits[q] = make_float4(t, __uint_as_float(prim), b1, b2);
+    } else if (prim != 0u) {
+        const uint32_t pixel = __float_as_uint(sh_d[q].w);
+        const float4 cc = sh_c[q];
+        float4 L = rad[pixel];
+        L.x = L.x + cc.x; L.y = L.y + cc.y; L.z = L.z + cc.z;
+        rad[pixel] = L;
+        if (rec.v) guide_record_nee_visible(rec, pixel, b_sh);
+    }
+}
+
+// Persistent traversal over the quantized 4-wide tree.  One launch drains a virtual queue: the
+// shadow rays of bounce b_sh (slots [0, n_sh)), then the extension rays of bounce b_ext (slots
+// [n_sh, n_sh + n_ext)); kNoQueue leaves a part out.  A wave keeps its lanes busy: whenever at
+// least `refill` of them are idle (at the start all 64) it hands them the next slots with one
+// atomic, so a lane whose ray ends early takes another instead of idling until the slowest ray
+// of its wave is done; every lane advances its ray one outer step at a time.  Hit records and
+// images equal those of k_trace_pair bit for bit.
+// Exit: `refill` is 1..64, so a wave whose lanes are all idle always asks for slots, and once the
+// queue has none left (`drained`, wave-uniform) it ends as soon as its last lane finishes.
 template <int STK>
 __global__ void __launch_bounds__(kTraceBlock) k_trace_queue_p(DevScene S, Q in, const uint32_t* __restrict__ cnt,
                                                                uint32_t b_ext, uint32_t b_sh, float4* __restrict__ hits,
@@ -500,11 +648,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_queue_p(DevScene S, Q in,
     auto st = make_stack<STK>(s_stack);
     const uint32_t n_sh = (b_sh != kNoQueue) ? cnt[64 + b_sh] : 0u;
     const uint32_t n = n_sh + ((b_ext != kNoQueue) ? cnt[b_ext] : 0u);
-    const float tmin = 0.001f;
     WRay r = wray_setup(mk(0, 0, 0), mk(0, 0, 1));
     float t = 0.0f, b1 = 0.0f, b2 = 0.0f;
     uint32_t prim = ~0u, slot = 0u;
-    int node = kTravDone, leaf = 0, sp = 0;
+    int node = kTravDone, sp = 0;
     bool active = false, drained = false, shadow = false;
     while (true) {
         const bool idle = !active;
@@ -529,94 +676,136 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_queue_p(DevScene S, Q in,
                 b1 = 0.0f;
                 b2 = 0.0f;
                 sp = 0;
-                leaf = 0;
-                node = (shadow && !(t >= tmin)) ? kTravDone : 0;  // an empty extent: never occluded
+                node = (shadow && !(t >= 0.001f)) ? kTravDone : 0;  // an empty extent: never occluded
             }
         }
         if (__ballot(active ? 1 : 0) == 0ull) {
             if (drained) break;
             continue;
         }
-        if (!active) continue;
-        // speculative descent: inner nodes until every lane in the loop holds a leaf
-        while ((uint32_t)node < (uint32_t)kTravDone) {
-            Node4 N;
-            load_node4<true>(S, node, N);
-            float k[4];
-            int c[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                float tn;
-                const int l = i4c(N.lk, i);
-                const bool hit = box_hit(f4c(N.xl, i), f4c(N.xh, i), f4c(N.yl, i), f4c(N.yh, i),
-                                                     f4c(N.zl, i), f4c(N.zh, i), r, tmin, t, tn) & (l != 0);
-                k[i] = hit ? tn : INFINITY;
-                c[i] = l;
-            }
-            cswap(k[0], c[0], k[1], c[1]);
-            cswap(k[2], c[2], k[3], c[3]);
-            cswap(k[0], c[0], k[2], c[2]);
-            cswap(k[1], c[1], k[3], c[3]);
-            cswap(k[1], c[1], k[2], c[2]);
-            if (!(k[0] < INFINITY)) {
-                node = (sp == 0) ? kTravDone : st.get(--sp);
-            } else {
-                if (k[3] < INFINITY) st.put(sp++, c[3]);
-                if (k[2] < INFINITY) st.put(sp++, c[2]);
-                if (k[1] < INFINITY) st.put(sp++, c[1]);
-                node = c[0];
-            }
-            if (node < 0 && leaf >= 0) {
-                leaf = node;
-                node = (sp == 0) ? kTravDone : st.get(--sp);
-            }
-            if (!__any(leaf >= 0)) break;
-        }
-        // the parked leaf, and a second one the descent stopped at
-        while (leaf < 0) {
-            const uint32_t first = RRH_LEAF_FIRST(leaf), nt = RRH_LEAF_COUNT(leaf);
-            bool occluded = false;
-            for (uint32_t i = 0; i < nt; ++i) {
-                const float4* tv = S.tris + (size_t)(first + i) * 3;
-                const float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
-                float tt, u, v;
-                const uint32_t id = __float_as_uint(p0.w);
-                if (wt_hit(r, xyz(p0), xyz(p1), xyz(p2), tmin, t, tt, u, v)) {
-                    if (shadow) {
-                        occluded = true;
-                        break;
-                    }
-                    if (better(tt, id, t, prim)) {
-                        t = tt;
-                        prim = id;
-                        b1 = u;
-                        b2 = v;
-                    }
-                }
-            }
-            if (occluded) {
-                prim = 0u;
-                node = kTravDone;
-                leaf = 0;
-                break;
-            }
-            leaf = node;
-            if (node < 0) node = (sp == 0) ? kTravDone : st.get(--sp);
-        }
-        if (node == kTravDone && leaf >= 0) {  // this lane's ray is finished
-            if (!shadow) {
-                hits[slot - n_sh] = make_float4(t, __uint_as_float(prim), b1, b2);
-            } else if (prim != 0u) {  // visible: the NEE contribution
-                const uint32_t pixel = __float_as_uint(sh_d[slot].w);
-                const float4 cc = sh_c[slot];
-                float4 L = rad[pixel];
-                L.x = L.x + cc.x; L.y = L.y + cc.y; L.z = L.z + cc.z;
-                rad[pixel] = L;
-                if (rec.v) guide_record_nee_visible(rec, pixel, b_sh);
-            }
+        if (active && trace4q_steps(shadow, S, r, t, prim, b1, b2, node, sp, st, 1u)) {
+            finish_ray(shadow, shadow ? slot : slot - n_sh, t, prim, b1, b2, hits, sh_d, sh_c, rad, rec, b_sh);
             active = false;
         }
     }
+}
+
+// ---- suspend and resume (RRH_OPT_SUSPEND_VISITS) ---------------------------------------------
+// The first launch traces one ray per lane as k_trace_pair does, but a ray that has visited
+// `budget` nodes stops at the start of an outer step and is appended to a continuation queue with
+// where it stands; a second launch resumes those rays, compacted into full waves.  So the first
+// launch's waves end at the budget instead of at their slowest ray, while every wave still starts
+// on neighbouring, coherent rays.  Extension and shadow rays have queues of their own (records
+// [0, cap) and [cap, 2 cap)); a record (`words` uint4) is (slot, node, sp, prim), (t, b1, b2, 0),
+// then the sp stack entries.  A lane that finds its queue full keeps tracing.
+struct SuspQ {
+    uint4* rec;
+    uint32_t *count_ext, *count_sh;
+    uint32_t cap, words;
+};
+
+// first launch: workgroups [0, g_sh) take the shadow rays of bounce b_sh, the rest the extension
+// rays of bounce b_ext (the wave-uniform loop keeps a wave's lanes together for the append)
+template <bool SHADOW, class Stk>
+__device__ __forceinline__ void trace_part_susp(const DevScene& S, const Q& in, uint32_t n, uint32_t blk, uint32_t nblk,
+                                                Stk& st, float4* __restrict__ hits, const float4* __restrict__ sh_o,
+                                                const float4* __restrict__ sh_d, const float4* __restrict__ sh_c,
+                                                float4* __restrict__ rad, const VertexRec& rec, uint32_t b_sh,
+                                                const SuspQ& sq, uint32_t budget) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t* count = SHADOW ? sq.count_sh : sq.count_ext;
+    uint4* recs = sq.rec + (SHADOW ? (size_t)sq.cap * sq.words : 0);
+    for (uint32_t base = blk * kTraceBlock; base < n; base += nblk * kTraceBlock) {
+        const uint32_t i = base + lane;
+        const bool valid = i < n;
+        float t = 1e9f, b1 = 0.0f, b2 = 0.0f;
+        uint32_t prim = ~0u;
+        int node = kTravDone, sp = 0;
+        WRay r = wray_setup(mk(0, 0, 0), mk(0, 0, 1));
+        if (valid) {
+            const float4 o = SHADOW ? sh_o[i] : in.o[i], d = SHADOW ? sh_d[i] : in.d[i];
+            r = wray_setup(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z));
+            if (SHADOW) t = o.w;
+            node = (SHADOW && !(t >= 0.001f)) ? kTravDone : 0;  // an empty extent: never occluded
+        }
+        bool done = trace4q_steps<SHADOW>(S, r, t, prim, b1, b2, node, sp, st, budget);
+        // the suspended lanes' records: one atomic per wave
+        const unsigned long long m = __ballot(done ? 0 : 1);
+        if (m != 0ull) {
+            const int leader = __ffsll((long long)m) - 1;
+            uint32_t at = 0u;
+            if ((int)lane == leader) at = atomicAdd(count, (uint32_t)__popcll(m));
+            at = __shfl(at, leader) + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            if (!done) {
+                if (at < sq.cap) {
+                    uint4* w = recs + (size_t)at * sq.words;
+                    w[0] = make_uint4(i, (uint32_t)node, (uint32_t)sp, prim);
+                    w[1] = make_uint4(__float_as_uint(t), __float_as_uint(b1), __float_as_uint(b2), 0u);
+                    uint32_t* ws = reinterpret_cast<uint32_t*>(w + 2);
+                    for (int k = 0; k < sp; ++k) ws[k] = (uint32_t)st.get(k);
+                } else {
+                    done = trace4q_steps<SHADOW>(S, r, t, prim, b1, b2, node, sp, st, ~0u);
+                }
+            }
+        }
+        if (valid && done) finish_ray(SHADOW, i, t, prim, b1, b2, hits, sh_d, sh_c, rad, rec, b_sh);
+    }
+}
+template <int STK>
+__global__ void __launch_bounds__(kTraceBlock) k_trace_susp(DevScene S, Q in, const uint32_t* __restrict__ cnt,
+                                                            uint32_t b_ext, uint32_t b_sh, uint32_t g_sh,
+                                                            float4* __restrict__ hits, const float4* __restrict__ sh_o,
+                                                            const float4* __restrict__ sh_d,
+                                                            const float4* __restrict__ sh_c, float4* __restrict__ rad,
+                                                            VertexRec rec, SuspQ sq, uint32_t budget, bool remap) {
+    __shared__ int s_stack[StackSel<STK>::kLds * 64];
+    auto st = make_stack<STK>(s_stack);
+    if (blockIdx.x < g_sh)
+        trace_part_susp<true>(S, in, cnt[64 + b_sh], xcd_block(blockIdx.x, g_sh, remap), g_sh, st, hits, sh_o, sh_d,
+                              sh_c, rad, rec, b_sh, sq, budget);
+    else
+        trace_part_susp<false>(S, in, cnt[b_ext], xcd_block(blockIdx.x - g_sh, gridDim.x - g_sh, remap),
+                               gridDim.x - g_sh, st, hits, sh_o, sh_d, sh_c, rad, rec, b_sh, sq, budget);
+}
+
+// second launch: the suspended rays to the end, one per lane in the records' order; workgroups
+// [0, g_sh) the shadow queue, the rest the extension queue
+template <bool SHADOW, class Stk>
+__device__ __forceinline__ void resume_part(const DevScene& S, const Q& in, uint32_t blk, uint32_t nblk, Stk& st,
+                                            float4* __restrict__ hits, const float4* __restrict__ sh_o,
+                                            const float4* __restrict__ sh_d, const float4* __restrict__ sh_c,
+                                            float4* __restrict__ rad, const VertexRec& rec, uint32_t b_sh,
+                                            const SuspQ& sq) {
+    const uint32_t n = min(*(SHADOW ? sq.count_sh : sq.count_ext), sq.cap);
+    const uint4* recs = sq.rec + (SHADOW ? (size_t)sq.cap * sq.words : 0);
+    for (uint32_t j = blk * kTraceBlock + (threadIdx.x & 63u); j < n; j += nblk * kTraceBlock) {
+        const uint4* w = recs + (size_t)j * sq.words;
+        const uint4 h0 = w[0], h1 = w[1];
+        const uint32_t q = h0.x;
+        int node = (int)h0.y, sp = (int)h0.z;
+        uint32_t prim = h0.w;
+        float t = __uint_as_float(h1.x), b1 = __uint_as_float(h1.y), b2 = __uint_as_float(h1.z);
+        const uint32_t* ws = reinterpret_cast<const uint32_t*>(w + 2);
+        for (int k = 0; k < sp; ++k) st.put(k, (int)ws[k]);
+        const float4 o = SHADOW ? sh_o[q] : in.o[q], d = SHADOW ? sh_d[q] : in.d[q];
+        const WRay r = wray_setup(mk(o.x, o.y, o.z), mk(d.x, d.y, d.z));
+        trace4q_steps<SHADOW>(S, r, t, prim, b1, b2, node, sp, st, ~0u);
+        finish_ray(SHADOW, q, t, prim, b1, b2, hits, sh_d, sh_c, rad, rec, b_sh);
+    }
+}
+template <int STK>
+__global__ void __launch_bounds__(kTraceBlock) k_trace_resume(DevScene S, Q in, uint32_t b_sh, uint32_t g_sh,
+                                                              float4* __restrict__ hits,
+                                                              const float4* __restrict__ sh_o,
+                                                              const float4* __restrict__ sh_d,
+                                                              const float4* __restrict__ sh_c, float4* __restrict__ rad,
+                                                              VertexRec rec, SuspQ sq) {
+    __shared__ int s_stack[StackSel<STK>::kLds * 64];
+    auto st = make_stack<STK>(s_stack);
+    if (blockIdx.x < g_sh)
+        resume_part<true>(S, in, blockIdx.x, g_sh, st, hits, sh_o, sh_d, sh_c, rad, rec, b_sh, sq);
+    else
+        resume_part<false>(S, in, blockIdx.x - g_sh, gridDim.x - g_sh, st, hits, sh_o, sh_d, sh_c, rad, rec, b_sh, sq);
 }
 
 // one-wave workgroups of k_trace_queue_p<STK> a CU holds at once (cached per stack kind)
@@ -645,6 +834,37 @@ int launch_queue_p(rrh_ctx* c, const DevScene& S, const Q& qi, uint32_t* cnt, ui
                        rec, next, c->refill);
     RRH_LAUNCH_CHECK();
     return RRH_OK;
+}
+
+// first launch plus resume of the suspend mode: g_sh shadow workgroups (bounce b_sh) and g_ext
+// extension workgroups (bounce b_ext); the suspended counts go to counters [128 + b_ext] and
+// [192 + b_sh] (zeroed with the frame's; no two launches of a frame share one)
+template <int STK>
+int launch_susp(rrh_ctx* c, const DevScene& S, const Q& qi, uint32_t* cnt, uint32_t b_ext, uint32_t b_sh,
+                uint32_t g_sh, uint32_t g_ext, float4* rad, const VertexRec& rec, const Bufs& B, const SuspQ& sq0) {
+    SuspQ sq = sq0;
+    sq.count_ext = cnt + 128 + b_ext;
+    sq.count_sh = cnt + 192 + b_sh;
+    hipLaunchKernelGGL(k_trace_susp<STK>, dim3(g_sh + g_ext), dim3(kTraceBlock), 0, c->stream, S, qi,
+                       (const uint32_t*)cnt, b_ext, b_sh, g_sh, B.hits, (const float4*)B.sh_o, (const float4*)B.sh_d,
+                       (const float4*)B.sh_c, rad, rec, sq, c->suspend_visits, c->xcd_remap);
+    RRH_LAUNCH_CHECK();
+    // resume: up to 16 waves per CU for each part that has rays (waves without records end at once)
+    const uint32_t gpart = std::max(1u, std::min((uint32_t)c->cu_count * 16u, (sq.cap + kTraceBlock - 1) / kTraceBlock));
+    const uint32_t rsh = g_sh ? gpart : 0u, rext = g_ext ? gpart : 0u;
+    hipLaunchKernelGGL(k_trace_resume<STK>, dim3(rsh + rext), dim3(kTraceBlock), 0, c->stream, S, qi, b_sh, rsh,
+                       B.hits, (const float4*)B.sh_o, (const float4*)B.sh_d, (const float4*)B.sh_c, rad, rec, sq);
+    RRH_LAUNCH_CHECK();
+    return RRH_OK;
+}
+int launch_susp_sel(int sk, rrh_ctx* c, const DevScene& S, const Q& qi, uint32_t* cnt, uint32_t b_ext, uint32_t b_sh,
+                    uint32_t g_sh, uint32_t g_ext, float4* rad, const VertexRec& rec, const Bufs& B, const SuspQ& sq) {
+    switch (sk) {
+        case RRH_STACK_LDS32: return launch_susp<RRH_STACK_LDS32>(c, S, qi, cnt, b_ext, b_sh, g_sh, g_ext, rad, rec, B, sq);
+        case RRH_STACK_HYBRID16: return launch_susp<RRH_STACK_HYBRID16>(c, S, qi, cnt, b_ext, b_sh, g_sh, g_ext, rad, rec, B, sq);
+        case RRH_STACK_HYBRID32: return launch_susp<RRH_STACK_HYBRID32>(c, S, qi, cnt, b_ext, b_sh, g_sh, g_ext, rad, rec, B, sq);
+        default: return launch_susp<RRH_STACK_PRIVATE>(c, S, qi, cnt, b_ext, b_sh, g_sh, g_ext, rad, rec, B, sq);
+    }
 }
 
 int launch_queue_p_sel(int sk, rrh_ctx* c, const DevScene& S, const Q& qi, uint32_t* cnt, uint32_t b_ext,
@@ -889,6 +1109,14 @@ static int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, 
     // traversal or the ray sort
     const bool pq = c->persistent && S.node_fmt == RRH_NODES_WIDE4Q && !cf && top_n == 0 && sk != RRH_STACK_NONE &&
                     c->ray_sort_bits == 0;
+    // suspend and resume (the same conditions, the speculative traversal, and buffers of its own)
+    const bool sq = !pq && !fixed && c->suspend_visits > 0 && S.node_fmt == RRH_NODES_WIDE4Q && S.speculative &&
+                    !cf && top_n == 0 && sk != RRH_STACK_NONE && c->ray_sort_bits == 0;
+    SuspQ susp{nullptr, nullptr, nullptr, npix, 2u + (s->stack_need() + 3u) / 4u};
+    if (sq) {
+        if ((rc = c->susp.ensure((size_t)2u * susp.cap * susp.words * 16u))) return rc;
+        susp.rec = (uint4*)c->susp.p;
+    }
     // profiling: an event pair around every traversal launch (closest hit, paired, shadow)
     const bool prof = c->profiling && !fixed;
     auto prof_begin = [&]() -> int {
@@ -934,6 +1162,9 @@ static int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, 
         if (pq)
             rc = launch_queue_p_sel(sk, c, S, qi, cnt, b, (pairs && b > 0) ? b - 1 : kNoQueue, cnt + 128 + b, rad,
                                     A.rec, Bf);
+        else if (sq)
+            rc = launch_susp_sel(sk, c, S, qi, cnt, b, (pairs && b > 0) ? b - 1 : 0u, (pairs && b > 0) ? gt : 0u, gt,
+                                 rad, A.rec, Bf, susp);
         else if (pairs && b > 0)  // this bounce's extension rays with the previous bounce's shadow rays
             rc = (sk == RRH_STACK_LDS32) ? launch_pair<RRH_STACK_LDS32>(c, S, qi, cnt, b, gt, perm, rad, A.rec, Bf)
                : (sk == RRH_STACK_HYBRID16) ? launch_pair<RRH_STACK_HYBRID16>(c, S, qi, cnt, b, gt, perm, rad, A.rec, Bf)
@@ -965,6 +1196,7 @@ static int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, 
         if (pairs && b + 1 < p->max_bounces) continue;  // traced with the next bounce's extension rays
         if ((rc = prof_begin())) return rc;
         rc = pq ? launch_queue_p_sel(sk, c, S, qi, cnt, kNoQueue, b, cnt + 192 + b, rad, A.rec, Bf)
+           : sq ? launch_susp_sel(sk, c, S, qi, cnt, b, b, gt, 0u, rad, A.rec, Bf, susp)
            : (sk == RRH_STACK_LDS32) ? launch_shadow<RRH_STACK_LDS32>(c, S, cnt, b, g, gt, rad, A.rec, Bf, top_n, cf)
            : (sk == RRH_STACK_HYBRID16) ? launch_shadow<RRH_STACK_HYBRID16>(c, S, cnt, b, g, gt, rad, A.rec, Bf, top_n, cf)
            : (sk == RRH_STACK_HYBRID32) ? launch_shadow<RRH_STACK_HYBRID32>(c, S, cnt, b, g, gt, rad, A.rec, Bf, top_n, cf)
@@ -1096,6 +1328,19 @@ int rrh_fetch_counts_shadow(rrh_ctx* c, uint64_t* nodes, uint64_t* tris, uint64_
     if (reset) RRH_HIP(hipMemsetAsync((unsigned long long*)c->fetch.p + 3, 0, 24, c->stream));
     RRH_HIP(hipStreamSynchronize(c->stream));
     *nodes = h[0]; *tris = h[1]; *rays = h[2];
+    return RRH_OK;
+}
+
+int rrh_fetch_wave_max(rrh_ctx* c, uint64_t* closest_nodes, uint64_t* closest_tris, uint64_t* shadow_nodes,
+                       uint64_t* shadow_tris, int reset) {
+    RRH_ARG(c && closest_nodes && closest_tris && shadow_nodes && shadow_tris, "rrh_fetch_wave_max: null argument");
+    *closest_nodes = *closest_tris = *shadow_nodes = *shadow_tris = 0;
+    if (!c->fetch.p) return RRH_OK;
+    unsigned long long h[4];
+    RRH_HIP(hipMemcpyAsync(h, (unsigned long long*)c->fetch.p + 6, 32, hipMemcpyDeviceToHost, c->stream));
+    if (reset) RRH_HIP(hipMemsetAsync((unsigned long long*)c->fetch.p + 6, 0, 32, c->stream));
+    RRH_HIP(hipStreamSynchronize(c->stream));
+    *closest_nodes = h[0]; *closest_tris = h[1]; *shadow_nodes = h[2]; *shadow_tris = h[3];
     return RRH_OK;
 }
 

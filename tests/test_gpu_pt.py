@@ -170,6 +170,34 @@ def test_persistent_queue_traversal_keeps_the_image(ctx, stack, pairs, waves, re
             assert np.array_equal(g.view(np.uint32), r.view(np.uint32))
 
 
+@pytest.mark.parametrize("stack,pairs,budget", [(L.STACK_LDS32, 1, 24), (L.STACK_LDS32, 0, 8), (L.STACK_PRIVATE, 1, 1),
+                                               (L.STACK_HYBRID32, 1, 64), (L.STACK_LDS32, 1, 4096)])
+def test_suspended_rays_keep_the_image(ctx, stack, pairs, budget):
+    """Suspend and resume (RRH_OPT_SUSPEND_VISITS): rays stop after `budget` node visits and a
+    second launch finishes them; images and ray counts equal those of one launch per queue bit for
+    bit.  A budget of 1 suspends nearly every ray of a paired launch, more than the continuation
+    buffer (one record per pixel) holds, so lanes that find it full finish in the first launch."""
+    for kind, seed, scale, W, H, depth in ((L.GEN_HEIGHTFIELD, 2, 0.25, 160, 90, 6), (L.GEN_INDOOR, 1, 0.3, 96, 64, 12)):
+        sa = core.generate_scene(kind, seed, scale)
+        gs = core.Scene(ctx, sa)
+        cam = core.camera_look_at(sa.eye, sa.center, sa.fov, W, H)
+        gs.set_camera(cam)
+        try:
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRACE_PAIRS, 0)
+            ref = [render_gpu(ctx, gs, cam, depth, f) for f in (0, 3)]
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAVERSAL_STACK, stack)
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRACE_PAIRS, pairs)
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_SUSPEND_VISITS, budget)
+            got = [render_gpu(ctx, gs, cam, depth, f) for f in (0, 3)]
+        finally:
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAVERSAL_STACK, L.STACK_LDS32)
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRACE_PAIRS, L.TRACE_PAIRS_DEFAULT)
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_SUSPEND_VISITS, L.SUSPEND_VISITS_DEFAULT)
+        for (g, gc), (r, rc) in zip(got, ref):
+            assert gc == rc
+            assert np.array_equal(g.view(np.uint32), r.view(np.uint32))
+
+
 def test_traversal_options_are_checked(ctx):
     with pytest.raises(L.RRHError, match="stack kind"):
         L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRAVERSAL_STACK, 5)
@@ -180,6 +208,8 @@ def test_traversal_options_are_checked(ctx):
     for bad in (0, 65):
         with pytest.raises(L.RRHError, match="refill threshold"):
             L.call("rrh_ctx_set_option", ctx.h, L.OPT_REFILL, bad)
+    with pytest.raises(L.RRHError, match="suspend visits"):
+        L.call("rrh_ctx_set_option", ctx.h, L.OPT_SUSPEND_VISITS, 4097)
 
 
 def test_row_bands_equal_the_whole_frame(ctx):
