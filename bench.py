@@ -330,7 +330,7 @@ def restir_gi_summary(ctx: core.Context, scene, sa, W: int, H: int, frames: int,
 
 
 # the options a traffic probe must share with the timed run (same kernels, same tree)
-PROBE_OPTS = ["stack", "top_nodes", "trace_pairs", "di_stream", "pt_bands", "xcd_remap", "persistent", "refill", "suspend", "max_leaf", "trav_cost", "speculative", "bvh_builder",
+PROBE_OPTS = ["stack", "top_nodes", "trace_pairs", "di_stream", "pt_bands", "xcd_remap", "persistent", "refill", "suspend", "rpt_steps", "max_leaf", "trav_cost", "speculative", "bvh_builder",
               "ploc_radius", "ray_sort", "bvh_nodes", "width", "height", "max_bounces"]
 COPY_BYTES = 1 << 30
 
@@ -346,6 +346,7 @@ def configure(ctx, a) -> None:
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_PERSISTENT, a.persistent)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_REFILL, a.refill)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_SUSPEND_VISITS, a.suspend)
+    L.call("rrh_ctx_set_option", ctx.h, L.OPT_RPT_STEP_VISITS, a.rpt_steps)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_SPECULATIVE, a.speculative)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_MAX_LEAF, a.max_leaf)
     L.call("rrh_ctx_set_option", ctx.h, L.OPT_BVH_TRAV_COST, int(round(a.trav_cost * 100)))
@@ -479,6 +480,8 @@ def main() -> int:
     ap.add_argument("--persistent", type=int, default=0,
                     help="1: persistent queue traversal with lane refill; 2..64: at most that many waves per CU")
     ap.add_argument("--refill", type=int, default=16, help="persistent waves take new rays at this many idle lanes")
+    ap.add_argument("--rpt-steps", type=int, default=0,
+                    help="ReSTIR PT initial stage: node visits per traversal pass (0: whole traversals)")
     ap.add_argument("--suspend", type=int, default=L.SUSPEND_VISITS_DEFAULT,
                     help="node visits after which a ray is suspended and resumed by a second launch (0: off)")
     ap.add_argument("--max-leaf", type=int, default=8, help="largest BVH leaf (1..8)")
@@ -627,7 +630,7 @@ def main() -> int:
                    "width": W, "height": H, "spp_per_frame": 1, "max_bounces": a.max_bounces,
                    "di_candidates": 32, "di_temporal_mcap": "20*M", "di_spatial": "5 neighbours, r=30",
                    "camera": "orbit 0.5 deg/frame", "parallelism": "single GPU",
-                   "traversal_stack": a.stack, "top_nodes_in_lds": a.top_nodes, "trace_pairs": a.trace_pairs, "di_pass_stream": "own context, overlapping the path tracer" if a.di_stream else "after the path tracer, same stream", "pt_row_bands": a.pt_bands, "xcd_remap": a.xcd_remap, "bvh_nodes": a.bvh_nodes, "persistent": a.persistent, "refill": a.refill, "suspend_visits": a.suspend, "speculative": a.speculative,
+                   "traversal_stack": a.stack, "top_nodes_in_lds": a.top_nodes, "trace_pairs": a.trace_pairs, "di_pass_stream": "own context, overlapping the path tracer" if a.di_stream else "after the path tracer, same stream", "pt_row_bands": a.pt_bands, "xcd_remap": a.xcd_remap, "bvh_nodes": a.bvh_nodes, "persistent": a.persistent, "refill": a.refill, "suspend_visits": a.suspend, "restir_pt_step_visits": a.rpt_steps, "speculative": a.speculative,
                    "bvh_max_leaf": a.max_leaf, "bvh_trav_cost": a.trav_cost, "bvh_builder": a.bvh_builder,
                    "bvh_build_ms": round(info["build_ms"], 1), "ray_sort_bits": a.ray_sort,
                    "rays_per_step": {"primary": prim // a.steps, "extension": ext // a.steps,

@@ -275,6 +275,10 @@ int rrh_pt_traversal_rays(rrh_ctx* ctx, uint64_t* closest, uint64_t* shadow, int
                                        that many waves per CU; 0 (default): one ray per lane */
 #define RRH_OPT_REFILL 14           /* persistent 4-wide traversal: a wave takes new rays once this many
                                        of its 64 lanes are idle, 1..64 (default 16) */
+#define RRH_OPT_RPT_STEP_VISITS 16  /* ReSTIR PT initial stage: 1..4096 node visits per traversal pass, a lane
+                                       whose ray has finished advancing its candidate while the others'
+                                       rays are in flight (speculative traversal); 0: every traversal
+                                       call runs all the wave's rays to their end */
 #define RRH_OPT_SUSPEND_VISITS 15   /* 1..4096: on the quantized 4-wide tree (speculative, LDS or private
                                        stack) the path tracer's rays stop after this many node visits
                                        and a second launch resumes them, compacted into full waves;

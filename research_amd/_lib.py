@@ -39,7 +39,7 @@ PT_COUNT_FETCHES = 1
 OPT_TRAVERSAL_STACK, OPT_XCD_REMAP, OPT_BVH_NODES, OPT_PERSISTENT, OPT_SPECULATIVE = 1, 2, 3, 4, 5
 OPT_BVH_MAX_LEAF, OPT_BVH_TRAV_COST = 6, 7
 OPT_BVH_BUILDER, OPT_PLOC_RADIUS, OPT_RAY_SORT, OPT_RESTIR_CULL = 8, 9, 10, 11
-OPT_TRAV_TOP_NODES, OPT_TRACE_PAIRS, OPT_REFILL, OPT_SUSPEND_VISITS = 12, 13, 14, 15
+OPT_TRAV_TOP_NODES, OPT_TRACE_PAIRS, OPT_REFILL, OPT_SUSPEND_VISITS, OPT_RPT_STEP_VISITS = 12, 13, 14, 15, 16
 TRACE_PAIRS_DEFAULT = 1  # a new context's RRH_OPT_TRACE_PAIRS
 SUSPEND_VISITS_DEFAULT = 0  # a new context's RRH_OPT_SUSPEND_VISITS
 BVH_BUILD_HOST, BVH_BUILD_DEVICE = 0, 1

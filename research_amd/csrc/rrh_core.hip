@@ -184,6 +184,11 @@ int rrh_ctx_set_option(rrh_ctx* c, int option, int value) {
         c->suspend_visits = (uint32_t)value;
         return RRH_OK;
     }
+    if (option == RRH_OPT_RPT_STEP_VISITS) {
+        RRH_ARG(value >= 0 && value <= 4096, "rrh_ctx_set_option: ReSTIR PT step visits is 0 (whole traversals) or 1..4096");
+        c->rpt_step_visits = (uint32_t)value;
+        return RRH_OK;
+    }
     if (option == RRH_OPT_REFILL) {
         RRH_ARG(value >= 1 && value <= 64, "rrh_ctx_set_option: refill threshold is 1..64 idle lanes");
         c->refill = (uint32_t)value;

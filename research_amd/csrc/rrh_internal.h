@@ -97,6 +97,7 @@ struct rrh_ctx {
     uint32_t persistent_waves = 1; // RRH_OPT_PERSISTENT's value: 1 = as many waves as the CUs hold, N > 1 caps them
     uint32_t refill = 16;          // RRH_OPT_REFILL: idle lanes at which a persistent wave takes new rays
     uint32_t suspend_visits = 0;   // RRH_OPT_SUSPEND_VISITS: node visits after which a ray is resumed later (0: off)
+    uint32_t rpt_step_visits = 0;  // RRH_OPT_RPT_STEP_VISITS: ReSTIR PT initial stage, node visits per traversal pass
     rrh::DevBuf susp;              // continuation records of suspended rays
     bool speculative = true;       // RRH_OPT_SPECULATIVE: traversal parks leaves (default: measured faster)
     uint32_t bvh_max_leaf = 8;     // RRH_OPT_BVH_MAX_LEAF: applies to later scene uploads

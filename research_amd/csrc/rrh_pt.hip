@@ -506,101 +506,11 @@ int launch_pair(rrh_ctx* c, const DevScene& S, Q& qi, uint32_t* cnt, uint32_t b,
 // queue has none left (`drained`, wave-uniform) it ends as soon as its last lane finishes.
 constexpr uint32_t kNoQueue = ~0u;
 
-// One ray of the speculative traversal over the quantized 4-wide tree, resumable.  From where
-// it stands (node, sp, stack, hit so far) it runs outer steps (a speculative descent until every
-// lane in it holds a leaf, then the parked leaves' tests) until the ray is finished (true) or
-// until, at the start of a step, it has visited `budget` nodes in this call (false; node, sp and
-// the stack then hold where to continue, with no leaf parked).  SHADOW: a fixed extent t, the
-// children pushed in link order and the first hit ends the ray (prim = 0 marks it); otherwise
-// the nearest child first and the nearest hit kept (ties to the smaller id).  Neither answer
-// depends on the visiting order or on where a ray is interrupted.
+// the path tracer's rays through the resumable traversal (rrh_device.h): tmin 0.001, no culling
 template <bool SHADOW, class Stk>
 __device__ __forceinline__ bool trace4q_steps(const DevScene& S, const WRay& r, float& t, uint32_t& prim, float& b1,
                                               float& b2, int& node, int& sp, Stk& st, uint32_t budget) {
-    const float tmin = 0.001f;
-    int leaf = 0;
-    uint32_t visits = 0;
-    while (node != kTravDone) {
-        if (visits >= budget) return false;
-        while ((uint32_t)node < (uint32_t)kTravDone) {
-            ++visits;
-            Node4 N;
-            load_node4<true>(S, node, N);
-            if constexpr (SHADOW) {
-                int next = kTravDone;
-                bool have = false;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    float tn;
-                    const int l = i4c(N.lk, i);
-                    if (box_hit(f4c(N.xl, i), f4c(N.xh, i), f4c(N.yl, i), f4c(N.yh, i), f4c(N.zl, i), f4c(N.zh, i), r,
-                                tmin, t, tn) & (l != 0)) {
-                        if (have) st.put(sp++, next);
-                        next = l;
-                        have = true;
-                    }
-                }
-                node = have ? next : ((sp == 0) ? kTravDone : st.get(--sp));
-            } else {
-                float k[4];
-                int c[4];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    float tn;
-                    const int l = i4c(N.lk, i);
-                    const bool hit = box_hit(f4c(N.xl, i), f4c(N.xh, i), f4c(N.yl, i), f4c(N.yh, i), f4c(N.zl, i),
-                                             f4c(N.zh, i), r, tmin, t, tn) & (l != 0);
-                    k[i] = hit ? tn : INFINITY;
-                    c[i] = l;
-                }
-                cswap(k[0], c[0], k[1], c[1]);
-                cswap(k[2], c[2], k[3], c[3]);
-                cswap(k[0], c[0], k[2], c[2]);
-                cswap(k[1], c[1], k[3], c[3]);
-                cswap(k[1], c[1], k[2], c[2]);
-                if (!(k[0] < INFINITY)) {
-                    node = (sp == 0) ? kTravDone : st.get(--sp);
-                } else {
-                    if (k[3] < INFINITY) st.put(sp++, c[3]);
-                    if (k[2] < INFINITY) st.put(sp++, c[2]);
-                    if (k[1] < INFINITY) st.put(sp++, c[1]);
-                    node = c[0];
-                }
-            }
-            if (node < 0 && leaf >= 0) {
-                leaf = node;
-                node = (sp == 0) ? kTravDone : st.get(--sp);
-            }
-            if (!__any(leaf >= 0)) break;
-        }
-        // the parked leaf, and a second one the descent stopped at
-        while (leaf < 0) {
-            const uint32_t first = RRH_LEAF_FIRST(leaf), nt = RRH_LEAF_COUNT(leaf);
-            for (uint32_t i = 0; i < nt; ++i) {
-                const float4* tv = S.tris + (size_t)(first + i) * 3;
-                const float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
-                float tt, u, v;
-                if (wt_hit(r, xyz(p0), xyz(p1), xyz(p2), tmin, t, tt, u, v)) {
-                    if constexpr (SHADOW) {
-                        prim = 0u;
-                        node = kTravDone;
-                        return true;
-                    } else {
-                        const uint32_t id = __float_as_uint(p0.w);
-                        if (better(tt, id, t, prim)) {
-                            t = tt;
-                            prim = id;
-                            b1 = u;
-                            b2 = v;
-                        }
-                    }
-                }
-            }
-            leaf = node;
-            if (node < 0) node = (sp == 0) ? kTravDone : st.get(--sp);
-        }
-    }
-    return true;
+    return rrh::trace4q_steps<SHADOW>(S, r, mk(0.0f, 0.0f, 0.0f), 0.001f, false, t, prim, b1, b2, node, sp, st, budget);
 }
 template <class Stk>
 __device__ __forceinline__ bool trace4q_steps(bool shadow, const DevScene& S, const WRay& r, float& t, uint32_t& prim,

@@ -95,11 +95,11 @@ RRH_D float target(f3 pp, f3 pn, const Mat& pm, f3 cam, const Reservoir& r) {
 // which lanes traverse together changes.
 enum : uint32_t { kNext, kToRc, kShadow, kBounce, kExt, kFinish, kDone };
 
-template <int KS>
+template <int KS, bool STEPS>
 __global__ void __launch_bounds__(64, 4) k_rpt_initial(DevScene S, rrh_rpt_params P, HashGrid G, Frame F, Res4 R,
                                                     float4* __restrict__ hpos, float4* __restrict__ hnrm,
                                                     uint32_t* __restrict__ counter, uint32_t tiles_x,
-                                                    uint32_t npad) {
+                                                    uint32_t npad, uint32_t step_visits) {
     constexpr int STK = KS & ~kCull;
     constexpr bool CULL = (KS & kCull) != 0;
     __shared__ int s_stack[StackSel<STK>::kLds * 64];
@@ -130,11 +130,19 @@ __global__ void __launch_bounds__(64, 4) k_rpt_initial(DevScene S, rrh_rpt_param
     Hit h;
     h.t = 0.0f; h.prim = ~0u; h.b1 = h.b2 = 0.0f;
     auto st = make_stack<STK>(s_stack);
+    // STEPS: a ray in flight is traced in passes of at most step_visits node visits
+    // (rrh::trace4q_steps), and a lane whose ray has finished runs its state machine while the
+    // others' rays are still in flight: `tracing` marks a ray in flight, (tr, tnode, tsp) and the
+    // lane's stack hold where it stands
+    constexpr bool steps = STEPS;  // the host picks it with the speculative traversal only
+    bool tracing = false;
+    int tnode = kTravDone, tsp = 0;
+    WRay tr = wray_setup(mk(0.0f, 0.0f, 0.0f), mk(0.0f, 0.0f, 1.0f));
     while (true) {
         bool want_ray = false;
         while (true) {
             // consume the last ray's result and advance to the next ray (or the pixel's end)
-            while (state != kDone && !want_ray) {
+            while (state != kDone && !want_ray && !tracing) {
                 switch (state) {
                 case kNext:
                     if (c >= P.num_initial_candidates) {
@@ -293,12 +301,27 @@ __global__ void __launch_bounds__(64, 4) k_rpt_initial(DevScene S, rrh_rpt_param
                 }
             }
         }
-        if (!__any(want_ray)) break;  // every lane is out of pixels
-        if (want_ray) {
-            if (S.speculative)
-                h = trace_closest4spec_s<false, true, CULL>(S, ro, rd, bias, rtmax, st, nullptr, nullptr, first_hit);
-            else
-                h = trace_closest4_s<false, true, CULL>(S, ro, rd, bias, rtmax, st, nullptr, nullptr, first_hit);
+        if constexpr (steps) {
+            if (want_ray) {
+                tr = wray_setup(ro, rd);
+                h.t = rtmax; h.prim = ~0u; h.b1 = h.b2 = 0.0f;
+                tnode = 0;
+                tsp = 0;
+                tracing = true;
+            }
+            if (!__any(tracing)) break;  // every lane is out of pixels
+            if (tracing && trace4q_steps<false, CULL>(S, tr, rd, bias, first_hit, h.t, h.prim, h.b1, h.b2, tnode, tsp, st,
+                                                      step_visits))
+                tracing = false;
+            continue;
+        } else {
+            if (!__any(want_ray)) break;  // every lane is out of pixels
+            if (want_ray) {
+                if (S.speculative)
+                    h = trace_closest4spec_s<false, true, CULL>(S, ro, rd, bias, rtmax, st, nullptr, nullptr, first_hit);
+                else
+                    h = trace_closest4_s<false, true, CULL>(S, ro, rd, bias, rtmax, st, nullptr, nullptr, first_hit);
+            }
         }
     }
     count_rays(counter, rays);
@@ -502,7 +525,8 @@ int prepare(rrh_ctx* c, rrh_scene* s, const rrh_rpt_params* p, const rrh_image* 
     return c->rpt_counter.ensure(16);
 }
 
-template <int S> struct KInit { static constexpr auto fn = k_rpt_initial<S>; };
+template <int S> struct KInit { static constexpr auto fn = k_rpt_initial<S, false>; };
+template <int S> struct KInitSteps { static constexpr auto fn = k_rpt_initial<S, true>; };
 template <int S> struct KTemp { static constexpr auto fn = k_rpt_temporal<S>; };
 template <int S> struct KSpat { static constexpr auto fn = k_rpt_spatial_shade<S>; };
 
@@ -527,10 +551,17 @@ int rrh_rpt_initial(rrh_ctx* c, rrh_scene* s, const rrh_rpt_params* p, rrh_grid*
     const int sk = kernel_kind(c, s);
     const uint32_t tiles_x = (F.W + 7) / 8, tiles = tiles_x * ((F.H + 7) / 8);
     uint32_t waves = 0;
-    rc = persistent_waves<KInit>(c, sk, tiles, waves);
-    if (rc) return rc;
-    launch<KInit>(sk, dim3(waves), c->stream, s->dev(), *p, G, F, R, (float4*)hist_pos->p, (float4*)hist_norm->p,
-                  (uint32_t*)c->rpt_counter.p, tiles_x, tiles * 64u);
+    if (c->rpt_step_visits > 0 && c->speculative) {  // traversal in passes (RRH_OPT_RPT_STEP_VISITS)
+        rc = persistent_waves<KInitSteps>(c, sk, tiles, waves);
+        if (rc) return rc;
+        launch<KInitSteps>(sk, dim3(waves), c->stream, s->dev(), *p, G, F, R, (float4*)hist_pos->p,
+                           (float4*)hist_norm->p, (uint32_t*)c->rpt_counter.p, tiles_x, tiles * 64u, c->rpt_step_visits);
+    } else {
+        rc = persistent_waves<KInit>(c, sk, tiles, waves);
+        if (rc) return rc;
+        launch<KInit>(sk, dim3(waves), c->stream, s->dev(), *p, G, F, R, (float4*)hist_pos->p, (float4*)hist_norm->p,
+                      (uint32_t*)c->rpt_counter.p, tiles_x, tiles * 64u, 0u);
+    }
     RRH_LAUNCH_CHECK();
     return RRH_OK;
 }

@@ -76,6 +76,21 @@ def test_restir_pt_stages_match_oracle(ctx, kind, seed, scale, step, W, H, cull)
         _restir_pt_stages_match_oracle(ctx, kind, seed, scale, step, W, H)
 
 
+@pytest.mark.parametrize("visits", [1, 24])
+def test_restir_pt_traversal_in_passes_matches_oracle(ctx, visits):
+    """RRH_OPT_RPT_STEP_VISITS: the initial stage traces its rays in passes of at most `visits`
+    node visits, a lane whose ray has finished advancing its candidate meanwhile; every stage
+    still equals the oracle bit for bit, with and without back-face culling."""
+    try:
+        L.call("rrh_ctx_set_option", ctx.h, L.OPT_RPT_STEP_VISITS, visits)
+        for cull in (0, 1):
+            with restir_cull(ctx, cull):
+                _restir_pt_stages_match_oracle(ctx, *CASES[2])
+        _restir_pt_stages_match_oracle(ctx, *CASES[1])
+    finally:
+        L.call("rrh_ctx_set_option", ctx.h, L.OPT_RPT_STEP_VISITS, 0)
+
+
 def _restir_pt_stages_match_oracle(ctx, kind, seed, scale, step, W, H):
     sa = core.generate_scene(kind, seed, scale)
     gs = core.Scene(ctx, sa)
