@@ -77,7 +77,7 @@ FP = C.POINTER(C.c_float)
 UP = C.POINTER(C.c_uint32)
 
 _SIGS = {
-    "orc_sinf": (F, [F]), "orc_cosf": (F, [F]), "orc_expf": (F, [F]), "orc_logf": (F, [F]),
+    "orc_sinf": (F, [F]), "orc_cosf": (F, [F]), "orc_expf": (F, [F]), "orc_logf": (F, [F]), "orc_wexpf": (F, [F]),
     "orc_pcg_hash": (U, [U]), "orc_uint_to_float01": (F, [U]), "orc_rand_float": (F, [UP]),
     "orc_make_seed": (U, [U, U, U]),
     "orc_camera_look_at": (None, [FP, FP, F, U, U, F, F, U, C.POINTER(Camera)]),

@@ -1410,11 +1410,32 @@ void orc_atrous_prepare(uint32_t W, uint32_t H, const float cpos[3], const float
  * buffers.  in.a carries the luminance variance; gbuf = (n.xyz, linear depth).
  * The weights' arithmetic is this build's own (the reference has no config mode), chosen for
  * few operations per tap: max(0, n.n)^sigma_n by repeated squaring when sigma_n is 2^k
- * (k <= 10; otherwise exp(sigma_n log x)); the luminance term |dl| x (1 / phi_l) with the
+ * (k <= 10; otherwise exp(sigma_n log x)); one exp per tap (orc_wexpf); the normal dot product,
+ * the exp's argument and the weighted sums as fused multiply-adds; the luminance term |dl| x (1 / phi_l) with the
  * reciprocal per pixel; the depth term |dz| x (1 / (|gx tx + gy ty| + 1e-3)) with
  * gx = sigma_z dz/dx S, gy = sigma_z dz/dy S per pixel and the 25 reciprocals per pixel (taps
  * o and -o share one value exactly). */
 static float powi_pos(float x, float e) { return (x > 0.0f) ? orc_expf(e * orc_logf(x)) : 0.0f; }
+/* exp of the config-mode weights (this build's own definition, as on the GPU): x = k ln2 + r by
+ * a two-part fused reduction, e^r by a degree-6 Taylor polynomial in fmaf (each step rounds once,
+ * so the kernel gives the same bits), times 2^k; relative error about 2e-7 for |r| <= ln2 / 2 */
+float orc_wexpf(float x) {
+    if (x < -87.0f) return 0.0f;
+    if (x > 88.0f) x = 88.0f;
+    const float k = floorf(fmaf(x, 1.44269502f, 0.5f));
+    float r = fmaf(-k, 0.693145751953125f, x);
+    r = fmaf(-k, 1.42860677e-6f, r);
+    float q = fmaf(r, 1.38888892e-3f, 8.33333377e-3f);
+    q = fmaf(r, q, 4.16666679e-2f);
+    q = fmaf(r, q, 0.166666672f);
+    q = fmaf(r, q, 0.5f);
+    q = fmaf(r, q, 1.0f);
+    q = fmaf(r, q, 1.0f);
+    uint32_t bits = (uint32_t)((int)k + 127) << 23;
+    float sc;
+    memcpy(&sc, &bits, 4);
+    return q * sc;
+}
 static int pow2_exponent(float e) {
     for (int k = 0; k <= 10; ++k)
         if (e == (float)(1 << k)) return k;
@@ -1474,13 +1495,14 @@ void orc_atrous_cfg(const orc_atrous_params* p, const uint16_t* in, const uint16
                     float vq = orc_f16_to_f32(in[j * 4 + 3]);
                     float bw = (kernel_weight(tx) * kernel_weight(ty)) / 256.0f;
                     float wz = fabsf(zp - zq) * inv_den[ty + 2][tx + 2];
-                    float wn = fmaxf(0.0f, vdot(np, nq));
+                    float wn = fmaxf(0.0f, fmaf(np.z, nq.z, fmaf(np.y, nq.y, np.x * nq.x)));
                     if (nsq >= 0) { for (int k = 0; k < nsq; ++k) wn = wn * wn; }
                     else wn = powi_pos(wn, p->sigma_normal);
-                    float wl = fabsf(lp - lum709(cq)) * inv_phil;
-                    float w = bw * orc_expf(0.0f - wl - wz) * wn;
-                    acc = vadd(acc, vscale(cq, w));
-                    accv += vq * (w * w);
+                    float w = bw * orc_wexpf(fmaf(-fabsf(lp - lum709(cq)), inv_phil, -wz)) * wn;
+                    acc.x = fmaf(cq.x, w, acc.x);
+                    acc.y = fmaf(cq.y, w, acc.y);
+                    acc.z = fmaf(cq.z, w, acc.z);
+                    accv = fmaf(vq, w * w, accv);
                     wsum += w;
                 }
             }

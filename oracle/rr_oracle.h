@@ -54,6 +54,7 @@ typedef struct orc_scene orc_scene;
 float orc_sinf(float x);
 float orc_cosf(float x);
 float orc_expf(float x);
+float orc_wexpf(float x);  /* the A-trous config mode's weight exp */
 float orc_logf(float x);
 
 /* ---- RNG: Common.slang:18-28, Sampling.slang:11-30 ------------------------- */

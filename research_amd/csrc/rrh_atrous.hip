@@ -39,6 +39,7 @@ struct RefPx { f3 pos; float w; f3 nrm; f3 rgb; };
 template <bool FAST> struct WMath;
 template <> struct WMath<false> {
     static __device__ __forceinline__ float ex(float x) { return exp_rr(x); }
+    static __device__ __forceinline__ float wex(float x) { return wexp_rr(x); }  // config mode's weights
     static __device__ __forceinline__ float div(float a, float b) { return a / b; }
     static __device__ __forceinline__ float rcp(float b) { return 1.0f / b; }
     static __device__ __forceinline__ f3 unit(f3 a) { return normalize(a); }
@@ -46,6 +47,7 @@ template <> struct WMath<false> {
 };
 template <> struct WMath<true> {
     static __device__ __forceinline__ float ex(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504f); }
+    static __device__ __forceinline__ float wex(float x) { return ex(x); }
     static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     static __device__ __forceinline__ float rcp(float b) { return __builtin_amdgcn_rcpf(b); }
     static __device__ __forceinline__ f3 unit(f3 a) { return scale(a, __builtin_amdgcn_rsqf(dot(a, a))); }
@@ -173,23 +175,30 @@ __device__ __forceinline__ H4 h4(ushort4 v) { return H4{v.x, v.y, v.z, v.w}; }
 
 // The taps come from the staged lattice window; the centre's 3x3 variance and its depth
 // gradient (stride 1, off the lattice) are read from global memory.
-// Per tap: one exp.  The normal weight max(0, n.n)^sn is nsq squarings when sn = 2^nsq (nsq < 0:
+// Per tap: one exp (wexp_rr, a fused-multiply-add polynomial); the normal dot product, the exp's
+// argument -|dl| / phi_l - wz and the weighted sums are fused multiply-adds.  The normal weight max(0, n.n)^sn is nsq squarings when sn = 2^nsq (nsq < 0:
 // exp(sn log x)); the luminance and depth terms multiply by reciprocals formed once per pixel
 // (the depth term's 13: tap -o shares tap o's denominator exactly).
 template <bool FAST, bool LAT>
 __global__ void __launch_bounds__(TX* TY) k_atrous_cfg(int W, int H, int S, float sz, float sn, int nsq, float sl,
                                                       const ushort4* __restrict__ in, const ushort4* __restrict__ gb,
                                                       ushort4* __restrict__ out) {
-    __shared__ ushort4 s_gb[LAT ? LW * LH : 1];
-    __shared__ ushort4 s_in[LAT ? LW * LH : 1];
+    // the lattice window staged once in fp32 (the fp16 inputs convert exactly) with each
+    // element's luminance, so the taps neither convert nor recompute it
+    __shared__ float4 s_gb[LAT ? LW * LH : 1];   // normal, linear depth
+    __shared__ float4 s_in[LAT ? LW * LH : 1];   // rgb, variance
+    __shared__ float s_lum[LAT ? LW * LH : 1];   // lum709(rgb)
     int bx, by;
     xcd_tile(bx, by);
     const LatTile t = LAT ? lat_tile(S, bx, by) : LatTile{0, 0, 0, 0};
     if constexpr (LAT) {
         for (int k = threadIdx.x; k < LW * LH; k += TX * TY) {
             const size_t g = lat_src(t, S, W, H, k);
-            s_gb[k] = gb[g];
-            s_in[k] = in[g];
+            const ushort4 a = gb[g], b = in[g];
+            s_gb[k] = make_float4(h2f(a.x), h2f(a.y), h2f(a.z), h2f(a.w));
+            const f3 c = mk(h2f(b.x), h2f(b.y), h2f(b.z));
+            s_in[k] = make_float4(c.x, c.y, c.z, h2f(b.w));
+            s_lum[k] = lum709(c);
         }
         __syncthreads();
     }
@@ -199,23 +208,25 @@ __global__ void __launch_bounds__(TX* TY) k_atrous_cfg(int W, int H, int S, floa
     if (x >= W || y >= H) return;
     const int lc = (ty + 2) * LW + (tx + 2);
     size_t i = (size_t)y * W + x;
-    auto tap = [&](int dx, int dy, ushort4& g, ushort4& r) {
+    // tap (dx, dy): normal + depth, rgb + variance, luminance
+    auto tap = [&](int dx, int dy, float4& g, float4& r, float& l) {
         if constexpr (LAT) {
-            const int l = lc + dy * LW + dx;
-            g = s_gb[l]; r = s_in[l];
+            const int k = lc + dy * LW + dx;
+            g = s_gb[k]; r = s_in[k]; l = s_lum[k];
         } else {
             const size_t j = (size_t)clampi(y + dy * S, 0, H - 1) * W + clampi(x + dx * S, 0, W - 1);
-            g = gb[j]; r = in[j];
+            const ushort4 a = gb[j], b = in[j];
+            g = make_float4(h2f(a.x), h2f(a.y), h2f(a.z), h2f(a.w));
+            r = make_float4(h2f(b.x), h2f(b.y), h2f(b.z), h2f(b.w));
+            l = lum709(xyz(r));
         }
     };
-    ushort4 cg4, ci4;
-    tap(0, 0, cg4, ci4);
-    H4 cg = h4(cg4);
-    float zp = h2f(cg.w);
-    if (!(zp > 0.0f)) { out[i] = ci4; return; }
-    f3 np = mk(h2f(cg.x), h2f(cg.y), h2f(cg.z));
-    H4 ci = h4(ci4);
-    float lp = lum709(mk(h2f(ci.x), h2f(ci.y), h2f(ci.z)));
+    float4 cg, ci;
+    float lp;
+    tap(0, 0, cg, ci, lp);
+    float zp = cg.w;
+    if (!(zp > 0.0f)) { out[i] = in[i]; return; }
+    f3 np = xyz(cg);
     const float gk[3] = {0.25f, 0.5f, 0.25f};
     float var = 0.0f;
     for (int dy = -1; dy <= 1; ++dy)
@@ -240,28 +251,28 @@ __global__ void __launch_bounds__(TX* TY) k_atrous_cfg(int W, int H, int S, floa
     for (int ty2 = -2; ty2 <= 2; ++ty2) {
 #pragma unroll
         for (int tx2 = -2; tx2 <= 2; ++tx2) {
-            ushort4 sg4, si4;
-            tap(tx2, ty2, sg4, si4);
-            H4 sg = h4(sg4);
-            float zq = h2f(sg.w);
+            float4 sg, si;
+            float lq;
+            tap(tx2, ty2, sg, si, lq);
+            float zq = sg.w;
             if (!(zq > 0.0f)) continue;
-            f3 nq = mk(h2f(sg.x), h2f(sg.y), h2f(sg.z));
-            H4 si = h4(si4);
-            f3 cq = mk(h2f(si.x), h2f(si.y), h2f(si.z));
-            float vq = h2f(si.w);
+            f3 nq = xyz(sg);
+            f3 cq = xyz(si);
+            float vq = si.w;
             float bw = (kw(tx2) * kw(ty2)) / 256.0f;
             const int k = 5 * (ty2 + 2) + (tx2 + 2);
             float wz = fabsf(zp - zq) * rden[k <= 12 ? k : 24 - k];
-            float wn = fmaxf(0.0f, dot(np, nq));
+            float wn = fmaxf(0.0f, __builtin_fmaf(np.z, nq.z, __builtin_fmaf(np.y, nq.y, np.x * nq.x)));
             if (nsq >= 0) {
                 for (int q = 0; q < nsq; ++q) wn = wn * wn;
             } else {
                 wn = WMath<FAST>::powp(wn, sn);
             }
-            float wl = fabsf(lp - lum709(cq)) * inv_phil;
-            float w = bw * WMath<FAST>::ex(0.0f - wl - wz) * wn;
-            acc = add(acc, scale(cq, w));
-            accv += vq * (w * w);
+            float w = bw * WMath<FAST>::wex(__builtin_fmaf(-fabsf(lp - lq), inv_phil, -wz)) * wn;
+            acc.x = __builtin_fmaf(cq.x, w, acc.x);
+            acc.y = __builtin_fmaf(cq.y, w, acc.y);
+            acc.z = __builtin_fmaf(cq.z, w, acc.z);
+            accv = __builtin_fmaf(vq, w * w, accv);
             wsum += w;
         }
     }
@@ -270,7 +281,7 @@ __global__ void __launch_bounds__(TX* TY) k_atrous_cfg(int W, int H, int S, floa
         o.x = f2h(acc.x / wsum); o.y = f2h(acc.y / wsum); o.z = f2h(acc.z / wsum); o.w = f2h(accv / (wsum * wsum));
         out[i] = o;
     } else {
-        out[i] = ci4;
+        out[i] = in[i];
     }
 }
 

@@ -65,6 +65,25 @@ RRH_D float exp_rr(float x) {
     int ki = (int)k;
     return p * __uint_as_float((uint32_t)(ki + 127) << 23);
 }
+// exp(x) of the A-trous config mode's weights (x <= 0 there; no reference counterpart, so the
+// arithmetic is this project's own): x = k ln2 + r by a two-part fused reduction, e^r by a
+// degree-6 Taylor polynomial in fused multiply-adds (|r| <= ln2 / 2: relative error about 2e-7),
+// times 2^k.  Every step is one correctly rounded operation, so the CPU restatement (fmaf) gives
+// the same bits.
+RRH_D float wexp_rr(float x) {
+    if (x < -87.0f) return 0.0f;
+    if (x > 88.0f) x = 88.0f;
+    const float k = floorf(__builtin_fmaf(x, 1.44269502f, 0.5f));
+    float r = __builtin_fmaf(-k, 0.693145751953125f, x);
+    r = __builtin_fmaf(-k, 1.42860677e-6f, r);
+    float p = __builtin_fmaf(r, 1.38888892e-3f, 8.33333377e-3f);
+    p = __builtin_fmaf(r, p, 4.16666679e-2f);
+    p = __builtin_fmaf(r, p, 0.166666672f);
+    p = __builtin_fmaf(r, p, 0.5f);
+    p = __builtin_fmaf(r, p, 1.0f);
+    p = __builtin_fmaf(r, p, 1.0f);
+    return p * __uint_as_float((uint32_t)((int)k + 127) << 23);
+}
 RRH_D float log_rr(float x) {
     if (!(x > 0.0f)) return (x == 0.0f) ? -__builtin_inff() : __builtin_nanf("");
     if (__builtin_isinf(x)) return x;

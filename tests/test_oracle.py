@@ -66,6 +66,18 @@ def test_sin_cos_exp_log_accuracy():
     assert abs(lib.orc_logf(1e-40) - math.log(1e-40)) < 1e-4  # subnormal input
 
 
+def test_config_weight_exp_accuracy():
+    """orc_wexpf (the A-trous config mode's weight exp: fused reduction, degree-6 polynomial in
+    fmaf) against float64 exp over the weights' range, exact at 0, zero below -87.  Dropping the
+    reduction's low part (about 2e-4 near -86) or the last polynomial term (about 2e-6) fails
+    the bound."""
+    y = np.concatenate([np.linspace(-86.5, 0.0, 40001), -np.logspace(-8, 0, 2001)]).astype(np.float32)
+    e = np.array([lib.orc_wexpf(float(t)) for t in y])
+    ref = np.exp(y.astype(np.float64))
+    assert (np.abs(e - ref) / ref).max() < 5e-7
+    assert lib.orc_wexpf(0.0) == 1.0 and lib.orc_wexpf(-100.0) == 0.0
+
+
 def test_fp16_conversion_matches_numpy():
     rng = np.random.default_rng(3)
     v = np.concatenate([rng.normal(scale=s, size=3000) for s in (1e-6, 1e-3, 1.0, 1e3, 6e4)]).astype(np.float32)
