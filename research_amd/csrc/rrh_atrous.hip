@@ -169,16 +169,14 @@ __global__ void __launch_bounds__(TX* TY) k_atrous_ref(int W, int H, int S, floa
 __device__ __forceinline__ float h2f(unsigned short h) { return __half2float(__ushort_as_half(h)); }
 __device__ __forceinline__ unsigned short f2h(float f) { return __half_as_ushort(__float2half_rn(f)); }
 
-struct H4 { unsigned short x, y, z, w; };
-__device__ __forceinline__ H4 ldh(const ushort4* p, size_t i) { ushort4 v = p[i]; return H4{v.x, v.y, v.z, v.w}; }
-__device__ __forceinline__ H4 h4(ushort4 v) { return H4{v.x, v.y, v.z, v.w}; }
 
 // The taps come from the staged lattice window; the centre's 3x3 variance and its depth
 // gradient (stride 1, off the lattice) are read from global memory.
-// Per tap: one exp (wexp_rr, a fused-multiply-add polynomial); the normal dot product, the exp's
-// argument -|dl| / phi_l - wz and the weighted sums are fused multiply-adds.  The normal weight max(0, n.n)^sn is nsq squarings when sn = 2^nsq (nsq < 0:
-// exp(sn log x)); the luminance and depth terms multiply by reciprocals formed once per pixel
-// (the depth term's 13: tap -o shares tap o's denominator exactly).
+// Per tap: one exp (wexp_rr, a fused-multiply-add polynomial); the normal dot product, the
+// exp's argument -|dl| / phi_l - wz and the weighted sums are fused multiply-adds.  The normal
+// weight max(0, n.n)^sn is nsq squarings when sn = 2^nsq (nsq < 0: exp(sn log x)); the luminance
+// and depth terms multiply by reciprocals formed once per pixel (the depth term's 13: tap -o
+// shares tap o's denominator exactly).
 template <bool FAST, bool LAT>
 __global__ void __launch_bounds__(TX* TY) k_atrous_cfg(int W, int H, int S, float sz, float sn, int nsq, float sl,
                                                       const ushort4* __restrict__ in, const ushort4* __restrict__ gb,

@@ -492,18 +492,7 @@ int launch_pair(rrh_ctx* c, const DevScene& S, Q& qi, uint32_t* cnt, uint32_t b,
     return RRH_OK;
 }
 
-// Persistent traversal over the quantized 4-wide tree.  One launch drains a virtual queue: the
-// shadow rays of bounce b_sh (slots [0, n_sh)), then the extension rays of bounce b_ext (slots
-// [n_sh, n_sh + n_ext)); kNoQueue leaves a part out.  A wave keeps its lanes busy: whenever at
-// least `refill` of them are idle (at the start all 64) it hands them the next slots with one
-// atomic, so a lane whose ray ends early takes another instead of idling until the slowest ray
-// of its wave is done.  Lanes descend speculatively (a lane that reaches a leaf parks it until
-// every lane holds one or is done) and test the parked leaves together; a shadow ray stops at its
-// first hit.  The closest hit does not depend on the visiting order and a shadow ray's answer only
-// on the leaves its fixed extent reaches, and every pixel has at most one shadow ray per bounce,
-// so hit records and images equal those of k_trace_pair bit for bit.
-// Exit: `refill` is 1..64, so a wave whose lanes are all idle always asks for slots, and once the
-// queue has none left (`drained`, wave-uniform) it ends as soon as its last lane finishes.
+// a launch's queue argument that leaves that part of the launch out
 constexpr uint32_t kNoQueue = ~0u;
 
 // the path tracer's rays through the resumable traversal (rrh_device.h): tmin 0.001, no culling
