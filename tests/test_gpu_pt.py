@@ -273,3 +273,49 @@ def test_row_band_arguments_are_checked(ctx):
             L.call("rrh_pt_render", ctx.h, gs.h, C.byref(L.PTParams(5, 0, 3, 0, 0.0, 0, 0, 0, r0, r1)), img.h)
     with pytest.raises(L.RRHError, match="unguided"):
         L.call("rrh_pt_render", ctx.h, gs.h, C.byref(L.PTParams(5, 0, 3, 0, 0.0, 1000, 0, 0, 0, 16)), img.h)
+
+
+def test_converged_image_matches_the_oracle_and_an_fp64_reference(ctx):
+    """Converged images: 256 spp of the seeded Cornell box accumulated in FP64 on the device equal
+    the oracle's FP64 accumulation of the same 256 frames bit for bit, and their relative MSE
+    against an independent 8,192-spp FP64 reference (other seeds) is the noise the image's own
+    batch variance predicts (an unbiased tracer; a biased one would add its bias squared).
+    relMSE = mean over pixels and channels of (I - R)^2 / (R^2 + 0.01)."""
+    sa = scene_arrays(L.GEN_CORNELL_RANDOM, 1, 1.0)
+    gs = core.Scene(ctx, sa)
+    os_ = O.Scene.from_arrays(sa)
+    W, H, depth, spp, batch = 48, 40, 8, 256, 32
+    cam = core.camera_look_at(sa.eye, sa.center, sa.fov, W, H)
+    gs.set_camera(cam)
+    oc = ocam(cam)
+    rad, acc = core.Image(ctx, W, H), core.Image(ctx, W, H, L.FMT_RGBA64F)
+    o_acc = np.zeros((W * H, 4), np.float64)
+    pix = np.arange(W * H, dtype=np.uint32)
+    means = []
+    for f in range(spp):
+        L.call("rrh_pt_render", ctx.h, gs.h, C.byref(L.PTParams(depth, f, 3, 0, 0.0, 0)), rad.h)
+        L.call("rrh_accumulate_f64", ctx.h, rad.h, acc.h, 0, f)
+        o, _ = os_.pt_render(oc, depth, f, pix)
+        O.lib().orc_accumulate_f64(O.p(o), O.p(o_acc), W * H, 0, f)
+        if (f + 1) % batch == 0:
+            means.append(acc.download().reshape(-1, 4)[:, :3].copy() / (f + 1))
+    g_sum = acc.download().reshape(-1, 4)
+    assert np.array_equal(g_sum, o_acc)
+    img = g_sum[:, :3] / spp
+    # reference: 8,192 spp with seeds 2^20 + k, FP64 on the device
+    ref_acc = core.Image(ctx, W, H, L.FMT_RGBA64F)
+    n_ref = 8192
+    for k in range(n_ref):
+        L.call("rrh_pt_render", ctx.h, gs.h, C.byref(L.PTParams(depth, (1 << 20) + k, 3, 0, 0.0, 0)), rad.h)
+        L.call("rrh_accumulate_f64", ctx.h, rad.h, ref_acc.h, 0, k)
+    ref = ref_acc.download().reshape(-1, 4)[:, :3] / n_ref
+    den = ref ** 2 + 0.01
+    rel_mse = float(np.mean((img - ref) ** 2 / den))
+    # the variance of the 256-spp mean from its 8 batch means (running means -> batch means)
+    sums = [m * (batch * (j + 1)) for j, m in enumerate(means)]
+    bmeans = np.stack([(sums[j] - (sums[j - 1] if j else 0.0)) / batch for j in range(len(sums))])
+    var_mean = bmeans.var(axis=0, ddof=1) / len(bmeans)
+    expected = float(np.mean(var_mean * (1.0 + spp / n_ref) / den))
+    print(f"converged 256 spp vs 8192-spp FP64 reference: relMSE {rel_mse:.3e}, expected from its variance "
+          f"{expected:.3e}")
+    assert 0.5 * expected < rel_mse < 2.0 * expected
