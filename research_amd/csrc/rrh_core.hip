@@ -53,6 +53,7 @@ rrh::DevScene rrh_scene::dev() const {
     d.speculative = (ctx && ctx->speculative) ? 1u : 0u;
     d.tris = (const float4*)tris.p;
     d.tri_ref = (const uint2*)tri_ref.p;
+    d.tri_vtx = (const uint4*)tri_vtx.p;
     d.tri_light = (const uint32_t*)tri_light.p;
     d.tl = (const float4*)tl.p;
     d.verts = (const rrh_vertex*)verts.p;
@@ -466,6 +467,7 @@ int rrh_scene_upload(rrh_ctx* c, const rrh_vertex* verts, uint32_t nv, const uin
     s->nl = nl;
     s->tri_world.resize((size_t)ntri * 9);
     std::vector<uint2> tri_ref(ntri);
+    std::vector<uint4> tri_vtx(ntri);
     std::vector<uint32_t> tri_light(ntri, ~0u);
     std::vector<float4> tl;
     uint32_t t = 0;
@@ -484,6 +486,8 @@ int rrh_scene_upload(rrh_ctx* c, const rrh_vertex* verts, uint32_t nv, const uin
                 w[k * 3 + 2] = mrow_h(T, 2, vv.pos[0], vv.pos[1], vv.pos[2], 1.0f);
             }
             tri_ref[t] = make_uint2(i, p);
+            tri_vtx[t] = make_uint4(me.vertex_base + idx[base], me.vertex_base + idx[base + 1],
+                                    me.vertex_base + idx[base + 2], i);
             if ((flags & RRH_SCENE_TRI_LIGHTS) && em) {
                 tri_light[t] = nl + (uint32_t)(tl.size() / kTlF4);
                 float4 a = {w[0], w[1], w[2], mt.emissive[0]};
@@ -531,6 +535,7 @@ int rrh_scene_upload(rrh_ctx* c, const rrh_vertex* verts, uint32_t nv, const uin
         return rc;
     }
     if (!rc) rc = upload_vec(s->tri_ref, tri_ref.data(), tri_ref.size(), st);
+    if (!rc) rc = upload_vec(s->tri_vtx, tri_vtx.data(), tri_vtx.size(), st);
     if (!rc) rc = upload_vec(s->tri_light, tri_light.data(), tri_light.size(), st);
     if (!rc) rc = upload_vec(s->tl, tl.data(), tl.size(), st);
     if (!rc) rc = upload_vec(s->verts, verts, nv, st);
@@ -554,7 +559,7 @@ int rrh_scene_destroy(rrh_scene* s) {
     if (!s) return RRH_OK;
     (void)hipDeviceSynchronize();  // the owning context may already be gone
     DevBuf* bufs[] = {&s->nodes, &s->nodes4, &s->nodesq, &s->tris, &s->tri_ref, &s->tri_light, &s->tl, &s->verts, &s->idx,
-                      &s->meshes, &s->mats, &s->insts, &s->lights, &s->cam, &s->twd, &s->refit};
+                      &s->meshes, &s->mats, &s->insts, &s->lights, &s->cam, &s->twd, &s->refit, &s->tri_vtx};
     for (DevBuf* b : bufs) b->release();
     delete s;
     return RRH_OK;

@@ -153,6 +153,7 @@ struct DevScene {
     uint32_t speculative;      // 1: traversal postpones leaves (Aila and Laine 2009; binary and quantized 4-wide)
     const float4* tris;        // 3 float4 per triangle, leaf order; tris[3i].w = flattened id bits
     const uint2* tri_ref;      // per flattened id: (instance, primitive within mesh)
+    const uint4* tri_vtx;      // per flattened id: (vertex_base + idx[3 p + k] for k < 3, instance)
     const uint32_t* tri_light; // per flattened id: light index or ~0u
     const float4* tl;          // per triangle light, kTlF4 float4: (v0, Le.x) (v1, Le.y) (v2, Le.z)
                                // (id bits, area, 0, 0) (unit normal, 0)
@@ -1258,16 +1259,17 @@ RRH_D f3 xform_dir(const float* T, f3 p) {
     return mk(T[0] * p.x + T[1] * p.y + T[2] * p.z, T[4] * p.x + T[5] * p.y + T[6] * p.z,
               T[8] * p.x + T[9] * p.y + T[10] * p.z);
 }
+// The vertex indices come from the per-triangle record built at upload (vertex_base plus the
+// mesh's indices, and the instance), so the vertices are two dependent loads from the hit
+// instead of four (instance, mesh, index buffer, vertices).
 RRH_D HitInfo fetch_hit(const DevScene& S, uint32_t prim, float b1, float b2) {
     HitInfo hi;
-    uint2 ref = S.tri_ref[prim];
-    const rrh_instance* in = S.insts + ref.x;
+    const uint4 tv = S.tri_vtx[prim];
+    const rrh_instance* in = S.insts + tv.w;
     const rrh_mesh* me = S.meshes + in->mesh_index;
-    uint32_t base = me->index_byte_offset / 4u + ref.y * 3u;
-    uint32_t vb = me->vertex_base;
-    const rrh_vertex* v0 = S.verts + vb + S.idx[base];
-    const rrh_vertex* v1 = S.verts + vb + S.idx[base + 1];
-    const rrh_vertex* v2 = S.verts + vb + S.idx[base + 2];
+    const rrh_vertex* v0 = S.verts + tv.x;
+    const rrh_vertex* v1 = S.verts + tv.y;
+    const rrh_vertex* v2 = S.verts + tv.z;
     float b0 = 1.0f - b1 - b2;
     f3 lp = add(add(scale(ld3(v0->pos), b0), scale(ld3(v1->pos), b1)), scale(ld3(v2->pos), b2));
     f3 ln = normalize(add(add(scale(ld3(v0->normal), b0), scale(ld3(v1->normal), b1)), scale(ld3(v2->normal), b2)));

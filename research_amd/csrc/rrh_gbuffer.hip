@@ -45,13 +45,12 @@ __global__ __launch_bounds__(kBlock) void k_gbuffer(DevScene S, GbufOut o, uint3
         const Hit h = trace_closest4_s<false, true, true>(S, org, d, cam.near_plane / dd, cam.far_plane / dd, st,
                                                            nullptr, nullptr);
         if (h.prim != ~0u) {
-            const uint2 ref = S.tri_ref[h.prim];
-            const rrh_instance* in = S.insts + ref.x;
+            const uint4 tv = S.tri_vtx[h.prim];  // global vertex indices, instance
+            const rrh_instance* in = S.insts + tv.w;
             const rrh_mesh* me = S.meshes + in->mesh_index;
-            const uint32_t base = me->index_byte_offset / 4u + ref.y * 3u, vb = me->vertex_base;
-            const rrh_vertex* v0 = S.verts + vb + S.idx[base];
-            const rrh_vertex* v1 = S.verts + vb + S.idx[base + 1];
-            const rrh_vertex* v2 = S.verts + vb + S.idx[base + 2];
+            const rrh_vertex* v0 = S.verts + tv.x;
+            const rrh_vertex* v1 = S.verts + tv.y;
+            const rrh_vertex* v2 = S.verts + tv.z;
             const float b0 = 1.0f - h.b1 - h.b2;
             const f3 w0 = xform_point(in->transform, ld3(v0->pos)), w1 = xform_point(in->transform, ld3(v1->pos)),
                      w2 = xform_point(in->transform, ld3(v2->pos));

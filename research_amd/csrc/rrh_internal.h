@@ -148,6 +148,7 @@ struct rrh_ctx {
 struct rrh_scene {
     rrh_ctx* ctx = nullptr;
     rrh::DevBuf nodes, nodes4, nodesq, tris, tri_ref, tri_light, tl, verts, idx, meshes, mats, insts, lights, cam;
+    rrh::DevBuf tri_vtx;          // per flattened id: global indices of its three vertices, instance
     uint32_t ntri = 0, nnodes = 0, nl = 0, ntl = 0, flags = 0, depth = 0, leaves = 0;
     uint32_t nnodes4 = 0, depth4 = 0, stack4 = 0;
     rrh::DevBuf parent4;          // parent of each 4-wide node (the stackless traversal climbs them)

@@ -80,6 +80,12 @@ struct HostSrc {  // arrays uploaded from the host (3 floats per vector)
     uint32_t n;
     __device__ uint32_t count() const { return n; }
     __device__ float weight(uint32_t i) const { return w[i]; }
+    __device__ bool load(uint32_t i, Sample& s) const {  // i < n: the inference inputs only
+        s.p = mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+        s.n = mk(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+        s.wo = mk(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]);
+        return true;
+    }
     __device__ bool get(uint32_t i, Sample& s) const {
         if (i >= n) return false;
         s.p = mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
@@ -130,16 +136,24 @@ __device__ __forceinline__ void freq_pair(float u, int j, float& sn, float& cs) 
     sn = __builtin_amdgcn_sinf(rev);
     cs = __builtin_amdgcn_cosf(rev);
 }
-// features [fb, fb + 8) of a sample (fb a multiple of 8)
+// two features rounded to BF16 by one paired conversion
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t bf16x2_bits(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
+}
+// features [fb, fb + 8) of a sample (fb a multiple of 8); MASK false leaves an invalid sample's
+// features as they come (inference: a sample's column of the MFMA operands reaches only its own
+// outputs, and an invalid sample's outputs are never stored)
+template <bool MASK = true>
 __device__ __forceinline__ bf16x8 encode8(const MlpDev& M, const Sample& s, bool valid, int fb) {
-    bf16x8 a;
+    uint32_t q[4];
     if (fb < 72) {
         float sn, cs;
         freq_pair(unit_coord(M, s, fb / 24), (fb % 24) >> 1, sn, cs);
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
-            a[2 * p] = (__bf16)(valid ? sn : 0.0f);
-            a[2 * p + 1] = (__bf16)(valid ? cs : 0.0f);
+            q[p] = bf16x2_bits(!MASK || valid ? sn : 0.0f, !MASK || valid ? cs : 0.0f);
             const float s2 = 2.0f * sn * cs, c2 = 1.0f - 2.0f * sn * sn;
             sn = s2;
             cs = c2;
@@ -150,11 +164,10 @@ __device__ __forceinline__ bf16x8 encode8(const MlpDev& M, const Sample& s, bool
             const int k = fb - 72 + 2 * p;  // 0..23: n (3), wo (3), zero padding
             const float v0 = (k == 0) ? s.n.x : (k == 2) ? s.n.z : (k == 4) ? s.wo.y : 0.0f;
             const float v1 = (k == 0) ? s.n.y : (k == 2) ? s.wo.x : (k == 4) ? s.wo.z : 0.0f;
-            a[2 * p] = (__bf16)(valid ? v0 : 0.0f);
-            a[2 * p + 1] = (__bf16)(valid ? v1 : 0.0f);
+            q[p] = bf16x2_bits(!MASK || valid ? v0 : 0.0f, !MASK || valid ? v1 : 0.0f);
         }
     }
-    return a;
+    return __builtin_bit_cast(bf16x8, (uint4){q[0], q[1], q[2], q[3]});
 }
 // 4 lanes per sample; lane quarter q writes features [24q, 24q + 24) into the sample's row
 __device__ __forceinline__ void encode_row(const MlpDev& M, const Sample& s, bool valid, int q, __bf16* row) {
@@ -430,14 +443,18 @@ __global__ void __launch_bounds__(256) k_mlp_train(MlpDev M, Src src, uint32_t n
 // each layer's output fragments become the next layer's input fragments after bias, ReLU and
 // BF16 rounding, in registers; layer-0 operands are encoded straight into registers and the head
 // runs on the four lanes (r, r + 16, r + 32, r + 48) that hold a sample's lobes.
+// ReLU after the BF16 rounding, as a packed 16-bit integer max with zero (one instruction per
+// pair): round-to-nearest keeps the sign, and a negative BF16 (or -0) is negative as an integer,
+// so this equals rounding the ReLU of the FP32 value
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t relu_bf16x2(float a, float b) {  // one v_cvt_pk_bf16_f32 + v_pk_max_i16
+    const bf16x2 h = __builtin_convertvector((f32x2){a, b}, bf16x2);
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(i16x2, h), (i16x2)0));
+}
 __device__ __forceinline__ bf16x8 relu_bf16x8(const f32x4& lo, const f32x4& hi) {
-    bf16x8 b;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        b[j] = (__bf16)fmaxf(lo[j], 0.0f);
-        b[4 + j] = (__bf16)fmaxf(hi[j], 0.0f);
-    }
-    return b;
+    const uint4 q = {relu_bf16x2(lo[0], lo[1]), relu_bf16x2(lo[2], lo[3]), relu_bf16x2(hi[0], hi[1]),
+                     relu_bf16x2(hi[2], hi[3])};
+    return __builtin_bit_cast(bf16x8, q);
 }
 __device__ __forceinline__ float group_sum4t(float v) {
     v += __shfl_xor(v, 16);
@@ -485,16 +502,18 @@ __device__ __forceinline__ void head_store_t(const f32x4 (&e)[3], bool valid, vo
     const float x0 = hexp(o[4] - emax), x1 = hexp(o[9] - emax);
     const float z = group_sum4t(x0 + x1);
     if (!valid) return;
+    // hardware reciprocal square root and reciprocal (about 1 ulp) in place of IEEE divisions
+    const float rz = __builtin_amdgcn_rcpf(z);
     float lb[10];
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
         const float* v = o + 5 * j;
-        const float len = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + 1e-12f);
-        lb[j * 5 + 0] = v[0] / len;
-        lb[j * 5 + 1] = v[1] / len;
-        lb[j * 5 + 2] = v[2] / len;
+        const float il = __builtin_amdgcn_rsqf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + 1e-12f);
+        lb[j * 5 + 0] = v[0] * il;
+        lb[j * 5 + 1] = v[1] * il;
+        lb[j * 5 + 2] = v[2] * il;
         lb[j * 5 + 3] = fminf(0.1f + softplus_f(v[3]), 1000.0f);
-        lb[j * 5 + 4] = (j == 0 ? x0 : x1) / z;
+        lb[j * 5 + 4] = (j == 0 ? x0 : x1) * rz;
     }
     if constexpr (H16) {
         uint16_t* rec = reinterpret_cast<uint16_t*>(out) + si * kMlpLobeHalves;
@@ -539,12 +558,16 @@ __global__ void __launch_bounds__(64 * kInferWaves, 4) k_mlp_infer(MlpDev M, Src
     const uint32_t nchunks = (n + 31) / 32;
     const uint32_t stride = gridDim.x * kInferWaves;
     uint32_t ch = blockIdx.x * kInferWaves + (threadIdx.x >> 6);
+    if (n == 0) return;
     // the next step's samples are loaded while this one computes (software pipelining); lanes
-    // (r, g = 0..3) of tile t hold sample 32 ch + 16 t + r
+    // (r, g = 0..3) of tile t hold sample 32 ch + 16 t + r.  Loads are unconditional, at the index
+    // clamped to the last sample (no branches, no register copies on the merge): a slot past the
+    // end or a miss holds some sample's finite inputs, and its outputs are dropped
+    auto fetch = [&](uint32_t i, Sample& s) { return src.load(min(i, n - 1), s) && i < n; };
     Sample nxt[2];
     bool nv[2];
 #pragma unroll
-    for (int t = 0; t < 2; ++t) nv[t] = ch < nchunks && src.get(ch * 32 + t * 16 + r, nxt[t]);
+    for (int t = 0; t < 2; ++t) nv[t] = fetch(ch * 32 + t * 16 + r, nxt[t]);
     for (; ch < nchunks; ch += stride) {
         // the weight fragments stay in LDS: keep the compiler from hoisting their reads into
         // registers across steps (that would cost the occupancy this layout is for)
@@ -554,16 +577,14 @@ __global__ void __launch_bounds__(64 * kInferWaves, 4) k_mlp_infer(MlpDev M, Src
         const bool valid[2] = {nv[0], nv[1]};
         const bool more = ch + stride < nchunks;
 #pragma unroll
-        for (int t = 0; t < 2; ++t) nv[t] = more && src.get((ch + stride) * 32 + t * 16 + r, nxt[t]);
+        for (int t = 0; t < 2; ++t) nv[t] = fetch((ch + stride) * 32 + t * 16 + r, nxt[t]) && more;
         bf16x8 h[2][2];
         {   // layer 0: B operand = features 32 kk + 8 g .. of sample r, encoded in registers
             bf16x8 x[2][3];
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                if (!valid[t]) { smp[t].p = smp[t].n = smp[t].wo = smp[t].dir = mk(0, 0, 0); smp[t].w = 0.0f; }
+            for (int t = 0; t < 2; ++t)
 #pragma unroll
-                for (int kk = 0; kk < 3; ++kk) x[t][kk] = encode8(M, smp[t], valid[t], kk * 32 + g * 8);
-            }
+                for (int kk = 0; kk < 3; ++kk) x[t][kk] = encode8<false>(M, smp[t], valid[t], kk * 32 + g * 8);
             f32x4 d[2][4];
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
@@ -990,7 +1011,8 @@ namespace rrh {
 // The network's inputs for the live ray queue: hit point, normal facing the ray origin and
 // outgoing direction (as the shade kernel derives them), written by a light, high-occupancy
 // kernel so the persistent inference waves read them with one coalesced load instead of
-// the chain of dependent fetches a hit lookup takes.
+// the chain of dependent fetches a hit lookup takes.  The record also carries the material
+// index, and the shade kernel of the same bounce reads it in place of its own hit lookup.
 __global__ void __launch_bounds__(256) k_mlp_prep(DevScene S, const float4* __restrict__ d,
                                                   const float4* __restrict__ hits, const uint32_t* __restrict__ cnt,
                                                   uint32_t b, float4* __restrict__ prep) {
@@ -1005,7 +1027,7 @@ __global__ void __launch_bounds__(256) k_mlp_prep(DevScene S, const float4* __re
             f3 nn = hi.normal;
             if (dot(nn, wo) < 0.0f) nn = neg(nn);
             p0 = make_float4(hi.pos.x, hi.pos.y, hi.pos.z, 1.0f);
-            p1 = make_float4(nn.x, nn.y, nn.z, 0.0f);
+            p1 = make_float4(nn.x, nn.y, nn.z, __uint_as_float(hi.mat));
             p2 = make_float4(wo.x, wo.y, wo.z, 0.0f);
         }
         prep[(size_t)i * 3] = p0;
@@ -1019,12 +1041,10 @@ struct QueueSrc {  // the prepared inputs of the live ray queue (miss -> invalid
     const uint32_t* cnt;
     uint32_t b;
     __device__ uint32_t count() const { return cnt[b]; }
-    __device__ bool get(uint32_t i, Sample& s) const {
-        if (i >= cnt[b]) return false;
+    __device__ bool load(uint32_t i, Sample& s) const {  // i < count(); false for a miss
         const float4 p0 = prep[(size_t)i * 3], p1 = prep[(size_t)i * 3 + 1], p2 = prep[(size_t)i * 3 + 2];
-        if (!(p0.w > 0.0f)) return false;
-        s.p = xyz(p0); s.n = xyz(p1); s.wo = xyz(p2); s.dir = mk(0, 0, 1); s.w = 0.0f;
-        return true;
+        s.p = xyz(p0); s.n = xyz(p1); s.wo = xyz(p2);
+        return p0.w > 0.0f;
     }
 };
 

@@ -343,6 +343,9 @@ struct ShadeArgs {
     float guide_alpha;
     GuideDev guide;
     VertexRec rec;
+    // when set: per queue slot, the hit's (pos, 1), (normal facing the ray origin, material index),
+    // (wo, 0) as k_mlp_prep wrote them from fetch_hit (the same floats), in place of the lookup
+    const float4* hit_prep = nullptr;
 };
 
 __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, Q in, Q out, const float4* __restrict__ hits,
@@ -374,7 +377,15 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, Q in, Q out, const
                 L = add(L, mul(thp, sky(dir)));
                 rad[pixel] = make_float4(L.x, L.y, L.z, L4.w);
             } else {
-                HitInfo hi = fetch_hit(S, prim, h4.z, h4.w);
+                HitInfo hi;
+                if (A.hit_prep) {
+                    const float4 p0 = A.hit_prep[(size_t)i * 3], p1 = A.hit_prep[(size_t)i * 3 + 1];
+                    hi.pos = xyz(p0);
+                    hi.normal = xyz(p1);
+                    hi.mat = __float_as_uint(p1.w);
+                } else {
+                    hi = fetch_hit(S, prim, h4.z, h4.w);
+                }
                 Mat m = load_mat(S, hi.mat);
                 if (is_emissive(m)) {
                     uint32_t lo = S.tri_light[prim];
@@ -1088,6 +1099,7 @@ static int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, 
             rc = mlp_infer_queue(c, mlp, S, qi.o, qi.d, (const float4*)Bf.hits, cnt, b, npix,
                                  const_cast<uint16_t*>(A.guide.mlp_lobes));
             if (rc) return rc;
+            Ab.hit_prep = (const float4*)c->mlp_prep.p;  // the hit lookup the network's inputs took
         }
         hipLaunchKernelGGL(k_shade, dim3(g), dim3(kBlock), 0, c->stream, S, qi, qo, (const float4*)Bf.hits, cnt,
                            Bf.sh_o, Bf.sh_d, Bf.sh_c, rad, Ab);
