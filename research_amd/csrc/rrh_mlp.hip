@@ -154,7 +154,8 @@ __device__ __forceinline__ bf16x8 encode8(const MlpDev& M, const Sample& s, bool
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
             q[p] = bf16x2_bits(!MASK || valid ? sn : 0.0f, !MASK || valid ? cs : 0.0f);
-            const float s2 = 2.0f * sn * cs, c2 = 1.0f - 2.0f * sn * sn;
+            const float t2 = sn + sn;  // sin 2x = (2 s) c, cos 2x = 1 - (2 s) s
+            const float s2 = t2 * cs, c2 = __builtin_fmaf(-t2, sn, 1.0f);
             sn = s2;
             cs = c2;
         }
@@ -508,23 +509,28 @@ __device__ __forceinline__ void head_store_t(const f32x4 (&e)[3], bool valid, vo
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
         const float* v = o + 5 * j;
-        const float il = __builtin_amdgcn_rsqf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + 1e-12f);
+        const float il = __builtin_amdgcn_rsqf(__builtin_fmaf(v[0], v[0], __builtin_fmaf(v[1], v[1], v[2] * v[2])) + 1e-12f);
         lb[j * 5 + 0] = v[0] * il;
         lb[j * 5 + 1] = v[1] * il;
         lb[j * 5 + 2] = v[2] * il;
-        lb[j * 5 + 3] = fminf(0.1f + softplus_f(v[3]), 1000.0f);
+        // softplus with the bare base-2 logarithm: its argument 1 + e^d is at least 1
+        const float sp = v[3] > 20.0f ? v[3] : 0.69314718f * __builtin_amdgcn_logf(1.0f + hexp(v[3]));
+        lb[j * 5 + 3] = fminf(0.1f + sp, 1000.0f);
         lb[j * 5 + 4] = (j == 0 ? x0 : x1) * rz;
     }
     if constexpr (H16) {
+        typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
         uint16_t* rec = reinterpret_cast<uint16_t*>(out) + si * kMlpLobeHalves;
-        auto hb = [](float f) { return (uint32_t)__half_as_ushort(__float2half_rn(f)); };
+        auto h2 = [](float a, float b) {  // one v_cvt_pk_f16_f32 (round to nearest even)
+            return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, f16x2));
+        };
         uint4 q;
-        q.x = hb(lb[0]) | hb(lb[1]) << 16;
-        q.y = hb(lb[2]) | hb(lb[3]) << 16;
-        q.z = hb(lb[5]) | hb(lb[6]) << 16;
-        q.w = hb(lb[7]) | hb(lb[8]) << 16;
+        q.x = h2(lb[0], lb[1]);
+        q.y = h2(lb[2], lb[3]);
+        q.z = h2(lb[5], lb[6]);
+        q.w = h2(lb[7], lb[8]);
         *reinterpret_cast<uint4*>(rec + 8 * g) = q;
-        *reinterpret_cast<uint32_t*>(rec + 32 + 2 * g) = hb(lb[4]) | hb(lb[9]) << 16;
+        *reinterpret_cast<uint32_t*>(rec + 32 + 2 * g) = h2(lb[4], lb[9]);
     } else {
         float* dst = reinterpret_cast<float*>(out) + si * (NK * 5) + g * 10;
 #pragma unroll
