@@ -528,7 +528,10 @@ __device__ __forceinline__ void head_store_t(const f32x4 (&e)[3], bool valid, vo
     const float emax = group_max4t(fmaxf(o[4], o[9]));
     const float x0 = hexp(o[4] - emax), x1 = hexp(o[9] - emax);
     const float z = group_sum4t(x0 + x1);
-    const float alpha = select_alpha(__shfl(e[2][2], lane & 15));  // from lane g = 0 of the sample
+    // the selection probability, from lane g = 0 of the sample (a uniform condition: every lane
+    // takes part in the shuffle or none does)
+    float alpha = 1.0f;
+    if (H16 ? select : sel != nullptr) alpha = select_alpha(__shfl(e[2][2], lane & 15));
     if (!valid) return;
     // hardware reciprocal square root and reciprocal (about 1 ulp) in place of IEEE divisions
     const float rz = __builtin_amdgcn_rcpf(z) * (H16 && select ? alpha : 1.0f);
