@@ -1286,6 +1286,10 @@ void orc_di_spatial_shade(const orc_scene* s, const orc_camera* cam, const orc_d
 /* ========================================================================== */
 /* A-trous -- atrous.slang:21-106 (reference mode)                            */
 /* ========================================================================== */
+/* The three edge-stopping weights exp(-d / sigma^2) (atrous.slang:83-96) are evaluated with
+ * orc_wexpf, the fused-multiply-add exponential of the config mode (relative error about 2e-7),
+ * as the kernel evaluates them; the reference's own exp is the driver's approximation, so no
+ * exponential reproduces its bits. */
 static float kernel_weight(int o) { return (o == -2 || o == 2) ? 1.0f : ((o == -1 || o == 1) ? 4.0f : 6.0f); }
 static float luma601(v3 c) { return c.x * 0.299f + c.y * 0.587f + c.z * 0.114f; }
 
@@ -1318,12 +1322,12 @@ void orc_atrous_ref(const orc_atrous_params* p, const float* in, const float* gp
                     v3 sr = vld(in + j * 4);
                     float bw = (kernel_weight(tx) * kernel_weight(ty)) / 256.0f;
                     v3 pd = vsub(vld(sp), vld(cp));
-                    float pw = orc_expf(-vdot(pd, pd) / sps);
+                    float pw = orc_wexpf(-vdot(pd, pd) / sps);
                     v3 sn = vnorm(vld(gnrm + j * 4));
                     float nd = fmaxf(0.0f, 1.0f - vdot(cn, sn));
-                    float nw = orc_expf(-nd / sns);
+                    float nw = orc_wexpf(-nd / sns);
                     float ld = luma601(sr) - cl;
-                    float lw = orc_expf(-(ld * ld) / sls);
+                    float lw = orc_wexpf(-(ld * ld) / sls);
                     float w = bw * pw * nw * lw;
                     acc = vadd(acc, vscale(sr, w));
                     wsum += w;

@@ -100,12 +100,15 @@ __device__ __forceinline__ size_t lat_src(const LatTile& t, int S, int W, int H,
 // LAT: the taps come from the staged sub-lattice window; otherwise every tap is read from
 // global memory (through L2) for a dense TX x TY tile of pixels, which is faster at stride 16,
 // where the lattice window's image reads (one pixel per 16) no longer share cache lines.
+// The staged window holds each element's unit normal and luminance, computed once per element
+// (about 1.6 per output) instead of once per tap: the same operations on the same inputs, so the
+// same bits.  The weights' exponential is wexp_rr (the oracle's orc_wexpf).
 template <bool FAST, bool LAT>
 __global__ void __launch_bounds__(TX* TY) k_atrous_ref(int W, int H, int S, float sps, float sns, float sls,
                                                       const float4* __restrict__ in, const float4* __restrict__ gpos,
                                                       const float4* __restrict__ gnrm, float4* __restrict__ out) {
     __shared__ float4 s_pos[LAT ? LW * LH : 1];
-    __shared__ float4 s_nrm[LAT ? LW * LH : 1];
+    __shared__ float4 s_nrm[LAT ? LW * LH : 1];  // unit normal, luminance of the radiance in w
     __shared__ float4 s_rad[LAT ? LW * LH : 1];
     int bx, by;
     xcd_tile(bx, by);
@@ -113,9 +116,11 @@ __global__ void __launch_bounds__(TX* TY) k_atrous_ref(int W, int H, int S, floa
     if constexpr (LAT) {
         for (int k = threadIdx.x; k < LW * LH; k += TX * TY) {
             const size_t g = lat_src(t, S, W, H, k);
+            const float4 n = gnrm[g], r = in[g];  // (the normal's w is unused)
+            const f3 u = WMath<FAST>::unit(xyz(n));
             s_pos[k] = gpos[g];
-            s_nrm[k] = gnrm[g];
-            s_rad[k] = in[g];
+            s_nrm[k] = make_float4(u.x, u.y, u.z, luma601(xyz(r)));
+            s_rad[k] = r;
         }
         __syncthreads();
     }
@@ -125,37 +130,43 @@ __global__ void __launch_bounds__(TX* TY) k_atrous_ref(int W, int H, int S, floa
     if (x >= W || y >= H) return;
     const int lc = (ty + 2) * LW + (tx + 2);
     size_t i = (size_t)y * W + x;
-    // tap (dx, dy) in lattice steps: its position, normal and radiance
-    auto tap = [&](int dx, int dy, float4& p, float4& n, float4& r) {
-        if constexpr (LAT) {
+    // tap (dx, dy) in lattice steps: its position, unit normal, radiance and luminance; off the
+    // lattice a position with w < 0 (no surface) leaves the normal and luminance unset
+    auto tap = [&](int dx, int dy, float4& p, f3& n, float4& r, float& lum) {
+        if constexpr (LAT) {  // LDS reads, unconditional (no branch per tap)
             const int l = lc + dy * LW + dx;
-            p = s_pos[l]; n = s_nrm[l]; r = s_rad[l];
+            const float4 nl = s_nrm[l];
+            p = s_pos[l]; n = xyz(nl); r = s_rad[l]; lum = nl.w;
         } else {
             const size_t j = (size_t)clampi(y + dy * S, 0, H - 1) * W + clampi(x + dx * S, 0, W - 1);
-            p = gpos[j]; n = gnrm[j]; r = in[j];
+            p = gpos[j];
+            r = in[j];
+            if (p.w < 0.0f) return;
+            n = WMath<FAST>::unit(xyz(gnrm[j])); lum = luma601(xyz(r));
         }
     };
-    float4 cp, cn4, cr;
-    tap(0, 0, cp, cn4, cr);
+    float4 cp, cr;
+    f3 cn;
+    float cl;
+    tap(0, 0, cp, cn, cr, cl);
     if (cp.w < 0.0f) { out[i] = cr; return; }
-    f3 cn = WMath<FAST>::unit(xyz(cn4));
-    float cl = luma601(xyz(cr));
     f3 acc = mk(0, 0, 0);
     float wsum = 0.0f;
     for (int ty2 = -2; ty2 <= 2; ++ty2) {
         for (int tx2 = -2; tx2 <= 2; ++tx2) {
-            float4 sp, sn4, sr4;
-            tap(tx2, ty2, sp, sn4, sr4);
+            float4 sp, sr4;
+            f3 sn;
+            float sl;
+            tap(tx2, ty2, sp, sn, sr4, sl);
             if (sp.w < 0.0f) continue;
             f3 sr = xyz(sr4);
             float bw = (kw(tx2) * kw(ty2)) / 256.0f;
             f3 pd = sub(xyz(sp), xyz(cp));
-            float pw = WMath<FAST>::ex(-WMath<FAST>::div(dot(pd, pd), sps));
-            f3 sn = WMath<FAST>::unit(xyz(sn4));
+            float pw = WMath<FAST>::wex(-WMath<FAST>::div(dot(pd, pd), sps));
             float nd = fmaxf(0.0f, 1.0f - dot(cn, sn));
-            float nw = WMath<FAST>::ex(-WMath<FAST>::div(nd, sns));
-            float ld = luma601(sr) - cl;
-            float lw = WMath<FAST>::ex(-WMath<FAST>::div(ld * ld, sls));
+            float nw = WMath<FAST>::wex(-WMath<FAST>::div(nd, sns));
+            float ld = sl - cl;
+            float lw = WMath<FAST>::wex(-WMath<FAST>::div(ld * ld, sls));
             float w = bw * pw * nw * lw;
             acc = add(acc, scale(sr, w));
             wsum += w;
