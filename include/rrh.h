@@ -229,9 +229,6 @@ typedef struct {
                                       over the whole frame bit for bit */
 } rrh_pt_params;
 #define RRH_PT_COUNT_FETCHES 1u    /* instrumented traversal: count node / triangle fetches (slower) */
-#define RRH_PT_MLP_SELECT 2u       /* RRH_GUIDE_MLP: the network's learned selection probability alpha(x, n, wo)
-                                      replaces guide_alpha at every guided vertex (guide_alpha > 0 still
-                                      switches the guide on, 0 off) */
 typedef struct { uint64_t primary, extension, shadow; } rrh_ray_counts;
 int rrh_pt_render(rrh_ctx* ctx, rrh_scene* s, const rrh_pt_params* p, rrh_image* radiance);
 int rrh_pt_render_guided(rrh_ctx* ctx, rrh_scene* s, const rrh_pt_params* p, rrh_grid* grid,
@@ -520,46 +517,31 @@ int rrh_grid_upload(rrh_ctx* ctx, rrh_grid* g, const void* fitted_or_lobes, size
 
 /* ---- learned guide (new; the reference guides with the single-lobe grid only).
  *      x(78) = 12-frequency sin/cos encoding of the normalised position (72) ++ n ++ wo;
- *      78 -> 64 -> 64 -> 64 -> 64 -> 41 ReLU MLP; outputs 0..39: 8 vMF lobes (v3, d, e):
- *      mu = v/|v|, kappa = 0.1 + softplus(d) (<= 1000), pi = softmax(e); output 40 (s): the
- *      probability of sampling the guide rather than the BSDF, alpha = 0.05 + 0.9 sigmoid(s).
- *      BF16 weights and activations on MFMA with FP32 accumulation, FP32 master weights, Adam,
- *      dynamic loss scaling.  Parameters: W0 (64x78), b0, W1, b1, W2, b2, W3, b3, W4 (41x64), b4.
- *      Loss over n samples (direction d_s, weight w_s >= 0, density ps_s the direction was
- *      drawn from, BSDF sampling density pb_s), with wn = w / sum(w) and q the lobe mixture:
- *        sum_s wn_s (-log q_s(d_s))  +  n sum_s wn_s^2 ps_s / (alpha_s q'_s + (1 - alpha_s) pb_s)
- *      The second term estimates the one-sample MIS blend's relative second moment (variance
- *      + 1); q' is q held constant and alpha is read from the last hidden layer held constant,
- *      so the term trains only the selection output's weights and bias.  Without the
- *      densities it is 0. */
-#define RRH_MLP_PARAMS 20201
+ *      78 -> 64 -> 64 -> 64 -> 64 -> 40 ReLU MLP; 8 vMF lobes: mu = v/|v|,
+ *      kappa = 0.1 + softplus(d) (<= 1000), pi = softmax(e).  BF16 weights and
+ *      activations on MFMA with FP32 accumulation, FP32 master weights, Adam, dynamic
+ *      loss scaling.  Parameters: W0 (64x78), b0, W1, b1, W2, b2, W3, b3, W4 (40x64), b4. */
+#define RRH_MLP_PARAMS 20136
 typedef struct {
     float bounds_min[3], bounds_max[3];   /* position normalisation box */
     float lr, beta1, beta2, eps;          /* Adam (1e-3, 0.9, 0.999, 1e-8) */
     uint32_t seed;                        /* weight initialisation */
     float loss_scale;                     /* initial loss scale (e.g. 1024) */
-    uint32_t flags;                       /* RRH_MLP_* */
 } rrh_mlp_params;
-#define RRH_MLP_TRAIN_SELECT 1u  /* rrh_mlp_train_from_vertices also trains the selection output (the
-                                    blend term below); without it the recorded vertices train the
-                                    lobes alone and the loss is the mixture NLL */
 int rrh_mlp_create(rrh_ctx* ctx, const rrh_mlp_params* p, rrh_mlp** out);
 int rrh_mlp_destroy(rrh_mlp* m);
 int rrh_mlp_set_weights(rrh_ctx* ctx, rrh_mlp* m, const float* params, uint32_t n);
 int rrh_mlp_get_weights(rrh_ctx* ctx, const rrh_mlp* m, float* params, uint32_t n);
-/* lobes: n x 8 x (mu3, kappa, pi); select (may be NULL): n selection probabilities alpha */
+/* lobes: n x 8 x (mu3, kappa, pi) */
 int rrh_mlp_infer(rrh_ctx* ctx, rrh_mlp* m, const float* pos, const float* nrm, const float* wo, uint32_t n,
-                  float* lobes, float* select);
-/* the loss above and its gradient (RRH_MLP_PARAMS floats), no update; sel_pdfs (n x 2 floats:
- * pb_s, ps_s; may be NULL: no selection term) */
+                  float* lobes);
+/* mean weighted negative log-likelihood and its gradient (RRH_MLP_PARAMS floats), no update */
 int rrh_mlp_loss_grad(rrh_ctx* ctx, rrh_mlp* m, const float* pos, const float* nrm, const float* wo,
-                      const float* dirs, const float* w, const float* sel_pdfs, uint32_t n, float* loss,
-                      float* grad);
+                      const float* dirs, const float* w, uint32_t n, float* loss, float* grad);
 /* one training step (forward, backward, Adam) on host samples / on the samples recorded by
- * the last rrh_pt_render_guided with record_vertices (at most `batch` of them; those carry both
- * densities of their sampled direction, so the selection term is on) */
+ * the last rrh_pt_render_guided with record_vertices (at most `batch` of them) */
 int rrh_mlp_train_step(rrh_ctx* ctx, rrh_mlp* m, const float* pos, const float* nrm, const float* wo,
-                       const float* dirs, const float* w, const float* sel_pdfs, uint32_t n, float* loss);
+                       const float* dirs, const float* w, uint32_t n, float* loss);
 int rrh_mlp_train_from_vertices(rrh_ctx* ctx, rrh_mlp* m, uint32_t batch, float* loss);
 /* one Adam step with a given (unscaled) gradient, for checking the optimiser */
 int rrh_mlp_adam_apply(rrh_ctx* ctx, rrh_mlp* m, const float* grad, uint32_t n);

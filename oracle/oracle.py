@@ -137,9 +137,8 @@ _SIGS = {
     "orc_mlp_encode": (None, [FP, FP, FP, FP, FP, P]),
     "orc_mlp_forward": (None, [P, P, P, P]),
     "orc_mlp_head": (None, [P, P]),
-    "orc_mlp_select": (C.c_double, [P]),
     "orc_mlp_pdf": (C.c_double, [P, FP]),
-    "orc_mlp_loss_grad": (C.c_double, [P, P, P, P, P, U, P]),
+    "orc_mlp_loss_grad": (C.c_double, [P, P, P, P, U, P]),
     "orc_adam": (None, [P, P, P, P, U, U, C.c_double, C.c_double, C.c_double, C.c_double]),
 }
 

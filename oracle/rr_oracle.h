@@ -238,12 +238,11 @@ double orc_vmm_pdf(const float* lobes_cell, uint32_t K, const float dir[3]);
 uint32_t orc_mlp_param_count(void);
 void   orc_mlp_encode(const float pos[3], const float bmin[3], const float bmax[3], const float n[3],
                       const float wo[3], double* x78);
-void   orc_mlp_forward(const double* params, const double* x78, double* acts256, double* out41);
-void   orc_mlp_head(const double* out41, double* lobes40);
-double orc_mlp_select(const double* out41);
+void   orc_mlp_forward(const double* params, const double* x78, double* acts256, double* out40);
+void   orc_mlp_head(const double* out40, double* lobes40);
 double orc_mlp_pdf(const double* lobes40, const float dir[3]);
 double orc_mlp_loss_grad(const double* params, const double* x, const float* dirs, const float* w,
-                         const float* bsdf_pdf, uint32_t B, double* grad);
+                         uint32_t B, double* grad);
 void   orc_adam(double* params, const double* grad, double* m, double* v, uint32_t n, uint32_t t,
                 double lr, double b1, double b2, double eps);
 

@@ -8,19 +8,14 @@
  *             u = (p - bmin) / (bmax - bmin)) ++ n (3) ++ wo (3)
  *   h1 = relu(W0 x + b0)   64        h2 = relu(W1 h1 + b1)   64
  *   h3 = relu(W2 h2 + b2)  64        h4 = relu(W3 h3 + b3)   64
- *   o  = W4 h4 + b4        41 = 8 lobes x (v3, d, e), then the selection output s
+ *   o  = W4 h4 + b4        40 = 8 lobes x (v3, d, e)
  *   mu_k = v_k / sqrt(|v_k|^2 + 1e-12), kappa_k = 0.1 + softplus(d_k) (<= 1000),
- *   pi = softmax(e), alpha = 0.05 + 0.9 sigmoid(s) (probability of sampling the guide)
+ *   pi = softmax(e)
  *   q(w) = sum_k pi_k vMF(w; mu_k, kappa_k),  vMF = k e^{k(mu.w - 1)} / (2 pi (1 - e^{-2k}))
- *   loss = sum_s wn_s (-log(q_s(w_s) + 1e-20))
- *        + n sum_s wn_s^2 ps_s / (alpha_s q'_s + (1 - alpha_s) pb_s + 1e-20),   wn = w / sum(w)
- *   over n samples, where pb_s is the BSDF's sampling density of w_s, ps_s the density w_s was
- *   drawn from, q' is q held constant and alpha reads h4 held constant (the second term, the
- *   one-sample MIS blend's relative second moment, trains only W4's row 40 and b4[40]); with
- *   no densities the second term is absent.
+ *   loss = mean_s  wn_s * -log(q_s(w_s) + 1e-20),  wn = w / mean(w)
  *
  * Parameters are one vector: W0 (64x78 row-major, out x in), b0, W1, b1, W2, b2,
- * W3, b3, W4 (41x64), b4 -> 20,201 values.
+ * W3, b3, W4 (40x64), b4 -> 20,136 values.
  */
 #include <math.h>
 #include <string.h>
@@ -30,7 +25,7 @@
 
 #define NI 78
 #define NH 64
-#define NO 41
+#define NO 40
 #define NK 8
 
 static const int kIn[5] = {NI, NH, NH, NH, NH};
@@ -59,7 +54,7 @@ void orc_mlp_encode(const float pos[3], const float bmin[3], const float bmax[3]
     for (int k = 0; k < 3; ++k) { x[72 + k] = n[k]; x[75 + k] = wo[k]; }
 }
 
-/* acts: 4 x 64 post-ReLU hidden activations; out: 41 raw outputs */
+/* acts: 4 x 64 post-ReLU hidden activations; out: 40 raw outputs */
 void orc_mlp_forward(const double* P, const double* x, double* acts, double* out) {
     const double* in = x;
     double tmp[NH];
@@ -98,9 +93,6 @@ void orc_mlp_head(const double* out, double* lobes) {
     }
 }
 
-/* selection probability of the guide from the raw outputs */
-double orc_mlp_select(const double* out) { return 0.05 + 0.9 * sigmoid(out[NK * 5]); }
-
 static double vmf(double kap, double c) {
     return kap * exp(kap * (c - 1.0)) / (2.0 * 3.14159265358979323846 * (1.0 - exp(-2.0 * kap)));
 }
@@ -114,10 +106,9 @@ double orc_mlp_pdf(const double* lobes, const float dir[3]) {
     return q;
 }
 
-/* loss and gradient over a batch: x (B x 78), dirs (B x 3), w (B), pdfs (B x (pb, ps) or NULL)
- * -> grad (param count) */
-double orc_mlp_loss_grad(const double* P, const double* x, const float* dirs, const float* w, const float* pdfs,
-                         uint32_t B, double* grad) {
+/* loss and gradient over a batch: x (B x 78), dirs (B x 3), w (B) -> grad (param count) */
+double orc_mlp_loss_grad(const double* P, const double* x, const float* dirs, const float* w, uint32_t B,
+                         double* grad) {
     uint32_t np = orc_mlp_param_count();
     memset(grad, 0, np * sizeof(double));
     double wmean = 0.0;
@@ -139,17 +130,6 @@ double orc_mlp_loss_grad(const double* P, const double* x, const float* dirs, co
         }
         loss += -ws * log(q + 1e-20);
         double dq = -ws / (q + 1e-20);
-        /* blend term: B ws^2 ps / qb; d/ds = -B ws^2 ps (q - pb) / qb^2 * 0.9 sigmoid'(s) */
-        dout[NK * 5] = 0.0;
-        if (pdfs) {
-            double pb = pdfs[2 * s], ps = pdfs[2 * s + 1];
-            double sg = sigmoid(out[NK * 5]);
-            double a = 0.05 + 0.9 * sg;
-            double qb = a * q + (1.0 - a) * pb + 1e-20;
-            double r = (double)B * ws * ps / qb;
-            loss += ws * r;
-            dout[NK * 5] = -ws * r * (q - pb) / qb * 0.9 * sg * (1.0 - sg);
-        }
         for (int k = 0; k < NK; ++k) {
             const double* v = out + k * 5;
             double pik = lobes[k * 5 + 4], kap = lobes[k * 5 + 3];
@@ -181,8 +161,7 @@ double orc_mlp_loss_grad(const double* P, const double* x, const float* dirs, co
             if (l == 0) break;
             for (int i = 0; i < kIn[l]; ++i) {
                 double sacc = 0.0;
-                /* the selection output (l = 4, o = 40) reads h4 held constant */
-                for (int o = 0; o < (l == 4 ? NK * 5 : kOut[l]); ++o) sacc += W[(size_t)o * kIn[l] + i] * dcur[o];
+                for (int o = 0; o < kOut[l]; ++o) sacc += W[(size_t)o * kIn[l] + i] * dcur[o];
                 dh[i] = (in[i] > 0.0) ? sacc : 0.0; /* ReLU' from the post-activation */
             }
             memcpy(dprev, dh, sizeof(dh));

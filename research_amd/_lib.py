@@ -36,8 +36,6 @@ FMT_RGBA32F, FMT_RGBA16F, FMT_RGBA64F, FMT_R32U, FMT_RGBA8 = 0, 1, 2, 3, 4
 FMT_BYTES = {FMT_RGBA32F: 16, FMT_RGBA16F: 8, FMT_RGBA64F: 32, FMT_R32U: 4, FMT_RGBA8: 4}
 GUIDE_NONE, GUIDE_VMM, GUIDE_MLP = 0, 1, 2
 PT_COUNT_FETCHES = 1
-PT_MLP_SELECT = 2
-MLP_TRAIN_SELECT = 1
 OPT_TRAVERSAL_STACK, OPT_XCD_REMAP, OPT_BVH_NODES, OPT_PERSISTENT, OPT_SPECULATIVE = 1, 2, 3, 4, 5
 OPT_BVH_MAX_LEAF, OPT_BVH_TRAV_COST = 6, 7
 OPT_BVH_BUILDER, OPT_PLOC_RADIUS, OPT_RAY_SORT, OPT_RESTIR_CULL = 8, 9, 10, 11
@@ -123,10 +121,10 @@ class GridParams(C.Structure):
 class MlpParams(C.Structure):
     _fields_ = [("bounds_min", C.c_float * 3), ("bounds_max", C.c_float * 3), ("lr", C.c_float),
                 ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint32),
-                ("loss_scale", C.c_float), ("flags", C.c_uint32)]
+                ("loss_scale", C.c_float)]
 
 
-MLP_PARAMS = 20201
+MLP_PARAMS = 20136
 
 P = C.c_void_p
 PP = C.POINTER(C.c_void_p)
@@ -213,9 +211,9 @@ SIGNATURES = {
     "rrh_mlp_destroy": [P],
     "rrh_mlp_set_weights": [P, P, P, u32],
     "rrh_mlp_get_weights": [P, P, P, u32],
-    "rrh_mlp_infer": [P, P, P, P, P, u32, P, P],
-    "rrh_mlp_loss_grad": [P, P, P, P, P, P, P, P, u32, P, P],
-    "rrh_mlp_train_step": [P, P, P, P, P, P, P, P, u32, P],
+    "rrh_mlp_infer": [P, P, P, P, P, u32, P],
+    "rrh_mlp_loss_grad": [P, P, P, P, P, P, P, u32, P, P],
+    "rrh_mlp_train_step": [P, P, P, P, P, P, P, u32, P],
     "rrh_mlp_train_from_vertices": [P, P, u32, P],
     "rrh_mlp_adam_apply": [P, P, P, u32],
     "rrh_mlp_state": [P, P, P, P],

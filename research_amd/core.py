@@ -283,17 +283,14 @@ class Grid:
 
 
 class Mlp:
-    """Learned guide: 78 -> 64x4 -> 41 BF16 MLP producing an 8-lobe vMF mixture and the
-    probability of sampling it rather than the BSDF."""
+    """Learned guide: 78 -> 64x4 -> 40 BF16 MLP producing an 8-lobe vMF mixture."""
 
     NPARAM = L.MLP_PARAMS
 
     def __init__(self, ctx: Context, bounds_min, bounds_max, lr: float = 1e-3, beta1: float = 0.9,
-                 beta2: float = 0.999, eps: float = 1e-8, seed: int = 1, loss_scale: float = 1024.0,
-                 train_select: bool = False):
+                 beta2: float = 0.999, eps: float = 1e-8, seed: int = 1, loss_scale: float = 1024.0):
         self.ctx = ctx
-        self.p = L.MlpParams(L.f3(bounds_min), L.f3(bounds_max), lr, beta1, beta2, eps, seed, loss_scale,
-                             L.MLP_TRAIN_SELECT if train_select else 0)
+        self.p = L.MlpParams(L.f3(bounds_min), L.f3(bounds_max), lr, beta1, beta2, eps, seed, loss_scale)
         h = C.c_void_p()
         L.call("rrh_mlp_create", ctx.h, C.byref(self.p), C.byref(h))
         self.h = h
@@ -311,35 +308,26 @@ class Mlp:
     def _v3(a) -> np.ndarray:
         return np.ascontiguousarray(a, dtype=np.float32).reshape(-1, 3)
 
-    def infer(self, pos, nrm, wo, select: bool = False):
-        """lobes (n, 8, 5); with select, also the selection probabilities (n,)"""
+    def infer(self, pos, nrm, wo) -> np.ndarray:
         pos, nrm, wo = self._v3(pos), self._v3(nrm), self._v3(wo)
         out = np.zeros((len(pos), 8, 5), np.float32)
-        sel = np.zeros(len(pos), np.float32) if select else None
-        L.call("rrh_mlp_infer", self.ctx.h, self.h, L.ptr(pos), L.ptr(nrm), L.ptr(wo), len(pos), L.ptr(out),
-               L.ptr(sel) if select else None)
-        return (out, sel) if select else out
+        L.call("rrh_mlp_infer", self.ctx.h, self.h, L.ptr(pos), L.ptr(nrm), L.ptr(wo), len(pos), L.ptr(out))
+        return out
 
-    @staticmethod
-    def _opt(a):
-        return None if a is None else np.ascontiguousarray(a, dtype=np.float32)
-
-    def loss_grad(self, pos, nrm, wo, dirs, w, sel_pdfs=None):
-        """sel_pdfs: (n, 2) BSDF sampling density and drawing density of each direction (the
-        selection output's term), or None"""
+    def loss_grad(self, pos, nrm, wo, dirs, w):
         pos, nrm, wo, dirs = self._v3(pos), self._v3(nrm), self._v3(wo), self._v3(dirs)
-        w, pb = np.ascontiguousarray(w, dtype=np.float32), self._opt(sel_pdfs)
+        w = np.ascontiguousarray(w, dtype=np.float32)
         loss, g = C.c_float(), np.zeros(self.NPARAM, np.float32)
         L.call("rrh_mlp_loss_grad", self.ctx.h, self.h, L.ptr(pos), L.ptr(nrm), L.ptr(wo), L.ptr(dirs), L.ptr(w),
-               L.ptr(pb) if pb is not None else None, len(w), C.byref(loss), L.ptr(g))
+               len(w), C.byref(loss), L.ptr(g))
         return loss.value, g
 
-    def train_step(self, pos, nrm, wo, dirs, w, sel_pdfs=None) -> float:
+    def train_step(self, pos, nrm, wo, dirs, w) -> float:
         pos, nrm, wo, dirs = self._v3(pos), self._v3(nrm), self._v3(wo), self._v3(dirs)
-        w, pb = np.ascontiguousarray(w, dtype=np.float32), self._opt(sel_pdfs)
+        w = np.ascontiguousarray(w, dtype=np.float32)
         loss = C.c_float()
         L.call("rrh_mlp_train_step", self.ctx.h, self.h, L.ptr(pos), L.ptr(nrm), L.ptr(wo), L.ptr(dirs), L.ptr(w),
-               L.ptr(pb) if pb is not None else None, len(w), C.byref(loss))
+               len(w), C.byref(loss))
         return loss.value
 
     def train_from_vertices(self, batch: int) -> float:

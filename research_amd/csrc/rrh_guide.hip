@@ -263,8 +263,7 @@ __global__ void k_vmm_mask(uint32_t ncell, uint32_t K, const float* __restrict__
 }
 
 // ---- path-vertex finalisation: incident radiance along the sampled direction ----
-// training sample: t0 (pos, lum(Li) / pdf)  t1 (wi, pdf)  t2 (n, lum(Li f cos) / pdf)
-//                  t3 (wo, BSDF sampling density of wi)
+// training sample: t0 (pos, lum(Li) / pdf)  t1 (wi, pdf)  t2 (n, lum(Li f cos) / pdf)  t3 (wo)
 // The vertex's training sample, if it is kept (deterministic: a hash of the record index
 // thins the live vertices to about the requested count).
 __device__ __forceinline__ bool finalize_one(const float4* __restrict__ rec, uint32_t i, uint32_t max_bounces,
@@ -298,7 +297,7 @@ __device__ __forceinline__ bool finalize_one(const float4* __restrict__ rec, uin
             f3 bw = xyz(r7);
             const float wp = fmaxf(lum709(mk(Li.x * bw.x, Li.y * bw.y, Li.z * bw.z)), 0.0f);
             t2 = make_float4(r1.x, r1.y, r1.z, isfinite(wp) ? wp : 0.0f);
-            t3 = make_float4(r2.x, r2.y, r2.z, r7.w);
+            t3 = make_float4(r2.x, r2.y, r2.z, 0.0f);
         }
     }
     return take;
@@ -430,7 +429,6 @@ int guide_prepare(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* gr
         int rc = c->scratch.ensure((size_t)npix * kMlpLobeHalves * 2);
         if (rc) return rc;
         G.mlp_lobes = (const uint16_t*)c->scratch.p;
-        G.mlp_select = (p->flags & RRH_PT_MLP_SELECT) != 0u;
         G.K = kMaxLobes;
     }
     if (p->record_vertices) {

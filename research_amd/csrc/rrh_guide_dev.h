@@ -79,7 +79,6 @@ RRH_D f3 hash_primary_dir(const HashGrid& G, f3 p, f3 N, uint32_t bsdf, f3 cam, 
 struct GuideDev {
     const float* lobes = nullptr;         // VMM: res^3 cells x K x (mu3, kappa, pi)
     const uint16_t* mlp_lobes = nullptr;  // MLP: per ray-queue slot kMlpLobeHalves fp16 (LobesH16)
-    bool mlp_select = false;              // MLP: the record's weights carry the learned selection alpha
     uint32_t res = 0, K = 0;
     float bmin[3] = {0, 0, 0}, bext[3] = {1, 1, 1};
 };
@@ -89,7 +88,7 @@ struct GuideDev {
 // recorded frame):
 //  r0 (pos, stamp) r1 (n, pdf) r2 (wo, mat) r3 (wi, nee visible)
 //  r4 (L before this vertex, alive) r5 (NEE contribution if visible) r6 (throughput after)
-//  r7 (BSDF weight f cos / pdf before Russian roulette, BSDF sampling density of wi)
+//  r7 (BSDF weight f cos / pdf before Russian roulette)
 struct VertexRec {
     float4* v = nullptr;
     uint32_t max_bounces = 0;
@@ -111,8 +110,7 @@ RRH_D uint32_t vmm_cell(const GuideDev& G, f3 p) {
 }
 
 // lobe accessors of the one-sample MIS below: fp32 (mu3, kappa, pi) records (the EM grid's cell),
-// or the learned guide's fp16 record of one queue slot: 8 x (mu.xyz, kappa) then the 8 weights
-// (with the learned selection: alpha pi_k, so the weights sum to the selection probability).
+// or the learned guide's fp16 record of one queue slot: 8 x (mu.xyz, kappa) then the 8 weights.
 // A fp16 mu is renormalised wherever it is read, so sampling and the pdf use the same unit
 // directions and the estimate stays unbiased with the rounded lobes.
 struct LobesF32 {
@@ -147,14 +145,13 @@ RRH_D float mixture_pdf(const Lobes& lc, uint32_t K, f3 wi) {
 }
 
 // one-sample MIS with the mixture lc (K lobes; K = 0: no guide): with probability alpha sample
-// the mixture, else the BSDF; pdf = alpha * p_guide + (1 - alpha) * p_bsdf; returns f(wo, wi).
-// learned: the weights sum to the selection probability, which replaces alpha (alpha > 0: on)
+// the mixture, else the BSDF; pdf = alpha * p_guide + (1 - alpha) * p_bsdf; returns f(wo, wi)
 template <class Lobes>
 RRH_D f3 mixture_guided_sample(const Lobes& lc, uint32_t K, float alpha, f3 n, f3 wo, const Mat& m, uint32_t& seed,
-                               f3& wi, float& pdf, bool learned = false) {
+                               f3& wi, float& pdf) {
     float tot = 0.0f;
     for (uint32_t k = 0; k < K; ++k) tot += fmaxf(lc.pi(k), 0.0f);
-    float a = (tot > 0.0f && alpha > 0.0f) ? (learned ? tot : alpha) : 0.0f;
+    float a = (tot > 0.0f) ? alpha : 0.0f;
     float usel = rnd(seed);
     if (usel < a) {
         float uk = rnd(seed) * tot;
@@ -184,7 +181,7 @@ RRH_D f3 guided_sample(const GuideDev& G, uint32_t mode, float alpha, const DevS
     (void)S;
     if (mode == RRH_GUIDE_MLP && G.mlp_lobes)
         return mixture_guided_sample(LobesH16{reinterpret_cast<const uint32_t*>(G.mlp_lobes + (size_t)slot * kMlpLobeHalves)},
-                                     kMaxLobes, alpha, n, wo, m, seed, wi, pdf, G.mlp_select);
+                                     kMaxLobes, alpha, n, wo, m, seed, wi, pdf);
     const bool vmm = mode == RRH_GUIDE_VMM && G.lobes;
     return mixture_guided_sample(LobesF32{vmm ? G.lobes + (size_t)vmm_cell(G, pos) * G.K * 5 : nullptr}, vmm ? G.K : 0u,
                                  alpha, n, wo, m, seed, wi, pdf);
@@ -192,7 +189,7 @@ RRH_D f3 guided_sample(const GuideDev& G, uint32_t mode, float alpha, const DevS
 
 RRH_D bool guide_records(const VertexRec& R, uint32_t pixel) { return (pixel & ((1u << R.shift) - 1u)) == R.phase; }
 RRH_D void guide_record_vertex(const VertexRec& R, uint32_t pixel, uint32_t b, f3 pos, f3 n, f3 wo, f3 wi, float pdf,
-                               f3 bsdf_w, f3 thp_after, f3 L_pre, f3 nee_c, uint32_t mat, bool alive, float bsdf_pdf) {
+                               f3 bsdf_w, f3 thp_after, f3 L_pre, f3 nee_c, uint32_t mat, bool alive) {
     if (!guide_records(R, pixel)) return;
     float4* r = R.v + ((size_t)(pixel >> R.shift) * R.max_bounces + b) * kRecF4;
     r[0] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(R.stamp));
@@ -202,7 +199,7 @@ RRH_D void guide_record_vertex(const VertexRec& R, uint32_t pixel, uint32_t b, f
     r[4] = make_float4(L_pre.x, L_pre.y, L_pre.z, alive ? 1.0f : 0.0f);
     r[5] = make_float4(nee_c.x, nee_c.y, nee_c.z, 0.0f);
     r[6] = make_float4(thp_after.x, thp_after.y, thp_after.z, 0.0f);
-    r[7] = make_float4(bsdf_w.x, bsdf_w.y, bsdf_w.z, bsdf_pdf);
+    r[7] = make_float4(bsdf_w.x, bsdf_w.y, bsdf_w.z, 0.0f);
 }
 RRH_D void guide_record_nee_visible(const VertexRec& R, uint32_t pixel, uint32_t b) {
     if (!guide_records(R, pixel)) return;
@@ -216,8 +213,7 @@ int guide_prepare(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* gr
 int guide_finish(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* grid, rrh_mlp* mlp, const VertexRec& R,
                  const float4* rad);
 // learned guide: the fp16 lobe record (kMlpLobeHalves) of every hit of queue b (rrh_mlp.hip)
-// (select: the weights are stored as alpha pi_k, alpha the learned selection probability)
 int mlp_infer_queue(rrh_ctx* c, rrh_mlp* m, const DevScene& S, const float4* qo, const float4* qd, const float4* hits,
-                    const uint32_t* cnt, uint32_t b, uint32_t cap, uint16_t* lobes, bool select);
+                    const uint32_t* cnt, uint32_t b, uint32_t cap, uint16_t* lobes);
 
 }  // namespace rrh

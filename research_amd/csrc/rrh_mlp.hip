@@ -4,8 +4,7 @@
 // GuidingGrid.slang); the network and loss are defined in include/rrh.h.
 //
 // Training step = k_mlp_wsum (per-block sums of sample weights) -> k_mlp_train (persistent blocks:
-// per 64-sample chunk, forward through five BF16 MFMA layers, vMF-mixture NLL head (and the
-// selection probability's blend term),
+// per 64-sample chunk, forward through five BF16 MFMA layers, vMF-mixture NLL head,
 // backward deltas, then the block's weight-gradient tiles, kept in registers across
 // chunks) -> k_mlp_reduce1/2 (fixed-order sum of the per-block partials, non-finite
 // check) -> k_mlp_adam (FP32 master weights) -> k_mlp_pack (BF16 copies for the next
@@ -23,7 +22,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 namespace {
 
-constexpr int NI = 78, KI = 96, NH = 64, NO = 41, NOP = 48, NK = 8;  // outputs: 8 x 5 lobe values, selection
+constexpr int NI = 78, KI = 96, NH = 64, NO = 40, NOP = 48, NK = 8;
 constexpr int NPARAM = RRH_MLP_PARAMS;
 constexpr int kBlockSamples = 64;   // 4 waves x 16 samples
 constexpr int kTiles = 80;          // weight-gradient tiles (16x16) over all layers
@@ -65,21 +64,19 @@ __device__ __forceinline__ const __bf16* WTl(const MlpDev& M, int l) { return M.
 // holds features 16 nt + 4 g + j (j < 4) of sample r.  Tiles 2 kk and 2 kk + 1 together hold
 // what the next layer's B operand needs at "virtual" columns 32 kk + 8 g + i, so the inference
 // fragments of layers 1..4 hold column v = feature perm_k(v), and layer 4's rows are ordered so
-// that lane g of sample r receives the raw outputs of lobes 2g and 2g + 1 (perm_out4), and lane
-// 0 of the sample the selection output in its eleventh slot.
+// that lane g of sample r receives the raw outputs of lobes 2g and 2g + 1 (perm_out4).
 __host__ __device__ constexpr int perm_k(int v) {
     return 32 * (v >> 5) + ((v & 7) < 4 ? 4 * ((v >> 3) & 3) + (v & 7) : 16 + 4 * ((v >> 3) & 3) + (v & 7) - 4);
 }
 __host__ __device__ constexpr int perm_out4(int o) {  // permuted row of layer 4 -> raw output, -1 = padding
-    return (4 * (o >> 4) + (o & 3)) < 10 ? 10 * ((o >> 2) & 3) + 4 * (o >> 4) + (o & 3)
-         : ((4 * (o >> 4) + (o & 3)) == 10 && ((o >> 2) & 3) == 0 ? NK * 5 : -1);
+    return (4 * (o >> 4) + (o & 3)) < 10 ? 10 * ((o >> 2) & 3) + 4 * (o >> 4) + (o & 3) : -1;
 }
 
 // ---- inputs --------------------------------------------------------------------------
-struct Sample { f3 p, n, wo, dir; float w, pb, ps; };  // pb, ps: BSDF / sampling density of dir (pb < 0: not given)
+struct Sample { f3 p, n, wo, dir; float w; };
 
 struct HostSrc {  // arrays uploaded from the host (3 floats per vector)
-    const float *pos, *nrm, *wo, *dir, *w, *pdf2;  // pdf2: (pb, ps) per sample
+    const float *pos, *nrm, *wo, *dir, *w;
     uint32_t n;
     __device__ uint32_t count() const { return n; }
     __device__ float weight(uint32_t i) const { return w[i]; }
@@ -96,8 +93,6 @@ struct HostSrc {  // arrays uploaded from the host (3 floats per vector)
         s.wo = mk(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]);
         s.dir = dir ? mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]) : mk(0, 0, 1);
         s.w = w ? w[i] : 0.0f;
-        s.pb = pdf2 ? pdf2[2 * i] : -1.0f;
-        s.ps = pdf2 ? pdf2[2 * i + 1] : 0.0f;
         return true;
     }
 };
@@ -109,7 +104,6 @@ struct TrainSrc {
     const float4* tr;
     const uint32_t* cnt;
     uint32_t cap, batch;
-    bool select;  // the samples' densities feed the selection output's term
     __device__ uint32_t avail() const { return min(*cnt, cap); }
     __device__ uint32_t count() const { return min(avail(), batch); }
     __device__ uint32_t pick(uint32_t j, uint32_t a) const { return (a > batch) ? (uint32_t)(((uint64_t)j * a) / batch) : j; }
@@ -120,8 +114,6 @@ struct TrainSrc {
         const uint32_t i = pick(j, a);
         float4 t0 = tr[(size_t)i * 4], t1 = tr[(size_t)i * 4 + 1], t2 = tr[(size_t)i * 4 + 2], t3 = tr[(size_t)i * 4 + 3];
         s.p = xyz(t0); s.w = t2.w; s.dir = xyz(t1); s.n = xyz(t2); s.wo = xyz(t3);  // product-weighted target
-        s.pb = select ? t3.w : -1.0f;
-        s.ps = t1.w;
         return true;
     }
 };
@@ -270,14 +262,9 @@ __device__ __forceinline__ void head_lobes2(const float* o, int qd, Lobe* lb) {
 __device__ __forceinline__ float vmf_val(float kap, float c) {
     return kap * hexp(kap * (c - 1.0f)) / (6.28318530717958647692f * (1.0f - hexp(-2.0f * kap)));
 }
-// selection probability from the selection output: alpha = 0.05 + 0.9 sigmoid(s)
-__device__ __forceinline__ float select_alpha(float s) { return 0.05f + 0.9f * sigmoid_f(s); }
 // NLL head of one sample on its 4 lanes: returns the loss term (on lane quarter 0, else 0)
-// and writes dL/dout (times `gscale`) of this lane's two lobes into g[10] and of the selection
-// output into gsel.  pb >= 0: the blend's second-moment term n wn^2 ps / (alpha q + (1 - alpha) pb)
-// with q held fixed (nf = n, the batch's sample count)
-__device__ __forceinline__ float head_grad2(const float* o, int qd, f3 d, float wn, float pb, float ps, float nf,
-                                           float gscale, float* g, float& gsel) {
+// and writes dL/dout (times `gscale`) of this lane's two lobes into g[10]
+__device__ __forceinline__ float head_grad2(const float* o, int qd, f3 d, float wn, float gscale, float* g) {
     Lobe lb[2];
     head_lobes2(o, qd, lb);
     float c[2], f[2];
@@ -289,16 +276,6 @@ __device__ __forceinline__ float head_grad2(const float* o, int qd, f3 d, float 
     const float q = group_sum4(lb[0].pi * f[0] + lb[1].pi * f[1]);
     const bool on = wn > 0.0f;
     const float dq = on ? -wn / (q + 1e-20f) * gscale : 0.0f;
-    float lsel = 0.0f;
-    gsel = 0.0f;
-    if (on && pb >= 0.0f) {
-        const float sg = sigmoid_f(o[NK * 5]);
-        const float al = 0.05f + 0.9f * sg;
-        const float qb = al * q + (1.0f - al) * pb + 1e-20f;
-        const float rr = nf * wn * ps / qb;  // the estimate this direction gives under the blend, over the mean
-        lsel = wn * rr;
-        gsel = -wn * rr * (q - pb) / qb * 0.9f * sg * (1.0f - sg) * gscale;
-    }
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
         const float* v = o + (2 * qd + j) * 5;
@@ -316,7 +293,7 @@ __device__ __forceinline__ float head_grad2(const float* o, int qd, f3 d, float 
         g[j * 5 + 1] = (g1 - lb[j].mu[1] * mg) / len;
         g[j * 5 + 2] = (g2 - lb[j].mu[2] * mg) / len;
     }
-    return (on && qd == 0) ? -wn * hlog(q + 1e-20f) + lsel : 0.0f;
+    return (on && qd == 0) ? -wn * hlog(q + 1e-20f) : 0.0f;
 }
 
 // ---- shared memory of a 4-wave block (64 samples) -------------------------------------
@@ -340,7 +317,7 @@ __device__ void forward_chunk(const MlpDev& M, const Src& src, uint32_t base, __
         const int sl = lane >> 2, q = lane & 3;
         Sample s;
         bool v = src.get(base + w * 16 + sl, s);
-        if (!v) { s.p = s.n = s.wo = s.dir = mk(0, 0, 0); s.w = 0.0f; s.pb = -1.0f; s.ps = 0.0f; }
+        if (!v) { s.p = s.n = s.wo = s.dir = mk(0, 0, 0); s.w = 0.0f; }
         encode_row(M, s, v, q, X + (w * 16 + sl) * KI);
         mine = s;  // every lane of the sample keeps it (the head runs on all four)
         valid = v;
@@ -387,14 +364,13 @@ __global__ void __launch_bounds__(256) k_mlp_train(MlpDev M, Src src, uint32_t n
         {
             const int sl = lane >> 2, qd = lane & 3;
             const float wn = valid ? mine.w * inv_w : 0.0f;
-            float g[10], gsel;
-            loss_acc += head_grad2(sm.O(w) + sl * NOP, qd, mine.dir, wn, mine.pb, mine.ps, (float)n, gscale, g, gsel);
+            float g[10];
+            loss_acc += head_grad2(sm.O(w) + sl * NOP, qd, mine.dir, wn, gscale, g);
             __bf16* Drow = sm.D[4] + (w * 16 + sl) * 64;
 #pragma unroll
             for (int k = 0; k < 10; ++k) Drow[qd * 10 + k] = (__bf16)g[k];
-            // rows 40..63: the selection output's delta, then zero padding
 #pragma unroll
-            for (int k = 0; k < 6; ++k) Drow[NK * 5 + qd * 6 + k] = (__bf16)((qd == 0 && k == 0) ? gsel : 0.0f);
+            for (int k = 0; k < 6; ++k) Drow[NO + qd * 6 + k] = (__bf16)0.0f;
         }
         __syncthreads();
         // deltas: D3 = (D4 W4) * (H3 > 0) ... D0 = (D1 W1) * (H0 > 0)
@@ -514,27 +490,21 @@ __device__ __forceinline__ void hidden2(const bf16x8* s_w, const float* bias, in
     }
 }
 // head of one tile: this lane holds the raw outputs of lobes 2g (o[0..4]) and 2g + 1 (o[5..9])
-// of sample `si` (lane g = 0 also the selection output, in e[2][2]); when the sample is valid,
-// writes them as (mu3, kappa, pi) floats (H16 false: 40 floats per sample, and the selection
-// probability into sel[si] when sel is given) or as the fp16 record LobesH16 reads (8 x (mu3,
-// kappa), then the 8 weights, each times the selection probability when `select`)
+// of sample `si`; when the sample is valid, writes them as (mu3, kappa, pi) floats (H16 false:
+// 40 floats per sample) or as the fp16 record LobesH16 reads (8 x (mu3, kappa), then
+// the 8 weights)
 template <bool H16>
-__device__ __forceinline__ void head_store_t(const f32x4 (&e)[3], bool valid, void* out, size_t si, float* sel,
-                                             bool select) {
-    const int lane = threadIdx.x & 63, g = lane >> 4;
+__device__ __forceinline__ void head_store_t(const f32x4 (&e)[3], bool valid, void* out, size_t si) {
+    const int g = (threadIdx.x & 63) >> 4;
     float o[10];
 #pragma unroll
     for (int q = 0; q < 10; ++q) o[q] = e[q >> 2][q & 3];
     const float emax = group_max4t(fmaxf(o[4], o[9]));
     const float x0 = hexp(o[4] - emax), x1 = hexp(o[9] - emax);
     const float z = group_sum4t(x0 + x1);
-    // the selection probability, from lane g = 0 of the sample (a uniform condition: every lane
-    // takes part in the shuffle or none does)
-    float alpha = 1.0f;
-    if (H16 ? select : sel != nullptr) alpha = select_alpha(__shfl(e[2][2], lane & 15));
     if (!valid) return;
     // hardware reciprocal square root and reciprocal (about 1 ulp) in place of IEEE divisions
-    const float rz = __builtin_amdgcn_rcpf(z) * (H16 && select ? alpha : 1.0f);
+    const float rz = __builtin_amdgcn_rcpf(z);
     float lb[10];
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
@@ -565,7 +535,6 @@ __device__ __forceinline__ void head_store_t(const f32x4 (&e)[3], bool valid, vo
         float* dst = reinterpret_cast<float*>(out) + si * (NK * 5) + g * 10;
 #pragma unroll
         for (int q = 0; q < 10; ++q) dst[q] = lb[q];
-        if (sel && g == 0) sel[si] = alpha;
     }
 }
 
@@ -574,8 +543,7 @@ __device__ __forceinline__ void head_store_t(const f32x4 (&e)[3], bool valid, vo
 constexpr int kInferWaves = 8;
 template <class Src, bool H16>
 __global__ void __launch_bounds__(64 * kInferWaves, 4) k_mlp_infer(MlpDev M, Src src, uint32_t nsamples,
-                                                               void* __restrict__ lobes, float* __restrict__ sel,
-                                                               bool select) {
+                                                               void* __restrict__ lobes) {
     __shared__ bf16x8 s_w[kFrags * 64];
     __shared__ __align__(16) float s_bias[4 * 64 + 48];
     const uint4* gw = reinterpret_cast<const uint4*>(M.pk + kFWOff);
@@ -660,7 +628,7 @@ __global__ void __launch_bounds__(64 * kInferWaves, 4) k_mlp_infer(MlpDev M, Src
         }
 #pragma unroll
         for (int t = 0; t < 2; ++t)
-            head_store_t<H16>(e[t], valid[t], lobes, (size_t)(base + t * 16 + r), sel, select);
+            head_store_t<H16>(e[t], valid[t], lobes, (size_t)(base + t * 16 + r));
     }
 }
 
@@ -756,8 +724,7 @@ __global__ void k_mlp_pack(const float* __restrict__ P, __bf16* __restrict__ pk)
         int local = e - kWTOff;
         int l = 1 + local / 4096;
         int i = (local % 4096) / 64, o = local % 64;
-        // the selection output reads the last hidden layer held constant: no delta flows back from it
-        if (o < out_of(l) && i < in_of(l) && !(l == 4 && o == NK * 5)) val = P[off_w(l) + o * in_of(l) + i];
+        if (o < out_of(l) && i < in_of(l)) val = P[off_w(l) + o * in_of(l) + i];
     } else {
         const int local = e - kFWOff;
         const int f = local / 512, lane = (local % 512) / 8, i = local % 8;
@@ -859,20 +826,20 @@ uint32_t infer_grid(rrh_ctx* c, rrh_mlp* m, K kernel, uint32_t n) {
 }
 
 struct HostTmp {
-    void* p[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    void* p[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     ~HostTmp() { for (void* q : p) if (q) (void)hipFree(q); }
 };
 int upload_samples(rrh_ctx* c, const float* pos, const float* nrm, const float* wo, const float* dir, const float* w,
-                   const float* pdf2, uint32_t n, HostTmp& T, HostSrc& src) {
-    const float* h[6] = {pos, nrm, wo, dir, w, pdf2};
-    size_t bytes[6] = {(size_t)n * 12, (size_t)n * 12, (size_t)n * 12, (size_t)n * 12, (size_t)n * 4, (size_t)n * 8};
-    for (int k = 0; k < 6; ++k) {
+                   uint32_t n, HostTmp& T, HostSrc& src) {
+    const float* h[5] = {pos, nrm, wo, dir, w};
+    size_t bytes[5] = {(size_t)n * 12, (size_t)n * 12, (size_t)n * 12, (size_t)n * 12, (size_t)n * 4};
+    for (int k = 0; k < 5; ++k) {
         if (!h[k]) continue;
         RRH_HIP(hipMalloc(&T.p[k], bytes[k]));
         RRH_HIP(hipMemcpyAsync(T.p[k], h[k], bytes[k], hipMemcpyHostToDevice, c->stream));
     }
     src = HostSrc{(const float*)T.p[0], (const float*)T.p[1], (const float*)T.p[2], (const float*)T.p[3],
-                  (const float*)T.p[4], (const float*)T.p[5], n};
+                  (const float*)T.p[4], n};
     return RRH_OK;
 }
 
@@ -958,22 +925,20 @@ int rrh_mlp_get_weights(rrh_ctx* c, const rrh_mlp* m, float* params, uint32_t n)
 }
 
 int rrh_mlp_infer(rrh_ctx* c, rrh_mlp* m, const float* pos, const float* nrm, const float* wo, uint32_t n,
-                  float* lobes, float* select) {
+                  float* lobes) {
     RRH_ARG(c && m && pos && nrm && wo && lobes, "rrh_mlp_infer: null argument");
     if (n == 0) return RRH_OK;
     HostTmp T;
     HostSrc src;
-    int rc = upload_samples(c, pos, nrm, wo, nullptr, nullptr, nullptr, n, T, src);
+    int rc = upload_samples(c, pos, nrm, wo, nullptr, nullptr, n, T, src);
     if (rc) return rc;
     float* dl = nullptr;
-    RRH_HIP(hipMalloc(&dl, (size_t)n * (NK * 5 + 1) * 4));
-    float* ds = dl + (size_t)n * NK * 5;  // selection probabilities
+    RRH_HIP(hipMalloc(&dl, (size_t)n * NK * 5 * 4));
     const uint32_t g = infer_grid(c, m, k_mlp_infer<HostSrc, false>, n);
     hipLaunchKernelGGL((k_mlp_infer<HostSrc, false>), dim3(g), dim3(64 * kInferWaves), 0, c->stream, m->dev(), src, n,
-                       (void*)dl, ds, false);
+                       (void*)dl);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(lobes, dl, (size_t)n * NK * 5 * 4, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess && select) e = hipMemcpyAsync(select, ds, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(dl);
     if (e != hipSuccess) { set_error(std::string("rrh_mlp_infer: ") + hipGetErrorString(e)); return RRH_E_HIP; }
@@ -981,12 +946,12 @@ int rrh_mlp_infer(rrh_ctx* c, rrh_mlp* m, const float* pos, const float* nrm, co
 }
 
 int rrh_mlp_loss_grad(rrh_ctx* c, rrh_mlp* m, const float* pos, const float* nrm, const float* wo, const float* dirs,
-                      const float* w, const float* sel_pdfs, uint32_t n, float* loss, float* grad) {
+                      const float* w, uint32_t n, float* loss, float* grad) {
     RRH_ARG(c && m && pos && nrm && wo && dirs && w && loss && grad, "rrh_mlp_loss_grad: null argument");
     RRH_ARG(n > 0, "rrh_mlp_loss_grad: no samples");
     HostTmp T;
     HostSrc src;
-    int rc = upload_samples(c, pos, nrm, wo, dirs, w, sel_pdfs, n, T, src);
+    int rc = upload_samples(c, pos, nrm, wo, dirs, w, n, T, src);
     if (rc) return rc;
     if ((rc = mlp_grad(c, m, src, n))) return rc;
     RRH_HIP(hipMemcpyAsync(grad, m->grad.p, (size_t)NPARAM * 4, hipMemcpyDeviceToHost, c->stream));
@@ -997,12 +962,12 @@ int rrh_mlp_loss_grad(rrh_ctx* c, rrh_mlp* m, const float* pos, const float* nrm
 }
 
 int rrh_mlp_train_step(rrh_ctx* c, rrh_mlp* m, const float* pos, const float* nrm, const float* wo, const float* dirs,
-                       const float* w, const float* sel_pdfs, uint32_t n, float* loss) {
+                       const float* w, uint32_t n, float* loss) {
     RRH_ARG(c && m && pos && nrm && wo && dirs && w, "rrh_mlp_train_step: null argument");
     RRH_ARG(n > 0, "rrh_mlp_train_step: no samples");
     HostTmp T;
     HostSrc src;
-    int rc = upload_samples(c, pos, nrm, wo, dirs, w, sel_pdfs, n, T, src);
+    int rc = upload_samples(c, pos, nrm, wo, dirs, w, n, T, src);
     if (rc) return rc;
     if ((rc = mlp_grad(c, m, src, n))) return rc;
     if ((rc = mlp_update(c, m))) return rc;
@@ -1015,8 +980,7 @@ int rrh_mlp_train_from_vertices(rrh_ctx* c, rrh_mlp* m, uint32_t batch, float* l
     RRH_ARG(c && m, "rrh_mlp_train_from_vertices: null argument");
     RRH_ARG(c->train.p && c->train_cnt.p, "rrh_mlp_train_from_vertices: no recorded vertices (render with record_vertices)");
     RRH_ARG(batch > 0, "rrh_mlp_train_from_vertices: batch must be > 0");
-    TrainSrc src{(const float4*)c->train.p, (const uint32_t*)c->train_cnt.p, c->train_capacity, batch,
-                 (m->p.flags & RRH_MLP_TRAIN_SELECT) != 0u};
+    TrainSrc src{(const float4*)c->train.p, (const uint32_t*)c->train_cnt.p, c->train_capacity, batch};
     int rc = mlp_grad(c, m, src, batch);
     if (rc) return rc;
     if ((rc = mlp_update(c, m))) return rc;
@@ -1091,7 +1055,7 @@ struct QueueSrc {  // the prepared inputs of the live ray queue (miss -> invalid
 };
 
 int mlp_infer_queue(rrh_ctx* c, rrh_mlp* m, const DevScene& S, const float4* qo, const float4* qd, const float4* hits,
-                    const uint32_t* cnt, uint32_t b, uint32_t cap, uint16_t* lobes, bool select) {
+                    const uint32_t* cnt, uint32_t b, uint32_t cap, uint16_t* lobes) {
     (void)qo;
     int rc = c->mlp_prep.ensure((size_t)cap * 48);
     if (rc) return rc;
@@ -1102,7 +1066,7 @@ int mlp_infer_queue(rrh_ctx* c, rrh_mlp* m, const DevScene& S, const float4* qo,
     QueueSrc src{(const float4*)c->mlp_prep.p, cnt, b};
     const uint32_t g = infer_grid(c, m, k_mlp_infer<QueueSrc, true>, cap);
     hipLaunchKernelGGL((k_mlp_infer<QueueSrc, true>), dim3(g), dim3(64 * kInferWaves), 0, c->stream, m->dev(), src, cap,
-                       (void*)lobes, (float*)nullptr, select);
+                       (void*)lobes);
     RRH_LAUNCH_CHECK();
     return RRH_OK;
 }

@@ -432,8 +432,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, Q in, Q out, const
                         }
                     }
                     if (A.rec.v) guide_record_vertex(A.rec, pixel, b, hi.pos, nn, wo, wi, pdf, bw,
-                                                     alive ? thp : mk(0, 0, 0), L, ne.shadow ? mul(thp_before, ne.c) : mk(0, 0, 0), hi.mat, alive,
-                                                     alive ? mat_sample_pdf(m, wo, wi, nn) : 0.0f);
+                                                     alive ? thp : mk(0, 0, 0), L, ne.shadow ? mul(thp_before, ne.c) : mk(0, 0, 0), hi.mat, alive);
                     if (alive && !last) {
                         push_ext = true;
                         no = make_float4(sorg.x, sorg.y, sorg.z, __uint_as_float(pixel));
@@ -1098,7 +1097,7 @@ static int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, 
         if (b + 1 >= p->max_bounces || (p->guide_bounces != 0 && b >= p->guide_bounces)) Ab.guide_mode = RRH_GUIDE_NONE;
         if (Ab.guide_mode == RRH_GUIDE_MLP) {
             rc = mlp_infer_queue(c, mlp, S, qi.o, qi.d, (const float4*)Bf.hits, cnt, b, npix,
-                                 const_cast<uint16_t*>(A.guide.mlp_lobes), A.guide.mlp_select);
+                                 const_cast<uint16_t*>(A.guide.mlp_lobes));
             if (rc) return rc;
             Ab.hit_prep = (const float4*)c->mlp_prep.p;  // the hit lookup the network's inputs took
         }

@@ -1,9 +1,8 @@
 """Equal-time relative MSE of guided path tracing on the indoor scene (a closed
 10 x 10 x 4 m room lit through one 0.5 x 0.5 m window): the unguided tracer, the
-8-lobe vMF grid fitted by online EM, and the learned MLP guide (method "mlp": blended
-with the BSDF at the fixed 0.5; "mlps": at the network's learned selection probability),
-each rendering and training for the same wall-clock budget, scored against an
-FP64-accumulated high-spp reference of the same view.
+8-lobe vMF grid fitted by online EM, and the learned MLP guide, each rendering and
+training for the same wall-clock budget, scored against an FP64-accumulated
+high-spp reference of the same view.
 
 relMSE = mean over pixels of mean_c (I_c - R_c)^2 / (R_c^2 + 0.01)   (eps as in
 Rousselle et al. 2011 and Mueller et al. 2017).  The reference's own noise adds Var(R) to
@@ -60,8 +59,8 @@ class Setup:
         self.acc = core.Image(ctx, width, height, L.FMT_RGBA64F)
 
     def render(self, seed: int, mode: int = L.GUIDE_NONE, alpha: float = 0.0, record: int = 0, grid=None,
-               mlp=None, guide_bounces: int = 0, flags: int = 0) -> None:
-        p = L.PTParams(self.bounces, seed, 3, mode, alpha, record, flags, guide_bounces)
+               mlp=None, guide_bounces: int = 0) -> None:
+        p = L.PTParams(self.bounces, seed, 3, mode, alpha, record, 0, guide_bounces)
         L.call("rrh_pt_render_guided", self.ctx.h, self.scene.h, C.byref(p), grid.h if grid else None,
                mlp.h if mlp else None, self.rad.h)
 
@@ -105,17 +104,16 @@ def run_method(S: Setup, method: str, seconds: float, ref: np.ndarray, ref_var: 
     ctx = S.ctx
     grid = mlp = None
     if record is None:  # vertices kept per frame: ~2M for the EM fit, one batch for the MLP
-        record = batch if method in ("mlp", "mlps") else 2_000_000
+        record = batch if method == "mlp" else 2_000_000
     if method == "vmm":
         prm = L.GridParams(L.GRID_VMM, 0, 0, 0, 0, 512.0, res, 8, L.f3(S.bmin), L.f3(S.bmax), decay, min_weight)
         grid = core.Grid(ctx, prm)
         mode = L.GUIDE_VMM
-    elif method in ("mlp", "mlps"):
-        mlp = core.Mlp(ctx, S.bmin, S.bmax, lr=lr, seed=1, train_select=method == "mlps")
+    elif method == "mlp":
+        mlp = core.Mlp(ctx, S.bmin, S.bmax, lr=lr, seed=1)
         mode = L.GUIDE_MLP
     else:
         mode = L.GUIDE_NONE
-    flags = L.PT_MLP_SELECT if method == "mlps" else 0
     ctx.synchronize()
     curve, spp, frame, elapsed = [], 0, 0, 0.0
     next_cp = seconds / checkpoints
@@ -125,12 +123,12 @@ def run_method(S: Setup, method: str, seconds: float, ref: np.ndarray, ref_var: 
         a = alpha if (mode != L.GUIDE_NONE and frame > 0) else 0.0
         for s in range(spp_per_frame):
             rec = record if (mode != L.GUIDE_NONE and s == 0) else 0
-            S.render(seed0 + spp, mode, a, rec, grid, mlp, guide_bounces, flags)
+            S.render(seed0 + spp, mode, a, rec, grid, mlp, guide_bounces)
             S.accumulate(spp)
             spp += 1
         if method == "vmm":
             L.call("rrh_grid_em_from_vertices", ctx.h, grid.h)
-        elif mlp is not None:
+        elif method == "mlp":
             for _ in range(train_steps):
                 losses.append(mlp.train_from_vertices(batch))
         frame += 1
@@ -218,7 +216,7 @@ def evaluate(ctx: core.Context, width: int = 320, height: int = 180, seconds: fl
            "ref_rel_var": None if ref_var is None else float(np.mean(ref_var / (ref ** 2 + EPS))),
            "methods": {}}
     for m in methods:
-        f = mlp_factor if m in ("mlp", "mlps") else 1.0
+        f = mlp_factor if m == "mlp" else 1.0
         res["methods"][m] = run_method(S, m, seconds * f, ref, ref_var, progress=progress,
                                        checkpoints=int(round(12 * f)), **kw)
         if progress:
@@ -226,20 +224,18 @@ def evaluate(ctx: core.Context, width: int = 320, height: int = 180, seconds: fl
             print(f"{m}: relMSE {r['rel_mse']:.5g} (unbiased {r['rel_mse_unbiased']}) at {r['spp']} spp in "
                   f"{r['seconds']} s", file=sys.stderr, flush=True)
     ms = res["methods"]
-    for name in ("mlp", "mlps"):
-        if "vmm" not in ms or name not in ms:
-            continue
+    if "vmm" in ms and "mlp" in ms:
         tv = ms["vmm"]["seconds"]
-        mc = ms[name]["curve"]
+        mc = ms["mlp"]["curve"]
         res["mlp_factor"] = mlp_factor
-        res[f"{name}_over_vmm_rel_mse"] = at_time(mc, tv, 2) / ms["vmm"]["rel_mse"]
+        res["mlp_over_vmm_rel_mse"] = at_time(mc, tv, 2) / ms["vmm"]["rel_mse"]
         tm = time_to_reach(mc, ms["vmm"]["rel_mse"])
-        res[f"{name}_time_to_vmm_rel_mse_ratio"] = None if tm is None else tm / tv
+        res["mlp_time_to_vmm_rel_mse_ratio"] = None if tm is None else tm / tv
         if ms["vmm"].get("rel_mse_unbiased"):
             mu = at_time(mc, tv, 3)
-            res[f"{name}_over_vmm_rel_mse_unbiased"] = None if mu is None else mu / ms["vmm"]["rel_mse_unbiased"]
+            res["mlp_over_vmm_rel_mse_unbiased"] = None if mu is None else mu / ms["vmm"]["rel_mse_unbiased"]
             tm = time_to_reach(mc, ms["vmm"]["rel_mse_unbiased"], col=3)
-            res[f"{name}_time_to_vmm_rel_mse_unbiased_ratio"] = None if tm is None else tm / tv
+            res["mlp_time_to_vmm_rel_mse_unbiased_ratio"] = None if tm is None else tm / tv
     if "none" in ms and "vmm" in ms:
         res["vmm_over_unguided_rel_mse"] = ms["vmm"]["rel_mse"] / ms["none"]["rel_mse"]
     return res

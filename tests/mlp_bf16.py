@@ -7,7 +7,7 @@ from __future__ import annotations
 
 import numpy as np
 
-LAYERS = [(78, 64), (64, 64), (64, 64), (64, 64), (64, 41)]
+LAYERS = [(78, 64), (64, 64), (64, 64), (64, 64), (64, 40)]
 
 
 def bf16(a) -> np.ndarray:
@@ -26,9 +26,8 @@ def unpack(params: np.ndarray):
     return out
 
 
-def loss_grad(params: np.ndarray, x: np.ndarray, dirs: np.ndarray, w: np.ndarray, pdfs: np.ndarray | None = None):
-    """x: (n, 78) encoded inputs; returns (loss, gradient) of the weighted mean NLL, plus the
-    selection probability's blend term when the densities pdfs (n x (pb, ps)) are given."""
+def loss_grad(params: np.ndarray, x: np.ndarray, dirs: np.ndarray, w: np.ndarray):
+    """x: (n, 78) encoded inputs; returns (loss, gradient) of the weighted mean NLL."""
     layers = unpack(params)
     n = len(x)
     a = bf16(x)
@@ -41,7 +40,7 @@ def loss_grad(params: np.ndarray, x: np.ndarray, dirs: np.ndarray, w: np.ndarray
             acts.append(a)
         else:
             out = z
-    o = out[:, :40].reshape(n, 8, 5)
+    o = out.reshape(n, 8, 5)
     d = dirs.astype(np.float64)
     v = o[..., :3]
     ln = np.sqrt((v ** 2).sum(-1) + 1e-12)
@@ -64,24 +63,12 @@ def loss_grad(params: np.ndarray, x: np.ndarray, dirs: np.ndarray, w: np.ndarray
     g[..., 3] = np.where(0.1 + sp > 1000, 0.0, dq[:, None] * pi * f * dlogf / (1 + np.exp(-o[..., 3])))
     s = (dq[:, None] * pi * f * kap)[..., None] * d[:, None, :]
     g[..., :3] = (s - mu * (mu * s).sum(-1, keepdims=True)) / ln[..., None]
-    gs = np.zeros(n)
-    if pdfs is not None:
-        pb, ps = pdfs[:, 0].astype(np.float64), pdfs[:, 1].astype(np.float64)
-        sg = 1 / (1 + np.exp(-out[:, 40]))
-        al = 0.05 + 0.9 * sg
-        qb = al * q + (1 - al) * pb + 1e-20
-        r = n * wn * ps / qb
-        loss += float((wn * r).sum())
-        gs = -wn * r * (q - pb) / qb * 0.9 * sg * (1 - sg)
-    D = bf16(np.concatenate([g.reshape(n, 40), gs[:, None]], 1))
+    D = bf16(g.reshape(n, 40))
     grads = [None] * 5
     for l in range(4, -1, -1):
         W, _ = layers[l]
         grads[l] = (D.T @ acts[l], D.sum(0))
         if l == 0:
             break
-        Wb = bf16(W)
-        if l == 4:
-            Wb[40] = 0.0  # the selection output reads the last hidden layer held constant
-        D = bf16((D @ Wb) * (acts[l] > 0))
+        D = bf16((D @ bf16(W)) * (acts[l] > 0))
     return loss, np.concatenate([np.concatenate([gw.ravel(), gb]) for gw, gb in grads])

@@ -53,7 +53,7 @@ def test_vmm_em_step_at_32cubed_by_8_with_2m_samples(ctx):
     assert np.allclose(lobes_gpu, lob, rtol=2e-4, atol=2e-4)
 
 
-def _oracle_loss_grad_chunked(wts, x, dirs, w, pb, chunks=16):
+def _oracle_loss_grad_chunked(wts, x, dirs, w, chunks=16):
     """orc_mlp_loss_grad over chunks in host threads; each chunk is normalised by its own
     weight sum, so the whole batch is the chunks' weighted mean"""
     lib = O.lib()
@@ -63,8 +63,8 @@ def _oracle_loss_grad_chunked(wts, x, dirs, w, pb, chunks=16):
     def one(k):
         a, b = bounds[k], bounds[k + 1]
         g = np.zeros(core.Mlp.NPARAM)
-        xs, ds, ws, pbs = (np.ascontiguousarray(v[a:b]) for v in (x, dirs, w, pb))
-        loss = lib.orc_mlp_loss_grad(O.p(P), O.p(xs), O.p(ds), O.p(ws), O.p(pbs), int(b - a), O.p(g))
+        xs, ds, ws = (np.ascontiguousarray(v[a:b]) for v in (x, dirs, w))
+        loss = lib.orc_mlp_loss_grad(O.p(P), O.p(xs), O.p(ds), O.p(ws), int(b - a), O.p(g))
         return float(ws.astype(np.float64).sum()), loss, g
 
     with cf.ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
@@ -78,25 +78,21 @@ def test_mlp_loss_grad_at_the_configured_batch(ctx):
     rng = np.random.default_rng(7)
     m = core.Mlp(ctx, BMIN, BMAX, seed=3)
     wts = m.get_weights()
-    wts[-41:-1] = np.tile(np.float32([0.3, -0.2, 0.5, 2.0, 0.0]), 8) + rng.normal(0, 0.3, 40).astype(np.float32)
-    wts[-1] = 0.4
+    wts[-40:] = np.tile(np.float32([0.3, -0.2, 0.5, 2.0, 0.0]), 8) + rng.normal(0, 0.3, 40).astype(np.float32)
     wts = bf16(wts).astype(np.float32)   # BF16-exact, so the oracle and the kernels share the network
     m.set_weights(wts)
     pos, nrm, wo, dirs, w = _samples(n, rng)
-    # densities as recorded vertices carry them (the cosine density about the normal, and a
-    # drawing density), so the selection output's blend term is on, as in training from vertices
-    pb = np.stack([np.maximum((nrm * dirs).sum(1), 0.0) / np.pi, rng.uniform(0.05, 1.5, n)], 1).astype(np.float32)
-    loss, g = m.loss_grad(pos, nrm, wo, dirs, w, pb)
+    loss, g = m.loss_grad(pos, nrm, wo, dirs, w)
     # halves: the full batch is their weight-normalised combination
     h = n // 2
-    l1, g1 = m.loss_grad(pos[:h], nrm[:h], wo[:h], dirs[:h], w[:h], pb[:h])
-    l2, g2 = m.loss_grad(pos[h:], nrm[h:], wo[h:], dirs[h:], w[h:], pb[h:])
+    l1, g1 = m.loss_grad(pos[:h], nrm[:h], wo[:h], dirs[:h], w[:h])
+    l2, g2 = m.loss_grad(pos[h:], nrm[h:], wo[h:], dirs[h:], w[h:])
     W1, W2 = float(w[:h].astype(np.float64).sum()), float(w[h:].astype(np.float64).sum())
     assert abs(loss - (W1 * l1 + W2 * l2) / (W1 + W2)) <= 1e-4 * abs(loss)
     assert _rel(g, (W1 * g1 + W2 * g2) / (W1 + W2)) <= 1e-4
     # against the FP64 oracle on the whole batch
     x = _encode(pos, nrm, wo)
-    lref, gref = _oracle_loss_grad_chunked(wts, x, dirs, w, pb)
+    lref, gref = _oracle_loss_grad_chunked(wts, x, dirs, w)
     cos = float(g @ gref / (np.linalg.norm(g) * np.linalg.norm(gref)))
     print(f"n={n}: loss {loss} oracle {lref}; gradient cosine {cos:.6f}, relative L2 {_rel(g, gref):.4f}")
     assert abs(loss - lref) <= 1e-2 * abs(lref), (loss, lref)

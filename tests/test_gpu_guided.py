@@ -23,8 +23,8 @@ def _scene(ctx):
     return gs
 
 
-def _render(ctx, gs, img, seed, mode=L.GUIDE_NONE, alpha=0.0, record=0, grid=None, mlp=None, flags=0):
-    p = L.PTParams(BOUNCES, seed, 3, mode, alpha, record, flags)
+def _render(ctx, gs, img, seed, mode=L.GUIDE_NONE, alpha=0.0, record=0, grid=None, mlp=None):
+    p = L.PTParams(BOUNCES, seed, 3, mode, alpha, record)
     L.call("rrh_pt_render_guided", ctx.h, gs.h, C.byref(p), grid.h if grid else None, mlp.h if mlp else None, img.h)
     return img.download().reshape(H, W, 4)[..., :3].astype(np.float64)
 
@@ -71,31 +71,20 @@ def test_vmm_guided_render_is_unbiased(ctx, unguided):
     _assert_same_mean(got, ref, "vmm")
 
 
-@pytest.mark.parametrize("flags", [0, L.PT_MLP_SELECT])
-def test_mlp_guided_render_is_unbiased_and_training_lowers_the_loss(ctx, unguided, flags):
-    """flags PT_MLP_SELECT: the network's learned selection probability replaces the fixed 0.5
-    (trained and rendered with it); it must have moved away from its initial 0.5."""
+def test_mlp_guided_render_is_unbiased_and_training_lowers_the_loss(ctx, unguided):
     gs, ref = unguided
     info = gs.info()
-    m = core.Mlp(ctx, info["bounds_min"], info["bounds_max"], lr=2e-3, seed=1, train_select=flags != 0)
+    m = core.Mlp(ctx, info["bounds_min"], info["bounds_max"], lr=2e-3, seed=1)
     img = core.Image(ctx, W, H)
     losses = []
     for k in range(60):
-        _render(ctx, gs, img, 70 + k, mode=L.GUIDE_MLP, alpha=0.5 if k >= 20 else 0.0, record=4096, mlp=m,
-                flags=flags)
+        _render(ctx, gs, img, 70 + k, mode=L.GUIDE_MLP, alpha=0.5 if k >= 20 else 0.0, record=4096, mlp=m)
         losses.append(m.train_from_vertices(4096))
     assert np.isfinite(losses).all()
     assert np.mean(losses[-10:]) < np.mean(losses[:5]), (losses[:5], losses[-10:])
     assert m.state()[0] >= 55
-    got = _frames(ctx, gs, 256, 9000, mode=L.GUIDE_MLP, alpha=0.5, mlp=m, flags=flags)
+    got = _frames(ctx, gs, 256, 9000, mode=L.GUIDE_MLP, alpha=0.5, mlp=m)
     _assert_same_mean(got, ref, "mlp")
-    if flags:
-        rng = np.random.default_rng(3)
-        pos = rng.uniform(info["bounds_min"], info["bounds_max"], (4096, 3)).astype(np.float32)
-        n = rng.normal(size=(4096, 3)).astype(np.float32)
-        n /= np.linalg.norm(n, axis=1, keepdims=True)
-        _, sel = m.infer(pos, n, n, select=True)
-        assert ((sel >= 0.05) & (sel <= 0.95)).all() and np.abs(sel - 0.5).mean() > 1e-3, sel.mean()
 
 
 def test_guide_training_is_deterministic(ctx):
@@ -134,8 +123,7 @@ def _noise(ctx, gs, w, h, n, seed0, bounces, **kw):
     img = core.Image(ctx, w, h)
     acc, acc2 = np.zeros((h, w, 3)), np.zeros((h, w, 3))
     for k in range(n):
-        p = L.PTParams(bounces, seed0 + k, 3, kw.get("mode", L.GUIDE_NONE), kw.get("alpha", 0.0), 0,
-                       kw.get("flags", 0))
+        p = L.PTParams(bounces, seed0 + k, 3, kw.get("mode", L.GUIDE_NONE), kw.get("alpha", 0.0), 0)
         L.call("rrh_pt_render_guided", ctx.h, gs.h, C.byref(p), kw["grid"].h if kw.get("grid") else None,
                kw["mlp"].h if kw.get("mlp") else None, img.h)
         r = img.download()[..., :3].astype(np.float64)
