@@ -207,13 +207,36 @@ RRH_D void guide_record_nee_visible(const VertexRec& R, uint32_t pixel, uint32_t
     r[3].w = 1.0f;
 }
 
+// The learned guide's input record of queue slot i: the hit point (w = 1), the normal facing
+// the ray origin with the material index, and the outgoing direction; zeros for a miss.  d is
+// the slot's ray direction, h its hit (t, prim bits, b1, b2).  The input pass (k_mlp_prep) and
+// the traversal kernels' epilogue both write it through this function, so their records agree
+// bit for bit.
+RRH_D void write_hit_prep(const DevScene& S, float4 d, float4 h, float4* __restrict__ prep, size_t i) {
+    const uint32_t prim = __float_as_uint(h.y);
+    float4 p0 = make_float4(0, 0, 0, 0), p1 = p0, p2 = p0;
+    if (prim != ~0u) {
+        HitInfo hi = fetch_hit(S, prim, h.z, h.w);
+        f3 wo = neg(xyz(d));
+        f3 nn = hi.normal;
+        if (dot(nn, wo) < 0.0f) nn = neg(nn);
+        p0 = make_float4(hi.pos.x, hi.pos.y, hi.pos.z, 1.0f);
+        p1 = make_float4(nn.x, nn.y, nn.z, __uint_as_float(hi.mat));
+        p2 = make_float4(wo.x, wo.y, wo.z, 0.0f);
+    }
+    prep[i * 3] = p0;
+    prep[i * 3 + 1] = p1;
+    prep[i * 3 + 2] = p2;
+}
+
 // host side (rrh_guide.hip)
 int guide_prepare(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* grid, rrh_mlp* mlp, GuideDev& G,
                   VertexRec& R);
 int guide_finish(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, rrh_grid* grid, rrh_mlp* mlp, const VertexRec& R,
                  const float4* rad);
 // learned guide: the fp16 lobe record (kMlpLobeHalves) of every hit of queue b (rrh_mlp.hip)
+// (prep_done: the traversal launch already wrote the input records into c->mlp_prep)
 int mlp_infer_queue(rrh_ctx* c, rrh_mlp* m, const DevScene& S, const float4* qo, const float4* qd, const float4* hits,
-                    const uint32_t* cnt, uint32_t b, uint32_t cap, uint16_t* lobes);
+                    const uint32_t* cnt, uint32_t b, uint32_t cap, uint16_t* lobes, bool prep_done = false);
 
 }  // namespace rrh

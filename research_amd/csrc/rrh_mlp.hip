@@ -1023,23 +1023,8 @@ __global__ void __launch_bounds__(256) k_mlp_prep(DevScene S, const float4* __re
                                                   const float4* __restrict__ hits, const uint32_t* __restrict__ cnt,
                                                   uint32_t b, float4* __restrict__ prep) {
     const uint32_t n = cnt[b];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const float4 h = hits[i];
-        const uint32_t prim = __float_as_uint(h.y);
-        float4 p0 = make_float4(0, 0, 0, 0), p1 = p0, p2 = p0;
-        if (prim != ~0u) {
-            HitInfo hi = fetch_hit(S, prim, h.z, h.w);
-            f3 wo = neg(xyz(d[i]));
-            f3 nn = hi.normal;
-            if (dot(nn, wo) < 0.0f) nn = neg(nn);
-            p0 = make_float4(hi.pos.x, hi.pos.y, hi.pos.z, 1.0f);
-            p1 = make_float4(nn.x, nn.y, nn.z, __uint_as_float(hi.mat));
-            p2 = make_float4(wo.x, wo.y, wo.z, 0.0f);
-        }
-        prep[(size_t)i * 3] = p0;
-        prep[(size_t)i * 3 + 1] = p1;
-        prep[(size_t)i * 3 + 2] = p2;
-    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        write_hit_prep(S, d[i], hits[i], prep, i);
 }
 
 struct QueueSrc {  // the prepared inputs of the live ray queue (miss -> invalid)
@@ -1055,14 +1040,16 @@ struct QueueSrc {  // the prepared inputs of the live ray queue (miss -> invalid
 };
 
 int mlp_infer_queue(rrh_ctx* c, rrh_mlp* m, const DevScene& S, const float4* qo, const float4* qd, const float4* hits,
-                    const uint32_t* cnt, uint32_t b, uint32_t cap, uint16_t* lobes) {
+                    const uint32_t* cnt, uint32_t b, uint32_t cap, uint16_t* lobes, bool prep_done) {
     (void)qo;
     int rc = c->mlp_prep.ensure((size_t)cap * 48);
     if (rc) return rc;
-    uint32_t gp = (cap + 255) / 256;
-    gp = gp < 1u ? 1u : (gp > 4096u ? 4096u : gp);
-    hipLaunchKernelGGL(k_mlp_prep, dim3(gp), dim3(256), 0, c->stream, S, qd, hits, cnt, b, (float4*)c->mlp_prep.p);
-    RRH_LAUNCH_CHECK();
+    if (!prep_done) {
+        uint32_t gp = (cap + 255) / 256;
+        gp = gp < 1u ? 1u : (gp > 4096u ? 4096u : gp);
+        hipLaunchKernelGGL(k_mlp_prep, dim3(gp), dim3(256), 0, c->stream, S, qd, hits, cnt, b, (float4*)c->mlp_prep.p);
+        RRH_LAUNCH_CHECK();
+    }
     QueueSrc src{(const float4*)c->mlp_prep.p, cnt, b};
     const uint32_t g = infer_grid(c, m, k_mlp_infer<QueueSrc, true>, cap);
     hipLaunchKernelGGL((k_mlp_infer<QueueSrc, true>), dim3(g), dim3(64 * kInferWaves), 0, c->stream, m->dev(), src, cap,

@@ -109,16 +109,22 @@ extern __shared__ uint4 s_top[];
 
 // closest hits of the extension queue's first n slots, workgroup blk of nblk (one wave each)
 // perm (optional): the queue slots in tracing order (the ray sort); results stay in slot order
-template <bool COUNT, class Stk, class Top>
+// PREP: the learned guide's input record of each slot as well (write_hit_prep into prep), in
+// place of a separate input pass over the queue; a variant of its own, so the kernels without it
+// keep their register count
+template <bool COUNT, bool PREP, class Stk, class Top>
 __device__ __forceinline__ void trace_ext_part(const DevScene& S, const Q& in, uint32_t n, uint32_t blk, uint32_t nblk,
-                                               float4* __restrict__ hits, const uint32_t* __restrict__ perm, Stk& st,
-                                               const Top& top, uint32_t& nn, uint32_t& nt, uint32_t& nr) {
+                                               float4* __restrict__ hits, const uint32_t* __restrict__ perm,
+                                               float4* __restrict__ prep, Stk& st, const Top& top, uint32_t& nn,
+                                               uint32_t& nt, uint32_t& nr) {
     for (uint32_t j = blk * kTraceBlock + threadIdx.x; j < n; j += nblk * kTraceBlock) {
         const uint32_t i = perm ? perm[j] : j;
         if (i >= n) continue;  // a sorted order holds only live slots; never trust it past the queue
         float4 o = in.o[i], d = in.d[i];
         Hit h = trace_closest_s<COUNT>(S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.001f, 1e9f, st, &nn, &nt, top);
-        hits[i] = make_float4(h.t, __uint_as_float(h.prim), h.b1, h.b2);
+        const float4 h4 = make_float4(h.t, __uint_as_float(h.prim), h.b1, h.b2);
+        hits[i] = h4;
+        if constexpr (PREP) write_hit_prep(S, d, h4, prep, i);
         nr++;
     }
 }
@@ -157,16 +163,18 @@ __device__ __forceinline__ void wave_count_max(uint32_t nn, uint32_t nt, unsigne
     }
 }
 
-// TOP: the first top_n quantized nodes are read from LDS
-template <bool COUNT, int STK, bool TOP>
+// TOP: the first top_n quantized nodes are read from LDS; PREP: see trace_ext_part
+template <bool COUNT, int STK, bool TOP, bool PREP = false>
 __global__ void __launch_bounds__(kTraceBlock) k_intersect(DevScene S, Q in, const uint32_t* __restrict__ cnt, uint32_t b,
                                                            float4* __restrict__ hits, unsigned long long* __restrict__ fetch,
-                                                           bool remap, const uint32_t* __restrict__ perm, uint32_t top_n) {
+                                                           bool remap, const uint32_t* __restrict__ perm, uint32_t top_n,
+                                                           float4* __restrict__ prep) {
     __shared__ int s_stack[(kTraceBlock / 64) * StackSel<STK>::kLds * 64];
     const auto top = stage_top<TOP>(S, s_top, top_n);
     uint32_t nn = 0, nt = 0, nr = 0;
     auto st = make_stack<STK>(s_stack);
-    trace_ext_part<COUNT>(S, in, cnt[b], xcd_block(blockIdx.x, gridDim.x, remap), gridDim.x, hits, perm, st, top, nn, nt, nr);
+    trace_ext_part<COUNT, PREP>(S, in, cnt[b], xcd_block(blockIdx.x, gridDim.x, remap), gridDim.x, hits, perm, prep, st, top,
+                                nn, nt, nr);
     if constexpr (COUNT) {
         atomicAdd(fetch + 0, (unsigned long long)nn);
         atomicAdd(fetch + 1, (unsigned long long)nt);
@@ -474,12 +482,13 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow(DevScene S, const uint32
 // queues depend only on the shading pass before them, and neither kernel's tail (its
 // slowest waves) then runs alone on the chip.  Workgroups [0, g_sh) trace the shadow
 // queue, the rest the extension queue.
-template <int STK>
+template <int STK, bool PREP = false>
 __global__ void __launch_bounds__(kTraceBlock) k_trace_pair(DevScene S, Q in, const uint32_t* __restrict__ cnt, uint32_t b,
                                                             float4* __restrict__ hits, const uint32_t* __restrict__ perm,
                                                             uint32_t g_sh, const float4* __restrict__ sh_o,
                                                             const float4* __restrict__ sh_d, const float4* __restrict__ sh_c,
-                                                            float4* __restrict__ rad, VertexRec rec, bool remap) {
+                                                            float4* __restrict__ rad, VertexRec rec, bool remap,
+                                                            float4* __restrict__ prep) {
     __shared__ int s_stack[(kTraceBlock / 64) * StackSel<STK>::kLds * 64];
     auto st = make_stack<STK>(s_stack);
     const TopNone top;
@@ -489,16 +498,22 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_pair(DevScene S, Q in, co
                                  rad, rec, st, top, nn, nt, nr);
     } else {
         const uint32_t ng = gridDim.x - g_sh;
-        trace_ext_part<false>(S, in, cnt[b], xcd_block(blockIdx.x - g_sh, ng, remap), ng, hits, perm, st, top, nn, nt, nr);
+        trace_ext_part<false, PREP>(S, in, cnt[b], xcd_block(blockIdx.x - g_sh, ng, remap), ng, hits, perm, prep, st, top, nn,
+                                    nt, nr);
     }
 }
 
 template <int STK>
 int launch_pair(rrh_ctx* c, const DevScene& S, Q& qi, uint32_t* cnt, uint32_t b, uint32_t gt, const uint32_t* perm,
-                float4* rad, const VertexRec& rec, const Bufs& B) {
-    hipLaunchKernelGGL(k_trace_pair<STK>, dim3(2 * gt), dim3(kTraceBlock), 0, c->stream, S, qi, (const uint32_t*)cnt, b,
-                       B.hits, perm, gt, (const float4*)B.sh_o, (const float4*)B.sh_d, (const float4*)B.sh_c, rad, rec,
-                       c->xcd_remap);
+                float4* rad, const VertexRec& rec, const Bufs& B, float4* prep = nullptr) {
+    if (prep)
+        hipLaunchKernelGGL((k_trace_pair<STK, true>), dim3(2 * gt), dim3(kTraceBlock), 0, c->stream, S, qi,
+                           (const uint32_t*)cnt, b, B.hits, perm, gt, (const float4*)B.sh_o, (const float4*)B.sh_d,
+                           (const float4*)B.sh_c, rad, rec, c->xcd_remap, prep);
+    else
+        hipLaunchKernelGGL((k_trace_pair<STK, false>), dim3(2 * gt), dim3(kTraceBlock), 0, c->stream, S, qi,
+                           (const uint32_t*)cnt, b, B.hits, perm, gt, (const float4*)B.sh_o, (const float4*)B.sh_d,
+                           (const float4*)B.sh_c, rad, rec, c->xcd_remap, prep);
     RRH_LAUNCH_CHECK();
     return RRH_OK;
 }
@@ -829,10 +844,13 @@ uint32_t resident_blocks(rrh_ctx* c, K kernel) {
     return per_cu * (uint32_t)c->cu_count;
 }
 
+// prep: also write the learned guide's input records (*prep_used says whether the launched
+// kernel did: the plain launch does, the counting, LDS-top and persistent binary-node ones do not)
 template <int STK>
 int launch_trace(rrh_ctx* c, const DevScene& S, Q& qi, uint32_t* cnt, uint32_t b, uint32_t g, uint32_t gt, bool count,
-                 const uint32_t* perm, float4* hp, uint32_t top_n) {
+                 const uint32_t* perm, float4* hp, uint32_t top_n, float4* prep = nullptr, bool* prep_used = nullptr) {
     unsigned long long* fc = (unsigned long long*)c->fetch.p;
+    if (prep_used) *prep_used = false;
     if constexpr (STK != RRH_STACK_NONE) {  // the stackless traversal is 4-wide only
     if (c->persistent && S.node_fmt == RRH_NODES_BINARY) {
         uint32_t* next = cnt + 128 + b;
@@ -848,18 +866,25 @@ int launch_trace(rrh_ctx* c, const DevScene& S, Q& qi, uint32_t* cnt, uint32_t b
     }
     }
     const size_t lds = (size_t)top_n * 4 * sizeof(uint4);
+    if (prep && !count && !top_n) {
+        hipLaunchKernelGGL((k_intersect<false, STK, false, true>), dim3(gt), dim3(kTraceBlock), 0, c->stream, S, qi, cnt, b,
+                           hp, fc, c->xcd_remap, perm, 0u, prep);
+        RRH_LAUNCH_CHECK();
+        if (prep_used) *prep_used = true;
+        return RRH_OK;
+    }
     if (count && top_n)
         hipLaunchKernelGGL((k_intersect<true, STK, true>), dim3(gt), dim3(kTraceBlock), lds, c->stream, S, qi, cnt, b, hp, fc,
-                           c->xcd_remap, perm, top_n);
+                           c->xcd_remap, perm, top_n, prep);
     else if (count)
         hipLaunchKernelGGL((k_intersect<true, STK, false>), dim3(gt), dim3(kTraceBlock), 0, c->stream, S, qi, cnt, b, hp, fc,
-                           c->xcd_remap, perm, 0u);
+                           c->xcd_remap, perm, 0u, prep);
     else if (top_n)
         hipLaunchKernelGGL((k_intersect<false, STK, true>), dim3(gt), dim3(kTraceBlock), lds, c->stream, S, qi, cnt, b, hp, fc,
-                           c->xcd_remap, perm, top_n);
+                           c->xcd_remap, perm, top_n, prep);
     else
         hipLaunchKernelGGL((k_intersect<false, STK, false>), dim3(gt), dim3(kTraceBlock), 0, c->stream, S, qi, cnt, b, hp, fc,
-                           c->xcd_remap, perm, 0u);
+                           c->xcd_remap, perm, 0u, prep);
     RRH_LAUNCH_CHECK();
     return RRH_OK;
 }
@@ -993,6 +1018,8 @@ static int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, 
     A.guide_alpha = p->guide_alpha;
     A.remap = c->xcd_remap;
     if ((rc = guide_prepare(c, s, p, grid, mlp, A.guide, A.rec))) return rc;
+    // the learned guide's per-slot input records (written by the traversal epilogue or the input pass)
+    if (p->guide_mode == RRH_GUIDE_MLP && (rc = c->mlp_prep.ensure((size_t)npix * 48))) return rc;
     DevScene S = s->dev();
     uint32_t* cnt = Bf.cnt;
     if (fixed) {
@@ -1068,6 +1095,13 @@ static int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, 
             if (e != hipSuccess) { set_error(std::string("ray sort: ") + hipGetErrorString(e)); return RRH_E_HIP; }
             perm = slots + npix;
         }
+        // the direction sampled at the last bounce is never traced (and its training record
+        // carries no radiance): the guide is skipped there
+        const bool guided_b = !(b + 1 >= p->max_bounces || (p->guide_bounces != 0 && b >= p->guide_bounces));
+        // an MLP-guided bounce: the traversal kernels write the network's input records in their
+        // epilogue (the persistent and suspend variants leave that to the input pass)
+        float4* prep = (A.guide_mode == RRH_GUIDE_MLP && guided_b && !pq && !sq) ? (float4*)c->mlp_prep.p : nullptr;
+        bool prep_done = false;
         if ((rc = prof_begin())) return rc;
         if (pq)
             rc = launch_queue_p_sel(sk, c, S, qi, cnt, b, (pairs && b > 0) ? b - 1 : kNoQueue, cnt + 128 + b, rad,
@@ -1075,29 +1109,30 @@ static int rrh_pt_render_impl(rrh_ctx* c, rrh_scene* s, const rrh_pt_params* p, 
         else if (sq)
             rc = launch_susp_sel(sk, c, S, qi, cnt, b, (pairs && b > 0) ? b - 1 : 0u, (pairs && b > 0) ? gt : 0u, gt,
                                  rad, A.rec, Bf, susp);
-        else if (pairs && b > 0)  // this bounce's extension rays with the previous bounce's shadow rays
-            rc = (sk == RRH_STACK_LDS32) ? launch_pair<RRH_STACK_LDS32>(c, S, qi, cnt, b, gt, perm, rad, A.rec, Bf)
-               : (sk == RRH_STACK_HYBRID16) ? launch_pair<RRH_STACK_HYBRID16>(c, S, qi, cnt, b, gt, perm, rad, A.rec, Bf)
-               : (sk == RRH_STACK_HYBRID32) ? launch_pair<RRH_STACK_HYBRID32>(c, S, qi, cnt, b, gt, perm, rad, A.rec, Bf)
-               : (sk == RRH_STACK_NONE) ? launch_pair<RRH_STACK_NONE>(c, S, qi, cnt, b, gt, perm, rad, A.rec, Bf)
-               : launch_pair<RRH_STACK_PRIVATE>(c, S, qi, cnt, b, gt, perm, rad, A.rec, Bf);
-        else
-            rc = (sk == RRH_STACK_LDS32) ? launch_trace<RRH_STACK_LDS32>(c, S, qi, cnt, b, g, gt, cf, perm, Bf.hits, top_n)
-               : (sk == RRH_STACK_HYBRID16) ? launch_trace<RRH_STACK_HYBRID16>(c, S, qi, cnt, b, g, gt, cf, perm, Bf.hits, top_n)
-               : (sk == RRH_STACK_HYBRID32) ? launch_trace<RRH_STACK_HYBRID32>(c, S, qi, cnt, b, g, gt, cf, perm, Bf.hits, top_n)
-               : (sk == RRH_STACK_NONE) ? launch_trace<RRH_STACK_NONE>(c, S, qi, cnt, b, g, gt, cf, perm, Bf.hits, top_n)
-               : launch_trace<RRH_STACK_PRIVATE>(c, S, qi, cnt, b, g, gt, cf, perm, Bf.hits, top_n);
+        else if (pairs && b > 0) {  // this bounce's extension rays with the previous bounce's shadow rays
+            rc = (sk == RRH_STACK_LDS32) ? launch_pair<RRH_STACK_LDS32>(c, S, qi, cnt, b, gt, perm, rad, A.rec, Bf, prep)
+               : (sk == RRH_STACK_HYBRID16) ? launch_pair<RRH_STACK_HYBRID16>(c, S, qi, cnt, b, gt, perm, rad, A.rec, Bf, prep)
+               : (sk == RRH_STACK_HYBRID32) ? launch_pair<RRH_STACK_HYBRID32>(c, S, qi, cnt, b, gt, perm, rad, A.rec, Bf, prep)
+               : (sk == RRH_STACK_NONE) ? launch_pair<RRH_STACK_NONE>(c, S, qi, cnt, b, gt, perm, rad, A.rec, Bf, prep)
+               : launch_pair<RRH_STACK_PRIVATE>(c, S, qi, cnt, b, gt, perm, rad, A.rec, Bf, prep);
+            prep_done = prep != nullptr;
+        } else {
+            bool* pu = &prep_done;
+            rc = (sk == RRH_STACK_LDS32) ? launch_trace<RRH_STACK_LDS32>(c, S, qi, cnt, b, g, gt, cf, perm, Bf.hits, top_n, prep, pu)
+               : (sk == RRH_STACK_HYBRID16) ? launch_trace<RRH_STACK_HYBRID16>(c, S, qi, cnt, b, g, gt, cf, perm, Bf.hits, top_n, prep, pu)
+               : (sk == RRH_STACK_HYBRID32) ? launch_trace<RRH_STACK_HYBRID32>(c, S, qi, cnt, b, g, gt, cf, perm, Bf.hits, top_n, prep, pu)
+               : (sk == RRH_STACK_NONE) ? launch_trace<RRH_STACK_NONE>(c, S, qi, cnt, b, g, gt, cf, perm, Bf.hits, top_n, prep, pu)
+               : launch_trace<RRH_STACK_PRIVATE>(c, S, qi, cnt, b, g, gt, cf, perm, Bf.hits, top_n, prep, pu);
+        }
         if (rc) return rc;
         RRH_LAUNCH_CHECK();
         if ((rc = prof_end())) return rc;
         A.b = b;
-        // the direction sampled at the last bounce is never traced (and its training record
-        // carries no radiance): the guide is skipped there
         ShadeArgs Ab = A;
-        if (b + 1 >= p->max_bounces || (p->guide_bounces != 0 && b >= p->guide_bounces)) Ab.guide_mode = RRH_GUIDE_NONE;
+        if (!guided_b) Ab.guide_mode = RRH_GUIDE_NONE;
         if (Ab.guide_mode == RRH_GUIDE_MLP) {
             rc = mlp_infer_queue(c, mlp, S, qi.o, qi.d, (const float4*)Bf.hits, cnt, b, npix,
-                                 const_cast<uint16_t*>(A.guide.mlp_lobes));
+                                 const_cast<uint16_t*>(A.guide.mlp_lobes), prep_done);
             if (rc) return rc;
             Ab.hit_prep = (const float4*)c->mlp_prep.p;  // the hit lookup the network's inputs took
         }

@@ -171,3 +171,35 @@ def test_trained_guides_lower_the_noise_of_the_window_lit_room(ctx):
     assert v_vmm < 0.85 * v_none and v_mlp < 0.7 * v_none
     for m in (m_vmm, m_mlp):  # the same image (within noise): guiding changes the variance, not the mean
         assert abs(m.mean() / m_none.mean() - 1) < 0.1
+
+
+def test_mlp_guided_frame_equal_through_the_traversal_epilogue_and_the_input_pass(ctx):
+    """The network's per-slot inputs come from the traversal kernels' epilogue (paired launches:
+    k_trace_pair; unpaired: k_intersect) or, where a traversal variant does not write them, from
+    the separate input pass (k_mlp_prep, taken here with fetch counting).  Both write them through
+    one function, so the guided frame, its ray counts and its recorded training set are the same
+    bit for bit whichever path ran."""
+    gs = _scene(ctx)
+    info = gs.info()
+    m = core.Mlp(ctx, info["bounds_min"], info["bounds_max"], seed=3)
+    img = core.Image(ctx, W, H)
+
+    def frame(flags, pairs):
+        L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRACE_PAIRS, pairs)
+        try:
+            p = L.PTParams(BOUNCES, 4242, 3, L.GUIDE_MLP, 0.5, 4096, flags)
+            L.call("rrh_pt_render_guided", ctx.h, gs.h, C.byref(p), None, m.h, img.h)
+            rc = L.RayCounts()
+            L.call("rrh_pt_ray_counts", ctx.h, C.byref(rc))
+            t = core.Mlp(ctx, info["bounds_min"], info["bounds_max"], seed=7)  # one step on the recorded set
+            loss = t.train_from_vertices(4096)
+            return img.download().copy(), (rc.primary, rc.extension, rc.shadow), (loss, t.get_weights().tobytes())
+        finally:
+            L.call("rrh_ctx_set_option", ctx.h, L.OPT_TRACE_PAIRS, L.TRACE_PAIRS_DEFAULT)
+
+    ref = frame(L.PT_COUNT_FETCHES, 1)  # fetch counting: unpaired, counting kernels, input pass
+    for pairs in (1, 0):
+        got = frame(0, pairs)
+        assert np.array_equal(got[0].view(np.uint32), ref[0].view(np.uint32)), pairs
+        assert got[1] == ref[1], (pairs, got[1], ref[1])
+        assert got[2] == ref[2], pairs
