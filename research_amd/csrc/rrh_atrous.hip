@@ -33,13 +33,13 @@ __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? l
 struct RefPx { f3 pos; float w; f3 nrm; f3 rgb; };
 
 // Per-tap weight arithmetic.  Exact: correctly rounded division and square root and the
-// library's own exp / log, bit-equal to the CPU restatement.  Fast: the hardware
+// library's own exp (wexp_rr for the weights) / log, bit-equal to the CPU restatement.  Fast: the hardware
 // approximations (v_exp_f32, v_log_f32, v_rcp_f32, v_rsq_f32, about 1 ulp each); the taps'
 // weights then differ in the last bits and the filtered pixel by far less than 1e-3.
 template <bool FAST> struct WMath;
 template <> struct WMath<false> {
     static __device__ __forceinline__ float ex(float x) { return exp_rr(x); }
-    static __device__ __forceinline__ float wex(float x) { return wexp_rr(x); }  // config mode's weights
+    static __device__ __forceinline__ float wex(float x) { return wexp_rr(x); }  // the weights' exp (both modes)
     static __device__ __forceinline__ float div(float a, float b) { return a / b; }
     static __device__ __forceinline__ float rcp(float b) { return 1.0f / b; }
     static __device__ __forceinline__ f3 unit(f3 a) { return normalize(a); }
